@@ -1,0 +1,70 @@
+/*
+ * st_oracle.h - CPU oracle for the per-pixel tracing path.
+ *
+ * TEST INFRASTRUCTURE ONLY: the oracle is a C restatement of the reference
+ * CPU renderer (SlimTracin render/raytracer.h:81-171 and everything below it)
+ * used by tests/ and by bench.py's CPU baseline leg to check the HIP library.
+ * Product code never links it.
+ */
+#ifndef ST_ORACLE_H
+#define ST_ORACLE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "st_blob.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Algorithmic event counts of one frame (bytes per event use the reference's
+   record sizes: node pair 64, root AABB 24, triangle hot part 60, primitive
+   72, mesh-hit attributes 68, texel quad 12, material 84, light 44). */
+typedef struct StOracleStats {
+    uint64_t primary_rays, shadow_rays, bounce_rays;
+    uint64_t node_pairs, root_tests, triangle_tests, primitive_visits;
+    uint64_t mesh_attr_fetches, texel_fetches, material_reads, light_reads;
+} StOracleStats;
+
+/* Render one frame. pixels and hits have (width+1)*(height+1) entries; hits
+   may be NULL. threads > 1 splits rows round-robin with one trace state per
+   thread. Returns 0 on success. */
+int oracle_render(const void *blob, size_t size, const StView *view, StFloatPixel *pixels,
+                  StHitRec *hits, int threads, StOracleStats *stats);
+
+/* Same, but loads once and renders `frames` times, recording per-frame
+   seconds of the render call alone. */
+int oracle_render_timed(const void *blob, size_t size, const StView *view, StFloatPixel *pixels,
+                        int threads, int frames, double *seconds, StOracleStats *stats);
+
+/* Scene BVH (SAH, leaves <= 2) as built for the blob. nodes: 2*prim_count. */
+int oracle_scene_bvh(const void *blob, size_t size, StBVHNodeRec *nodes, uint32_t *leaf_ids,
+                     uint32_t *node_count, uint32_t *height);
+
+/* Mesh BVH (SAH, leaves <= 4) and triangle records of mesh m. nodes: 2*tris. */
+int oracle_mesh_bvh(const void *blob, size_t size, uint32_t m, StBVHNodeRec *nodes,
+                    StTriangleRec *tris, uint32_t *leaf_ids, uint32_t *node_count, uint32_t *height);
+
+/* Screen-space bounds (4 ints per primitive) and flags for a view. */
+int oracle_ssb(const void *blob, size_t size, const StView *view, int32_t *rects, uint8_t *flags);
+
+/* Camera set-up for a view: 9 floats projection plane (start, right, down),
+   4 floats inverted rotation quaternion, 4 floats rotation quaternion. */
+int oracle_camera(const StView *view, float *out);
+
+/* Plain SAH build over N caller-given leaf boxes (6 floats each), as
+   buildBVH does for a scene or mesh. Used to check the HIP library's fast
+   builder against the plain quicksort. Returns node count. */
+uint32_t oracle_sah_build(const float *boxes, uint32_t n, uint32_t max_leaf, StBVHNodeRec *nodes,
+                          uint32_t *leaf_ids, uint32_t *height);
+
+/* Morton LBVH over primitive boxes (CPU restatement of the device builder).
+   nodes: 2*n entries. Returns node count. */
+uint32_t oracle_lbvh_build(const float *boxes, uint32_t n, StBVHNodeRec *nodes, uint32_t *leaf_ids,
+                           uint32_t *height);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,219 @@
+"""ctypes access to the two CPU checkers used by the tests.
+
+- the oracle (oracle/lib/libstoracle.so): the C restatement of the reference
+  CPU renderer;
+- the reference build (oracle/_ref/libslimref.so): the reference's own CPU
+  renderer compiled from its headers. Present wherever oracle/Makefile's
+  `ref` target ran (it needs the reference checkout).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from slimtracin_amd.scene import HIT_DTYPE, NODE_DTYPE, PIXEL_DTYPE, TRIANGLE_DTYPE, grid_size
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_SO = os.path.join(ROOT, "oracle", "lib", "libstoracle.so")
+REF_SO = os.path.join(ROOT, "oracle", "_ref", "libslimref.so")
+
+
+class OracleStats(C.Structure):
+    _fields_ = [(n, C.c_uint64) for n in (
+        "primary_rays", "shadow_rays", "bounce_rays", "node_pairs", "root_tests", "triangle_tests",
+        "primitive_visits", "mesh_attr_fetches", "texel_fetches", "material_reads", "light_reads")]
+
+    def as_dict(self):
+        return {n: int(getattr(self, n)) for n, _ in self._fields_}
+
+    def algorithmic_bytes(self):
+        """Bytes the reference algorithm reads, at the reference's record sizes."""
+        return (64 * self.node_pairs + 24 * self.root_tests + 60 * self.triangle_tests + 72 * self.primitive_visits
+                + 68 * self.mesh_attr_fetches + 12 * self.texel_fetches + 84 * self.material_reads
+                + 44 * self.light_reads)
+
+
+_libs = {}
+
+
+def _load(path):
+    if path not in _libs:
+        _libs[path] = C.CDLL(path)
+    return _libs[path]
+
+
+def oracle_available():
+    return os.path.exists(ORACLE_SO)
+
+
+def ref_available():
+    return os.path.exists(REF_SO)
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def _blob(blob: bytes):
+    buf = np.frombuffer(blob, dtype=np.uint8)
+    return _ptr(buf), C.c_size_t(len(blob)), buf
+
+
+def oracle_render(blob, view, threads=1, hits=True):
+    lib = _load(ORACLE_SO)
+    n = grid_size(view)
+    pixels = np.zeros(n, dtype=PIXEL_DTYPE)
+    hrec = np.zeros(n, dtype=HIT_DTYPE) if hits else None
+    st = OracleStats()
+    bp, bs, keep = _blob(blob)
+    rc = lib.oracle_render(bp, bs, _ptr(view), _ptr(pixels), _ptr(hrec) if hits else None, C.c_int(threads),
+                           C.byref(st))
+    assert rc == 0, "oracle_render failed"
+    return pixels, hrec, st
+
+
+def oracle_render_timed(blob, view, threads, frames):
+    lib = _load(ORACLE_SO)
+    pixels = np.zeros(grid_size(view), dtype=PIXEL_DTYPE)
+    secs = np.zeros(frames, dtype=np.float64)
+    st = OracleStats()
+    bp, bs, keep = _blob(blob)
+    rc = lib.oracle_render_timed(bp, bs, _ptr(view), _ptr(pixels), C.c_int(threads), C.c_int(frames), _ptr(secs),
+                                 C.byref(st))
+    assert rc == 0
+    return pixels, secs, st
+
+
+def ref_render(blob, view, threads=1):
+    lib = _load(REF_SO)
+    pixels = np.zeros(grid_size(view), dtype=PIXEL_DTYPE)
+    bp, bs, keep = _blob(blob)
+    rc = lib.ref_render(bp, bs, _ptr(view), _ptr(pixels), C.c_int(threads))
+    assert rc == 0, "ref_render failed"
+    return pixels
+
+
+def ref_render_timed(blob, view, threads, frames):
+    lib = _load(REF_SO)
+    pixels = np.zeros(grid_size(view), dtype=PIXEL_DTYPE)
+    secs = np.zeros(frames, dtype=np.float64)
+    bp, bs, keep = _blob(blob)
+    rc = lib.ref_render_timed(bp, bs, _ptr(view), _ptr(pixels), C.c_int(threads), C.c_int(frames), _ptr(secs))
+    assert rc == 0
+    return pixels, secs
+
+
+def ref_primary_hits(blob, view):
+    lib = _load(REF_SO)
+    hits = np.zeros(grid_size(view), dtype=HIT_DTYPE)
+    bp, bs, keep = _blob(blob)
+    assert lib.ref_primary_hits(bp, bs, _ptr(view), _ptr(hits)) == 0
+    return hits
+
+
+def _bvh(lib, fn, blob, max_nodes):
+    nodes = np.zeros(max_nodes, dtype=NODE_DTYPE)
+    leaf = np.zeros(max_nodes, dtype=np.uint32)
+    cnt = C.c_uint32()
+    h = C.c_uint32()
+    bp, bs, keep = _blob(blob)
+    assert getattr(lib, fn)(bp, bs, _ptr(nodes), _ptr(leaf), C.byref(cnt), C.byref(h)) == 0
+    return nodes[:cnt.value], leaf, h.value
+
+
+def oracle_scene_bvh(blob, prim_count):
+    n, l, h = _bvh(_load(ORACLE_SO), "oracle_scene_bvh", blob, 2 * prim_count + 2)
+    return n, l[:prim_count], h
+
+
+def ref_scene_bvh(blob, prim_count):
+    n, l, h = _bvh(_load(REF_SO), "ref_scene_bvh", blob, 2 * prim_count + 2)
+    return n, l[:prim_count], h
+
+
+def _mesh_bvh(lib, fn, blob, m, tri_count):
+    nodes = np.zeros(2 * tri_count + 2, dtype=NODE_DTYPE)
+    tris = np.zeros(tri_count, dtype=TRIANGLE_DTYPE)
+    leaf = np.zeros(tri_count, dtype=np.uint32)
+    cnt = C.c_uint32()
+    h = C.c_uint32()
+    bp, bs, keep = _blob(blob)
+    assert getattr(lib, fn)(bp, bs, C.c_uint32(m), _ptr(nodes), _ptr(tris), _ptr(leaf), C.byref(cnt), C.byref(h)) == 0
+    return nodes[:cnt.value], tris, leaf, h.value
+
+
+def oracle_mesh_bvh(blob, m, tri_count):
+    return _mesh_bvh(_load(ORACLE_SO), "oracle_mesh_bvh", blob, m, tri_count)
+
+
+def ref_mesh_bvh(blob, m, tri_count):
+    return _mesh_bvh(_load(REF_SO), "ref_mesh_bvh", blob, m, tri_count)
+
+
+def _ssb(lib, fn, blob, view, prim_count):
+    rects = np.zeros((prim_count, 4), dtype=np.int32)
+    flags = np.zeros(prim_count, dtype=np.uint8)
+    bp, bs, keep = _blob(blob)
+    assert getattr(lib, fn)(bp, bs, _ptr(view), _ptr(rects), _ptr(flags)) == 0
+    return rects, flags
+
+
+def oracle_ssb(blob, view, prim_count):
+    return _ssb(_load(ORACLE_SO), "oracle_ssb", blob, view, prim_count)
+
+
+def ref_ssb(blob, view, prim_count):
+    return _ssb(_load(REF_SO), "ref_ssb", blob, view, prim_count)
+
+
+def oracle_camera(view):
+    out = np.zeros(17, dtype=np.float32)
+    _load(ORACLE_SO).oracle_camera(_ptr(view), _ptr(out))
+    return out
+
+
+def ref_camera(view):
+    out = np.zeros(17, dtype=np.float32)
+    _load(REF_SO).ref_camera(_ptr(view), _ptr(out))
+    return out
+
+
+def oracle_sah_build(boxes, max_leaf):
+    boxes = np.ascontiguousarray(boxes, dtype=np.float32).reshape(-1, 6)
+    n = len(boxes)
+    nodes = np.zeros(2 * n + 2, dtype=NODE_DTYPE)
+    leaf = np.zeros(n + 1, dtype=np.uint32)
+    h = C.c_uint32()
+    lib = _load(ORACLE_SO)
+    lib.oracle_sah_build.restype = C.c_uint32
+    cnt = lib.oracle_sah_build(_ptr(boxes), C.c_uint32(n), C.c_uint32(max_leaf), _ptr(nodes), _ptr(leaf), C.byref(h))
+    return nodes[:cnt], leaf[:n], h.value
+
+
+def ref_quat_axis(axis, amount):
+    out = np.zeros(4, dtype=np.float32)
+    a = np.asarray(axis, dtype=np.float32)
+    _load(REF_SO).ref_quat_axis(_ptr(a), C.c_float(amount), _ptr(out))
+    return out
+
+
+def ref_quat_mul_norm(a, b):
+    out = np.zeros(4, dtype=np.float32)
+    a = np.asarray(a, dtype=np.float32)
+    b = np.asarray(b, dtype=np.float32)
+    _load(REF_SO).ref_quat_mul_norm(_ptr(a), _ptr(b), _ptr(out))
+    return out
+
+
+def pixels_equal(a, b):
+    """Bit-exact FloatPixel comparison (NaN-safe through the raw bytes)."""
+    return a.tobytes() == b.tobytes()
+
+
+def first_pixel_mismatches(a, b, k=5):
+    av = a.view(np.uint8).reshape(len(a), -1)
+    bv = b.view(np.uint8).reshape(len(b), -1)
+    bad = np.nonzero((av != bv).any(axis=1))[0]
+    return bad[:k], len(bad)
