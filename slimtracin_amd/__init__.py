@@ -1,0 +1,161 @@
+"""MI355X-native tracing path of SlimTracin: Python binding of libslimtracin.so.
+
+The library (slimtracin_amd/lib/libslimtracin.so, built by
+`make -C slimtracin_amd/csrc`) is the product: host C + gfx950 HIP kernels
+behind a C ABI (include/slimtracin.h). This module only marshals arguments.
+It raises if the library is missing: there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from .scene import HIT_DTYPE, NODE_DTYPE, PIXEL_DTYPE, VIEW_DTYPE, grid_size  # noqa: F401
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib", "libslimtracin.so")
+
+ST_BATCH_PERSISTENT = 1
+
+
+class SlimTracinError(RuntimeError):
+    pass
+
+
+class RayCounts(C.Structure):
+    _fields_ = [("primary", C.c_uint64), ("shadow", C.c_uint64), ("bounce", C.c_uint64)]
+
+    def total(self):
+        return int(self.primary + self.shadow + self.bounce)
+
+
+class SceneInfo(C.Structure):
+    _fields_ = [(n, C.c_uint32) for n in ("prim_count", "mesh_count", "triangle_count", "texture_count",
+                                          "scene_node_count", "scene_height", "mesh_node_count",
+                                          "max_mesh_height")] + [("device_bytes", C.c_uint64)]
+
+
+_lib = None
+
+
+def lib():
+    """Load the HIP library (raises when it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise SlimTracinError(f"{LIB_PATH} is missing: build it with `make -C slimtracin_amd/csrc`")
+        L = C.CDLL(LIB_PATH)
+        L.st_last_error.restype = C.c_char_p
+        for name in ("st_gen_implicit", "st_gen_pbr", "st_gen_mesh_scene"):
+            getattr(L, name).restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != 0:
+        raise SlimTracinError(f"status {rc}: {lib().st_last_error().decode(errors='replace')}")
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def init(device: int = 0):
+    _check(lib().st_init(C.c_int(device)))
+
+
+def _take_blob(rc, out, size):
+    _check(rc)
+    try:
+        return C.string_at(out, size.value)
+    finally:
+        lib().st_free(out)
+
+
+def gen_implicit(seed: int) -> bytes:
+    out, size = C.c_void_p(), C.c_size_t()
+    return _take_blob(lib().st_gen_implicit(C.c_uint64(seed), C.byref(out), C.byref(size)), out, size)
+
+
+def gen_pbr(seed: int, k: int) -> bytes:
+    out, size = C.c_void_p(), C.c_size_t()
+    return _take_blob(lib().st_gen_pbr(C.c_uint64(seed), C.c_uint32(k), C.byref(out), C.byref(size)), out, size)
+
+
+def gen_mesh_scene(seed: int, triangles: int) -> bytes:
+    out, size = C.c_void_p(), C.c_size_t()
+    return _take_blob(lib().st_gen_mesh_scene(C.c_uint64(seed), C.c_uint32(triangles), C.byref(out), C.byref(size)),
+                      out, size)
+
+
+class DeviceScene:
+    """A scene resident in HBM (st_scene_create)."""
+
+    def __init__(self, blob: bytes):
+        self._blob = np.frombuffer(blob, dtype=np.uint8)
+        h = C.c_void_p()
+        _check(lib().st_scene_create(_ptr(self._blob), C.c_size_t(len(blob)), C.byref(h)))
+        self.handle = h
+
+    def close(self):
+        if self.handle:
+            lib().st_scene_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def info(self) -> SceneInfo:
+        inf = SceneInfo()
+        _check(lib().st_scene_info(self.handle, C.byref(inf)))
+        return inf
+
+    def bvh(self):
+        inf = self.info()
+        nodes = np.zeros(2 * inf.prim_count + 2, dtype=NODE_DTYPE)
+        leaf = np.zeros(inf.prim_count + 1, dtype=np.uint32)
+        cnt, h = C.c_uint32(), C.c_uint32()
+        _check(lib().st_scene_bvh(self.handle, _ptr(nodes), _ptr(leaf), C.byref(cnt), C.byref(h)))
+        return nodes[:cnt.value], leaf[:inf.prim_count], h.value
+
+
+def render(scene: DeviceScene, view: np.ndarray, hits: bool = True):
+    """Plain per-request launch with blocking download (renderSceneOnGPU)."""
+    n = grid_size(view)
+    pixels = np.zeros(n, dtype=PIXEL_DTYPE)
+    hrec = np.zeros(n, dtype=HIT_DTYPE) if hits else None
+    counts = RayCounts()
+    _check(lib().st_render(scene.handle, _ptr(view), _ptr(pixels), _ptr(hrec) if hits else None, C.byref(counts)))
+    return pixels, hrec, counts
+
+
+def render_batch(scenes, views: np.ndarray, hits: bool = False, persistent: bool = True):
+    """Fuse len(views) requests into one launch; returns per-request outputs."""
+    n = len(views)
+    handles = (C.c_void_p * n)(*[s.handle for s in scenes])
+    pix = [np.zeros(grid_size(views[i:i + 1]), dtype=PIXEL_DTYPE) for i in range(n)]
+    hr = [np.zeros(grid_size(views[i:i + 1]), dtype=HIT_DTYPE) for i in range(n)] if hits else None
+    pp = (C.c_void_p * n)(*[p.ctypes.data for p in pix])
+    hp = (C.c_void_p * n)(*[h.ctypes.data for h in hr]) if hits else None
+    counts = (RayCounts * n)()
+    views = np.ascontiguousarray(views, dtype=VIEW_DTYPE)
+    _check(lib().st_render_batch(handles, _ptr(views), C.c_uint32(n), pp, hp, counts,
+                                 C.c_uint32(ST_BATCH_PERSISTENT if persistent else 0)))
+    return pix, hr, list(counts)
+
+
+def render_timed(scenes, views: np.ndarray, persistent: bool = True):
+    """One launch without downloads; returns (render-kernel ms from HIP events, counts)."""
+    n = len(views)
+    handles = (C.c_void_p * n)(*[s.handle for s in scenes])
+    counts = (RayCounts * n)()
+    ms = C.c_float()
+    views = np.ascontiguousarray(views, dtype=VIEW_DTYPE)
+    _check(lib().st_render_timed(handles, _ptr(views), C.c_uint32(n), C.c_uint32(ST_BATCH_PERSISTENT if persistent else 0),
+                                 C.byref(ms), counts))
+    return ms.value, list(counts)

@@ -1,0 +1,535 @@
+/*
+ * st_runtime.hip - device residency, kernels and the extern "C" API of the
+ * HIP tracing library (gfx950, wave64).
+ *
+ * Kernels
+ *   k_primary_dirs  one lane per traced row: walks the row with the
+ *                   reference's sequential `current += right` fp32 sum
+ *                   (render/raytracer.h:156-170) and stores the normalised
+ *                   primary direction of every pixel. Rows of every request
+ *                   of a launch are flattened into one grid.
+ *   k_render        one wavefront per 8x8 pixel tile, the whole per-pixel
+ *                   path of st_trace.h per lane. Tiles of all requests of a
+ *                   launch form one list; in persistent mode each wavefront
+ *                   keeps pulling the next tile from a global counter until
+ *                   the list is exhausted (the loop's exit condition is the
+ *                   same for every wave), otherwise tile = wave id with an
+ *                   XCD-aware block order. A single request is just a list
+ *                   with one entry, so plain and batched launches run the
+ *                   same code.
+ */
+#include <hip/hip_runtime.h>
+
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+
+#include "st_trace.h"
+#include "../host/st_host.h"
+#include "slimtracin.h"
+
+#define TILE_W 8
+#define TILE_H 8
+#define WAVES_PER_BLOCK 4
+#define BLOCK_THREADS (64 * WAVES_PER_BLOCK)
+
+/* ---- errors ------------------------------------------------------------ */
+
+static __thread char g_err[512];
+
+static int set_err(int code, const char *fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof g_err, fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+#define HIP_TRY(call)                                                                          \
+    do {                                                                                       \
+        hipError_t e_ = (call);                                                                \
+        if (e_ != hipSuccess)                                                                  \
+            return set_err(ST_ERR_HIP, "%s failed: %s", #call, hipGetErrorString(e_));        \
+    } while (0)
+
+extern "C" const char *st_last_error(void) { return g_err; }
+
+/* ---- kernels ------------------------------------------------------------ */
+
+/* Per-request launch record shared by both kernels (device copy). */
+struct KReq {
+    StDView v;
+    uint32_t row_base;     /* first flattened row of this request */
+    uint32_t tile_count;
+    uint32_t pad[2];
+};
+
+__device__ static inline uint32_t find_req(const KReq *reqs, uint32_t n, uint32_t key, bool by_row) {
+    uint32_t lo = 0, hi = n - 1;
+    while (lo < hi) {
+        uint32_t mid = (lo + hi + 1) >> 1;
+        uint32_t base = by_row ? reqs[mid].row_base : reqs[mid].v.tile_base;
+        if (base <= key) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+__global__ void __launch_bounds__(256) k_primary_dirs(const KReq *reqs, uint32_t nreq, uint32_t total_rows) {
+    uint32_t row = blockIdx.x * blockDim.x + threadIdx.x;
+    if (row >= total_rows) return;
+    uint32_t r = find_req(reqs, nreq, row, true);
+    const StDView &v = reqs[r].v;
+    uint32_t y = row - reqs[r].row_base;
+    if (y >= v.height1) return;
+    const float4 rs = reinterpret_cast<const float4 *>(v.row_starts)[y];
+    sv3 current = V3(rs.x, rs.y, rs.z);
+    const sv3 right = V3(v.right[0], v.right[1], v.right[2]);
+    float4 *out = reinterpret_cast<float4 *>(v.dirs) + (size_t)y * v.width1;
+    for (uint32_t x = 0; x < v.width1; x++) {
+        sv3 d = v_norm(current);
+        out[x] = make_float4(d.x, d.y, d.z, 0.0f);
+        current = v_add(current, right);
+    }
+}
+
+__device__ static inline uint32_t wave_sum(uint32_t v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+__device__ static void render_tile(const StDScene *scenes, const KReq *reqs, uint32_t nreq, uint32_t tile,
+                                   uint32_t *scene_stack, uint32_t *mesh_stack) {
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t r = find_req(reqs, nreq, tile, false);
+    const StDView v = reqs[r].v;
+    const StDScene sc = scenes[v.scene];
+    const uint32_t local = tile - v.tile_base;
+    const uint32_t tx = local % v.tiles_x, ty = local / v.tiles_x;
+    const uint32_t x = tx * TILE_W + (lane % TILE_W), y = ty * TILE_H + (lane / TILE_W);
+    uint32_t n_shadow = 0, n_bounce = 0, n_primary = 0;
+    if (x < v.width1 && y < v.height1) {
+        const size_t pix = (size_t)y * v.width1 + x;
+        const float4 d4 = reinterpret_cast<const float4 *>(v.dirs)[pix];
+        DTrace tr = {};
+        tr.closest_mesh_hit.object_type = ST_PRIM_MESH;
+        tr.depth = v.depth;
+        tr.mesh_stack_size = sc.mesh_stack_size;
+        tr.scene_stack = scene_stack;
+        tr.mesh_stack = mesh_stack;
+        float4 rgba;
+        double depth;
+        StHitRec hrec;
+        trace_pixel(sc, v, tr, V3(d4.x, d4.y, d4.z), x, y, rgba, depth, hrec);
+        /* FloatPixel is 24 bytes: store as three 8-byte words */
+        uint2 *px = reinterpret_cast<uint2 *>(static_cast<char *>(v.pixels) + 24 * pix);
+        px[0] = make_uint2(__float_as_uint(rgba.x), __float_as_uint(rgba.y));
+        px[1] = make_uint2(__float_as_uint(rgba.z), __float_as_uint(rgba.w));
+        unsigned long long db = (unsigned long long)__double_as_longlong(depth);
+        px[2] = make_uint2((uint32_t)db, (uint32_t)(db >> 32));
+        if (v.hits) {
+            int4 *h = reinterpret_cast<int4 *>(v.hits) + pix;
+            *h = make_int4(hrec.prim_id, hrec.object_type, hrec.triangle_id, __float_as_int(hrec.distance));
+        }
+        n_primary = 1;
+        n_shadow = tr.n_shadow;
+        n_bounce = tr.n_bounce;
+    }
+    n_primary = wave_sum(n_primary);
+    n_shadow = wave_sum(n_shadow);
+    n_bounce = wave_sum(n_bounce);
+    if (lane == 0 && v.counters) {
+        atomicAdd(v.counters + 0, (unsigned long long)n_primary);
+        atomicAdd(v.counters + 1, (unsigned long long)n_shadow);
+        atomicAdd(v.counters + 2, (unsigned long long)n_bounce);
+    }
+}
+
+__global__ void __launch_bounds__(BLOCK_THREADS) k_render(const StDScene *scenes, const KReq *reqs, uint32_t nreq,
+                                                           uint32_t total_tiles, uint32_t *tile_counter,
+                                                           uint32_t persistent, uint32_t xcd_blocks) {
+    uint32_t scene_stack[ST_SCENE_STACK_MAX];
+    uint32_t mesh_stack[ST_MESH_STACK_MAX];
+    const uint32_t wave = threadIdx.x >> 6;
+    if (persistent) {
+        for (;;) {
+            uint32_t t = 0;
+            if ((threadIdx.x & 63u) == 0) t = atomicAdd(tile_counter, 1u);
+            t = __shfl(t, 0, 64);
+            if (t >= total_tiles) break;
+            render_tile(scenes, reqs, nreq, t, scene_stack, mesh_stack);
+        }
+    } else {
+        /* blocks are dealt to the 8 XCDs round-robin; give each XCD a
+           contiguous run of tiles so neighbouring tiles share its L2 */
+        uint32_t b = blockIdx.x;
+        uint32_t block = (b % 8u) * (xcd_blocks / 8u) + b / 8u;
+        uint32_t t = block * WAVES_PER_BLOCK + wave;
+        if (t < total_tiles) render_tile(scenes, reqs, nreq, t, scene_stack, mesh_stack);
+    }
+}
+
+__global__ void k_zero_u32(uint32_t *p, uint32_t n) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+__global__ void k_zero_u64(unsigned long long *p, uint32_t n) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+/* ---- context ------------------------------------------------------------ */
+
+struct Arena {
+    char *base;
+    size_t cap, used;
+};
+
+static struct {
+    int ready;
+    int device;
+    hipStream_t stream;
+    Arena work;            /* per-launch device buffers (grow-only) */
+    char *staging;         /* pinned host staging for per-launch uploads */
+    size_t staging_cap;
+    uint32_t *tile_counter;
+    StDScene *scene_table; /* device copy of the scene descriptors of a launch */
+    uint32_t scene_table_cap;
+    hipEvent_t ev[4];
+} g;
+
+static int ensure_init(void) {
+    if (g.ready) return ST_OK;
+    return st_init(0);
+}
+
+extern "C" int st_init(int device) {
+    if (g.ready) {
+        if (device == g.device) return ST_OK;
+        return set_err(ST_ERR_ARG, "library already initialised on device %d", g.device);
+    }
+    int count = 0;
+    HIP_TRY(hipGetDeviceCount(&count));
+    if (device < 0 || device >= count) return set_err(ST_ERR_ARG, "device %d not present (%d devices)", device, count);
+    HIP_TRY(hipSetDevice(device));
+    HIP_TRY(hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking));
+    HIP_TRY(hipMalloc(&g.tile_counter, 256));
+    for (int i = 0; i < 4; i++) HIP_TRY(hipEventCreate(&g.ev[i]));
+    g.device = device;
+    g.ready = 1;
+    return ST_OK;
+}
+
+static int arena_reserve(Arena *a, size_t bytes) {
+    if (a->cap >= bytes) { a->used = 0; return ST_OK; }
+    if (a->base) HIP_TRY(hipFree(a->base));
+    a->base = NULL;
+    a->cap = 0;
+    size_t cap = bytes + bytes / 4 + (1u << 20);
+    HIP_TRY(hipMalloc(&a->base, cap));
+    a->cap = cap;
+    a->used = 0;
+    return ST_OK;
+}
+static void *arena_take(Arena *a, size_t bytes) {
+    size_t off = (a->used + 255) & ~(size_t)255;
+    a->used = off + bytes;
+    return a->base + off;
+}
+static size_t round256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+static int staging_reserve(size_t bytes) {
+    if (g.staging_cap >= bytes) return ST_OK;
+    if (g.staging) HIP_TRY(hipHostFree(g.staging));
+    g.staging = NULL;
+    g.staging_cap = 0;
+    size_t cap = bytes + bytes / 4 + 65536;
+    HIP_TRY(hipHostMalloc((void **)&g.staging, cap, hipHostMallocDefault));
+    g.staging_cap = cap;
+    return ST_OK;
+}
+
+/* ---- resident scenes ---------------------------------------------------- */
+
+struct StSceneHandle {
+    StScene *host;
+    StDScene dev;
+    char *dbuf;
+    uint64_t dbytes;
+};
+
+extern "C" int st_scene_create(const void *blob, size_t size, StSceneHandle **out) {
+    *out = NULL;
+    int rc = ensure_init();
+    if (rc) return rc;
+    const char *why = NULL;
+    StScene *sc = st_scene_from_blob(blob, size, &why);
+    if (!sc) return set_err(ST_ERR_SCENE, "invalid scene: %s", why ? why : "unknown");
+    StSceneHandle *h = (StSceneHandle *)calloc(1, sizeof(StSceneHandle));
+    h->host = sc;
+    rc = st_scene_upload(h);
+    if (rc) { st_scene_destroy(h); return rc; }
+    *out = h;
+    return ST_OK;
+}
+
+/* One allocation per scene. BVH node arrays start 32 bytes past a 64-byte
+   boundary: node 0 (the root) sits alone and every child pair (2k+1, 2k+2)
+   covers exactly one 64-byte segment. */
+extern "C" int st_scene_upload(StSceneHandle *h) {
+    StScene *sc = h->host;
+    size_t o_prims = 0;
+    size_t o_mats = o_prims + round256((sc->prim_count + 1) * sizeof(StDPrim));
+    size_t o_lights = o_mats + round256((sc->material_count + 1) * sizeof(StDMat));
+    size_t o_snodes = o_lights + round256((sc->light_count + 1) * sizeof(StDLight));
+    size_t o_sleaf = o_snodes + round256(32 + (2 * (size_t)sc->prim_count + 2) * sizeof(StBVHNodeRec));
+    size_t o_meshes = o_sleaf + round256(((size_t)sc->prim_count + 1) * 4);
+    size_t o_mnodes = o_meshes + round256((sc->mesh_count + 1) * sizeof(StDMesh));
+    size_t o_hot = o_mnodes + round256(((size_t)sc->mesh_node_total + 1) * sizeof(StBVHNodeRec));
+    size_t o_cold = o_hot + round256(((size_t)sc->tri_total + 1) * sizeof(StDTriHot));
+    size_t o_tex = o_cold + round256(((size_t)sc->tri_total + 1) * sizeof(StDTriCold));
+    size_t o_mips = o_tex + round256((sc->texture_count + 1) * sizeof(StDTex));
+    size_t o_texels = o_mips + round256((sc->mip_total + 1) * sizeof(StDMip));
+    size_t total = o_texels + round256(sc->texel_bytes + 16);
+    if (h->dbuf) HIP_TRY(hipFree(h->dbuf));
+    h->dbuf = NULL;
+    hipError_t e = hipMalloc(&h->dbuf, total);
+    if (e != hipSuccess) return set_err(ST_ERR_MEMORY, "hipMalloc(%zu) for scene failed: %s", total, hipGetErrorString(e));
+    h->dbytes = total;
+    char *b = h->dbuf;
+    /* mesh nodes: the host array already has odd per-mesh offsets; index 0
+       is a dummy node, so the device array starts on a 64-byte boundary */
+    struct { size_t off; const void *src; size_t bytes; } parts[] = {
+        {o_prims, sc->prims, sc->prim_count * sizeof(StDPrim)},
+        {o_mats, sc->mats, sc->material_count * sizeof(StDMat)},
+        {o_lights, sc->lights, sc->light_count * sizeof(StDLight)},
+        {o_snodes + 32, sc->scene_nodes, (size_t)sc->scene_node_count * sizeof(StBVHNodeRec)},
+        {o_sleaf, sc->scene_leaf_ids, (size_t)sc->prim_count * 4},
+        {o_meshes, sc->meshes, sc->mesh_count * sizeof(StDMesh)},
+        {o_mnodes, sc->mesh_nodes, (size_t)sc->mesh_node_total * sizeof(StBVHNodeRec)},
+        {o_hot, sc->tri_hot, (size_t)sc->tri_total * sizeof(StDTriHot)},
+        {o_cold, sc->tri_cold, (size_t)sc->tri_total * sizeof(StDTriCold)},
+        {o_tex, sc->textures, sc->texture_count * sizeof(StDTex)},
+        {o_mips, sc->mips, sc->mip_total * sizeof(StDMip)},
+        {o_texels, sc->texels, sc->texel_bytes},
+    };
+    for (size_t i = 0; i < sizeof parts / sizeof parts[0]; i++)
+        if (parts[i].bytes) HIP_TRY(hipMemcpy(b + parts[i].off, parts[i].src, parts[i].bytes, hipMemcpyHostToDevice));
+    StDScene &d = h->dev;
+    memset(&d, 0, sizeof d);
+    d.prims = (const StDPrim *)(b + o_prims);
+    d.mats = (const StDMat *)(b + o_mats);
+    d.lights = (const StDLight *)(b + o_lights);
+    d.scene_nodes = (const uint32_t *)(b + o_snodes + 32);
+    d.scene_leaf_ids = (const uint32_t *)(b + o_sleaf);
+    d.meshes = (const StDMesh *)(b + o_meshes);
+    d.mesh_nodes = (const uint32_t *)(b + o_mnodes);
+    d.tri_hot = (const StDTriHot *)(b + o_hot);
+    d.tri_cold = (const StDTriCold *)(b + o_cold);
+    d.textures = (const StDTex *)(b + o_tex);
+    d.mips = (const StDMip *)(b + o_mips);
+    d.texels = (const uint8_t *)(b + o_texels);
+    d.prim_count = sc->prim_count;
+    d.light_count = sc->light_count;
+    d.mesh_count = sc->mesh_count;
+    d.texture_count = sc->texture_count;
+    memcpy(d.ambient, sc->ambient, 12);
+    d.mesh_stack_size = sc->mesh_stack_size;
+    d.has_emissive_quads = sc->has_emissive_quads;
+    return ST_OK;
+}
+
+extern "C" void st_scene_destroy(StSceneHandle *h) {
+    if (!h) return;
+    if (h->dbuf) (void)hipFree(h->dbuf);
+    st_scene_free(h->host);
+    free(h);
+}
+
+extern "C" int st_scene_info(const StSceneHandle *h, StSceneInfo *info) {
+    memset(info, 0, sizeof *info);
+    const StScene *sc = h->host;
+    info->prim_count = sc->prim_count;
+    info->mesh_count = sc->mesh_count;
+    info->triangle_count = sc->tri_total;
+    info->texture_count = sc->texture_count;
+    info->scene_node_count = sc->scene_node_count;
+    info->scene_height = sc->scene_height;
+    info->mesh_node_count = sc->mesh_node_total;
+    info->max_mesh_height = sc->max_mesh_height;
+    info->device_bytes = h->dbytes;
+    return ST_OK;
+}
+
+extern "C" int st_scene_bvh(const StSceneHandle *h, StBVHNodeRec *nodes, uint32_t *leaf_ids, uint32_t *node_count,
+                            uint32_t *height) {
+    const StScene *sc = h->host;
+    memcpy(nodes, sc->scene_nodes, sizeof(StBVHNodeRec) * sc->scene_node_count);
+    memcpy(leaf_ids, sc->scene_leaf_ids, 4u * sc->prim_count);
+    *node_count = sc->scene_node_count;
+    *height = sc->scene_height;
+    return ST_OK;
+}
+
+/* ---- launches ----------------------------------------------------------- */
+
+struct LaunchPlan {
+    uint32_t nreq, total_rows, total_tiles, nscenes;
+    size_t work_bytes, stage_bytes;
+};
+
+/* Build, upload and run one launch over `n` requests. pixels_dev/hits_dev
+   receive device pointers of each request's outputs inside the work arena. */
+static int launch(StSceneHandle *const *scenes, const StView *views, uint32_t n, int want_hits, int persistent,
+                  StRayCounts *counts, StFloatPixel *const *pixels_host, StHitRec *const *hits_host,
+                  float *kernel_ms) {
+    int rc = ensure_init();
+    if (rc) return rc;
+    if (n == 0) return set_err(ST_ERR_ARG, "no requests");
+    StHostView *hv = (StHostView *)calloc(n, sizeof(StHostView));
+    uint32_t *scene_idx = (uint32_t *)calloc(n, sizeof(uint32_t));
+    StSceneHandle **uniq = (StSceneHandle **)calloc(n, sizeof(void *));
+    uint32_t nuniq = 0;
+    size_t work = 0, stage = 0;
+    uint32_t rows = 0, tiles = 0;
+    for (uint32_t i = 0; i < n; i++) {
+        StSceneHandle *s = scenes[i];
+        if (!s) { rc = set_err(ST_ERR_ARG, "request %u has no scene", i); goto done; }
+        uint32_t k = 0;
+        while (k < nuniq && uniq[k] != s) k++;
+        if (k == nuniq) uniq[nuniq++] = s;
+        scene_idx[i] = k;
+        if (st_view_prepare(s->host, views + i, hv + i)) {
+            rc = set_err(ST_ERR_ARG, "request %u: invalid view (%ux%u, mode %u)", i, views[i].width, views[i].height,
+                         views[i].render_mode);
+            goto done;
+        }
+        size_t px = (size_t)hv[i].dv.width1 * hv[i].dv.height1;
+        work += round256(px * 16) + round256(px * 24) + (want_hits ? round256(px * 16) : 0);
+        work += round256(hv[i].dv.height1 * 16ull) + round256((s->host->prim_count + 1) * 16ull) + 256;
+        stage += round256(hv[i].dv.height1 * 16ull) + round256((s->host->prim_count + 1) * 16ull);
+        hv[i].dv.tiles_x = (hv[i].dv.width1 + TILE_W - 1) / TILE_W;
+        uint32_t ty = (hv[i].dv.height1 + TILE_H - 1) / TILE_H;
+        hv[i].dv.tile_base = tiles;
+        tiles += hv[i].dv.tiles_x * ty;
+        rows += hv[i].dv.height1;
+    }
+    work += round256(n * sizeof(KReq)) + round256(nuniq * sizeof(StDScene)) + round256(n * 24ull);
+    stage += round256(n * sizeof(KReq)) + round256(nuniq * sizeof(StDScene));
+    if ((rc = arena_reserve(&g.work, work))) goto done;
+    if ((rc = staging_reserve(stage))) goto done;
+    {
+        char *st = g.staging;
+        size_t so = 0;
+        KReq *reqs_h = (KReq *)(st + so);
+        so += round256(n * sizeof(KReq));
+        StDScene *sc_h = (StDScene *)(st + so);
+        so += round256(nuniq * sizeof(StDScene));
+        KReq *reqs_d = (KReq *)arena_take(&g.work, n * sizeof(KReq));
+        StDScene *sc_d = (StDScene *)arena_take(&g.work, nuniq * sizeof(StDScene));
+        unsigned long long *cnt_d = (unsigned long long *)arena_take(&g.work, n * 24ull);
+        for (uint32_t k = 0; k < nuniq; k++) sc_h[k] = uniq[k]->dev;
+        uint32_t row_base = 0;
+        /* per-request row starts and rects are staged right after the tables */
+        size_t stage_tables = so;
+        for (uint32_t i = 0; i < n; i++) {
+            size_t px = (size_t)hv[i].dv.width1 * hv[i].dv.height1;
+            StDView &dv = hv[i].dv;
+            dv.scene = scene_idx[i];
+            float *rs_d = (float *)arena_take(&g.work, dv.height1 * 16ull);
+            int32_t *rect_d = (int32_t *)arena_take(&g.work, (scenes[i]->host->prim_count + 1) * 16ull);
+            memcpy(st + so, hv[i].row_starts, dv.height1 * 16ull);
+            size_t rs_off = so;
+            so += round256(dv.height1 * 16ull);
+            memcpy(st + so, hv[i].rects, scenes[i]->host->prim_count * 16ull);
+            size_t rect_off = so;
+            so += round256((scenes[i]->host->prim_count + 1) * 16ull);
+            HIP_TRY(hipMemcpyAsync(rs_d, st + rs_off, dv.height1 * 16ull, hipMemcpyHostToDevice, g.stream));
+            if (scenes[i]->host->prim_count)
+                HIP_TRY(hipMemcpyAsync(rect_d, st + rect_off, scenes[i]->host->prim_count * 16ull,
+                                       hipMemcpyHostToDevice, g.stream));
+            dv.row_starts = rs_d;
+            dv.rects = rect_d;
+            dv.dirs = (float *)arena_take(&g.work, px * 16);
+            dv.pixels = arena_take(&g.work, px * 24);
+            dv.hits = want_hits ? arena_take(&g.work, px * 16) : NULL;
+            dv.counters = cnt_d + 3 * i;
+            reqs_h[i].v = dv;
+            reqs_h[i].row_base = row_base;
+            reqs_h[i].tile_count = 0;
+            row_base += dv.height1;
+        }
+        (void)stage_tables;
+        HIP_TRY(hipMemcpyAsync(reqs_d, reqs_h, n * sizeof(KReq), hipMemcpyHostToDevice, g.stream));
+        HIP_TRY(hipMemcpyAsync(sc_d, sc_h, nuniq * sizeof(StDScene), hipMemcpyHostToDevice, g.stream));
+        hipLaunchKernelGGL(k_zero_u64, dim3((3 * n + 255) / 256), dim3(256), 0, g.stream, cnt_d, 3 * n);
+        hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, g.stream, g.tile_counter, 64u);
+        hipLaunchKernelGGL(k_primary_dirs, dim3((rows + 255) / 256), dim3(256), 0, g.stream, reqs_d, n, rows);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipEventRecord(g.ev[0], g.stream));
+        uint32_t blocks;
+        uint32_t xcd_blocks = 0;
+        if (persistent) {
+            int cus = 256;
+            hipDeviceProp_t prop;
+            if (hipGetDeviceProperties(&prop, g.device) == hipSuccess) cus = prop.multiProcessorCount;
+            uint32_t want = (uint32_t)cus * 8u;
+            uint32_t need = (tiles + WAVES_PER_BLOCK - 1) / WAVES_PER_BLOCK;
+            blocks = need < want ? need : want;
+        } else {
+            blocks = (tiles + WAVES_PER_BLOCK - 1) / WAVES_PER_BLOCK;
+            blocks = (blocks + 7u) & ~7u;
+            xcd_blocks = blocks;
+        }
+        hipLaunchKernelGGL(k_render, dim3(blocks), dim3(BLOCK_THREADS), 0, g.stream, (const StDScene *)sc_d,
+                           (const KReq *)reqs_d, n, tiles, g.tile_counter, (uint32_t)persistent, xcd_blocks);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipEventRecord(g.ev[1], g.stream));
+        for (uint32_t i = 0; i < n; i++) {
+            size_t px = (size_t)hv[i].dv.width1 * hv[i].dv.height1;
+            if (pixels_host && pixels_host[i])
+                HIP_TRY(hipMemcpyAsync(pixels_host[i], reqs_h[i].v.pixels, px * 24, hipMemcpyDeviceToHost, g.stream));
+            if (hits_host && hits_host[i] && want_hits)
+                HIP_TRY(hipMemcpyAsync(hits_host[i], reqs_h[i].v.hits, px * 16, hipMemcpyDeviceToHost, g.stream));
+        }
+        unsigned long long *cnt_h = (unsigned long long *)calloc(3 * n, 8);
+        HIP_TRY(hipMemcpyAsync(cnt_h, cnt_d, 24ull * n, hipMemcpyDeviceToHost, g.stream));
+        HIP_TRY(hipStreamSynchronize(g.stream));
+        if (counts)
+            for (uint32_t i = 0; i < n; i++) {
+                counts[i].primary = cnt_h[3 * i];
+                counts[i].shadow = cnt_h[3 * i + 1];
+                counts[i].bounce = cnt_h[3 * i + 2];
+            }
+        free(cnt_h);
+        if (kernel_ms) HIP_TRY(hipEventElapsedTime(kernel_ms, g.ev[0], g.ev[1]));
+    }
+done:
+    for (uint32_t i = 0; i < n; i++) st_view_release(hv + i);
+    free(hv);
+    free(scene_idx);
+    free(uniq);
+    return rc;
+}
+
+extern "C" int st_render(StSceneHandle *scene, const StView *view, StFloatPixel *pixels, StHitRec *hits,
+                         StRayCounts *counts) {
+    if (!scene || !view) return set_err(ST_ERR_ARG, "null argument");
+    return launch(&scene, view, 1, hits != NULL, 0, counts, &pixels, &hits, NULL);
+}
+
+extern "C" int st_render_batch(StSceneHandle *const *scenes, const StView *views, uint32_t n,
+                               StFloatPixel *const *pixels, StHitRec *const *hits, StRayCounts *counts,
+                               uint32_t flags) {
+    if (!scenes || !views) return set_err(ST_ERR_ARG, "null argument");
+    int want_hits = 0;
+    if (hits)
+        for (uint32_t i = 0; i < n; i++) want_hits |= hits[i] != NULL;
+    return launch(scenes, views, n, want_hits, (flags & ST_BATCH_PERSISTENT) ? 1 : 0, counts, pixels, hits, NULL);
+}
+
+extern "C" int st_render_timed(StSceneHandle *const *scenes, const StView *views, uint32_t n, uint32_t flags,
+                               float *kernel_ms, StRayCounts *counts) {
+    return launch(scenes, views, n, 0, (flags & ST_BATCH_PERSISTENT) ? 1 : 0, counts, NULL, NULL, kernel_ms);
+}

@@ -1,0 +1,1118 @@
+/*
+ * st_trace.h - per-ray device code of the tracing path for gfx950.
+ *
+ * One lane traces one QCAA grid pixel through the full reference path:
+ * primary visibility over all primitives in scene-BVH leaf order with the
+ * screen-space-bounds gate (SlimTracin render/raytracer.h:81-134,
+ * render/shaders/intersection/primitives.h:9-111), two-level BVH traversal
+ * (render/shaders/trace.h:8-76, intersection/mesh.h:38-111), the implicit and
+ * triangle intersection shaders, and Beauty/Depth/Normal/UV shading
+ * (closest_hit/surface.h, lights.h, debug.h). Hit IDs must equal the
+ * reference CPU renderer's, so every arithmetic step keeps the reference's
+ * operation order and the shading passes keep the reference's hit-state
+ * data flow (shadow and area-light passes overwrite the closest hit, exactly
+ * as the reference's shared `Trace` does). The values the reference leaves
+ * uninitialised (hitTetrahedron's local hit, an unset dUV.v) are zero here,
+ * as in the CPU oracle.
+ *
+ * Memory: primitives, materials and lights are tiny and stay in L2/scalar
+ * caches; BVH node pairs are one aligned 64-byte segment (four 16-byte
+ * loads); a triangle test reads one aligned 64-byte record; per-hit
+ * attributes come from a separate cold record.
+ */
+#ifndef ST_TRACE_H
+#define ST_TRACE_H
+
+#include <hip/hip_runtime.h>
+
+#include "../st_math.h"
+#include "../st_layout.h"
+#include "st_blob.h"
+
+#define D_EPS 0.0001f
+#define D_SQRT3 1.73205080757f
+#define D_PI 3.14159265f
+#define D_ONE_OVER_PI (1.0f / D_PI)
+#define D_UNIT_SPHERE_AREA_OVER_SIX ((4.0f * D_PI) / 6.0f)
+#define D_COLOR_TO_FLOAT 0.00392156862f
+#define D_TRACE_OFFSET 0.0001f
+#define D_TET_A 0.577350259f
+#define D_TET_B 0.288675159f
+
+#define DFN __device__ static inline __attribute__((always_inline))
+#define DNOINLINE __device__ static __attribute__((noinline))
+
+struct DHit {
+    sv3 position, normal;
+    sv2 uv, dUV;
+    float distance, distance_squared, cone_angle, cone_width, NdotV, area, uv_area;
+    uint32_t material_id, object_id, object_type;
+    uint32_t from_behind;
+};
+
+struct DSphereHit { float b, c, t_near, t_far, furthest, closest_hit_distance, closest_hit_density; };
+
+struct DRay {
+    sv3 origin, scaled_origin, direction, direction_reciprocal;
+    uint32_t neg_x, neg_y, neg_z;   /* octant bits: direction sign */
+};
+
+struct DPrim {
+    sq4 rot;
+    sv3 pos, scale, inv_scale;
+    uint32_t flags, material_id, type, mesh_id;
+};
+
+struct DTrace {
+    DSphereHit sphere_hit;
+    DHit closest_hit, closest_mesh_hit, current_hit;
+    DRay local;
+    uint32_t depth, mesh_stack_size;
+    int32_t primary_triangle;
+    uint32_t n_shadow, n_bounce;
+    uint32_t *scene_stack, *mesh_stack;
+};
+
+struct DShaded {
+    sv3 albedo, position, normal, viewing_direction, viewing_origin, reflected_direction, light_direction;
+    sv3 quad_vertices[4];
+    sv2 uv;
+    uint32_t prim_index;
+};
+
+struct DMat {
+    sv3 albedo, reflectivity, emission;
+    sv2 uv_repeat;
+    float metallic, roughness, n1_over_n2, n2_over_n1;
+    uint32_t brdf, is, use, texture_count, tex0, tex1;
+};
+
+/* ---- loads -------------------------------------------------------------- */
+
+DFN DPrim load_prim(const StDScene &sc, uint32_t i) {
+    const float4 *p = reinterpret_cast<const float4 *>(sc.prims + i);
+    float4 a = p[0], b = p[1], c = p[2], d = p[3];
+    DPrim o;
+    o.rot.v = V3(a.x, a.y, a.z);
+    o.rot.w = a.w;
+    o.pos = V3(b.x, b.y, b.z);
+    uint32_t fm = __float_as_uint(b.w);
+    o.flags = fm & 0xFFu;
+    o.material_id = fm >> 8;
+    o.scale = V3(c.x, c.y, c.z);
+    o.type = __float_as_uint(c.w);
+    o.inv_scale = V3(d.x, d.y, d.z);
+    o.mesh_id = __float_as_uint(d.w);
+    return o;
+}
+
+DFN DMat load_mat(const StDScene &sc, uint32_t i) {
+    const float4 *p = reinterpret_cast<const float4 *>(sc.mats + i);
+    float4 a = p[0], b = p[1], c = p[2], d = p[3], e = p[4];
+    DMat m;
+    m.albedo = V3(a.x, a.y, a.z);
+    m.reflectivity = V3(a.w, b.x, b.y);
+    m.emission = V3(b.z, b.w, c.x);
+    m.uv_repeat = V2(c.y, c.z);
+    m.metallic = c.w;
+    m.roughness = d.x;
+    m.n1_over_n2 = d.y;
+    m.n2_over_n1 = d.z;
+    m.brdf = __float_as_uint(d.w);
+    uint32_t bits = __float_as_uint(e.x);
+    m.is = bits & 0xFFu;
+    m.use = (bits >> 8) & 0xFFu;
+    m.texture_count = (bits >> 16) & 0xFFu;
+    m.tex0 = __float_as_uint(e.y);
+    m.tex1 = __float_as_uint(e.z);
+    return m;
+}
+
+DFN uint32_t mat_is(const StDScene &sc, uint32_t i) { return sc.mats[i].bits & 0xFFu; }
+
+struct DNodePair {
+    sbox lb, rb;
+    uint32_t lfirst, lcount, rfirst, rcount;
+};
+
+DFN void load_node(const uint32_t *nodes, uint32_t i, sbox &b, uint32_t &first, uint32_t &count) {
+    const float4 *p = reinterpret_cast<const float4 *>(nodes + 8 * (size_t)i);
+    float4 a = p[0], c = p[1];
+    b.min = V3(a.x, a.y, a.z);
+    b.max = V3(a.w, c.x, c.y);
+    first = __float_as_uint(c.z);
+    count = __float_as_uint(c.w) & 0xFFFFu;
+}
+
+DFN DNodePair load_pair(const uint32_t *nodes, uint32_t left) {
+    const float4 *p = reinterpret_cast<const float4 *>(nodes + 8 * (size_t)left);
+    float4 a = p[0], c = p[1], d = p[2], e = p[3];
+    DNodePair o;
+    o.lb.min = V3(a.x, a.y, a.z);
+    o.lb.max = V3(a.w, c.x, c.y);
+    o.lfirst = __float_as_uint(c.z);
+    o.lcount = __float_as_uint(c.w) & 0xFFFFu;
+    o.rb.min = V3(d.x, d.y, d.z);
+    o.rb.max = V3(d.w, e.x, e.y);
+    o.rfirst = __float_as_uint(e.z);
+    o.rcount = __float_as_uint(e.w) & 0xFFFFu;
+    return o;
+}
+
+/* ---- ray set-up and slab test (core/init.h:536-543, render/AABB.h:6-23) -- */
+
+DFN void prep_ray(DRay &r) {
+    r.neg_x = __builtin_signbit(r.direction.x) ? 1u : 0u;
+    r.neg_y = __builtin_signbit(r.direction.y) ? 1u : 0u;
+    r.neg_z = __builtin_signbit(r.direction.z) ? 1u : 0u;
+    r.scaled_origin.x = -r.origin.x * r.direction_reciprocal.x;
+    r.scaled_origin.y = -r.origin.y * r.direction_reciprocal.y;
+    r.scaled_origin.z = -r.origin.z * r.direction_reciprocal.z;
+}
+
+DFN bool hit_aabb(const sbox &b, const DRay &r, float closest, float &dist) {
+    sv3 lo = V3(r.neg_x ? b.max.x : b.min.x, r.neg_y ? b.max.y : b.min.y, r.neg_z ? b.max.z : b.min.z);
+    sv3 hi = V3(r.neg_x ? b.min.x : b.max.x, r.neg_y ? b.min.y : b.max.y, r.neg_z ? b.min.z : b.max.z);
+    lo = v_mad(lo, r.direction_reciprocal, r.scaled_origin);
+    hi = v_mad(hi, r.direction_reciprocal, r.scaled_origin);
+    float h = hi.x < hi.y ? hi.x : hi.y;
+    h = h < hi.z ? h : hi.z;
+    h = h < closest ? h : closest;
+    float l = lo.x > lo.y ? lo.x : lo.y;
+    l = l > lo.z ? l : lo.z;
+    l = l > 0 ? l : 0;
+    dist = l;
+    return l <= h;
+}
+
+/* ---- instance transforms (scene/primitive.h:8-51) --------------------- */
+
+DFN void to_object(sv3 pos, sv3 dir, const DPrim &p, sv3 &opos, sv3 &odir) {
+    opos = (p.flags & ST_IS_TRANSLATED) ? v_sub(pos, p.pos) : pos;
+    if (p.flags & ST_IS_ROTATED) {
+        sq4 inv = q_conj(p.rot);
+        opos = q_rot(opos, inv);
+        odir = q_rot(dir, inv);
+    } else
+        odir = dir;
+    if (p.flags & ST_IS_SCALED) {
+        opos = v_mul(opos, p.inv_scale);
+        if (p.flags & ST_IS_SCALED_NON_UNIFORMLY) odir = v_norm(v_mul(odir, p.inv_scale));
+    }
+}
+DFN sv3 pos_to_world(sv3 pos, const DPrim &p) {
+    if (p.flags & ST_IS_SCALED) pos = v_mul(pos, p.scale);
+    if (p.flags & ST_IS_ROTATED) pos = q_rot(pos, p.rot);
+    if (p.flags & ST_IS_TRANSLATED) pos = v_add(pos, p.pos);
+    return pos;
+}
+DFN sv3 pos_to_object(sv3 pos, const DPrim &p) {
+    if (p.flags & ST_IS_TRANSLATED) pos = v_sub(pos, p.pos);
+    if (p.flags & ST_IS_ROTATED) pos = q_rot(pos, q_conj(p.rot));
+    if (p.flags & ST_IS_SCALED) pos = v_mul(pos, p.inv_scale);
+    return pos;
+}
+DFN sv3 dir_to_world(sv3 dir, const DPrim &p) {
+    if (p.flags & ST_IS_SCALED_NON_UNIFORMLY) dir = v_mul(dir, p.inv_scale);
+    if (p.flags & ST_IS_ROTATED) dir = q_rot(dir, p.rot);
+    return dir;
+}
+
+/* ---- intersection shaders --------------------------------------------- */
+
+/* common.h:115-119 (float -> u8 through a 32-bit integer) */
+DFN bool transparent_uv(sv2 uv) {
+    uint32_t v = (uint32_t)(uint8_t)(int32_t)(uv.y * 4);
+    uint32_t u = (uint32_t)(uint8_t)(int32_t)(uv.x * 4);
+    return (v % 2 != 0) == (u % 2);
+}
+
+/* common.h:177-195 */
+DFN bool hit_plane(sv3 po, sv3 pn, sv3 ro, sv3 rd, DHit &hit) {
+    hit.NdotV = -v_dot(rd, pn);
+    if (hit.NdotV == 0) return false;
+    uint32_t facing = hit.NdotV > 0;
+    sv3 RP = v_sub(po, ro);
+    float RPdotN = v_dot(RP, pn);
+    hit.from_behind = RPdotN > 0;
+    if (hit.from_behind == facing) return false;
+    hit.distance = fabsf(RPdotN / hit.NdotV);
+    hit.position = v_scale(rd, hit.distance);
+    hit.position = v_add(ro, hit.position);
+    hit.normal = pn;
+    return true;
+}
+
+/* intersection/quad.h:7-42 */
+DFN bool hit_quad(DHit &hit, sv3 Ro, sv3 Rd, uint32_t flags) {
+    if (Rd.y == 0) return false;
+    if (Ro.y == 0) return false;
+    hit.from_behind = Ro.y < 0;
+    if (hit.from_behind == (uint32_t)(Rd.y < 0)) return false;
+    hit.distance = fabsf(Ro.y / Rd.y);
+    hit.position = v_smad(Rd, hit.distance, Ro);
+    if (hit.position.x < -1 || hit.position.x > +1 || hit.position.z < -1 || hit.position.z > +1) return false;
+    hit.uv.x = (hit.position.x + 1.0f) * 0.5f;
+    hit.uv.y = (hit.position.z + 1.0f) * 0.5f;
+    if ((flags & ST_IS_TRANSPARENT) && transparent_uv(hit.uv)) return false;
+    hit.normal = V3(0, 1, 0);
+    hit.NdotV = -Rd.y;
+    hit.area = 4;
+    hit.uv_area = 1;
+    return true;
+}
+
+enum { D_SIDE_NONE = 0, D_TOP = 1, D_BOTTOM = 2, D_LEFT = 4, D_RIGHT = 8, D_FRONT = 16, D_BACK = 32 };
+
+/* common.h:121-130 */
+DFN int box_side(sv3 oct, int axis) {
+    switch (axis) {
+        case 0: return oct.x > 0 ? D_RIGHT : D_LEFT;
+        case 3: return oct.x > 0 ? D_LEFT : D_RIGHT;
+        case 1: return oct.y > 0 ? D_TOP : D_BOTTOM;
+        case 4: return oct.y > 0 ? D_BOTTOM : D_TOP;
+        case 2: return oct.z > 0 ? D_FRONT : D_BACK;
+        default: return oct.z > 0 ? D_BACK : D_FRONT;
+    }
+}
+/* common.h:132-168 */
+DFN sv2 cube_uv(sv3 p, int side) {
+    sv2 uv;
+    switch (side) {
+        case D_TOP: uv.x = p.x; uv.y = p.z; break;
+        case D_BOTTOM: uv.x = -p.x; uv.y = -p.z; break;
+        case D_LEFT: uv.x = -p.z; uv.y = p.y; break;
+        case D_RIGHT: uv.x = p.z; uv.y = p.y; break;
+        case D_FRONT: uv.x = p.x; uv.y = p.y; break;
+        default: uv.x = -p.x; uv.y = p.y; break;
+    }
+    uv.x += 1;
+    uv.y += 1;
+    uv.x *= 0.5f;
+    uv.y *= 0.5f;
+    return uv;
+}
+
+/* intersection/box.h:7-72 */
+DFN bool hit_box(DHit &hit, sv3 Ro, sv3 Rd, uint32_t flags) {
+    sv3 rcp = v_rcp(Rd), oct;
+    oct.x = __builtin_signbit(Rd.x) ? 1.0f : -1.0f;
+    oct.y = __builtin_signbit(Rd.y) ? 1.0f : -1.0f;
+    oct.z = __builtin_signbit(Rd.z) ? 1.0f : -1.0f;
+    float t0 = (oct.x - Ro.x) * rcp.x;
+    float t1 = (oct.y - Ro.y) * rcp.y;
+    float t2 = (oct.z - Ro.z) * rcp.z;
+    float t3 = (-oct.x - Ro.x) * rcp.x;
+    float t4 = (-oct.y - Ro.y) * rcp.y;
+    float t5 = (-oct.z - Ro.z) * rcp.z;
+    int max_axis = t3 < t4 ? 3 : 4;
+    float tmax_ax = max_axis == 3 ? t3 : t4;
+    if (t5 < tmax_ax) { max_axis = 5; tmax_ax = t5; }
+    float maxt = tmax_ax;
+    if (maxt < 0) return false;
+    int min_axis = t0 > t1 ? 0 : 1;
+    float tmin_ax = min_axis == 0 ? t0 : t1;
+    if (t2 > tmin_ax) { min_axis = 2; tmin_ax = t2; }
+    float mint = tmin_ax;
+    if (maxt < (mint > 0 ? mint : 0)) return false;
+    hit.from_behind = mint < 0;
+    if (hit.from_behind) { mint = maxt; min_axis = max_axis; }
+    int side = box_side(oct, min_axis);
+    hit.position = v_smad(Rd, mint, Ro);
+    hit.uv = cube_uv(hit.position, side);
+    if ((flags & ST_IS_TRANSPARENT) && transparent_uv(hit.uv)) {
+        if (hit.from_behind) return false;
+        side = box_side(oct, max_axis);
+        hit.position = v_smad(Rd, maxt, Ro);
+        hit.uv = cube_uv(hit.position, side);
+        if (transparent_uv(hit.uv)) return false;
+        hit.from_behind = 1;
+    }
+    hit.normal = V3s(0);
+    switch (side) {
+        case D_LEFT: hit.normal.x = -1.0f; hit.NdotV = +Rd.x; break;
+        case D_RIGHT: hit.normal.x = +1.0f; hit.NdotV = -Rd.x; break;
+        case D_BOTTOM: hit.normal.y = -1.0f; hit.NdotV = +Rd.y; break;
+        case D_TOP: hit.normal.y = +1.0f; hit.NdotV = -Rd.y; break;
+        case D_BACK: hit.normal.z = -1.0f; hit.NdotV = +Rd.z; break;
+        case D_FRONT: hit.normal.z = +1.0f; hit.NdotV = -Rd.z; break;
+        default: return false;
+    }
+    if (hit.from_behind) { hit.normal = v_neg(hit.normal); hit.NdotV = -hit.NdotV; }
+    hit.area = 4;
+    hit.uv_area = 1;
+    return true;
+}
+
+/* intersection/sphere.h:7-42 */
+DFN sv2 sphere_uv(sv3 d) {
+    sv2 uv;
+    float x = d.x, y = d.y, z = d.z;
+    float z_over_x = x != 0 ? (z / x) : 2;
+    float y_over_x = x != 0 ? (y / x) : 2;
+    if (z_over_x <= 1 && z_over_x >= -1 && y_over_x <= 1 && y_over_x >= -1) {
+        uv.x = z_over_x;
+        uv.y = x > 0 ? y_over_x : -y_over_x;
+    } else {
+        float x_over_z = z != 0 ? (x / z) : 2;
+        float y_over_z = z != 0 ? (y / z) : 2;
+        if (x_over_z <= 1 && x_over_z >= -1 && y_over_z <= 1 && y_over_z >= -1) {
+            uv.x = -x_over_z;
+            uv.y = z > 0 ? y_over_z : -y_over_z;
+        } else {
+            uv.x = x / (y > 0 ? y : -y);
+            uv.y = z / y;
+        }
+    }
+    uv.x += 1; uv.x /= 2;
+    uv.y += 1; uv.y /= 2;
+    return uv;
+}
+
+/* sphere.h:44-59 */
+DFN bool hit_sphere_simple(sv3 Ro, sv3 Rd, sv3 target, float one_over_radius, DSphereHit &h) {
+    sv3 rc = v_scale(v_sub(target, Ro), one_over_radius);
+    h.b = v_dot(Rd, rc);
+    h.c = v_dot(rc, rc) - 1;
+    float d = h.b * h.b - h.c;
+    if (d < 0) return false;
+    d = sqrtf(d);
+    h.t_near = h.b - d;
+    h.t_far = h.b + d;
+    return h.t_far > 0 && h.t_near < h.furthest;
+}
+
+/* sphere.h:61-97 */
+DFN bool hit_sphere(DHit &hit, sv3 Ro, sv3 Rd, uint32_t flags) {
+    float t_to_closest = -v_dot(Ro, Rd);
+    if (t_to_closest <= 0) return false;
+    hit.distance_squared = v_sqlen(Ro) - t_to_closest * t_to_closest;
+    float delta_squared = 1 - hit.distance_squared;
+    if (delta_squared <= 0) { hit.distance = t_to_closest; return false; }
+    float delta = sqrtf(delta_squared);
+    hit.distance = t_to_closest - delta;
+    bool has_outer_hit = hit.distance > 0;
+    if (has_outer_hit) {
+        hit.position = v_smad(Rd, hit.distance, Ro);
+        hit.uv = sphere_uv(hit.position);
+        if ((flags & ST_IS_TRANSPARENT) && transparent_uv(hit.uv)) has_outer_hit = false;
+    }
+    if (!has_outer_hit) {
+        hit.distance = t_to_closest + delta;
+        hit.position = v_smad(Rd, hit.distance, Ro);
+        hit.uv = sphere_uv(hit.position);
+        if ((flags & ST_IS_TRANSPARENT) && transparent_uv(hit.uv)) return false;
+    }
+    hit.from_behind = !has_outer_hit;
+    hit.normal = hit.position;
+    hit.NdotV = -v_dot(hit.normal, Rd);
+    hit.area = hit.uv_area = D_UNIT_SPHERE_AREA_OVER_SIX;
+    return true;
+}
+
+/* intersection/tetrahedra.h:9-107 */
+DFN sm3 tet_frame(int t) {
+    const float A = D_TET_A, B = D_TET_B;
+    sm3 m;
+    if (t == 0) { m.X = V3(A, -B, -A); m.Y = V3(B, B, A); m.Z = V3(-B, A, -A); }
+    else if (t == 1) { m.X = V3(B, B, A); m.Y = V3(-B, A, -A); m.Z = V3(A, -B, -A); }
+    else if (t == 2) { m.X = V3(-B, A, -A); m.Y = V3(A, -B, -A); m.Z = V3(B, B, A); }
+    else { m.X = V3(-A, B, A); m.Y = V3(B, B, A); m.Z = V3(B, -A, A); }
+    return m;
+}
+DFN bool hit_tetrahedron(DHit &closest_hit, sv3 Ro, sv3 Rd, uint32_t flags) {
+    bool found = false;
+    float closest_distance = INFINITY;
+    DHit hit = {};
+    for (int t = 0; t < 4; t++) {
+        sv3 tangent_pos = V3s(t == 3 ? D_TET_A : -D_TET_A);
+        hit.normal = tangent_pos;
+        if (t == 0) hit.normal.y = D_TET_A;
+        else if (t == 1) hit.normal.x = D_TET_A;
+        else if (t == 2) hit.normal.z = D_TET_A;
+        else tangent_pos.y = -D_TET_A;
+        if (!hit_plane(tangent_pos, hit.normal, Ro, Rd, hit)) continue;
+        sm3 m = tet_frame(t);
+        hit.position = v_smad(Rd, hit.distance, Ro);
+        tangent_pos = v_sub(hit.position, tangent_pos);
+        tangent_pos = v_mat(tangent_pos, m);
+        if (tangent_pos.x < 0 || tangent_pos.y < 0 || tangent_pos.y + tangent_pos.x > 1) continue;
+        hit.uv.x = tangent_pos.x;
+        hit.uv.y = tangent_pos.y;
+        if ((flags & ST_IS_TRANSPARENT) && transparent_uv(hit.uv)) continue;
+        if (hit.distance < closest_distance) {
+            closest_distance = hit.distance;
+            hit.area = D_SQRT3 / 4.0f;
+            hit.uv_area = 1;
+            hit.NdotV = -v_dot(hit.normal, Rd);
+            closest_hit = hit;
+            found = true;
+        }
+    }
+    return found;
+}
+
+/* intersection/mesh.h:5-37 over the 64-byte hot triangle records */
+DFN bool hit_triangles(DTrace &tr, const StDTriHot *tris, uint32_t first, uint32_t count, bool any_hit) {
+    DRay &ray = tr.local;
+    DHit &hit = tr.current_hit;
+    DHit &closest = tr.closest_mesh_hit;
+    bool found = false;
+    for (uint32_t i = 0; i < count; i++) {
+        const float4 *p = reinterpret_cast<const float4 *>(tris + first + i);
+        float4 a = p[0], b = p[1], c = p[2], d = p[3];
+        sv3 tpos = V3(c.y, c.z, c.w);
+        sv3 tnrm = V3(d.x, d.y, d.z);
+        if (hit_plane(tpos, tnrm, ray.origin, ray.direction, hit) && hit.distance < closest.distance) {
+            sm3 w;
+            w.X = V3(a.x, a.y, a.z);
+            w.Y = V3(a.w, b.x, b.y);
+            w.Z = V3(b.z, b.w, c.x);
+            sv3 UV = v_sub(hit.position, tpos);
+            UV = v_mat(UV, w);
+            if (UV.x < 0 || UV.y < 0 || (UV.x + UV.y) > 1) continue;
+            closest.NdotV = hit.NdotV;
+            closest.normal = hit.normal;
+            closest.uv.x = UV.x;
+            closest.uv.y = UV.y;
+            closest.object_id = i;
+            closest.from_behind = hit.from_behind;
+            closest.distance = hit.distance;
+            closest.position = hit.position;
+            closest.area = 0;       /* areas are fetched from the cold record once the hit is final */
+            closest.uv_area = 0;
+            found = true;
+            if (any_hit) break;
+        }
+    }
+    return found;
+}
+
+/* intersection/mesh.h:38-111 */
+DNOINLINE bool trace_mesh(DTrace &tr, const StDScene &sc, uint32_t mesh_id, bool any_hit) {
+    const StDMesh mesh = sc.meshes[mesh_id];
+    const uint32_t *nodes = sc.mesh_nodes + 8 * (size_t)mesh.node_offset;
+    const StDTriHot *tris = sc.tri_hot + mesh.tri_offset;
+    DRay &ray = tr.local;
+    DHit &closest = tr.closest_mesh_hit;
+    uint32_t *stack = tr.mesh_stack;
+    bool found = false;
+    float ld, rd;
+    sbox root;
+    uint32_t root_first, root_count;
+    load_node(nodes, 0, root, root_first, root_count);
+    if (!hit_aabb(root, ray, closest.distance, ld)) return false;
+    if (root_count) return hit_triangles(tr, tris, 0, mesh.triangle_count, any_hit);
+    uint32_t left = root_first;
+    bool has_left, has_right;
+    uint32_t stack_size = 0;
+    for (;;) {
+        DNodePair np = load_pair(nodes, left);
+        bool hl = hit_aabb(np.lb, ray, closest.distance, ld);
+        bool hr = hit_aabb(np.rb, ray, closest.distance, rd);
+        has_left = hl;
+        has_right = hr;
+        uint32_t lnext = np.lfirst, rnext = np.rfirst;
+        if (hl && np.lcount) {
+            if (hit_triangles(tr, tris, np.lfirst, np.lcount, any_hit)) {
+                closest.object_id += np.lfirst;
+                found = true;
+                if (any_hit) break;
+            }
+            has_left = false;
+        }
+        if (hr && np.rcount) {
+            if (hit_triangles(tr, tris, np.rfirst, np.rcount, any_hit)) {
+                closest.object_id += np.rfirst;
+                found = true;
+                if (any_hit) break;
+            }
+            has_right = false;
+        }
+        if (has_left) {
+            if (has_right) {
+                if (!any_hit && ld > rd) { uint32_t t = lnext; lnext = rnext; rnext = t; }
+                stack[stack_size++] = rnext;
+                if (stack_size == tr.mesh_stack_size) return false;
+            }
+            left = lnext;
+        } else if (has_right) {
+            left = rnext;
+        } else {
+            if (stack_size == 0) break;
+            left = stack[--stack_size];
+        }
+    }
+    return found;
+}
+
+/* intersection/primitives.h:9-111. `ids` are scene-BVH leaf ids. */
+DNOINLINE bool hit_primitives(const DRay &ray, DTrace &tr, const StDScene &sc, const uint32_t *ids, uint32_t count,
+                              bool any_hit, const int32_t *rects, uint32_t x, uint32_t y) {
+    bool found = false, cur = false;
+    DHit &hit = tr.current_hit;
+    DHit &closest = tr.closest_hit;
+    float cone_angle = closest.cone_angle;
+    uint32_t hit_prim_id = 0;
+    for (uint32_t i = 0; i < count; i++) {
+        uint32_t pid = ids[i];
+        DPrim p = load_prim(sc, pid);
+        if (any_hit && !(p.flags & ST_IS_SHADOWING)) continue;
+        if (rects) {
+            /* IS_VISIBLE is folded into the per-view rect: an invisible
+               primitive has an empty rect. */
+            int4 r = reinterpret_cast<const int4 *>(rects)[pid];
+            if ((int32_t)x < r.x || (int32_t)x >= r.z || (int32_t)y < r.y || (int32_t)y >= r.w) continue;
+        }
+        sv3 Ro, Rd;
+        to_object(ray.origin, ray.direction, p, Ro, Rd);
+        Ro = v_smad(Rd, D_TRACE_OFFSET, Ro);
+        tr.local.origin = Ro;
+        tr.local.direction = Rd;
+        switch (p.type) {
+            case ST_PRIM_QUAD: cur = hit_quad(hit, Ro, Rd, p.flags); break;
+            case ST_PRIM_BOX: cur = hit_box(hit, Ro, Rd, p.flags); break;
+            case ST_PRIM_SPHERE: cur = hit_sphere(hit, Ro, Rd, p.flags); break;
+            case ST_PRIM_TETRAHEDRON: cur = hit_tetrahedron(hit, Ro, Rd, p.flags); break;
+            case ST_PRIM_MESH:
+                tr.local.direction_reciprocal = v_rcp(Rd);
+                tr.closest_mesh_hit.distance = closest.distance == INFINITY
+                    ? INFINITY : v_len(v_sub(pos_to_object(closest.position, p), Ro));
+                prep_ray(tr.local);
+                cur = trace_mesh(tr, sc, p.mesh_id, any_hit);
+                if (cur) hit = tr.closest_mesh_hit;
+                break;
+            default: continue;
+        }
+        if (cur) {
+            hit.position = pos_to_world(hit.position, p);
+            hit.distance_squared = v_sqlen(v_sub(hit.position, ray.origin));
+            if (hit.distance_squared < closest.distance_squared) {
+                closest = hit;
+                hit_prim_id = pid;
+                closest.object_type = p.type;
+                closest.material_id = p.material_id;
+                found = true;
+                if (any_hit) break;
+            }
+        }
+    }
+    if (found) {
+        DPrim hp = load_prim(sc, hit_prim_id);
+        tr.primary_triangle = -1;
+        if (closest.object_type == ST_PRIM_MESH) {
+            const StDMesh mesh = sc.meshes[hp.mesh_id];
+            uint32_t tri = closest.object_id;
+            tr.primary_triangle = (int32_t)tri;
+            const float4 *q = reinterpret_cast<const float4 *>(sc.tri_cold + mesh.tri_offset + tri);
+            float4 c0 = q[0], c1 = q[1], c2 = q[2], c3 = q[3], c4 = q[4];
+            /* the hot record carried no areas: restore the reference's
+               closest_hit->area / uv_area (mesh.h:137-138) */
+            closest.area = c4.x;
+            closest.uv_area = c4.y;
+            if (mesh.attrs) {
+                float u = closest.uv.x, v = closest.uv.y;
+                float w = 1 - u - v;
+                if (mesh.attrs & 2u) {
+                    /* uvs[0] = (c2.y, c2.z), uvs[1] = (c2.w, c3.x), uvs[2] = (c3.y, c3.z) */
+                    closest.uv.x = c3.y * u + (c2.w * v + c2.y * w);
+                    closest.uv.y = c3.z * u + (c3.x * v + c2.z * w);
+                }
+                if (mesh.attrs & 1u) {
+                    /* vn[0] = (c0.x,c0.y,c0.z), vn[1] = (c0.w,c1.x,c1.y), vn[2] = (c1.z,c1.w,c2.x) */
+                    closest.normal.x = c1.z * u + (c0.w * v + c0.x * w);
+                    closest.normal.y = c1.w * u + (c1.x * v + c0.y * w);
+                    closest.normal.z = c2.x * u + (c1.y * v + c0.z * w);
+                    closest.normal = v_norm(closest.normal);
+                }
+            }
+        }
+        closest.object_id = hit_prim_id;
+        closest.normal = v_norm(dir_to_world(closest.normal, hp));
+        closest.distance = sqrtf(closest.distance_squared);
+        closest.cone_width = 2.0f * cone_angle * closest.distance;
+        sv3 rotated_scale = q_rot(hp.scale, hp.rot);
+        closest.area *= v_dot(v_one_minus(closest.normal), rotated_scale);
+    }
+    return found;
+}
+
+/* trace.h:8-76 */
+DNOINLINE bool trace_scene(DRay &ray, DTrace &tr, const StDScene &sc, bool any_hit) {
+    ray.direction_reciprocal = v_rcp(ray.direction);
+    prep_ray(ray);
+    bool found = false;
+    float ld, rd;
+    sbox root;
+    uint32_t root_first, root_count;
+    load_node(sc.scene_nodes, 0, root, root_first, root_count);
+    if (!hit_aabb(root, ray, tr.closest_hit.distance, ld)) return false;
+    if (root_count) return hit_primitives(ray, tr, sc, sc.scene_leaf_ids, sc.prim_count, any_hit, nullptr, 0, 0);
+    uint32_t left = root_first;
+    uint32_t *stack = tr.scene_stack;
+    uint32_t stack_size = 0;
+    for (;;) {
+        DNodePair np = load_pair(sc.scene_nodes, left);
+        bool hl = hit_aabb(np.lb, ray, tr.closest_hit.distance, ld);
+        bool hr = hit_aabb(np.rb, ray, tr.closest_hit.distance, rd);
+        bool has_left = hl, has_right = hr;
+        uint32_t lnext = np.lfirst, rnext = np.rfirst;
+        if (hl && np.lcount) {
+            if (hit_primitives(ray, tr, sc, sc.scene_leaf_ids + np.lfirst, np.lcount, any_hit, nullptr, 0, 0)) {
+                found = true;
+                if (any_hit) break;
+            }
+            has_left = false;
+        }
+        if (hr && np.rcount) {
+            if (hit_primitives(ray, tr, sc, sc.scene_leaf_ids + np.rfirst, np.rcount, any_hit, nullptr, 0, 0)) {
+                found = true;
+                if (any_hit) break;
+            }
+            has_right = false;
+        }
+        if (has_left) {
+            if (has_right) {
+                if (!any_hit && ld > rd) { uint32_t t = lnext; lnext = rnext; rnext = t; }
+                stack[stack_size++] = rnext;
+            }
+            left = lnext;
+        } else if (has_right) {
+            left = rnext;
+        } else {
+            if (stack_size == 0) break;
+            left = stack[--stack_size];
+        }
+    }
+    return found;
+}
+
+DFN bool in_shadow(DRay &ray, DTrace &tr, const StDScene &sc) {
+    tr.n_shadow++;
+    return trace_scene(ray, tr, sc, true);
+}
+DFN bool trace_ray(DRay &ray, DTrace &tr, const StDScene &sc) {
+    tr.n_bounce++;
+    tr.closest_hit.distance = tr.closest_hit.distance_squared = INFINITY;
+    return trace_scene(ray, tr, sc, false);
+}
+
+/* ---- shading ----------------------------------------------------------- */
+
+DFN float tone_map(float c) {
+    float x = c - 0.004f;
+    if (x < 0.0f) x = 0.0f;
+    float x2s = x * x * 6.2f;
+    return (x2s + x * 0.5f) / (x2s + x * 1.7f + 0.06f);
+}
+DFN float inv_dot(sv3 a, sv3 b) { return f_clamp01(-v_dot(a, b)); }
+DFN sv3 reflect_with_dot(sv3 V, sv3 N, float NdotV) { return v_smad(N, -2 * NdotV, V); }
+DFN sv3 refract_dir(sv3 V, sv3 N, float n1_over_n2, float NdotV) {
+    float c = n1_over_n2 * n1_over_n2 * (1 - (NdotV * NdotV));
+    if (c > 1) return V3s(0);
+    V = v_scale(V, n1_over_n2);
+    V = v_smad(N, n1_over_n2 * -NdotV - sqrtf(1 - c), V);
+    return v_norm(V);
+}
+
+/* common.h:197-228 */
+DFN sv3 area_light_vector(const DPrim &p, sv3 P, sv3 *v) {
+    if (p.scale.x == 0 || p.scale.z == 0) return V3s(0);
+    sv3 U = V3s(0), W = V3s(0);
+    U.x = p.scale.x; if (U.x < 0) U.x = -U.x;
+    W.z = p.scale.z; if (W.z < 0) W.z = -W.z;
+    U = q_rot(U, p.rot);
+    W = q_rot(W, p.rot);
+    v[0] = v_sub(v_sub(p.pos, U), W);
+    v[1] = v_sub(v_add(p.pos, U), W);
+    v[2] = v_add(v_add(p.pos, U), W);
+    v[3] = v_add(v_sub(p.pos, U), W);
+    sv3 u1n = v_norm(v_sub(v[0], P));
+    sv3 u2n = v_norm(v_sub(v[1], P));
+    sv3 u3n = v_norm(v_sub(v[2], P));
+    sv3 u4n = v_norm(v_sub(v[3], P));
+    float a12 = acosf(v_dot(u1n, u2n));
+    float a23 = acosf(v_dot(u2n, u3n));
+    float a34 = acosf(v_dot(u3n, u4n));
+    float a41 = acosf(v_dot(u4n, u1n));
+    sv3 out = v_scale(v_cross(u1n, u2n), a12);
+    out = v_add(out, v_scale(v_cross(u2n, u3n), a23));
+    out = v_add(out, v_scale(v_cross(u3n, u4n), a34));
+    out = v_add(out, v_scale(v_cross(u4n, u1n), a41));
+    return v_scale(out, 0.5f);
+}
+
+/* common.h:277-334 */
+DFN float ggx_ndf(float r2, float NdotH) {
+    float demon = NdotH * NdotH * (r2 - 1.0f) + 1.0f;
+    return D_ONE_OVER_PI * r2 / (demon * demon);
+}
+DFN float ggx_smith_schlick(float NdotL, float NdotV, float roughness) {
+    float k = roughness * 0.5f;
+    float one_minus_k = 1.0f - k;
+    float denom = NdotV * one_minus_k + k;
+    float result = NdotV / fmaxf(denom, D_EPS);
+    denom = NdotL * one_minus_k + k;
+    return result * NdotL / fmaxf(denom, D_EPS);
+}
+DFN sv3 shade_classic(sv3 diffuse, float NdotL, sv3 specular, float f, float exponent, float roughness) {
+    float shininess = 1.0f - roughness;
+    f = NdotL * powf(f, exponent * shininess) * shininess;
+    return v_smad(specular, f, diffuse);
+}
+DFN sv3 shade_point(const DShaded &s, const DMat &M, float NdotL) {
+    sv3 diffuse = v_scale(s.albedo, M.roughness * NdotL * D_ONE_OVER_PI);
+    if (M.brdf == ST_BRDF_LAMBERT) return diffuse;
+    if (M.brdf == ST_BRDF_PHONG) {
+        float RdotL = v_dotc(s.reflected_direction, s.light_direction);
+        return RdotL != 0 ? shade_classic(diffuse, NdotL, M.reflectivity, RdotL, 4.0f, M.roughness) : diffuse;
+    }
+    sv3 N = s.normal, L = s.light_direction, V = v_neg(s.viewing_direction);
+    sv3 H = v_norm(v_add(L, V));
+    float NdotH = v_dotc(N, H);
+    if (M.brdf == ST_BRDF_BLINN)
+        return NdotH != 0 ? shade_classic(diffuse, NdotL, M.reflectivity, NdotH, 16.0f, M.roughness) : diffuse;
+    sv3 F = V3s(0);
+    diffuse = v_scale(s.albedo, (1.0f - M.metallic) * NdotL * D_ONE_OVER_PI);
+    float LdotH = v_dotc(L, H);
+    if (LdotH != 1.0f) {
+        F = v_smad(v_one_minus(M.reflectivity), powf(1.0f - LdotH, 5.0f), M.reflectivity);
+        diffuse = v_mul(diffuse, v_one_minus(F));
+    }
+    if (M.roughness == 0 && NdotH == 1.0f) return diffuse;
+    float NdotV = v_dotc(N, V);
+    if (NdotV == 0) return diffuse;
+    float D = ggx_ndf(M.roughness * M.roughness, NdotH);
+    float G = ggx_smith_schlick(NdotL, NdotV, M.roughness);
+    return v_smad(F, D * G / (4.0f * NdotV), diffuse);
+}
+
+DFN float duv_ray_cone(float NdotV, float cone_width, float area, float uv_area) {
+    float pcw = cone_width / fabsf(NdotV);
+    return sqrtf((pcw * pcw) * (uv_area / area));
+}
+
+/* scene/texture.h:5-40: one 12-byte quad load per bilinear sample. Indices
+   are clamped to the quad grid so that out-of-range uvs (undefined in the
+   reference) cannot leave the texture. */
+DFN sv3 sample_texture(const StDScene &sc, uint32_t tex_id, sv2 UV, sv2 dUV) {
+    const StDTex tex = sc.textures[tex_id];
+    uint32_t mip_level = 0;
+    if (tex.mipmap) {
+        const float pw = dUV.x * (float)tex.width;
+        const float ph = dUV.y * (float)tex.height;
+        float pixel_size = (float)(pw + ph) * 0.5f;
+        const uint32_t last_mip = (uint8_t)(tex.mip_count - 1);
+        while (pixel_size > 1 && mip_level < last_mip) { pixel_size /= 2; mip_level += 1; }
+    }
+    const StDMip mip = sc.mips[tex.mip_base + mip_level];
+    float u = UV.x, v = UV.y;
+    if (u > 1) u -= (float)((uint32_t)u);
+    if (v > 1) v -= (float)((uint32_t)v);
+    const float U = u * (float)mip.width + 0.5f;
+    const float Vv = v * (float)mip.height + 0.5f;
+    uint32_t x = (uint32_t)U, y = (uint32_t)Vv;
+    const float r = U - (float)x, b = Vv - (float)y;
+    const float l = 1 - r, t = 1 - b;
+    const float tl = t * l * D_COLOR_TO_FLOAT, trr = t * r * D_COLOR_TO_FLOAT;
+    const float bl = b * l * D_COLOR_TO_FLOAT, br = b * r * D_COLOR_TO_FLOAT;
+    x = x < mip.width ? x : mip.width;
+    y = y < mip.height ? y : mip.height;
+    const uint32_t *q = reinterpret_cast<const uint32_t *>(sc.texels + 12 * (mip.quad_offset + (uint64_t)y * (mip.width + 1) + x));
+    uint32_t R = q[0], G = q[1], B = q[2];
+    return V3((float)(R >> 24) * br + ((float)((R >> 16) & 255) * bl + ((float)((R >> 8) & 255) * trr + (float)(R & 255) * tl)),
+              (float)(G >> 24) * br + ((float)((G >> 16) & 255) * bl + ((float)((G >> 8) & 255) * trr + (float)(G & 255) * tl)),
+              (float)(B >> 24) * br + ((float)((B >> 16) & 255) * bl + ((float)((B >> 8) & 255) * trr + (float)(B & 255) * tl)));
+}
+DFN sv3 sample_normal(const StDScene &sc, uint32_t tex_id, sv2 uv, sv2 dUV) {
+    sv3 n = sample_texture(sc, tex_id, uv, dUV);
+    float y = n.z;
+    n.z = n.y;
+    n.y = y;
+    return v_norm(v_smad(n, 2.0f, V3s(-1.0f)));
+}
+DFN sq4 normal_rotation(sv3 n) {
+    sq4 q;
+    q.v.x = n.z * 0.25f;
+    q.v.y = 0;
+    q.v.z = n.x * 0.25f;
+    q.w = n.y * 0.25f;
+    return q_norm(q);
+}
+
+/* surface.h:9-35 */
+DNOINLINE sv3 shade_lights_direct(DShaded &s, const DMat &M, DRay &ray, DTrace &tr, const StDScene &sc, sv3 color) {
+    DHit &hit = tr.closest_hit;
+    for (uint32_t i = 0; i < sc.light_count; i++) {
+        const float4 *lp = reinterpret_cast<const float4 *>(sc.lights + i);
+        float4 la = lp[0], lb = lp[1];
+        hit.position = V3(la.x, la.y, la.z);
+        s.light_direction = v_sub(hit.position, s.position);
+        hit.distance_squared = v_sqlen(s.light_direction);
+        hit.distance = sqrtf(hit.distance_squared);
+        s.light_direction = v_scale(s.light_direction, 1.0f / hit.distance);
+        float NdotL = v_dot(s.normal, s.light_direction);
+        if (NdotL <= 0) continue;
+        float intensity = la.w / hit.distance_squared;
+        ray.origin = s.position;
+        ray.direction = s.light_direction;
+        if (in_shadow(ray, tr, sc)) continue;
+        color = v_mad(shade_point(s, M, NdotL), v_scale(V3(lb.x, lb.y, lb.z), intensity), color);
+    }
+    return color;
+}
+
+/* surface.h:37-129 */
+DNOINLINE bool shade_emissive_quads(DShaded &s, const DMat &M, DRay &ray, DTrace &tr, const StDScene &sc, sv3 &color) {
+    DHit &hit = tr.current_hit;
+    sv3 Ro, Rd;
+    bool found = false;
+    for (uint32_t i = 0; i < sc.prim_count; i++) {
+        DPrim quad = load_prim(sc, i);
+        if (quad.type != ST_PRIM_QUAD) continue;
+        uint32_t em_is = mat_is(sc, quad.material_id);
+        if (!(em_is & ST_EMISSIVE)) continue;
+        to_object(s.viewing_origin, s.viewing_direction, quad, Ro, Rd);
+        tr.local.origin = Ro;
+        tr.local.direction = Rd;
+        if (hit_quad(hit, Ro, Rd, quad.flags)) {
+            hit.position = pos_to_world(hit.position, quad);
+            hit.distance_squared = v_sqlen(v_sub(hit.position, s.viewing_origin));
+            if (hit.distance_squared < tr.closest_hit.distance_squared) {
+                hit.object_id = i;
+                hit.object_type = ST_PRIM_QUAD;
+                hit.material_id = quad.material_id;
+                tr.closest_hit = hit;
+                found = true;
+            }
+        }
+        sv3 qn = q_rot(V3(0, 1, 0), quad.rot);
+        s.light_direction = ray.direction = v_sub(quad.pos, s.position);
+        if (v_dot(qn, s.light_direction) >= 0) continue;
+        float emission_intensity = v_dot(s.normal, area_light_vector(quad, s.position, s.quad_vertices));
+        if (emission_intensity > 0) {
+            bool skip = true;
+            for (int j = 0; j < 4; j++)
+                if (v_dot(s.normal, v_sub(s.quad_vertices[j], s.position)) >= 0) { skip = false; break; }
+            if (skip) continue;
+            float shaded_light = 1;
+            for (uint32_t k = 0; k < sc.prim_count; k++) {
+                if (i == s.prim_index || i == k) continue;
+                DPrim sp = load_prim(sc, k);
+                if (v_dot(qn, v_sub(sp.pos, quad.pos)) <= 0) continue;
+                to_object(s.position, s.light_direction, sp, Ro, Rd);
+                Ro = v_smad(Rd, D_TRACE_OFFSET, Ro);
+                tr.local.origin = Ro;
+                tr.local.direction = Rd;
+                float d = 1;
+                if (sp.type == ST_PRIM_SPHERE) {
+                    if (hit_sphere(hit, Ro, Rd, sp.flags))
+                        d -= (1.0f - sqrtf(hit.distance_squared)) / (hit.distance * emission_intensity * 3);
+                } else if (sp.type == ST_PRIM_QUAD) {
+                    if (hit_quad(hit, Ro, Rd, sp.flags)) {
+                        hit.position.y = 0;
+                        hit.position.x = hit.position.x < 0 ? -hit.position.x : hit.position.x;
+                        hit.position.z = hit.position.z < 0 ? -hit.position.z : hit.position.z;
+                        if (hit.position.x > hit.position.z) {
+                            hit.position.y = hit.position.z;
+                            hit.position.z = hit.position.x;
+                            hit.position.x = hit.position.y;
+                            hit.position.y = 0;
+                        }
+                        d -= (1.0f - hit.position.z) / (hit.distance * emission_intensity);
+                    }
+                }
+                if (d < shaded_light) shaded_light = d;
+            }
+            if (shaded_light > 0) {
+                float NdotL = v_dotc(s.normal, s.light_direction);
+                if (NdotL != 0) {
+                    const float4 *mp = reinterpret_cast<const float4 *>(sc.mats + quad.material_id);
+                    float4 mb = mp[1], mc = mp[2];
+                    sv3 emission = V3(mb.z, mb.w, mc.x);
+                    color = v_mad(shade_point(s, M, NdotL), v_scale(emission, emission_intensity * shaded_light), color);
+                }
+            }
+        }
+    }
+    if (found) tr.closest_hit.distance = sqrtf(tr.closest_hit.distance_squared);
+    return found;
+}
+
+/* closest_hit/lights.h:7-39 */
+DFN float volume_density(const DSphereHit &h) {
+    float t1 = h.t_near > 0 ? h.t_near : 0;
+    float t2 = h.t_far < h.furthest ? h.t_far : h.furthest;
+    return (h.c * t1 - h.b * t1 * t1 + t1 * t1 * t1 / 3.0f - (h.c * t2 - h.b * t2 * t2 + t2 * t2 * t2 / 3.0f)) * (3.0f / 4.0f);
+}
+DNOINLINE bool shade_light_glow(const StDScene &sc, sv3 Ro, sv3 Rd, float max_distance, DSphereHit &sh, sv3 &color) {
+    bool hit_light = false;
+    sh.closest_hit_distance = INFINITY;
+    for (uint32_t i = 0; i < sc.light_count; i++) {
+        const float4 *lp = reinterpret_cast<const float4 *>(sc.lights + i);
+        float4 la = lp[0], lb = lp[1];
+        float size = la.w * (1.0f / 16.0f);
+        float one_over_radius = 8.0f / size;
+        sh.furthest = max_distance * one_over_radius;
+        if (hit_sphere_simple(Ro, Rd, V3(la.x, la.y, la.z), one_over_radius, sh)) {
+            float distance = sh.t_far - size * 8;
+            float density = powf(volume_density(sh), 8) * 8;
+            if (distance < sh.closest_hit_distance) {
+                sh.closest_hit_distance = distance;
+                sh.closest_hit_density = density;
+            }
+            color = v_smad(V3(lb.x, lb.y, lb.z), density, color);
+            hit_light = true;
+        }
+    }
+    return hit_light;
+}
+
+/* surface.h:159-168 (both dUV components) and :227-236 (dUV.u only) */
+DFN void apply_textures(DTrace &tr, const StDScene &sc, DShaded &s, const DMat &M, sv2 &dUV, bool set_both) {
+    DHit &hit = tr.closest_hit;
+    s.uv = V2(hit.uv.x * M.uv_repeat.x, hit.uv.y * M.uv_repeat.y);
+    hit.uv_area /= M.uv_repeat.x / M.uv_repeat.y;
+    float d = duv_ray_cone(hit.NdotV, hit.cone_width, hit.area, hit.uv_area);
+    dUV.x = d;
+    if (set_both) dUV.y = d;
+    if (M.use & ST_ALBEDO_MAP) s.albedo = sample_texture(sc, M.tex0, s.uv, dUV);
+    if ((M.use & ST_NORMAL_MAP) && M.texture_count > 1) {
+        sq4 rot = normal_rotation(sample_normal(sc, M.tex1, s.uv, dUV));
+        hit.normal = q_rot(hit.normal, rot);
+    }
+}
+
+/* surface.h:146-255 */
+DNOINLINE sv3 shade_surface(DRay &ray, DTrace &tr, const StDScene &sc, bool &lights_shaded) {
+    DHit &hit = tr.closest_hit;
+    sv3 color = V3s(0);
+    DMat M = load_mat(sc, hit.material_id);
+    if (M.is & ST_EMISSIVE) return hit.from_behind ? color : M.emission;
+    sv2 dUV = V2(0, 0);
+    DShaded s;
+    s.viewing_origin = ray.origin;
+    s.viewing_direction = ray.direction;
+    s.prim_index = hit.object_id;
+    s.albedo = M.albedo;
+    s.reflected_direction = V3s(0);
+    s.light_direction = V3s(0);
+    if (M.use && M.texture_count) apply_textures(tr, sc, s, M, dUV, true);
+
+    float NdotRd = 0, ior, max_distance = hit.distance;
+    sv3 current_color, throughput = V3s(1);
+    uint32_t depth = tr.depth;
+    while (depth) {
+        s.position = ray.origin = hit.position;
+        s.normal = hit.from_behind ? v_neg(hit.normal) : hit.normal;
+        ior = hit.from_behind ? M.n2_over_n1 : M.n1_over_n2;
+        bool is_ref = (M.is & ST_REFLECTIVE) || (M.is & ST_REFRACTIVE);
+        if (is_ref || M.brdf == ST_BRDF_PHONG) {
+            NdotRd = -inv_dot(s.normal, s.viewing_direction);
+            s.reflected_direction = reflect_with_dot(s.viewing_direction, s.normal, NdotRd);
+        }
+        current_color = is_ref ? V3s(0) : V3(sc.ambient[0], sc.ambient[1], sc.ambient[2]);
+        if (sc.light_count) current_color = shade_lights_direct(s, M, ray, tr, sc, current_color);
+        if (sc.has_emissive_quads)
+            if (shade_emissive_quads(s, M, ray, tr, sc, current_color)) max_distance = hit.distance;
+        color = v_mad(current_color, throughput, color);
+        if (sc.light_count && shade_light_glow(sc, s.viewing_origin, s.viewing_direction, max_distance, tr.sphere_hit, color))
+            lights_shaded = true;
+        if (is_ref && --depth) {
+            if (M.is & ST_REFLECTIVE) {
+                ray.direction = s.reflected_direction;
+            } else {
+                ray.direction = refract_dir(s.viewing_direction, s.normal, ior, NdotRd);
+                if (ray.direction.x == 0 && ray.direction.y == 0 && ray.direction.z == 0)
+                    ray.direction = s.reflected_direction;
+            }
+            if (trace_ray(ray, tr, sc)) {
+                s.prim_index = hit.object_id;
+                M = load_mat(sc, hit.material_id);
+                s.albedo = M.albedo;
+                if (M.use && M.texture_count) apply_textures(tr, sc, s, M, dUV, false);
+                if (hit.object_type == ST_PRIM_QUAD && (M.is & ST_EMISSIVE)) {
+                    color = hit.from_behind ? V3s(0) : M.emission;
+                    break;
+                }
+                if (M.brdf != ST_BRDF_COOK_TORRANCE) throughput = v_mul(throughput, M.reflectivity);
+                s.viewing_origin = ray.origin;
+                s.viewing_direction = ray.direction;
+                continue;
+            }
+        }
+        break;
+    }
+    return color;
+}
+
+/* render/raytracer.h:81-134 for one pixel. Writes the pixel (cleared value
+   when nothing is hit) and, when asked, the primary hit record. */
+DFN void trace_pixel(const StDScene &sc, const StDView &v, DTrace &tr, sv3 dir, uint32_t x, uint32_t y, float4 &rgba,
+                     double &depth_out, StHitRec &hrec) {
+    DRay ray;
+    ray.origin = V3(v.cam_pos[0], v.cam_pos[1], v.cam_pos[2]);
+    ray.direction = dir;
+    ray.direction_reciprocal = v_rcp(dir);
+    tr.closest_hit.distance = tr.closest_hit.distance_squared = INFINITY;
+    tr.closest_hit.cone_angle = v.cone_angle;
+    tr.closest_hit.cone_width = 0;
+    DHit &hit = tr.closest_hit;
+    sv3 Ro = ray.origin, Rd = ray.direction;
+    sv3 color = V3s(0);
+    bool lights_shaded = false;
+    tr.primary_triangle = -1;
+    bool hit_found = hit_primitives(ray, tr, sc, sc.scene_leaf_ids, sc.prim_count, false, v.rects, x, y);
+    hrec.prim_id = hit_found ? (int32_t)hit.object_id : -1;
+    hrec.object_type = hit_found ? (int32_t)hit.object_type : 0;
+    hrec.triangle_id = hit_found ? tr.primary_triangle : -1;
+    hrec.distance = hit_found ? hit.distance : INFINITY;
+    float closest_distance = hit_found ? hit.distance : INFINITY;
+    float z = INFINITY;
+    if (hit_found) {
+        sq4 cr;
+        cr.v = V3(v.cam_rot_inv[0], v.cam_rot_inv[1], v.cam_rot_inv[2]);
+        cr.w = v.cam_rot_inv[3];
+        z = q_rot(v_sub(hit.position, Ro), cr).z;
+        if (v.mode == ST_MODE_BEAUTY) {
+            color = shade_surface(ray, tr, sc, lights_shaded);
+        } else if (v.mode == ST_MODE_DEPTH) {
+            color = V3s(4 / hit.distance);
+        } else if (v.mode == ST_MODE_NORMALS) {
+            DMat M = load_mat(sc, hit.material_id);
+            if (M.texture_count > 1 && (M.use & ST_NORMAL_MAP)) {
+                sv2 uv = V2(hit.uv.x * M.uv_repeat.x, hit.uv.y * M.uv_repeat.y);
+                hit.uv_area /= M.uv_repeat.x / M.uv_repeat.y;
+                sv2 dUV = V2(0, 0);
+                dUV.x = duv_ray_cone(hit.NdotV, hit.cone_width, hit.area, hit.uv_area);
+                sq4 rot = normal_rotation(sample_normal(sc, M.tex1, uv, dUV));
+                hit.normal = q_rot(hit.normal, rot);
+            }
+            color = V3(0.5f * (hit.normal.x + 1), 0.5f * (hit.normal.y + 1), 0.5f * (hit.normal.z + 1));
+        } else {
+            color = V3(0.5f * hit.uv.x, 0.5f * hit.uv.y, 0.5f);
+        }
+    }
+    if (v.mode == ST_MODE_BEAUTY && sc.light_count && !lights_shaded) {
+        if (shade_light_glow(sc, Ro, Rd, closest_distance, tr.sphere_hit, color)) {
+            hit_found = true;
+            z = tr.sphere_hit.closest_hit_distance;
+        }
+    }
+    if (hit_found) {
+        if (v.mode == ST_MODE_BEAUTY) {
+            color.x = tone_map(color.x);
+            color.y = tone_map(color.y);
+            color.z = tone_map(color.z);
+        }
+        color = v_scale(color, 255.0f);
+        color = v_mul(color, color);
+        rgba = make_float4(color.x, color.y, color.z, 1.0f);
+        depth_out = (double)z;
+    } else {
+        rgba = make_float4(0, 0, 0, 0);
+        depth_out = (double)INFINITY;
+    }
+}
+
+#endif

@@ -1,0 +1,99 @@
+/*
+ * st_host.h - host-side scene model of the HIP library (C).
+ *
+ * A StScene is a parsed scene blob whose BVHs are built and whose records
+ * are already in the device layout of st_layout.h, ready to be copied to HBM
+ * in one pass. Building follows the reference's init order (app.h:287-297):
+ * mesh BVHs (updateMeshBVH), then the scene BVH over the primitives
+ * (updateSceneBVH). Per-request state (camera basis, SSB rects) lives in a
+ * StHostView.
+ */
+#ifndef ST_HOST_H
+#define ST_HOST_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "st_blob.h"
+#include "../st_layout.h"
+#include "../st_math.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct StScene {
+    uint32_t prim_count, material_count, light_count, mesh_count, texture_count;
+    float ambient[3];
+    StDPrim *prims;
+    StDMat *mats;
+    StDLight *lights;
+    /* scene BVH (reference node layout), node 0 = root */
+    StBVHNodeRec *scene_nodes;
+    uint32_t scene_node_count, scene_height;
+    uint32_t *scene_leaf_ids;
+    /* meshes: nodes of all meshes concatenated; mesh m's node k is
+       mesh_nodes[meshes[m].node_offset + k]; offsets are odd so that child
+       pairs (odd, even) start on 64-byte boundaries. */
+    StDMesh *meshes;
+    sbox *mesh_aabbs;
+    StBVHNodeRec *mesh_nodes;
+    uint32_t mesh_node_total;
+    StDTriHot *tri_hot;
+    StDTriCold *tri_cold;
+    uint32_t tri_total;
+    uint32_t max_mesh_height;
+    /* textures */
+    StDTex *textures;
+    StDMip *mips;
+    uint32_t mip_total;
+    uint8_t *texels;
+    uint64_t texel_bytes;
+    uint32_t has_emissive_quads;
+    uint32_t mesh_stack_size;
+} StScene;
+
+/* Parse a blob and build its BVHs. Returns NULL on a malformed blob; *err
+   (if given) receives a short reason. */
+StScene *st_scene_from_blob(const void *blob, size_t size, const char **err);
+void st_scene_free(StScene *sc);
+
+/* Device bytes the resident scene occupies (for HBM budgeting). */
+uint64_t st_scene_device_bytes(const StScene *sc);
+
+/* Recompute the scene BVH over the current primitive transforms with the SAH
+   builder (updateSceneBVH, builder_top_down.h:366-381). */
+int st_scene_rebuild_sah(StScene *sc);
+
+/* World AABB of a primitive (getPrimitiveAABB + transformAABB). */
+sbox st_prim_world_box(const StScene *sc, uint32_t i);
+
+/* SAH BVH over leaf boxes (buildBVH, builder_top_down.h:204-270): returns the
+   node count; nodes needs 2n entries, leaf_ids n. */
+uint32_t st_sah_build(const sbox *boxes, uint32_t n, uint32_t max_leaf, StBVHNodeRec *nodes, uint32_t *leaf_ids,
+                      uint32_t *height);
+
+/* Mesh BVH + triangle records (updateMeshBVH, builder_top_down.h:272-364). */
+int st_build_mesh(uint32_t tri_count, const float *pos, const uint32_t *pidx, const float *nrm,
+                  const uint32_t *nidx, uint32_t normals_count, const float *uvs, const uint32_t *uidx,
+                  uint32_t uvs_count, StBVHNodeRec *nodes, uint32_t *node_count, uint32_t *height,
+                  StTriangleRec *tris, uint32_t *leaf_ids);
+
+/* Camera + frame set-up for one request. */
+typedef struct StHostView {
+    StDView dv;            /* camera, grid, mode; pointers filled by the runtime */
+    float start[3], right[3], down[3];
+    float rot[4], rot_inv[4];
+    sm3 rotation_matrix_inverted;
+    float *row_starts;     /* 4 floats per row */
+    int32_t *rects;        /* 4 ints per primitive, empty when not visible */
+    uint8_t *flags;        /* primitive flags after the SSB pass */
+} StHostView;
+
+int st_view_prepare(const StScene *sc, const StView *v, StHostView *hv);
+void st_view_release(StHostView *hv);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,223 @@
+/*
+ * st_view.c - per-request camera, frame and screen-space-bounds set-up.
+ *
+ * Restates, for one StView:
+ *   rotateXform3 from identity (scene/xform.h:8-20, math/mat3.h:137-183),
+ *   updateDimensions with QCAA (core/base.h:344-358),
+ *   setViewportProjectionPlane (core/init.h:545-559),
+ *   the row-start accumulation of renderSceneOnCPU (render/raytracer.h:156-170:
+ *     each row starts at the previous row's start + down, a sequential fp32 sum),
+ *   updatePrimitiveSSB / computeSSB (render/SSB.h:5-166) with its (u16) casts.
+ * The SSB result is folded into one rect per primitive: a primitive that is
+ * not IS_VISIBLE gets an empty rect, which the primary-ray gate rejects for
+ * every pixel exactly as the flag test does.
+ */
+#include <stdlib.h>
+#include <string.h>
+
+#include "st_host.h"
+
+typedef struct {
+    uint16_t width, height;
+    float f_width, f_height, h_width, h_height, width_over_height, height_over_width;
+} Dims;
+
+static void dims_init(Dims *d, uint16_t w, uint16_t h) {
+    d->width = (uint16_t)(w + 1);
+    d->height = (uint16_t)(h + 1);
+    d->f_width = (float)d->width;
+    d->f_height = (float)d->height;
+    d->h_width = d->f_width / 2;
+    d->h_height = d->f_height / 2;
+    d->width_over_height = d->f_width / d->f_height;
+    d->height_over_width = d->f_height / d->f_width;
+}
+
+static sm3 rotation_matrix(float yaw_amt, float pitch_amt, float roll_amt) {
+    sm3 yaw = m_identity(), pitch = m_identity(), roll = m_identity();
+    if (yaw_amt) {
+        sv2 xy = unit_circle(yaw_amt);
+        sm3 m = yaw;
+        yaw.X.x = xy.x * m.X.x - xy.y * m.X.z;
+        yaw.Y.x = xy.x * m.Y.x - xy.y * m.Y.z;
+        yaw.Z.x = xy.x * m.Z.x - xy.y * m.Z.z;
+        yaw.X.z = xy.x * m.X.z + xy.y * m.X.x;
+        yaw.Y.z = xy.x * m.Y.z + xy.y * m.Y.x;
+        yaw.Z.z = xy.x * m.Z.z + xy.y * m.Z.x;
+    }
+    if (pitch_amt) {
+        sv2 xy = unit_circle(pitch_amt);
+        sm3 m = pitch;
+        pitch.X.y = xy.x * m.X.y + xy.y * m.X.z;
+        pitch.Y.y = xy.x * m.Y.y + xy.y * m.Y.z;
+        pitch.Z.y = xy.x * m.Z.y + xy.y * m.Z.z;
+        pitch.X.z = xy.x * m.X.z - xy.y * m.X.y;
+        pitch.Y.z = xy.x * m.Y.z - xy.y * m.Y.y;
+        pitch.Z.z = xy.x * m.Z.z - xy.y * m.Z.y;
+    }
+    if (roll_amt) {
+        sv2 xy = unit_circle(roll_amt);
+        sm3 m = roll;
+        roll.X.x = xy.x * m.X.x + xy.y * m.X.y;
+        roll.Y.x = xy.x * m.Y.x + xy.y * m.Y.y;
+        roll.Z.x = xy.x * m.Z.x + xy.y * m.Z.y;
+        roll.X.y = xy.x * m.X.y - xy.y * m.X.x;
+        roll.Y.y = xy.x * m.Y.y - xy.y * m.Y.x;
+        roll.Z.y = xy.x * m.Z.y - xy.y * m.Z.x;
+    }
+    return m_mul(m_mul(pitch, yaw), roll);
+}
+
+/* computeSSB (render/SSB.h:5-135); b = min.x, min.y, max.x, max.y */
+static int compute_ssb(int32_t *b, sv3 pos, float r, float fl, const Dims *d) {
+    b[0] = d->width + 1;
+    b[2] = d->width + 1;
+    b[1] = d->height + 1;
+    b[3] = d->height + 1;
+    if (pos.z <= -r) return 0;
+    float x = pos.x, y = pos.y, z = pos.z;
+    float left, right, top, bottom, factor;
+    if (z <= r) {
+        float h = fabsf(z) / fl;
+        top = y + r;
+        bottom = y - r;
+        if (h < bottom || top < -h) return 0;
+        float w = h * d->width_over_height;
+        right = x + r;
+        left = x - r;
+        if (w < left || right < -w) return 0;
+        b[0] = -w < left ? (uint16_t)(int32_t)(d->f_width * (left + w) / (2 * w)) : 0;
+        b[2] = right < w ? (uint16_t)(int32_t)(d->f_width * (right + w) / (2 * w)) : d->width;
+        b[1] = top < h ? (uint16_t)(int32_t)(d->f_height * (h - top) / (2 * h)) : 0;
+        b[3] = -h < bottom ? (uint16_t)(int32_t)(d->f_height * (h - bottom) / (2 * h)) : d->height;
+        return 1;
+    }
+    float den = z * z - r * r;
+    factor = fl / den;
+    float yz = y * z;
+    float sqr = y * y + den;
+    sqr = r * sqrtf(sqr);
+    bottom = factor * (yz - sqr);
+    top = factor * (yz + sqr);
+    if (bottom < 1 && top > -1) {
+        factor *= d->height_over_width;
+        float xz = x * z;
+        sqr = r * sqrtf(x * x + den);
+        left = factor * (xz - sqr);
+        right = factor * (xz + sqr);
+        if (left < 1 && right > -1) {
+            bottom = bottom > -1 ? bottom : -1; bottom += 1;
+            top = top < 1 ? top : 1; top += 1;
+            left = left > -1 ? left : -1; left += 1;
+            right = right < 1 ? right : 1; right += 1;
+            top = 2 - top;
+            bottom = 2 - bottom;
+            b[0] = (uint16_t)(int32_t)(d->h_width * left);
+            b[2] = (uint16_t)(int32_t)(d->h_width * right);
+            b[3] = (uint16_t)(int32_t)(d->h_height * bottom);
+            b[1] = (uint16_t)(int32_t)(d->h_height * top);
+            return 1;
+        }
+    }
+    return 0;
+}
+
+int st_view_prepare(const StScene *sc, const StView *v, StHostView *hv) {
+    memset(hv, 0, sizeof *hv);
+    if (v->width == 0 || v->height == 0 || v->width >= 0xFFFF || v->height >= 0xFFFF) return -1;
+    if (v->render_mode > ST_MODE_UVS) return -1;
+    sm3 rm = rotation_matrix(v->yaw, v->pitch, v->roll);
+    sm3 rmi = m_transpose(rm);
+    sq4 rot = q_from_matrix(rm), rot_inv = q_from_matrix(rmi);
+    sv3 right_dir = rm.X, up = rm.Y, forward = rm.Z;
+    Dims d;
+    dims_init(&d, v->width, v->height);
+
+    /* projection plane with QCAA offset 0 */
+    const float offset = 0.0f;
+    sv3 start = v_scale(forward, d.f_height * v->focal_length);
+    sv3 right = v_scale(right_dir, offset - (float)d.width);
+    sv3 down = v_scale(up, d.f_height - offset);
+    start = v_add(start, right);
+    start = v_add(start, down);
+    right = v_scale(right_dir, 2);
+    down = v_scale(up, -2);
+
+    hv->row_starts = (float *)malloc(4 * sizeof(float) * d.height);
+    hv->rects = (int32_t *)malloc(4 * sizeof(int32_t) * (sc->prim_count + 1));
+    hv->flags = (uint8_t *)malloc(sc->prim_count + 1);
+    if (!hv->row_starts || !hv->rects || !hv->flags) { st_view_release(hv); return -2; }
+    sv3 s = start;
+    for (uint32_t y = 0; y < d.height; y++) {
+        hv->row_starts[4 * y] = s.x;
+        hv->row_starts[4 * y + 1] = s.y;
+        hv->row_starts[4 * y + 2] = s.z;
+        hv->row_starts[4 * y + 3] = 0;
+        s = v_add(s, down);
+    }
+
+    sv3 cam = V3(v->position[0], v->position[1], v->position[2]);
+    for (uint32_t i = 0; i < sc->prim_count; i++) {
+        const StDPrim *p = sc->prims + i;
+        uint8_t flags = (uint8_t)(p->flags_mat & 0xFFu);
+        int32_t *rc = hv->rects + 4 * i;
+        sv3 pos = V3(p->pos[0], p->pos[1], p->pos[2]);
+        sv3 scale = V3(p->scale[0], p->scale[1], p->scale[2]);
+        sv3 vsp = v_mat(v_sub(pos, cam), rmi);
+        float radius;
+        int known = 1;
+        switch (p->type) {
+            case ST_PRIM_QUAD:
+            case ST_PRIM_BOX: radius = v_len(scale); break;
+            case ST_PRIM_TETRAHEDRON: radius = v_maxc(scale) * 1.41421356237f; break;
+            case ST_PRIM_SPHERE: radius = v_maxc(scale); break;
+            case ST_PRIM_MESH: {
+                sbox bb = sc->mesh_aabbs[p->mesh_id];
+                float min_r = v_len(v_mul(scale, bb.min));
+                float max_r = v_len(v_mul(scale, bb.max));
+                radius = max_r > min_r ? max_r : min_r;
+            } break;
+            default: known = 0; radius = 0; break;
+        }
+        if (known) {
+            flags &= (uint8_t)~ST_IS_VISIBLE;
+            if (compute_ssb(rc, vsp, radius, v->focal_length, &d)) flags |= ST_IS_VISIBLE;
+        } else {
+            /* updatePrimitiveSSB returns early: flags and (zero) bounds stay */
+            rc[0] = rc[1] = rc[2] = rc[3] = 0;
+        }
+        hv->flags[i] = flags;
+        if (!(flags & ST_IS_VISIBLE)) { rc[0] = rc[1] = 1; rc[2] = rc[3] = 0; }
+    }
+
+    StDView *dv = &hv->dv;
+    memcpy(dv->cam_pos, v->position, 12);
+    dv->cone_angle = v->cone_angle;
+    dv->cam_rot_inv[0] = rot_inv.v.x;
+    dv->cam_rot_inv[1] = rot_inv.v.y;
+    dv->cam_rot_inv[2] = rot_inv.v.z;
+    dv->cam_rot_inv[3] = rot_inv.w;
+    dv->right[0] = right.x;
+    dv->right[1] = right.y;
+    dv->right[2] = right.z;
+    dv->width1 = d.width;
+    dv->height1 = d.height;
+    dv->mode = v->render_mode;
+    dv->depth = (uint8_t)v->depth;   /* Trace.depth is a u8 (core/types.h:199) */
+    hv->start[0] = start.x; hv->start[1] = start.y; hv->start[2] = start.z;
+    hv->right[0] = right.x; hv->right[1] = right.y; hv->right[2] = right.z;
+    hv->down[0] = down.x; hv->down[1] = down.y; hv->down[2] = down.z;
+    hv->rot[0] = rot.v.x; hv->rot[1] = rot.v.y; hv->rot[2] = rot.v.z; hv->rot[3] = rot.w;
+    memcpy(hv->rot_inv, dv->cam_rot_inv, 16);
+    hv->rotation_matrix_inverted = rmi;
+    return 0;
+}
+
+void st_view_release(StHostView *hv) {
+    free(hv->row_starts);
+    free(hv->rects);
+    free(hv->flags);
+    hv->row_starts = NULL;
+    hv->rects = NULL;
+    hv->flags = NULL;
+}
