@@ -1,0 +1,218 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X tracing path (one JSON line on stdout).
+
+Workload (one step = one frame): the single-mesh PBR frame - a seeded
+noise-displaced geodesic icosphere with 200,000 fp32 triangles (frequency
+100), Cook-Torrance materials, a glass sphere, a mirror quad, 4 point lights
+and 2 emissive quads; 1920x1080 window (1921x1081 traced QCAA grid), 1 spp,
+trace depth 4, Beauty mode. The scene is resident in HBM before timing;
+a step is the per-frame host set-up (camera basis, SSB rects, ~20 KB of
+uploads), the primary-direction kernel and the render kernel; the frame
+stays in HBM (the PCIe-inclusive rate is reported separately).
+
+Also measured in the same run:
+  - plain vs batched launches: 64 mixed requests over 4 resident scenes,
+    first as 64 sequential per-request launches with a blocking download
+    each (the reference's renderSceneOnGPU pattern), then as one persistent
+    launch over a global tile queue plus the downloads;
+  - roofline of the render kernel: algorithmic bytes per ray from the CPU
+    oracle's event counters on a 480x270 view of the same scene, times the
+    device ray count of one launch, over the kernel's HIP-event duration;
+  - the reference CPU renderer (compiled from the reference's own sources)
+    on that 480x270 view, all threads used stated;
+  - hit-id parity of the GPU 480x270 frame against the CPU oracle.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import slimtracin_amd as st  # noqa: E402
+from slimtracin_amd.scene import MODE_BEAUTY, MODE_DEPTH, MODE_NORMALS, MODE_UVS, VIEW_DTYPE, make_view  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+PBR_K = 100            # 20 * 100^2 = 200,000 triangles
+PBR_SEED = 2024
+PBR_CAM = dict(position=(0.0, 6.0, -14.0), pitch=-0.15)
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def pbr_view(w, h):
+    return make_view(width=w, height=h, mode=MODE_BEAUTY, depth=4, **PBR_CAM)
+
+
+def batched_workload(rng, nscenes):
+    """Mixed requests: log-uniform resolution 128x128..1920x1080, depth 1-6,
+    mode uniform over Beauty/Depth/Normal/UV, camera on a sphere."""
+    views = []
+    scene_ids = []
+    for i in range(64):
+        w = int(round(np.exp(rng.uniform(np.log(128), np.log(1920)))))
+        h = int(round(np.exp(rng.uniform(np.log(128), np.log(1080)))))
+        mode = int(rng.choice([MODE_BEAUTY, MODE_DEPTH, MODE_NORMALS, MODE_UVS]))
+        depth = int(rng.integers(1, 7))
+        yaw = float(rng.uniform(-1, 1))
+        dist = 14.0
+        # camera on a circle around the scene centre, looking at it: position
+        # from the yaw angle, the rational yaw amount tan(angle/2)
+        ang = 2 * np.arctan(yaw)
+        pos = (float(dist * np.sin(ang)), 6.0, float(-dist * np.cos(ang)))
+        views.append(make_view(width=w, height=h, mode=mode, depth=depth, position=pos, yaw=yaw, pitch=-0.15))
+        scene_ids.append(int(rng.integers(0, nscenes)))
+    return np.concatenate(views), scene_ids
+
+
+def cpu_baseline(blob, view, threads, frames=1):
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import checkers as ck
+    pixels, hits, stats = ck.oracle_render(blob, view, threads=threads)
+    rays = stats.primary_rays + stats.shadow_rays + stats.bounce_rays
+    kind = "reference" if ck.ref_available() else "port"
+    if kind == "reference":
+        _, secs = ck.ref_render_timed(blob, view, threads, frames)
+    else:
+        _, secs, _ = ck.oracle_render_timed(blob, view, threads, frames)
+    t = float(np.median(secs))
+    return dict(value=rays / t / 1e6, unit="Mrays/s", cores=threads, kind=kind,
+                sample=f"{int(view['width'][0])}x{int(view['height'][0])} Beauty view of the same scene, "
+                       f"{threads} threads, median of {frames} frame(s), {rays} rays, {t:.2f} s/frame"), stats, hits
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and oracle legs")
+    ap.add_argument("--no-batch", action="store_true", help="skip the plain-vs-batched leg")
+    ap.add_argument("--cpu-frames", type=int, default=3, help="frames of the CPU baseline (median)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: every box in this setup has a single MI355X, so --gpus {args.gpus} "
+              "is not supported (the multi-GPU design routes whole requests to the GPU that holds their scene; "
+              "no per-frame collective exists to measure).", file=sys.stderr)
+        sys.exit(2)
+
+    st.init(0)
+    t0 = time.time()
+    blob = st.gen_pbr(PBR_SEED, PBR_K)
+    scene = st.DeviceScene(blob)
+    info = scene.info()
+    log(f"scene: {info.triangle_count} triangles, mesh BVH height {info.max_mesh_height}, "
+        f"{info.device_bytes / 2**20:.1f} MiB in HBM, built in {time.time() - t0:.1f} s")
+    view = pbr_view(1920, 1080)
+
+    for _ in range(args.warmup):
+        st.render_timed([scene], view)
+    kms, frame_ms, rays = [], [], 0
+    t_start = time.perf_counter()
+    for _ in range(args.steps):
+        f0 = time.perf_counter()
+        ms, counts = st.render_timed([scene], view)
+        frame_ms.append((time.perf_counter() - f0) * 1e3)
+        kms.append(ms)
+        rays += counts[0].total()
+    elapsed = time.perf_counter() - t_start
+    rays_per_frame = rays / args.steps
+    mrays = rays / elapsed / 1e6
+    kernel_ms = float(np.mean(kms))
+    log(f"frame: {rays_per_frame / 1e6:.2f} Mrays, render kernel {kernel_ms:.3f} ms, "
+        f"frame {np.median(frame_ms):.3f} ms, {mrays:.1f} Mrays/s")
+
+    # PCIe-inclusive: a plain frame with the blocking download (renderSceneOnGPU)
+    t1 = time.perf_counter()
+    for _ in range(3):
+        st.render(scene, view, hits=False)
+    pcie_frame_ms = (time.perf_counter() - t1) / 3 * 1e3
+
+    result = {
+        "metric": "Mrays/s + p50/p99 frame latency, batched/served vs plain launches, hit-IDs exact",
+        "value": round(mrays, 2),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(elapsed / args.steps * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "pbr_icosphere_200k_tris_1920x1080_depth4_beauty", "triangles": info.triangle_count,
+                   "traced_pixels": 1921 * 1081, "rays_per_frame": int(rays_per_frame), "spp": 1},
+        "p50_ms": round(float(np.percentile(frame_ms, 50)), 4),
+        "p99_ms": round(float(np.percentile(frame_ms, 99)), 4),
+        "frame_ms_with_download": round(pcie_frame_ms, 4),
+        "render_kernel_ms": round(kernel_ms, 4),
+    }
+
+    if not args.no_batch:
+        rng = np.random.default_rng(7)
+        sblobs = [st.gen_mesh_scene(100 + i, tri) for i, tri in enumerate([200_000, 300_000, 500_000, 800_000])]
+        dscenes = [st.DeviceScene(b) for b in sblobs]
+        views, sids = batched_workload(rng, len(dscenes))
+        req_scenes = [dscenes[i] for i in sids]
+        # warm both paths once
+        st.render_batch(req_scenes, views, persistent=True)
+        lat_plain, plain_rays = [], 0
+        tp = time.perf_counter()
+        for i in range(len(views)):
+            r0 = time.perf_counter()
+            _, _, c = st.render(req_scenes[i], views[i:i + 1], hits=False)
+            lat_plain.append((time.perf_counter() - r0) * 1e3)
+            plain_rays += c.total()
+        plain_s = time.perf_counter() - tp
+        tb = time.perf_counter()
+        _, _, cb = st.render_batch(req_scenes, views, persistent=True)
+        batch_s = time.perf_counter() - tb
+        batch_rays = sum(c.total() for c in cb)
+        assert batch_rays == plain_rays, (batch_rays, plain_rays)
+        result["batched"] = {
+            "requests": len(views), "scenes": len(dscenes),
+            "scene_triangles": [d.info().triangle_count for d in dscenes],
+            "rays": int(batch_rays),
+            "plain_mrays": round(plain_rays / plain_s / 1e6, 2),
+            "batched_mrays": round(batch_rays / batch_s / 1e6, 2),
+            "speedup": round(plain_s / batch_s, 3),
+            "plain_latency_p50_ms": round(float(np.percentile(lat_plain, 50)), 3),
+            "plain_latency_p99_ms": round(float(np.percentile(lat_plain, 99)), 3),
+            "batched_wall_ms": round(batch_s * 1e3, 3),
+        }
+        log("batched:", result["batched"])
+        for d in dscenes:
+            d.close()
+
+    if not args.no_cpu:
+        threads = min(os.cpu_count() or 1, 16)
+        sample = pbr_view(480, 270)
+        cpu, stats, ohits = cpu_baseline(blob, sample, threads, frames=args.cpu_frames)
+        result["cpu_baseline"] = cpu
+        log("cpu baseline:", cpu)
+        sample_rays = stats.primary_rays + stats.shadow_rays + stats.bounce_rays
+        bytes_per_ray = stats.algorithmic_bytes() / sample_rays
+        achieved = bytes_per_ray * rays_per_frame / (kernel_ms * 1e-3) / 1e9
+        result["roofline"] = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                              "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                              "bytes_per_ray": round(bytes_per_ray, 2)}
+        gp, gh, gc = st.render(scene, sample)
+        same = bool((gh["prim_id"] == ohits["prim_id"]).all() and (gh["triangle_id"] == ohits["triangle_id"]).all()
+                    and (gh["distance"].view(np.uint32) == ohits["distance"].view(np.uint32)).all())
+        result["hit_ids_exact"] = same
+        result["gpu_vs_cpu"] = round(mrays / cpu["value"], 1)
+    scene.close()
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -5,9 +5,10 @@
  * Kernels
  *   k_primary_dirs  one lane per traced row: walks the row with the
  *                   reference's sequential `current += right` fp32 sum
- *                   (render/raytracer.h:156-170) and stores the normalised
- *                   primary direction of every pixel. Rows of every request
- *                   of a launch are flattened into one grid.
+ *                   (render/raytracer.h:156-170) and stores every pixel's
+ *                   unnormalised `current` vector (the render kernel
+ *                   normalises it). Rows of every request of a launch are
+ *                   flattened into one grid.
  *   k_render        one wavefront per 8x8 pixel tile, the whole per-pixel
  *                   path of st_trace.h per lane. Tiles of all requests of a
  *                   launch form one list; in persistent mode each wavefront
@@ -34,6 +35,12 @@
 #define TILE_H 8
 #define WAVES_PER_BLOCK 4
 #define BLOCK_THREADS (64 * WAVES_PER_BLOCK)
+/* Two waves per SIMD: measured fastest on gfx950 (the fully inlined path
+   needs ~250 VGPRs; the compiler keeps 256 and spills the rest rather than
+   dropping to one wave per SIMD). */
+#ifndef K_RENDER_ATTR
+#define K_RENDER_ATTR __attribute__((amdgpu_waves_per_eu(2)))
+#endif
 
 /* ---- errors ------------------------------------------------------------ */
 
@@ -88,8 +95,7 @@ __global__ void __launch_bounds__(256) k_primary_dirs(const KReq *reqs, uint32_t
     const sv3 right = V3(v.right[0], v.right[1], v.right[2]);
     float4 *out = reinterpret_cast<float4 *>(v.dirs) + (size_t)y * v.width1;
     for (uint32_t x = 0; x < v.width1; x++) {
-        sv3 d = v_norm(current);
-        out[x] = make_float4(d.x, d.y, d.z, 0.0f);
+        out[x] = make_float4(current.x, current.y, current.z, 0.0f);
         current = v_add(current, right);
     }
 }
@@ -121,7 +127,8 @@ __device__ static void render_tile(const StDScene *scenes, const KReq *reqs, uin
         float4 rgba;
         double depth;
         StHitRec hrec;
-        trace_pixel(sc, v, tr, V3(d4.x, d4.y, d4.z), x, y, rgba, depth, hrec);
+        /* dir = normVec3(current) (render/raytracer.h:159) */
+        trace_pixel(sc, v, tr, v_norm(V3(d4.x, d4.y, d4.z)), x, y, rgba, depth, hrec);
         /* FloatPixel is 24 bytes: store as three 8-byte words */
         uint2 *px = reinterpret_cast<uint2 *>(static_cast<char *>(v.pixels) + 24 * pix);
         px[0] = make_uint2(__float_as_uint(rgba.x), __float_as_uint(rgba.y));
@@ -146,12 +153,17 @@ __device__ static void render_tile(const StDScene *scenes, const KReq *reqs, uin
     }
 }
 
-__global__ void __launch_bounds__(BLOCK_THREADS) k_render(const StDScene *scenes, const KReq *reqs, uint32_t nreq,
+__global__ void __launch_bounds__(BLOCK_THREADS) K_RENDER_ATTR k_render(const StDScene *scenes, const KReq *reqs, uint32_t nreq,
                                                            uint32_t total_tiles, uint32_t *tile_counter,
-                                                           uint32_t persistent, uint32_t xcd_blocks) {
-    uint32_t scene_stack[ST_SCENE_STACK_MAX];
-    uint32_t mesh_stack[ST_MESH_STACK_MAX];
+                                                           uint32_t persistent, uint32_t xcd_blocks,
+                                                           uint32_t scene_entries, uint32_t mesh_entries) {
+    /* traversal stacks live in LDS, one column per lane (no bank conflicts):
+       [wave][entry][lane] */
+    extern __shared__ uint32_t st_lds[];
     const uint32_t wave = threadIdx.x >> 6;
+    uint32_t *wave_base = st_lds + (size_t)wave * 64u * (scene_entries + mesh_entries) + (threadIdx.x & 63u);
+    uint32_t *scene_stack = wave_base;
+    uint32_t *mesh_stack = wave_base + 64u * scene_entries;
     if (persistent) {
         for (;;) {
             uint32_t t = 0;
@@ -337,6 +349,7 @@ extern "C" int st_scene_upload(StSceneHandle *h) {
     memcpy(d.ambient, sc->ambient, 12);
     d.mesh_stack_size = sc->mesh_stack_size;
     d.has_emissive_quads = sc->has_emissive_quads;
+    d.scene_stack_need = sc->scene_real_height + 1;
     return ST_OK;
 }
 
@@ -482,8 +495,16 @@ static int launch(StSceneHandle *const *scenes, const StView *views, uint32_t n,
             blocks = (blocks + 7u) & ~7u;
             xcd_blocks = blocks;
         }
-        hipLaunchKernelGGL(k_render, dim3(blocks), dim3(BLOCK_THREADS), 0, g.stream, (const StDScene *)sc_d,
-                           (const KReq *)reqs_d, n, tiles, g.tile_counter, (uint32_t)persistent, xcd_blocks);
+        uint32_t s_entries = 1, m_entries = 1;
+        for (uint32_t k = 0; k < nuniq; k++) {
+            if (sc_h[k].scene_stack_need > s_entries) s_entries = sc_h[k].scene_stack_need;
+            if (sc_h[k].mesh_stack_size > m_entries) m_entries = sc_h[k].mesh_stack_size;
+        }
+        size_t lds = (size_t)BLOCK_THREADS * (s_entries + m_entries) * sizeof(uint32_t);
+        if (lds > 160 * 1024) { rc = set_err(ST_ERR_SCENE, "traversal stacks need %zu B of LDS", lds); goto done; }
+        hipLaunchKernelGGL(k_render, dim3(blocks), dim3(BLOCK_THREADS), lds, g.stream, (const StDScene *)sc_d,
+                           (const KReq *)reqs_d, n, tiles, g.tile_counter, (uint32_t)persistent, xcd_blocks,
+                           s_entries, m_entries);
         HIP_TRY(hipGetLastError());
         HIP_TRY(hipEventRecord(g.ev[1], g.stream));
         for (uint32_t i = 0; i < n; i++) {
@@ -516,7 +537,7 @@ done:
 extern "C" int st_render(StSceneHandle *scene, const StView *view, StFloatPixel *pixels, StHitRec *hits,
                          StRayCounts *counts) {
     if (!scene || !view) return set_err(ST_ERR_ARG, "null argument");
-    return launch(&scene, view, 1, hits != NULL, 0, counts, &pixels, &hits, NULL);
+    return launch(&scene, view, 1, hits != NULL, 1, counts, &pixels, &hits, NULL);
 }
 
 extern "C" int st_render_batch(StSceneHandle *const *scenes, const StView *views, uint32_t n,

@@ -39,8 +39,12 @@
 #define D_TET_A 0.577350259f
 #define D_TET_B 0.288675159f
 
+/* Everything is inlined into the render kernel so the per-ray state stays in
+   registers (a call boundary would force the shared trace state to scratch). */
 #define DFN __device__ static inline __attribute__((always_inline))
-#define DNOINLINE __device__ static __attribute__((noinline))
+#ifndef DNOINLINE
+#define DNOINLINE DFN
+#endif
 
 struct DHit {
     sv3 position, normal;
@@ -70,8 +74,10 @@ struct DTrace {
     uint32_t depth, mesh_stack_size;
     int32_t primary_triangle;
     uint32_t n_shadow, n_bounce;
-    uint32_t *scene_stack, *mesh_stack;
+    uint32_t *scene_stack, *mesh_stack;   /* lane columns of LDS: entry i at [i * ST_STACK_STRIDE] */
 };
+
+#define ST_STACK_STRIDE 64u
 
 struct DShaded {
     sv3 albedo, position, normal, viewing_direction, viewing_origin, reflected_direction, light_direction;
@@ -532,7 +538,7 @@ DNOINLINE bool trace_mesh(DTrace &tr, const StDScene &sc, uint32_t mesh_id, bool
         if (has_left) {
             if (has_right) {
                 if (!any_hit && ld > rd) { uint32_t t = lnext; lnext = rnext; rnext = t; }
-                stack[stack_size++] = rnext;
+                stack[ST_STACK_STRIDE * stack_size++] = rnext;
                 if (stack_size == tr.mesh_stack_size) return false;
             }
             left = lnext;
@@ -540,7 +546,7 @@ DNOINLINE bool trace_mesh(DTrace &tr, const StDScene &sc, uint32_t mesh_id, bool
             left = rnext;
         } else {
             if (stack_size == 0) break;
-            left = stack[--stack_size];
+            left = stack[ST_STACK_STRIDE * --stack_size];
         }
     }
     return found;
@@ -674,14 +680,14 @@ DNOINLINE bool trace_scene(DRay &ray, DTrace &tr, const StDScene &sc, bool any_h
         if (has_left) {
             if (has_right) {
                 if (!any_hit && ld > rd) { uint32_t t = lnext; lnext = rnext; rnext = t; }
-                stack[stack_size++] = rnext;
+                stack[ST_STACK_STRIDE * stack_size++] = rnext;
             }
             left = lnext;
         } else if (has_right) {
             left = rnext;
         } else {
             if (stack_size == 0) break;
-            left = stack[--stack_size];
+            left = stack[ST_STACK_STRIDE * --stack_size];
         }
     }
     return found;

@@ -51,6 +51,7 @@ typedef struct StScene {
     uint64_t texel_bytes;
     uint32_t has_emissive_quads;
     uint32_t mesh_stack_size;
+    uint32_t scene_real_height;    /* levels of the scene BVH as walked */
 } StScene;
 
 /* Parse a blob and build its BVHs. Returns NULL on a malformed blob; *err

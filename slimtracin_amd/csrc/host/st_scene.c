@@ -377,6 +377,7 @@ StScene *st_scene_from_blob(const void *blob, size_t size, const char **err) {
     if (P) {
         int real_height = bvh_validate(sc->scene_nodes, sc->scene_node_count, P);
         if (real_height < 0) { *err = "malformed scene BVH"; st_scene_free(sc); return NULL; }
+        sc->scene_real_height = (uint32_t)real_height;
         if (real_height + 1 > ST_SCENE_STACK_MAX) {
             *err = "scene BVH deeper than the device traversal stack"; st_scene_free(sc); return NULL;
         }

@@ -94,7 +94,8 @@ typedef struct StDScene {
     float ambient[3];
     uint32_t mesh_stack_size;      /* (u8)(max mesh BVH height + 2), core/init.h:523-527 */
     uint32_t has_emissive_quads;
-    uint32_t pad[3];
+    uint32_t scene_stack_need;     /* entries the scene-BVH traversal stack can reach */
+    uint32_t pad[2];
 } StDScene;
 
 /* One render request on the device. */
@@ -109,7 +110,7 @@ typedef struct StDView {
     uint32_t tiles_x;              /* tiles per row */
     uint32_t pad;
     const float *row_starts;       /* 4 floats per row: sequential start vectors */
-    float *dirs;                   /* 4 floats per pixel: normalised primary directions */
+    float *dirs;                   /* 4 floats per pixel: the sequential `current` vectors */
     const int32_t *rects;          /* 4 ints per primitive: SSB min.x, min.y, max.x, max.y */
     void *pixels;                  /* StFloatPixel per pixel */
     void *hits;                    /* StHitRec per pixel, or NULL */
