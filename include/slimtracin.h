@@ -81,6 +81,30 @@ int st_render_batch(StSceneHandle *const *scenes, const StView *views, uint32_t 
 int st_render_timed(StSceneHandle *const *scenes, const StView *views, uint32_t n, uint32_t flags, float *kernel_ms,
                     StRayCounts *counts);
 
+/* Host-side introspection (no device needed): what the library computes on
+   the host for a blob and a view. */
+/* SSB rects (min.x, min.y, max.x, max.y, as computeSSB leaves screen_bounds)
+   and flags after updateSceneSSB (render/SSB.h:137-173); camera as 17 floats:
+   projection plane start/right/down (core/init.h:545-559), rotation_inverted,
+   rotation quaternions (scene/xform.h:8-20). */
+int st_host_view_info(const void *blob, size_t size, const StView *view, int32_t *rects, uint8_t *flags,
+                      float *camera);
+/* Scene BVH built for a blob (updateSceneBVH, builder_top_down.h:366-381). */
+int st_host_scene_bvh(const void *blob, size_t size, StBVHNodeRec *nodes, uint32_t *leaf_ids, uint32_t *node_count,
+                      uint32_t *height);
+/* Parse and validate a blob; `why` receives the reason of a rejection. */
+int st_host_validate(const void *blob, size_t size, char *why, size_t cap);
+/* Reference SAH build over n leaf boxes (6 floats: min xyz, max xyz each;
+   buildBVH, builder_top_down.h:204-270). nodes: 2n entries. Returns the node
+   count. */
+uint32_t st_sah_build_boxes(const float *boxes, uint32_t n, uint32_t max_leaf, StBVHNodeRec *nodes, uint32_t *leaf_ids,
+                            uint32_t *height);
+/* Mesh BVH and triangle records (updateMeshBVH, builder_top_down.h:272-364). */
+int st_build_mesh(uint32_t tri_count, const float *pos, const uint32_t *pidx, const float *nrm,
+                  const uint32_t *nidx, uint32_t normals_count, const float *uvs, const uint32_t *uidx,
+                  uint32_t uvs_count, StBVHNodeRec *nodes, uint32_t *node_count, uint32_t *height,
+                  StTriangleRec *tris, uint32_t *leaf_ids);
+
 #ifdef __cplusplus
 }
 #endif

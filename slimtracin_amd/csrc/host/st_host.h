@@ -89,6 +89,7 @@ typedef struct StHostView {
     float *row_starts;     /* 4 floats per row */
     int32_t *rects;        /* 4 ints per primitive, empty when not visible */
     uint8_t *flags;        /* primitive flags after the SSB pass */
+    int32_t *raw_rects;    /* screen_bounds as computeSSB left them */
 } StHostView;
 
 int st_view_prepare(const StScene *sc, const StView *v, StHostView *hv);

@@ -28,6 +28,7 @@
 #include <string.h>
 
 #include "st_host.h"
+#include "slimtracin.h"
 
 typedef struct {
     const sbox *boxes;
@@ -311,4 +312,17 @@ int st_build_mesh(uint32_t T, const float *pos, const uint32_t *pidx, const floa
     free(boxes);
     free(order);
     return 0;
+}
+
+uint32_t st_sah_build_boxes(const float *boxes, uint32_t n, uint32_t max_leaf, StBVHNodeRec *nodes, uint32_t *leaf_ids,
+                            uint32_t *height) {
+    sbox *b = (sbox *)malloc(((size_t)n + 1) * sizeof(sbox));
+    if (!b) return 0;
+    for (uint32_t i = 0; i < n; i++) {
+        b[i].min = V3(boxes[6 * (size_t)i], boxes[6 * (size_t)i + 1], boxes[6 * (size_t)i + 2]);
+        b[i].max = V3(boxes[6 * (size_t)i + 3], boxes[6 * (size_t)i + 4], boxes[6 * (size_t)i + 5]);
+    }
+    uint32_t c = st_sah_build(b, n, max_leaf, nodes, leaf_ids, height);
+    free(b);
+    return c;
 }

@@ -16,6 +16,7 @@
 #include <string.h>
 
 #include "st_host.h"
+#include "slimtracin.h"
 
 typedef struct {
     uint16_t width, height;
@@ -146,7 +147,8 @@ int st_view_prepare(const StScene *sc, const StView *v, StHostView *hv) {
     hv->row_starts = (float *)malloc(4 * sizeof(float) * d.height);
     hv->rects = (int32_t *)malloc(4 * sizeof(int32_t) * (sc->prim_count + 1));
     hv->flags = (uint8_t *)malloc(sc->prim_count + 1);
-    if (!hv->row_starts || !hv->rects || !hv->flags) { st_view_release(hv); return -2; }
+    hv->raw_rects = (int32_t *)malloc(4 * sizeof(int32_t) * (sc->prim_count + 1));
+    if (!hv->row_starts || !hv->rects || !hv->flags || !hv->raw_rects) { st_view_release(hv); return -2; }
     sv3 s = start;
     for (uint32_t y = 0; y < d.height; y++) {
         hv->row_starts[4 * y] = s.x;
@@ -187,6 +189,7 @@ int st_view_prepare(const StScene *sc, const StView *v, StHostView *hv) {
             rc[0] = rc[1] = rc[2] = rc[3] = 0;
         }
         hv->flags[i] = flags;
+        memcpy(hv->raw_rects + 4 * i, rc, 16);
         if (!(flags & ST_IS_VISIBLE)) { rc[0] = rc[1] = 1; rc[2] = rc[3] = 0; }
     }
 
@@ -217,7 +220,62 @@ void st_view_release(StHostView *hv) {
     free(hv->row_starts);
     free(hv->rects);
     free(hv->flags);
+    free(hv->raw_rects);
+    hv->raw_rects = NULL;
     hv->row_starts = NULL;
     hv->rects = NULL;
     hv->flags = NULL;
+}
+
+/* Host-only introspection of the set-up a request gets (no device needed):
+   SSB rects and flags after updateSceneSSB, and the camera as 17 floats
+   (projection-plane start, right, down; rotation_inverted; rotation). */
+int st_host_view_info(const void *blob, size_t size, const StView *v, int32_t *rects, uint8_t *flags, float *camera) {
+    const char *why = NULL;
+    StScene *sc = st_scene_from_blob(blob, size, &why);
+    if (!sc) return -2;
+    StHostView hv;
+    int rc = st_view_prepare(sc, v, &hv);
+    if (rc) { st_scene_free(sc); return -1; }
+    for (uint32_t i = 0; i < sc->prim_count; i++) {
+        if (rects) memcpy(rects + 4 * i, hv.raw_rects + 4 * i, 16);
+        if (flags) flags[i] = hv.flags[i];
+    }
+    if (camera) {
+        float c[17] = {hv.start[0], hv.start[1], hv.start[2], hv.right[0], hv.right[1], hv.right[2],
+                       hv.down[0], hv.down[1], hv.down[2], hv.rot_inv[0], hv.rot_inv[1], hv.rot_inv[2], hv.rot_inv[3],
+                       hv.rot[0], hv.rot[1], hv.rot[2], hv.rot[3]};
+        memcpy(camera, c, sizeof c);
+    }
+    st_view_release(&hv);
+    st_scene_free(sc);
+    return 0;
+}
+
+/* Host-only: the scene BVH the library builds for a blob. */
+int st_host_scene_bvh(const void *blob, size_t size, StBVHNodeRec *nodes, uint32_t *leaf_ids, uint32_t *node_count,
+                      uint32_t *height) {
+    const char *why = NULL;
+    StScene *sc = st_scene_from_blob(blob, size, &why);
+    if (!sc) return -2;
+    memcpy(nodes, sc->scene_nodes, sizeof(StBVHNodeRec) * sc->scene_node_count);
+    memcpy(leaf_ids, sc->scene_leaf_ids, 4u * sc->prim_count);
+    *node_count = sc->scene_node_count;
+    *height = sc->scene_height;
+    st_scene_free(sc);
+    return 0;
+}
+
+/* Host-only: parse + validate a blob; returns 0 or a negative status and
+   copies the reason into `why` (up to `cap` bytes). */
+int st_host_validate(const void *blob, size_t size, char *why, size_t cap) {
+    const char *reason = NULL;
+    StScene *sc = st_scene_from_blob(blob, size, &reason);
+    if (!sc) {
+        if (why && cap) { strncpy(why, reason ? reason : "invalid", cap - 1); why[cap - 1] = 0; }
+        return -2;
+    }
+    st_scene_free(sc);
+    if (why && cap) why[0] = 0;
+    return 0;
 }
