@@ -67,6 +67,19 @@ int st_scene_info(const StSceneHandle *scene, StSceneInfo *info);
 int st_scene_bvh(const StSceneHandle *scene, StBVHNodeRec *nodes, uint32_t *leaf_ids, uint32_t *node_count,
                  uint32_t *height);
 
+#define ST_REBUILD_SAH 0u   /* the reference's SAH builder, on the host */
+#define ST_REBUILD_LBVH 1u  /* Morton LBVH on the device */
+
+/* Move primitives: replace records [first, first+count) (transforms, flags;
+   updates both copies). The scene BVH is not touched: call st_scene_rebuild,
+   or keep the stale BVH as the reference does when it only calls
+   uploadPrimitives (SlimTracin examples/02_Geometry.c:70-75). */
+int st_scene_set_prims(StSceneHandle *scene, const StPrimRec *prims, uint32_t first, uint32_t count);
+/* Rebuild the scene BVH over the current primitives (the reference's
+   updateSceneBVH, builder_top_down.h:366-381, when primitives move). `ms`
+   (optional) receives the build time: device time for LBVH. */
+int st_scene_rebuild(StSceneHandle *scene, uint32_t mode, float *ms);
+
 /* One request, one launch, blocking download: the plain per-request path,
    like renderSceneOnGPU (render/raytracer.h:258-299). hits and counts may be
    NULL. */

@@ -214,6 +214,20 @@ static RefScene *refLoad(const void *blob, size_t size) {
             }
         }
     }
+    /* Optional scene BVH built elsewhere (the Morton LBVH): loaded into the
+       reference's Scene.bvh in place of updateSceneBVH. */
+    const uint32_t *pre_scene_head = NULL;
+    const StBVHNodeRec *pre_scene_nodes = NULL;
+    const uint32_t *pre_scene_leaf = NULL;
+    if ((h->flags & ST_BLOB_HAS_SCENE_BVH) && !r.bad) {
+        pre_scene_head = (const uint32_t *)take(&r, 8);
+        if (pre_scene_head && pre_scene_head[0] <= 2u * h->prim_count) {
+            pre_scene_nodes = (const StBVHNodeRec *)take(&r, sizeof(StBVHNodeRec) * pre_scene_head[0]);
+            pre_scene_leaf = (const uint32_t *)take(&r, 4u * h->prim_count);
+        } else {
+            r.bad = 1;
+        }
+    }
     if (r.bad) { free(mrecs); free(pre_nodes); free(pre_tris); free(pre_leaf); refFree(rs); return NULL; }
 
     /* Builder sized like _initApp (app.h:277-279) then initBVHBuilder (builder_top_down.h:42-64). */
@@ -257,7 +271,22 @@ static RefScene *refLoad(const void *blob, size_t size) {
     }
     free(mrecs); free(pre_nodes); free(pre_tris); free(pre_leaf);
 
-    if (h->prim_count) updateSceneBVH(s, &rs->builder);
+    if (pre_scene_nodes) {
+        s->bvh.node_count = pre_scene_head[0];
+        s->bvh.height = pre_scene_head[1];
+        for (u32 n = 0; n < pre_scene_head[0]; n++) {
+            const StBVHNodeRec *q = pre_scene_nodes + n;
+            BVHNode *node = s->bvh.nodes + n;
+            node->aabb.min = Vec3(q->min[0], q->min[1], q->min[2]);
+            node->aabb.max = Vec3(q->max[0], q->max[1], q->max[2]);
+            node->first_child_id = q->first_child_id;
+            node->child_count = q->child_count;
+            node->depth = q->depth;
+        }
+        for (u32 i = 0; i < h->prim_count; i++) s->bvh.leaf_ids[i] = pre_scene_leaf[i];
+    } else if (h->prim_count) {
+        updateSceneBVH(s, &rs->builder);
+    }
     return rs;
 }
 

@@ -2055,7 +2055,148 @@ uint32_t oracle_sah_build(const float *boxes, uint32_t n, uint32_t max_leaf, StB
     return count;
 }
 
+/* ---- Morton LBVH (no live reference: the reference's Z-curve builder,
+   builder_bottom_up.h, never compiles) ------------------------
+   Specification shared with the device builder:
+     centroid c = (min + max) * 0.5f + 0.0f (the + 0 folds -0 to +0);
+     bounds = componentwise min/max of the centroids;
+     q = clamp((c - cmin) * (extent > 0 ? 1023 / extent : 0), 0, 1023) -> u32;
+     code = interleave(qx, qy, qz) with x in the highest of each bit triple;
+     primitives sorted by (code, original index);
+     binary radix tree over the unique keys (code << 32 | sorted position);
+     output: root = node 0, the two children of radix-tree internal node k
+     are nodes 2k+1 and 2k+2, leaves hold one primitive (child_count 1,
+     first_child_id = sorted position), boxes merged bottom-up as
+     merge(left, right), depth = tree depth, height = levels.
+   This restatement builds the radix tree top-down by splitting each range
+   at the highest differing key bit; the device builds it bottom-up with
+   Karras's per-node search (Karras 2012). The tree is unique, so both must
+   produce the same nodes. */
+static uint32_t expand10(uint32_t v) {
+    v &= 0x3FFu;
+    v = (v | (v << 16)) & 0x030000FFu;
+    v = (v | (v << 8)) & 0x0300F00Fu;
+    v = (v | (v << 4)) & 0x030C30C3u;
+    v = (v | (v << 2)) & 0x09249249u;
+    return v;
+}
+
+typedef struct { uint32_t code, index; } LKey;
+static int lkey_cmp(const void *a, const void *b) {
+    const LKey *x = (const LKey *)a, *y = (const LKey *)b;
+    if (x->code != y->code) return x->code < y->code ? -1 : 1;
+    return x->index < y->index ? -1 : x->index > y->index;
+}
+
+typedef struct {
+    const LKey *keys;
+    const Box3 *boxes;      /* by original index */
+    ONode *nodes;
+    uint32_t max_depth;
+} LBuild;
+
+static uint64_t lkey64(const LBuild *b, uint32_t pos) { return ((uint64_t)b->keys[pos].code << 32) | pos; }
+
+/* last position in [first, last) whose key shares more than the common
+   prefix of keys[first] and keys[last] with keys[first] */
+static uint32_t lsplit(const LBuild *b, uint32_t first, uint32_t last) {
+    uint64_t kf = lkey64(b, first), kl = lkey64(b, last);
+    int common = __builtin_clzll(kf ^ kl);
+    uint32_t split = first, step = last - first;
+    do {
+        step = (step + 1) >> 1;
+        uint32_t cand = split + step;
+        if (cand < last && __builtin_clzll(kf ^ lkey64(b, cand)) > common) split = cand;
+    } while (step > 1);
+    return split;
+}
+
+static Box3 lbuild(LBuild *b, uint32_t k, uint32_t first, uint32_t last, uint32_t out, uint32_t depth) {
+    uint32_t g = lsplit(b, first, last);
+    ONode *node = b->nodes + out;
+    node->first_child_id = 2 * k + 1;
+    node->child_count = 0;
+    node->depth = (uint16_t)depth;
+    if (depth + 1 > b->max_depth) b->max_depth = depth + 1;
+    Box3 bl, br;
+    uint32_t lo = 2 * k + 1, ro = 2 * k + 2;
+    if (first == g) {
+        ONode *l = b->nodes + lo;
+        l->aabb = bl = b->boxes[b->keys[g].index];
+        l->child_count = 1; l->first_child_id = g; l->depth = (uint16_t)(depth + 1);
+        if (depth + 2 > b->max_depth) b->max_depth = depth + 2;
+    } else
+        bl = lbuild(b, g, first, g, lo, depth + 1);
+    if (g + 1 == last) {
+        ONode *r = b->nodes + ro;
+        r->aabb = br = b->boxes[b->keys[g + 1].index];
+        r->child_count = 1; r->first_child_id = g + 1; r->depth = (uint16_t)(depth + 1);
+        if (depth + 2 > b->max_depth) b->max_depth = depth + 2;
+    } else
+        br = lbuild(b, g + 1, g + 1, last, ro, depth + 1);
+    node->aabb = merge_boxes(bl, br);
+    return node->aabb;
+}
+
 uint32_t oracle_lbvh_build(const float *boxes, uint32_t n, StBVHNodeRec *nodes, uint32_t *leaf_ids, uint32_t *height) {
-    (void)boxes; (void)n; (void)nodes; (void)leaf_ids; (void)height;
-    return 0; /* restated together with the device LBVH builder */
+    if (n == 0) { *height = 0; return 0; }
+    Box3 *bx = (Box3 *)calloc(n, sizeof(Box3));
+    LKey *keys = (LKey *)calloc(n, sizeof(LKey));
+    for (uint32_t i = 0; i < n; i++) {
+        bx[i].min = v3(boxes[6 * i], boxes[6 * i + 1], boxes[6 * i + 2]);
+        bx[i].max = v3(boxes[6 * i + 3], boxes[6 * i + 4], boxes[6 * i + 5]);
+    }
+    V3 cmin = v3s(INFINITY), cmax = v3s(-INFINITY);
+    V3 *cent = (V3 *)calloc(n, sizeof(V3));
+    for (uint32_t i = 0; i < n; i++) {
+        V3 c = v3((bx[i].min.x + bx[i].max.x) * 0.5f + 0.0f, (bx[i].min.y + bx[i].max.y) * 0.5f + 0.0f,
+                  (bx[i].min.z + bx[i].max.z) * 0.5f + 0.0f);
+        cent[i] = c;
+        cmin.x = c.x < cmin.x ? c.x : cmin.x; cmin.y = c.y < cmin.y ? c.y : cmin.y; cmin.z = c.z < cmin.z ? c.z : cmin.z;
+        cmax.x = c.x > cmax.x ? c.x : cmax.x; cmax.y = c.y > cmax.y ? c.y : cmax.y; cmax.z = c.z > cmax.z ? c.z : cmax.z;
+    }
+    V3 ext = sub3(cmax, cmin);
+    V3 sc = v3(ext.x > 0 ? 1023.0f / ext.x : 0.0f, ext.y > 0 ? 1023.0f / ext.y : 0.0f, ext.z > 0 ? 1023.0f / ext.z : 0.0f);
+    for (uint32_t i = 0; i < n; i++) {
+        float t[3] = {(cent[i].x - cmin.x) * sc.x, (cent[i].y - cmin.y) * sc.y, (cent[i].z - cmin.z) * sc.z};
+        uint32_t q[3];
+        for (int a = 0; a < 3; a++) {
+            float v = t[a] < 0 ? 0 : t[a];
+            v = v > 1023.0f ? 1023.0f : v;
+            q[a] = (uint32_t)v;
+        }
+        keys[i].code = (expand10(q[0]) << 2) | (expand10(q[1]) << 1) | expand10(q[2]);
+        keys[i].index = i;
+    }
+    qsort(keys, n, sizeof(LKey), lkey_cmp);
+    for (uint32_t j = 0; j < n; j++) leaf_ids[j] = keys[j].index;
+    ONode *on = (ONode *)calloc(2 * (size_t)n, sizeof(ONode));
+    uint32_t count;
+    if (n == 1) {
+        on[0].aabb = bx[0];
+        on[0].child_count = 1;
+        on[0].first_child_id = 0;
+        *height = 1;
+        count = 1;
+    } else {
+        LBuild b = {keys, bx, on, 0};
+        lbuild(&b, 0, 0, n - 1, 0, 0);
+        *height = b.max_depth;
+        count = 2 * n - 1;
+    }
+    for (uint32_t i = 0; i < count; i++) node_to_rec(nodes + i, on + i);
+    free(bx); free(keys); free(cent); free(on);
+    return count;
+}
+
+int oracle_prim_boxes(const void *blob, size_t size, float *boxes) {
+    OScene *sc = scene_load(blob, size);
+    if (!sc) return -1;
+    for (uint32_t i = 0; i < sc->prim_count; i++) {
+        Box3 b = prim_world_box(sc, sc->prims + i);
+        float v[6] = {b.min.x, b.min.y, b.min.z, b.max.x, b.max.y, b.max.z};
+        memcpy(boxes + 6 * i, v, sizeof v);
+    }
+    scene_free(sc);
+    return 0;
 }

@@ -64,6 +64,9 @@ uint32_t oracle_sah_build(const float *boxes, uint32_t n, uint32_t max_leaf, StB
 uint32_t oracle_lbvh_build(const float *boxes, uint32_t n, StBVHNodeRec *nodes, uint32_t *leaf_ids,
                            uint32_t *height);
 
+/* World AABBs of the blob's primitives (getPrimitiveAABB + transformAABB). */
+int oracle_prim_boxes(const void *blob, size_t size, float *boxes);
+
 #ifdef __cplusplus
 }
 #endif

@@ -12,11 +12,13 @@ import os
 
 import numpy as np
 
-from .scene import HIT_DTYPE, NODE_DTYPE, PIXEL_DTYPE, VIEW_DTYPE, grid_size  # noqa: F401
+from .scene import HIT_DTYPE, NODE_DTYPE, PIXEL_DTYPE, PRIM_DTYPE, VIEW_DTYPE, grid_size  # noqa: F401
 
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib", "libslimtracin.so")
 
 ST_BATCH_PERSISTENT = 1
+ST_REBUILD_SAH = 0
+ST_REBUILD_LBVH = 1
 
 
 class SlimTracinError(RuntimeError):
@@ -122,6 +124,17 @@ class DeviceScene:
         cnt, h = C.c_uint32(), C.c_uint32()
         _check(lib().st_scene_bvh(self.handle, _ptr(nodes), _ptr(leaf), C.byref(cnt), C.byref(h)))
         return nodes[:cnt.value], leaf[:inf.prim_count], h.value
+
+    def set_prims(self, prims: np.ndarray, first: int = 0):
+        """Replace primitive records [first, first+len(prims)) (st_scene_set_prims)."""
+        prims = np.ascontiguousarray(prims, dtype=PRIM_DTYPE)
+        _check(lib().st_scene_set_prims(self.handle, _ptr(prims), C.c_uint32(first), C.c_uint32(len(prims))))
+
+    def rebuild(self, mode: int = ST_REBUILD_LBVH) -> float:
+        """Rebuild the scene BVH (st_scene_rebuild); returns the build time in ms."""
+        ms = C.c_float()
+        _check(lib().st_scene_rebuild(self.handle, C.c_uint32(mode), C.byref(ms)))
+        return ms.value
 
 
 def render(scene: DeviceScene, view: np.ndarray, hits: bool = True):

@@ -28,6 +28,7 @@
 #include <time.h>
 
 #include "st_trace.h"
+#include "st_lbvh.h"
 #include "../host/st_host.h"
 #include "slimtracin.h"
 
@@ -269,6 +270,10 @@ struct StSceneHandle {
     StDScene dev;
     char *dbuf;
     uint64_t dbytes;
+    float *d_mesh_aabbs;      /* 6 floats per mesh, for device BVH rebuilds */
+    char *lbvh_work;          /* scratch of device LBVH rebuilds */
+    uint32_t *d_height;       /* height written by the device builder */
+    int bvh_on_device_only;   /* host copy of the scene BVH is stale */
 };
 
 extern "C" int st_scene_create(const void *blob, size_t size, StSceneHandle **out) {
@@ -303,7 +308,8 @@ extern "C" int st_scene_upload(StSceneHandle *h) {
     size_t o_tex = o_cold + round256(((size_t)sc->tri_total + 1) * sizeof(StDTriCold));
     size_t o_mips = o_tex + round256((sc->texture_count + 1) * sizeof(StDTex));
     size_t o_texels = o_mips + round256((sc->mip_total + 1) * sizeof(StDMip));
-    size_t total = o_texels + round256(sc->texel_bytes + 16);
+    size_t o_maabb = o_texels + round256(sc->texel_bytes + 16);
+    size_t total = o_maabb + round256((sc->mesh_count + 1) * 24ull);
     if (h->dbuf) HIP_TRY(hipFree(h->dbuf));
     h->dbuf = NULL;
     hipError_t e = hipMalloc(&h->dbuf, total);
@@ -328,6 +334,19 @@ extern "C" int st_scene_upload(StSceneHandle *h) {
     };
     for (size_t i = 0; i < sizeof parts / sizeof parts[0]; i++)
         if (parts[i].bytes) HIP_TRY(hipMemcpy(b + parts[i].off, parts[i].src, parts[i].bytes, hipMemcpyHostToDevice));
+    if (sc->mesh_count) {
+        float *ma = (float *)malloc(24ull * sc->mesh_count);
+        for (uint32_t m = 0; m < sc->mesh_count; m++) {
+            const sbox *bb = sc->mesh_aabbs + m;
+            float v[6] = {bb->min.x, bb->min.y, bb->min.z, bb->max.x, bb->max.y, bb->max.z};
+            memcpy(ma + 6 * m, v, 24);
+        }
+        hipError_t ce = hipMemcpy(b + o_maabb, ma, 24ull * sc->mesh_count, hipMemcpyHostToDevice);
+        free(ma);
+        if (ce != hipSuccess) return set_err(ST_ERR_HIP, "mesh AABB upload failed: %s", hipGetErrorString(ce));
+    }
+    h->d_mesh_aabbs = (float *)(b + o_maabb);
+    h->bvh_on_device_only = 0;
     StDScene &d = h->dev;
     memset(&d, 0, sizeof d);
     d.prims = (const StDPrim *)(b + o_prims);
@@ -356,6 +375,8 @@ extern "C" int st_scene_upload(StSceneHandle *h) {
 extern "C" void st_scene_destroy(StSceneHandle *h) {
     if (!h) return;
     if (h->dbuf) (void)hipFree(h->dbuf);
+    if (h->lbvh_work) (void)hipFree(h->lbvh_work);
+    if (h->d_height) (void)hipFree(h->d_height);
     st_scene_free(h->host);
     free(h);
 }
@@ -378,10 +399,93 @@ extern "C" int st_scene_info(const StSceneHandle *h, StSceneInfo *info) {
 extern "C" int st_scene_bvh(const StSceneHandle *h, StBVHNodeRec *nodes, uint32_t *leaf_ids, uint32_t *node_count,
                             uint32_t *height) {
     const StScene *sc = h->host;
+    if (h->bvh_on_device_only) {
+        HIP_TRY(hipStreamSynchronize(g.stream));
+        HIP_TRY(hipMemcpy(sc->scene_nodes, h->dev.scene_nodes, sizeof(StBVHNodeRec) * sc->scene_node_count,
+                          hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(sc->scene_leaf_ids, h->dev.scene_leaf_ids, 4u * sc->prim_count, hipMemcpyDeviceToHost));
+        ((StSceneHandle *)h)->bvh_on_device_only = 0;
+    }
     memcpy(nodes, sc->scene_nodes, sizeof(StBVHNodeRec) * sc->scene_node_count);
     memcpy(leaf_ids, sc->scene_leaf_ids, 4u * sc->prim_count);
     *node_count = sc->scene_node_count;
     *height = sc->scene_height;
+    return ST_OK;
+}
+
+/* ---- moving primitives ------------------------------------------------- */
+
+extern "C" int st_scene_set_prims(StSceneHandle *h, const StPrimRec *recs, uint32_t first, uint32_t count) {
+    if (!h || !recs) return set_err(ST_ERR_ARG, "null argument");
+    StScene *sc = h->host;
+    if ((uint64_t)first + count > sc->prim_count) return set_err(ST_ERR_ARG, "primitive range out of bounds");
+    for (uint32_t i = 0; i < count; i++) {
+        const StPrimRec *r = recs + i;
+        if (r->material_id >= sc->material_count || r->material_id > 255)
+            return set_err(ST_ERR_ARG, "primitive %u: material_id out of range", first + i);
+        if (r->type == ST_PRIM_MESH && r->mesh_id >= sc->mesh_count)
+            return set_err(ST_ERR_ARG, "primitive %u: mesh_id out of range", first + i);
+    }
+    for (uint32_t i = 0; i < count; i++) sc->prims[first + i] = st_prim_from_rec(recs + i);
+    sc->has_emissive_quads = 0;
+    for (uint32_t i = 0; i < sc->prim_count; i++) {
+        uint32_t mid = (sc->prims[i].flags_mat >> 8) & 0xFFu;
+        if (sc->prims[i].type == ST_PRIM_QUAD && (sc->mats[mid].bits & ST_EMISSIVE)) sc->has_emissive_quads = 1;
+    }
+    h->dev.has_emissive_quads = sc->has_emissive_quads;
+    HIP_TRY(hipMemcpyAsync((void *)(h->dev.prims + first), sc->prims + first, count * sizeof(StDPrim),
+                           hipMemcpyHostToDevice, g.stream));
+    HIP_TRY(hipStreamSynchronize(g.stream));
+    return ST_OK;
+}
+
+/* Rebuild the scene BVH over the current primitive transforms: the
+   reference's SAH builder on the host, or the Morton LBVH on the device. */
+extern "C" int st_scene_rebuild(StSceneHandle *h, uint32_t mode, float *ms) {
+    if (!h) return set_err(ST_ERR_ARG, "null scene");
+    int rc = ensure_init();
+    if (rc) return rc;
+    StScene *sc = h->host;
+    uint32_t P = sc->prim_count;
+    if (!P) return ST_OK;
+    if (mode == ST_REBUILD_SAH) {
+        struct timespec t0, t1;
+        clock_gettime(CLOCK_MONOTONIC, &t0);
+        if (st_scene_rebuild_sah(sc)) return set_err(ST_ERR_MEMORY, "SAH rebuild failed");
+        int height = st_bvh_validate(sc->scene_nodes, sc->scene_node_count, P);
+        if (height < 0 || height + 1 > ST_SCENE_STACK_MAX) return set_err(ST_ERR_SCENE, "rebuilt BVH too deep");
+        sc->scene_real_height = (uint32_t)height;
+        HIP_TRY(hipMemcpy((void *)h->dev.scene_nodes, sc->scene_nodes, sizeof(StBVHNodeRec) * sc->scene_node_count,
+                          hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy((void *)h->dev.scene_leaf_ids, sc->scene_leaf_ids, 4u * P, hipMemcpyHostToDevice));
+        h->dev.scene_stack_need = sc->scene_real_height + 1;
+        h->bvh_on_device_only = 0;
+        clock_gettime(CLOCK_MONOTONIC, &t1);
+        if (ms) *ms = (float)((t1.tv_sec - t0.tv_sec) * 1e3 + (t1.tv_nsec - t0.tv_nsec) * 1e-6);
+        return ST_OK;
+    }
+    if (mode != ST_REBUILD_LBVH) return set_err(ST_ERR_ARG, "unknown rebuild mode %u", mode);
+    if (P > ST_LBVH_MAX) return set_err(ST_ERR_ARG, "device LBVH supports up to %u primitives", ST_LBVH_MAX);
+    if (!h->lbvh_work) {
+        HIP_TRY(hipMalloc(&h->lbvh_work, st_lbvh_work_bytes(ST_LBVH_MAX)));
+        HIP_TRY(hipMalloc(&h->d_height, 256));
+    }
+    StLbvhWork w = st_lbvh_work_carve(h->lbvh_work, P);
+    HIP_TRY(hipEventRecord(g.ev[2], g.stream));
+    if (st_lbvh_launch(g.stream, h->dev.prims, h->d_mesh_aabbs, sc->mesh_count, P, w, (uint32_t *)h->dev.scene_nodes,
+                       (uint32_t *)h->dev.scene_leaf_ids, h->d_height))
+        return set_err(ST_ERR_HIP, "LBVH launch failed: %s", hipGetErrorString(hipGetLastError()));
+    HIP_TRY(hipEventRecord(g.ev[3], g.stream));
+    uint32_t height = 0;
+    HIP_TRY(hipMemcpyAsync(&height, h->d_height, 4, hipMemcpyDeviceToHost, g.stream));
+    HIP_TRY(hipStreamSynchronize(g.stream));
+    if (ms) HIP_TRY(hipEventElapsedTime(ms, g.ev[2], g.ev[3]));
+    if (height == 0 || height + 1 > ST_SCENE_STACK_MAX) return set_err(ST_ERR_SCENE, "LBVH height %u too deep", height);
+    sc->scene_node_count = 2 * P - 1;
+    sc->scene_height = height;
+    sc->scene_real_height = height;
+    h->dev.scene_stack_need = height + 1;
+    h->bvh_on_device_only = 1;
     return ST_OK;
 }
 

@@ -66,6 +66,12 @@ uint64_t st_scene_device_bytes(const StScene *sc);
    builder (updateSceneBVH, builder_top_down.h:366-381). */
 int st_scene_rebuild_sah(StScene *sc);
 
+/* Tree check of a BVH in the reference layout; returns its height or -1. */
+int st_bvh_validate(const StBVHNodeRec *nodes, uint32_t count, uint64_t leaf_limit);
+
+/* Device record of a blob primitive record. */
+StDPrim st_prim_from_rec(const StPrimRec *r);
+
 /* World AABB of a primitive (getPrimitiveAABB + transformAABB). */
 sbox st_prim_world_box(const StScene *sc, uint32_t i);
 

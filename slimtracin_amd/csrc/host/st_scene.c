@@ -33,11 +33,11 @@ void st_scene_free(StScene *sc) {
 }
 
 
-/* Reject BVHs the device traversal could not walk safely: every inner node's
-   children must exist and have larger ids than the node (so no descent can
-   loop), every node is reachable once (a tree, not a DAG), and leaves must
-   index inside [0, leaf_limit). Returns the tree's real height or -1. */
-static int bvh_validate(const StBVHNodeRec *nodes, uint32_t count, uint64_t leaf_limit) {
+/* Reject BVHs the device traversal could not walk safely: every inner
+   node's two children must exist, no node may be reached twice from the root
+   (a tree: no cycle, no shared subtree, so every descent ends), and leaves
+   must index inside [0, leaf_limit). Returns the tree's real height or -1. */
+int st_bvh_validate(const StBVHNodeRec *nodes, uint32_t count, uint64_t leaf_limit) {
     if (!count) return -1;
     uint8_t *seen = (uint8_t *)calloc(count, 1);
     uint32_t *stack = (uint32_t *)malloc(2 * sizeof(uint32_t) * ((size_t)count + 1));
@@ -56,7 +56,7 @@ static int bvh_validate(const StBVHNodeRec *nodes, uint32_t count, uint64_t leaf
             continue;
         }
         uint32_t c = nd->first_child_id;
-        if (c <= n || (uint64_t)c + 1 >= count || seen[c] || seen[c + 1]) { ok = 0; break; }
+        if ((uint64_t)c + 1 >= count || seen[c] || seen[c + 1]) { ok = 0; break; }
         seen[c] = seen[c + 1] = 1;
         stack[top++] = c; stack[top++] = depth + 1;
         stack[top++] = c + 1; stack[top++] = depth + 1;
@@ -66,7 +66,7 @@ static int bvh_validate(const StBVHNodeRec *nodes, uint32_t count, uint64_t leaf
     return ok ? max_depth : -1;
 }
 
-static StDPrim prim_from_rec(const StPrimRec *r) {
+StDPrim st_prim_from_rec(const StPrimRec *r) {
     StDPrim p;
     memset(&p, 0, sizeof p);
     memcpy(p.rot, r->rotation, 16);
@@ -169,7 +169,7 @@ StScene *st_scene_from_blob(const void *blob, size_t size, const char **err) {
         if (pr[i].type == ST_PRIM_MESH && pr[i].mesh_id >= h->mesh_count) {
             *err = "primitive mesh_id out of range"; st_scene_free(sc); return NULL;
         }
-        sc->prims[i] = prim_from_rec(pr + i);
+        sc->prims[i] = st_prim_from_rec(pr + i);
     }
     sc->mats = (StDMat *)calloc(h->material_count + 1, sizeof(StDMat));
     for (uint32_t i = 0; i < h->material_count; i++) {
@@ -294,7 +294,7 @@ StScene *st_scene_from_blob(const void *blob, size_t size, const char **err) {
         }
         dm->height = height;
         if (height > sc->max_mesh_height) sc->max_mesh_height = height;
-        if (bvh_validate(sc->mesh_nodes + dm->node_offset, count, q->triangle_count) < 0) {
+        if (st_bvh_validate(sc->mesh_nodes + dm->node_offset, count, q->triangle_count) < 0) {
             *err = "malformed mesh BVH"; free(src); st_scene_free(sc); return NULL;
         }
     }
@@ -375,7 +375,7 @@ StScene *st_scene_from_blob(const void *blob, size_t size, const char **err) {
         st_scene_rebuild_sah(sc);
     }
     if (P) {
-        int real_height = bvh_validate(sc->scene_nodes, sc->scene_node_count, P);
+        int real_height = st_bvh_validate(sc->scene_nodes, sc->scene_node_count, P);
         if (real_height < 0) { *err = "malformed scene BVH"; st_scene_free(sc); return NULL; }
         sc->scene_real_height = (uint32_t)real_height;
         if (real_height + 1 > ST_SCENE_STACK_MAX) {

@@ -41,7 +41,11 @@ NODE_DTYPE = np.dtype([("min", "<f4", 3), ("max", "<f4", 3), ("first_child_id", 
 TRIANGLE_DTYPE = np.dtype([("world_to_tangent", "<f4", 9), ("position", "<f4", 3), ("normal", "<f4", 3),
                            ("vertex_normals", "<f4", 9), ("uvs", "<f4", 6),
                            ("area_of_parallelogram", "<f4"), ("area_of_uv", "<f4")])
+PRIM_DTYPE = np.dtype([("position", "<f4", 3), ("scale", "<f4", 3), ("rotation", "<f4", 4), ("type", "<u4"),
+                       ("mesh_id", "<u4"), ("material_id", "<u4"), ("flags", "<u4")])
+BLOB_HEADER_BYTES = 56
 assert VIEW_DTYPE.itemsize == 48 and PIXEL_DTYPE.itemsize == 24 and HIT_DTYPE.itemsize == 16
+assert PRIM_DTYPE.itemsize == 56
 assert NODE_DTYPE.itemsize == 32 and TRIANGLE_DTYPE.itemsize == 128
 
 
@@ -225,3 +229,32 @@ def make_view(position=(0.0, 0.0, 0.0), yaw=0.0, pitch=0.0, roll=0.0, width=64, 
 def grid_size(view) -> int:
     """Traced pixel count: (width+1)*(height+1) (QCAA, core/base.h:344-358)."""
     return (int(view["width"][0]) + 1) * (int(view["height"][0]) + 1)
+
+
+# ---- blob edits (the primitive array follows the 56-byte header) ----------
+
+def blob_prims(blob: bytes) -> np.ndarray:
+    """Copy of the blob's StPrimRec array."""
+    n = struct.unpack_from("<I", blob, 8)[0]
+    return np.frombuffer(blob, dtype=PRIM_DTYPE, count=n, offset=BLOB_HEADER_BYTES).copy()
+
+
+def blob_with_prims(blob: bytes, prims: np.ndarray) -> bytes:
+    """The same scene with its primitive records replaced (same count)."""
+    n = struct.unpack_from("<I", blob, 8)[0]
+    prims = np.ascontiguousarray(prims, dtype=PRIM_DTYPE)
+    assert len(prims) == n
+    end = BLOB_HEADER_BYTES + n * PRIM_DTYPE.itemsize
+    return blob[:BLOB_HEADER_BYTES] + prims.tobytes() + blob[end:]
+
+
+def blob_with_scene_bvh(blob: bytes, nodes: np.ndarray, leaf_ids: np.ndarray, height: int) -> bytes:
+    """The same scene carrying a prebuilt scene BVH (e.g. one built by the
+    device LBVH), used instead of the SAH build when the blob is loaded."""
+    flags = struct.unpack_from("<I", blob, 40)[0]
+    assert not flags & BLOB_HAS_SCENE_BVH, "blob already carries a scene BVH"
+    head = bytearray(blob[:BLOB_HEADER_BYTES])
+    struct.pack_into("<I", head, 40, flags | BLOB_HAS_SCENE_BVH)
+    return (bytes(head) + blob[BLOB_HEADER_BYTES:] + struct.pack("<2I", len(nodes), int(height))
+            + np.ascontiguousarray(nodes, dtype=NODE_DTYPE).tobytes()
+            + np.ascontiguousarray(leaf_ids, dtype=np.uint32).tobytes())

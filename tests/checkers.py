@@ -217,3 +217,22 @@ def first_pixel_mismatches(a, b, k=5):
     bv = b.view(np.uint8).reshape(len(b), -1)
     bad = np.nonzero((av != bv).any(axis=1))[0]
     return bad[:k], len(bad)
+
+
+def oracle_lbvh_build(boxes):
+    boxes = np.ascontiguousarray(boxes, dtype=np.float32).reshape(-1, 6)
+    n = len(boxes)
+    nodes = np.zeros(2 * n + 2, dtype=NODE_DTYPE)
+    leaf = np.zeros(n + 1, dtype=np.uint32)
+    h = C.c_uint32()
+    lib = _load(ORACLE_SO)
+    lib.oracle_lbvh_build.restype = C.c_uint32
+    cnt = lib.oracle_lbvh_build(_ptr(boxes), C.c_uint32(n), _ptr(nodes), _ptr(leaf), C.byref(h))
+    return nodes[:cnt], leaf[:n], h.value
+
+
+def oracle_prim_boxes(blob, prim_count):
+    out = np.zeros((prim_count, 6), dtype=np.float32)
+    bp, bs, keep = _blob(blob)
+    assert _load(ORACLE_SO).oracle_prim_boxes(bp, bs, _ptr(out)) == 0
+    return out

@@ -254,3 +254,55 @@ ALL_EXAMPLES = {
     "ex01": ex01_lights, "ex02": ex02_geometry, "ex03": ex03_blinn_phong, "ex04": ex04_glass_mirror,
     "ex05": ex05_pbr, "ex06": ex06_area_lights,
 }
+
+
+def crowd(seed=0, n=300, meshes=True):
+    """A field of n randomly placed primitives of every type (boxes, spheres,
+    tetrahedra, quads, monkey instances) over a floor quad, with lambert,
+    Blinn/Phong, Cook-Torrance, glass and mirror materials: the scene-BVH
+    rebuild workload (primitives that move every frame)."""
+    rng = np.random.default_rng(seed)
+    s = Scene(ambient=(0.004, 0.004, 0.007))
+    s.lights = _three_point_lights()
+    s.materials = [
+        Material(albedo=(0.8, 1.0, 0.8)),
+        Material(brdf=BRDF_PHONG, roughness=0.5, albedo=(0.2, 0.2, 0.3), reflectivity=(1.0, 1.0, 0.4)),
+        Material(brdf=BRDF_BLINN, roughness=0.5, albedo=(1.0, 0.3, 1.0), reflectivity=(1.0, 0.4, 1.0)),
+        Material(brdf=BRDF_COOK_TORRANCE, roughness=0.4, metallic=0.3, albedo=(0.6, 0.5, 0.3)),
+        Material(is_=REFRACTIVE, n1_over_n2=IOR_AIR / IOR_GLASS, n2_over_n1=IOR_GLASS / IOR_AIR,
+                 reflectivity=(0.7, 0.9, 0.7), roughness=0.0, brdf=BRDF_BLINN),
+        Material(is_=REFLECTIVE, reflectivity=(0.9, 0.9, 0.9), roughness=0.0, brdf=BRDF_BLINN),
+    ]
+    types = [PRIM_BOX, PRIM_SPHERE, PRIM_TETRAHEDRON, PRIM_QUAD]
+    if meshes:
+        s.meshes = [read_reference_mesh(os.path.join(GOLDEN, "monkey.mesh"))]
+        types.append(PRIM_MESH)
+    s.primitives = [Primitive(PRIM_QUAD, scale=(60, 1, 60))]
+    for i in range(n - 1):
+        t = types[int(rng.integers(len(types)))]
+        s.primitives.append(Primitive(t, position=random_position(rng), scale=tuple(rng.uniform(0.4, 1.6, 3)),
+                                      rotation=random_rotation(rng), material_id=int(rng.integers(len(s.materials))),
+                                      flags=63))
+    return s, dict(position=(0.0, 14.0, -24.0), pitch=-0.35, depth=3)
+
+
+def random_position(rng):
+    return (float(rng.uniform(-25, 25)), float(rng.uniform(0.5, 9)), float(rng.uniform(-8, 40)))
+
+
+def random_rotation(rng):
+    """A float32 unit rotation as the reference composes them (axis-angle
+    quaternions multiplied and normalised)."""
+    q = quat_axis((1, 0, 0), float(rng.uniform(-3, 3)))
+    q = quat_mul_norm(q, quat_axis((0, 1, 0), float(rng.uniform(-3, 3))))
+    return tuple(quat_mul_norm(q, quat_axis((0, 0, 1), float(rng.uniform(-3, 3)))))
+
+
+def move_prims(prims, rng, first=1):
+    """Random rigid motion of every primitive from `first` on: new position and
+    rotation, as an animation step would set them."""
+    out = prims.copy()
+    for i in range(first, len(out)):
+        out["position"][i] = random_position(rng)
+        out["rotation"][i] = random_rotation(rng)
+    return out

@@ -1,0 +1,181 @@
+"""The oracle's Morton LBVH against the definition of the binary radix tree.
+
+The LBVH has no reference counterpart (the reference's Z-curve builder never
+compiles). Its specification (st_oracle.c, "Morton LBVH") is pinned here by
+an independent construction: float32 Morton codes computed with numpy, keys
+made unique with the sorted position, and the tree built by recursively
+splitting each key range at its highest differing bit with Python integers.
+The node numbering (children of radix node k at 2k+1, 2k+2), leaf order and
+merged boxes must match bit for bit, and the tree invariants must hold.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import checkers as ck
+
+pytestmark = pytest.mark.skipif(not ck.oracle_available(), reason="oracle not built")
+f32 = np.float32
+
+
+def expand10(v):
+    out = 0
+    for b in range(10):
+        out |= ((v >> b) & 1) << (3 * b)
+    return out
+
+
+def reference_lbvh(boxes):
+    boxes = boxes.astype(f32)
+    n = len(boxes)
+    c = ((boxes[:, :3] + boxes[:, 3:]) * f32(0.5) + f32(0.0)).astype(f32)
+    cmin, cmax = c.min(axis=0), c.max(axis=0)
+    ext = (cmax - cmin).astype(f32)
+    scale = np.array([f32(1023.0) / e if e > 0 else f32(0) for e in ext], dtype=f32)
+    codes = []
+    for i in range(n):
+        q = []
+        for a in range(3):
+            t = f32((c[i, a] - cmin[a]) * scale[a])
+            t = f32(0) if t < 0 else t
+            t = f32(1023.0) if t > 1023 else t
+            q.append(int(t))
+        codes.append((expand10(q[0]) << 2) | (expand10(q[1]) << 1) | expand10(q[2]))
+    order = sorted(range(n), key=lambda i: (codes[i], i))
+    keys = [(codes[i] << 32) | pos for pos, i in enumerate(order)]
+    nodes = {}
+    depth_max = [0]
+
+    def merge(a, b):
+        # mergeAABBs' ternaries (render/AABB.h:25-37), signed zeros included
+        return np.array([a[0] if a[0] < b[0] else b[0],
+                         a[1] if a[1] < b[1] else b[1], a[2] if a[2] < b[2] else b[2],
+                         a[3] if a[3] > b[3] else b[3], a[4] if a[4] > b[4] else b[4],
+                         a[5] if a[5] > b[5] else b[5]], dtype=f32)
+
+    def build(k, first, last, out, depth):
+        kf, kl = keys[first], keys[last]
+        bit = (kf ^ kl).bit_length() - 1            # highest differing bit
+        g = first
+        while g + 1 <= last and (keys[g + 1] >> bit) == (kf >> bit):
+            g += 1
+        depth_max[0] = max(depth_max[0], depth + 2 if (first == g or g + 1 == last) else depth + 1)
+        if first == g:
+            lb = boxes[order[g]]
+            nodes[2 * k + 1] = (lb, g, 1, depth + 1)
+        else:
+            lb = build(g, first, g, 2 * k + 1, depth + 1)
+        if g + 1 == last:
+            rb = boxes[order[g + 1]]
+            nodes[2 * k + 2] = (rb, g + 1, 1, depth + 1)
+        else:
+            rb = build(g + 1, g + 1, last, 2 * k + 2, depth + 1)
+        b = merge(lb, rb)
+        nodes[out] = (b, 2 * k + 1, 0, depth)
+        return b
+
+    build(0, 0, n - 1, 0, 0)
+    return nodes, order, depth_max[0]
+
+
+@pytest.mark.parametrize("seed", range(8))
+def test_oracle_lbvh_equals_radix_tree_definition(seed):
+    rng = np.random.default_rng(seed)
+    n = int(rng.integers(2, 70))
+    lo = rng.uniform(-5, 5, (n, 3)).astype(f32)
+    if seed % 3 == 0:
+        lo = np.round(lo).astype(f32)          # duplicate centroids -> equal Morton codes
+    hi = (lo + rng.uniform(0, 2, (n, 3))).astype(f32)
+    boxes = np.concatenate([lo, hi], axis=1)
+    nodes, leaf, height = ck.oracle_lbvh_build(boxes)
+    want, order, h = reference_lbvh(boxes)
+    assert len(nodes) == 2 * n - 1 == len(want)
+    assert list(leaf) == order
+    assert height == h
+    for o, (b, first, count, depth) in want.items():
+        node = nodes[o]
+        assert node["first_child_id"] == first and node["child_count"] == count and node["depth"] == depth, o
+        assert node["min"].tobytes() == b[:3].tobytes() and node["max"].tobytes() == b[3:].tobytes(), o
+
+
+def test_lbvh_tree_invariants():
+    rng = np.random.default_rng(42)
+    n = 500
+    lo = rng.uniform(-50, 50, (n, 3)).astype(f32)
+    boxes = np.concatenate([lo, lo + rng.uniform(0, 3, (n, 3)).astype(f32)], axis=1)
+    nodes, leaf, height = ck.oracle_lbvh_build(boxes)
+    assert sorted(leaf.tolist()) == list(range(n))          # a permutation
+    seen, stack, levels = set(), [(0, 1)], 0
+    while stack:
+        o, d = stack.pop()
+        assert o not in seen
+        seen.add(o)
+        levels = max(levels, d)
+        nd = nodes[o]
+        if nd["child_count"]:
+            b = boxes[leaf[nd["first_child_id"]]]
+            assert (nd["min"] == b[:3]).all() and (nd["max"] == b[3:]).all()
+            continue
+        for c in (nd["first_child_id"], nd["first_child_id"] + 1):
+            assert (nodes[c]["min"] >= nd["min"]).all() and (nodes[c]["max"] <= nd["max"]).all()
+            stack.append((c, d + 1))
+    assert len(seen) == 2 * n - 1 and levels == height
+
+
+# ---- the LBVH's inputs and its use by the reference traversal -------------
+
+import os  # noqa: E402
+
+import scenes  # noqa: E402
+from slimtracin_amd.scene import (  # noqa: E402
+    MODE_BEAUTY, MODE_NORMALS, blob_prims, blob_with_prims, blob_with_scene_bvh, make_view,
+)
+
+FIX = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "reference_cases.npz")
+
+
+def test_prim_boxes_match_reference_scene_bvh_leaves():
+    """oracle_prim_boxes (getPrimitiveAABB + transformAABB) against the
+    reference's own scene BVH leaves in the golden fixtures: every leaf box is
+    the merge of its 1-2 primitives' boxes."""
+    with np.load(FIX, allow_pickle=False) as z:
+        for name in [str(n) for n in z["names"]]:
+            blob, nodes, leaf = z[f"{name}/blob"], z[f"{name}/scene_nodes"], z[f"{name}/scene_leaf"]
+            boxes = ck.oracle_prim_boxes(blob.tobytes(), len(leaf))
+            for nd in nodes[nodes["child_count"] > 0]:
+                ids = leaf[nd["first_child_id"]:nd["first_child_id"] + nd["child_count"]]
+                b = boxes[ids[0]]
+                for i in ids[1:]:
+                    c = boxes[i]
+                    b = np.array([b[k] if b[k] < c[k] else c[k] for k in range(3)]
+                                 + [b[k] if b[k] > c[k] else c[k] for k in range(3, 6)], dtype=f32)
+                assert nd["min"].tobytes() == b[:3].tobytes() and nd["max"].tobytes() == b[3:].tobytes(), name
+
+
+def _lbvh_blob(blob):
+    P = len(blob_prims(blob))
+    nodes, leaf, h = ck.oracle_lbvh_build(ck.oracle_prim_boxes(blob, P))
+    return blob_with_scene_bvh(blob, nodes, leaf, h)
+
+
+@pytest.mark.skipif(not ck.ref_available(), reason="reference build missing")
+@pytest.mark.parametrize("seed", [0, 1])
+def test_reference_traversal_over_lbvh(seed):
+    """The reference renderer, handed the LBVH instead of its SAH tree, finds
+    the same primary hits (the LBVH bounds every primitive), and the oracle
+    renders the LBVH scene bit-exactly like the reference."""
+    s, cam = scenes.crowd(seed, 120)
+    blob = s.to_blob()
+    moved = blob_with_prims(blob, scenes.move_prims(blob_prims(blob), np.random.default_rng(seed + 10)))
+    lb = _lbvh_blob(moved)
+    view = make_view(width=96, height=72, mode=MODE_NORMALS, **{k: v for k, v in cam.items() if k != "depth"})
+    sah_hits = ck.ref_primary_hits(moved, view)
+    lbvh_hits = ck.ref_primary_hits(lb, view)
+    assert sah_hits.tobytes() == lbvh_hits.tobytes()
+    for mode in (MODE_BEAUTY, MODE_NORMALS):
+        view["render_mode"] = mode
+        view["depth"] = cam["depth"]
+        ref = ck.ref_render(lb, view)
+        ora, _, _ = ck.oracle_render(lb, view)
+        assert ck.pixels_equal(ref, ora), mode
