@@ -15,12 +15,15 @@ Also measured in the same run:
     first as 64 sequential per-request launches with a blocking download
     each (the reference's renderSceneOnGPU pattern), then as one persistent
     launch over a global tile queue plus the downloads;
-  - roofline of the render kernel: algorithmic bytes per ray from the CPU
-    oracle's event counters on a 480x270 view of the same scene, times the
-    device ray count of one launch, over the kernel's HIP-event duration;
+  - roofline of the render kernel: algorithmic bytes of the frame from the
+    CPU oracle's event counters on the same 1920x1080 frame (bytes per event
+    at the reference's record sizes), over the kernel's HIP-event duration;
+    the measured HBM traffic per launch comes from the rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary kept in profiles/ (calibrated against a copy kernel of
+    known size, --copy-probe);
   - the reference CPU renderer (compiled from the reference's own sources)
-    on that 480x270 view, all threads used stated;
-  - hit-id parity of the GPU 480x270 frame against the CPU oracle.
+    on the same full frame, all threads used stated;
+  - hit-id parity of the GPU frame against the CPU oracle (every pixel).
 """
 from __future__ import annotations
 
@@ -85,8 +88,38 @@ def cpu_baseline(blob, view, threads, frames=1):
         _, secs, _ = ck.oracle_render_timed(blob, view, threads, frames)
     t = float(np.median(secs))
     return dict(value=rays / t / 1e6, unit="Mrays/s", cores=threads, kind=kind,
-                sample=f"{int(view['width'][0])}x{int(view['height'][0])} Beauty view of the same scene, "
+                sample=f"the same {int(view['width'][0])}x{int(view['height'][0])} Beauty frame, "
                        f"{threads} threads, median of {frames} frame(s), {rays} rays, {t:.2f} s/frame"), stats, hits
+
+
+TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "k_render_pbr200k_1920x1080_traffic.txt")
+
+
+def measured_traffic():
+    """HBM bytes per render launch from the committed rocprofv3 PMC summary
+    (line 'traffic_bytes_per_launch: N'), or None when absent."""
+    try:
+        with open(TRAFFIC_SUMMARY) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
+def copy_probe(mib=512, reps=5):
+    """A plain device copy of known size (elementwise negation, reads and
+    writes `mib` MiB per launch) for calibrating rocprofv3's FETCH_SIZE and
+    WRITE_SIZE counters."""
+    import torch
+    n = mib * 2**20 // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for i in range(reps):
+        torch.neg(a, out=b)
+        torch.cuda.synchronize()
+        print(f"copy probe {i}: {mib} MiB read, {mib} MiB written", flush=True)
 
 
 def main():
@@ -97,7 +130,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and oracle legs")
     ap.add_argument("--no-batch", action="store_true", help="skip the plain-vs-batched leg")
     ap.add_argument("--cpu-frames", type=int, default=3, help="frames of the CPU baseline (median)")
+    ap.add_argument("--copy-probe", action="store_true", help="only run the copy kernel used to calibrate PMC bytes")
     args = ap.parse_args()
+    if args.copy_probe:
+        copy_probe()
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU: every box in this setup has a single MI355X, so --gpus {args.gpus} "
               "is not supported (the multi-GPU design routes whole requests to the GPU that holds their scene; "
@@ -195,17 +232,18 @@ def main():
 
     if not args.no_cpu:
         threads = min(os.cpu_count() or 1, 16)
-        sample = pbr_view(480, 270)
-        cpu, stats, ohits = cpu_baseline(blob, sample, threads, frames=args.cpu_frames)
+        cpu, stats, ohits = cpu_baseline(blob, view, threads, frames=args.cpu_frames)
         result["cpu_baseline"] = cpu
         log("cpu baseline:", cpu)
-        sample_rays = stats.primary_rays + stats.shadow_rays + stats.bounce_rays
-        bytes_per_ray = stats.algorithmic_bytes() / sample_rays
-        achieved = bytes_per_ray * rays_per_frame / (kernel_ms * 1e-3) / 1e9
+        oracle_rays = stats.primary_rays + stats.shadow_rays + stats.bounce_rays
+        assert oracle_rays == int(rays_per_frame), (oracle_rays, rays_per_frame)
+        frame_bytes = stats.algorithmic_bytes()
+        achieved = frame_bytes / (kernel_ms * 1e-3) / 1e9
         result["roofline"] = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                              "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
-                              "bytes_per_ray": round(bytes_per_ray, 2)}
-        gp, gh, gc = st.render(scene, sample)
+                              "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": measured_traffic(),
+                              "algorithmic_bytes_per_launch": int(frame_bytes),
+                              "bytes_per_ray": round(frame_bytes / oracle_rays, 2)}
+        gp, gh, gc = st.render(scene, view)
         same = bool((gh["prim_id"] == ohits["prim_id"]).all() and (gh["triangle_id"] == ohits["triangle_id"]).all()
                     and (gh["distance"].view(np.uint32) == ohits["distance"].view(np.uint32)).all())
         result["hit_ids_exact"] = same

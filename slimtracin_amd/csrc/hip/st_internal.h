@@ -1,0 +1,38 @@
+/*
+ * st_internal.h - what the library's translation units share: the resident
+ * scene handle and the launch entry over prepared host views.
+ */
+#ifndef ST_INTERNAL_H
+#define ST_INTERNAL_H
+
+#include <hip/hip_runtime.h>
+
+#include "../host/st_host.h"
+#include "slimtracin.h"
+
+struct StSeamState;
+
+struct StSceneHandle {
+    StScene *host;
+    StDScene dev;
+    char *dbuf;
+    uint64_t dbytes;
+    float *d_mesh_aabbs;       /* 6 floats per mesh, for device BVH rebuilds */
+    char *lbvh_work;           /* scratch of device LBVH rebuilds */
+    uint32_t *d_height;        /* height written by the device builder */
+    int bvh_on_device_only;    /* host copy of the scene BVH is stale */
+    struct StSeamState *seam;  /* host copies of a scene fed through slimtracin_seam.h */
+};
+
+int st_rt_ensure_init(void);
+hipStream_t st_rt_stream(void);
+/* Record `msg` as the last error and return `code`. */
+int st_rt_error(int code, const char *msg);
+
+int st_rt_launch_prepared(StSceneHandle *const *scenes, StHostView *hv, uint32_t n, int want_hits, int persistent,
+                          StRayCounts *counts, StFloatPixel *const *pixels_host, StHitRec *const *hits_host,
+                          float *kernel_ms);
+
+void st_seam_state_free(struct StSeamState *s);
+
+#endif

@@ -27,10 +27,13 @@
 #include <string.h>
 #include <time.h>
 
+#include <vector>
+
 #include "st_trace.h"
 #include "st_lbvh.h"
 #include "../host/st_host.h"
 #include "slimtracin.h"
+#include "st_internal.h"
 
 #define TILE_W 8
 #define TILE_H 8
@@ -234,6 +237,10 @@ extern "C" int st_init(int device) {
     return ST_OK;
 }
 
+int st_rt_ensure_init(void) { return ensure_init(); }
+hipStream_t st_rt_stream(void) { return g.stream; }
+int st_rt_error(int code, const char *msg) { return set_err(code, "%s", msg); }
+
 static int arena_reserve(Arena *a, size_t bytes) {
     if (a->cap >= bytes) { a->used = 0; return ST_OK; }
     if (a->base) HIP_TRY(hipFree(a->base));
@@ -265,17 +272,6 @@ static int staging_reserve(size_t bytes) {
 
 /* ---- resident scenes ---------------------------------------------------- */
 
-struct StSceneHandle {
-    StScene *host;
-    StDScene dev;
-    char *dbuf;
-    uint64_t dbytes;
-    float *d_mesh_aabbs;      /* 6 floats per mesh, for device BVH rebuilds */
-    char *lbvh_work;          /* scratch of device LBVH rebuilds */
-    uint32_t *d_height;       /* height written by the device builder */
-    int bvh_on_device_only;   /* host copy of the scene BVH is stale */
-};
-
 extern "C" int st_scene_create(const void *blob, size_t size, StSceneHandle **out) {
     *out = NULL;
     int rc = ensure_init();
@@ -295,6 +291,7 @@ extern "C" int st_scene_create(const void *blob, size_t size, StSceneHandle **ou
    boundary: node 0 (the root) sits alone and every child pair (2k+1, 2k+2)
    covers exactly one 64-byte segment. */
 extern "C" int st_scene_upload(StSceneHandle *h) {
+    if (!h || !h->host) return set_err(ST_ERR_ARG, "scene has nothing to upload");
     StScene *sc = h->host;
     size_t o_prims = 0;
     size_t o_mats = o_prims + round256((sc->prim_count + 1) * sizeof(StDPrim));
@@ -377,12 +374,15 @@ extern "C" void st_scene_destroy(StSceneHandle *h) {
     if (h->dbuf) (void)hipFree(h->dbuf);
     if (h->lbvh_work) (void)hipFree(h->lbvh_work);
     if (h->d_height) (void)hipFree(h->d_height);
+    st_seam_state_free(h->seam);
     st_scene_free(h->host);
     free(h);
 }
 
 extern "C" int st_scene_info(const StSceneHandle *h, StSceneInfo *info) {
+    if (!h || !info) return set_err(ST_ERR_ARG, "null argument");
     memset(info, 0, sizeof *info);
+    if (!h->host) return set_err(ST_ERR_ARG, "scene is not resident yet");
     const StScene *sc = h->host;
     info->prim_count = sc->prim_count;
     info->mesh_count = sc->mesh_count;
@@ -398,6 +398,8 @@ extern "C" int st_scene_info(const StSceneHandle *h, StSceneInfo *info) {
 
 extern "C" int st_scene_bvh(const StSceneHandle *h, StBVHNodeRec *nodes, uint32_t *leaf_ids, uint32_t *node_count,
                             uint32_t *height) {
+    if (!h || !nodes || !leaf_ids || !node_count || !height) return set_err(ST_ERR_ARG, "null argument");
+    if (!h->host) return set_err(ST_ERR_ARG, "scene is not resident yet");
     const StScene *sc = h->host;
     if (h->bvh_on_device_only) {
         HIP_TRY(hipStreamSynchronize(g.stream));
@@ -417,6 +419,7 @@ extern "C" int st_scene_bvh(const StSceneHandle *h, StBVHNodeRec *nodes, uint32_
 
 extern "C" int st_scene_set_prims(StSceneHandle *h, const StPrimRec *recs, uint32_t first, uint32_t count) {
     if (!h || !recs) return set_err(ST_ERR_ARG, "null argument");
+    if (!h->host || !h->dbuf) return set_err(ST_ERR_ARG, "scene is not resident yet");
     StScene *sc = h->host;
     if ((uint64_t)first + count > sc->prim_count) return set_err(ST_ERR_ARG, "primitive range out of bounds");
     for (uint32_t i = 0; i < count; i++) {
@@ -445,6 +448,7 @@ extern "C" int st_scene_rebuild(StSceneHandle *h, uint32_t mode, float *ms) {
     if (!h) return set_err(ST_ERR_ARG, "null scene");
     int rc = ensure_init();
     if (rc) return rc;
+    if (!h->host || !h->dbuf) return set_err(ST_ERR_ARG, "scene is not resident yet");
     StScene *sc = h->host;
     uint32_t P = sc->prim_count;
     if (!P) return ST_OK;
@@ -496,32 +500,27 @@ struct LaunchPlan {
     size_t work_bytes, stage_bytes;
 };
 
-/* Build, upload and run one launch over `n` requests. pixels_dev/hits_dev
-   receive device pointers of each request's outputs inside the work arena. */
-static int launch(StSceneHandle *const *scenes, const StView *views, uint32_t n, int want_hits, int persistent,
-                  StRayCounts *counts, StFloatPixel *const *pixels_host, StHitRec *const *hits_host,
-                  float *kernel_ms) {
+/* Upload and run one launch over `n` requests whose host views are already
+   prepared (camera basis, row starts, SSB rects). Each request's frame (and
+   hit records) is copied to pixels_host[i] / hits_host[i] when given. */
+int st_rt_launch_prepared(StSceneHandle *const *scenes, StHostView *hv, uint32_t n, int want_hits, int persistent,
+                          StRayCounts *counts, StFloatPixel *const *pixels_host, StHitRec *const *hits_host,
+                          float *kernel_ms) {
     int rc = ensure_init();
     if (rc) return rc;
     if (n == 0) return set_err(ST_ERR_ARG, "no requests");
-    StHostView *hv = (StHostView *)calloc(n, sizeof(StHostView));
-    uint32_t *scene_idx = (uint32_t *)calloc(n, sizeof(uint32_t));
-    StSceneHandle **uniq = (StSceneHandle **)calloc(n, sizeof(void *));
+    std::vector<uint32_t> scene_idx(n);
+    std::vector<StSceneHandle *> uniq(n);
     uint32_t nuniq = 0;
     size_t work = 0, stage = 0;
     uint32_t rows = 0, tiles = 0;
     for (uint32_t i = 0; i < n; i++) {
         StSceneHandle *s = scenes[i];
-        if (!s) { rc = set_err(ST_ERR_ARG, "request %u has no scene", i); goto done; }
+        if (!s || !s->host || !s->dbuf) return set_err(ST_ERR_ARG, "request %u has no resident scene", i);
         uint32_t k = 0;
         while (k < nuniq && uniq[k] != s) k++;
         if (k == nuniq) uniq[nuniq++] = s;
         scene_idx[i] = k;
-        if (st_view_prepare(s->host, views + i, hv + i)) {
-            rc = set_err(ST_ERR_ARG, "request %u: invalid view (%ux%u, mode %u)", i, views[i].width, views[i].height,
-                         views[i].render_mode);
-            goto done;
-        }
         size_t px = (size_t)hv[i].dv.width1 * hv[i].dv.height1;
         work += round256(px * 16) + round256(px * 24) + (want_hits ? round256(px * 16) : 0);
         work += round256(hv[i].dv.height1 * 16ull) + round256((s->host->prim_count + 1) * 16ull) + 256;
@@ -534,8 +533,8 @@ static int launch(StSceneHandle *const *scenes, const StView *views, uint32_t n,
     }
     work += round256(n * sizeof(KReq)) + round256(nuniq * sizeof(StDScene)) + round256(n * 24ull);
     stage += round256(n * sizeof(KReq)) + round256(nuniq * sizeof(StDScene));
-    if ((rc = arena_reserve(&g.work, work))) goto done;
-    if ((rc = staging_reserve(stage))) goto done;
+    if ((rc = arena_reserve(&g.work, work))) return rc;
+    if ((rc = staging_reserve(stage))) return rc;
     {
         char *st = g.staging;
         size_t so = 0;
@@ -605,7 +604,7 @@ static int launch(StSceneHandle *const *scenes, const StView *views, uint32_t n,
             if (sc_h[k].mesh_stack_size > m_entries) m_entries = sc_h[k].mesh_stack_size;
         }
         size_t lds = (size_t)BLOCK_THREADS * (s_entries + m_entries) * sizeof(uint32_t);
-        if (lds > 160 * 1024) { rc = set_err(ST_ERR_SCENE, "traversal stacks need %zu B of LDS", lds); goto done; }
+        if (lds > 160 * 1024) return set_err(ST_ERR_SCENE, "traversal stacks need %zu B of LDS", lds);
         hipLaunchKernelGGL(k_render, dim3(blocks), dim3(BLOCK_THREADS), lds, g.stream, (const StDScene *)sc_d,
                            (const KReq *)reqs_d, n, tiles, g.tile_counter, (uint32_t)persistent, xcd_blocks,
                            s_entries, m_entries);
@@ -618,8 +617,8 @@ static int launch(StSceneHandle *const *scenes, const StView *views, uint32_t n,
             if (hits_host && hits_host[i] && want_hits)
                 HIP_TRY(hipMemcpyAsync(hits_host[i], reqs_h[i].v.hits, px * 16, hipMemcpyDeviceToHost, g.stream));
         }
-        unsigned long long *cnt_h = (unsigned long long *)calloc(3 * n, 8);
-        HIP_TRY(hipMemcpyAsync(cnt_h, cnt_d, 24ull * n, hipMemcpyDeviceToHost, g.stream));
+        std::vector<unsigned long long> cnt_h(3 * (size_t)n);
+        HIP_TRY(hipMemcpyAsync(cnt_h.data(), cnt_d, 24ull * n, hipMemcpyDeviceToHost, g.stream));
         HIP_TRY(hipStreamSynchronize(g.stream));
         if (counts)
             for (uint32_t i = 0; i < n; i++) {
@@ -627,14 +626,28 @@ static int launch(StSceneHandle *const *scenes, const StView *views, uint32_t n,
                 counts[i].shadow = cnt_h[3 * i + 1];
                 counts[i].bounce = cnt_h[3 * i + 2];
             }
-        free(cnt_h);
         if (kernel_ms) HIP_TRY(hipEventElapsedTime(kernel_ms, g.ev[0], g.ev[1]));
     }
-done:
+    return rc;
+}
+
+static int launch(StSceneHandle *const *scenes, const StView *views, uint32_t n, int want_hits, int persistent,
+                  StRayCounts *counts, StFloatPixel *const *pixels_host, StHitRec *const *hits_host,
+                  float *kernel_ms) {
+    int rc = ensure_init();
+    if (rc) return rc;
+    if (n == 0) return set_err(ST_ERR_ARG, "no requests");
+    StHostView *hv = (StHostView *)calloc(n, sizeof(StHostView));
+    if (!hv) return set_err(ST_ERR_MEMORY, "out of host memory");
+    for (uint32_t i = 0; i < n && !rc; i++) {
+        if (!scenes[i] || !scenes[i]->host) rc = set_err(ST_ERR_ARG, "request %u has no resident scene", i);
+        else if (st_view_prepare(scenes[i]->host, views + i, hv + i))
+            rc = set_err(ST_ERR_ARG, "request %u: invalid view (%ux%u, mode %u)", i, views[i].width, views[i].height,
+                         views[i].render_mode);
+    }
+    if (!rc) rc = st_rt_launch_prepared(scenes, hv, n, want_hits, persistent, counts, pixels_host, hits_host, kernel_ms);
     for (uint32_t i = 0; i < n; i++) st_view_release(hv + i);
     free(hv);
-    free(scene_idx);
-    free(uniq);
     return rc;
 }
 

@@ -15,6 +15,7 @@
 #include <stdint.h>
 
 #include "st_blob.h"
+#include "slimtracin_seam.h"
 #include "../st_layout.h"
 #include "../st_math.h"
 
@@ -54,9 +55,51 @@ typedef struct StScene {
     uint32_t scene_real_height;    /* levels of the scene BVH as walked */
 } StScene;
 
-/* Parse a blob and build its BVHs. Returns NULL on a malformed blob; *err
-   (if given) receives a short reason. */
+/* A scene as separate arrays in the blob record layouts (st_blob.h); the
+   pointers are borrowed. Meshes with rec.prebuilt set carry their BVH and
+   Triangle records (nodes, tris) and may omit the geometry (pos/pidx NULL):
+   such a mesh is device-ready, as uploadMeshBVHs hands it over. */
+typedef struct StMeshPart {
+    StMeshRec rec;
+    const float *pos, *nrm, *uvs;
+    const uint32_t *pidx, *nidx, *uidx;
+    const StBVHNodeRec *nodes;
+    const StTriangleRec *tris;
+} StMeshPart;
+
+typedef struct StMipPart {
+    uint16_t width, height;
+    const uint8_t *quads;          /* (width+1)*(height+1) texel quads of 12 B */
+} StMipPart;
+
+typedef struct StTexturePart {
+    StTextureRec rec;
+    const StMipPart *mips;         /* rec.mip_count entries */
+} StTexturePart;
+
+typedef struct StSceneParts {
+    uint32_t prim_count, material_count, light_count, mesh_count, texture_count;
+    float ambient[3];
+    const StPrimRec *prims;
+    const StMaterialRec *mats;
+    const StLightRec *lights;
+    const StMeshPart *meshes;
+    const StTexturePart *textures;
+    /* optional scene BVH built elsewhere; NULL = SAH build over the primitives */
+    const StBVHNodeRec *scene_nodes;
+    uint32_t scene_node_count, scene_height;
+    const uint32_t *scene_leaf_ids;
+} StSceneParts;
+
+/* Validate the parts and build the device-ready scene. Returns NULL on
+   malformed input; *err (if given) receives a short reason. */
+StScene *st_scene_from_parts(const StSceneParts *parts, const char **err);
+
+/* Parse a blob into parts and build its BVHs (st_scene_from_parts). */
 StScene *st_scene_from_blob(const void *blob, size_t size, const char **err);
+
+/* Device material record of a blob material record. */
+StDMat st_mat_from_rec(const StMaterialRec *q);
 void st_scene_free(StScene *sc);
 
 /* Device bytes the resident scene occupies (for HBM budgeting). */
@@ -99,6 +142,11 @@ typedef struct StHostView {
 } StHostView;
 
 int st_view_prepare(const StScene *sc, const StView *v, StHostView *hv);
+
+/* Set-up for a frame described the way renderSceneOnGPU receives it: the
+   viewport's projection plane and camera, and per-primitive screen bounds
+   already computed by the caller (gated by each primitive's IS_VISIBLE). */
+int st_view_prepare_frame(const StScene *sc, const StSeamFrame *f, const int32_t *screen_bounds, StHostView *hv);
 void st_view_release(StHostView *hv);
 
 #ifdef __cplusplus

@@ -141,6 +141,207 @@ static void tri_to_device(const StTriangleRec *t, StDTriHot *h, StDTriCold *c) {
     c->uv_area = t->area_of_uv;
 }
 
+StDMat st_mat_from_rec(const StMaterialRec *q) {
+    StDMat m;
+    memset(&m, 0, sizeof m);
+    memcpy(m.albedo, q->albedo, 12);
+    memcpy(m.reflectivity, q->reflectivity, 12);
+    memcpy(m.emission, q->emission, 12);
+    memcpy(m.uv_repeat, q->uv_repeat, 8);
+    m.metallic = q->metallic;
+    m.roughness = q->roughness;
+    m.n1_over_n2 = q->n1_over_n2;
+    m.n2_over_n1 = q->n2_over_n1;
+    m.brdf = q->brdf;
+    m.bits = (uint32_t)q->is | ((uint32_t)q->use << 8) | ((uint32_t)q->texture_count << 16);
+    m.tex0 = q->texture_ids[0];
+    m.tex1 = q->texture_ids[1];
+    return m;
+}
+
+#define FAIL(msg) do { *err = (msg); st_scene_free(sc); return NULL; } while (0)
+
+StScene *st_scene_from_parts(const StSceneParts *pp, const char **err) {
+    const char *dummy;
+    if (!err) err = &dummy;
+    *err = NULL;
+    StScene *sc = (StScene *)calloc(1, sizeof(StScene));
+    if (!sc) { *err = "out of host memory"; return NULL; }
+    const uint32_t P = pp->prim_count;
+    sc->prim_count = P;
+    sc->material_count = pp->material_count;
+    sc->light_count = pp->light_count;
+    sc->mesh_count = pp->mesh_count;
+    sc->texture_count = pp->texture_count;
+    memcpy(sc->ambient, pp->ambient, 12);
+    if ((P && !pp->prims) || (pp->material_count && !pp->mats) || (pp->light_count && !pp->lights) ||
+        (pp->mesh_count && !pp->meshes) || (pp->texture_count && !pp->textures))
+        FAIL("missing scene array");
+
+    sc->prims = (StDPrim *)calloc((size_t)P + 1, sizeof(StDPrim));
+    sc->mats = (StDMat *)calloc((size_t)pp->material_count + 1, sizeof(StDMat));
+    sc->lights = (StDLight *)calloc((size_t)pp->light_count + 1, sizeof(StDLight));
+    sc->meshes = (StDMesh *)calloc((size_t)pp->mesh_count + 1, sizeof(StDMesh));
+    sc->mesh_aabbs = (sbox *)calloc((size_t)pp->mesh_count + 1, sizeof(sbox));
+    sc->textures = (StDTex *)calloc((size_t)pp->texture_count + 1, sizeof(StDTex));
+    if (!sc->prims || !sc->mats || !sc->lights || !sc->meshes || !sc->mesh_aabbs || !sc->textures)
+        FAIL("out of host memory");
+    for (uint32_t i = 0; i < P; i++) {
+        const StPrimRec *r = pp->prims + i;
+        if (r->material_id >= pp->material_count || r->material_id > 255) FAIL("primitive material_id out of range");
+        if (r->type == ST_PRIM_MESH && r->mesh_id >= pp->mesh_count) FAIL("primitive mesh_id out of range");
+        sc->prims[i] = st_prim_from_rec(r);
+    }
+    for (uint32_t i = 0; i < pp->material_count; i++) {
+        const StMaterialRec *q = pp->mats + i;
+        sc->mats[i] = st_mat_from_rec(q);
+        if (q->texture_count && q->use) {
+            uint32_t need = q->texture_count > 1 ? 2 : 1;
+            for (uint32_t t = 0; t < need; t++)
+                if (q->texture_ids[t] >= pp->texture_count) FAIL("material texture id out of range");
+        }
+    }
+    for (uint32_t i = 0; i < pp->light_count; i++) {
+        memcpy(sc->lights[i].pos, pp->lights[i].position, 12);
+        sc->lights[i].intensity = pp->lights[i].intensity;
+        memcpy(sc->lights[i].color, pp->lights[i].color, 12);
+    }
+
+    /* meshes: offsets first, so node/triangle totals are known */
+    uint64_t node_total = 1, tri_total = 0;
+    for (uint32_t m = 0; m < pp->mesh_count; m++) {
+        const StMeshPart *s = pp->meshes + m;
+        const StMeshRec *q = &s->rec;
+        if (!q->triangle_count) FAIL("mesh without triangles");
+        if (q->prebuilt) {
+            if (!s->nodes || !s->tris || !q->node_count) FAIL("prebuilt mesh without BVH or triangles");
+        } else if (!s->pos || !s->pidx) {
+            FAIL("mesh without geometry");
+        }
+        if (s->pidx)
+            for (uint64_t t = 0; t < 3ull * q->triangle_count; t++)
+                if (s->pidx[t] >= q->vertex_count || (s->nidx && s->nidx[t] >= q->normals_count) ||
+                    (s->uidx && s->uidx[t] >= q->uvs_count))
+                    FAIL("mesh index out of range");
+        uint64_t nodes = q->prebuilt ? q->node_count : 2ull * q->triangle_count;
+        if (!(node_total & 1)) node_total++;   /* odd offset: child pairs start 64-byte aligned */
+        sc->meshes[m].node_offset = (uint32_t)node_total;
+        node_total += nodes;
+        sc->meshes[m].tri_offset = (uint32_t)tri_total;
+        sc->meshes[m].triangle_count = q->triangle_count;
+        sc->meshes[m].attrs = (q->normals_count ? 1u : 0u) | (q->uvs_count ? 2u : 0u);
+        tri_total += q->triangle_count;
+        sc->mesh_aabbs[m].min = V3(q->aabb[0], q->aabb[1], q->aabb[2]);
+        sc->mesh_aabbs[m].max = V3(q->aabb[3], q->aabb[4], q->aabb[5]);
+        if (node_total > 0xFFFFFFF0ull || tri_total > 0xFFFFFFF0ull) FAIL("scene too large");
+    }
+    sc->mesh_node_total = (uint32_t)node_total;
+    sc->tri_total = (uint32_t)tri_total;
+    sc->mesh_nodes = (StBVHNodeRec *)calloc(node_total + 1, sizeof(StBVHNodeRec));
+    sc->tri_hot = (StDTriHot *)calloc(tri_total + 1, sizeof(StDTriHot));
+    sc->tri_cold = (StDTriCold *)calloc(tri_total + 1, sizeof(StDTriCold));
+    if (!sc->mesh_nodes || !sc->tri_hot || !sc->tri_cold) FAIL("out of host memory");
+    for (uint32_t m = 0; m < pp->mesh_count; m++) {
+        const StMeshPart *s = pp->meshes + m;
+        const StMeshRec *q = &s->rec;
+        StDMesh *dm = sc->meshes + m;
+        uint32_t height = 0, count = 0;
+        if (q->prebuilt) {
+            memcpy(sc->mesh_nodes + dm->node_offset, s->nodes, sizeof(StBVHNodeRec) * q->node_count);
+            count = q->node_count;
+            height = q->height;
+            for (uint32_t t = 0; t < q->triangle_count; t++)
+                tri_to_device(s->tris + t, sc->tri_hot + dm->tri_offset + t, sc->tri_cold + dm->tri_offset + t);
+        } else {
+            StTriangleRec *tris = (StTriangleRec *)malloc(sizeof(StTriangleRec) * q->triangle_count);
+            uint32_t *leaf = (uint32_t *)malloc(4u * (size_t)q->triangle_count);
+            if (!tris || !leaf || st_build_mesh(q->triangle_count, s->pos, s->pidx, s->nrm, s->nidx, q->normals_count,
+                                                s->uvs, s->uidx, q->uvs_count, sc->mesh_nodes + dm->node_offset, &count,
+                                                &height, tris, leaf)) {
+                free(tris); free(leaf);
+                FAIL("mesh BVH build failed");
+            }
+            for (uint32_t t = 0; t < q->triangle_count; t++)
+                tri_to_device(tris + t, sc->tri_hot + dm->tri_offset + t, sc->tri_cold + dm->tri_offset + t);
+            free(tris);
+            free(leaf);
+        }
+        dm->height = height;
+        if (height > sc->max_mesh_height) sc->max_mesh_height = height;
+        if (st_bvh_validate(sc->mesh_nodes + dm->node_offset, count, q->triangle_count) < 0) FAIL("malformed mesh BVH");
+    }
+    if (pp->mesh_count) {
+        sc->mesh_stack_size = (uint8_t)(sc->max_mesh_height + 2); /* core/init.h:523-527 */
+        if (sc->max_mesh_height + 2 > ST_MESH_STACK_MAX) FAIL("mesh BVH deeper than the device traversal stack");
+    }
+
+    /* textures */
+    uint64_t quads = 0, mips = 0;
+    for (uint32_t t = 0; t < pp->texture_count; t++) {
+        const StTexturePart *q = pp->textures + t;
+        if (!q->rec.mip_count || !q->mips) FAIL("texture without mips");
+        for (uint32_t i = 0; i < q->rec.mip_count; i++) {
+            if (!q->mips[i].quads) FAIL("texture mip without texels");
+            quads += (uint64_t)(q->mips[i].width + 1) * (q->mips[i].height + 1);
+            mips++;
+        }
+    }
+    sc->mips = (StDMip *)calloc(mips + 1, sizeof(StDMip));
+    sc->texel_bytes = 12 * quads;
+    sc->texels = (uint8_t *)malloc(sc->texel_bytes + 16);
+    if (!sc->mips || !sc->texels) FAIL("out of host memory");
+    sc->mip_total = (uint32_t)mips;
+    uint64_t qoff = 0;
+    uint32_t mip_index = 0;
+    for (uint32_t t = 0; t < pp->texture_count; t++) {
+        const StTexturePart *q = pp->textures + t;
+        StDTex *dt = sc->textures + t;
+        dt->width = q->rec.width;
+        dt->height = q->rec.height;
+        dt->mip_count = q->rec.mip_count;
+        dt->mipmap = q->rec.mipmap;
+        dt->mip_base = mip_index;
+        for (uint32_t i = 0; i < q->rec.mip_count; i++) {
+            uint64_t nq = (uint64_t)(q->mips[i].width + 1) * (q->mips[i].height + 1);
+            sc->mips[mip_index].width = q->mips[i].width;
+            sc->mips[mip_index].height = q->mips[i].height;
+            sc->mips[mip_index].quad_offset = qoff;
+            memcpy(sc->texels + 12 * qoff, q->mips[i].quads, 12 * nq);
+            qoff += nq;
+            mip_index++;
+        }
+    }
+
+    /* scene BVH */
+    sc->scene_nodes = (StBVHNodeRec *)calloc(2 * (size_t)P + 2, sizeof(StBVHNodeRec));
+    sc->scene_leaf_ids = (uint32_t *)calloc((size_t)P + 1, sizeof(uint32_t));
+    if (!sc->scene_nodes || !sc->scene_leaf_ids) FAIL("out of host memory");
+    if (pp->scene_nodes) {
+        if (!pp->scene_leaf_ids || pp->scene_node_count > 2 * (size_t)P + 1) FAIL("bad scene BVH");
+        memcpy(sc->scene_nodes, pp->scene_nodes, sizeof(StBVHNodeRec) * pp->scene_node_count);
+        memcpy(sc->scene_leaf_ids, pp->scene_leaf_ids, 4u * P);
+        sc->scene_node_count = pp->scene_node_count;
+        sc->scene_height = pp->scene_height;
+        for (uint32_t i = 0; i < P; i++)
+            if (pp->scene_leaf_ids[i] >= P) FAIL("scene BVH leaf id out of range");
+    } else if (P) {
+        if (st_scene_rebuild_sah(sc)) FAIL("scene BVH build failed");
+    }
+    if (P) {
+        int real_height = st_bvh_validate(sc->scene_nodes, sc->scene_node_count, P);
+        if (real_height < 0) FAIL("malformed scene BVH");
+        sc->scene_real_height = (uint32_t)real_height;
+        if (real_height + 1 > ST_SCENE_STACK_MAX) FAIL("scene BVH deeper than the device traversal stack");
+    }
+    for (uint32_t i = 0; i < P; i++) {
+        uint32_t mid = (sc->prims[i].flags_mat >> 8) & 0xFFu;
+        if (sc->prims[i].type == ST_PRIM_QUAD && (sc->mats[mid].bits & ST_EMISSIVE)) sc->has_emissive_quads = 1;
+    }
+    return sc;
+}
+
+#undef FAIL
+
 StScene *st_scene_from_blob(const void *blob, size_t size, const char **err) {
     const char *dummy;
     if (!err) err = &dummy;
@@ -148,77 +349,29 @@ StScene *st_scene_from_blob(const void *blob, size_t size, const char **err) {
     Rd r = {(const uint8_t *)blob, (const uint8_t *)blob + size, 0};
     const StBlobHeader *h = (const StBlobHeader *)take(&r, sizeof *h);
     if (!h || h->magic != ST_BLOB_MAGIC || h->version != ST_BLOB_VERSION) { *err = "bad blob header"; return NULL; }
-    StScene *sc = (StScene *)calloc(1, sizeof(StScene));
-    if (!sc) { *err = "out of host memory"; return NULL; }
-    sc->prim_count = h->prim_count;
-    sc->material_count = h->material_count;
-    sc->light_count = h->light_count;
-    sc->mesh_count = h->mesh_count;
-    sc->texture_count = h->texture_count;
-    memcpy(sc->ambient, h->ambient, 12);
-    const StPrimRec *pr = (const StPrimRec *)take(&r, sizeof(StPrimRec) * h->prim_count);
-    const StMaterialRec *mr = (const StMaterialRec *)take(&r, sizeof(StMaterialRec) * h->material_count);
-    const StLightRec *lr = (const StLightRec *)take(&r, sizeof(StLightRec) * h->light_count);
-    if (r.bad) { *err = "truncated blob"; st_scene_free(sc); return NULL; }
+    StSceneParts pp;
+    memset(&pp, 0, sizeof pp);
+    pp.prim_count = h->prim_count;
+    pp.material_count = h->material_count;
+    pp.light_count = h->light_count;
+    pp.mesh_count = h->mesh_count;
+    pp.texture_count = h->texture_count;
+    memcpy(pp.ambient, h->ambient, 12);
+    pp.prims = (const StPrimRec *)take(&r, sizeof(StPrimRec) * h->prim_count);
+    pp.mats = (const StMaterialRec *)take(&r, sizeof(StMaterialRec) * h->material_count);
+    pp.lights = (const StLightRec *)take(&r, sizeof(StLightRec) * h->light_count);
+    if (r.bad) { *err = "truncated blob"; return NULL; }
 
-    sc->prims = (StDPrim *)calloc(h->prim_count + 1, sizeof(StDPrim));
-    for (uint32_t i = 0; i < h->prim_count; i++) {
-        if (pr[i].material_id >= h->material_count || pr[i].material_id > 255) {
-            *err = "primitive material_id out of range"; st_scene_free(sc); return NULL;
-        }
-        if (pr[i].type == ST_PRIM_MESH && pr[i].mesh_id >= h->mesh_count) {
-            *err = "primitive mesh_id out of range"; st_scene_free(sc); return NULL;
-        }
-        sc->prims[i] = st_prim_from_rec(pr + i);
-    }
-    sc->mats = (StDMat *)calloc(h->material_count + 1, sizeof(StDMat));
-    for (uint32_t i = 0; i < h->material_count; i++) {
-        const StMaterialRec *q = mr + i;
-        StDMat *m = sc->mats + i;
-        memcpy(m->albedo, q->albedo, 12);
-        memcpy(m->reflectivity, q->reflectivity, 12);
-        memcpy(m->emission, q->emission, 12);
-        memcpy(m->uv_repeat, q->uv_repeat, 8);
-        m->metallic = q->metallic;
-        m->roughness = q->roughness;
-        m->n1_over_n2 = q->n1_over_n2;
-        m->n2_over_n1 = q->n2_over_n1;
-        m->brdf = q->brdf;
-        m->bits = (uint32_t)q->is | ((uint32_t)q->use << 8) | ((uint32_t)q->texture_count << 16);
-        m->tex0 = q->texture_ids[0];
-        m->tex1 = q->texture_ids[1];
-        if (q->texture_count && q->use) {
-            uint32_t need = q->texture_count > 1 ? 2 : 1;
-            for (uint32_t t = 0; t < need; t++)
-                if (q->texture_ids[t] >= h->texture_count) {
-                    *err = "material texture id out of range"; st_scene_free(sc); return NULL;
-                }
-        }
-    }
-    sc->lights = (StDLight *)calloc(h->light_count + 1, sizeof(StDLight));
-    for (uint32_t i = 0; i < h->light_count; i++) {
-        memcpy(sc->lights[i].pos, lr[i].position, 12);
-        sc->lights[i].intensity = lr[i].intensity;
-        memcpy(sc->lights[i].color, lr[i].color, 12);
-    }
-
-    /* meshes: first pass collects records so node/triangle totals are known */
-    sc->meshes = (StDMesh *)calloc(h->mesh_count + 1, sizeof(StDMesh));
-    sc->mesh_aabbs = (sbox *)calloc(h->mesh_count + 1, sizeof(sbox));
-    typedef struct {
-        const StMeshRec *rec;
-        const float *pos, *nrm, *uvs;
-        const uint32_t *pidx, *nidx, *uidx;
-        const StBVHNodeRec *nodes;
-        const StTriangleRec *tris;
-    } MeshSrc;
-    MeshSrc *src = (MeshSrc *)calloc(h->mesh_count + 1, sizeof(MeshSrc));
-    uint64_t node_total = 1, tri_total = 0;
+    StMeshPart *meshes = (StMeshPart *)calloc((size_t)h->mesh_count + 1, sizeof(StMeshPart));
+    StTexturePart *texs = (StTexturePart *)calloc((size_t)h->texture_count + 1, sizeof(StTexturePart));
+    StMipPart **mip_arrays = (StMipPart **)calloc((size_t)h->texture_count + 1, sizeof(StMipPart *));
+    StScene *sc = NULL;
+    if (!meshes || !texs || !mip_arrays) { *err = "out of host memory"; goto out; }
     for (uint32_t m = 0; m < h->mesh_count; m++) {
-        MeshSrc *s = src + m;
-        s->rec = (const StMeshRec *)take(&r, sizeof(StMeshRec));
-        if (!s->rec) break;
-        const StMeshRec *q = s->rec;
+        StMeshPart *s = meshes + m;
+        const StMeshRec *q = (const StMeshRec *)take(&r, sizeof(StMeshRec));
+        if (!q) break;
+        s->rec = *q;
         s->pos = (const float *)take(&r, 12u * (size_t)q->vertex_count);
         s->pidx = (const uint32_t *)take(&r, 12u * (size_t)q->triangle_count);
         if (q->normals_count) {
@@ -235,157 +388,44 @@ StScene *st_scene_from_blob(const void *blob, size_t size, const char **err) {
             take(&r, 4u * (size_t)q->triangle_count);
         }
         if (r.bad) break;
-        if (!q->triangle_count) { *err = "mesh without triangles"; r.bad = 1; break; }
-        for (uint64_t t = 0; t < 3ull * q->triangle_count; t++) {
-            if (s->pidx[t] >= q->vertex_count || (s->nidx && s->nidx[t] >= q->normals_count) ||
-                (s->uidx && s->uidx[t] >= q->uvs_count)) {
-                *err = "mesh index out of range"; r.bad = 1; break;
-            }
-        }
-        if (r.bad) break;
-        uint64_t nodes = q->prebuilt ? q->node_count : 2ull * q->triangle_count;
-        if (!(node_total & 1)) node_total++;   /* odd offset: child pairs start 64-byte aligned */
-        sc->meshes[m].node_offset = (uint32_t)node_total;
-        node_total += nodes;
-        sc->meshes[m].tri_offset = (uint32_t)tri_total;
-        sc->meshes[m].triangle_count = q->triangle_count;
-        sc->meshes[m].attrs = (q->normals_count ? 1u : 0u) | (q->uvs_count ? 2u : 0u);
-        tri_total += q->triangle_count;
-        sc->mesh_aabbs[m].min = V3(q->aabb[0], q->aabb[1], q->aabb[2]);
-        sc->mesh_aabbs[m].max = V3(q->aabb[3], q->aabb[4], q->aabb[5]);
     }
-    if (r.bad) { if (!*err) *err = "truncated mesh section"; free(src); st_scene_free(sc); return NULL; }
-    if (node_total > 0xFFFFFFF0ull || tri_total > 0xFFFFFFF0ull) {
-        *err = "scene too large"; free(src); st_scene_free(sc); return NULL;
-    }
-    sc->mesh_node_total = (uint32_t)node_total;
-    sc->tri_total = (uint32_t)tri_total;
-    sc->mesh_nodes = (StBVHNodeRec *)calloc(node_total + 1, sizeof(StBVHNodeRec));
-    sc->tri_hot = (StDTriHot *)calloc(tri_total + 1, sizeof(StDTriHot));
-    sc->tri_cold = (StDTriCold *)calloc(tri_total + 1, sizeof(StDTriCold));
-    if (!sc->mesh_nodes || !sc->tri_hot || !sc->tri_cold) {
-        *err = "out of host memory"; free(src); st_scene_free(sc); return NULL;
-    }
-    for (uint32_t m = 0; m < h->mesh_count; m++) {
-        const MeshSrc *s = src + m;
-        const StMeshRec *q = s->rec;
-        StDMesh *dm = sc->meshes + m;
-        StTriangleRec *tris = NULL;
-        uint32_t height = 0, count = 0;
-        if (q->prebuilt) {
-            memcpy(sc->mesh_nodes + dm->node_offset, s->nodes, sizeof(StBVHNodeRec) * q->node_count);
-            count = q->node_count;
-            height = q->height;
-            for (uint32_t t = 0; t < q->triangle_count; t++)
-                tri_to_device(s->tris + t, sc->tri_hot + dm->tri_offset + t, sc->tri_cold + dm->tri_offset + t);
-        } else {
-            tris = (StTriangleRec *)malloc(sizeof(StTriangleRec) * q->triangle_count);
-            uint32_t *leaf = (uint32_t *)malloc(4u * (size_t)q->triangle_count);
-            if (!tris || !leaf || st_build_mesh(q->triangle_count, s->pos, s->pidx, s->nrm, s->nidx, q->normals_count,
-                                                s->uvs, s->uidx, q->uvs_count, sc->mesh_nodes + dm->node_offset, &count,
-                                                &height, tris, leaf)) {
-                free(tris); free(leaf);
-                *err = "mesh BVH build failed"; free(src); st_scene_free(sc); return NULL;
-            }
-            for (uint32_t t = 0; t < q->triangle_count; t++)
-                tri_to_device(tris + t, sc->tri_hot + dm->tri_offset + t, sc->tri_cold + dm->tri_offset + t);
-            free(tris);
-            free(leaf);
-        }
-        dm->height = height;
-        if (height > sc->max_mesh_height) sc->max_mesh_height = height;
-        if (st_bvh_validate(sc->mesh_nodes + dm->node_offset, count, q->triangle_count) < 0) {
-            *err = "malformed mesh BVH"; free(src); st_scene_free(sc); return NULL;
-        }
-    }
-    free(src);
-    if (sc->mesh_count) {
-        sc->mesh_stack_size = (uint8_t)(sc->max_mesh_height + 2); /* core/init.h:523-527 */
-        if (sc->max_mesh_height + 2 > ST_MESH_STACK_MAX) {
-            *err = "mesh BVH deeper than the device traversal stack"; st_scene_free(sc); return NULL;
-        }
-    }
-
-    /* textures */
-    sc->textures = (StDTex *)calloc(h->texture_count + 1, sizeof(StDTex));
-    const uint8_t *tex_start = r.p;
-    uint64_t quads = 0, mips = 0;
-    {
-        Rd probe = r;
-        for (uint32_t t = 0; t < h->texture_count && !probe.bad; t++) {
-            const StTextureRec *q = (const StTextureRec *)take(&probe, sizeof *q);
-            if (!q) break;
-            if (!q->mip_count) { probe.bad = 1; break; }
-            for (uint32_t i = 0; i < q->mip_count; i++) {
-                const uint16_t *d = (const uint16_t *)take(&probe, 4);
-                if (!d) break;
-                uint64_t nq = (uint64_t)(d[0] + 1) * (d[1] + 1);
-                take(&probe, 12 * nq);
-                quads += nq;
-                mips++;
-            }
-        }
-        if (probe.bad) { *err = "truncated texture section"; st_scene_free(sc); return NULL; }
-    }
-    (void)tex_start;
-    sc->mips = (StDMip *)calloc(mips + 1, sizeof(StDMip));
-    sc->texel_bytes = 12 * quads;
-    sc->texels = (uint8_t *)malloc(sc->texel_bytes + 16);
-    sc->mip_total = (uint32_t)mips;
-    uint64_t qoff = 0;
-    uint32_t mip_index = 0;
-    for (uint32_t t = 0; t < h->texture_count; t++) {
+    if (r.bad) { *err = "truncated mesh section"; goto out; }
+    for (uint32_t t = 0; t < h->texture_count && !r.bad; t++) {
         const StTextureRec *q = (const StTextureRec *)take(&r, sizeof *q);
-        StDTex *dt = sc->textures + t;
-        dt->width = q->width;
-        dt->height = q->height;
-        dt->mip_count = q->mip_count;
-        dt->mipmap = q->mipmap;
-        dt->mip_base = mip_index;
+        if (!q) break;
+        texs[t].rec = *q;
+        if (!q->mip_count) { r.bad = 1; break; }
+        StMipPart *mp = (StMipPart *)calloc(q->mip_count, sizeof(StMipPart));
+        if (!mp) { *err = "out of host memory"; goto out; }
+        mip_arrays[t] = mp;
+        texs[t].mips = mp;
         for (uint32_t i = 0; i < q->mip_count; i++) {
             const uint16_t *d = (const uint16_t *)take(&r, 4);
-            uint64_t nq = (uint64_t)(d[0] + 1) * (d[1] + 1);
-            const uint8_t *qs = (const uint8_t *)take(&r, 12 * nq);
-            sc->mips[mip_index].width = d[0];
-            sc->mips[mip_index].height = d[1];
-            sc->mips[mip_index].quad_offset = qoff;
-            memcpy(sc->texels + 12 * qoff, qs, 12 * nq);
-            qoff += nq;
-            mip_index++;
+            if (!d) break;
+            mp[i].width = d[0];
+            mp[i].height = d[1];
+            mp[i].quads = (const uint8_t *)take(&r, 12 * (size_t)(d[0] + 1) * (d[1] + 1));
+            if (!mp[i].quads) break;
         }
     }
-
-    /* scene BVH */
-    uint32_t P = h->prim_count;
-    sc->scene_nodes = (StBVHNodeRec *)calloc(2 * (size_t)P + 2, sizeof(StBVHNodeRec));
-    sc->scene_leaf_ids = (uint32_t *)calloc((size_t)P + 1, sizeof(uint32_t));
+    if (r.bad) { *err = "truncated texture section"; goto out; }
+    pp.meshes = meshes;
+    pp.textures = texs;
     if (h->flags & ST_BLOB_HAS_SCENE_BVH) {
         const uint32_t *cnt = (const uint32_t *)take(&r, 8);
-        if (!cnt || cnt[0] > 2 * (size_t)P + 1) { *err = "bad scene BVH section"; st_scene_free(sc); return NULL; }
-        const StBVHNodeRec *nodes = (const StBVHNodeRec *)take(&r, sizeof(StBVHNodeRec) * cnt[0]);
-        const uint32_t *leaf = (const uint32_t *)take(&r, 4u * P);
-        if (r.bad) { *err = "truncated scene BVH"; st_scene_free(sc); return NULL; }
-        memcpy(sc->scene_nodes, nodes, sizeof(StBVHNodeRec) * cnt[0]);
-        memcpy(sc->scene_leaf_ids, leaf, 4u * P);
-        sc->scene_node_count = cnt[0];
-        sc->scene_height = cnt[1];
-        for (uint32_t i = 0; i < P; i++)
-            if (leaf[i] >= P) { *err = "scene BVH leaf id out of range"; st_scene_free(sc); return NULL; }
-    } else if (P) {
-        st_scene_rebuild_sah(sc);
+        if (!cnt || cnt[0] > 2 * (size_t)h->prim_count + 1) { *err = "bad scene BVH section"; goto out; }
+        pp.scene_node_count = cnt[0];
+        pp.scene_height = cnt[1];
+        pp.scene_nodes = (const StBVHNodeRec *)take(&r, sizeof(StBVHNodeRec) * cnt[0]);
+        pp.scene_leaf_ids = (const uint32_t *)take(&r, 4u * (size_t)h->prim_count);
+        if (r.bad) { *err = "truncated scene BVH"; goto out; }
     }
-    if (P) {
-        int real_height = st_bvh_validate(sc->scene_nodes, sc->scene_node_count, P);
-        if (real_height < 0) { *err = "malformed scene BVH"; st_scene_free(sc); return NULL; }
-        sc->scene_real_height = (uint32_t)real_height;
-        if (real_height + 1 > ST_SCENE_STACK_MAX) {
-            *err = "scene BVH deeper than the device traversal stack"; st_scene_free(sc); return NULL;
-        }
-    }
-    for (uint32_t i = 0; i < P; i++) {
-        uint32_t mid = (sc->prims[i].flags_mat >> 8) & 0xFFu;
-        if (sc->prims[i].type == ST_PRIM_QUAD && (sc->mats[mid].bits & ST_EMISSIVE)) sc->has_emissive_quads = 1;
-    }
+    sc = st_scene_from_parts(&pp, err);
+out:
+    for (uint32_t t = 0; t < h->texture_count; t++) free(mip_arrays ? mip_arrays[t] : NULL);
+    free(mip_arrays);
+    free(texs);
+    free(meshes);
     return sc;
 }
 

@@ -216,6 +216,50 @@ int st_view_prepare(const StScene *sc, const StView *v, StHostView *hv) {
     return 0;
 }
 
+int st_view_prepare_frame(const StScene *sc, const StSeamFrame *f, const int32_t *screen_bounds, StHostView *hv) {
+    memset(hv, 0, sizeof *hv);
+    if (f->width == 0 || f->height == 0 || f->width == 0xFFFF || f->height == 0xFFFF) return -1;
+    if (f->render_mode > ST_MODE_UVS) return -1;
+    if (sc->prim_count && !screen_bounds) return -1;
+    hv->row_starts = (float *)malloc(4 * sizeof(float) * f->height);
+    hv->rects = (int32_t *)malloc(4 * sizeof(int32_t) * (sc->prim_count + 1));
+    hv->flags = (uint8_t *)malloc(sc->prim_count + 1);
+    hv->raw_rects = (int32_t *)malloc(4 * sizeof(int32_t) * (sc->prim_count + 1));
+    if (!hv->row_starts || !hv->rects || !hv->flags || !hv->raw_rects) { st_view_release(hv); return -2; }
+    /* start += down per row (render/raytracer.h:156-170) */
+    sv3 s = V3(f->start[0], f->start[1], f->start[2]);
+    const sv3 down = V3(f->down[0], f->down[1], f->down[2]);
+    for (uint32_t y = 0; y < f->height; y++) {
+        hv->row_starts[4 * y] = s.x;
+        hv->row_starts[4 * y + 1] = s.y;
+        hv->row_starts[4 * y + 2] = s.z;
+        hv->row_starts[4 * y + 3] = 0;
+        s = v_add(s, down);
+    }
+    for (uint32_t i = 0; i < sc->prim_count; i++) {
+        uint8_t flags = (uint8_t)(sc->prims[i].flags_mat & 0xFFu);
+        int32_t *rc = hv->rects + 4 * i;
+        memcpy(rc, screen_bounds + 4 * i, 16);
+        memcpy(hv->raw_rects + 4 * i, rc, 16);
+        hv->flags[i] = flags;
+        if (!(flags & ST_IS_VISIBLE)) { rc[0] = rc[1] = 1; rc[2] = rc[3] = 0; }
+    }
+    StDView *dv = &hv->dv;
+    memcpy(dv->cam_pos, f->camera_position, 12);
+    dv->cone_angle = f->cone_angle;
+    memcpy(dv->cam_rot_inv, f->camera_rotation_inverted, 16);
+    memcpy(dv->right, f->right, 12);
+    dv->width1 = f->width;
+    dv->height1 = f->height;
+    dv->mode = f->render_mode;
+    dv->depth = (uint8_t)f->depth;   /* Trace.depth is a u8 (core/types.h:199) */
+    memcpy(hv->start, f->start, 12);
+    memcpy(hv->right, f->right, 12);
+    memcpy(hv->down, f->down, 12);
+    memcpy(hv->rot_inv, f->camera_rotation_inverted, 16);
+    return 0;
+}
+
 void st_view_release(StHostView *hv) {
     free(hv->row_starts);
     free(hv->rects);

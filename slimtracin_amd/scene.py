@@ -258,3 +258,18 @@ def blob_with_scene_bvh(blob: bytes, nodes: np.ndarray, leaf_ids: np.ndarray, he
     return (bytes(head) + blob[BLOB_HEADER_BYTES:] + struct.pack("<2I", len(nodes), int(height))
             + np.ascontiguousarray(nodes, dtype=NODE_DTYPE).tobytes()
             + np.ascontiguousarray(leaf_ids, dtype=np.uint32).tobytes())
+
+
+def blob_records(blob: bytes) -> dict:
+    """The blob's header counts, ambient light and the raw primitive,
+    material (84 B) and light (28 B) record arrays."""
+    magic, version, P, M, L, NM, NT = struct.unpack_from("<7I", blob, 0)
+    ambient = struct.unpack_from("<3f", blob, 28)
+    o = BLOB_HEADER_BYTES
+    prims = np.frombuffer(blob, dtype=PRIM_DTYPE, count=P, offset=o).copy()
+    o += P * PRIM_DTYPE.itemsize
+    mats = bytes(blob[o:o + 84 * M])
+    o += 84 * M
+    lights = bytes(blob[o:o + 28 * L])
+    return dict(prim_count=P, material_count=M, light_count=L, mesh_count=NM, texture_count=NT, ambient=ambient,
+                prims=prims, materials=mats, lights=lights)
