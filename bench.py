@@ -92,7 +92,7 @@ def cpu_baseline(blob, view, threads, frames=1):
                        f"{threads} threads, median of {frames} frame(s), {rays} rays, {t:.2f} s/frame"), stats, hits
 
 
-TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "k_render_pbr200k_1920x1080_traffic.txt")
+TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "k_render_pbr200k_1920x1080.txt")
 
 
 def measured_traffic():
