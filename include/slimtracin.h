@@ -39,7 +39,7 @@ enum {
     ST_ERR_SCENE = -2,    /* malformed scene blob or unsupported scene */
     ST_ERR_MEMORY = -3,   /* device or host allocation failed / over budget */
     ST_ERR_HIP = -4,      /* a HIP runtime call failed */
-    ST_ERR_BUSY = -5,     /* serving queue full */
+    ST_ERR_BUSY = -5,     /* over the serving HBM budget, or a wait timed out */
 };
 
 #define ST_BATCH_PERSISTENT 1u  /* one persistent launch over a global tile queue */
@@ -88,6 +88,17 @@ int st_render(StSceneHandle *scene, const StView *view, StFloatPixel *pixels, St
 /* n requests fused into one launch. pixels[i] / hits[i] may be NULL. */
 int st_render_batch(StSceneHandle *const *scenes, const StView *views, uint32_t n, StFloatPixel *const *pixels,
                     StHitRec *const *hits, StRayCounts *counts, uint32_t flags);
+
+/* n requests fused into one launch, each frame resolved on the device to its
+   window image as drawViewportToWindowContent does on the host
+   (viewport/viewport.h:12-66): bgra[i] receives width*height BGRA8 pixels
+   (4 B each instead of the traced grid's 24-B FloatPixels). Alpha is 255. */
+int st_render_batch_bgra8(StSceneHandle *const *scenes, const StView *views, uint32_t n, uint32_t *const *bgra,
+                          StRayCounts *counts, uint32_t flags);
+
+/* The same display resolve of a caller's traced grid (grid_w x grid_h
+   FloatPixels, host memory) into (grid_w-1) x (grid_h-1) BGRA8 pixels. */
+int st_resolve_bgra8(const StFloatPixel *grid, uint32_t grid_w, uint32_t grid_h, uint32_t *out);
 
 /* As st_render_batch without downloads; returns the render kernel's
    duration measured with HIP events on the launch stream. */

@@ -634,3 +634,36 @@ void ref_quat_axis(const float *axis, float amount, float *out) {
     quat q = getRotationAroundAxis(Vec3(axis[0], axis[1], axis[2]), amount);
     out[0] = q.axis.x; out[1] = q.axis.y; out[2] = q.axis.z; out[3] = q.amount;
 }
+
+/* The reference's display resolve (drawViewportToWindowContent,
+   viewport/viewport.h:12-66) over a caller's traced grid. */
+int ref_resolve(const StFloatPixel *grid, uint32_t grid_w, uint32_t grid_h, uint32_t *out) {
+    if (grid_w < 2 || grid_h < 2 || grid_w > 0xFFFF || grid_h > 0xFFFF) return -1;
+    PixelGrid frame;
+    memset(&frame, 0, sizeof frame);
+    frame.QCAA = true;
+    frame.dimensions.width = (u16)grid_w;
+    frame.dimensions.height = (u16)grid_h;
+    frame.dimensions.width_times_height = grid_w * grid_h;
+    frame.float_pixels = (FloatPixel *)calloc((size_t)grid_w * grid_h, sizeof(FloatPixel));
+    /* Pixel holds a u32, which is 8 bytes in an LP64 build of the reference:
+       resolve into the reference's own Pixel array, then keep the RGBA bytes */
+    size_t npx = (size_t)(grid_w - 1) * (grid_h - 1);
+    frame.pixels = (Pixel *)calloc(npx, sizeof(Pixel));
+    for (size_t i = 0; i < (size_t)grid_w * grid_h; i++) {
+        frame.float_pixels[i].color = Vec3(grid[i].r, grid[i].g, grid[i].b);
+        frame.float_pixels[i].opacity = grid[i].opacity;
+        frame.float_pixels[i].depth = grid[i].depth;
+    }
+    Viewport vp;
+    memset(&vp, 0, sizeof vp);
+    vp.frame_buffer = &frame;
+    drawViewportToWindowContent(&vp);
+    for (size_t i = 0; i < npx; i++) {
+        RGBA c = frame.pixels[i].color;
+        out[i] = (uint32_t)c.B | ((uint32_t)c.G << 8) | ((uint32_t)c.R << 16) | ((uint32_t)c.A << 24);
+    }
+    free(frame.pixels);
+    free(frame.float_pixels);
+    return 0;
+}

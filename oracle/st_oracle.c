@@ -2200,3 +2200,49 @@ int oracle_prim_boxes(const void *blob, size_t size, float *boxes) {
     scene_free(sc);
     return 0;
 }
+
+/* ---- display resolve --------------------------------------------------------
+ * drawViewportToWindowContent (viewport/viewport.h:12-66), QCAA branch: each
+ * window pixel averages the 4 traced grid corners around it,
+ *   color = (TL.c*TL.o + (TR.c*TR.o + (BL.c*BL.o + BR.c*BR.o))) * 0.25
+ * (scaleAddVec3 / scaleVec3 with fast_mul_add = multiply then add), and each
+ * channel becomes (u8)(c > 255*255 ? 255 : sqrt(c)) with a double sqrt.
+ * A pixel whose 4 corners all have depth INFINITY is the background
+ * (0, 0, 0, 255). Bytes are B, G, R, A (core/base.h:206).
+ * Defined here where the reference is not: alpha is 255 for every pixel (the
+ * reference keeps the previous pixel's alpha, uninitialised until the first
+ * background pixel), and a NaN or negative channel gives 0 (the reference's
+ * (u8) cast of a NaN double is undefined; x86-64 yields 0).
+ */
+static uint8_t resolve_channel(float c) {
+    if (c > 65025.0f) return 255;
+    double s = sqrt((double)c);
+    if (!(s >= 0.0)) return 0;
+    return (uint8_t)s;
+}
+
+void oracle_resolve(const StFloatPixel *grid, uint32_t grid_w, uint32_t grid_h, uint32_t *out) {
+    if (grid_w < 2 || grid_h < 2) return;
+    uint32_t w = grid_w - 1, h = grid_h - 1;
+    for (uint32_t y = 0; y < h; y++)
+        for (uint32_t x = 0; x < w; x++) {
+            const StFloatPixel *tl = grid + (size_t)y * grid_w + x, *tr = tl + 1, *bl = tl + grid_w, *br = bl + 1;
+            uint32_t px;
+            if (tl->depth == INFINITY && tr->depth == INFINITY && bl->depth == INFINITY && br->depth == INFINITY) {
+                px = 0xFF000000u;
+            } else {
+                float c[3];
+                const float *a = &tl->r, *b = &tr->r, *d = &bl->r, *e = &br->r;
+                for (int k = 0; k < 3; k++) {
+                    float inner = e[k] * br->opacity;
+                    inner = d[k] * bl->opacity + inner;
+                    inner = b[k] * tr->opacity + inner;
+                    inner = a[k] * tl->opacity + inner;
+                    c[k] = inner * 0.25f;
+                }
+                px = (uint32_t)resolve_channel(c[2]) | ((uint32_t)resolve_channel(c[1]) << 8) |
+                     ((uint32_t)resolve_channel(c[0]) << 16) | 0xFF000000u;
+            }
+            out[(size_t)y * w + x] = px;
+        }
+}

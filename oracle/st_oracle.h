@@ -67,6 +67,10 @@ uint32_t oracle_lbvh_build(const float *boxes, uint32_t n, StBVHNodeRec *nodes, 
 /* World AABBs of the blob's primitives (getPrimitiveAABB + transformAABB). */
 int oracle_prim_boxes(const void *blob, size_t size, float *boxes);
 
+/* Display resolve of a traced QCAA grid (grid_w x grid_h FloatPixels) into
+   (grid_w-1) x (grid_h-1) BGRA8 pixels (drawViewportToWindowContent). */
+void oracle_resolve(const StFloatPixel *grid, uint32_t grid_w, uint32_t grid_h, uint32_t *out);
+
 #ifdef __cplusplus
 }
 #endif

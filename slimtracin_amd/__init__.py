@@ -172,3 +172,25 @@ def render_timed(scenes, views: np.ndarray, persistent: bool = True):
     _check(lib().st_render_timed(handles, _ptr(views), C.c_uint32(n), C.c_uint32(ST_BATCH_PERSISTENT if persistent else 0),
                                  C.byref(ms), counts))
     return ms.value, list(counts)
+
+
+def render_batch_bgra8(scenes, views: np.ndarray, persistent: bool = True):
+    """Fuse the requests into one launch and resolve each frame on the device
+    to its width x height BGRA8 window image (uint32 per pixel)."""
+    n = len(views)
+    handles = (C.c_void_p * n)(*[s.handle for s in scenes])
+    outs = [np.zeros(int(views["width"][i]) * int(views["height"][i]), dtype=np.uint32) for i in range(n)]
+    op = (C.c_void_p * n)(*[o.ctypes.data for o in outs])
+    counts = (RayCounts * n)()
+    views = np.ascontiguousarray(views, dtype=VIEW_DTYPE)
+    _check(lib().st_render_batch_bgra8(handles, _ptr(views), C.c_uint32(n), op, counts,
+                                       C.c_uint32(ST_BATCH_PERSISTENT if persistent else 0)))
+    return outs, list(counts)
+
+
+def resolve_bgra8(grid: np.ndarray, grid_w: int, grid_h: int) -> np.ndarray:
+    """Device display resolve of a traced FloatPixel grid (st_resolve_bgra8)."""
+    g = np.ascontiguousarray(grid, dtype=PIXEL_DTYPE)
+    out = np.zeros((grid_w - 1) * (grid_h - 1), dtype=np.uint32)
+    _check(lib().st_resolve_bgra8(_ptr(g), C.c_uint32(grid_w), C.c_uint32(grid_h), _ptr(out)))
+    return out

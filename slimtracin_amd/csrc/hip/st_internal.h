@@ -25,13 +25,16 @@ struct StSceneHandle {
 };
 
 int st_rt_ensure_init(void);
+/* The runtime's lock over the shared stream and buffers (recursive). */
+void st_rt_lock(void);
+void st_rt_unlock(void);
 hipStream_t st_rt_stream(void);
 /* Record `msg` as the last error and return `code`. */
 int st_rt_error(int code, const char *msg);
 
 int st_rt_launch_prepared(StSceneHandle *const *scenes, StHostView *hv, uint32_t n, int want_hits, int persistent,
                           StRayCounts *counts, StFloatPixel *const *pixels_host, StHitRec *const *hits_host,
-                          float *kernel_ms);
+                          uint32_t *const *bgra_host, float *kernel_ms);
 
 void st_seam_state_free(struct StSeamState *s);
 

@@ -27,6 +27,7 @@
 #include <string.h>
 #include <time.h>
 
+#include <mutex>
 #include <vector>
 
 #include "st_trace.h"
@@ -45,6 +46,11 @@
 #ifndef K_RENDER_ATTR
 #define K_RENDER_ATTR __attribute__((amdgpu_waves_per_eu(2)))
 #endif
+
+/* Every use of the shared stream, arena and staging buffer holds this lock
+   (the server's worker and direct callers may run on different threads). */
+static std::recursive_mutex g_mu;
+#define RT_LOCK std::lock_guard<std::recursive_mutex> rt_lock_(g_mu)
 
 /* ---- errors ------------------------------------------------------------ */
 
@@ -74,7 +80,10 @@ struct KReq {
     StDView v;
     uint32_t row_base;     /* first flattened row of this request */
     uint32_t tile_count;
-    uint32_t pad[2];
+    uint32_t out_base;     /* first flattened window pixel (display resolve) */
+    uint32_t pad;
+    uint32_t *bgra;        /* window image (width1-1)*(height1-1), or NULL */
+    uint64_t pad2;
 };
 
 __device__ static inline uint32_t find_req(const KReq *reqs, uint32_t n, uint32_t key, bool by_row) {
@@ -186,6 +195,65 @@ __global__ void __launch_bounds__(BLOCK_THREADS) K_RENDER_ATTR k_render(const St
     }
 }
 
+/* Display resolve of one window pixel (drawViewportToWindowContent,
+   viewport/viewport.h:12-66, QCAA branch): the 4 traced corners are averaged
+   as (TL.c*TL.o + (TR.c*TR.o + (BL.c*BL.o + BR.c*BR.o))) * 0.25 and each
+   channel is (u8)sqrt(c), 255 above 255^2. (u8) of the double square root is
+   floor(sqrt(c)): computed exactly in integers from the correctly rounded
+   fp32 root (squares up to 255^2 are exact in fp32). Background (all four
+   depths infinite) is (0,0,0,255); alpha is 255 everywhere; NaN or negative
+   channels give 0. Bytes B, G, R, A. */
+__device__ static inline uint32_t resolve_channel(float c) {
+    if (c > 65025.0f) return 255u;
+    if (!(c >= 0.0f)) return 0u;
+    uint32_t s = (uint32_t)sqrtf(c);
+    if (s > 255u) s = 255u;
+    while (s < 255u && (float)((s + 1u) * (s + 1u)) <= c) s++;
+    while (s > 0u && (float)(s * s) > c) s--;
+    return s;
+}
+
+__global__ void __launch_bounds__(256) k_resolve(const KReq *reqs, uint32_t nreq, uint32_t total_out) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= total_out) return;
+    uint32_t lo = 0, hi = nreq - 1;
+    while (lo < hi) {
+        uint32_t mid = (lo + hi + 1) >> 1;
+        if (reqs[mid].out_base <= i) lo = mid; else hi = mid - 1;
+    }
+    const KReq &r = reqs[lo];
+    if (!r.bgra) return;
+    const uint32_t w1 = r.v.width1, w = w1 - 1u;
+    const uint32_t local = i - r.out_base;
+    if (local >= w * (r.v.height1 - 1u)) return;
+    const uint32_t x = local % w, y = local / w;
+    const char *base = static_cast<const char *>(r.v.pixels);
+    const size_t tl = (size_t)y * w1 + x, corner[4] = {tl, tl + 1, tl + w1, tl + w1 + 1};
+    float c[4][4];
+    bool background = true;
+    for (int k = 0; k < 4; k++) {
+        /* 24-byte records are 8-byte aligned: three 8-byte loads */
+        const float2 rg = *reinterpret_cast<const float2 *>(base + 24 * corner[k]);
+        const float2 bo = *reinterpret_cast<const float2 *>(base + 24 * corner[k] + 8);
+        const double d = *reinterpret_cast<const double *>(base + 24 * corner[k] + 16);
+        c[k][0] = rg.x; c[k][1] = rg.y; c[k][2] = bo.x; c[k][3] = bo.y;
+        background = background && d == (double)INFINITY;
+    }
+    uint32_t px = 0xFF000000u;
+    if (!background) {
+        uint32_t ch[3];
+        for (int k = 0; k < 3; k++) {
+            float acc = c[3][k] * c[3][3];
+            acc = c[2][k] * c[2][3] + acc;
+            acc = c[1][k] * c[1][3] + acc;
+            acc = c[0][k] * c[0][3] + acc;
+            ch[k] = resolve_channel(acc * 0.25f);
+        }
+        px |= ch[2] | (ch[1] << 8) | (ch[0] << 16);
+    }
+    r.bgra[local] = px;
+}
+
 __global__ void k_zero_u32(uint32_t *p, uint32_t n) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0;
@@ -213,6 +281,7 @@ static struct {
     StDScene *scene_table; /* device copy of the scene descriptors of a launch */
     uint32_t scene_table_cap;
     hipEvent_t ev[4];
+    int cus;               /* compute units of the device */
 } g;
 
 static int ensure_init(void) {
@@ -221,6 +290,7 @@ static int ensure_init(void) {
 }
 
 extern "C" int st_init(int device) {
+    RT_LOCK;
     if (g.ready) {
         if (device == g.device) return ST_OK;
         return set_err(ST_ERR_ARG, "library already initialised on device %d", g.device);
@@ -232,10 +302,17 @@ extern "C" int st_init(int device) {
     HIP_TRY(hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking));
     HIP_TRY(hipMalloc(&g.tile_counter, 256));
     for (int i = 0; i < 4; i++) HIP_TRY(hipEventCreate(&g.ev[i]));
+    hipDeviceProp_t prop;
+    g.cus = 256;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
+        g.cus = prop.multiProcessorCount;
     g.device = device;
     g.ready = 1;
     return ST_OK;
 }
+
+void st_rt_lock(void) { g_mu.lock(); }
+void st_rt_unlock(void) { g_mu.unlock(); }
 
 int st_rt_ensure_init(void) { return ensure_init(); }
 hipStream_t st_rt_stream(void) { return g.stream; }
@@ -291,6 +368,7 @@ extern "C" int st_scene_create(const void *blob, size_t size, StSceneHandle **ou
    boundary: node 0 (the root) sits alone and every child pair (2k+1, 2k+2)
    covers exactly one 64-byte segment. */
 extern "C" int st_scene_upload(StSceneHandle *h) {
+    RT_LOCK;
     if (!h || !h->host) return set_err(ST_ERR_ARG, "scene has nothing to upload");
     StScene *sc = h->host;
     size_t o_prims = 0;
@@ -370,6 +448,7 @@ extern "C" int st_scene_upload(StSceneHandle *h) {
 }
 
 extern "C" void st_scene_destroy(StSceneHandle *h) {
+    RT_LOCK;
     if (!h) return;
     if (h->dbuf) (void)hipFree(h->dbuf);
     if (h->lbvh_work) (void)hipFree(h->lbvh_work);
@@ -398,6 +477,7 @@ extern "C" int st_scene_info(const StSceneHandle *h, StSceneInfo *info) {
 
 extern "C" int st_scene_bvh(const StSceneHandle *h, StBVHNodeRec *nodes, uint32_t *leaf_ids, uint32_t *node_count,
                             uint32_t *height) {
+    RT_LOCK;
     if (!h || !nodes || !leaf_ids || !node_count || !height) return set_err(ST_ERR_ARG, "null argument");
     if (!h->host) return set_err(ST_ERR_ARG, "scene is not resident yet");
     const StScene *sc = h->host;
@@ -418,6 +498,7 @@ extern "C" int st_scene_bvh(const StSceneHandle *h, StBVHNodeRec *nodes, uint32_
 /* ---- moving primitives ------------------------------------------------- */
 
 extern "C" int st_scene_set_prims(StSceneHandle *h, const StPrimRec *recs, uint32_t first, uint32_t count) {
+    RT_LOCK;
     if (!h || !recs) return set_err(ST_ERR_ARG, "null argument");
     if (!h->host || !h->dbuf) return set_err(ST_ERR_ARG, "scene is not resident yet");
     StScene *sc = h->host;
@@ -445,6 +526,7 @@ extern "C" int st_scene_set_prims(StSceneHandle *h, const StPrimRec *recs, uint3
 /* Rebuild the scene BVH over the current primitive transforms: the
    reference's SAH builder on the host, or the Morton LBVH on the device. */
 extern "C" int st_scene_rebuild(StSceneHandle *h, uint32_t mode, float *ms) {
+    RT_LOCK;
     if (!h) return set_err(ST_ERR_ARG, "null scene");
     int rc = ensure_init();
     if (rc) return rc;
@@ -505,15 +587,21 @@ struct LaunchPlan {
    hit records) is copied to pixels_host[i] / hits_host[i] when given. */
 int st_rt_launch_prepared(StSceneHandle *const *scenes, StHostView *hv, uint32_t n, int want_hits, int persistent,
                           StRayCounts *counts, StFloatPixel *const *pixels_host, StHitRec *const *hits_host,
-                          float *kernel_ms) {
+                          uint32_t *const *bgra_host, float *kernel_ms) {
+    RT_LOCK;
     int rc = ensure_init();
     if (rc) return rc;
     if (n == 0) return set_err(ST_ERR_ARG, "no requests");
     std::vector<uint32_t> scene_idx(n);
     std::vector<StSceneHandle *> uniq(n);
     uint32_t nuniq = 0;
-    size_t work = 0, stage = 0;
+    /* one upload block (requests, scene table, row starts, rects), staged in
+       pinned memory and copied with a single transfer */
+    std::vector<size_t> rs_off(n), rect_off(n);
+    size_t up = round256(n * sizeof(KReq));
+    size_t per_req_out = 0;
     uint32_t rows = 0, tiles = 0;
+    uint64_t out_px = 0;
     for (uint32_t i = 0; i < n; i++) {
         StSceneHandle *s = scenes[i];
         if (!s || !s->host || !s->dbuf) return set_err(ST_ERR_ARG, "request %u has no resident scene", i);
@@ -522,118 +610,122 @@ int st_rt_launch_prepared(StSceneHandle *const *scenes, StHostView *hv, uint32_t
         if (k == nuniq) uniq[nuniq++] = s;
         scene_idx[i] = k;
         size_t px = (size_t)hv[i].dv.width1 * hv[i].dv.height1;
-        work += round256(px * 16) + round256(px * 24) + (want_hits ? round256(px * 16) : 0);
-        work += round256(hv[i].dv.height1 * 16ull) + round256((s->host->prim_count + 1) * 16ull) + 256;
-        stage += round256(hv[i].dv.height1 * 16ull) + round256((s->host->prim_count + 1) * 16ull);
+        per_req_out += round256(px * 16) + round256(px * 24) + (want_hits ? round256(px * 16) : 0);
+        if (bgra_host && bgra_host[i]) {
+            if (hv[i].dv.width1 < 2 || hv[i].dv.height1 < 2) return set_err(ST_ERR_ARG, "request %u: empty window", i);
+            per_req_out += round256((size_t)(hv[i].dv.width1 - 1) * (hv[i].dv.height1 - 1) * 4);
+        }
         hv[i].dv.tiles_x = (hv[i].dv.width1 + TILE_W - 1) / TILE_W;
         uint32_t ty = (hv[i].dv.height1 + TILE_H - 1) / TILE_H;
         hv[i].dv.tile_base = tiles;
         tiles += hv[i].dv.tiles_x * ty;
         rows += hv[i].dv.height1;
+        out_px += (uint64_t)(hv[i].dv.width1 - 1) * (hv[i].dv.height1 - 1);
     }
-    work += round256(n * sizeof(KReq)) + round256(nuniq * sizeof(StDScene)) + round256(n * 24ull);
-    stage += round256(n * sizeof(KReq)) + round256(nuniq * sizeof(StDScene));
+    if (out_px > 0xFFFFFFF0ull || tiles > 0x7FFFFFF0u) return set_err(ST_ERR_ARG, "batch too large");
+    size_t o_scenes = up;
+    up += round256(nuniq * sizeof(StDScene));
+    for (uint32_t i = 0; i < n; i++) {
+        rs_off[i] = up;
+        up += round256(hv[i].dv.height1 * 16ull);
+        rect_off[i] = up;
+        up += round256((scenes[i]->host->prim_count + 1) * 16ull);
+    }
+    size_t work = up + round256(n * 24ull) + per_req_out + 256;
     if ((rc = arena_reserve(&g.work, work))) return rc;
-    if ((rc = staging_reserve(stage))) return rc;
-    {
-        char *st = g.staging;
-        size_t so = 0;
-        KReq *reqs_h = (KReq *)(st + so);
-        so += round256(n * sizeof(KReq));
-        StDScene *sc_h = (StDScene *)(st + so);
-        so += round256(nuniq * sizeof(StDScene));
-        KReq *reqs_d = (KReq *)arena_take(&g.work, n * sizeof(KReq));
-        StDScene *sc_d = (StDScene *)arena_take(&g.work, nuniq * sizeof(StDScene));
-        unsigned long long *cnt_d = (unsigned long long *)arena_take(&g.work, n * 24ull);
-        for (uint32_t k = 0; k < nuniq; k++) sc_h[k] = uniq[k]->dev;
-        uint32_t row_base = 0;
-        /* per-request row starts and rects are staged right after the tables */
-        size_t stage_tables = so;
-        for (uint32_t i = 0; i < n; i++) {
-            size_t px = (size_t)hv[i].dv.width1 * hv[i].dv.height1;
-            StDView &dv = hv[i].dv;
-            dv.scene = scene_idx[i];
-            float *rs_d = (float *)arena_take(&g.work, dv.height1 * 16ull);
-            int32_t *rect_d = (int32_t *)arena_take(&g.work, (scenes[i]->host->prim_count + 1) * 16ull);
-            memcpy(st + so, hv[i].row_starts, dv.height1 * 16ull);
-            size_t rs_off = so;
-            so += round256(dv.height1 * 16ull);
-            memcpy(st + so, hv[i].rects, scenes[i]->host->prim_count * 16ull);
-            size_t rect_off = so;
-            so += round256((scenes[i]->host->prim_count + 1) * 16ull);
-            HIP_TRY(hipMemcpyAsync(rs_d, st + rs_off, dv.height1 * 16ull, hipMemcpyHostToDevice, g.stream));
-            if (scenes[i]->host->prim_count)
-                HIP_TRY(hipMemcpyAsync(rect_d, st + rect_off, scenes[i]->host->prim_count * 16ull,
-                                       hipMemcpyHostToDevice, g.stream));
-            dv.row_starts = rs_d;
-            dv.rects = rect_d;
-            dv.dirs = (float *)arena_take(&g.work, px * 16);
-            dv.pixels = arena_take(&g.work, px * 24);
-            dv.hits = want_hits ? arena_take(&g.work, px * 16) : NULL;
-            dv.counters = cnt_d + 3 * i;
-            reqs_h[i].v = dv;
-            reqs_h[i].row_base = row_base;
-            reqs_h[i].tile_count = 0;
-            row_base += dv.height1;
-        }
-        (void)stage_tables;
-        HIP_TRY(hipMemcpyAsync(reqs_d, reqs_h, n * sizeof(KReq), hipMemcpyHostToDevice, g.stream));
-        HIP_TRY(hipMemcpyAsync(sc_d, sc_h, nuniq * sizeof(StDScene), hipMemcpyHostToDevice, g.stream));
-        hipLaunchKernelGGL(k_zero_u64, dim3((3 * n + 255) / 256), dim3(256), 0, g.stream, cnt_d, 3 * n);
-        hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, g.stream, g.tile_counter, 64u);
-        hipLaunchKernelGGL(k_primary_dirs, dim3((rows + 255) / 256), dim3(256), 0, g.stream, reqs_d, n, rows);
-        HIP_TRY(hipGetLastError());
-        HIP_TRY(hipEventRecord(g.ev[0], g.stream));
-        uint32_t blocks;
-        uint32_t xcd_blocks = 0;
-        if (persistent) {
-            int cus = 256;
-            hipDeviceProp_t prop;
-            if (hipGetDeviceProperties(&prop, g.device) == hipSuccess) cus = prop.multiProcessorCount;
-            uint32_t want = (uint32_t)cus * 8u;
-            uint32_t need = (tiles + WAVES_PER_BLOCK - 1) / WAVES_PER_BLOCK;
-            blocks = need < want ? need : want;
-        } else {
-            blocks = (tiles + WAVES_PER_BLOCK - 1) / WAVES_PER_BLOCK;
-            blocks = (blocks + 7u) & ~7u;
-            xcd_blocks = blocks;
-        }
-        uint32_t s_entries = 1, m_entries = 1;
-        for (uint32_t k = 0; k < nuniq; k++) {
-            if (sc_h[k].scene_stack_need > s_entries) s_entries = sc_h[k].scene_stack_need;
-            if (sc_h[k].mesh_stack_size > m_entries) m_entries = sc_h[k].mesh_stack_size;
-        }
-        size_t lds = (size_t)BLOCK_THREADS * (s_entries + m_entries) * sizeof(uint32_t);
-        if (lds > 160 * 1024) return set_err(ST_ERR_SCENE, "traversal stacks need %zu B of LDS", lds);
-        hipLaunchKernelGGL(k_render, dim3(blocks), dim3(BLOCK_THREADS), lds, g.stream, (const StDScene *)sc_d,
-                           (const KReq *)reqs_d, n, tiles, g.tile_counter, (uint32_t)persistent, xcd_blocks,
-                           s_entries, m_entries);
-        HIP_TRY(hipGetLastError());
-        HIP_TRY(hipEventRecord(g.ev[1], g.stream));
-        for (uint32_t i = 0; i < n; i++) {
-            size_t px = (size_t)hv[i].dv.width1 * hv[i].dv.height1;
-            if (pixels_host && pixels_host[i])
-                HIP_TRY(hipMemcpyAsync(pixels_host[i], reqs_h[i].v.pixels, px * 24, hipMemcpyDeviceToHost, g.stream));
-            if (hits_host && hits_host[i] && want_hits)
-                HIP_TRY(hipMemcpyAsync(hits_host[i], reqs_h[i].v.hits, px * 16, hipMemcpyDeviceToHost, g.stream));
-        }
-        std::vector<unsigned long long> cnt_h(3 * (size_t)n);
-        HIP_TRY(hipMemcpyAsync(cnt_h.data(), cnt_d, 24ull * n, hipMemcpyDeviceToHost, g.stream));
-        HIP_TRY(hipStreamSynchronize(g.stream));
-        if (counts)
-            for (uint32_t i = 0; i < n; i++) {
-                counts[i].primary = cnt_h[3 * i];
-                counts[i].shadow = cnt_h[3 * i + 1];
-                counts[i].bounce = cnt_h[3 * i + 2];
-            }
-        if (kernel_ms) HIP_TRY(hipEventElapsedTime(kernel_ms, g.ev[0], g.ev[1]));
+    if ((rc = staging_reserve(up))) return rc;
+    char *st = g.staging;
+    char *blk = (char *)arena_take(&g.work, up);
+    KReq *reqs_h = (KReq *)st;
+    KReq *reqs_d = (KReq *)blk;
+    StDScene *sc_h = (StDScene *)(st + o_scenes);
+    StDScene *sc_d = (StDScene *)(blk + o_scenes);
+    unsigned long long *cnt_d = (unsigned long long *)arena_take(&g.work, n * 24ull);
+    for (uint32_t k = 0; k < nuniq; k++) sc_h[k] = uniq[k]->dev;
+    uint32_t row_base = 0, out_base = 0;
+    for (uint32_t i = 0; i < n; i++) {
+        size_t px = (size_t)hv[i].dv.width1 * hv[i].dv.height1;
+        StDView &dv = hv[i].dv;
+        dv.scene = scene_idx[i];
+        uint32_t P = scenes[i]->host->prim_count;
+        memcpy(st + rs_off[i], hv[i].row_starts, dv.height1 * 16ull);
+        memcpy(st + rect_off[i], hv[i].rects, P * 16ull);
+        dv.row_starts = (const float *)(blk + rs_off[i]);
+        dv.rects = (const int32_t *)(blk + rect_off[i]);
+        dv.dirs = (float *)arena_take(&g.work, px * 16);
+        dv.pixels = arena_take(&g.work, px * 24);
+        dv.hits = want_hits ? arena_take(&g.work, px * 16) : NULL;
+        dv.counters = cnt_d + 3 * i;
+        memset(&reqs_h[i], 0, sizeof(KReq));
+        reqs_h[i].v = dv;
+        reqs_h[i].row_base = row_base;
+        reqs_h[i].out_base = out_base;
+        uint32_t wpx = (dv.width1 - 1) * (dv.height1 - 1);
+        reqs_h[i].bgra = (bgra_host && bgra_host[i]) ? (uint32_t *)arena_take(&g.work, (size_t)wpx * 4) : NULL;
+        row_base += dv.height1;
+        out_base += wpx;
     }
+    HIP_TRY(hipMemcpyAsync(blk, st, up, hipMemcpyHostToDevice, g.stream));
+    hipLaunchKernelGGL(k_zero_u64, dim3((3 * n + 255) / 256), dim3(256), 0, g.stream, cnt_d, 3 * n);
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, g.stream, g.tile_counter, 64u);
+    hipLaunchKernelGGL(k_primary_dirs, dim3((rows + 255) / 256), dim3(256), 0, g.stream, reqs_d, n, rows);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(g.ev[0], g.stream));
+    uint32_t blocks;
+    uint32_t xcd_blocks = 0;
+    if (persistent) {
+        uint32_t want = (uint32_t)g.cus * 8u;
+        uint32_t need = (tiles + WAVES_PER_BLOCK - 1) / WAVES_PER_BLOCK;
+        blocks = need < want ? need : want;
+    } else {
+        blocks = (tiles + WAVES_PER_BLOCK - 1) / WAVES_PER_BLOCK;
+        blocks = (blocks + 7u) & ~7u;
+        xcd_blocks = blocks;
+    }
+    uint32_t s_entries = 1, m_entries = 1;
+    for (uint32_t k = 0; k < nuniq; k++) {
+        if (sc_h[k].scene_stack_need > s_entries) s_entries = sc_h[k].scene_stack_need;
+        if (sc_h[k].mesh_stack_size > m_entries) m_entries = sc_h[k].mesh_stack_size;
+    }
+    size_t lds = (size_t)BLOCK_THREADS * (s_entries + m_entries) * sizeof(uint32_t);
+    if (lds > 160 * 1024) return set_err(ST_ERR_SCENE, "traversal stacks need %zu B of LDS", lds);
+    hipLaunchKernelGGL(k_render, dim3(blocks), dim3(BLOCK_THREADS), lds, g.stream, (const StDScene *)sc_d,
+                       (const KReq *)reqs_d, n, tiles, g.tile_counter, (uint32_t)persistent, xcd_blocks, s_entries,
+                       m_entries);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(g.ev[1], g.stream));
+    if (bgra_host && out_px) {
+        hipLaunchKernelGGL(k_resolve, dim3((uint32_t)((out_px + 255) / 256)), dim3(256), 0, g.stream,
+                           (const KReq *)reqs_d, n, (uint32_t)out_px);
+        HIP_TRY(hipGetLastError());
+    }
+    for (uint32_t i = 0; i < n; i++) {
+        size_t px = (size_t)hv[i].dv.width1 * hv[i].dv.height1;
+        if (pixels_host && pixels_host[i])
+            HIP_TRY(hipMemcpyAsync(pixels_host[i], reqs_h[i].v.pixels, px * 24, hipMemcpyDeviceToHost, g.stream));
+        if (hits_host && hits_host[i] && want_hits)
+            HIP_TRY(hipMemcpyAsync(hits_host[i], reqs_h[i].v.hits, px * 16, hipMemcpyDeviceToHost, g.stream));
+        if (reqs_h[i].bgra)
+            HIP_TRY(hipMemcpyAsync(bgra_host[i], reqs_h[i].bgra,
+                                   (size_t)(hv[i].dv.width1 - 1) * (hv[i].dv.height1 - 1) * 4, hipMemcpyDeviceToHost,
+                                   g.stream));
+    }
+    std::vector<unsigned long long> cnt_h(3 * (size_t)n);
+    HIP_TRY(hipMemcpyAsync(cnt_h.data(), cnt_d, 24ull * n, hipMemcpyDeviceToHost, g.stream));
+    HIP_TRY(hipStreamSynchronize(g.stream));
+    if (counts)
+        for (uint32_t i = 0; i < n; i++) {
+            counts[i].primary = cnt_h[3 * i];
+            counts[i].shadow = cnt_h[3 * i + 1];
+            counts[i].bounce = cnt_h[3 * i + 2];
+        }
+    if (kernel_ms) HIP_TRY(hipEventElapsedTime(kernel_ms, g.ev[0], g.ev[1]));
     return rc;
 }
 
 static int launch(StSceneHandle *const *scenes, const StView *views, uint32_t n, int want_hits, int persistent,
                   StRayCounts *counts, StFloatPixel *const *pixels_host, StHitRec *const *hits_host,
-                  float *kernel_ms) {
+                  uint32_t *const *bgra_host, float *kernel_ms) {
     int rc = ensure_init();
     if (rc) return rc;
     if (n == 0) return set_err(ST_ERR_ARG, "no requests");
@@ -645,7 +737,9 @@ static int launch(StSceneHandle *const *scenes, const StView *views, uint32_t n,
             rc = set_err(ST_ERR_ARG, "request %u: invalid view (%ux%u, mode %u)", i, views[i].width, views[i].height,
                          views[i].render_mode);
     }
-    if (!rc) rc = st_rt_launch_prepared(scenes, hv, n, want_hits, persistent, counts, pixels_host, hits_host, kernel_ms);
+    if (!rc)
+        rc = st_rt_launch_prepared(scenes, hv, n, want_hits, persistent, counts, pixels_host, hits_host, bgra_host,
+                                   kernel_ms);
     for (uint32_t i = 0; i < n; i++) st_view_release(hv + i);
     free(hv);
     return rc;
@@ -654,7 +748,7 @@ static int launch(StSceneHandle *const *scenes, const StView *views, uint32_t n,
 extern "C" int st_render(StSceneHandle *scene, const StView *view, StFloatPixel *pixels, StHitRec *hits,
                          StRayCounts *counts) {
     if (!scene || !view) return set_err(ST_ERR_ARG, "null argument");
-    return launch(&scene, view, 1, hits != NULL, 1, counts, &pixels, &hits, NULL);
+    return launch(&scene, view, 1, hits != NULL, 1, counts, &pixels, &hits, NULL, NULL);
 }
 
 extern "C" int st_render_batch(StSceneHandle *const *scenes, const StView *views, uint32_t n,
@@ -664,10 +758,44 @@ extern "C" int st_render_batch(StSceneHandle *const *scenes, const StView *views
     int want_hits = 0;
     if (hits)
         for (uint32_t i = 0; i < n; i++) want_hits |= hits[i] != NULL;
-    return launch(scenes, views, n, want_hits, (flags & ST_BATCH_PERSISTENT) ? 1 : 0, counts, pixels, hits, NULL);
+    return launch(scenes, views, n, want_hits, (flags & ST_BATCH_PERSISTENT) ? 1 : 0, counts, pixels, hits, NULL, NULL);
 }
 
 extern "C" int st_render_timed(StSceneHandle *const *scenes, const StView *views, uint32_t n, uint32_t flags,
                                float *kernel_ms, StRayCounts *counts) {
-    return launch(scenes, views, n, 0, (flags & ST_BATCH_PERSISTENT) ? 1 : 0, counts, NULL, NULL, kernel_ms);
+    return launch(scenes, views, n, 0, (flags & ST_BATCH_PERSISTENT) ? 1 : 0, counts, NULL, NULL, NULL, kernel_ms);
+}
+
+extern "C" int st_render_batch_bgra8(StSceneHandle *const *scenes, const StView *views, uint32_t n,
+                                     uint32_t *const *bgra, StRayCounts *counts, uint32_t flags) {
+    if (!scenes || !views || !bgra) return set_err(ST_ERR_ARG, "null argument");
+    return launch(scenes, views, n, 0, (flags & ST_BATCH_PERSISTENT) ? 1 : 0, counts, NULL, NULL, bgra, NULL);
+}
+
+extern "C" int st_resolve_bgra8(const StFloatPixel *grid, uint32_t grid_w, uint32_t grid_h, uint32_t *out) {
+    RT_LOCK;
+    if (!grid || !out) return set_err(ST_ERR_ARG, "null argument");
+    if (grid_w < 2 || grid_h < 2 || grid_w > 0xFFFF || grid_h > 0xFFFF) return set_err(ST_ERR_ARG, "bad grid size");
+    int rc = ensure_init();
+    if (rc) return rc;
+    size_t gbytes = (size_t)grid_w * grid_h * 24, opx = (size_t)(grid_w - 1) * (grid_h - 1);
+    if ((rc = arena_reserve(&g.work, round256(sizeof(KReq)) + round256(gbytes) + round256(opx * 4) + 256))) return rc;
+    if ((rc = staging_reserve(round256(sizeof(KReq))))) return rc;
+    KReq *req_d = (KReq *)arena_take(&g.work, sizeof(KReq));
+    void *grid_d = arena_take(&g.work, gbytes);
+    uint32_t *out_d = (uint32_t *)arena_take(&g.work, opx * 4);
+    KReq *req_h = (KReq *)g.staging;
+    memset(req_h, 0, sizeof(KReq));
+    req_h->v.pixels = grid_d;
+    req_h->v.width1 = grid_w;
+    req_h->v.height1 = grid_h;
+    req_h->bgra = out_d;
+    HIP_TRY(hipMemcpyAsync(grid_d, grid, gbytes, hipMemcpyHostToDevice, g.stream));
+    HIP_TRY(hipMemcpyAsync(req_d, req_h, sizeof(KReq), hipMemcpyHostToDevice, g.stream));
+    hipLaunchKernelGGL(k_resolve, dim3((uint32_t)((opx + 255) / 256)), dim3(256), 0, g.stream, (const KReq *)req_d, 1u,
+                       (uint32_t)opx);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(out, out_d, opx * 4, hipMemcpyDeviceToHost, g.stream));
+    HIP_TRY(hipStreamSynchronize(g.stream));
+    return ST_OK;
 }

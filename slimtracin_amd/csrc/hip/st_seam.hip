@@ -271,7 +271,15 @@ static void refresh_emissive(StSceneHandle *h) {
     h->dev.has_emissive_quads = sc->has_emissive_quads;
 }
 
+namespace {
+struct RtLock {
+    RtLock() { st_rt_lock(); }
+    ~RtLock() { st_rt_unlock(); }
+};
+}  // namespace
+
 extern "C" int st_seam_sync(StSceneHandle *h) {
+    RtLock lock;
     StSeamState *s = state(h);
     if (!s) return fail(ST_ERR_ARG, "not a seam scene");
     if (!h->host || !h->dbuf || s->dirty_full) return seam_build(h);
@@ -322,6 +330,7 @@ extern "C" int st_seam_sync(StSceneHandle *h) {
 
 extern "C" int st_seam_render(StSceneHandle *h, const StSeamFrame *frame, StFloatPixel *pixels, StHitRec *hits,
                               StRayCounts *counts) {
+    RtLock lock;
     StSeamState *s = state(h);
     if (!s || !frame) return fail(ST_ERR_ARG, "not a seam scene or null frame");
     memcpy(s->ambient, frame->ambient, 12);
@@ -332,7 +341,7 @@ extern "C" int st_seam_render(StSceneHandle *h, const StSeamFrame *frame, StFloa
     StHostView hv;
     if (st_view_prepare_frame(h->host, frame, s->bounds.data(), &hv))
         return fail(ST_ERR_ARG, "invalid frame (size or render mode)");
-    rc = st_rt_launch_prepared(&h, &hv, 1, hits != NULL, 1, counts, &pixels, &hits, NULL);
+    rc = st_rt_launch_prepared(&h, &hv, 1, hits != NULL, 1, counts, &pixels, &hits, NULL, NULL);
     st_view_release(&hv);
     return rc;
 }

@@ -236,3 +236,36 @@ def oracle_prim_boxes(blob, prim_count):
     bp, bs, keep = _blob(blob)
     assert _load(ORACLE_SO).oracle_prim_boxes(bp, bs, _ptr(out)) == 0
     return out
+
+
+def oracle_resolve(grid, grid_w, grid_h):
+    out = np.zeros((grid_h - 1) * (grid_w - 1), dtype=np.uint32)
+    g = np.ascontiguousarray(grid, dtype=PIXEL_DTYPE)
+    _load(ORACLE_SO).oracle_resolve(_ptr(g), C.c_uint32(grid_w), C.c_uint32(grid_h), _ptr(out))
+    return out
+
+
+def ref_resolve(grid, grid_w, grid_h):
+    out = np.zeros((grid_h - 1) * (grid_w - 1), dtype=np.uint32)
+    g = np.ascontiguousarray(grid, dtype=PIXEL_DTYPE)
+    assert _load(REF_SO).ref_resolve(_ptr(g), C.c_uint32(grid_w), C.c_uint32(grid_h), _ptr(out)) == 0
+    return out
+
+
+def resolve_test_grid(rng, w1, h1):
+    """A traced grid exercising the resolve's edges: background corners
+    (depth inf), channels at exact squares and one ulp either side, above
+    255^2, zero, negative zero, fractional opacities."""
+    n = w1 * h1
+    g = np.zeros(n, dtype=PIXEL_DTYPE)
+    k = rng.integers(0, 256, (n, 3)).astype(np.float32) ** 2
+    bump = rng.integers(-1, 2, (n, 3))
+    vals = np.where(bump > 0, np.nextafter(k, np.float32(np.inf)), np.where(bump < 0, np.nextafter(k, np.float32(-1)), k))
+    vals = vals.astype(np.float32)
+    big = rng.random((n, 3)) < 0.05
+    vals[big] = np.float32(70000.0)
+    vals[rng.random((n, 3)) < 0.03] = np.float32(-0.0)
+    g["r"], g["g"], g["b"] = vals[:, 0], vals[:, 1], vals[:, 2]
+    g["opacity"] = np.where(rng.random(n) < 0.8, np.float32(1.0), rng.random(n).astype(np.float32))
+    g["depth"] = np.where(rng.random(n) < 0.3, np.inf, rng.random(n) * 10)
+    return g
