@@ -59,6 +59,19 @@ const char *st_last_error(void);
 /* Parse a scene blob, build its BVHs (the reference's SAH builder unless the
    blob carries prebuilt ones) and copy it to HBM. */
 int st_scene_create(const void *blob, size_t size, StSceneHandle **out);
+#define ST_SCENE_MESH_LBVH 1u  /* build meshes without a prebuilt BVH on the device (Morton LBVH, leaves <= 4) */
+
+/* st_scene_create with options: ST_SCENE_MESH_LBVH replaces the reference's
+   host SAH mesh build (updateMeshBVH, builder_top_down.h:272-364), which
+   takes minutes for meshes of millions of triangles, with a device build
+   of milliseconds. */
+int st_scene_create_ex(const void *blob, size_t size, uint32_t flags, StSceneHandle **out);
+
+/* Mesh m's BVH node slots and Triangle records (leaf order) as resident on
+   the device (reference layouts). node_cap: capacity of nodes. */
+int st_scene_mesh_bvh(const StSceneHandle *scene, uint32_t m, StBVHNodeRec *nodes, uint32_t node_cap,
+                      StTriangleRec *tris, uint32_t *node_count, uint32_t *height);
+
 /* Re-copy the host-side scene to the device (after st_scene_* edits). */
 int st_scene_upload(StSceneHandle *scene);
 void st_scene_destroy(StSceneHandle *scene);

@@ -1540,6 +1540,13 @@ static uint32_t build_bvh(OBuilder *b, uint32_t N, uint32_t max_leaf, ONode *nod
     return node_count;
 }
 
+/* Mesh builder used by scene_load: 0 = the reference's SAH (leaves <= 4),
+   1 = the Morton LBVH with leaves of up to 4 triangles (the device mesh
+   builder's specification). */
+static int g_mesh_lbvh = 0;
+static uint32_t lbvh_nodes(const Box3 *bx, uint32_t n, uint32_t max_leaf, ONode *on, uint32_t *leaf_ids,
+                           uint32_t *height);
+
 /* builder_top_down.h:272-364 */
 static int build_mesh(OMesh *mesh, const float *pos, const uint32_t *pidx, const float *nrm, const uint32_t *nidx,
                       const float *uvs, const uint32_t *uidx) {
@@ -1572,7 +1579,8 @@ static int build_mesh(OMesh *mesh, const float *pos, const uint32_t *pidx, const
     mesh->nodes = (ONode *)calloc(2 * (size_t)T + 2, sizeof(ONode));
     mesh->leaf_ids = (uint32_t *)calloc(T + 1, 4);
     mesh->triangles = (OTri *)calloc(T + 1, sizeof(OTri));
-    mesh->node_count = build_bvh(b, T, 4, mesh->nodes, mesh->leaf_ids, &mesh->height);
+    if (g_mesh_lbvh) mesh->node_count = lbvh_nodes(b->leaf_boxes, T, 4, mesh->nodes, mesh->leaf_ids, &mesh->height);
+    else mesh->node_count = build_bvh(b, T, 4, mesh->nodes, mesh->leaf_ids, &mesh->height);
     uint32_t *order = b->ids;
     memcpy(order, mesh->leaf_ids, T * sizeof(uint32_t));
     for (uint32_t i = 0; i < T; i++) {
@@ -1977,9 +1985,11 @@ int oracle_scene_bvh(const void *blob, size_t size, StBVHNodeRec *nodes, uint32_
     return 0;
 }
 
-int oracle_mesh_bvh(const void *blob, size_t size, uint32_t m, StBVHNodeRec *nodes, StTriangleRec *tris,
-                    uint32_t *leaf_ids, uint32_t *node_count, uint32_t *height) {
+static int mesh_bvh_export(const void *blob, size_t size, uint32_t m, StBVHNodeRec *nodes, StTriangleRec *tris,
+                           uint32_t *leaf_ids, uint32_t *node_count, uint32_t *height, int lbvh) {
+    g_mesh_lbvh = lbvh;
     OScene *sc = scene_load(blob, size);
+    g_mesh_lbvh = 0;
     if (!sc || m >= sc->mesh_count) { scene_free(sc); return -1; }
     const OMesh *mesh = sc->meshes + m;
     for (uint32_t n = 0; n < mesh->node_count; n++) node_to_rec(nodes + n, mesh->nodes + n);
@@ -2007,6 +2017,16 @@ int oracle_mesh_bvh(const void *blob, size_t size, uint32_t m, StBVHNodeRec *nod
     *height = mesh->height;
     scene_free(sc);
     return 0;
+}
+
+int oracle_mesh_bvh(const void *blob, size_t size, uint32_t m, StBVHNodeRec *nodes, StTriangleRec *tris,
+                    uint32_t *leaf_ids, uint32_t *node_count, uint32_t *height) {
+    return mesh_bvh_export(blob, size, m, nodes, tris, leaf_ids, node_count, height, 0);
+}
+
+int oracle_mesh_lbvh(const void *blob, size_t size, uint32_t m, StBVHNodeRec *nodes, StTriangleRec *tris,
+                     uint32_t *leaf_ids, uint32_t *node_count, uint32_t *height) {
+    return mesh_bvh_export(blob, size, m, nodes, tris, leaf_ids, node_count, height, 1);
 }
 
 int oracle_ssb(const void *blob, size_t size, const StView *view, int32_t *rects, uint8_t *flags) {
@@ -2093,6 +2113,7 @@ typedef struct {
     const Box3 *boxes;      /* by original index */
     ONode *nodes;
     uint32_t max_depth;
+    uint32_t max_leaf;      /* ranges of at most this many become one leaf */
 } LBuild;
 
 static uint64_t lkey64(const LBuild *b, uint32_t pos) { return ((uint64_t)b->keys[pos].code << 32) | pos; }
@@ -2111,9 +2132,26 @@ static uint32_t lsplit(const LBuild *b, uint32_t first, uint32_t last) {
     return split;
 }
 
-static Box3 lbuild(LBuild *b, uint32_t k, uint32_t first, uint32_t last, uint32_t out, uint32_t depth) {
+/* box of the radix-tree node over [first, last]: merge(left, right) down to
+   single leaves, also inside ranges that become one leaf */
+static Box3 lbox(const LBuild *b, uint32_t first, uint32_t last) {
+    if (first == last) return b->boxes[b->keys[first].index];
     uint32_t g = lsplit(b, first, last);
+    Box3 l = lbox(b, first, g), r = lbox(b, g + 1, last);
+    return merge_boxes(l, r);
+}
+
+static Box3 lbuild(LBuild *b, uint32_t k, uint32_t first, uint32_t last, uint32_t out, uint32_t depth) {
     ONode *node = b->nodes + out;
+    if (last - first + 1 <= b->max_leaf) {
+        node->aabb = lbox(b, first, last);
+        node->child_count = (uint16_t)(last - first + 1);
+        node->first_child_id = first;
+        node->depth = (uint16_t)depth;
+        if (depth + 1 > b->max_depth) b->max_depth = depth + 1;
+        return node->aabb;
+    }
+    uint32_t g = lsplit(b, first, last);
     node->first_child_id = 2 * k + 1;
     node->child_count = 0;
     node->depth = (uint16_t)depth;
@@ -2138,14 +2176,12 @@ static Box3 lbuild(LBuild *b, uint32_t k, uint32_t first, uint32_t last, uint32_
     return node->aabb;
 }
 
-uint32_t oracle_lbvh_build(const float *boxes, uint32_t n, StBVHNodeRec *nodes, uint32_t *leaf_ids, uint32_t *height) {
-    if (n == 0) { *height = 0; return 0; }
-    Box3 *bx = (Box3 *)calloc(n, sizeof(Box3));
+/* Morton LBVH over n boxes; leaves hold up to max_leaf sorted positions.
+   Writes 2n-1 node slots (slots under a range that became one leaf stay
+   zero) and the sorted order; returns the slot count. */
+static uint32_t lbvh_nodes(const Box3 *bx, uint32_t n, uint32_t max_leaf, ONode *on, uint32_t *leaf_ids,
+                           uint32_t *height) {
     LKey *keys = (LKey *)calloc(n, sizeof(LKey));
-    for (uint32_t i = 0; i < n; i++) {
-        bx[i].min = v3(boxes[6 * i], boxes[6 * i + 1], boxes[6 * i + 2]);
-        bx[i].max = v3(boxes[6 * i + 3], boxes[6 * i + 4], boxes[6 * i + 5]);
-    }
     V3 cmin = v3s(INFINITY), cmax = v3s(-INFINITY);
     V3 *cent = (V3 *)calloc(n, sizeof(V3));
     for (uint32_t i = 0; i < n; i++) {
@@ -2170,7 +2206,6 @@ uint32_t oracle_lbvh_build(const float *boxes, uint32_t n, StBVHNodeRec *nodes, 
     }
     qsort(keys, n, sizeof(LKey), lkey_cmp);
     for (uint32_t j = 0; j < n; j++) leaf_ids[j] = keys[j].index;
-    ONode *on = (ONode *)calloc(2 * (size_t)n, sizeof(ONode));
     uint32_t count;
     if (n == 1) {
         on[0].aabb = bx[0];
@@ -2179,13 +2214,26 @@ uint32_t oracle_lbvh_build(const float *boxes, uint32_t n, StBVHNodeRec *nodes, 
         *height = 1;
         count = 1;
     } else {
-        LBuild b = {keys, bx, on, 0};
+        LBuild b = {keys, bx, on, 0, max_leaf ? max_leaf : 1};
         lbuild(&b, 0, 0, n - 1, 0, 0);
         *height = b.max_depth;
         count = 2 * n - 1;
     }
+    free(keys); free(cent);
+    return count;
+}
+
+uint32_t oracle_lbvh_build(const float *boxes, uint32_t n, StBVHNodeRec *nodes, uint32_t *leaf_ids, uint32_t *height) {
+    if (n == 0) { *height = 0; return 0; }
+    Box3 *bx = (Box3 *)calloc(n, sizeof(Box3));
+    for (uint32_t i = 0; i < n; i++) {
+        bx[i].min = v3(boxes[6 * i], boxes[6 * i + 1], boxes[6 * i + 2]);
+        bx[i].max = v3(boxes[6 * i + 3], boxes[6 * i + 4], boxes[6 * i + 5]);
+    }
+    ONode *on = (ONode *)calloc(2 * (size_t)n, sizeof(ONode));
+    uint32_t count = lbvh_nodes(bx, n, 1, on, leaf_ids, height);
     for (uint32_t i = 0; i < count; i++) node_to_rec(nodes + i, on + i);
-    free(bx); free(keys); free(cent); free(on);
+    free(bx); free(on);
     return count;
 }
 

@@ -46,6 +46,13 @@ int oracle_scene_bvh(const void *blob, size_t size, StBVHNodeRec *nodes, uint32_
 int oracle_mesh_bvh(const void *blob, size_t size, uint32_t m, StBVHNodeRec *nodes,
                     StTriangleRec *tris, uint32_t *leaf_ids, uint32_t *node_count, uint32_t *height);
 
+/* Mesh BVH of mesh m built as the Morton LBVH with leaves of up to 4
+   triangles (the device mesh builder's specification), and the triangle
+   records in its leaf order. nodes: 2*tris slots; slots under a range that
+   became one leaf are zero. */
+int oracle_mesh_lbvh(const void *blob, size_t size, uint32_t m, StBVHNodeRec *nodes, StTriangleRec *tris,
+                     uint32_t *leaf_ids, uint32_t *node_count, uint32_t *height);
+
 /* Screen-space bounds (4 ints per primitive) and flags for a view. */
 int oracle_ssb(const void *blob, size_t size, const StView *view, int32_t *rects, uint8_t *flags);
 

@@ -92,13 +92,18 @@ def gen_mesh_scene(seed: int, triangles: int) -> bytes:
                       out, size)
 
 
-class DeviceScene:
-    """A scene resident in HBM (st_scene_create)."""
+ST_SCENE_MESH_LBVH = 1
 
-    def __init__(self, blob: bytes):
+
+class DeviceScene:
+    """A scene resident in HBM (st_scene_create_ex). mesh_lbvh: build meshes
+    without a prebuilt BVH on the device (Morton LBVH) instead of the host SAH."""
+
+    def __init__(self, blob: bytes, mesh_lbvh: bool = False):
         self._blob = np.frombuffer(blob, dtype=np.uint8)
         h = C.c_void_p()
-        _check(lib().st_scene_create(_ptr(self._blob), C.c_size_t(len(blob)), C.byref(h)))
+        _check(lib().st_scene_create_ex(_ptr(self._blob), C.c_size_t(len(blob)),
+                                        C.c_uint32(ST_SCENE_MESH_LBVH if mesh_lbvh else 0), C.byref(h)))
         self.handle = h
 
     def close(self):
@@ -124,6 +129,17 @@ class DeviceScene:
         cnt, h = C.c_uint32(), C.c_uint32()
         _check(lib().st_scene_bvh(self.handle, _ptr(nodes), _ptr(leaf), C.byref(cnt), C.byref(h)))
         return nodes[:cnt.value], leaf[:inf.prim_count], h.value
+
+    def mesh_bvh(self, m: int, triangles: int):
+        """(node slots, Triangle records in leaf order, height) of mesh m as resident."""
+        from .scene import TRIANGLE_DTYPE
+        cap = max(2 * triangles, 2)
+        nodes = np.zeros(cap, dtype=NODE_DTYPE)
+        tris = np.zeros(triangles, dtype=TRIANGLE_DTYPE)
+        cnt, h = C.c_uint32(), C.c_uint32()
+        _check(lib().st_scene_mesh_bvh(self.handle, C.c_uint32(m), _ptr(nodes), C.c_uint32(cap), _ptr(tris),
+                                       C.byref(cnt), C.byref(h)))
+        return nodes[:cnt.value], tris, h.value
 
     def set_prims(self, prims: np.ndarray, first: int = 0):
         """Replace primitive records [first, first+len(prims)) (st_scene_set_prims)."""

@@ -32,6 +32,8 @@
 
 #include "st_trace.h"
 #include "st_lbvh.h"
+#include "st_mesh_lbvh.h"
+#include "../st_mesh.h"
 #include "../host/st_host.h"
 #include "slimtracin.h"
 #include "st_internal.h"
@@ -349,18 +351,148 @@ static int staging_reserve(size_t bytes) {
 
 /* ---- resident scenes ---------------------------------------------------- */
 
-extern "C" int st_scene_create(const void *blob, size_t size, StSceneHandle **out) {
+static int build_device_meshes(StSceneHandle *h);
+
+extern "C" int st_scene_create_ex(const void *blob, size_t size, uint32_t flags, StSceneHandle **out) {
+    if (!out || !blob) return set_err(ST_ERR_ARG, "null argument");
     *out = NULL;
+    RT_LOCK;
     int rc = ensure_init();
     if (rc) return rc;
     const char *why = NULL;
-    StScene *sc = st_scene_from_blob(blob, size, &why);
+    StScene *sc = st_scene_from_blob_ex(blob, size, (flags & ST_SCENE_MESH_LBVH) ? 1u : 0u, &why);
     if (!sc) return set_err(ST_ERR_SCENE, "invalid scene: %s", why ? why : "unknown");
     StSceneHandle *h = (StSceneHandle *)calloc(1, sizeof(StSceneHandle));
+    if (!h) { st_scene_free(sc); return set_err(ST_ERR_MEMORY, "out of host memory"); }
     h->host = sc;
     rc = st_scene_upload(h);
+    if (!rc) rc = build_device_meshes(h);
     if (rc) { st_scene_destroy(h); return rc; }
     *out = h;
+    return ST_OK;
+}
+
+extern "C" int st_scene_create(const void *blob, size_t size, StSceneHandle **out) {
+    return st_scene_create_ex(blob, size, 0, out);
+}
+
+/* Device LBVH builds of the meshes the host left unbuilt: geometry goes to a
+   scratch buffer, nodes and triangle records straight into the scene's
+   allocation, then the heights come back for the traversal stack size. */
+static int build_device_meshes(StSceneHandle *h) {
+    StScene *sc = h->host;
+    size_t geom = 0, work = 0, sort_bytes = 0;
+    int any = 0;
+    for (uint32_t m = 0; m < sc->mesh_count; m++) {
+        if (!sc->mesh_on_device[m]) continue;
+        const StMeshPart *s = sc->deferred + m;
+        const StMeshRec *q = &s->rec;
+        size_t gb = round256(12ull * q->vertex_count) + round256(12ull * q->triangle_count) +
+                    (q->normals_count ? round256(12ull * q->normals_count) + round256(12ull * q->triangle_count) : 0) +
+                    (q->uvs_count ? round256(8ull * q->uvs_count) + round256(12ull * q->triangle_count) : 0);
+        size_t sb = 0;
+        size_t wb = st_mesh_lbvh_work_bytes(q->triangle_count, &sb);
+        if (gb > geom) geom = gb;
+        if (wb > work) { work = wb; sort_bytes = sb; }
+        any = 1;
+    }
+    if (!any) return ST_OK;
+    char *buf = NULL;
+    hipError_t e = hipMalloc(&buf, geom + work + 256);
+    if (e != hipSuccess) return set_err(ST_ERR_MEMORY, "hipMalloc(%zu) for mesh builds failed", geom + work);
+    uint32_t *height_d = (uint32_t *)(buf + geom + work);
+    int rc = ST_OK;
+    uint32_t max_height = 0;
+    for (uint32_t m = 0; m < sc->mesh_count && !rc; m++) {
+        if (!sc->mesh_on_device[m]) { if (sc->meshes[m].height > max_height) max_height = sc->meshes[m].height; continue; }
+        const StMeshPart *s = sc->deferred + m;
+        const StMeshRec *q = &s->rec;
+        char *p = buf;
+        StMeshGeomDev in;
+        memset(&in, 0, sizeof in);
+        in.T = q->triangle_count;
+        struct { const void *src; size_t bytes; const void **dst; } up[6] = {
+            {s->pos, 12ull * q->vertex_count, (const void **)&in.pos},
+            {s->pidx, 12ull * q->triangle_count, (const void **)&in.pidx},
+            {q->normals_count ? s->nrm : NULL, 12ull * q->normals_count, (const void **)&in.nrm},
+            {q->normals_count ? s->nidx : NULL, 12ull * q->triangle_count, (const void **)&in.nidx},
+            {q->uvs_count ? s->uvs : NULL, 8ull * q->uvs_count, (const void **)&in.uvs},
+            {q->uvs_count ? s->uidx : NULL, 12ull * q->triangle_count, (const void **)&in.uidx},
+        };
+        for (int k = 0; k < 6 && !rc; k++) {
+            if (!up[k].src || !up[k].bytes) continue;
+            if (hipMemcpyAsync(p, up[k].src, up[k].bytes, hipMemcpyHostToDevice, g.stream) != hipSuccess)
+                rc = set_err(ST_ERR_HIP, "mesh geometry upload failed");
+            *up[k].dst = p;
+            p += round256(up[k].bytes);
+        }
+        if (rc) break;
+        const StDMesh &dm = sc->meshes[m];
+        uint32_t *nodes = (uint32_t *)h->dev.mesh_nodes + 8ull * dm.node_offset;
+        StDTriHot *hot = (StDTriHot *)h->dev.tri_hot + dm.tri_offset;
+        StDTriCold *cold = (StDTriCold *)h->dev.tri_cold + dm.tri_offset;
+        if (st_mesh_lbvh_build(g.stream, in, buf + geom, sort_bytes, nodes, hot, cold, height_d)) {
+            rc = set_err(ST_ERR_HIP, "mesh %u: device BVH build failed: %s", m, hipGetErrorString(hipGetLastError()));
+            break;
+        }
+        uint32_t height = 0;
+        if (hipMemcpyAsync(&height, height_d, 4, hipMemcpyDeviceToHost, g.stream) != hipSuccess ||
+            hipStreamSynchronize(g.stream) != hipSuccess) {
+            rc = set_err(ST_ERR_HIP, "mesh %u: device BVH build failed", m);
+            break;
+        }
+        if (height == 0 || height + 2 > ST_MESH_STACK_MAX) {
+            rc = set_err(ST_ERR_SCENE, "mesh %u: BVH height %u exceeds the device traversal stack", m, height);
+            break;
+        }
+        sc->meshes[m].height = height;
+        if (height > max_height) max_height = height;
+        memset(sc->deferred + m, 0, sizeof(StMeshPart));
+    }
+    (void)hipStreamSynchronize(g.stream);
+    (void)hipFree(buf);
+    if (rc) return rc;
+    sc->max_mesh_height = max_height;
+    sc->mesh_stack_size = (uint8_t)(max_height + 2); /* core/init.h:523-527 */
+    h->dev.mesh_stack_size = sc->mesh_stack_size;
+    HIP_TRY(hipMemcpy((void *)h->dev.meshes, sc->meshes, sc->mesh_count * sizeof(StDMesh), hipMemcpyHostToDevice));
+    return ST_OK;
+}
+
+/* Mesh m's BVH node slots and Triangle records (leaf order) as resident on
+   the device. A device-built mesh has 2T-1 slots (1 for one triangle);
+   slots below a multi-triangle leaf are unused. */
+extern "C" int st_scene_mesh_bvh(const StSceneHandle *h, uint32_t m, StBVHNodeRec *nodes, uint32_t node_cap,
+                                 StTriangleRec *tris, uint32_t *node_count, uint32_t *height) {
+    RT_LOCK;
+    if (!h || !h->host || !h->dbuf || !nodes || !tris || !node_count || !height)
+        return set_err(ST_ERR_ARG, "null argument");
+    const StScene *sc = h->host;
+    if (m >= sc->mesh_count) return set_err(ST_ERR_ARG, "no mesh %u", m);
+    const StDMesh &dm = sc->meshes[m];
+    uint32_t next = (m + 1 < sc->mesh_count) ? sc->meshes[m + 1].node_offset : sc->mesh_node_total;
+    uint32_t count = sc->mesh_on_device[m] ? (dm.triangle_count > 1 ? 2 * dm.triangle_count - 1 : 1)
+                                           : next - dm.node_offset;
+    if (!sc->mesh_on_device[m]) {
+        /* host-built: trailing unused slots (odd-offset padding) are not part of the tree */
+        while (count > 1 && sc->mesh_nodes[dm.node_offset + count - 1].first_child_id == 0 &&
+               sc->mesh_nodes[dm.node_offset + count - 1].child_count == 0 &&
+               sc->mesh_nodes[dm.node_offset + count - 1].depth == 0)
+            count--;
+    }
+    if (count > node_cap) return set_err(ST_ERR_ARG, "node buffer holds %u of %u slots", node_cap, count);
+    HIP_TRY(hipStreamSynchronize(g.stream));
+    HIP_TRY(hipMemcpy(nodes, (const uint32_t *)h->dev.mesh_nodes + 8ull * dm.node_offset, 32ull * count,
+                      hipMemcpyDeviceToHost));
+    std::vector<StDTriHot> hot(dm.triangle_count);
+    std::vector<StDTriCold> cold(dm.triangle_count);
+    HIP_TRY(hipMemcpy(hot.data(), h->dev.tri_hot + dm.tri_offset, sizeof(StDTriHot) * dm.triangle_count,
+                      hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(cold.data(), h->dev.tri_cold + dm.tri_offset, sizeof(StDTriCold) * dm.triangle_count,
+                      hipMemcpyDeviceToHost));
+    for (uint32_t t = 0; t < dm.triangle_count; t++) st_tri_join(&hot[t], &cold[t], tris + t);
+    *node_count = count;
+    *height = dm.height;
     return ST_OK;
 }
 

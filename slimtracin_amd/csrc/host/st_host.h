@@ -53,19 +53,25 @@ typedef struct StScene {
     uint32_t has_emissive_quads;
     uint32_t mesh_stack_size;
     uint32_t scene_real_height;    /* levels of the scene BVH as walked */
+    /* meshes whose BVH and triangle records are built on the device: their
+       host node/triangle ranges stay zero; geometry points into the caller's
+       scene data and is valid only while the scene is being created */
+    uint8_t *mesh_on_device;
+    struct StMeshPart *deferred;
 } StScene;
 
 /* A scene as separate arrays in the blob record layouts (st_blob.h); the
    pointers are borrowed. Meshes with rec.prebuilt set carry their BVH and
    Triangle records (nodes, tris) and may omit the geometry (pos/pidx NULL):
    such a mesh is device-ready, as uploadMeshBVHs hands it over. */
-typedef struct StMeshPart {
+typedef struct StMeshPart StMeshPart;
+struct StMeshPart {
     StMeshRec rec;
     const float *pos, *nrm, *uvs;
     const uint32_t *pidx, *nidx, *uidx;
     const StBVHNodeRec *nodes;
     const StTriangleRec *tris;
-} StMeshPart;
+};
 
 typedef struct StMipPart {
     uint16_t width, height;
@@ -89,6 +95,8 @@ typedef struct StSceneParts {
     const StBVHNodeRec *scene_nodes;
     uint32_t scene_node_count, scene_height;
     const uint32_t *scene_leaf_ids;
+    /* leave meshes without a prebuilt BVH for the device builder */
+    uint32_t defer_mesh_build;
 } StSceneParts;
 
 /* Validate the parts and build the device-ready scene. Returns NULL on
@@ -97,6 +105,7 @@ StScene *st_scene_from_parts(const StSceneParts *parts, const char **err);
 
 /* Parse a blob into parts and build its BVHs (st_scene_from_parts). */
 StScene *st_scene_from_blob(const void *blob, size_t size, const char **err);
+StScene *st_scene_from_blob_ex(const void *blob, size_t size, uint32_t defer_mesh_build, const char **err);
 
 /* Device material record of a blob material record. */
 StDMat st_mat_from_rec(const StMaterialRec *q);

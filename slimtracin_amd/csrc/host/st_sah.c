@@ -28,6 +28,7 @@
 #include <string.h>
 
 #include "st_host.h"
+#include "../st_mesh.h"
 #include "slimtracin.h"
 
 typedef struct {
@@ -241,73 +242,17 @@ int st_build_mesh(uint32_t T, const float *pos, const uint32_t *pidx, const floa
                   uint32_t normals_count, const float *uvs, const uint32_t *uidx, uint32_t uvs_count,
                   StBVHNodeRec *nodes, uint32_t *node_count, uint32_t *height, StTriangleRec *tris,
                   uint32_t *leaf_ids) {
-    const float EPS = 0.0001f;
     sbox *boxes = (sbox *)calloc((size_t)T + 1, sizeof(sbox));
     uint32_t *order = (uint32_t *)malloc(((size_t)T + 1) * sizeof(uint32_t));
     if (!boxes || !order) { free(boxes); free(order); return -1; }
-    for (uint32_t i = 0; i < T; i++) {
-        const float *v1 = pos + 3 * (size_t)pidx[3 * (size_t)i];
-        const float *v2 = pos + 3 * (size_t)pidx[3 * (size_t)i + 1];
-        const float *v3 = pos + 3 * (size_t)pidx[3 * (size_t)i + 2];
-        sbox *bx = boxes + i;
-        bx->min.x = v2[0] < v3[0] ? v2[0] : v3[0];
-        bx->min.y = v2[1] < v3[1] ? v2[1] : v3[1];
-        bx->min.z = v2[2] < v3[2] ? v2[2] : v3[2];
-        bx->max.x = v2[0] > v3[0] ? v2[0] : v3[0];
-        bx->max.y = v2[1] > v3[1] ? v2[1] : v3[1];
-        bx->max.z = v2[2] > v3[2] ? v2[2] : v3[2];
-        bx->min.x = bx->min.x < v1[0] ? bx->min.x : v1[0];
-        bx->min.y = bx->min.y < v1[1] ? bx->min.y : v1[1];
-        bx->min.z = bx->min.z < v1[2] ? bx->min.z : v1[2];
-        bx->max.x = bx->max.x > v1[0] ? bx->max.x : v1[0];
-        bx->max.y = bx->max.y > v1[1] ? bx->max.y : v1[1];
-        bx->max.z = bx->max.z > v1[2] ? bx->max.z : v1[2];
-        float d;
-        d = bx->max.x - bx->min.x; if (d < 0) d = -d;
-        if (d < EPS) { bx->min.x -= EPS; bx->max.x += EPS; }
-        d = bx->max.y - bx->min.y; if (d < 0) d = -d;
-        if (d < EPS) { bx->min.y -= EPS; bx->max.y += EPS; }
-        d = bx->max.z - bx->min.z; if (d < 0) d = -d;
-        if (d < EPS) { bx->min.z -= EPS; bx->max.z += EPS; }
-    }
+    for (uint32_t i = 0; i < T; i++)
+        boxes[i] = st_tri_box(pos + 3 * (size_t)pidx[3 * (size_t)i], pos + 3 * (size_t)pidx[3 * (size_t)i + 1],
+                              pos + 3 * (size_t)pidx[3 * (size_t)i + 2]);
     *node_count = st_sah_build(boxes, T, 4, nodes, order, height);
     for (uint32_t i = 0; i < T; i++) {
         uint32_t tid = order[i];
         leaf_ids[tid] = i;
-        StTriangleRec *t = tris + i;
-        memset(t, 0, sizeof *t);
-        const float *a = pos + 3 * (size_t)pidx[3 * (size_t)tid];
-        const float *bb = pos + 3 * (size_t)pidx[3 * (size_t)tid + 1];
-        const float *c = pos + 3 * (size_t)pidx[3 * (size_t)tid + 2];
-        sv3 v1 = V3(a[0], a[1], a[2]), v2 = V3(bb[0], bb[1], bb[2]), v3 = V3(c[0], c[1], c[2]);
-        sm3 m;
-        m.X = v_sub(v3, v1);
-        m.Y = v_sub(v2, v1);
-        m.Z = v_cross(m.X, m.Y);
-        t->area_of_parallelogram = v_len(m.Z);
-        m.Z = v_scale(m.Z, 1.0f / t->area_of_parallelogram);
-        sm3 w = m_inverse(m);
-        float wt[9] = {w.X.x, w.X.y, w.X.z, w.Y.x, w.Y.y, w.Y.z, w.Z.x, w.Z.y, w.Z.z};
-        memcpy(t->world_to_tangent, wt, sizeof wt);
-        t->normal[0] = m.Z.x; t->normal[1] = m.Z.y; t->normal[2] = m.Z.z;
-        t->position[0] = v1.x; t->position[1] = v1.y; t->position[2] = v1.z;
-        if (normals_count) {
-            for (int k = 0; k < 3; k++) {
-                const float *n = nrm + 3 * (size_t)nidx[3 * (size_t)tid + k];
-                t->vertex_normals[3 * k] = n[0];
-                t->vertex_normals[3 * k + 1] = n[1];
-                t->vertex_normals[3 * k + 2] = n[2];
-            }
-        }
-        if (uvs_count) {
-            const float *p0 = uvs + 2 * (size_t)uidx[3 * (size_t)tid];
-            const float *p1 = uvs + 2 * (size_t)uidx[3 * (size_t)tid + 1];
-            const float *p2 = uvs + 2 * (size_t)uidx[3 * (size_t)tid + 2];
-            t->uvs[0] = p0[0]; t->uvs[1] = p0[1];
-            t->uvs[2] = p1[0]; t->uvs[3] = p1[1];
-            t->uvs[4] = p2[0]; t->uvs[5] = p2[1];
-            t->area_of_uv = fabsf((p1[0] - p0[0]) * (p2[1] - p0[1]) - (p2[0] - p0[0]) * (p1[1] - p0[1]));
-        }
+        st_tri_record(tid, pos, pidx, normals_count ? nrm : NULL, nidx, uvs_count ? uvs : NULL, uidx, tris + i);
     }
     free(boxes);
     free(order);

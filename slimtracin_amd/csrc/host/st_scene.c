@@ -11,6 +11,7 @@
 #include <string.h>
 
 #include "st_host.h"
+#include "../st_mesh.h"
 
 typedef struct { const uint8_t *p, *end; int bad; } Rd;
 
@@ -29,6 +30,7 @@ void st_scene_free(StScene *sc) {
     free(sc->meshes); free(sc->mesh_aabbs); free(sc->mesh_nodes);
     free(sc->tri_hot); free(sc->tri_cold);
     free(sc->textures); free(sc->mips); free(sc->texels);
+    free(sc->mesh_on_device); free(sc->deferred);
     free(sc);
 }
 
@@ -129,18 +131,6 @@ int st_scene_rebuild_sah(StScene *sc) {
     return 0;
 }
 
-static void tri_to_device(const StTriangleRec *t, StDTriHot *h, StDTriCold *c) {
-    memset(h, 0, sizeof *h);
-    memset(c, 0, sizeof *c);
-    memcpy(h->w2t, t->world_to_tangent, 36);
-    memcpy(h->pos, t->position, 12);
-    memcpy(h->normal, t->normal, 12);
-    memcpy(c->vn, t->vertex_normals, 36);
-    memcpy(c->uv, t->uvs, 24);
-    c->area = t->area_of_parallelogram;
-    c->uv_area = t->area_of_uv;
-}
-
 StDMat st_mat_from_rec(const StMaterialRec *q) {
     StDMat m;
     memset(&m, 0, sizeof m);
@@ -237,6 +227,9 @@ StScene *st_scene_from_parts(const StSceneParts *pp, const char **err) {
     }
     sc->mesh_node_total = (uint32_t)node_total;
     sc->tri_total = (uint32_t)tri_total;
+    sc->mesh_on_device = (uint8_t *)calloc((size_t)pp->mesh_count + 1, 1);
+    sc->deferred = (StMeshPart *)calloc((size_t)pp->mesh_count + 1, sizeof(StMeshPart));
+    if (!sc->mesh_on_device || !sc->deferred) FAIL("out of host memory");
     sc->mesh_nodes = (StBVHNodeRec *)calloc(node_total + 1, sizeof(StBVHNodeRec));
     sc->tri_hot = (StDTriHot *)calloc(tri_total + 1, sizeof(StDTriHot));
     sc->tri_cold = (StDTriCold *)calloc(tri_total + 1, sizeof(StDTriCold));
@@ -246,12 +239,18 @@ StScene *st_scene_from_parts(const StSceneParts *pp, const char **err) {
         const StMeshRec *q = &s->rec;
         StDMesh *dm = sc->meshes + m;
         uint32_t height = 0, count = 0;
+        if (!q->prebuilt && pp->defer_mesh_build) {
+            sc->mesh_on_device[m] = 1;
+            sc->deferred[m] = *s;
+            dm->height = 0;
+            continue;
+        }
         if (q->prebuilt) {
             memcpy(sc->mesh_nodes + dm->node_offset, s->nodes, sizeof(StBVHNodeRec) * q->node_count);
             count = q->node_count;
             height = q->height;
             for (uint32_t t = 0; t < q->triangle_count; t++)
-                tri_to_device(s->tris + t, sc->tri_hot + dm->tri_offset + t, sc->tri_cold + dm->tri_offset + t);
+                st_tri_split(s->tris + t, sc->tri_hot + dm->tri_offset + t, sc->tri_cold + dm->tri_offset + t);
         } else {
             StTriangleRec *tris = (StTriangleRec *)malloc(sizeof(StTriangleRec) * q->triangle_count);
             uint32_t *leaf = (uint32_t *)malloc(4u * (size_t)q->triangle_count);
@@ -262,7 +261,7 @@ StScene *st_scene_from_parts(const StSceneParts *pp, const char **err) {
                 FAIL("mesh BVH build failed");
             }
             for (uint32_t t = 0; t < q->triangle_count; t++)
-                tri_to_device(tris + t, sc->tri_hot + dm->tri_offset + t, sc->tri_cold + dm->tri_offset + t);
+                st_tri_split(tris + t, sc->tri_hot + dm->tri_offset + t, sc->tri_cold + dm->tri_offset + t);
             free(tris);
             free(leaf);
         }
@@ -343,6 +342,10 @@ StScene *st_scene_from_parts(const StSceneParts *pp, const char **err) {
 #undef FAIL
 
 StScene *st_scene_from_blob(const void *blob, size_t size, const char **err) {
+    return st_scene_from_blob_ex(blob, size, 0, err);
+}
+
+StScene *st_scene_from_blob_ex(const void *blob, size_t size, uint32_t defer_mesh_build, const char **err) {
     const char *dummy;
     if (!err) err = &dummy;
     *err = NULL;
@@ -420,6 +423,7 @@ StScene *st_scene_from_blob(const void *blob, size_t size, const char **err) {
         pp.scene_leaf_ids = (const uint32_t *)take(&r, 4u * (size_t)h->prim_count);
         if (r.bad) { *err = "truncated scene BVH"; goto out; }
     }
+    pp.defer_mesh_build = defer_mesh_build;
     sc = st_scene_from_parts(&pp, err);
 out:
     for (uint32_t t = 0; t < h->texture_count; t++) free(mip_arrays ? mip_arrays[t] : NULL);

@@ -273,3 +273,49 @@ def blob_records(blob: bytes) -> dict:
     lights = bytes(blob[o:o + 28 * L])
     return dict(prim_count=P, material_count=M, light_count=L, mesh_count=NM, texture_count=NT, ambient=ambient,
                 prims=prims, materials=mats, lights=lights)
+
+
+def blob_mesh_info(blob: bytes) -> list:
+    """Per mesh: dict of the StMeshRec counts and the byte offset of its record."""
+    magic, version, P, M, L, NM, NT = struct.unpack_from("<7I", blob, 0)
+    o = BLOB_HEADER_BYTES + P * PRIM_DTYPE.itemsize + 84 * M + 28 * L
+    out = []
+    for _ in range(NM):
+        V, T, N, U, pre, nc, h, _r = struct.unpack_from("<8I", blob, o)
+        out.append(dict(offset=o, vertex_count=V, triangle_count=T, normals_count=N, uvs_count=U, prebuilt=pre,
+                        node_count=nc, height=h))
+        o += 56 + 12 * V + 12 * T
+        if N:
+            o += 12 * N + 12 * T
+        if U:
+            o += 8 * U + 12 * T
+        if pre:
+            o += 32 * nc + 128 * T + 4 * T
+    return out
+
+
+def blob_with_prebuilt_meshes(blob: bytes, builds: list) -> bytes:
+    """The same scene with each mesh carrying a prebuilt BVH: builds[m] =
+    (nodes, triangles, leaf_ids, height) or None to keep mesh m as it is."""
+    info = blob_mesh_info(blob)
+    out = bytearray()
+    pos = 0
+    for m, mi in enumerate(info):
+        o = mi["offset"]
+        out += blob[pos:o]
+        T, V, N, U = mi["triangle_count"], mi["vertex_count"], mi["normals_count"], mi["uvs_count"]
+        end = o + 56 + 12 * V + 12 * T + (12 * N + 12 * T if N else 0) + (8 * U + 12 * T if U else 0)
+        body_end = end + (32 * mi["node_count"] + 132 * T if mi["prebuilt"] else 0)
+        if builds[m] is None:
+            out += blob[o:body_end]
+        else:
+            nodes, tris, leaf, h = builds[m]
+            rec = bytearray(blob[o:o + 56])
+            struct.pack_into("<3I", rec, 16, 1, len(nodes), int(h))
+            out += rec + blob[o + 56:end]
+            out += np.ascontiguousarray(nodes, dtype=NODE_DTYPE).tobytes()
+            out += np.ascontiguousarray(tris, dtype=TRIANGLE_DTYPE).tobytes()
+            out += np.ascontiguousarray(leaf, dtype=np.uint32).tobytes()
+        pos = body_end
+    out += blob[pos:]
+    return bytes(out)

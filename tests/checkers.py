@@ -269,3 +269,19 @@ def resolve_test_grid(rng, w1, h1):
     g["opacity"] = np.where(rng.random(n) < 0.8, np.float32(1.0), rng.random(n).astype(np.float32))
     g["depth"] = np.where(rng.random(n) < 0.3, np.inf, rng.random(n) * 10)
     return g
+
+
+def oracle_mesh_lbvh(blob, m, tri_count):
+    """Morton LBVH (leaves <= 4) of mesh m and its triangles in leaf order."""
+    return _mesh_bvh(_load(ORACLE_SO), "oracle_mesh_lbvh", blob, m, tri_count)
+
+
+def reachable_nodes(nodes):
+    """Slots reachable from the root (node 0) of a reference-layout BVH."""
+    seen, stack = [], [0]
+    while stack:
+        o = stack.pop()
+        seen.append(o)
+        if nodes["child_count"][o] == 0:
+            stack += [int(nodes["first_child_id"][o]), int(nodes["first_child_id"][o]) + 1]
+    return np.array(sorted(seen), dtype=np.int64)
