@@ -11,10 +11,20 @@ uploads), the primary-direction kernel and the render kernel; the frame
 stays in HBM (the PCIe-inclusive rate is reported separately).
 
 Also measured in the same run:
-  - plain vs batched launches: 64 mixed requests over 4 resident scenes,
-    first as 64 sequential per-request launches with a blocking download
-    each (the reference's renderSceneOnGPU pattern), then as one persistent
-    launch over a global tile queue plus the downloads;
+  - plain vs batched launches ("batched"): 256 requests over 8 resident
+    scenes of 0.2-5M triangles (noise-displaced spheres, mesh BVHs built on
+    the device), resolutions log-uniform 128x128..1920x1080, trace depth
+    uniform 1-6, mode uniform over Beauty/Depth/Normal/UV, cameras on a
+    circle around each scene; first as 256 sequential per-request launches
+    with a blocking FloatPixel download each (the reference's
+    renderSceneOnGPU pattern), then as one persistent launch over one tile
+    list, with FloatPixel downloads and with the window images resolved on
+    the device (BGRA8); the batched frames are checked equal to the plain
+    ones;
+  - serving ("served"): 40 resident scenes of 1-20M triangles (log-uniform,
+    admitted against a 200 GB HBM budget), requests of the same mix arriving
+    as a Poisson process at a stated rate for a stated time, served by the
+    continuous-batching loop; Mrays/s, p50/p99 submit-to-image latency;
   - roofline of the render kernel: algorithmic bytes of the frame from the
     CPU oracle's event counters on the same 1920x1080 frame (bytes per event
     at the reference's record sizes), over the kernel's HIP-event duration;
@@ -55,12 +65,13 @@ def pbr_view(w, h):
     return make_view(width=w, height=h, mode=MODE_BEAUTY, depth=4, **PBR_CAM)
 
 
-def batched_workload(rng, nscenes):
+def request_mix(rng, nscenes, count):
     """Mixed requests: log-uniform resolution 128x128..1920x1080, depth 1-6,
-    mode uniform over Beauty/Depth/Normal/UV, camera on a sphere."""
+    mode uniform over Beauty/Depth/Normal/UV, camera on a circle around the
+    scene looking at it."""
     views = []
     scene_ids = []
-    for i in range(64):
+    for i in range(count):
         w = int(round(np.exp(rng.uniform(np.log(128), np.log(1920)))))
         h = int(round(np.exp(rng.uniform(np.log(128), np.log(1080)))))
         mode = int(rng.choice([MODE_BEAUTY, MODE_DEPTH, MODE_NORMALS, MODE_UVS]))
@@ -122,6 +133,158 @@ def copy_probe(mib=512, reps=5):
         print(f"copy probe {i}: {mib} MiB read, {mib} MiB written", flush=True)
 
 
+BATCH_TRIANGLES = [200_000, 320_000, 500_000, 800_000, 1_250_000, 2_000_000, 3_200_000, 5_000_000]
+
+
+def batched_leg(args):
+    rng = np.random.default_rng(7)
+    dscenes = []
+    t0 = time.perf_counter()
+    for i, tri in enumerate(BATCH_TRIANGLES):
+        dscenes.append(st.DeviceScene(st.gen_mesh_scene(100 + i, tri), mesh_lbvh=True))
+    build_s = time.perf_counter() - t0
+    views, sids = request_mix(rng, len(dscenes), 256)
+    req_scenes = [dscenes[i] for i in sids]
+    st.render_batch(req_scenes[:16], views[:16], persistent=True)          # warm-up
+    lat_plain, plain_rays, plain_pix = [], 0, []
+    tp = time.perf_counter()
+    for i in range(len(views)):
+        r0 = time.perf_counter()
+        px, _, c = st.render(req_scenes[i], views[i:i + 1], hits=False)
+        lat_plain.append((time.perf_counter() - r0) * 1e3)
+        plain_rays += c.total()
+        plain_pix.append(px)
+    plain_s = time.perf_counter() - tp
+    # page-locked outputs, allocated once outside the timed regions (as the
+    # serving loop's buffer pool does)
+    grid = [(int(v["width"]) + 1) * (int(v["height"]) + 1) for v in views]
+    win = [int(v["width"]) * int(v["height"]) for v in views]
+    fp_out = st.pinned_split(grid, st.PIXEL_DTYPE)
+    bgra_out = st.pinned_split(win, np.uint32)
+    tq = time.perf_counter()
+    for i in range(len(views)):
+        st.render(req_scenes[i], views[i:i + 1], hits=False, out=fp_out[i])
+    plain_pinned_s = time.perf_counter() - tq
+    tb = time.perf_counter()
+    bpix, _, cb = st.render_batch(req_scenes, views, persistent=True, outs=fp_out)
+    batch_s = time.perf_counter() - tb
+    tr = time.perf_counter()
+    _, cr = st.render_batch_bgra8(req_scenes, views, persistent=True, outs=bgra_out)
+    served_s = time.perf_counter() - tr
+    kernel_ms, _ = st.render_timed(req_scenes, views)
+    batch_rays = sum(c.total() for c in cb)
+    assert batch_rays == plain_rays == sum(c.total() for c in cr), (batch_rays, plain_rays)
+    same = all(a.tobytes() == b.tobytes() for a, b in zip(plain_pix, bpix))
+    out = {
+        "requests": len(views), "scenes": len(dscenes),
+        "scene_triangles": [d.info().triangle_count for d in dscenes],
+        "mesh_bvh": "device Morton LBVH (leaves <= 4)", "scene_build_s": round(build_s, 2),
+        "traced_pixels": int(sum((int(v["width"]) + 1) * (int(v["height"]) + 1) for v in views)),
+        "rays": int(batch_rays),
+        "plain_mrays": round(plain_rays / plain_s / 1e6, 2),
+        "plain_pinned_mrays": round(plain_rays / plain_pinned_s / 1e6, 2),
+        "batched_mrays": round(batch_rays / batch_s / 1e6, 2),
+        "batched_bgra8_mrays": round(batch_rays / served_s / 1e6, 2),
+        "speedup": round(plain_s / batch_s, 3),
+        "speedup_bgra8": round(plain_s / served_s, 3),
+        "plain_wall_ms": round(plain_s * 1e3, 2),
+        "batched_wall_ms": round(batch_s * 1e3, 2),
+        "batched_bgra8_wall_ms": round(served_s * 1e3, 2),
+        "plain_latency_p50_ms": round(float(np.percentile(lat_plain, 50)), 3),
+        "plain_latency_p99_ms": round(float(np.percentile(lat_plain, 99)), 3),
+        "batched_equals_plain": same,
+        "batched_render_kernel_ms": round(kernel_ms, 3),
+    }
+    for d in dscenes:
+        d.close()
+    return out
+
+
+def served_leg(args):
+    import threading
+    from slimtracin_amd.server import OUTPUT_BGRA8, Server
+    rng = np.random.default_rng(11)
+    tri_counts = np.exp(rng.uniform(np.log(1e6), np.log(20e6), args.serve_scenes)).astype(np.int64)
+    srv = Server(hbm_budget=200 * 10**9, output=OUTPUT_BGRA8)
+    ids, admitted_tris, rejected = [], [], 0
+    t0 = time.perf_counter()
+    for i, tri in enumerate(tri_counts):
+        blob = st.gen_mesh_scene(500 + i, int(tri))
+        sid = srv.try_add_scene(blob)
+        if sid is None:
+            rejected += 1
+        else:
+            ids.append(sid)
+            admitted_tris.append(int(tri))
+        del blob
+        if (i + 1) % 8 == 0:
+            log(f"served: {i + 1}/{len(tri_counts)} scenes admitted-or-rejected, {time.perf_counter() - t0:.1f} s")
+    build_s = time.perf_counter() - t0
+
+    def burst(n):
+        vs, ps = request_mix(rng, len(ids), n)
+        tb = time.perf_counter()
+        ts = [srv.submit(ids[ps[k]], vs[k:k + 1]) for k in range(n)]
+        for t in ts:
+            srv.wait(t, copy=False)
+            srv.release(t)
+        return n / (time.perf_counter() - tb)
+
+    burst(32)                                       # warm-up
+    capacity = burst(128)                           # requests/s with a full queue
+    rate = args.serve_rate if args.serve_rate > 0 else round(0.6 * capacity, 1)
+    log(f"served: {len(ids)} scenes resident in {build_s:.1f} s; burst capacity {capacity:.0f} req/s; "
+        f"Poisson rate {rate} req/s for {args.serve_seconds} s")
+    stats0 = srv.stats()
+    n = max(1, int(rate * args.serve_seconds))
+    views, picks = request_mix(rng, len(ids), n)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, n))
+    tickets = [None] * n
+    lat = []
+    rays = [0]
+
+    def collector():
+        last = time.perf_counter()
+        for k in range(n):
+            while tickets[k] is None:
+                time.sleep(0.0002)
+            _, c, l_ms = srv.wait(tickets[k], copy=False)
+            rays[0] += c.primary + c.shadow + c.bounce
+            lat.append(l_ms)
+            srv.release(tickets[k])
+            if time.perf_counter() - last > 1.0:
+                last = time.perf_counter()
+                log(f"served: {k + 1}/{n} requests done")
+
+    th = threading.Thread(target=collector)
+    th.start()
+    start = time.perf_counter()
+    for k in range(n):
+        while True:
+            gap = arrivals[k] - (time.perf_counter() - start)
+            if gap <= 0:
+                break
+            time.sleep(gap if gap > 0.001 else 0)
+        tickets[k] = srv.submit(ids[picks[k]], views[k:k + 1])
+    th.join()
+    wall = time.perf_counter() - start
+    stats = srv.stats()
+    srv.close()
+    return {
+        "scenes_requested": int(args.serve_scenes), "scenes_admitted": len(ids), "scenes_rejected": rejected,
+        "scene_triangles_total": int(sum(admitted_tris)), "hbm_budget_gb": 200,
+        "resident_gb": round(stats["resident_bytes"] / 1e9, 2), "scene_build_s": round(build_s, 1),
+        "mesh_bvh": "device Morton LBVH (leaves <= 4)", "output": "BGRA8 window image (device resolve)",
+        "burst_capacity_req_per_s": round(capacity, 1), "arrival_rate_per_s": rate, "requests": n,
+        "wall_s": round(wall, 3), "mrays": round(rays[0] / wall / 1e6, 2),
+        "latency_p50_ms": round(float(np.percentile(lat, 50)), 3),
+        "latency_p99_ms": round(float(np.percentile(lat, 99)), 3),
+        "latency_max_ms": round(float(np.max(lat)), 3),
+        "launches": int(stats["launches"] - stats0["launches"]),
+        "mean_batch": round((stats["completed"] - stats0["completed"]) / max(1, stats["launches"] - stats0["launches"]), 2),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -131,6 +294,11 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the plain-vs-batched leg")
     ap.add_argument("--cpu-frames", type=int, default=3, help="frames of the CPU baseline (median)")
     ap.add_argument("--copy-probe", action="store_true", help="only run the copy kernel used to calibrate PMC bytes")
+    ap.add_argument("--no-serve", action="store_true", help="skip the served (Poisson arrivals) leg")
+    ap.add_argument("--serve-rate", type=float, default=0.0,
+                    help="served leg: Poisson arrival rate, requests/s (0: 60%% of the measured burst capacity)")
+    ap.add_argument("--serve-seconds", type=float, default=8.0, help="served leg: arrival window, s")
+    ap.add_argument("--serve-scenes", type=int, default=40, help="served leg: resident scenes requested")
     args = ap.parse_args()
     if args.copy_probe:
         copy_probe()
@@ -195,40 +363,11 @@ def main():
     }
 
     if not args.no_batch:
-        rng = np.random.default_rng(7)
-        sblobs = [st.gen_mesh_scene(100 + i, tri) for i, tri in enumerate([200_000, 300_000, 500_000, 800_000])]
-        dscenes = [st.DeviceScene(b) for b in sblobs]
-        views, sids = batched_workload(rng, len(dscenes))
-        req_scenes = [dscenes[i] for i in sids]
-        # warm both paths once
-        st.render_batch(req_scenes, views, persistent=True)
-        lat_plain, plain_rays = [], 0
-        tp = time.perf_counter()
-        for i in range(len(views)):
-            r0 = time.perf_counter()
-            _, _, c = st.render(req_scenes[i], views[i:i + 1], hits=False)
-            lat_plain.append((time.perf_counter() - r0) * 1e3)
-            plain_rays += c.total()
-        plain_s = time.perf_counter() - tp
-        tb = time.perf_counter()
-        _, _, cb = st.render_batch(req_scenes, views, persistent=True)
-        batch_s = time.perf_counter() - tb
-        batch_rays = sum(c.total() for c in cb)
-        assert batch_rays == plain_rays, (batch_rays, plain_rays)
-        result["batched"] = {
-            "requests": len(views), "scenes": len(dscenes),
-            "scene_triangles": [d.info().triangle_count for d in dscenes],
-            "rays": int(batch_rays),
-            "plain_mrays": round(plain_rays / plain_s / 1e6, 2),
-            "batched_mrays": round(batch_rays / batch_s / 1e6, 2),
-            "speedup": round(plain_s / batch_s, 3),
-            "plain_latency_p50_ms": round(float(np.percentile(lat_plain, 50)), 3),
-            "plain_latency_p99_ms": round(float(np.percentile(lat_plain, 99)), 3),
-            "batched_wall_ms": round(batch_s * 1e3, 3),
-        }
+        result["batched"] = batched_leg(args)
         log("batched:", result["batched"])
-        for d in dscenes:
-            d.close()
+    if not args.no_serve:
+        result["served"] = served_leg(args)
+        log("served:", result["served"])
 
     if not args.no_cpu:
         threads = min(os.cpu_count() or 1, 16)

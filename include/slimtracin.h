@@ -109,6 +109,11 @@ int st_render_batch(StSceneHandle *const *scenes, const StView *views, uint32_t 
 int st_render_batch_bgra8(StSceneHandle *const *scenes, const StView *views, uint32_t n, uint32_t *const *bgra,
                           StRayCounts *counts, uint32_t flags);
 
+/* Page-locked host memory for frame outputs: downloads into it run at the
+   link's full rate (pageable destinations are staged by the runtime). */
+int st_host_alloc(size_t bytes, void **out);
+void st_host_free(void *p);
+
 /* The same display resolve of a caller's traced grid (grid_w x grid_h
    FloatPixels, host memory) into (grid_w-1) x (grid_h-1) BGRA8 pixels. */
 int st_resolve_bgra8(const StFloatPixel *grid, uint32_t grid_w, uint32_t grid_h, uint32_t *out);

@@ -153,21 +153,59 @@ class DeviceScene:
         return ms.value
 
 
-def render(scene: DeviceScene, view: np.ndarray, hits: bool = True):
-    """Plain per-request launch with blocking download (renderSceneOnGPU)."""
+class _Pinned:
+    """Owner of one st_host_alloc block."""
+
+    def __init__(self, nbytes):
+        p = C.c_void_p()
+        _check(lib().st_host_alloc(C.c_size_t(nbytes), C.byref(p)))
+        self.ptr = p.value
+
+    def __del__(self):
+        try:
+            lib().st_host_free(C.c_void_p(self.ptr))
+        except Exception:
+            pass
+
+
+def pinned_empty(n: int, dtype) -> np.ndarray:
+    """Uninitialised numpy array in page-locked host memory (st_host_alloc)."""
+    dtype = np.dtype(dtype)
+    owner = _Pinned(max(1, n * dtype.itemsize))
+    buf = (C.c_char * max(1, n * dtype.itemsize)).from_address(owner.ptr)
+    buf._owner = owner
+    return np.frombuffer(buf, dtype=dtype, count=n)
+
+
+def pinned_split(sizes, dtype) -> list:
+    """Arrays of the given lengths carved from one page-locked block."""
+    dtype = np.dtype(dtype)
+    offs, total = [], 0
+    for n in sizes:
+        offs.append(total)
+        total += (n * dtype.itemsize + 255) // 256 * 256
+    block = pinned_empty(max(total, 1), np.uint8)
+    return [block[o:o + n * dtype.itemsize].view(dtype) for o, n in zip(offs, sizes)]
+
+
+def render(scene: DeviceScene, view: np.ndarray, hits: bool = True, out: np.ndarray | None = None):
+    """Plain per-request launch with blocking download (renderSceneOnGPU);
+    out: a preallocated FloatPixel array to download into."""
     n = grid_size(view)
-    pixels = np.zeros(n, dtype=PIXEL_DTYPE)
+    pixels = out if out is not None else np.zeros(n, dtype=PIXEL_DTYPE)
+    assert len(pixels) >= n and pixels.dtype == PIXEL_DTYPE
     hrec = np.zeros(n, dtype=HIT_DTYPE) if hits else None
     counts = RayCounts()
     _check(lib().st_render(scene.handle, _ptr(view), _ptr(pixels), _ptr(hrec) if hits else None, C.byref(counts)))
     return pixels, hrec, counts
 
 
-def render_batch(scenes, views: np.ndarray, hits: bool = False, persistent: bool = True):
-    """Fuse len(views) requests into one launch; returns per-request outputs."""
+def render_batch(scenes, views: np.ndarray, hits: bool = False, persistent: bool = True, outs=None):
+    """Fuse len(views) requests into one launch; returns per-request outputs
+    (outs: preallocated FloatPixel arrays, e.g. from pinned_split)."""
     n = len(views)
     handles = (C.c_void_p * n)(*[s.handle for s in scenes])
-    pix = [np.zeros(grid_size(views[i:i + 1]), dtype=PIXEL_DTYPE) for i in range(n)]
+    pix = outs if outs is not None else [np.zeros(grid_size(views[i:i + 1]), dtype=PIXEL_DTYPE) for i in range(n)]
     hr = [np.zeros(grid_size(views[i:i + 1]), dtype=HIT_DTYPE) for i in range(n)] if hits else None
     pp = (C.c_void_p * n)(*[p.ctypes.data for p in pix])
     hp = (C.c_void_p * n)(*[h.ctypes.data for h in hr]) if hits else None
@@ -190,12 +228,14 @@ def render_timed(scenes, views: np.ndarray, persistent: bool = True):
     return ms.value, list(counts)
 
 
-def render_batch_bgra8(scenes, views: np.ndarray, persistent: bool = True):
+def render_batch_bgra8(scenes, views: np.ndarray, persistent: bool = True, outs=None):
     """Fuse the requests into one launch and resolve each frame on the device
-    to its width x height BGRA8 window image (uint32 per pixel)."""
+    to its width x height BGRA8 window image (uint32 per pixel); outs:
+    preallocated arrays (e.g. from pinned_split)."""
     n = len(views)
     handles = (C.c_void_p * n)(*[s.handle for s in scenes])
-    outs = [np.zeros(int(views["width"][i]) * int(views["height"][i]), dtype=np.uint32) for i in range(n)]
+    if outs is None:
+        outs = [np.zeros(int(views["width"][i]) * int(views["height"][i]), dtype=np.uint32) for i in range(n)]
     op = (C.c_void_p * n)(*[o.ctypes.data for o in outs])
     counts = (RayCounts * n)()
     views = np.ascontiguousarray(views, dtype=VIEW_DTYPE)

@@ -532,9 +532,9 @@ extern "C" int st_scene_upload(StSceneHandle *h) {
         {o_snodes + 32, sc->scene_nodes, (size_t)sc->scene_node_count * sizeof(StBVHNodeRec)},
         {o_sleaf, sc->scene_leaf_ids, (size_t)sc->prim_count * 4},
         {o_meshes, sc->meshes, sc->mesh_count * sizeof(StDMesh)},
-        {o_mnodes, sc->mesh_nodes, (size_t)sc->mesh_node_total * sizeof(StBVHNodeRec)},
-        {o_hot, sc->tri_hot, (size_t)sc->tri_total * sizeof(StDTriHot)},
-        {o_cold, sc->tri_cold, (size_t)sc->tri_total * sizeof(StDTriCold)},
+        {o_mnodes, sc->mesh_nodes, sc->mesh_host_data ? (size_t)sc->mesh_node_total * sizeof(StBVHNodeRec) : 0},
+        {o_hot, sc->tri_hot, sc->mesh_host_data ? (size_t)sc->tri_total * sizeof(StDTriHot) : 0},
+        {o_cold, sc->tri_cold, sc->mesh_host_data ? (size_t)sc->tri_total * sizeof(StDTriCold) : 0},
         {o_tex, sc->textures, sc->texture_count * sizeof(StDTex)},
         {o_mips, sc->mips, sc->mip_total * sizeof(StDMip)},
         {o_texels, sc->texels, sc->texel_bytes},
@@ -930,4 +930,17 @@ extern "C" int st_resolve_bgra8(const StFloatPixel *grid, uint32_t grid_w, uint3
     HIP_TRY(hipMemcpyAsync(out, out_d, opx * 4, hipMemcpyDeviceToHost, g.stream));
     HIP_TRY(hipStreamSynchronize(g.stream));
     return ST_OK;
+}
+
+extern "C" int st_host_alloc(size_t bytes, void **out) {
+    if (!out) return set_err(ST_ERR_ARG, "null argument");
+    *out = NULL;
+    int rc = ensure_init();
+    if (rc) return rc;
+    HIP_TRY(hipHostMalloc(out, bytes ? bytes : 1, hipHostMallocDefault));
+    return ST_OK;
+}
+
+extern "C" void st_host_free(void *p) {
+    if (p) (void)hipHostFree(p);
 }

@@ -58,6 +58,7 @@ typedef struct StScene {
        scene data and is valid only while the scene is being created */
     uint8_t *mesh_on_device;
     struct StMeshPart *deferred;
+    uint32_t mesh_host_data;       /* 0: mesh_nodes / tri_hot / tri_cold are not held on the host */
 } StScene;
 
 /* A scene as separate arrays in the blob record layouts (st_blob.h); the

@@ -230,9 +230,14 @@ StScene *st_scene_from_parts(const StSceneParts *pp, const char **err) {
     sc->mesh_on_device = (uint8_t *)calloc((size_t)pp->mesh_count + 1, 1);
     sc->deferred = (StMeshPart *)calloc((size_t)pp->mesh_count + 1, sizeof(StMeshPart));
     if (!sc->mesh_on_device || !sc->deferred) FAIL("out of host memory");
-    sc->mesh_nodes = (StBVHNodeRec *)calloc(node_total + 1, sizeof(StBVHNodeRec));
-    sc->tri_hot = (StDTriHot *)calloc(tri_total + 1, sizeof(StDTriHot));
-    sc->tri_cold = (StDTriCold *)calloc(tri_total + 1, sizeof(StDTriCold));
+    /* host copies of the mesh records only when some mesh is built or
+       loaded on the host; device-built meshes never have one */
+    sc->mesh_host_data = pp->mesh_count ? 0u : 1u;
+    for (uint32_t m = 0; m < pp->mesh_count; m++)
+        if (pp->meshes[m].rec.prebuilt || !pp->defer_mesh_build) sc->mesh_host_data = 1;
+    sc->mesh_nodes = (StBVHNodeRec *)calloc(sc->mesh_host_data ? node_total + 1 : 1, sizeof(StBVHNodeRec));
+    sc->tri_hot = (StDTriHot *)calloc(sc->mesh_host_data ? tri_total + 1 : 1, sizeof(StDTriHot));
+    sc->tri_cold = (StDTriCold *)calloc(sc->mesh_host_data ? tri_total + 1 : 1, sizeof(StDTriCold));
     if (!sc->mesh_nodes || !sc->tri_hot || !sc->tri_cold) FAIL("out of host memory");
     for (uint32_t m = 0; m < pp->mesh_count; m++) {
         const StMeshPart *s = pp->meshes + m;

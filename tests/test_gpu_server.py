@@ -151,3 +151,24 @@ def test_many_submitting_threads():
     finally:
         srv.close()
         ref.close()
+
+
+def test_pinned_outputs_equal_pageable():
+    s, cam = scenes.ex09_textures()
+    ds = st.DeviceScene(s.to_blob())
+    try:
+        views = np.concatenate([make_view(width=50 + 7 * i, height=30 + 5 * i, mode=MODES[i % 4], **cam)
+                                for i in range(5)])
+        grid = [(int(v["width"]) + 1) * (int(v["height"]) + 1) for v in views]
+        fp = st.pinned_split(grid, st.PIXEL_DTYPE)
+        bg = st.pinned_split([int(v["width"]) * int(v["height"]) for v in views], np.uint32)
+        a, _, _ = st.render_batch([ds] * 5, views, outs=fp)
+        b, _, _ = st.render_batch([ds] * 5, views)
+        c, _ = st.render_batch_bgra8([ds] * 5, views, outs=bg)
+        d, _ = st.render_batch_bgra8([ds] * 5, views)
+        for i in range(5):
+            one, _, _ = st.render(ds, views[i:i + 1], hits=False, out=st.pinned_empty(grid[i], st.PIXEL_DTYPE))
+            assert a[i].tobytes() == b[i].tobytes() == one.tobytes()
+            assert (c[i] == d[i]).all()
+    finally:
+        ds.close()
