@@ -21,6 +21,12 @@ Also measured in the same run:
     list, with FloatPixel downloads and with the window images resolved on
     the device (BGRA8); the batched frames are checked equal to the plain
     ones;
+  - instanced ("instanced"): 64 instances of 3 noise-displaced meshes
+    (50k/200k/500k triangles, reference SAH mesh BVHs), 2048^2 Perlin
+    albedo and normal maps with full mip chains, ray-cone angle 0.001,
+    1920x1080, depth 3, Beauty; every frame moves all instances (new random
+    positions and rotations), rebuilds the scene BVH on the device (Morton
+    LBVH) and renders: frame time includes upload, rebuild and render;
   - serving ("served"): 40 resident scenes of 1-20M triangles (log-uniform,
     admitted against a 200 GB HBM budget), requests of the same mix arriving
     as a Poisson process at a stated rate for a stated time, served by the
@@ -133,6 +139,56 @@ def copy_probe(mib=512, reps=5):
         print(f"copy probe {i}: {mib} MiB read, {mib} MiB written", flush=True)
 
 
+def instanced_leg(args):
+    from slimtracin_amd.scene import PRIM_DTYPE, blob_prims, quat_axis, quat_mul_norm
+    t0 = time.perf_counter()
+    blob = st.gen_instanced(2025, 64, [50_000, 200_000, 500_000], 2048)
+    ds = st.DeviceScene(blob)
+    build_s = time.perf_counter() - t0
+    prims = blob_prims(blob)
+    rng = np.random.default_rng(3)
+    view = make_view(width=1920, height=1080, mode=MODE_BEAUTY, depth=3, cone_angle=0.001,
+                     position=(0.0, 14.0, -35.0), pitch=-0.3)
+
+    def moved():
+        p = prims.copy()
+        for i in range(len(p) - 1):
+            p["position"][i] = (rng.uniform(-30, 30), rng.uniform(1.5, 6), rng.uniform(-20, 40))
+            q = quat_axis((1, 0, 0), float(rng.uniform(-3, 3)))
+            q = quat_mul_norm(q, quat_axis((0, 1, 0), float(rng.uniform(-3, 3))))
+            p["rotation"][i] = quat_mul_norm(q, quat_axis((0, 0, 1), float(rng.uniform(-3, 3))))
+        return np.ascontiguousarray(p, dtype=PRIM_DTYPE)
+
+    frames = [moved() for _ in range(args.instanced_frames + 2)]
+    for p in frames[:2]:                                            # warm-up
+        ds.set_prims(p)
+        ds.rebuild(st.ST_REBUILD_LBVH)
+        st.render_timed([ds], view)
+    frame_ms, rebuild_ms, kernel_ms, rays = [], [], [], 0
+    tstart = time.perf_counter()
+    for p in frames[2:]:
+        f0 = time.perf_counter()
+        ds.set_prims(p)
+        rebuild_ms.append(ds.rebuild(st.ST_REBUILD_LBVH))
+        k, c = st.render_timed([ds], view)
+        frame_ms.append((time.perf_counter() - f0) * 1e3)
+        kernel_ms.append(k)
+        rays += c[0].total()
+    wall = time.perf_counter() - tstart
+    info = ds.info()
+    ds.close()
+    return {
+        "instances": 64, "mesh_triangles": [50_000, 200_000, 500_000], "triangles": int(info.triangle_count),
+        "texture": "2 x 2048^2 Perlin (albedo, normal), 10 mips", "cone_angle": 0.001, "depth": 3,
+        "scene_build_s": round(build_s, 2), "frames": len(frame_ms),
+        "rays_per_frame": int(rays / len(frame_ms)), "mrays": round(rays / wall / 1e6, 2),
+        "frame_p50_ms": round(float(np.percentile(frame_ms, 50)), 3),
+        "frame_p99_ms": round(float(np.percentile(frame_ms, 99)), 3),
+        "lbvh_rebuild_ms": round(float(np.median(rebuild_ms)), 4),
+        "render_kernel_ms": round(float(np.median(kernel_ms)), 3),
+    }
+
+
 BATCH_TRIANGLES = [200_000, 320_000, 500_000, 800_000, 1_250_000, 2_000_000, 3_200_000, 5_000_000]
 
 
@@ -145,7 +201,7 @@ def batched_leg(args):
     build_s = time.perf_counter() - t0
     views, sids = request_mix(rng, len(dscenes), 256)
     req_scenes = [dscenes[i] for i in sids]
-    st.render_batch(req_scenes[:16], views[:16], persistent=True)          # warm-up
+    st.render_timed(req_scenes, views)          # warm-up: grows the launch arena to the whole batch
     lat_plain, plain_rays, plain_pix = [], 0, []
     tp = time.perf_counter()
     for i in range(len(views)):
@@ -295,6 +351,8 @@ def main():
     ap.add_argument("--cpu-frames", type=int, default=3, help="frames of the CPU baseline (median)")
     ap.add_argument("--copy-probe", action="store_true", help="only run the copy kernel used to calibrate PMC bytes")
     ap.add_argument("--no-serve", action="store_true", help="skip the served (Poisson arrivals) leg")
+    ap.add_argument("--no-instanced", action="store_true", help="skip the instanced (per-frame LBVH) leg")
+    ap.add_argument("--instanced-frames", type=int, default=10, help="instanced leg: timed frames")
     ap.add_argument("--serve-rate", type=float, default=0.0,
                     help="served leg: Poisson arrival rate, requests/s (0: 60%% of the measured burst capacity)")
     ap.add_argument("--serve-seconds", type=float, default=8.0, help="served leg: arrival window, s")
@@ -362,6 +420,9 @@ def main():
         "render_kernel_ms": round(kernel_ms, 4),
     }
 
+    if not args.no_instanced:
+        result["instanced"] = instanced_leg(args)
+        log("instanced:", result["instanced"])
     if not args.no_batch:
         result["batched"] = batched_leg(args)
         log("batched:", result["batched"])

@@ -21,6 +21,12 @@ int st_gen_implicit(uint64_t seed, void **out, size_t *size);
 int st_gen_pbr(uint64_t seed, uint32_t k, void **out, size_t *size);
 /* Noise-displaced sphere mesh with about `triangles` triangles + floor. */
 int st_gen_mesh_scene(uint64_t seed, uint32_t triangles, void **out, size_t *size);
+/* `instances` mesh primitives over `mesh_count` noise-displaced meshes of
+   mesh_triangles[i] triangles each, a floor, 3 point lights, materials
+   using a Perlin albedo map and a Perlin normal map of tex_size^2 texels
+   (power of two, full mip chains to 4x4). */
+int st_gen_instanced(uint64_t seed, uint32_t instances, const uint32_t *mesh_triangles, uint32_t mesh_count,
+                     uint32_t tex_size, void **out, size_t *size);
 void st_free(void *p);
 
 #ifdef __cplusplus

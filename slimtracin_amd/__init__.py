@@ -49,7 +49,7 @@ def lib():
             raise SlimTracinError(f"{LIB_PATH} is missing: build it with `make -C slimtracin_amd/csrc`")
         L = C.CDLL(LIB_PATH)
         L.st_last_error.restype = C.c_char_p
-        for name in ("st_gen_implicit", "st_gen_pbr", "st_gen_mesh_scene"):
+        for name in ("st_gen_implicit", "st_gen_pbr", "st_gen_mesh_scene", "st_gen_instanced"):
             getattr(L, name).restype = C.c_int
         _lib = L
     return _lib
@@ -93,6 +93,14 @@ def gen_mesh_scene(seed: int, triangles: int) -> bytes:
 
 
 ST_SCENE_MESH_LBVH = 1
+
+
+def gen_instanced(seed: int, instances: int, mesh_triangles, tex_size: int) -> bytes:
+    out, size = C.c_void_p(), C.c_size_t()
+    tri = (C.c_uint32 * len(mesh_triangles))(*mesh_triangles)
+    return _take_blob(lib().st_gen_instanced(C.c_uint64(seed), C.c_uint32(instances), tri,
+                                             C.c_uint32(len(mesh_triangles)), C.c_uint32(tex_size),
+                                             C.byref(out), C.byref(size)), out, size)
 
 
 class DeviceScene:

@@ -509,4 +509,205 @@ int st_gen_mesh_scene(uint64_t seed, uint32_t triangles, void **out, size_t *siz
     return finish_blob(&b, out, size);
 }
 
+/* ---- Perlin textures ------------------------------------------------------ */
+
+/* Tileable 2D gradient noise (Perlin 1985/2002: fade 6t^5-15t^4+10t^3,
+   pseudo-random unit gradients on the lattice) with period `per` cells. */
+static double g_grad[256][2];
+static int g_grad_ready = 0;
+static double grad_dot(int32_t ix, int32_t iy, int32_t per, uint32_t seed, double dx, double dy) {
+    if (!g_grad_ready) {
+        for (int k = 0; k < 256; k++) {
+            g_grad[k][0] = cos(k * (2 * M_PI / 256.0));
+            g_grad[k][1] = sin(k * (2 * M_PI / 256.0));
+        }
+        g_grad_ready = 1;
+    }
+    uint32_t h = hash3(((ix % per) + per) % per, ((iy % per) + per) % per, 0, seed) & 255u;
+    return g_grad[h][0] * dx + g_grad[h][1] * dy;
+}
+static double fade(double t) { return t * t * t * (t * (t * 6 - 15) + 10); }
+static double perlin2(double x, double y, int32_t per, uint32_t seed) {
+    int32_t ix = (int32_t)floor(x), iy = (int32_t)floor(y);
+    double fx = x - ix, fy = y - iy;
+    double n00 = grad_dot(ix, iy, per, seed, fx, fy), n10 = grad_dot(ix + 1, iy, per, seed, fx - 1, fy);
+    double n01 = grad_dot(ix, iy + 1, per, seed, fx, fy - 1), n11 = grad_dot(ix + 1, iy + 1, per, seed, fx - 1, fy - 1);
+    double u = fade(fx), v = fade(fy);
+    double a = n00 + u * (n10 - n00), b = n01 + u * (n11 - n01);
+    return a + v * (b - a);
+}
+/* 5 octaves, base period 8 cells over the texture, result about [-1, 1] */
+static double perlin_fbm(double u, double v, uint32_t seed) {
+    double sum = 0, amp = 1, norm = 0;
+    int32_t per = 8;
+    for (int o = 0; o < 5; o++) {
+        sum += amp * perlin2(u * per, v * per, per, seed + (uint32_t)o);
+        norm += amp;
+        amp *= 0.5;
+        per *= 2;
+    }
+    return sum / norm * 1.6;
+}
+
+/* One RGB8 texture of size x size and its mip chain down to 4x4 (each level
+   the rounded mean of 2x2 texels of the one above), written as TexelQuad
+   levels: quad (x, y) of a w x h level holds the four texels around grid
+   corner (x, y), indices clamped, per channel TL, TR, BL, BR. */
+static void write_texture(Buf *b, const uint8_t *rgb, uint32_t size) {
+    uint32_t levels = 0;
+    for (uint32_t w = size; w >= 4; w >>= 1) { levels++; if (w == 4) break; }
+    StTextureRec tr;
+    memset(&tr, 0, sizeof tr);
+    tr.width = (uint16_t)size;
+    tr.height = (uint16_t)size;
+    tr.mip_count = (uint8_t)levels;
+    tr.wrap = 1;
+    tr.mipmap = 1;
+    put(b, &tr, sizeof tr);
+    uint8_t *cur = (uint8_t *)malloc(3u * size * size);
+    memcpy(cur, rgb, 3u * size * size);
+    uint32_t w = size;
+    uint8_t *quads = (uint8_t *)malloc(12u * (size + 1) * (size + 1));
+    for (uint32_t l = 0; l < levels; l++) {
+        uint16_t dims[2] = {(uint16_t)w, (uint16_t)w};
+        put(b, dims, 4);
+        for (uint32_t y = 0; y <= w; y++)
+            for (uint32_t x = 0; x <= w; x++) {
+                uint32_t x0 = x ? x - 1 : 0, y0 = y ? y - 1 : 0, x1 = x < w ? x : w - 1, y1 = y < w ? y : w - 1;
+                const uint8_t *tl = cur + 3 * ((size_t)y0 * w + x0), *trp = cur + 3 * ((size_t)y0 * w + x1);
+                const uint8_t *bl = cur + 3 * ((size_t)y1 * w + x0), *br = cur + 3 * ((size_t)y1 * w + x1);
+                uint8_t *q = quads + 12 * ((size_t)y * (w + 1) + x);
+                for (int c = 0; c < 3; c++) {
+                    q[4 * c] = tl[c]; q[4 * c + 1] = trp[c]; q[4 * c + 2] = bl[c]; q[4 * c + 3] = br[c];
+                }
+            }
+        put(b, quads, 12u * (size_t)(w + 1) * (w + 1));
+        if (l + 1 < levels) {
+            uint32_t h = w / 2;
+            uint8_t *nxt = (uint8_t *)malloc(3u * h * h);
+            for (uint32_t y = 0; y < h; y++)
+                for (uint32_t x = 0; x < h; x++)
+                    for (int c = 0; c < 3; c++) {
+                        uint32_t s4 = cur[3 * ((2 * y) * (size_t)w + 2 * x) + c] + cur[3 * ((2 * y) * (size_t)w + 2 * x + 1) + c] +
+                                      cur[3 * ((2 * y + 1) * (size_t)w + 2 * x) + c] +
+                                      cur[3 * ((2 * y + 1) * (size_t)w + 2 * x + 1) + c];
+                        nxt[3 * ((size_t)y * h + x) + c] = (uint8_t)((s4 + 2) / 4);
+                    }
+            free(cur);
+            cur = nxt;
+            w = h;
+        }
+    }
+    free(quads);
+    free(cur);
+}
+
+/* Albedo (three noise channels around a seeded base colour) and a normal
+   map from the gradient of a noise height field, both size x size RGB8. */
+static void perlin_textures(uint32_t size, uint32_t seed, uint8_t *albedo, uint8_t *normal) {
+    double *hgt = (double *)malloc(sizeof(double) * size * size);
+    for (uint32_t y = 0; y < size; y++)
+        for (uint32_t x = 0; x < size; x++) {
+            double u = (x + 0.5) / size, v = (y + 0.5) / size;
+            double r = 0.55 + 0.35 * perlin_fbm(u, v, seed), g = 0.45 + 0.35 * perlin_fbm(u, v, seed + 17),
+                   bb = 0.35 + 0.35 * perlin_fbm(u, v, seed + 31);
+            double c[3] = {r, g, bb};
+            for (int k = 0; k < 3; k++) {
+                double t = c[k] < 0 ? 0 : c[k] > 1 ? 1 : c[k];
+                albedo[3 * ((size_t)y * size + x) + k] = (uint8_t)(t * 255.0 + 0.5);
+            }
+            hgt[(size_t)y * size + x] = perlin_fbm(u, v, seed + 101);
+        }
+    for (uint32_t y = 0; y < size; y++)
+        for (uint32_t x = 0; x < size; x++) {
+            double hx = hgt[(size_t)y * size + (x + 1) % size] - hgt[(size_t)y * size + (x + size - 1) % size];
+            double hy = hgt[(size_t)((y + 1) % size) * size + x] - hgt[(size_t)((y + size - 1) % size) * size + x];
+            double n[3] = {-hx * size * 0.02, -hy * size * 0.02, 1.0};
+            double l = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+            for (int k = 0; k < 3; k++)
+                normal[3 * ((size_t)y * size + x) + k] = (uint8_t)((n[k] / l * 0.5 + 0.5) * 255.0 + 0.5);
+        }
+    free(hgt);
+}
+
+/* Config "instanced": `instances` mesh primitives drawing from `mesh_count`
+   noise-displaced sphere meshes of mesh_triangles[i] triangles, random
+   positions over a 60 x 60 field, random yaw/pitch/roll amounts, uniform
+   scale U[0.8, 2.5]; a floor; 3 point lights; materials textured with a
+   Perlin albedo map and a Perlin normal map (tex_size^2, full mip chains)
+   - Cook-Torrance and Blinn, one untextured mirror. */
+int st_gen_instanced(uint64_t seed, uint32_t instances, const uint32_t *mesh_triangles, uint32_t mesh_count,
+                     uint32_t tex_size, void **out, size_t *size) {
+    if (!mesh_count || !instances || tex_size < 4 || (tex_size & (tex_size - 1))) return ST_ERR_ARG;
+    Pcg r;
+    pcg_seed(&r, seed, 5);
+    Buf b = {0};
+    float ambient[3] = {0.01f, 0.01f, 0.012f};
+    const uint32_t nmat = 4;
+    write_header(&b, instances + 1, nmat, 3, mesh_count, 2, ambient);
+    for (uint32_t i = 0; i < instances; i++) {
+        StPrimRec p = prim_default(ST_PRIM_MESH, pcg_next(&r) % mesh_count);
+        set3(p.position, rnd(&r, -30, 30), rnd(&r, 1.5f, 6), rnd(&r, -20, 40));
+        float sc = rnd(&r, 0.8f, 2.5f);
+        set3(p.scale, sc, sc, sc);
+        quat_out(p.rotation, ypr_quat(rnd(&r, -1, 1), rnd(&r, -1, 1), rnd(&r, -1, 1)));
+        p.material_id = pcg_next(&r) % nmat;
+        put(&b, &p, sizeof p);
+    }
+    StPrimRec floor_p = prim_default(ST_PRIM_QUAD, 0);
+    set3(floor_p.scale, 80, 1, 80);
+    floor_p.material_id = 0;
+    put(&b, &floor_p, sizeof floor_p);
+    StMaterialRec m[4];
+    for (int i = 0; i < 4; i++) m[i] = mat_default();
+    m[0].brdf = ST_BRDF_COOK_TORRANCE;
+    m[0].roughness = 0.6f;
+    set3(m[0].reflectivity, 0.04f, 0.04f, 0.04f);
+    m[0].texture_count = 2;
+    m[0].texture_ids[0] = 0;
+    m[0].texture_ids[1] = 1;
+    m[0].uv_repeat[0] = m[0].uv_repeat[1] = 4;
+    m[1] = m[0];
+    m[1].roughness = 0.25f;
+    m[1].metallic = 1;
+    set3(m[1].reflectivity, 0.9f, 0.8f, 0.6f);
+    m[1].uv_repeat[0] = m[1].uv_repeat[1] = 2;
+    m[2] = m[0];
+    m[2].brdf = ST_BRDF_BLINN;
+    m[2].roughness = 0.5f;
+    set3(m[2].reflectivity, 0.5f, 0.5f, 0.5f);
+    m[3].brdf = ST_BRDF_BLINN;
+    m[3].is = ST_REFLECTIVE;
+    m[3].roughness = 0;
+    m[3].use = 0;
+    set3(m[3].reflectivity, 0.9f, 0.9f, 0.9f);
+    put(&b, m, sizeof m);
+    for (int i = 0; i < 3; i++) {
+        StLightRec l;
+        set3(l.position, rnd(&r, -25, 25), rnd(&r, 12, 20), rnd(&r, -15, 35));
+        set3(l.color, 1, 1, 1);
+        l.intensity = rnd(&r, 150, 300);
+        put(&b, &l, sizeof l);
+    }
+    for (uint32_t k = 0; k < mesh_count; k++) {
+        uint32_t half = mesh_triangles[k] / 2 ? mesh_triangles[k] / 2 : 1;
+        uint32_t rings = (uint32_t)sqrt((double)half / 2.0);
+        if (rings < 2) rings = 2;
+        uint32_t segments = half / rings;
+        if (segments < 3) segments = 3;
+        GMesh gm;
+        if (gen_latlong(&gm, rings, segments, (uint32_t)seed + 77 * k, 0.15)) { gmesh_free(&gm); free(b.p); return ST_ERR_MEMORY; }
+        write_mesh(&b, &gm);
+        gmesh_free(&gm);
+    }
+    uint8_t *albedo = (uint8_t *)malloc(3u * tex_size * tex_size), *normal = (uint8_t *)malloc(3u * tex_size * tex_size);
+    if (!albedo || !normal) { free(albedo); free(normal); free(b.p); return ST_ERR_MEMORY; }
+    perlin_textures(tex_size, (uint32_t)seed * 7u + 3u, albedo, normal);
+    write_texture(&b, albedo, tex_size);
+    write_texture(&b, normal, tex_size);
+    free(albedo);
+    free(normal);
+    return finish_blob(&b, out, size);
+}
+
 void st_free(void *p) { free(p); }
