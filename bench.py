@@ -202,6 +202,7 @@ def batched_leg(args):
     views, sids = request_mix(rng, len(dscenes), 256)
     req_scenes = [dscenes[i] for i in sids]
     st.render_timed(req_scenes, views)          # warm-up: grows the launch arena to the whole batch
+    tune, _ = st.autotune(req_scenes, views, reps=1)
     lat_plain, plain_rays, plain_pix = [], 0, []
     tp = time.perf_counter()
     for i in range(len(views)):
@@ -250,6 +251,7 @@ def batched_leg(args):
         "plain_latency_p99_ms": round(float(np.percentile(lat_plain, 99)), 3),
         "batched_equals_plain": same,
         "batched_render_kernel_ms": round(kernel_ms, 3),
+        "launch_shape": tune.as_dict(),
     }
     for d in dscenes:
         d.close()
@@ -375,6 +377,8 @@ def main():
     log(f"scene: {info.triangle_count} triangles, mesh BVH height {info.max_mesh_height}, "
         f"{info.device_bytes / 2**20:.1f} MiB in HBM, built in {time.time() - t0:.1f} s")
     view = pbr_view(1920, 1080)
+    tune, tune_ms = st.autotune([scene], view)
+    log(f"autotune: {tune.as_dict()} ({tune_ms:.3f} ms)")
 
     for _ in range(args.warmup):
         st.render_timed([scene], view)
@@ -418,6 +422,7 @@ def main():
         "p99_ms": round(float(np.percentile(frame_ms, 99)), 4),
         "frame_ms_with_download": round(pcie_frame_ms, 4),
         "render_kernel_ms": round(kernel_ms, 4),
+        "launch_shape": tune.as_dict(),
     }
 
     if not args.no_instanced:

@@ -109,6 +109,20 @@ int st_render_batch(StSceneHandle *const *scenes, const StView *views, uint32_t 
 int st_render_batch_bgra8(StSceneHandle *const *scenes, const StView *views, uint32_t n, uint32_t *const *bgra,
                           StRayCounts *counts, uint32_t flags);
 
+/* Launch shape of the render kernel (process-wide). */
+typedef struct StTuning {
+    uint32_t tile_shift;     /* tile = (1 << tile_shift) x (64 >> tile_shift) pixels, one wavefront per tile */
+    uint32_t blocks_per_cu;  /* persistent grid: blocks per compute unit (4 wavefronts each) */
+    uint32_t persistent;     /* 1: persistent tile queue (st_render); 0: one wavefront per tile */
+} StTuning;
+
+int st_set_tuning(const StTuning *tuning);
+int st_get_tuning(StTuning *tuning);
+/* Autotuner: times each candidate shape on these requests (median of reps
+   render-kernel times) and keeps the fastest; returns it and its time. */
+int st_autotune(StSceneHandle *const *scenes, const StView *views, uint32_t n, uint32_t reps, StTuning *best,
+                float *best_ms);
+
 /* Page-locked host memory for frame outputs: downloads into it run at the
    link's full rate (pageable destinations are staged by the runtime). */
 int st_host_alloc(size_t bytes, void **out);

@@ -258,3 +258,33 @@ def resolve_bgra8(grid: np.ndarray, grid_w: int, grid_h: int) -> np.ndarray:
     out = np.zeros((grid_w - 1) * (grid_h - 1), dtype=np.uint32)
     _check(lib().st_resolve_bgra8(_ptr(g), C.c_uint32(grid_w), C.c_uint32(grid_h), _ptr(out)))
     return out
+
+
+class Tuning(C.Structure):
+    _fields_ = [("tile_shift", C.c_uint32), ("blocks_per_cu", C.c_uint32), ("persistent", C.c_uint32)]
+
+    def as_dict(self):
+        return {"tile": f"{1 << self.tile_shift}x{64 >> self.tile_shift}", "blocks_per_cu": self.blocks_per_cu,
+                "persistent": bool(self.persistent)}
+
+
+def get_tuning() -> Tuning:
+    t = Tuning()
+    _check(lib().st_get_tuning(C.byref(t)))
+    return t
+
+
+def set_tuning(tile_shift: int = 3, blocks_per_cu: int = 8, persistent: bool = True):
+    t = Tuning(tile_shift, blocks_per_cu, 1 if persistent else 0)
+    _check(lib().st_set_tuning(C.byref(t)))
+
+
+def autotune(scenes, views: np.ndarray, reps: int = 3):
+    """Pick the fastest launch shape for these requests (st_autotune);
+    returns (Tuning, render-kernel ms)."""
+    n = len(views)
+    handles = (C.c_void_p * n)(*[s.handle for s in scenes])
+    views = np.ascontiguousarray(views, dtype=VIEW_DTYPE)
+    best, ms = Tuning(), C.c_float()
+    _check(lib().st_autotune(handles, _ptr(views), C.c_uint32(n), C.c_uint32(reps), C.byref(best), C.byref(ms)))
+    return best, ms.value

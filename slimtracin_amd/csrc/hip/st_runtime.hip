@@ -27,6 +27,7 @@
 #include <string.h>
 #include <time.h>
 
+#include <algorithm>
 #include <mutex>
 #include <vector>
 
@@ -38,8 +39,6 @@
 #include "slimtracin.h"
 #include "st_internal.h"
 
-#define TILE_W 8
-#define TILE_H 8
 #define WAVES_PER_BLOCK 4
 #define BLOCK_THREADS (64 * WAVES_PER_BLOCK)
 /* Two waves per SIMD: measured fastest on gfx950 (the fully inlined path
@@ -128,7 +127,8 @@ __device__ static void render_tile(const StDScene *scenes, const KReq *reqs, uin
     const StDScene sc = scenes[v.scene];
     const uint32_t local = tile - v.tile_base;
     const uint32_t tx = local % v.tiles_x, ty = local / v.tiles_x;
-    const uint32_t x = tx * TILE_W + (lane % TILE_W), y = ty * TILE_H + (lane / TILE_W);
+    const uint32_t tw = 1u << v.tile_shift, th = 64u >> v.tile_shift;
+    const uint32_t x = tx * tw + (lane & (tw - 1u)), y = ty * th + (lane >> v.tile_shift);
     uint32_t n_shadow = 0, n_bounce = 0, n_primary = 0;
     if (x < v.width1 && y < v.height1) {
         const size_t pix = (size_t)y * v.width1 + x;
@@ -284,6 +284,7 @@ static struct {
     uint32_t scene_table_cap;
     hipEvent_t ev[4];
     int cus;               /* compute units of the device */
+    StTuning tune;         /* launch shape chosen by st_set_tuning / st_autotune */
 } g;
 
 static int ensure_init(void) {
@@ -304,6 +305,9 @@ extern "C" int st_init(int device) {
     HIP_TRY(hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking));
     HIP_TRY(hipMalloc(&g.tile_counter, 256));
     for (int i = 0; i < 4; i++) HIP_TRY(hipEventCreate(&g.ev[i]));
+    g.tune.tile_shift = 3;          /* 8 x 8 tiles */
+    g.tune.blocks_per_cu = 8;
+    g.tune.persistent = 1;
     hipDeviceProp_t prop;
     g.cus = 256;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
@@ -747,8 +751,10 @@ int st_rt_launch_prepared(StSceneHandle *const *scenes, StHostView *hv, uint32_t
             if (hv[i].dv.width1 < 2 || hv[i].dv.height1 < 2) return set_err(ST_ERR_ARG, "request %u: empty window", i);
             per_req_out += round256((size_t)(hv[i].dv.width1 - 1) * (hv[i].dv.height1 - 1) * 4);
         }
-        hv[i].dv.tiles_x = (hv[i].dv.width1 + TILE_W - 1) / TILE_W;
-        uint32_t ty = (hv[i].dv.height1 + TILE_H - 1) / TILE_H;
+        const uint32_t tw = 1u << g.tune.tile_shift, th = 64u >> g.tune.tile_shift;
+        hv[i].dv.tile_shift = g.tune.tile_shift;
+        hv[i].dv.tiles_x = (hv[i].dv.width1 + tw - 1) / tw;
+        uint32_t ty = (hv[i].dv.height1 + th - 1) / th;
         hv[i].dv.tile_base = tiles;
         tiles += hv[i].dv.tiles_x * ty;
         rows += hv[i].dv.height1;
@@ -806,7 +812,7 @@ int st_rt_launch_prepared(StSceneHandle *const *scenes, StHostView *hv, uint32_t
     uint32_t blocks;
     uint32_t xcd_blocks = 0;
     if (persistent) {
-        uint32_t want = (uint32_t)g.cus * 8u;
+        uint32_t want = (uint32_t)g.cus * g.tune.blocks_per_cu;
         uint32_t need = (tiles + WAVES_PER_BLOCK - 1) / WAVES_PER_BLOCK;
         blocks = need < want ? need : want;
     } else {
@@ -880,7 +886,9 @@ static int launch(StSceneHandle *const *scenes, const StView *views, uint32_t n,
 extern "C" int st_render(StSceneHandle *scene, const StView *view, StFloatPixel *pixels, StHitRec *hits,
                          StRayCounts *counts) {
     if (!scene || !view) return set_err(ST_ERR_ARG, "null argument");
-    return launch(&scene, view, 1, hits != NULL, 1, counts, &pixels, &hits, NULL, NULL);
+    int rc = ensure_init();
+    if (rc) return rc;
+    return launch(&scene, view, 1, hits != NULL, (int)g.tune.persistent, counts, &pixels, &hits, NULL, NULL);
 }
 
 extern "C" int st_render_batch(StSceneHandle *const *scenes, const StView *views, uint32_t n,
@@ -943,4 +951,68 @@ extern "C" int st_host_alloc(size_t bytes, void **out) {
 
 extern "C" void st_host_free(void *p) {
     if (p) (void)hipHostFree(p);
+}
+
+/* ---- launch tuning ------------------------------------------------------ */
+
+static int tuning_valid(const StTuning *t) {
+    return t->tile_shift <= 6 && t->blocks_per_cu >= 1 && t->blocks_per_cu <= 64 && t->persistent <= 1;
+}
+
+extern "C" int st_set_tuning(const StTuning *t) {
+    RT_LOCK;
+    if (!t || !tuning_valid(t)) return set_err(ST_ERR_ARG, "invalid tuning");
+    int rc = ensure_init();
+    if (rc) return rc;
+    g.tune = *t;
+    return ST_OK;
+}
+
+extern "C" int st_get_tuning(StTuning *t) {
+    RT_LOCK;
+    if (!t) return set_err(ST_ERR_ARG, "null argument");
+    int rc = ensure_init();
+    if (rc) return rc;
+    *t = g.tune;
+    return ST_OK;
+}
+
+/* Time every candidate launch shape on the caller's requests (median of
+   `reps` render-kernel times from HIP events) and keep the fastest. The
+   candidates: 8x8, 16x4 and 32x2 tiles; persistent grids of 2, 4 or 8
+   blocks per CU, or one block per 4 tiles in XCD-aware order. */
+extern "C" int st_autotune(StSceneHandle *const *scenes, const StView *views, uint32_t n, uint32_t reps,
+                           StTuning *best, float *best_ms) {
+    RT_LOCK;
+    if (!scenes || !views || !n) return set_err(ST_ERR_ARG, "no requests");
+    int rc = ensure_init();
+    if (rc) return rc;
+    if (reps == 0) reps = 3;
+    if (reps > 15) reps = 15;
+    const StTuning saved = g.tune;
+    StTuning cand[12];
+    uint32_t nc = 0;
+    for (uint32_t sh = 3; sh <= 5; sh++) {
+        const uint32_t bpc[3] = {2, 4, 8};
+        for (int k = 0; k < 3; k++) cand[nc++] = StTuning{sh, bpc[k], 1u};
+        cand[nc++] = StTuning{sh, 8u, 0u};
+    }
+    float best_t = INFINITY;
+    StTuning best_c = saved;
+    for (uint32_t c = 0; c < nc; c++) {
+        g.tune = cand[c];
+        float times[15];
+        for (uint32_t r = 0; r < reps && !rc; r++)
+            rc = launch(scenes, views, n, 0, (int)cand[c].persistent, NULL, NULL, NULL, NULL, &times[r]);
+        if (rc) break;
+        std::sort(times, times + reps);
+        if (times[reps / 2] < best_t) {
+            best_t = times[reps / 2];
+            best_c = cand[c];
+        }
+    }
+    g.tune = rc ? saved : best_c;
+    if (best) *best = g.tune;
+    if (best_ms) *best_ms = best_t;
+    return rc;
 }

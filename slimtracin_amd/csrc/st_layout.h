@@ -108,7 +108,7 @@ typedef struct StDView {
     uint32_t scene;                /* index into the scene table (batched launches) */
     uint32_t tile_base;            /* first tile of this request in a batched launch */
     uint32_t tiles_x;              /* tiles per row */
-    uint32_t pad;
+    uint32_t tile_shift;           /* tile width = 1 << tile_shift lanes, height = 64 >> tile_shift */
     const float *row_starts;       /* 4 floats per row: sequential start vectors */
     float *dirs;                   /* 4 floats per pixel: the sequential `current` vectors */
     const int32_t *rects;          /* 4 ints per primitive: SSB min.x, min.y, max.x, max.y */

@@ -1,0 +1,38 @@
+"""GPU: every launch shape the autotuner may pick renders the same frames
+bit for bit (tiles only change which wavefront traces a pixel), and the
+autotuner keeps one of its candidates."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import scenes
+import slimtracin_amd as st
+from slimtracin_amd.scene import MODE_BEAUTY, MODE_NORMALS, make_view
+
+pytestmark = pytest.mark.gpu
+
+
+def test_all_launch_shapes_give_identical_frames():
+    s, cam = scenes.ex09_textures()
+    ds = st.DeviceScene(s.to_blob())
+    saved = st.get_tuning()
+    try:
+        views = np.concatenate([make_view(width=131, height=77, mode=MODE_BEAUTY, **cam),
+                                make_view(width=64, height=200, mode=MODE_NORMALS, **cam)])
+        ref = None
+        for shift in (3, 4, 5):
+            for bpc in (2, 8):
+                for persistent in (True, False):
+                    st.set_tuning(shift, bpc, persistent)
+                    pix, hits, counts = st.render_batch([ds, ds], views, hits=True)
+                    got = b"".join(p.tobytes() for p in pix) + b"".join(h.tobytes() for h in hits)
+                    if ref is None:
+                        ref = got
+                    assert got == ref, (shift, bpc, persistent)
+        best, ms = st.autotune([ds, ds], views, reps=3)
+        assert best.tile_shift in (3, 4, 5) and ms > 0
+        print("autotune:", best.as_dict(), f"{ms:.3f} ms", flush=True)
+    finally:
+        st.set_tuning(saved.tile_shift, saved.blocks_per_cu, bool(saved.persistent))
+        ds.close()
