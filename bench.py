@@ -268,7 +268,7 @@ def served_leg(args):
     t0 = time.perf_counter()
     for i, tri in enumerate(tri_counts):
         blob = st.gen_mesh_scene(500 + i, int(tri))
-        sid = srv.try_add_scene(blob)
+        sid = srv.try_add_scene(blob, mesh_lbvh=True)
         if sid is None:
             rejected += 1
         else:

@@ -56,6 +56,9 @@ void st_server_destroy(StServer *srv);
 /* Admit a scene (st_scene_create) if it fits the remaining HBM budget;
    ST_ERR_BUSY when it does not (remove scenes first). */
 int st_server_add_scene(StServer *srv, const void *blob, size_t size, uint32_t *scene_id);
+/* The same with st_scene_create_ex flags (ST_SCENE_MESH_LBVH: build meshes
+   on the device, for scenes of millions of triangles). */
+int st_server_add_scene_ex(StServer *srv, const void *blob, size_t size, uint32_t flags, uint32_t *scene_id);
 /* Waits until no queued request uses the scene, then frees it. */
 int st_server_remove_scene(StServer *srv, uint32_t scene_id);
 

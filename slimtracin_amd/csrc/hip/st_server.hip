@@ -236,9 +236,15 @@ extern "C" void st_server_destroy(StServer *srv) {
 }
 
 extern "C" int st_server_add_scene(StServer *srv, const void *blob, size_t size, uint32_t *scene_id) {
+    return st_server_add_scene_ex(srv, blob, size, 0, scene_id);
+}
+
+extern "C" int st_server_add_scene_ex(StServer *srv, const void *blob, size_t size, uint32_t flags,
+                                      uint32_t *scene_id) {
     if (!srv || !blob || !scene_id) return fail(ST_ERR_ARG, "null argument");
     const char *why = NULL;
-    StScene *probe = st_scene_from_blob(blob, size, &why);
+    /* size the scene without building any BVH (mesh builds are deferred) */
+    StScene *probe = st_scene_from_blob_ex(blob, size, 1u, &why);
     if (!probe) return fail(ST_ERR_SCENE, why ? why : "invalid scene");
     uint64_t need = st_scene_device_bytes(probe) + 64 * 1024;
     st_scene_free(probe);
@@ -248,7 +254,7 @@ extern "C" int st_server_add_scene(StServer *srv, const void *blob, size_t size,
         srv->resident += need;   /* reserve while building */
     }
     StSceneHandle *h = NULL;
-    int rc = st_scene_create(blob, size, &h);
+    int rc = st_scene_create_ex(blob, size, flags, &h);
     std::lock_guard<std::mutex> lk(srv->mu);
     srv->resident -= need;
     if (rc) return rc;

@@ -613,9 +613,10 @@ DNOINLINE bool hit_primitives(const DRay &ray, DTrace &tr, const StDScene &sc, c
             const float4 *q = reinterpret_cast<const float4 *>(sc.tri_cold + mesh.tri_offset + tri);
             float4 c0 = q[0], c1 = q[1], c2 = q[2], c3 = q[3], c4 = q[4];
             /* the hot record carried no areas: restore the reference's
-               closest_hit->area / uv_area (mesh.h:137-138) */
-            closest.area = c4.x;
-            closest.uv_area = c4.y;
+               closest_hit->area / uv_area (mesh.h:26-27). Cold record
+               floats: vn 0-8, uv 9-14, area 15 (c3.w), uv_area 16 (c4.x) */
+            closest.area = c3.w;
+            closest.uv_area = c4.x;
             if (mesh.attrs) {
                 float u = closest.uv.x, v = closest.uv.y;
                 float w = 1 - u - v;

@@ -46,17 +46,19 @@ class Server:
         except Exception:
             pass
 
-    def add_scene(self, blob: bytes) -> int:
+    def add_scene(self, blob: bytes, mesh_lbvh: bool = False) -> int:
         b = np.frombuffer(blob, dtype=np.uint8)
         sid = C.c_uint32()
-        _check(lib().st_server_add_scene(self.handle, _ptr(b), C.c_size_t(len(blob)), C.byref(sid)))
+        _check(lib().st_server_add_scene_ex(self.handle, _ptr(b), C.c_size_t(len(blob)),
+                                            C.c_uint32(1 if mesh_lbvh else 0), C.byref(sid)))
         return sid.value
 
-    def try_add_scene(self, blob: bytes):
+    def try_add_scene(self, blob: bytes, mesh_lbvh: bool = False):
         """Scene id, or None when the scene does not fit the HBM budget."""
         b = np.frombuffer(blob, dtype=np.uint8)
         sid = C.c_uint32()
-        rc = lib().st_server_add_scene(self.handle, _ptr(b), C.c_size_t(len(blob)), C.byref(sid))
+        rc = lib().st_server_add_scene_ex(self.handle, _ptr(b), C.c_size_t(len(blob)),
+                                          C.c_uint32(1 if mesh_lbvh else 0), C.byref(sid))
         if rc == ST_ERR_BUSY:
             return None
         _check(rc)
