@@ -47,6 +47,9 @@ typedef struct StServerStats {
     uint32_t resident_scenes, queued;
     double latency_p50_ms, latency_p99_ms, latency_max_ms;  /* submit -> frame ready, completed requests */
     double mean_batch;                                       /* requests per launch */
+    double prep_ms, launch_ms, kernel_ms, alloc_ms;          /* totals: host set-up, launch wall (uploads,
+                                                                kernels, downloads), render kernel, output
+                                                                buffer allocation at submit */
 } StServerStats;
 
 int st_server_create(const StServerConfig *cfg, StServer **out);

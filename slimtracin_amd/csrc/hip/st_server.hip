@@ -102,6 +102,7 @@ struct StServer {
     bool stop = false;
     PinnedPool pool;
     uint64_t submitted = 0, completed = 0, launches = 0, rays = 0, batched = 0;
+    double prep_ms = 0, launch_ms = 0, kernel_ms = 0, alloc_ms = 0;
     std::vector<float> latencies;     /* ms, completed requests */
 };
 
@@ -138,6 +139,7 @@ static void worker_loop(StServer *srv) {
             }
         }
         const size_t n = ids.size();
+        const double t_prep = now_ms();
         std::vector<StHostView> hv(n);
         std::vector<int> status(n, ST_OK);
         std::vector<uint32_t> live;
@@ -149,6 +151,8 @@ static void worker_loop(StServer *srv) {
         }
         std::vector<StRayCounts> counts(n);
         int rc = ST_OK;
+        const double t_launch = now_ms();
+        float kms = 0;
         if (!live.empty()) {
             const size_t m = live.size();
             std::vector<StSceneHandle *> lh(m);
@@ -164,7 +168,7 @@ static void worker_loop(StServer *srv) {
             }
             rc = st_rt_launch_prepared(lh.data(), lv.data(), (uint32_t)m, 0, 1, lc.data(),
                                        srv->cfg.output == ST_OUTPUT_FLOAT_PIXELS ? fp.data() : NULL, NULL,
-                                       srv->cfg.output == ST_OUTPUT_FLOAT_PIXELS ? NULL : bp.data(), NULL);
+                                       srv->cfg.output == ST_OUTPUT_FLOAT_PIXELS ? NULL : bp.data(), &kms);
             for (size_t k = 0; k < m; k++) {
                 counts[live[k]] = lc[k];
                 if (rc) status[live[k]] = rc;
@@ -177,6 +181,9 @@ static void worker_loop(StServer *srv) {
             if (!live.empty()) {
                 srv->launches++;
                 srv->batched += live.size();
+                srv->prep_ms += t_launch - t_prep;
+                srv->launch_ms += t - t_launch;
+                srv->kernel_ms += kms;
             }
             for (size_t i = 0; i < n; i++) {
                 Ticket &tk = srv->tickets[ids[i]];
@@ -299,9 +306,10 @@ extern "C" int st_server_submit(StServer *srv, uint32_t scene_id, const StView *
     t.scene = scene_id;
     t.view = *view;
     t.bytes = output_bytes(srv->cfg, *view);
+    t.t_submit = now_ms();
     t.buf = srv->pool.get(t.bytes, &t.cls);
     if (!t.buf) return fail(ST_ERR_MEMORY, "pinned output allocation failed");
-    t.t_submit = now_ms();
+    const double t_alloc = now_ms() - t.t_submit;
     {
         std::lock_guard<std::mutex> lk(srv->mu);
         if (scene_id >= srv->scenes.size() || !srv->scenes[scene_id].h || srv->scenes[scene_id].removing) {
@@ -313,6 +321,7 @@ extern "C" int st_server_submit(StServer *srv, uint32_t scene_id, const StView *
         srv->tickets.emplace(id, t);
         srv->queue.push_back(id);
         srv->submitted++;
+        srv->alloc_ms += t_alloc;
         *ticket = id;
     }
     srv->cv_work.notify_one();
@@ -381,6 +390,10 @@ extern "C" int st_server_stats(StServer *srv, StServerStats *s) {
         for (auto &r : srv->scenes) s->resident_scenes += r.h ? 1 : 0;
         s->queued = (uint32_t)srv->queue.size();
         s->mean_batch = srv->launches ? (double)srv->batched / srv->launches : 0.0;
+        s->prep_ms = srv->prep_ms;
+        s->launch_ms = srv->launch_ms;
+        s->kernel_ms = srv->kernel_ms;
+        s->alloc_ms = srv->alloc_ms;
         lat = srv->latencies;
     }
     if (!lat.empty()) {

@@ -21,7 +21,9 @@ class ServerStats(C.Structure):
     _fields_ = [("submitted", C.c_uint64), ("completed", C.c_uint64), ("launches", C.c_uint64),
                 ("rays", C.c_uint64), ("resident_bytes", C.c_uint64), ("budget_bytes", C.c_uint64),
                 ("resident_scenes", C.c_uint32), ("queued", C.c_uint32), ("latency_p50_ms", C.c_double),
-                ("latency_p99_ms", C.c_double), ("latency_max_ms", C.c_double), ("mean_batch", C.c_double)]
+                ("latency_p99_ms", C.c_double), ("latency_max_ms", C.c_double), ("mean_batch", C.c_double),
+                ("prep_ms", C.c_double), ("launch_ms", C.c_double), ("kernel_ms", C.c_double),
+                ("alloc_ms", C.c_double)]
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}
