@@ -6,18 +6,22 @@
  *   k_primary_dirs  one lane per traced row: walks the row with the
  *                   reference's sequential `current += right` fp32 sum
  *                   (render/raytracer.h:156-170) and stores every pixel's
- *                   unnormalised `current` vector (the render kernel
+ *                   unnormalised `current` vector (the primary stage
  *                   normalises it). Rows of every request of a launch are
  *                   flattened into one grid.
- *   k_render        one wavefront per 8x8 pixel tile, the whole per-pixel
- *                   path of st_trace.h per lane. Tiles of all requests of a
- *                   launch form one list; in persistent mode each wavefront
- *                   keeps pulling the next tile from a global counter until
- *                   the list is exhausted (the loop's exit condition is the
- *                   same for every wave), otherwise tile = wave id with an
- *                   XCD-aware block order. A single request is just a list
- *                   with one entry, so plain and batched launches run the
- *                   same code.
+ *   k_primary       one wavefront per 8x8 pixel tile: primary ray and the
+ *                   Beauty shading entry (st_wave.h). Tiles of all requests
+ *                   of a launch form one list; in persistent mode each
+ *                   wavefront keeps pulling the next tile from a global
+ *                   counter until the list is exhausted (the loop's exit
+ *                   condition is the same for every wave), otherwise tile =
+ *                   wave id with an XCD-aware block order. A single request
+ *                   is just a list with one entry, so plain and batched
+ *                   launches run the same code.
+ *   k_shadow, k_shade, k_bounce
+ *                   the Beauty depth loop as queue stages, one launch per
+ *                   stage and level (st_wave.h).
+ *   k_counts        per-request shadow/bounce ray totals.
  */
 #include <hip/hip_runtime.h>
 
@@ -31,7 +35,7 @@
 #include <mutex>
 #include <vector>
 
-#include "st_trace.h"
+#include "st_wave.h"
 #include "st_lbvh.h"
 #include "st_mesh_lbvh.h"
 #include "../st_mesh.h"
@@ -41,11 +45,12 @@
 
 #define WAVES_PER_BLOCK 4
 #define BLOCK_THREADS (64 * WAVES_PER_BLOCK)
-/* Two waves per SIMD: measured fastest on gfx950 (the fully inlined path
-   needs ~250 VGPRs; the compiler keeps 256 and spills the rest rather than
-   dropping to one wave per SIMD). */
-#ifndef K_RENDER_ATTR
-#define K_RENDER_ATTR __attribute__((amdgpu_waves_per_eu(2)))
+/* Launch attributes of the primary stage (occupancy experiments). */
+#ifndef K_PRIMARY_ATTR
+#define K_PRIMARY_ATTR
+#endif
+#ifndef K_SHADOW_ATTR
+#define K_SHADOW_ATTR
 #endif
 
 /* Every use of the shared stream, arena and staging buffer holds this lock
@@ -82,7 +87,7 @@ struct KReq {
     uint32_t row_base;     /* first flattened row of this request */
     uint32_t tile_count;
     uint32_t out_base;     /* first flattened window pixel (display resolve) */
-    uint32_t pad;
+    uint32_t count_block;  /* first block of k_counts for this request */
     uint32_t *bgra;        /* window image (width1-1)*(height1-1), or NULL */
     uint64_t pad2;
 };
@@ -97,103 +102,409 @@ __device__ static inline uint32_t find_req(const KReq *reqs, uint32_t n, uint32_
     return lo;
 }
 
-__global__ void __launch_bounds__(256) k_primary_dirs(const KReq *reqs, uint32_t nreq, uint32_t total_rows) {
-    uint32_t row = blockIdx.x * blockDim.x + threadIdx.x;
-    if (row >= total_rows) return;
-    uint32_t r = find_req(reqs, nreq, row, true);
-    const StDView &v = reqs[r].v;
-    uint32_t y = row - reqs[r].row_base;
-    if (y >= v.height1) return;
-    const float4 rs = reinterpret_cast<const float4 *>(v.row_starts)[y];
-    sv3 current = V3(rs.x, rs.y, rs.z);
-    const sv3 right = V3(v.right[0], v.right[1], v.right[2]);
-    float4 *out = reinterpret_cast<float4 *>(v.dirs) + (size_t)y * v.width1;
-    for (uint32_t x = 0; x < v.width1; x++) {
-        out[x] = make_float4(current.x, current.y, current.z, 0.0f);
-        current = v_add(current, right);
+/* One wave per DIRS_ROWS rows, one lane per row. The sum along a row is
+   sequential, so each row lane walks DIRS_STEP pixels at a time into an
+   LDS tile, and then the whole wave stores the tile one row at a time, 64
+   consecutive pixels per store (coalesced), instead of one scattered store
+   per row and pixel. */
+#define DIRS_ROWS 16u
+#define DIRS_STEP 64u
+__global__ void __launch_bounds__(64) k_primary_dirs(const KReq *reqs, uint32_t nreq, uint32_t total_rows) {
+    __shared__ float4 tile[DIRS_ROWS][DIRS_STEP + 1];
+    __shared__ uint32_t row_w[DIRS_ROWS];
+    __shared__ float4 *row_out[DIRS_ROWS];
+    const uint32_t lane = threadIdx.x;
+    const uint32_t row = blockIdx.x * DIRS_ROWS + lane;
+    uint32_t width = 0;
+    float4 *out = nullptr;
+    sv3 current = V3s(0), right = V3s(0);
+    if (lane < DIRS_ROWS && row < total_rows) {
+        const uint32_t r = find_req(reqs, nreq, row, true);
+        const StDView &v = reqs[r].v;
+        const uint32_t y = row - reqs[r].row_base;
+        if (y < v.height1) {
+            const float4 rs = ldg4(v.row_starts, y);
+            current = V3(rs.x, rs.y, rs.z);
+            right = V3(v.right[0], v.right[1], v.right[2]);
+            width = v.width1;
+            out = reinterpret_cast<float4 *>(v.dirs) + (size_t)y * v.width1;
+        }
+    }
+    if (lane < DIRS_ROWS) {
+        row_w[lane] = width;
+        row_out[lane] = out;
+    }
+    uint32_t max_width = width;
+    for (int off = 32; off > 0; off >>= 1) max_width = max(max_width, (uint32_t)__shfl_xor((int)max_width, off, 64));
+    __syncthreads();
+    uint32_t rw[DIRS_ROWS];
+    float4 *ro[DIRS_ROWS];
+    for (uint32_t j = 0; j < DIRS_ROWS; j++) {
+        rw[j] = row_w[j];
+        ro[j] = row_out[j];
+    }
+    for (uint32_t x0 = 0; x0 < max_width; x0 += DIRS_STEP) {
+        /* sums past the row's end are computed but never stored */
+        if (lane < DIRS_ROWS) {
+#pragma unroll
+            for (uint32_t k = 0; k < DIRS_STEP; k++) {
+                tile[lane][k] = make_float4(current.x, current.y, current.z, 0.0f);
+                current = v_add(current, right);
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (uint32_t j = 0; j < DIRS_ROWS; j++) {
+            if (x0 + lane < rw[j]) {
+                const float4 t = tile[j][lane];
+                st_vf4 val = {t.x, t.y, t.z, t.w};
+                ((ST_GLOBAL st_vf4 *)ro[j])[x0 + lane] = val;
+            }
+        }
+        __syncthreads();
     }
 }
 
-__device__ static inline uint32_t wave_sum(uint32_t v) {
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
+/* Launch tile `tile`'s pixel for lane `lane`; false outside the traced grid. */
+__device__ static inline bool tile_pixel(const StDView &v, uint32_t tile, uint32_t lane, uint32_t &x, uint32_t &y) {
+    const uint32_t local = tile - v.tile_base;
+    const uint32_t tx = local % v.tiles_x, ty = local / v.tiles_x;
+    const uint32_t tw = 1u << v.tile_shift, th = 64u >> v.tile_shift;
+    x = tx * tw + (lane & (tw - 1u));
+    y = ty * th + (lane >> v.tile_shift);
+    return x < v.width1 && y < v.height1;
 }
 
-__device__ static void render_tile(const StDScene *scenes, const KReq *reqs, uint32_t nreq, uint32_t tile,
-                                   uint32_t *scene_stack, uint32_t *mesh_stack) {
+/* Primary stage of one tile (raytracer.h:81-134 up to the Beauty depth
+   loop): the primary ray over all primitives with the screen-space gate, the
+   hit record, and then either the finished pixel (Depth/Normals/UVs,
+   misses, emissive hits, depth 0) or the Beauty shading entry
+   (surface.h:146-170) and the first level of its depth loop. */
+__device__ static inline __attribute__((always_inline)) void primary_tile(const StDScene *scenes, const KReq *reqs,
+                                                                         uint32_t nreq, uint32_t tile,
+                                                                         uint32_t *scene_stack, uint32_t *mesh_stack,
+                                                                         const KWave &w) {
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t r = find_req(reqs, nreq, tile, false);
     const StDView v = reqs[r].v;
     const StDScene sc = scenes[v.scene];
-    const uint32_t local = tile - v.tile_base;
-    const uint32_t tx = local % v.tiles_x, ty = local / v.tiles_x;
-    const uint32_t tw = 1u << v.tile_shift, th = 64u >> v.tile_shift;
-    const uint32_t x = tx * tw + (lane & (tw - 1u)), y = ty * th + (lane >> v.tile_shift);
-    uint32_t n_shadow = 0, n_bounce = 0, n_primary = 0;
-    if (x < v.width1 && y < v.height1) {
-        const size_t pix = (size_t)y * v.width1 + x;
-        const float4 d4 = reinterpret_cast<const float4 *>(v.dirs)[pix];
-        DTrace tr = {};
-        tr.closest_mesh_hit.object_type = ST_PRIM_MESH;
-        tr.depth = v.depth;
-        tr.mesh_stack_size = sc.mesh_stack_size;
-        tr.scene_stack = scene_stack;
-        tr.mesh_stack = mesh_stack;
-        float4 rgba;
-        double depth;
-        StHitRec hrec;
+    uint32_t x, y;
+    const bool valid = tile_pixel(v, tile, lane, x, y);
+    bool entered = false;   /* entered the Beauty depth loop */
+    if (valid) {
+        const uint32_t pix = y * v.width1 + x;
+        const float4 d4 = ldg4(v.dirs, pix);
+        DTrace tr;
+        trace_init(tr, sc, scene_stack, mesh_stack, 0.0f);
+        DRay ray;
+        ray.origin = V3(v.cam_pos[0], v.cam_pos[1], v.cam_pos[2]);
         /* dir = normVec3(current) (render/raytracer.h:159) */
-        trace_pixel(sc, v, tr, v_norm(V3(d4.x, d4.y, d4.z)), x, y, rgba, depth, hrec);
-        /* FloatPixel is 24 bytes: store as three 8-byte words */
-        uint2 *px = reinterpret_cast<uint2 *>(static_cast<char *>(v.pixels) + 24 * pix);
-        px[0] = make_uint2(__float_as_uint(rgba.x), __float_as_uint(rgba.y));
-        px[1] = make_uint2(__float_as_uint(rgba.z), __float_as_uint(rgba.w));
-        unsigned long long db = (unsigned long long)__double_as_longlong(depth);
-        px[2] = make_uint2((uint32_t)db, (uint32_t)(db >> 32));
-        if (v.hits) {
-            int4 *h = reinterpret_cast<int4 *>(v.hits) + pix;
-            *h = make_int4(hrec.prim_id, hrec.object_type, hrec.triangle_id, __float_as_int(hrec.distance));
+        ray.direction = v_norm(V3(d4.x, d4.y, d4.z));
+        ray.direction_reciprocal = v_rcp(ray.direction);
+        DHit &hit = tr.closest_hit;
+        hit.distance = hit.distance_squared = INFINITY;
+        hit.cone_angle = v.cone_angle;
+        hit.cone_width = 0;
+        tr.primary_triangle = -1;
+        const bool found = hit_primitives(ray, tr, sc, sc.scene_leaf_ids, sc.prim_count, false, v.rects, x, y);
+        if (v.hits)
+            stgi4(v.hits, pix, found ? (int)hit.object_id : -1, found ? (int)hit.object_type : 0,
+                  found ? tr.primary_triangle : -1, __float_as_int(found ? hit.distance : INFINITY));
+        const float closest_distance = found ? hit.distance : INFINITY;
+        float z = INFINITY;
+        sv3 color = V3s(0);
+        if (found) {
+            sq4 cr;
+            cr.v = V3(v.cam_rot_inv[0], v.cam_rot_inv[1], v.cam_rot_inv[2]);
+            cr.w = v.cam_rot_inv[3];
+            z = q_rot(v_sub(hit.position, ray.origin), cr).z;
+            if (v.mode == ST_MODE_BEAUTY) {
+                const DMat M = load_mat(sc, hit.material_id);
+                if (M.is & ST_EMISSIVE) {
+                    color = hit.from_behind ? V3s(0) : M.emission;
+                } else if (v.depth) {
+                    DPix p;
+                    p.color = V3s(0);
+                    p.throughput = V3s(1);
+                    p.vo = ray.origin;
+                    p.vd = ray.direction;
+                    p.ref = V3s(0);
+                    p.bdir = V3s(0);
+                    p.prim_index = hit.object_id;
+                    p.alb = M.albedo;
+                    sv2 dUV = V2(0, 0);
+                    if (M.use && M.texture_count) {
+                        DShaded s = shaded_of(p);
+                        apply_textures(tr, sc, s, M, dUV, true);
+                        p.alb = s.albedo;
+                    }
+                    p.duv_y = dUV.y;
+                    p.NdotRd = 0;
+                    p.ior = 0;
+                    p.max_distance = hit.distance;
+                    p.hit_d2 = hit.distance_squared;
+                    p.z = z;
+                    p.closest_distance = closest_distance;
+                    p.cur_dist = tr.current_hit.distance;
+                    p.material_id = hit.material_id;
+                    p.depth_left = v.depth;
+                    p.flags = 0;
+                    p.n_shadow = p.n_bounce = 0;
+                    begin_level(w, 0, sc, v, M, hit, p, tile * 64u + lane);
+                    entered = true;
+                }
+            } else if (v.mode == ST_MODE_DEPTH) {
+                color = V3s(4 / hit.distance);
+            } else if (v.mode == ST_MODE_NORMALS) {
+                const DMat M = load_mat(sc, hit.material_id);
+                if (M.texture_count > 1 && (M.use & ST_NORMAL_MAP)) {
+                    sv2 uv = V2(hit.uv.x * M.uv_repeat.x, hit.uv.y * M.uv_repeat.y);
+                    hit.uv_area /= M.uv_repeat.x / M.uv_repeat.y;
+                    sv2 dUV = V2(0, 0);
+                    dUV.x = duv_ray_cone(hit.NdotV, hit.cone_width, hit.area, hit.uv_area);
+                    sq4 rot = normal_rotation(sample_normal(sc, M.tex1, uv, dUV));
+                    hit.normal = q_rot(hit.normal, rot);
+                }
+                color = V3(0.5f * (hit.normal.x + 1), 0.5f * (hit.normal.y + 1), 0.5f * (hit.normal.z + 1));
+            } else {
+                color = V3(0.5f * hit.uv.x, 0.5f * hit.uv.y, 0.5f);
+            }
         }
-        n_primary = 1;
-        n_shadow = tr.n_shadow;
-        n_bounce = tr.n_bounce;
+        if (!entered) finish_pixel(sc, v, pix, color, found, false, z, closest_distance);
     }
-    n_primary = wave_sum(n_primary);
-    n_shadow = wave_sum(n_shadow);
-    n_bounce = wave_sum(n_bounce);
-    if (lane == 0 && v.counters) {
-        atomicAdd(v.counters + 0, (unsigned long long)n_primary);
-        atomicAdd(v.counters + 1, (unsigned long long)n_shadow);
-        atomicAdd(v.counters + 2, (unsigned long long)n_bounce);
+    if (w.n && !entered) {
+        pix_store_idle(w, tile * 64u + lane);
+        record_counts(w, tile * 64u + lane, 0, 0);
     }
 }
 
-__global__ void __launch_bounds__(BLOCK_THREADS) K_RENDER_ATTR k_render(const StDScene *scenes, const KReq *reqs, uint32_t nreq,
-                                                           uint32_t total_tiles, uint32_t *tile_counter,
-                                                           uint32_t persistent, uint32_t xcd_blocks,
-                                                           uint32_t scene_entries, uint32_t mesh_entries) {
-    /* traversal stacks live in LDS, one column per lane (no bank conflicts):
-       [wave][entry][lane] */
-    extern __shared__ uint32_t st_lds[];
-    const uint32_t wave = threadIdx.x >> 6;
-    uint32_t *wave_base = st_lds + (size_t)wave * 64u * (scene_entries + mesh_entries) + (threadIdx.x & 63u);
-    uint32_t *scene_stack = wave_base;
-    uint32_t *mesh_stack = wave_base + 64u * scene_entries;
-    if (persistent) {
-        for (;;) {
-            uint32_t t = 0;
-            if ((threadIdx.x & 63u) == 0) t = atomicAdd(tile_counter, 1u);
-            t = __shfl(t, 0, 64);
-            if (t >= total_tiles) break;
-            render_tile(scenes, reqs, nreq, t, scene_stack, mesh_stack);
+/* Traversal stacks live in LDS, one column per lane (no bank conflicts):
+   [wave][entry][lane]. */
+#define ST_LDS_STACKS(scene_entries, mesh_entries)                                                              \
+    extern __shared__ uint32_t st_lds[];                                                                        \
+    uint32_t *wave_base_ = st_lds + (size_t)(threadIdx.x >> 6) * 64u * ((scene_entries) + (mesh_entries)) +     \
+                           (threadIdx.x & 63u);                                                                 \
+    uint32_t *scene_stack = wave_base_;                                                                         \
+    uint32_t *mesh_stack = wave_base_ + 64u * (scene_entries)
+
+__global__ void __launch_bounds__(BLOCK_THREADS) K_PRIMARY_ATTR
+k_primary(const StDScene *__restrict__ scenes, const KReq *__restrict__ reqs, uint32_t nreq, uint32_t total_tiles,
+          uint32_t *tile_counter, uint32_t persistent, uint32_t xcd_blocks, uint32_t scene_entries,
+          uint32_t mesh_entries, KWave w) {
+    ST_LDS_STACKS(scene_entries, mesh_entries);
+    /* One call site, so primary_tile is inlined once. Persistent: waves
+       claim tiles from the global counter until it runs past the end.
+       Otherwise each wave renders one tile: blocks are dealt to the 8 XCDs
+       round-robin, so each XCD gets a contiguous run of tiles and
+       neighbouring tiles share its L2. */
+    const uint32_t b = blockIdx.x;
+    uint32_t t = ((b % 8u) * (xcd_blocks / 8u) + b / 8u) * WAVES_PER_BLOCK + (threadIdx.x >> 6);
+    for (uint32_t round = 0;; round++) {
+        if (persistent) {
+            uint32_t c = 0;
+            if ((threadIdx.x & 63u) == 0) c = atomicAdd(tile_counter, 1u);
+            t = c;
+        } else if (round > 0) {
+            break;
         }
-    } else {
-        /* blocks are dealt to the 8 XCDs round-robin; give each XCD a
-           contiguous run of tiles so neighbouring tiles share its L2 */
-        uint32_t b = blockIdx.x;
-        uint32_t block = (b % 8u) * (xcd_blocks / 8u) + b / 8u;
-        uint32_t t = block * WAVES_PER_BLOCK + wave;
-        if (t < total_tiles) render_tile(scenes, reqs, nreq, t, scene_stack, mesh_stack);
+        /* the tile is wave-uniform: say so, so that the request, view and
+           scene records load into scalar registers */
+        t = __builtin_amdgcn_readfirstlane(t);
+        if (t >= total_tiles) break;
+        primary_tile(scenes, reqs, nreq, t, scene_stack, mesh_stack, w);
+    }
+}
+
+/* Wave loop over `count` queue entries. entry(i, slot, aux) decodes entry i
+   into its state slot (and an extra word) and says whether it is live.
+   Waves take 64-entry chunks in a fixed interleaved order (no shared
+   counter: same-address atomics serialise). Lanes are grouped by request
+   and f(request, slot, aux) runs once per group with a wave-uniform
+   request, so the view and scene records load into scalar registers. */
+template <class E, class F>
+__device__ static inline __attribute__((always_inline)) void drain_queue(const KReq *reqs, uint32_t nreq,
+                                                                        uint32_t count, E &&entry, F &&f) {
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t waves = gridDim.x * WAVES_PER_BLOCK;
+    for (uint32_t chunk = __builtin_amdgcn_readfirstlane(blockIdx.x * WAVES_PER_BLOCK + (threadIdx.x >> 6));
+         (size_t)chunk * 64 < count; chunk += waves) {
+        const uint32_t qi = chunk * 64 + lane;
+        uint32_t slot = 0, aux = 0;
+        bool pending = qi < count && entry(qi, slot, aux);
+        const uint32_t r = pending ? find_req(reqs, nreq, slot >> 6, false) : 0u;
+        while (__ballot(pending)) {
+            if (pending) {
+                const uint32_t r0 = __builtin_amdgcn_readfirstlane(r);
+                if (r == r0) {
+                    pending = false;
+                    f(r0, slot, aux);
+                }
+            }
+        }
+    }
+}
+
+__device__ static inline uint32_t queue_count(const KWave &w, uint32_t level, uint32_t which) {
+    return __builtin_amdgcn_readfirstlane(ldgu(w.ctr, (size_t)level * WC_PER_LEVEL + which));
+}
+
+/* Shadow stage: any-hit traversal of the level's shadow rays. */
+__global__ void __launch_bounds__(BLOCK_THREADS) K_SHADOW_ATTR k_shadow(const StDScene *__restrict__ scenes,
+                                                          const KReq *__restrict__ reqs, uint32_t nreq, KWave w,
+                                                          uint32_t level, uint32_t scene_entries,
+                                                          uint32_t mesh_entries) {
+    ST_LDS_STACKS(scene_entries, mesh_entries);
+    /* level 0: every (light, slot) pair, light-major, so a wave traces one
+       tile's rays to one light (coherent); the pixel's light mask says which
+       exist. Later levels: the compacted queue. */
+    const uint32_t count = level ? queue_count(w, level, WC_SHADOW) : w.lmax * w.n;
+    drain_queue(reqs, nreq, count,
+                [&](uint32_t qi, uint32_t &slot, uint32_t &light) {
+                    if (level) {
+                        slot = ldgu(w.shadow_q, (size_t)qi * 2);
+                        light = ldgu(w.shadow_q, (size_t)qi * 2 + 1);
+                        return true;
+                    }
+                    light = qi / w.n;
+                    slot = qi - light * w.n;
+                    return ((ldgu(w.mask, (size_t)(light >> 5) * w.n + slot) >> (light & 31u)) & 1u) != 0;
+                },
+                [&](uint32_t r, uint32_t slot, uint32_t light) {
+                    const StDScene sc = scenes[reqs[r].v.scene];
+                    DTrace tr;
+                    trace_init(tr, sc, scene_stack, mesh_stack, __uint_as_float(ST_DIST_UNSET));
+                    shadow_ray(w, sc, tr, slot, light);
+                });
+}
+
+/* Shade stage: lights, emissive quads and glow of the level's pixels; each
+   either queues its bounce ray or is finished. */
+__global__ void __launch_bounds__(BLOCK_THREADS) k_shade(const StDScene *__restrict__ scenes,
+                                                         const KReq *__restrict__ reqs, uint32_t nreq, KWave w,
+                                                         uint32_t level) {
+    const uint32_t count = level ? queue_count(w, level, WC_SHADE) : w.n;
+    drain_queue(reqs, nreq, count,
+                [&](uint32_t qi, uint32_t &slot, uint32_t &) {
+                    if (level) {
+                        slot = ldgu(w.shade_q[level & 1], qi);
+                        return true;
+                    }
+                    slot = qi;
+                    return pix_depth_left(w, slot) != 0;
+                },
+                [&](uint32_t r, uint32_t slot, uint32_t) {
+                    const StDView v = reqs[r].v;
+                    const StDScene sc = scenes[v.scene];
+                    DPix p = pix_load(w, slot);
+                    DTrace tr;
+                    trace_init(tr, sc, nullptr, nullptr, p.cur_dist);
+                    const bool bounce = shade_level(w, sc, tr, slot, p);
+                    const uint32_t q = wave_append(w.ctr + (size_t)level * WC_PER_LEVEL + WC_BOUNCE, bounce);
+                    if (bounce) {
+                        p.n_bounce++;
+                        ((ST_GLOBAL uint32_t *)w.bounce_q)[q] = slot;
+                        pix_store_all(w, slot, p);
+                    } else {
+                        uint32_t x, y;
+                        tile_pixel(v, slot >> 6, slot & 63u, x, y);
+                        finish_pixel(sc, v, y * v.width1 + x, p.color, true, (p.flags & PF_LIGHTS_SHADED) != 0,
+                                     p.z, p.closest_distance);
+                        record_counts(w, slot, p.n_shadow, p.n_bounce);
+                    }
+                });
+}
+
+/* Bounce stage (surface.h:221-240): closest-hit traversal of the level's
+   bounce rays; a hit that is not an emissive quad starts the next level. */
+__global__ void __launch_bounds__(BLOCK_THREADS) k_bounce(const StDScene *__restrict__ scenes,
+                                                          const KReq *__restrict__ reqs, uint32_t nreq, KWave w,
+                                                          uint32_t level, uint32_t scene_entries,
+                                                          uint32_t mesh_entries) {
+    ST_LDS_STACKS(scene_entries, mesh_entries);
+    const uint32_t count = queue_count(w, level, WC_BOUNCE);
+    drain_queue(reqs, nreq, count,
+                [&](uint32_t qi, uint32_t &slot, uint32_t &) {
+                    slot = ldgu(w.bounce_q, qi);
+                    return true;
+                },
+                [&](uint32_t r, uint32_t slot, uint32_t) {
+                    const StDView v = reqs[r].v;
+                    const StDScene sc = scenes[v.scene];
+                    DPix p = pix_load(w, slot);
+                    DTrace tr;
+                    trace_init(tr, sc, scene_stack, mesh_stack, p.cur_dist);
+                    DRay ray;
+                    ray.origin = p.pos;
+                    ray.direction = p.bdir;
+                    DHit &hit = tr.closest_hit;
+                    hit.distance = hit.distance_squared = INFINITY;
+                    const bool found = trace_scene(ray, tr, sc, false);
+                    p.cur_dist = tr.current_hit.distance;
+                    bool done = !found;
+                    if (found) {
+                        p.prim_index = hit.object_id;
+                        const DMat M = load_mat(sc, hit.material_id);
+                        p.alb = M.albedo;
+                        if (M.use && M.texture_count) {
+                            DShaded s = shaded_of(p);
+                            sv2 dUV = V2(0, p.duv_y);
+                            apply_textures(tr, sc, s, M, dUV, false);
+                            p.alb = s.albedo;
+                        }
+                        if (hit.object_type == ST_PRIM_QUAD && (M.is & ST_EMISSIVE)) {
+                            p.color = hit.from_behind ? V3s(0) : M.emission;
+                            done = true;
+                        } else {
+                            if (M.brdf != ST_BRDF_COOK_TORRANCE) p.throughput = v_mul(p.throughput, M.reflectivity);
+                            p.vo = ray.origin;
+                            p.vd = ray.direction;
+                            p.material_id = hit.material_id;
+                            p.hit_d2 = hit.distance_squared;
+                            begin_level(w, level + 1, sc, v, M, hit, p, slot);
+                        }
+                    }
+                    if (done) {
+                        uint32_t x, y;
+                        tile_pixel(v, slot >> 6, slot & 63u, x, y);
+                        finish_pixel(sc, v, y * v.width1 + x, p.color, true, (p.flags & PF_LIGHTS_SHADED) != 0,
+                                     p.z, p.closest_distance);
+                        record_counts(w, slot, p.n_shadow, p.n_bounce);
+                    }
+                });
+}
+
+/* Per-request shadow and bounce ray totals from the per-slot counts: block b
+   sums COUNT_TILES tiles of one request (reqs[r].count_block = its first) and
+   adds them with one atomic per counter. */
+#define COUNT_TILES 64u
+__global__ void __launch_bounds__(256) k_counts(const KReq *__restrict__ reqs, uint32_t nreq,
+                                                const uint32_t *__restrict__ counts) {
+    __shared__ unsigned long long part[2][4];
+    uint32_t lo = 0, hi = nreq - 1;
+    while (lo < hi) {
+        uint32_t mid = (lo + hi + 1) >> 1;
+        if (reqs[mid].count_block <= blockIdx.x) lo = mid; else hi = mid - 1;
+    }
+    const KReq &q = reqs[lo];
+    const uint32_t t0 = q.v.tile_base + (blockIdx.x - q.count_block) * COUNT_TILES;
+    const uint32_t t1 = min(t0 + COUNT_TILES, q.v.tile_base + q.tile_count);
+    unsigned long long sh = 0, bo = 0;
+    for (size_t s = (size_t)t0 * 64 + threadIdx.x; s < (size_t)t1 * 64; s += 256) {
+        sh += ldgu(counts, 2 * s);
+        bo += ldgu(counts, 2 * s + 1);
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        sh += __shfl_xor(sh, off, 64);
+        bo += __shfl_xor(bo, off, 64);
+    }
+    if ((threadIdx.x & 63u) == 0) {
+        part[0][threadIdx.x >> 6] = sh;
+        part[1][threadIdx.x >> 6] = bo;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0 && q.v.counters) {
+        atomicAdd(q.v.counters + 1, part[0][0] + part[0][1] + part[0][2] + part[0][3]);
+        atomicAdd(q.v.counters + 2, part[1][0] + part[1][1] + part[1][2] + part[1][3]);
     }
 }
 
@@ -713,6 +1024,14 @@ extern "C" int st_scene_rebuild(StSceneHandle *h, uint32_t mode, float *ms) {
 
 /* ---- launches ----------------------------------------------------------- */
 
+/* blocks of BLOCK_THREADS that are resident on the whole device at once */
+static uint32_t resident_blocks(const void *kernel, size_t lds) {
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, BLOCK_THREADS, lds) != hipSuccess || per_cu < 1)
+        per_cu = 1;
+    return (uint32_t)per_cu * (uint32_t)g.cus;
+}
+
 struct LaunchPlan {
     uint32_t nreq, total_rows, total_tiles, nscenes;
     size_t work_bytes, stage_bytes;
@@ -769,7 +1088,23 @@ int st_rt_launch_prepared(StSceneHandle *const *scenes, StHostView *hv, uint32_t
         rect_off[i] = up;
         up += round256((scenes[i]->host->prim_count + 1) * 16ull);
     }
-    size_t work = up + round256(n * 24ull) + per_req_out + 256;
+    /* wavefront buffers of the Beauty depth loop: levels = largest depth,
+       shadow planes = largest light count among Beauty requests */
+    uint32_t levels = 0, lmax = 1;
+    for (uint32_t i = 0; i < n; i++)
+        if (hv[i].dv.mode == ST_MODE_BEAUTY && hv[i].dv.depth) {
+            if (hv[i].dv.depth > levels) levels = hv[i].dv.depth;
+            if (scenes[i]->host->light_count > lmax) lmax = scenes[i]->host->light_count;
+        }
+    const size_t slots = (size_t)tiles * 64;
+    size_t wave_bytes = 0;
+    if (levels) {
+        if (slots > 0xFFFFFFC0ull || slots * lmax > 0xFFFFFFC0ull) return set_err(ST_ERR_ARG, "batch too large");
+        wave_bytes = round256(slots * 16 * ST_PLANES) + 2 * round256(slots * lmax * 8) + round256(slots * 4) +
+                     3 * round256(slots * 4) + round256((size_t)levels * WC_PER_LEVEL * 4) + round256(slots * 8) +
+                     round256(slots * 4 * ((lmax + 31) / 32)) + 9 * 256;
+    }
+    size_t work = up + round256(n * 24ull) + per_req_out + wave_bytes + 256;
     if ((rc = arena_reserve(&g.work, work))) return rc;
     if ((rc = staging_reserve(up))) return rc;
     char *st = g.staging;
@@ -780,7 +1115,7 @@ int st_rt_launch_prepared(StSceneHandle *const *scenes, StHostView *hv, uint32_t
     StDScene *sc_d = (StDScene *)(blk + o_scenes);
     unsigned long long *cnt_d = (unsigned long long *)arena_take(&g.work, n * 24ull);
     for (uint32_t k = 0; k < nuniq; k++) sc_h[k] = uniq[k]->dev;
-    uint32_t row_base = 0, out_base = 0;
+    uint32_t row_base = 0, out_base = 0, count_blocks = 0;
     for (uint32_t i = 0; i < n; i++) {
         size_t px = (size_t)hv[i].dv.width1 * hv[i].dv.height1;
         StDView &dv = hv[i].dv;
@@ -798,15 +1133,37 @@ int st_rt_launch_prepared(StSceneHandle *const *scenes, StHostView *hv, uint32_t
         reqs_h[i].v = dv;
         reqs_h[i].row_base = row_base;
         reqs_h[i].out_base = out_base;
+        reqs_h[i].tile_count = dv.tiles_x * ((dv.height1 + (64u >> dv.tile_shift) - 1) / (64u >> dv.tile_shift));
+        reqs_h[i].count_block = count_blocks;
+        count_blocks += (reqs_h[i].tile_count + COUNT_TILES - 1) / COUNT_TILES;
         uint32_t wpx = (dv.width1 - 1) * (dv.height1 - 1);
         reqs_h[i].bgra = (bgra_host && bgra_host[i]) ? (uint32_t *)arena_take(&g.work, (size_t)wpx * 4) : NULL;
         row_base += dv.height1;
         out_base += wpx;
     }
+    KWave w;
+    memset(&w, 0, sizeof w);
+    if (levels) {
+        w.n = (uint32_t)slots;
+        w.lmax = lmax;
+        w.st = (float *)arena_take(&g.work, slots * 16 * ST_PLANES);
+        w.shadow_q = (uint32_t *)arena_take(&g.work, slots * lmax * 8);
+        w.shadow_res = (uint32_t *)arena_take(&g.work, slots * lmax * 8);
+        w.occ_d2 = (float *)arena_take(&g.work, slots * 4);
+        w.shade_q[0] = (uint32_t *)arena_take(&g.work, slots * 4);
+        w.shade_q[1] = (uint32_t *)arena_take(&g.work, slots * 4);
+        w.bounce_q = (uint32_t *)arena_take(&g.work, slots * 4);
+        w.ctr = (uint32_t *)arena_take(&g.work, (size_t)levels * WC_PER_LEVEL * 4);
+        w.counts = (uint32_t *)arena_take(&g.work, slots * 8);
+        w.mask = (uint32_t *)arena_take(&g.work, slots * 4 * ((lmax + 31) / 32));
+    }
     HIP_TRY(hipMemcpyAsync(blk, st, up, hipMemcpyHostToDevice, g.stream));
     hipLaunchKernelGGL(k_zero_u64, dim3((3 * n + 255) / 256), dim3(256), 0, g.stream, cnt_d, 3 * n);
     hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, g.stream, g.tile_counter, 64u);
-    hipLaunchKernelGGL(k_primary_dirs, dim3((rows + 255) / 256), dim3(256), 0, g.stream, reqs_d, n, rows);
+    if (levels)
+        hipLaunchKernelGGL(k_zero_u32, dim3((levels * WC_PER_LEVEL + 255) / 256), dim3(256), 0, g.stream, w.ctr,
+                           levels * WC_PER_LEVEL);
+    hipLaunchKernelGGL(k_primary_dirs, dim3((rows + DIRS_ROWS - 1) / DIRS_ROWS), dim3(64), 0, g.stream, reqs_d, n, rows);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(g.ev[0], g.stream));
     uint32_t blocks;
@@ -827,9 +1184,32 @@ int st_rt_launch_prepared(StSceneHandle *const *scenes, StHostView *hv, uint32_t
     }
     size_t lds = (size_t)BLOCK_THREADS * (s_entries + m_entries) * sizeof(uint32_t);
     if (lds > 160 * 1024) return set_err(ST_ERR_SCENE, "traversal stacks need %zu B of LDS", lds);
-    hipLaunchKernelGGL(k_render, dim3(blocks), dim3(BLOCK_THREADS), lds, g.stream, (const StDScene *)sc_d,
+    hipLaunchKernelGGL(k_primary, dim3(blocks), dim3(BLOCK_THREADS), lds, g.stream, (const StDScene *)sc_d,
                        (const KReq *)reqs_d, n, tiles, g.tile_counter, (uint32_t)persistent, xcd_blocks, s_entries,
-                       m_entries);
+                       m_entries, w);
+    HIP_TRY(hipGetLastError());
+    /* the depth loop: the queue kernels read their queue lengths on the
+       device, so the sequence needs no host round trip. Each runs exactly
+       as many waves as fit on the device at once: they take queue chunks in
+       a fixed interleaved order, so a wave that had to wait for a slot would
+       run its whole share late. */
+    if (levels) {
+        const uint32_t b_shadow = resident_blocks(reinterpret_cast<const void *>(k_shadow), lds);
+        const uint32_t b_shade = resident_blocks(reinterpret_cast<const void *>(k_shade), 0);
+        const uint32_t b_bounce = resident_blocks(reinterpret_cast<const void *>(k_bounce), lds);
+        for (uint32_t lv = 0; lv < levels; lv++) {
+            hipLaunchKernelGGL(k_shadow, dim3(b_shadow), dim3(BLOCK_THREADS), lds, g.stream, (const StDScene *)sc_d,
+                               (const KReq *)reqs_d, n, w, lv, s_entries, m_entries);
+            hipLaunchKernelGGL(k_shade, dim3(b_shade), dim3(BLOCK_THREADS), 0, g.stream, (const StDScene *)sc_d,
+                               (const KReq *)reqs_d, n, w, lv);
+            if (lv + 1 < levels)
+                hipLaunchKernelGGL(k_bounce, dim3(b_bounce), dim3(BLOCK_THREADS), lds, g.stream,
+                                   (const StDScene *)sc_d, (const KReq *)reqs_d, n, w, lv, s_entries, m_entries);
+        }
+    }
+    if (levels && count_blocks)
+        hipLaunchKernelGGL(k_counts, dim3(count_blocks), dim3(256), 0, g.stream, (const KReq *)reqs_d, n,
+                           (const uint32_t *)w.counts);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(g.ev[1], g.stream));
     if (bgra_host && out_px) {
@@ -853,7 +1233,7 @@ int st_rt_launch_prepared(StSceneHandle *const *scenes, StHostView *hv, uint32_t
     HIP_TRY(hipStreamSynchronize(g.stream));
     if (counts)
         for (uint32_t i = 0; i < n; i++) {
-            counts[i].primary = cnt_h[3 * i];
+            counts[i].primary = (uint64_t)hv[i].dv.width1 * hv[i].dv.height1;
             counts[i].shadow = cnt_h[3 * i + 1];
             counts[i].bounce = cnt_h[3 * i + 2];
         }

@@ -1,19 +1,19 @@
 /*
  * st_trace.h - per-ray device code of the tracing path for gfx950.
  *
- * One lane traces one QCAA grid pixel through the full reference path:
- * primary visibility over all primitives in scene-BVH leaf order with the
- * screen-space-bounds gate (SlimTracin render/raytracer.h:81-134,
+ * The building blocks of one lane's ray: primary visibility over all
+ * primitives in scene-BVH leaf order with the screen-space-bounds gate
+ * (SlimTracin render/raytracer.h:81-134,
  * render/shaders/intersection/primitives.h:9-111), two-level BVH traversal
  * (render/shaders/trace.h:8-76, intersection/mesh.h:38-111), the implicit and
- * triangle intersection shaders, and Beauty/Depth/Normal/UV shading
- * (closest_hit/surface.h, lights.h, debug.h). Hit IDs must equal the
- * reference CPU renderer's, so every arithmetic step keeps the reference's
- * operation order and the shading passes keep the reference's hit-state
- * data flow (shadow and area-light passes overwrite the closest hit, exactly
- * as the reference's shared `Trace` does). The values the reference leaves
- * uninitialised (hitTetrahedron's local hit, an unset dUV.v) are zero here,
- * as in the CPU oracle.
+ * triangle intersection shaders, and the shading functions of
+ * closest_hit/surface.h, lights.h and debug.h (the stages that sequence them
+ * are in st_wave.h). Hit IDs must equal the reference CPU renderer's, so
+ * every arithmetic step keeps the reference's operation order and the
+ * reference's hit-state data flow (shadow and area-light passes overwrite
+ * the closest hit, exactly as the reference's shared `Trace` does). The
+ * values the reference leaves uninitialised (hitTetrahedron's local hit, an
+ * unset dUV.v) are zero here, as in the CPU oracle.
  *
  * Memory: primitives, materials and lights are tiny and stay in L2/scalar
  * caches; BVH node pairs are one aligned 64-byte segment (four 16-byte
@@ -71,9 +71,8 @@ struct DTrace {
     DSphereHit sphere_hit;
     DHit closest_hit, closest_mesh_hit, current_hit;
     DRay local;
-    uint32_t depth, mesh_stack_size;
+    uint32_t mesh_stack_size;
     int32_t primary_triangle;
-    uint32_t n_shadow, n_bounce;
     uint32_t *scene_stack, *mesh_stack;   /* lane columns of LDS: entry i at [i * ST_STACK_STRIDE] */
 };
 
@@ -95,9 +94,68 @@ struct DMat {
 
 /* ---- loads -------------------------------------------------------------- */
 
+/* Scene data is read through global-address-space pointers. The scene
+   record holds plain C pointers, and a load through a plain (generic)
+   pointer is a flat load, which also counts against the LDS wait counter:
+   every wait for a traversal-stack read would then wait for all outstanding
+   node and triangle fetches too. */
+#define ST_GLOBAL __attribute__((address_space(1)))
+typedef float st_vf4 __attribute__((ext_vector_type(4)));
+typedef int st_vi4 __attribute__((ext_vector_type(4)));
+
+/* i-th 16-byte element at base */
+DFN float4 ldg4(const void *base, size_t i) {
+    st_vf4 v = ((const ST_GLOBAL st_vf4 *)base)[i];
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+DFN int4 ldgi4(const void *base, size_t i) {
+    st_vi4 v = ((const ST_GLOBAL st_vi4 *)base)[i];
+    return make_int4(v.x, v.y, v.z, v.w);
+}
+DFN uint32_t ldgu(const uint32_t *base, size_t i) { return ((const ST_GLOBAL uint32_t *)base)[i]; }
+typedef uint32_t st_vu2 __attribute__((ext_vector_type(2)));
+/* i-th 8-byte element at base */
+DFN void stg2(void *base, size_t i, uint32_t a, uint32_t b) {
+    st_vu2 v = {a, b};
+    ((ST_GLOBAL st_vu2 *)base)[i] = v;
+}
+/* i-th 16-byte element at base */
+DFN void stgi4(void *base, size_t i, int a, int b, int c, int d) {
+    st_vi4 v = {a, b, c, d};
+    ((ST_GLOBAL st_vi4 *)base)[i] = v;
+}
+DFN StDMesh load_mesh(const StDScene &sc, uint32_t i) {
+    int4 a = ldgi4(sc.meshes + i, 0);
+    StDMesh m = {};
+    m.node_offset = (uint32_t)a.x;
+    m.tri_offset = (uint32_t)a.y;
+    m.triangle_count = (uint32_t)a.z;
+    m.attrs = (uint32_t)a.w;
+    return m;
+}
+DFN StDTex load_tex(const StDScene &sc, uint32_t i) {
+    const void *p = sc.textures + i;
+    int4 a = ldgi4(p, 0), b = ldgi4(p, 1);
+    StDTex t = {};
+    t.width = (uint32_t)a.x;
+    t.height = (uint32_t)a.y;
+    t.mip_count = (uint32_t)a.z;
+    t.mipmap = (uint32_t)a.w;
+    t.mip_base = (uint32_t)b.x;
+    return t;
+}
+DFN StDMip load_mip(const StDScene &sc, uint32_t i) {
+    int4 a = ldgi4(sc.mips + i, 0);
+    StDMip m;
+    m.width = (uint32_t)a.x;
+    m.height = (uint32_t)a.y;
+    m.quad_offset = (uint64_t)(uint32_t)a.z | ((uint64_t)(uint32_t)a.w << 32);
+    return m;
+}
+
 DFN DPrim load_prim(const StDScene &sc, uint32_t i) {
-    const float4 *p = reinterpret_cast<const float4 *>(sc.prims + i);
-    float4 a = p[0], b = p[1], c = p[2], d = p[3];
+    const void *p = sc.prims + i;
+    float4 a = ldg4(p, 0), b = ldg4(p, 1), c = ldg4(p, 2), d = ldg4(p, 3);
     DPrim o;
     o.rot.v = V3(a.x, a.y, a.z);
     o.rot.w = a.w;
@@ -113,8 +171,8 @@ DFN DPrim load_prim(const StDScene &sc, uint32_t i) {
 }
 
 DFN DMat load_mat(const StDScene &sc, uint32_t i) {
-    const float4 *p = reinterpret_cast<const float4 *>(sc.mats + i);
-    float4 a = p[0], b = p[1], c = p[2], d = p[3], e = p[4];
+    const void *p = sc.mats + i;
+    float4 a = ldg4(p, 0), b = ldg4(p, 1), c = ldg4(p, 2), d = ldg4(p, 3), e = ldg4(p, 4);
     DMat m;
     m.albedo = V3(a.x, a.y, a.z);
     m.reflectivity = V3(a.w, b.x, b.y);
@@ -134,7 +192,7 @@ DFN DMat load_mat(const StDScene &sc, uint32_t i) {
     return m;
 }
 
-DFN uint32_t mat_is(const StDScene &sc, uint32_t i) { return sc.mats[i].bits & 0xFFu; }
+DFN uint32_t mat_is(const StDScene &sc, uint32_t i) { return ldgu(&sc.mats[i].bits, 0) & 0xFFu; }
 
 struct DNodePair {
     sbox lb, rb;
@@ -142,8 +200,8 @@ struct DNodePair {
 };
 
 DFN void load_node(const uint32_t *nodes, uint32_t i, sbox &b, uint32_t &first, uint32_t &count) {
-    const float4 *p = reinterpret_cast<const float4 *>(nodes + 8 * (size_t)i);
-    float4 a = p[0], c = p[1];
+    const void *p = nodes + 8 * (size_t)i;
+    float4 a = ldg4(p, 0), c = ldg4(p, 1);
     b.min = V3(a.x, a.y, a.z);
     b.max = V3(a.w, c.x, c.y);
     first = __float_as_uint(c.z);
@@ -151,8 +209,8 @@ DFN void load_node(const uint32_t *nodes, uint32_t i, sbox &b, uint32_t &first, 
 }
 
 DFN DNodePair load_pair(const uint32_t *nodes, uint32_t left) {
-    const float4 *p = reinterpret_cast<const float4 *>(nodes + 8 * (size_t)left);
-    float4 a = p[0], c = p[1], d = p[2], e = p[3];
+    const void *p = nodes + 8 * (size_t)left;
+    float4 a = ldg4(p, 0), c = ldg4(p, 1), d = ldg4(p, 2), e = ldg4(p, 3);
     DNodePair o;
     o.lb.min = V3(a.x, a.y, a.z);
     o.lb.max = V3(a.w, c.x, c.y);
@@ -465,8 +523,8 @@ DFN bool hit_triangles(DTrace &tr, const StDTriHot *tris, uint32_t first, uint32
     DHit &closest = tr.closest_mesh_hit;
     bool found = false;
     for (uint32_t i = 0; i < count; i++) {
-        const float4 *p = reinterpret_cast<const float4 *>(tris + first + i);
-        float4 a = p[0], b = p[1], c = p[2], d = p[3];
+        const void *p = tris + first + i;
+        float4 a = ldg4(p, 0), b = ldg4(p, 1), c = ldg4(p, 2), d = ldg4(p, 3);
         sv3 tpos = V3(c.y, c.z, c.w);
         sv3 tnrm = V3(d.x, d.y, d.z);
         if (hit_plane(tpos, tnrm, ray.origin, ray.direction, hit) && hit.distance < closest.distance) {
@@ -496,7 +554,7 @@ DFN bool hit_triangles(DTrace &tr, const StDTriHot *tris, uint32_t first, uint32
 
 /* intersection/mesh.h:38-111 */
 DNOINLINE bool trace_mesh(DTrace &tr, const StDScene &sc, uint32_t mesh_id, bool any_hit) {
-    const StDMesh mesh = sc.meshes[mesh_id];
+    const StDMesh mesh = load_mesh(sc, mesh_id);
     const uint32_t *nodes = sc.mesh_nodes + 8 * (size_t)mesh.node_offset;
     const StDTriHot *tris = sc.tri_hot + mesh.tri_offset;
     DRay &ray = tr.local;
@@ -561,13 +619,13 @@ DNOINLINE bool hit_primitives(const DRay &ray, DTrace &tr, const StDScene &sc, c
     float cone_angle = closest.cone_angle;
     uint32_t hit_prim_id = 0;
     for (uint32_t i = 0; i < count; i++) {
-        uint32_t pid = ids[i];
+        uint32_t pid = ldgu(ids, i);
         DPrim p = load_prim(sc, pid);
         if (any_hit && !(p.flags & ST_IS_SHADOWING)) continue;
         if (rects) {
             /* IS_VISIBLE is folded into the per-view rect: an invisible
                primitive has an empty rect. */
-            int4 r = reinterpret_cast<const int4 *>(rects)[pid];
+            int4 r = ldgi4(rects, pid);
             if ((int32_t)x < r.x || (int32_t)x >= r.z || (int32_t)y < r.y || (int32_t)y >= r.w) continue;
         }
         sv3 Ro, Rd;
@@ -607,11 +665,11 @@ DNOINLINE bool hit_primitives(const DRay &ray, DTrace &tr, const StDScene &sc, c
         DPrim hp = load_prim(sc, hit_prim_id);
         tr.primary_triangle = -1;
         if (closest.object_type == ST_PRIM_MESH) {
-            const StDMesh mesh = sc.meshes[hp.mesh_id];
+            const StDMesh mesh = load_mesh(sc, hp.mesh_id);
             uint32_t tri = closest.object_id;
             tr.primary_triangle = (int32_t)tri;
-            const float4 *q = reinterpret_cast<const float4 *>(sc.tri_cold + mesh.tri_offset + tri);
-            float4 c0 = q[0], c1 = q[1], c2 = q[2], c3 = q[3], c4 = q[4];
+            const void *q = sc.tri_cold + mesh.tri_offset + tri;
+            float4 c0 = ldg4(q, 0), c1 = ldg4(q, 1), c2 = ldg4(q, 2), c3 = ldg4(q, 3), c4 = ldg4(q, 4);
             /* the hot record carried no areas: restore the reference's
                closest_hit->area / uv_area (mesh.h:26-27). Cold record
                floats: vn 0-8, uv 9-14, area 15 (c3.w), uv_area 16 (c4.x) */
@@ -694,15 +752,6 @@ DNOINLINE bool trace_scene(DRay &ray, DTrace &tr, const StDScene &sc, bool any_h
     return found;
 }
 
-DFN bool in_shadow(DRay &ray, DTrace &tr, const StDScene &sc) {
-    tr.n_shadow++;
-    return trace_scene(ray, tr, sc, true);
-}
-DFN bool trace_ray(DRay &ray, DTrace &tr, const StDScene &sc) {
-    tr.n_bounce++;
-    tr.closest_hit.distance = tr.closest_hit.distance_squared = INFINITY;
-    return trace_scene(ray, tr, sc, false);
-}
 
 /* ---- shading ----------------------------------------------------------- */
 
@@ -803,7 +852,7 @@ DFN float duv_ray_cone(float NdotV, float cone_width, float area, float uv_area)
    are clamped to the quad grid so that out-of-range uvs (undefined in the
    reference) cannot leave the texture. */
 DFN sv3 sample_texture(const StDScene &sc, uint32_t tex_id, sv2 UV, sv2 dUV) {
-    const StDTex tex = sc.textures[tex_id];
+    const StDTex tex = load_tex(sc, tex_id);
     uint32_t mip_level = 0;
     if (tex.mipmap) {
         const float pw = dUV.x * (float)tex.width;
@@ -812,7 +861,7 @@ DFN sv3 sample_texture(const StDScene &sc, uint32_t tex_id, sv2 UV, sv2 dUV) {
         const uint32_t last_mip = (uint8_t)(tex.mip_count - 1);
         while (pixel_size > 1 && mip_level < last_mip) { pixel_size /= 2; mip_level += 1; }
     }
-    const StDMip mip = sc.mips[tex.mip_base + mip_level];
+    const StDMip mip = load_mip(sc, tex.mip_base + mip_level);
     float u = UV.x, v = UV.y;
     if (u > 1) u -= (float)((uint32_t)u);
     if (v > 1) v -= (float)((uint32_t)v);
@@ -826,7 +875,7 @@ DFN sv3 sample_texture(const StDScene &sc, uint32_t tex_id, sv2 UV, sv2 dUV) {
     x = x < mip.width ? x : mip.width;
     y = y < mip.height ? y : mip.height;
     const uint32_t *q = reinterpret_cast<const uint32_t *>(sc.texels + 12 * (mip.quad_offset + (uint64_t)y * (mip.width + 1) + x));
-    uint32_t R = q[0], G = q[1], B = q[2];
+    uint32_t R = ldgu(q, 0), G = ldgu(q, 1), B = ldgu(q, 2);
     return V3((float)(R >> 24) * br + ((float)((R >> 16) & 255) * bl + ((float)((R >> 8) & 255) * trr + (float)(R & 255) * tl)),
               (float)(G >> 24) * br + ((float)((G >> 16) & 255) * bl + ((float)((G >> 8) & 255) * trr + (float)(G & 255) * tl)),
               (float)(B >> 24) * br + ((float)((B >> 16) & 255) * bl + ((float)((B >> 8) & 255) * trr + (float)(B & 255) * tl)));
@@ -845,28 +894,6 @@ DFN sq4 normal_rotation(sv3 n) {
     q.v.z = n.x * 0.25f;
     q.w = n.y * 0.25f;
     return q_norm(q);
-}
-
-/* surface.h:9-35 */
-DNOINLINE sv3 shade_lights_direct(DShaded &s, const DMat &M, DRay &ray, DTrace &tr, const StDScene &sc, sv3 color) {
-    DHit &hit = tr.closest_hit;
-    for (uint32_t i = 0; i < sc.light_count; i++) {
-        const float4 *lp = reinterpret_cast<const float4 *>(sc.lights + i);
-        float4 la = lp[0], lb = lp[1];
-        hit.position = V3(la.x, la.y, la.z);
-        s.light_direction = v_sub(hit.position, s.position);
-        hit.distance_squared = v_sqlen(s.light_direction);
-        hit.distance = sqrtf(hit.distance_squared);
-        s.light_direction = v_scale(s.light_direction, 1.0f / hit.distance);
-        float NdotL = v_dot(s.normal, s.light_direction);
-        if (NdotL <= 0) continue;
-        float intensity = la.w / hit.distance_squared;
-        ray.origin = s.position;
-        ray.direction = s.light_direction;
-        if (in_shadow(ray, tr, sc)) continue;
-        color = v_mad(shade_point(s, M, NdotL), v_scale(V3(lb.x, lb.y, lb.z), intensity), color);
-    }
-    return color;
 }
 
 /* surface.h:37-129 */
@@ -934,8 +961,8 @@ DNOINLINE bool shade_emissive_quads(DShaded &s, const DMat &M, DRay &ray, DTrace
             if (shaded_light > 0) {
                 float NdotL = v_dotc(s.normal, s.light_direction);
                 if (NdotL != 0) {
-                    const float4 *mp = reinterpret_cast<const float4 *>(sc.mats + quad.material_id);
-                    float4 mb = mp[1], mc = mp[2];
+                    const void *mp = sc.mats + quad.material_id;
+                    float4 mb = ldg4(mp, 1), mc = ldg4(mp, 2);
                     sv3 emission = V3(mb.z, mb.w, mc.x);
                     color = v_mad(shade_point(s, M, NdotL), v_scale(emission, emission_intensity * shaded_light), color);
                 }
@@ -956,8 +983,8 @@ DNOINLINE bool shade_light_glow(const StDScene &sc, sv3 Ro, sv3 Rd, float max_di
     bool hit_light = false;
     sh.closest_hit_distance = INFINITY;
     for (uint32_t i = 0; i < sc.light_count; i++) {
-        const float4 *lp = reinterpret_cast<const float4 *>(sc.lights + i);
-        float4 la = lp[0], lb = lp[1];
+        const void *lp = sc.lights + i;
+        float4 la = ldg4(lp, 0), lb = ldg4(lp, 1);
         float size = la.w * (1.0f / 16.0f);
         float one_over_radius = 8.0f / size;
         sh.furthest = max_distance * one_over_radius;
@@ -987,138 +1014,6 @@ DFN void apply_textures(DTrace &tr, const StDScene &sc, DShaded &s, const DMat &
     if ((M.use & ST_NORMAL_MAP) && M.texture_count > 1) {
         sq4 rot = normal_rotation(sample_normal(sc, M.tex1, s.uv, dUV));
         hit.normal = q_rot(hit.normal, rot);
-    }
-}
-
-/* surface.h:146-255 */
-DNOINLINE sv3 shade_surface(DRay &ray, DTrace &tr, const StDScene &sc, bool &lights_shaded) {
-    DHit &hit = tr.closest_hit;
-    sv3 color = V3s(0);
-    DMat M = load_mat(sc, hit.material_id);
-    if (M.is & ST_EMISSIVE) return hit.from_behind ? color : M.emission;
-    sv2 dUV = V2(0, 0);
-    DShaded s;
-    s.viewing_origin = ray.origin;
-    s.viewing_direction = ray.direction;
-    s.prim_index = hit.object_id;
-    s.albedo = M.albedo;
-    s.reflected_direction = V3s(0);
-    s.light_direction = V3s(0);
-    if (M.use && M.texture_count) apply_textures(tr, sc, s, M, dUV, true);
-
-    float NdotRd = 0, ior, max_distance = hit.distance;
-    sv3 current_color, throughput = V3s(1);
-    uint32_t depth = tr.depth;
-    while (depth) {
-        s.position = ray.origin = hit.position;
-        s.normal = hit.from_behind ? v_neg(hit.normal) : hit.normal;
-        ior = hit.from_behind ? M.n2_over_n1 : M.n1_over_n2;
-        bool is_ref = (M.is & ST_REFLECTIVE) || (M.is & ST_REFRACTIVE);
-        if (is_ref || M.brdf == ST_BRDF_PHONG) {
-            NdotRd = -inv_dot(s.normal, s.viewing_direction);
-            s.reflected_direction = reflect_with_dot(s.viewing_direction, s.normal, NdotRd);
-        }
-        current_color = is_ref ? V3s(0) : V3(sc.ambient[0], sc.ambient[1], sc.ambient[2]);
-        if (sc.light_count) current_color = shade_lights_direct(s, M, ray, tr, sc, current_color);
-        if (sc.has_emissive_quads)
-            if (shade_emissive_quads(s, M, ray, tr, sc, current_color)) max_distance = hit.distance;
-        color = v_mad(current_color, throughput, color);
-        if (sc.light_count && shade_light_glow(sc, s.viewing_origin, s.viewing_direction, max_distance, tr.sphere_hit, color))
-            lights_shaded = true;
-        if (is_ref && --depth) {
-            if (M.is & ST_REFLECTIVE) {
-                ray.direction = s.reflected_direction;
-            } else {
-                ray.direction = refract_dir(s.viewing_direction, s.normal, ior, NdotRd);
-                if (ray.direction.x == 0 && ray.direction.y == 0 && ray.direction.z == 0)
-                    ray.direction = s.reflected_direction;
-            }
-            if (trace_ray(ray, tr, sc)) {
-                s.prim_index = hit.object_id;
-                M = load_mat(sc, hit.material_id);
-                s.albedo = M.albedo;
-                if (M.use && M.texture_count) apply_textures(tr, sc, s, M, dUV, false);
-                if (hit.object_type == ST_PRIM_QUAD && (M.is & ST_EMISSIVE)) {
-                    color = hit.from_behind ? V3s(0) : M.emission;
-                    break;
-                }
-                if (M.brdf != ST_BRDF_COOK_TORRANCE) throughput = v_mul(throughput, M.reflectivity);
-                s.viewing_origin = ray.origin;
-                s.viewing_direction = ray.direction;
-                continue;
-            }
-        }
-        break;
-    }
-    return color;
-}
-
-/* render/raytracer.h:81-134 for one pixel. Writes the pixel (cleared value
-   when nothing is hit) and, when asked, the primary hit record. */
-DFN void trace_pixel(const StDScene &sc, const StDView &v, DTrace &tr, sv3 dir, uint32_t x, uint32_t y, float4 &rgba,
-                     double &depth_out, StHitRec &hrec) {
-    DRay ray;
-    ray.origin = V3(v.cam_pos[0], v.cam_pos[1], v.cam_pos[2]);
-    ray.direction = dir;
-    ray.direction_reciprocal = v_rcp(dir);
-    tr.closest_hit.distance = tr.closest_hit.distance_squared = INFINITY;
-    tr.closest_hit.cone_angle = v.cone_angle;
-    tr.closest_hit.cone_width = 0;
-    DHit &hit = tr.closest_hit;
-    sv3 Ro = ray.origin, Rd = ray.direction;
-    sv3 color = V3s(0);
-    bool lights_shaded = false;
-    tr.primary_triangle = -1;
-    bool hit_found = hit_primitives(ray, tr, sc, sc.scene_leaf_ids, sc.prim_count, false, v.rects, x, y);
-    hrec.prim_id = hit_found ? (int32_t)hit.object_id : -1;
-    hrec.object_type = hit_found ? (int32_t)hit.object_type : 0;
-    hrec.triangle_id = hit_found ? tr.primary_triangle : -1;
-    hrec.distance = hit_found ? hit.distance : INFINITY;
-    float closest_distance = hit_found ? hit.distance : INFINITY;
-    float z = INFINITY;
-    if (hit_found) {
-        sq4 cr;
-        cr.v = V3(v.cam_rot_inv[0], v.cam_rot_inv[1], v.cam_rot_inv[2]);
-        cr.w = v.cam_rot_inv[3];
-        z = q_rot(v_sub(hit.position, Ro), cr).z;
-        if (v.mode == ST_MODE_BEAUTY) {
-            color = shade_surface(ray, tr, sc, lights_shaded);
-        } else if (v.mode == ST_MODE_DEPTH) {
-            color = V3s(4 / hit.distance);
-        } else if (v.mode == ST_MODE_NORMALS) {
-            DMat M = load_mat(sc, hit.material_id);
-            if (M.texture_count > 1 && (M.use & ST_NORMAL_MAP)) {
-                sv2 uv = V2(hit.uv.x * M.uv_repeat.x, hit.uv.y * M.uv_repeat.y);
-                hit.uv_area /= M.uv_repeat.x / M.uv_repeat.y;
-                sv2 dUV = V2(0, 0);
-                dUV.x = duv_ray_cone(hit.NdotV, hit.cone_width, hit.area, hit.uv_area);
-                sq4 rot = normal_rotation(sample_normal(sc, M.tex1, uv, dUV));
-                hit.normal = q_rot(hit.normal, rot);
-            }
-            color = V3(0.5f * (hit.normal.x + 1), 0.5f * (hit.normal.y + 1), 0.5f * (hit.normal.z + 1));
-        } else {
-            color = V3(0.5f * hit.uv.x, 0.5f * hit.uv.y, 0.5f);
-        }
-    }
-    if (v.mode == ST_MODE_BEAUTY && sc.light_count && !lights_shaded) {
-        if (shade_light_glow(sc, Ro, Rd, closest_distance, tr.sphere_hit, color)) {
-            hit_found = true;
-            z = tr.sphere_hit.closest_hit_distance;
-        }
-    }
-    if (hit_found) {
-        if (v.mode == ST_MODE_BEAUTY) {
-            color.x = tone_map(color.x);
-            color.y = tone_map(color.y);
-            color.z = tone_map(color.z);
-        }
-        color = v_scale(color, 255.0f);
-        color = v_mul(color, color);
-        rgba = make_float4(color.x, color.y, color.z, 1.0f);
-        depth_out = (double)z;
-    } else {
-        rgba = make_float4(0, 0, 0, 0);
-        depth_out = (double)INFINITY;
     }
 }
 

@@ -1,0 +1,369 @@
+/*
+ * st_wave.h - the per-pixel path of the reference renderer split into
+ * wavefront stages (gfx950, wave64).
+ *
+ * The reference traces one pixel start to finish (render/raytracer.h:81-134,
+ * closest_hit/surface.h:146-255): primary ray, then a depth loop that shades
+ * direct lights with one shadow ray each, emissive quads, light glow, and a
+ * reflection/refraction bounce ray. Run as one GPU thread per pixel, that
+ * loop keeps the shading state of a pixel live across three different
+ * traversals and needs far more registers than a lane has. Here each
+ * traversal kind runs in its own kernel over a compacted queue:
+ *
+ *   primary   one wave per 8x8 tile: primary ray, Depth/Normals/UVs output,
+ *             Beauty shading entry and the first level's shadow rays
+ *   shadow    any-hit traversal of every queued shadow ray
+ *   shade     per queued pixel: direct light from the shadow results,
+ *             emissive quads, glow, and the bounce decision
+ *   bounce    closest-hit traversal of every queued bounce ray, then the
+ *             next level's shading entry and shadow rays
+ *
+ * Between stages a pixel's state lives in HBM as planes of float4 (one plane
+ * per four state values, slot = tile * 64 + lane), so that every state read
+ * and write of a wave is one contiguous 1 KiB access.
+ *
+ * Results equal the sequential reference bit for bit, including its shared
+ * hit state: the shading values are recomputed with the reference's
+ * operation order, the closest-hit fields that later code reads after a
+ * shadow ray (the last light's squared distance or its occluder's) are
+ * carried explicitly, and so is the one field of the reference's
+ * `current_hit` that flows from one traversal into a later one (its
+ * `distance`, which a box hit leaves unwritten and copies into the closest
+ * hit). A shadow ray's traversal does not depend on that value, so shadow
+ * rays of one pixel run in parallel and the shade stage replays their
+ * writes to it in light order.
+ */
+#ifndef ST_WAVE_H
+#define ST_WAVE_H
+
+#include "st_trace.h"
+
+/* Per-launch wavefront buffers (device memory, passed by value). */
+struct KWave {
+    float *st;             /* ST_PLANES planes of n float4 */
+    uint32_t n;            /* state slots: 64 per tile of the launch */
+    uint32_t lmax;         /* planes of shadow_res (largest light count) */
+    uint32_t *shadow_q;    /* 2 words per shadow ray: slot, light */
+    uint32_t *shadow_res;  /* [light][slot] 2 words: occluded, current-hit distance bits */
+    float *occ_d2;         /* [slot] occluder's squared distance, last light's shadow ray */
+    uint32_t *shade_q[2];  /* pixels entering a level's shading (ping-pong by level) */
+    uint32_t *bounce_q;    /* pixels with a bounce ray */
+    uint32_t *ctr;         /* WC_PER_LEVEL queue lengths per level */
+    uint32_t *counts;      /* [slot] 2 words: shadow rays, bounce rays of the pixel */
+    uint32_t *mask;        /* [light / 32][slot] level 0: bit set = the light's shadow ray exists */
+};
+
+/* Queue lengths per level. Level 0 has no queues: its shade stage runs over
+   every slot (depth_left = 0 marks the pixels that did not enter the loop)
+   and its shadow stage over every (light, slot) pair, with the pixel's
+   light mask marking the rays that exist. */
+enum { WC_SHADOW = 0, WC_SHADE = 1, WC_BOUNCE = 2, WC_PER_LEVEL = 4 };
+
+/* state planes */
+enum { P_COLOR, P_THRU, P_VO, P_VD, P_POS, P_NRM, P_REF, P_ALB, P_BDIR, P_MISC, ST_PLANES };
+
+#define PF_LIGHTS_SHADED 1u
+
+/* "not written" marker for the current-hit distance of a shadow ray: a
+   signalling-NaN pattern, which no arithmetic result can have */
+#define ST_DIST_UNSET 0x7FA5A5A5u
+
+/* Shading state of one pixel between stages (surface.h:146-255 locals). */
+struct DPix {
+    sv3 color, throughput;
+    sv3 vo, vd;            /* s.viewing_origin, s.viewing_direction */
+    sv3 pos, nrm;          /* s.position, s.normal */
+    sv3 ref, alb;          /* s.reflected_direction, s.albedo */
+    sv3 bdir;              /* bounce ray direction */
+    float z, closest_distance, cur_dist, max_distance, NdotRd, ior, hit_d2, duv_y;
+    uint32_t prim_index, material_id, depth_left, flags;
+    uint32_t n_shadow, n_bounce;   /* rays this pixel has traced so far */
+};
+
+DFN float4 pl_ld(const KWave &w, uint32_t plane, uint32_t slot) { return ldg4(w.st, (size_t)plane * w.n + slot); }
+DFN void pl_st(const KWave &w, uint32_t plane, uint32_t slot, float a, float b, float c, float d) {
+    st_vf4 v = {a, b, c, d};
+    ((ST_GLOBAL st_vf4 *)w.st)[(size_t)plane * w.n + slot] = v;
+}
+DFN void pl_st3(const KWave &w, uint32_t plane, uint32_t slot, sv3 a, float d) { pl_st(w, plane, slot, a.x, a.y, a.z, d); }
+DFN sv3 f4v3(float4 f) { return V3(f.x, f.y, f.z); }
+
+DFN DPix pix_load(const KWave &w, uint32_t slot) {
+    DPix p;
+    float4 f;
+    f = pl_ld(w, P_COLOR, slot); p.color = f4v3(f); p.throughput.x = f.w;
+    f = pl_ld(w, P_THRU, slot); p.throughput.y = f.x; p.throughput.z = f.y; p.z = f.z; p.closest_distance = f.w;
+    f = pl_ld(w, P_VO, slot); p.vo = f4v3(f); p.cur_dist = f.w;
+    f = pl_ld(w, P_VD, slot); p.vd = f4v3(f); p.max_distance = f.w;
+    f = pl_ld(w, P_POS, slot); p.pos = f4v3(f); p.NdotRd = f.w;
+    f = pl_ld(w, P_NRM, slot); p.nrm = f4v3(f); p.ior = f.w;
+    f = pl_ld(w, P_REF, slot); p.ref = f4v3(f); p.hit_d2 = f.w;
+    f = pl_ld(w, P_ALB, slot); p.alb = f4v3(f); p.duv_y = f.w;
+    f = pl_ld(w, P_BDIR, slot); p.bdir = f4v3(f); p.prim_index = __float_as_uint(f.w);
+    int4 m = ldgi4(w.st, (size_t)P_MISC * w.n + slot);
+    p.material_id = (uint32_t)m.x; p.depth_left = (uint32_t)m.y; p.flags = (uint32_t)m.z & 0xFFu;
+    p.n_bounce = (uint32_t)m.z >> 8; p.n_shadow = (uint32_t)m.w;
+    return p;
+}
+DFN void pix_store_all(const KWave &w, uint32_t slot, const DPix &p) {
+    pl_st3(w, P_COLOR, slot, p.color, p.throughput.x);
+    pl_st(w, P_THRU, slot, p.throughput.y, p.throughput.z, p.z, p.closest_distance);
+    pl_st3(w, P_VO, slot, p.vo, p.cur_dist);
+    pl_st3(w, P_VD, slot, p.vd, p.max_distance);
+    pl_st3(w, P_POS, slot, p.pos, p.NdotRd);
+    pl_st3(w, P_NRM, slot, p.nrm, p.ior);
+    pl_st3(w, P_REF, slot, p.ref, p.hit_d2);
+    pl_st3(w, P_ALB, slot, p.alb, p.duv_y);
+    pl_st3(w, P_BDIR, slot, p.bdir, __uint_as_float(p.prim_index));
+    stgi4(w.st, (size_t)P_MISC * w.n + slot, (int)p.material_id, (int)p.depth_left, (int)(p.flags | (p.n_bounce << 8)),
+          (int)p.n_shadow);
+}
+/* a slot that does not enter the depth loop: no level-0 shading, no shadow rays */
+DFN void pix_store_idle(const KWave &w, uint32_t slot) {
+    stgi4(w.st, (size_t)P_MISC * w.n + slot, 0, 0, 0, 0);
+    for (uint32_t wi = 0; wi * 32u < w.lmax; wi++) ((ST_GLOBAL uint32_t *)w.mask)[(size_t)wi * w.n + slot] = 0;
+}
+DFN uint32_t pix_depth_left(const KWave &w, uint32_t slot) { return ldgu((const uint32_t *)w.st, 4 * ((size_t)P_MISC * w.n + slot) + 1); }
+
+/* ---- wave-level queue helpers ------------------------------------------ */
+
+DFN uint32_t lane_id() { return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u)); }
+
+/* Append one entry per active lane with `want` to a queue whose length is
+   *ctr: one atomic per wave. Returns this lane's index (valid if want). */
+DFN uint32_t wave_append(uint32_t *ctr, bool want) {
+    const unsigned long long m = __ballot(want);
+    uint32_t base = 0;
+    if (m) {
+        const uint32_t leader = (uint32_t)__builtin_ctzll(m);
+        if (lane_id() == leader) base = atomicAdd(ctr, (uint32_t)__builtin_popcountll(m));
+        base = __builtin_amdgcn_readlane(base, leader);
+    }
+    return base + __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+/* Reserve `total` (wave-uniform) entries of a queue whose length is *ctr
+   with one atomic from the first active lane; returns the first index. */
+DFN uint32_t wave_reserve(uint32_t *ctr, uint32_t total) {
+    uint32_t base = 0;
+    if (total) {
+        const uint32_t leader = (uint32_t)__builtin_ctzll(__ballot(1));
+        if (lane_id() == leader) base = atomicAdd(ctr, total);
+        base = __builtin_amdgcn_readlane(base, leader);
+    }
+    return base;
+}
+DFN uint32_t lanes_below(unsigned long long m) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+/* ---- shared shading steps ------------------------------------------------ */
+
+/* Per-light set-up of shade_lights_direct (surface.h:9-35): the vector to the
+   light, its squared length and length, and the normalised direction. */
+struct DLightDir {
+    sv3 pos, dir;
+    float d2, dist;
+    float4 la, lb;
+};
+DFN DLightDir light_dir(const StDScene &sc, uint32_t i, sv3 P) {
+    DLightDir L;
+    const void *lp = sc.lights + i;
+    L.la = ldg4(lp, 0);
+    L.lb = ldg4(lp, 1);
+    L.pos = V3(L.la.x, L.la.y, L.la.z);
+    sv3 d = v_sub(L.pos, P);
+    L.d2 = v_sqlen(d);
+    L.dist = sqrtf(L.d2);
+    L.dir = v_scale(d, 1.0f / L.dist);
+    return L;
+}
+
+DFN DShaded shaded_of(const DPix &p) {
+    DShaded s;
+    s.albedo = p.alb;
+    s.position = p.pos;
+    s.normal = p.nrm;
+    s.viewing_direction = p.vd;
+    s.viewing_origin = p.vo;
+    s.reflected_direction = p.ref;
+    s.light_direction = V3s(0);
+    s.uv = V2(0, 0);
+    s.prim_index = p.prim_index;
+    for (int j = 0; j < 4; j++) s.quad_vertices[j] = V3s(0);
+    return s;
+}
+
+/* Fresh traversal state around one ray; `cur_dist` is the reference's
+   current_hit.distance carried over from the pixel's earlier traversals. */
+DFN void trace_init(DTrace &tr, const StDScene &sc, uint32_t *scene_stack, uint32_t *mesh_stack, float cur_dist) {
+    tr = DTrace{};
+    tr.closest_mesh_hit.object_type = ST_PRIM_MESH;
+    tr.current_hit.distance = cur_dist;
+    tr.mesh_stack_size = sc.mesh_stack_size;
+    tr.scene_stack = scene_stack;
+    tr.mesh_stack = mesh_stack;
+}
+
+/* The window pixel's final value (raytracer.h:105-134): light glow along
+   the camera ray when no shading pass hit a light, tone mapping, and the
+   squared 0-255 colour with the camera-space depth. */
+DFN void finish_pixel(const StDScene &sc, const StDView &v, uint32_t pix, sv3 color, bool hit_found,
+                      bool lights_shaded, float z, float closest_distance) {
+    if (v.mode == ST_MODE_BEAUTY && sc.light_count && !lights_shaded) {
+        const float4 d4 = ldg4(v.dirs, pix);
+        const sv3 Rd = v_norm(V3(d4.x, d4.y, d4.z));
+        const sv3 Ro = V3(v.cam_pos[0], v.cam_pos[1], v.cam_pos[2]);
+        DSphereHit sh = {};
+        if (shade_light_glow(sc, Ro, Rd, closest_distance, sh, color)) {
+            hit_found = true;
+            z = sh.closest_hit_distance;
+        }
+    }
+    float4 rgba;
+    double depth;
+    if (hit_found) {
+        if (v.mode == ST_MODE_BEAUTY) {
+            color.x = tone_map(color.x);
+            color.y = tone_map(color.y);
+            color.z = tone_map(color.z);
+        }
+        color = v_scale(color, 255.0f);
+        color = v_mul(color, color);
+        rgba = make_float4(color.x, color.y, color.z, 1.0f);
+        depth = (double)z;
+    } else {
+        rgba = make_float4(0, 0, 0, 0);
+        depth = (double)INFINITY;
+    }
+    /* FloatPixel is 24 bytes: three 8-byte words */
+    void *px = static_cast<char *>(v.pixels) + 24 * (size_t)pix;
+    stg2(px, 0, __float_as_uint(rgba.x), __float_as_uint(rgba.y));
+    stg2(px, 1, __float_as_uint(rgba.z), __float_as_uint(rgba.w));
+    const unsigned long long db = (unsigned long long)__double_as_longlong(depth);
+    stg2(px, 2, (uint32_t)db, (uint32_t)(db >> 32));
+}
+
+/* Top of the reference's depth loop (surface.h:190-200) for the hit in
+   tr.closest_hit, then its light loop's shadow rays (surface.h:9-35): one
+   per light that faces the surface, queued for the shadow stage; the pixel
+   itself is queued for the level's shade stage. Called by the active lanes
+   only; counts the level's shadow rays on the request. */
+DFN void begin_level(const KWave &w, uint32_t level, const StDScene &sc, const StDView &v, const DMat &M,
+                     const DHit &hit, DPix &p, uint32_t slot) {
+    p.pos = hit.position;
+    p.nrm = hit.from_behind ? v_neg(hit.normal) : hit.normal;
+    p.ior = hit.from_behind ? M.n2_over_n1 : M.n1_over_n2;
+    const bool is_ref = (M.is & ST_REFLECTIVE) || (M.is & ST_REFRACTIVE);
+    if (is_ref || M.brdf == ST_BRDF_PHONG) {
+        p.NdotRd = -inv_dot(p.nrm, p.vd);
+        p.ref = reflect_with_dot(p.vd, p.nrm, p.NdotRd);
+    }
+    if (level == 0) {
+        /* light mask words (all of the launch's, zero past this scene's
+           lights); the shadow stage covers every (light, slot) */
+        for (uint32_t wi = 0; wi * 32u < w.lmax; wi++) {
+            uint32_t word = 0;
+            for (uint32_t i = wi * 32u; i < sc.light_count && i < wi * 32u + 32u; i++)
+                if (v_dot(p.nrm, light_dir(sc, i, p.pos).dir) > 0) {
+                    word |= 1u << (i & 31u);
+                    p.n_shadow++;
+                }
+            ((ST_GLOBAL uint32_t *)w.mask)[(size_t)wi * w.n + slot] = word;
+        }
+    } else {
+        uint32_t *ctr = w.ctr + (size_t)level * WC_PER_LEVEL;
+        /* one reservation per wave for all its shadow rays; within it the
+           rays are light-major, so consecutive rays share a light */
+        uint32_t total = 0;
+        for (uint32_t i = 0; i < sc.light_count; i++)
+            total += (uint32_t)__builtin_popcountll(__ballot(v_dot(p.nrm, light_dir(sc, i, p.pos).dir) > 0));
+        uint32_t q = wave_reserve(ctr + WC_SHADOW, total);
+        for (uint32_t i = 0; i < sc.light_count; i++) {
+            const bool want = v_dot(p.nrm, light_dir(sc, i, p.pos).dir) > 0;
+            const unsigned long long m = __ballot(want);
+            if (want) {
+                stg2(w.shadow_q, q + lanes_below(m), slot, i);
+                p.n_shadow++;
+            }
+            q += (uint32_t)__builtin_popcountll(m);
+        }
+        const uint32_t e = wave_append(ctr + WC_SHADE, true);
+        ((ST_GLOBAL uint32_t *)w.shade_q[level & 1])[e] = slot;
+    }
+    pix_store_all(w, slot, p);
+}
+
+DFN void record_counts(const KWave &w, uint32_t slot, uint32_t n_shadow, uint32_t n_bounce) {
+    stg2(w.counts, slot, n_shadow, n_bounce);
+}
+
+/* Shadow stage for one queued ray (surface.h:13-25 and trace.h:8-76 any-hit):
+   records whether the light is occluded and the last current-hit distance
+   the traversal wrote. */
+DFN void shadow_ray(const KWave &w, const StDScene &sc, DTrace &tr, uint32_t slot, uint32_t light) {
+    const float4 p4 = pl_ld(w, P_POS, slot);
+    const DLightDir L = light_dir(sc, light, f4v3(p4));
+    tr.closest_hit.position = L.pos;
+    tr.closest_hit.distance_squared = L.d2;
+    tr.closest_hit.distance = L.dist;
+    DRay ray;
+    ray.origin = f4v3(p4);
+    ray.direction = L.dir;
+    const bool occluded = trace_scene(ray, tr, sc, true);
+    stg2(w.shadow_res, (size_t)light * w.n + slot, occluded ? 1u : 0u, __float_as_uint(tr.current_hit.distance));
+    if (occluded && light + 1 == sc.light_count)
+        ((ST_GLOBAL float *)w.occ_d2)[slot] = tr.closest_hit.distance_squared;
+}
+
+/* Shade stage for one queued pixel (surface.h:200-243): direct light from
+   the shadow results, emissive quads and glow. Returns true when the pixel
+   continues with a bounce ray (direction in p.bdir), false when it is done. */
+DFN bool shade_level(const KWave &w, const StDScene &sc, DTrace &tr, uint32_t slot, DPix &p) {
+    const DMat M = load_mat(sc, p.material_id);
+    const bool is_ref = (M.is & ST_REFLECTIVE) || (M.is & ST_REFRACTIVE);
+    DShaded s = shaded_of(p);
+    sv3 current_color = is_ref ? V3s(0) : V3(sc.ambient[0], sc.ambient[1], sc.ambient[2]);
+    float cur = p.cur_dist;
+    float closest_d2 = p.hit_d2;
+    for (uint32_t i = 0; i < sc.light_count; i++) {
+        const DLightDir L = light_dir(sc, i, s.position);
+        closest_d2 = L.d2;
+        s.light_direction = L.dir;
+        const float NdotL = v_dot(s.normal, s.light_direction);
+        if (NdotL <= 0) continue;
+        const float intensity = L.la.w / L.d2;
+        const uint32_t *res = w.shadow_res + 2 * ((size_t)i * w.n + slot);
+        const uint32_t occluded = ldgu(res, 0), dist_bits = ldgu(res, 1);
+        if (dist_bits != ST_DIST_UNSET) cur = __uint_as_float(dist_bits);
+        if (occluded) {
+            if (i + 1 == sc.light_count) closest_d2 = ((const ST_GLOBAL float *)w.occ_d2)[slot];
+            continue;
+        }
+        current_color = v_mad(shade_point(s, M, NdotL), v_scale(V3(L.lb.x, L.lb.y, L.lb.z), intensity), current_color);
+    }
+    if (sc.has_emissive_quads) {
+        tr.current_hit.distance = cur;
+        tr.closest_hit.distance_squared = closest_d2;
+        DRay ray;
+        ray.origin = s.position;
+        ray.direction = V3s(0);
+        if (shade_emissive_quads(s, M, ray, tr, sc, current_color)) p.max_distance = tr.closest_hit.distance;
+        cur = tr.current_hit.distance;
+    }
+    p.cur_dist = cur;
+    p.color = v_mad(current_color, p.throughput, p.color);
+    DSphereHit sh = {};
+    if (sc.light_count && shade_light_glow(sc, s.viewing_origin, s.viewing_direction, p.max_distance, sh, p.color))
+        p.flags |= PF_LIGHTS_SHADED;
+    if (!(is_ref && --p.depth_left)) return false;
+    if (M.is & ST_REFLECTIVE) {
+        p.bdir = s.reflected_direction;
+    } else {
+        p.bdir = refract_dir(s.viewing_direction, s.normal, p.ior, p.NdotRd);
+        if (p.bdir.x == 0 && p.bdir.y == 0 && p.bdir.z == 0) p.bdir = s.reflected_direction;
+    }
+    return true;
+}
+
+#endif
