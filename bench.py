@@ -7,15 +7,16 @@ noise-displaced geodesic icosphere with 200,000 fp32 triangles (frequency
 and 2 emissive quads; 1920x1080 window (1921x1081 traced QCAA grid), 1 spp,
 trace depth 4, Beauty mode. The scene is resident in HBM before timing;
 a step is the per-frame host set-up (camera basis, SSB rects, ~20 KB of
-uploads), the primary-direction kernel and the render kernel; the frame
-stays in HBM (the PCIe-inclusive rate is reported separately).
+uploads), the primary-direction kernel and the render stages (primary,
+then shadow / shade / bounce per depth-loop level); the frame stays in HBM
+(the PCIe-inclusive rate is reported separately).
 
 Also measured in the same run:
   - plain vs batched launches ("batched"): 256 requests over 8 resident
     scenes of 0.2-5M triangles (noise-displaced spheres, mesh BVHs built on
     the device), resolutions log-uniform 128x128..1920x1080, trace depth
     uniform 1-6, mode uniform over Beauty/Depth/Normal/UV, cameras on a
-    circle around each scene; first as 256 sequential per-request launches
+    sphere around each scene; first as 256 sequential per-request launches
     with a blocking FloatPixel download each (the reference's
     renderSceneOnGPU pattern), then as one persistent launch over one tile
     list, with FloatPixel downloads and with the window images resolved on
@@ -31,12 +32,15 @@ Also measured in the same run:
     admitted against a 200 GB HBM budget), requests of the same mix arriving
     as a Poisson process at a stated rate for a stated time, served by the
     continuous-batching loop; Mrays/s, p50/p99 submit-to-image latency;
-  - roofline of the render kernel: algorithmic bytes of the frame from the
-    CPU oracle's event counters on the same 1920x1080 frame (bytes per event
-    at the reference's record sizes), over the kernel's HIP-event duration;
-    the measured HBM traffic per launch comes from the rocprofv3 FETCH_SIZE /
-    WRITE_SIZE summary kept in profiles/ (calibrated against a copy kernel of
-    known size, --copy-probe);
+  - roofline of the dominant stage launch (the first level's shadow stage,
+    k_shadow): the algorithmic bytes of that level's shadow rays from the CPU
+    oracle's per-ray-class event counters on the same 1920x1080 frame (bytes
+    per event at the reference's record sizes), over the launch's duration
+    from HIP events recorded around each stage on the launch stream; the
+    same for the primary stage and for the whole frame ("stages",
+    "roofline_frame"); the measured HBM traffic per launch comes from the
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary kept in profiles/ (calibrated
+    against a copy kernel of known size, --copy-probe);
   - the reference CPU renderer (compiled from the reference's own sources)
     on the same full frame, all threads used stated;
   - hit-id parity of the GPU frame against the CPU oracle (every pixel).
@@ -73,22 +77,24 @@ def pbr_view(w, h):
 
 def request_mix(rng, nscenes, count):
     """Mixed requests: log-uniform resolution 128x128..1920x1080, depth 1-6,
-    mode uniform over Beauty/Depth/Normal/UV, camera on a circle around the
-    scene looking at it."""
+    mode uniform over Beauty/Depth/Normal/UV, camera on a sphere around the
+    scene (azimuth uniform over +-90 degrees, elevation uniform over 8-50
+    degrees, 14 units from the centre) looking at its centre."""
     views = []
     scene_ids = []
+    target_y, dist = 1.5, 14.0
     for i in range(count):
         w = int(round(np.exp(rng.uniform(np.log(128), np.log(1920)))))
         h = int(round(np.exp(rng.uniform(np.log(128), np.log(1080)))))
         mode = int(rng.choice([MODE_BEAUTY, MODE_DEPTH, MODE_NORMALS, MODE_UVS]))
         depth = int(rng.integers(1, 7))
-        yaw = float(rng.uniform(-1, 1))
-        dist = 14.0
-        # camera on a circle around the scene centre, looking at it: position
-        # from the yaw angle, the rational yaw amount tan(angle/2)
-        ang = 2 * np.arctan(yaw)
-        pos = (float(dist * np.sin(ang)), 6.0, float(-dist * np.cos(ang)))
-        views.append(make_view(width=w, height=h, mode=mode, depth=depth, position=pos, yaw=yaw, pitch=-0.15))
+        # rotations are given as rational amounts tan(angle/2)
+        azim = float(rng.uniform(-np.pi / 2, np.pi / 2))
+        elev = float(np.radians(rng.uniform(8.0, 50.0)))
+        pos = (float(dist * np.cos(elev) * np.sin(azim)), float(target_y + dist * np.sin(elev)),
+               float(-dist * np.cos(elev) * np.cos(azim)))
+        views.append(make_view(width=w, height=h, mode=mode, depth=depth, position=pos, yaw=float(np.tan(azim / 2)),
+                               pitch=float(np.tan(-elev / 2))))
         scene_ids.append(int(rng.integers(0, nscenes)))
     return np.concatenate(views), scene_ids
 
@@ -109,12 +115,13 @@ def cpu_baseline(blob, view, threads, frames=1):
                        f"{threads} threads, median of {frames} frame(s), {rays} rays, {t:.2f} s/frame"), stats, hits
 
 
-TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "k_render_pbr200k_1920x1080.txt")
+TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "k_shadow_level0_pbr200k_1920x1080.txt")
 
 
 def measured_traffic():
-    """HBM bytes per render launch from the committed rocprofv3 PMC summary
-    (line 'traffic_bytes_per_launch: N'), or None when absent."""
+    """HBM bytes per first-level shadow-stage launch from the committed
+    rocprofv3 PMC summary (line 'traffic_bytes_per_launch: N'), or None when
+    absent."""
     try:
         with open(TRAFFIC_SUMMARY) as f:
             for line in f:
@@ -185,7 +192,7 @@ def instanced_leg(args):
         "frame_p50_ms": round(float(np.percentile(frame_ms, 50)), 3),
         "frame_p99_ms": round(float(np.percentile(frame_ms, 99)), 3),
         "lbvh_rebuild_ms": round(float(np.median(rebuild_ms)), 4),
-        "render_kernel_ms": round(float(np.median(kernel_ms)), 3),
+        "render_ms": round(float(np.median(kernel_ms)), 3),
     }
 
 
@@ -250,7 +257,7 @@ def batched_leg(args):
         "plain_latency_p50_ms": round(float(np.percentile(lat_plain, 50)), 3),
         "plain_latency_p99_ms": round(float(np.percentile(lat_plain, 99)), 3),
         "batched_equals_plain": same,
-        "batched_render_kernel_ms": round(kernel_ms, 3),
+        "batched_render_ms": round(kernel_ms, 3),
         "launch_shape": tune.as_dict(),
     }
     for d in dscenes:
@@ -400,8 +407,19 @@ def main():
     rays_per_frame = rays / args.steps
     mrays = rays / elapsed / 1e6
     kernel_ms = float(np.mean(kms))
-    log(f"frame: {rays_per_frame / 1e6:.2f} Mrays, render kernel {kernel_ms:.3f} ms, "
+    log(f"frame: {rays_per_frame / 1e6:.2f} Mrays, render stages {kernel_ms:.3f} ms, "
         f"frame {np.median(frame_ms):.3f} ms, {mrays:.1f} Mrays/s")
+
+    # per-stage durations (events between the stage launches), after the
+    # timed region so that the extra events do not touch the headline figure
+    stage_runs = []
+    for _ in range(max(3, min(args.steps, 10))):
+        _, _, sms = st.render_timed([scene], view, stages=True)
+        stage_runs.append(sms)
+    stage_ms = [float(np.median([r[i] for r in stage_runs])) for i in range(len(stage_runs[0]))]
+    names = st.stage_names(int(view["depth"][0]))
+    stages = {n: round(m, 4) for n, m in zip(names, stage_ms)}
+    log("stages (ms):", stages)
 
     # PCIe-inclusive: a plain frame with the blocking download (renderSceneOnGPU)
     t1 = time.perf_counter()
@@ -427,7 +445,8 @@ def main():
         "p50_ms": round(float(np.percentile(frame_ms, 50)), 4),
         "p99_ms": round(float(np.percentile(frame_ms, 99)), 4),
         "frame_ms_with_download": round(pcie_frame_ms, 4),
-        "render_kernel_ms": round(kernel_ms, 4),
+        "render_ms": round(kernel_ms, 4),
+        "stage_ms": stages,
         "launch_shape": tune.as_dict(),
     }
 
@@ -449,11 +468,21 @@ def main():
         oracle_rays = stats.primary_rays + stats.shadow_rays + stats.bounce_rays
         assert oracle_rays == int(rays_per_frame), (oracle_rays, rays_per_frame)
         frame_bytes = stats.algorithmic_bytes()
-        achieved = frame_bytes / (kernel_ms * 1e-3) / 1e9
-        result["roofline"] = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                              "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": measured_traffic(),
-                              "algorithmic_bytes_per_launch": int(frame_bytes),
-                              "bytes_per_ray": round(frame_bytes / oracle_rays, 2)}
+
+        def line(nbytes, ms):
+            a = nbytes / (ms * 1e-3) / 1e9
+            return {"achieved": round(a, 2), "frac": round(a / HBM_PEAK_GBS, 5),
+                    "algorithmic_bytes_per_launch": int(nbytes), "ms": round(ms, 4)}
+
+        shadow0 = line(stats.traversal_bytes("shadow0"), stages["shadow0"])
+        result["roofline"] = {"bound": "hbm", "achieved": shadow0["achieved"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                              "frac": shadow0["frac"], "traffic": measured_traffic(),
+                              "kernel": "k_shadow, first depth-loop level",
+                              "algorithmic_bytes_per_launch": shadow0["algorithmic_bytes_per_launch"],
+                              "launch_ms": shadow0["ms"]}
+        result["roofline_stages"] = {"primary": line(stats.traversal_bytes("primary"), stages["primary"]),
+                                     "shadow0": shadow0}
+        result["roofline_frame"] = dict(line(frame_bytes, kernel_ms), bytes_per_ray=round(frame_bytes / oracle_rays, 2))
         gp, gh, gc = st.render(scene, view)
         same = bool((gh["prim_id"] == ohits["prim_id"]).all() and (gh["triangle_id"] == ohits["triangle_id"]).all()
                     and (gh["distance"].view(np.uint32) == ohits["distance"].view(np.uint32)).all())

@@ -43,6 +43,7 @@ enum {
 };
 
 #define ST_BATCH_PERSISTENT 1u  /* one persistent launch over a global tile queue */
+#define ST_BATCH_STAGE_TIMES 2u /* st_render_timed: also time every stage launch (st_last_stage_ms) */
 
 typedef struct StSceneHandle StSceneHandle;
 
@@ -136,6 +137,14 @@ int st_resolve_bgra8(const StFloatPixel *grid, uint32_t grid_w, uint32_t grid_h,
    duration measured with HIP events on the launch stream. */
 int st_render_timed(StSceneHandle *const *scenes, const StView *views, uint32_t n, uint32_t flags, float *kernel_ms,
                     StRayCounts *counts);
+
+/* Per-stage durations (ms, HIP events on the launch stream) of the last
+   st_render_timed call made with ST_BATCH_STAGE_TIMES, in launch order: the
+   primary stage, then for each level of the Beauty depth loop its shadow,
+   shade and (all but the last level) bounce stages. *count = number of
+   stages; up to cap values are written. Not in the reference (bench
+   instrumentation). */
+int st_last_stage_ms(float *ms, uint32_t cap, uint32_t *count);
 
 /* Host-side introspection (no device needed): what the library computes on
    the host for a blob and a view. */

@@ -168,6 +168,7 @@ typedef struct {
     uint32_t scene_stack[1024], mesh_stack[256];
     uint8_t depth, mesh_stack_size;
     int32_t primary_triangle; /* triangle index consumed by the last hitPrimitives */
+    uint32_t ray_class, level;  /* class of the ray being traced (ST_ORC_*), depth-loop level */
     StOracleStats st;
 } OTrace;
 
@@ -501,6 +502,7 @@ static inline int hit_triangles(OTrace *tr, const OTri *tris, uint32_t count, in
     for (uint32_t i = 0; i < count; i++) {
         const OTri *t = tris + i;
         tr->st.triangle_tests++;
+        tr->st.class_events[tr->ray_class][2]++;
         if (hit_plane(t->position, t->normal, ray->origin, ray->direction, hit) && hit->distance < closest->distance) {
             V3 UV = sub3(hit->position, t->position);
             UV = vmat3(UV, t->world_to_tangent);
@@ -530,6 +532,7 @@ static int trace_mesh(OTrace *tr, const OMesh *mesh, int any_hit) {
     int found = 0;
     float ld, rd;
     tr->st.root_tests++;
+    tr->st.class_events[tr->ray_class][1]++;
     if (!hit_aabb(&mesh->nodes[0].aabb, ray, closest->distance, &ld)) return 0;
     if (mesh->nodes[0].child_count) return hit_triangles(tr, mesh->triangles, mesh->triangle_count, any_hit);
     const ONode *left = mesh->nodes + mesh->nodes[0].first_child_id, *right, *tmp;
@@ -537,6 +540,7 @@ static int trace_mesh(OTrace *tr, const OMesh *mesh, int any_hit) {
     for (;;) {
         right = left + 1;
         tr->st.node_pairs++;
+        tr->st.class_events[tr->ray_class][0]++;
         int hl = hit_aabb(&left->aabb, ray, closest->distance, &ld);
         int hr = hit_aabb(&right->aabb, ray, closest->distance, &rd);
         if (hl) {
@@ -590,6 +594,7 @@ static int hit_primitives(const ORay *ray, OTrace *tr, const OScene *sc, const u
     for (uint32_t i = 0; i < count; i++) {
         const OPrim *p = sc->prims + ids[i];
         tr->st.primitive_visits++;
+        tr->st.class_events[tr->ray_class][3]++;
         if (any_hit && !(p->flags & ST_IS_SHADOWING)) continue;
         if (check_visibility) {
             if (!(p->flags & ST_IS_VISIBLE)) continue;
@@ -665,6 +670,7 @@ static int trace_scene(ORay *ray, OTrace *tr, const OScene *sc, int any_hit) {
     int found = 0;
     float ld, rd;
     tr->st.root_tests++;
+    tr->st.class_events[tr->ray_class][1]++;
     if (!hit_aabb(&sc->bvh_nodes[0].aabb, ray, tr->closest_hit.distance, &ld)) return 0;
     if (sc->bvh_nodes[0].child_count)
         return hit_primitives(ray, tr, sc, sc->bvh_leaf_ids, sc->prim_count, any_hit, 0, 0, 0);
@@ -674,6 +680,7 @@ static int trace_scene(ORay *ray, OTrace *tr, const OScene *sc, int any_hit) {
     for (;;) {
         right = left + 1;
         tr->st.node_pairs++;
+        tr->st.class_events[tr->ray_class][0]++;
         int hl = hit_aabb(&left->aabb, ray, tr->closest_hit.distance, &ld);
         int hr = hit_aabb(&right->aabb, ray, tr->closest_hit.distance, &rd);
         if (hl) {
@@ -715,10 +722,13 @@ static int trace_scene(ORay *ray, OTrace *tr, const OScene *sc, int any_hit) {
 /* trace.h:85-92 */
 static inline int in_shadow(ORay *ray, OTrace *tr, const OScene *sc) {
     tr->st.shadow_rays++;
+    tr->ray_class = tr->level ? ST_ORC_SHADOW_DEEP : ST_ORC_SHADOW0;
     return trace_scene(ray, tr, sc, 1);
 }
 static inline int trace_ray(ORay *ray, OTrace *tr, const OScene *sc) {
     tr->st.bounce_rays++;
+    tr->ray_class = ST_ORC_BOUNCE;
+    tr->level++;   /* shadow rays after this bounce belong to the next level */
     tr->closest_hit.distance = tr->closest_hit.distance_squared = INFINITY;
     return trace_scene(ray, tr, sc, 0);
 }
@@ -1085,6 +1095,8 @@ static void ray_trace(ORay *ray, OTrace *tr, const OScene *sc, uint32_t mode, St
     V3 color = v3s(0);
     int lights_shaded = 0;
     tr->st.primary_rays++;
+    tr->ray_class = ST_ORC_PRIMARY;
+    tr->level = 0;
     tr->primary_triangle = -1;
     int hit_found = hit_primitives(ray, tr, sc, sc->bvh_leaf_ids, sc->prim_count, 0, 1, x, y);
     if (hrec) {

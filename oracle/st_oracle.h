@@ -25,7 +25,13 @@ typedef struct StOracleStats {
     uint64_t primary_rays, shadow_rays, bounce_rays;
     uint64_t node_pairs, root_tests, triangle_tests, primitive_visits;
     uint64_t mesh_attr_fetches, texel_fetches, material_reads, light_reads;
+    /* the traversal events (node pairs, root tests, triangle tests,
+       primitive visits) split by ray class: primary, shadow rays of the
+       first depth-loop level, shadow rays of later levels, bounce rays */
+    uint64_t class_events[4][4];
 } StOracleStats;
+
+enum { ST_ORC_PRIMARY = 0, ST_ORC_SHADOW0 = 1, ST_ORC_SHADOW_DEEP = 2, ST_ORC_BOUNCE = 3 };
 
 /* Render one frame. pixels and hits have (width+1)*(height+1) entries; hits
    may be NULL. threads > 1 splits rows round-robin with one trace state per

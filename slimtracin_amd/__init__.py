@@ -17,6 +17,7 @@ from .scene import HIT_DTYPE, NODE_DTYPE, PIXEL_DTYPE, PRIM_DTYPE, VIEW_DTYPE, g
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib", "libslimtracin.so")
 
 ST_BATCH_PERSISTENT = 1
+ST_BATCH_STAGE_TIMES = 2
 ST_REBUILD_SAH = 0
 ST_REBUILD_LBVH = 1
 
@@ -224,16 +225,32 @@ def render_batch(scenes, views: np.ndarray, hits: bool = False, persistent: bool
     return pix, hr, list(counts)
 
 
-def render_timed(scenes, views: np.ndarray, persistent: bool = True):
-    """One launch without downloads; returns (render-kernel ms from HIP events, counts)."""
+def render_timed(scenes, views: np.ndarray, persistent: bool = True, stages: bool = False):
+    """One launch without downloads; returns (render ms from HIP events, counts)
+    or, with stages=True, (ms, counts, per-stage ms in launch order: primary,
+    then shadow/shade/bounce per depth-loop level)."""
     n = len(views)
     handles = (C.c_void_p * n)(*[s.handle for s in scenes])
     counts = (RayCounts * n)()
     ms = C.c_float()
     views = np.ascontiguousarray(views, dtype=VIEW_DTYPE)
-    _check(lib().st_render_timed(handles, _ptr(views), C.c_uint32(n), C.c_uint32(ST_BATCH_PERSISTENT if persistent else 0),
-                                 C.byref(ms), counts))
-    return ms.value, list(counts)
+    flags = (ST_BATCH_PERSISTENT if persistent else 0) | (ST_BATCH_STAGE_TIMES if stages else 0)
+    _check(lib().st_render_timed(handles, _ptr(views), C.c_uint32(n), C.c_uint32(flags), C.byref(ms), counts))
+    if not stages:
+        return ms.value, list(counts)
+    buf = (C.c_float * 64)()
+    cnt = C.c_uint32()
+    _check(lib().st_last_stage_ms(buf, C.c_uint32(64), C.byref(cnt)))
+    return ms.value, list(counts), [float(buf[i]) for i in range(min(cnt.value, 64))]
+
+
+def stage_names(levels: int):
+    """Names of the stage launches of a frame whose Beauty depth loop has
+    `levels` levels (the order render_timed(stages=True) reports)."""
+    names = ["primary"]
+    for lv in range(levels):
+        names += [f"shadow{lv}", f"shade{lv}"] + ([f"bounce{lv}"] if lv + 1 < levels else [])
+    return names
 
 
 def render_batch_bgra8(scenes, views: np.ndarray, persistent: bool = True, outs=None):

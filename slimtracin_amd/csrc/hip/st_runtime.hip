@@ -348,19 +348,20 @@ __device__ static inline uint32_t queue_count(const KWave &w, uint32_t level, ui
     return __builtin_amdgcn_readfirstlane(ldgu(w.ctr, (size_t)level * WC_PER_LEVEL + which));
 }
 
-/* Shadow stage: any-hit traversal of the level's shadow rays. */
+/* Shadow stage: any-hit traversal of the level's shadow rays. LEVEL0 (the
+   first level, its own kernel): every (light, slot) pair, light-major, so a
+   wave traces one tile's rays to one light (coherent); the pixel's light
+   mask says which exist. Later levels: the compacted queue. */
+template <bool LEVEL0>
 __global__ void __launch_bounds__(BLOCK_THREADS) K_SHADOW_ATTR k_shadow(const StDScene *__restrict__ scenes,
                                                           const KReq *__restrict__ reqs, uint32_t nreq, KWave w,
                                                           uint32_t level, uint32_t scene_entries,
                                                           uint32_t mesh_entries) {
     ST_LDS_STACKS(scene_entries, mesh_entries);
-    /* level 0: every (light, slot) pair, light-major, so a wave traces one
-       tile's rays to one light (coherent); the pixel's light mask says which
-       exist. Later levels: the compacted queue. */
-    const uint32_t count = level ? queue_count(w, level, WC_SHADOW) : w.lmax * w.n;
+    const uint32_t count = LEVEL0 ? w.lmax * w.n : queue_count(w, level, WC_SHADOW);
     drain_queue(reqs, nreq, count,
                 [&](uint32_t qi, uint32_t &slot, uint32_t &light) {
-                    if (level) {
+                    if (!LEVEL0) {
                         slot = ldgu(w.shadow_q, (size_t)qi * 2);
                         light = ldgu(w.shadow_q, (size_t)qi * 2 + 1);
                         return true;
@@ -378,14 +379,17 @@ __global__ void __launch_bounds__(BLOCK_THREADS) K_SHADOW_ATTR k_shadow(const St
 }
 
 /* Shade stage: lights, emissive quads and glow of the level's pixels; each
-   either queues its bounce ray or is finished. */
+   either queues its bounce ray or is finished. LEVEL0 (own kernel): every
+   slot, skipping those that did not enter the loop; later levels: the
+   level's shade queue. */
+template <bool LEVEL0>
 __global__ void __launch_bounds__(BLOCK_THREADS) k_shade(const StDScene *__restrict__ scenes,
                                                          const KReq *__restrict__ reqs, uint32_t nreq, KWave w,
                                                          uint32_t level) {
-    const uint32_t count = level ? queue_count(w, level, WC_SHADE) : w.n;
+    const uint32_t count = LEVEL0 ? w.n : queue_count(w, level, WC_SHADE);
     drain_queue(reqs, nreq, count,
                 [&](uint32_t qi, uint32_t &slot, uint32_t &) {
-                    if (level) {
+                    if (!LEVEL0) {
                         slot = ldgu(w.shade_q[level & 1], qi);
                         return true;
                     }
@@ -583,6 +587,7 @@ struct Arena {
     size_t cap, used;
 };
 
+#define ST_STAGES_MAX 64
 static struct {
     int ready;
     int device;
@@ -596,6 +601,10 @@ static struct {
     hipEvent_t ev[4];
     int cus;               /* compute units of the device */
     StTuning tune;         /* launch shape chosen by st_set_tuning / st_autotune */
+    int stage_timing;      /* record an event after every stage launch */
+    hipEvent_t stage_ev[ST_STAGES_MAX + 1];
+    uint32_t stage_n;
+    float stage_ms[ST_STAGES_MAX];
 } g;
 
 static int ensure_init(void) {
@@ -1166,6 +1175,17 @@ int st_rt_launch_prepared(StSceneHandle *const *scenes, StHostView *hv, uint32_t
     hipLaunchKernelGGL(k_primary_dirs, dim3((rows + DIRS_ROWS - 1) / DIRS_ROWS), dim3(64), 0, g.stream, reqs_d, n, rows);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(g.ev[0], g.stream));
+    /* per-stage timing (bench): an event before the first stage and after each */
+    uint32_t marks = 0;
+    auto mark = [&]() -> hipError_t {
+        if (!g.stage_timing || marks > ST_STAGES_MAX) return hipSuccess;
+        if (!g.stage_ev[marks]) {
+            hipError_t e = hipEventCreate(&g.stage_ev[marks]);
+            if (e != hipSuccess) return e;
+        }
+        return hipEventRecord(g.stage_ev[marks++], g.stream);
+    };
+    HIP_TRY(mark());
     uint32_t blocks;
     uint32_t xcd_blocks = 0;
     if (persistent) {
@@ -1188,23 +1208,38 @@ int st_rt_launch_prepared(StSceneHandle *const *scenes, StHostView *hv, uint32_t
                        (const KReq *)reqs_d, n, tiles, g.tile_counter, (uint32_t)persistent, xcd_blocks, s_entries,
                        m_entries, w);
     HIP_TRY(hipGetLastError());
+    HIP_TRY(mark());
     /* the depth loop: the queue kernels read their queue lengths on the
        device, so the sequence needs no host round trip. Each runs exactly
        as many waves as fit on the device at once: they take queue chunks in
        a fixed interleaved order, so a wave that had to wait for a slot would
        run its whole share late. */
     if (levels) {
-        const uint32_t b_shadow = resident_blocks(reinterpret_cast<const void *>(k_shadow), lds);
-        const uint32_t b_shade = resident_blocks(reinterpret_cast<const void *>(k_shade), 0);
+        const uint32_t b_shadow0 = resident_blocks(reinterpret_cast<const void *>(k_shadow<true>), lds);
+        const uint32_t b_shadow = resident_blocks(reinterpret_cast<const void *>(k_shadow<false>), lds);
+        const uint32_t b_shade0 = resident_blocks(reinterpret_cast<const void *>(k_shade<true>), 0);
+        const uint32_t b_shade = resident_blocks(reinterpret_cast<const void *>(k_shade<false>), 0);
         const uint32_t b_bounce = resident_blocks(reinterpret_cast<const void *>(k_bounce), lds);
         for (uint32_t lv = 0; lv < levels; lv++) {
-            hipLaunchKernelGGL(k_shadow, dim3(b_shadow), dim3(BLOCK_THREADS), lds, g.stream, (const StDScene *)sc_d,
-                               (const KReq *)reqs_d, n, w, lv, s_entries, m_entries);
-            hipLaunchKernelGGL(k_shade, dim3(b_shade), dim3(BLOCK_THREADS), 0, g.stream, (const StDScene *)sc_d,
-                               (const KReq *)reqs_d, n, w, lv);
-            if (lv + 1 < levels)
+            if (lv == 0)
+                hipLaunchKernelGGL(k_shadow<true>, dim3(b_shadow0), dim3(BLOCK_THREADS), lds, g.stream,
+                                   (const StDScene *)sc_d, (const KReq *)reqs_d, n, w, lv, s_entries, m_entries);
+            else
+                hipLaunchKernelGGL(k_shadow<false>, dim3(b_shadow), dim3(BLOCK_THREADS), lds, g.stream,
+                                   (const StDScene *)sc_d, (const KReq *)reqs_d, n, w, lv, s_entries, m_entries);
+            HIP_TRY(mark());
+            if (lv == 0)
+                hipLaunchKernelGGL(k_shade<true>, dim3(b_shade0), dim3(BLOCK_THREADS), 0, g.stream,
+                                   (const StDScene *)sc_d, (const KReq *)reqs_d, n, w, lv);
+            else
+                hipLaunchKernelGGL(k_shade<false>, dim3(b_shade), dim3(BLOCK_THREADS), 0, g.stream,
+                                   (const StDScene *)sc_d, (const KReq *)reqs_d, n, w, lv);
+            HIP_TRY(mark());
+            if (lv + 1 < levels) {
                 hipLaunchKernelGGL(k_bounce, dim3(b_bounce), dim3(BLOCK_THREADS), lds, g.stream,
                                    (const StDScene *)sc_d, (const KReq *)reqs_d, n, w, lv, s_entries, m_entries);
+                HIP_TRY(mark());
+            }
         }
     }
     if (levels && count_blocks)
@@ -1238,6 +1273,9 @@ int st_rt_launch_prepared(StSceneHandle *const *scenes, StHostView *hv, uint32_t
             counts[i].bounce = cnt_h[3 * i + 2];
         }
     if (kernel_ms) HIP_TRY(hipEventElapsedTime(kernel_ms, g.ev[0], g.ev[1]));
+    g.stage_n = marks ? marks - 1 : 0;
+    for (uint32_t k = 0; k < g.stage_n; k++)
+        HIP_TRY(hipEventElapsedTime(&g.stage_ms[k], g.stage_ev[k], g.stage_ev[k + 1]));
     return rc;
 }
 
@@ -1283,7 +1321,20 @@ extern "C" int st_render_batch(StSceneHandle *const *scenes, const StView *views
 
 extern "C" int st_render_timed(StSceneHandle *const *scenes, const StView *views, uint32_t n, uint32_t flags,
                                float *kernel_ms, StRayCounts *counts) {
-    return launch(scenes, views, n, 0, (flags & ST_BATCH_PERSISTENT) ? 1 : 0, counts, NULL, NULL, NULL, kernel_ms);
+    RT_LOCK;
+    g.stage_timing = (flags & ST_BATCH_STAGE_TIMES) != 0;
+    g.stage_n = 0;
+    int rc = launch(scenes, views, n, 0, (flags & ST_BATCH_PERSISTENT) ? 1 : 0, counts, NULL, NULL, NULL, kernel_ms);
+    g.stage_timing = 0;
+    return rc;
+}
+
+extern "C" int st_last_stage_ms(float *ms, uint32_t cap, uint32_t *count) {
+    RT_LOCK;
+    if (!count) return set_err(ST_ERR_ARG, "null argument");
+    *count = g.stage_n;
+    for (uint32_t k = 0; k < g.stage_n && k < cap && ms; k++) ms[k] = g.stage_ms[k];
+    return ST_OK;
 }
 
 extern "C" int st_render_batch_bgra8(StSceneHandle *const *scenes, const StView *views, uint32_t n,

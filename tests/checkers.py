@@ -20,13 +20,29 @@ ORACLE_SO = os.path.join(ROOT, "oracle", "lib", "libstoracle.so")
 REF_SO = os.path.join(ROOT, "oracle", "_ref", "libslimref.so")
 
 
+RAY_CLASSES = ("primary", "shadow0", "shadow_deep", "bounce")
+TRAVERSAL_EVENTS = ("node_pairs", "root_tests", "triangle_tests", "primitive_visits")
+
+
 class OracleStats(C.Structure):
     _fields_ = [(n, C.c_uint64) for n in (
         "primary_rays", "shadow_rays", "bounce_rays", "node_pairs", "root_tests", "triangle_tests",
-        "primitive_visits", "mesh_attr_fetches", "texel_fetches", "material_reads", "light_reads")]
+        "primitive_visits", "mesh_attr_fetches", "texel_fetches", "material_reads", "light_reads")] + [
+        ("class_events", (C.c_uint64 * 4) * 4)]
 
     def as_dict(self):
-        return {n: int(getattr(self, n)) for n, _ in self._fields_}
+        d = {n: int(getattr(self, n)) for n, _ in self._fields_ if n != "class_events"}
+        for c, cname in enumerate(RAY_CLASSES):
+            for e, ename in enumerate(TRAVERSAL_EVENTS):
+                d[f"{cname}_{ename}"] = int(self.class_events[c][e])
+        return d
+
+    def traversal_bytes(self, ray_class):
+        """Traversal bytes of one ray class (primary, shadow0, shadow_deep,
+        bounce) at the reference's record sizes: node pairs 64, root AABB
+        tests 24, triangle tests 60, primitive records 72."""
+        ev = self.class_events[RAY_CLASSES.index(ray_class)]
+        return 64 * ev[0] + 24 * ev[1] + 60 * ev[2] + 72 * ev[3]
 
     def algorithmic_bytes(self):
         """Bytes the reference algorithm reads, at the reference's record sizes."""

@@ -67,3 +67,4 @@ def test_quaternion_helpers_match_reference():
     a = quat_axis((1, 0, 0), f32(0.3))
     b = quat_axis((0, 1, 0), f32(-0.2))
     assert np.array(quat_mul_norm(a, b), dtype=np.float32).tobytes() == ck.ref_quat_mul_norm(a, b).tobytes()
+
