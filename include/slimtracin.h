@@ -44,6 +44,9 @@ enum {
 
 #define ST_BATCH_PERSISTENT 1u  /* one persistent launch over a global tile queue */
 #define ST_BATCH_STAGE_TIMES 2u /* st_render_timed: also time every stage launch (st_last_stage_ms) */
+#define ST_BATCH_GRAPH 4u       /* capture the launch's device work (upload, every stage kernel) in a HIP graph
+                                   and replay it while the next launches have the same shape (scenes' stack
+                                   sizes, request count, sizes, modes, depths); downloads stay outside */
 
 typedef struct StSceneHandle StSceneHandle;
 
@@ -145,6 +148,10 @@ int st_render_timed(StSceneHandle *const *scenes, const StView *views, uint32_t 
    stages; up to cap values are written. Not in the reference (bench
    instrumentation). */
 int st_last_stage_ms(float *ms, uint32_t cap, uint32_t *count);
+
+/* Graphs captured and graph launches made so far (ST_BATCH_GRAPH). Not in
+   the reference (the reference issues one blocking launch per frame). */
+int st_graph_stats(uint64_t *captures, uint64_t *launches);
 
 /* Host-side introspection (no device needed): what the library computes on
    the host for a blob and a view. */

@@ -18,6 +18,7 @@ LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib", "libs
 
 ST_BATCH_PERSISTENT = 1
 ST_BATCH_STAGE_TIMES = 2
+ST_BATCH_GRAPH = 4
 ST_REBUILD_SAH = 0
 ST_REBUILD_LBVH = 1
 
@@ -209,9 +210,11 @@ def render(scene: DeviceScene, view: np.ndarray, hits: bool = True, out: np.ndar
     return pixels, hrec, counts
 
 
-def render_batch(scenes, views: np.ndarray, hits: bool = False, persistent: bool = True, outs=None):
+def render_batch(scenes, views: np.ndarray, hits: bool = False, persistent: bool = True, outs=None,
+                 graph: bool = False):
     """Fuse len(views) requests into one launch; returns per-request outputs
-    (outs: preallocated FloatPixel arrays, e.g. from pinned_split)."""
+    (outs: preallocated FloatPixel arrays, e.g. from pinned_split). graph:
+    replay a captured HIP graph of the launch while its shape repeats."""
     n = len(views)
     handles = (C.c_void_p * n)(*[s.handle for s in scenes])
     pix = outs if outs is not None else [np.zeros(grid_size(views[i:i + 1]), dtype=PIXEL_DTYPE) for i in range(n)]
@@ -220,12 +223,19 @@ def render_batch(scenes, views: np.ndarray, hits: bool = False, persistent: bool
     hp = (C.c_void_p * n)(*[h.ctypes.data for h in hr]) if hits else None
     counts = (RayCounts * n)()
     views = np.ascontiguousarray(views, dtype=VIEW_DTYPE)
-    _check(lib().st_render_batch(handles, _ptr(views), C.c_uint32(n), pp, hp, counts,
-                                 C.c_uint32(ST_BATCH_PERSISTENT if persistent else 0)))
+    flags = (ST_BATCH_PERSISTENT if persistent else 0) | (ST_BATCH_GRAPH if graph else 0)
+    _check(lib().st_render_batch(handles, _ptr(views), C.c_uint32(n), pp, hp, counts, C.c_uint32(flags)))
     return pix, hr, list(counts)
 
 
-def render_timed(scenes, views: np.ndarray, persistent: bool = True, stages: bool = False):
+def graph_stats():
+    """(graphs captured, graph launches) so far in this process."""
+    c, l = C.c_uint64(), C.c_uint64()
+    _check(lib().st_graph_stats(C.byref(c), C.byref(l)))
+    return c.value, l.value
+
+
+def render_timed(scenes, views: np.ndarray, persistent: bool = True, stages: bool = False, graph: bool = False):
     """One launch without downloads; returns (render ms from HIP events, counts)
     or, with stages=True, (ms, counts, per-stage ms in launch order: primary,
     then shadow/shade/bounce per depth-loop level)."""
@@ -234,7 +244,8 @@ def render_timed(scenes, views: np.ndarray, persistent: bool = True, stages: boo
     counts = (RayCounts * n)()
     ms = C.c_float()
     views = np.ascontiguousarray(views, dtype=VIEW_DTYPE)
-    flags = (ST_BATCH_PERSISTENT if persistent else 0) | (ST_BATCH_STAGE_TIMES if stages else 0)
+    flags = ((ST_BATCH_PERSISTENT if persistent else 0) | (ST_BATCH_STAGE_TIMES if stages else 0)
+             | (ST_BATCH_GRAPH if graph else 0))
     _check(lib().st_render_timed(handles, _ptr(views), C.c_uint32(n), C.c_uint32(flags), C.byref(ms), counts))
     if not stages:
         return ms.value, list(counts)

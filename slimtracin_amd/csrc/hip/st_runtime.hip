@@ -602,6 +602,10 @@ static struct {
     int cus;               /* compute units of the device */
     StTuning tune;         /* launch shape chosen by st_set_tuning / st_autotune */
     int stage_timing;      /* record an event after every stage launch */
+    int graph_mode;        /* capture launches into a graph, replay while the launch shape repeats */
+    hipGraphExec_t graph_exec;
+    std::vector<uint64_t> graph_sig;
+    uint64_t graph_captures, graph_launches;
     hipEvent_t stage_ev[ST_STAGES_MAX + 1];
     uint32_t stage_n;
     float stage_ms[ST_STAGES_MAX];
@@ -1166,26 +1170,8 @@ int st_rt_launch_prepared(StSceneHandle *const *scenes, StHostView *hv, uint32_t
         w.counts = (uint32_t *)arena_take(&g.work, slots * 8);
         w.mask = (uint32_t *)arena_take(&g.work, slots * 4 * ((lmax + 31) / 32));
     }
-    HIP_TRY(hipMemcpyAsync(blk, st, up, hipMemcpyHostToDevice, g.stream));
-    hipLaunchKernelGGL(k_zero_u64, dim3((3 * n + 255) / 256), dim3(256), 0, g.stream, cnt_d, 3 * n);
-    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, g.stream, g.tile_counter, 64u);
-    if (levels)
-        hipLaunchKernelGGL(k_zero_u32, dim3((levels * WC_PER_LEVEL + 255) / 256), dim3(256), 0, g.stream, w.ctr,
-                           levels * WC_PER_LEVEL);
-    hipLaunchKernelGGL(k_primary_dirs, dim3((rows + DIRS_ROWS - 1) / DIRS_ROWS), dim3(64), 0, g.stream, reqs_d, n, rows);
-    HIP_TRY(hipGetLastError());
-    HIP_TRY(hipEventRecord(g.ev[0], g.stream));
-    /* per-stage timing (bench): an event before the first stage and after each */
-    uint32_t marks = 0;
-    auto mark = [&]() -> hipError_t {
-        if (!g.stage_timing || marks > ST_STAGES_MAX) return hipSuccess;
-        if (!g.stage_ev[marks]) {
-            hipError_t e = hipEventCreate(&g.stage_ev[marks]);
-            if (e != hipSuccess) return e;
-        }
-        return hipEventRecord(g.stage_ev[marks++], g.stream);
-    };
-    HIP_TRY(mark());
+    /* launch shape, fixed before anything is enqueued so that a captured
+       graph of this launch can be matched against the next one */
     uint32_t blocks;
     uint32_t xcd_blocks = 0;
     if (persistent) {
@@ -1204,22 +1190,51 @@ int st_rt_launch_prepared(StSceneHandle *const *scenes, StHostView *hv, uint32_t
     }
     size_t lds = (size_t)BLOCK_THREADS * (s_entries + m_entries) * sizeof(uint32_t);
     if (lds > 160 * 1024) return set_err(ST_ERR_SCENE, "traversal stacks need %zu B of LDS", lds);
-    hipLaunchKernelGGL(k_primary, dim3(blocks), dim3(BLOCK_THREADS), lds, g.stream, (const StDScene *)sc_d,
-                       (const KReq *)reqs_d, n, tiles, g.tile_counter, (uint32_t)persistent, xcd_blocks, s_entries,
-                       m_entries, w);
-    HIP_TRY(hipGetLastError());
-    HIP_TRY(mark());
-    /* the depth loop: the queue kernels read their queue lengths on the
-       device, so the sequence needs no host round trip. Each runs exactly
-       as many waves as fit on the device at once: they take queue chunks in
-       a fixed interleaved order, so a wave that had to wait for a slot would
-       run its whole share late. */
+    /* the depth loop's queue kernels run exactly as many waves as fit on
+       the device at once: they take queue chunks in a fixed interleaved
+       order, so a wave that had to wait for a slot would run its whole
+       share late */
+    uint32_t b_shadow0 = 0, b_shadow = 0, b_shade0 = 0, b_shade = 0, b_bounce = 0;
     if (levels) {
-        const uint32_t b_shadow0 = resident_blocks(reinterpret_cast<const void *>(k_shadow<true>), lds);
-        const uint32_t b_shadow = resident_blocks(reinterpret_cast<const void *>(k_shadow<false>), lds);
-        const uint32_t b_shade0 = resident_blocks(reinterpret_cast<const void *>(k_shade<true>), 0);
-        const uint32_t b_shade = resident_blocks(reinterpret_cast<const void *>(k_shade<false>), 0);
-        const uint32_t b_bounce = resident_blocks(reinterpret_cast<const void *>(k_bounce), lds);
+        b_shadow0 = resident_blocks(reinterpret_cast<const void *>(k_shadow<true>), lds);
+        b_shadow = resident_blocks(reinterpret_cast<const void *>(k_shadow<false>), lds);
+        b_shade0 = resident_blocks(reinterpret_cast<const void *>(k_shade<true>), 0);
+        b_shade = resident_blocks(reinterpret_cast<const void *>(k_shade<false>), 0);
+        b_bounce = resident_blocks(reinterpret_cast<const void *>(k_bounce), lds);
+    }
+
+    /* per-stage timing (bench): an event before the first stage and after each */
+    uint32_t marks = 0;
+    const bool graph = g.graph_mode != 0;
+    auto mark = [&]() -> hipError_t {
+        if (graph || !g.stage_timing || marks > ST_STAGES_MAX) return hipSuccess;
+        if (!g.stage_ev[marks]) {
+            hipError_t e = hipEventCreate(&g.stage_ev[marks]);
+            if (e != hipSuccess) return e;
+        }
+        return hipEventRecord(g.stage_ev[marks++], g.stream);
+    };
+    /* everything from the upload to the ray counts; the same stream work is
+       what a graph captures */
+    auto enqueue = [&]() -> int {
+        HIP_TRY(hipMemcpyAsync(blk, st, up, hipMemcpyHostToDevice, g.stream));
+        hipLaunchKernelGGL(k_zero_u64, dim3((3 * n + 255) / 256), dim3(256), 0, g.stream, cnt_d, 3 * n);
+        hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, g.stream, g.tile_counter, 64u);
+        if (levels)
+            hipLaunchKernelGGL(k_zero_u32, dim3((levels * WC_PER_LEVEL + 255) / 256), dim3(256), 0, g.stream, w.ctr,
+                               levels * WC_PER_LEVEL);
+        hipLaunchKernelGGL(k_primary_dirs, dim3((rows + DIRS_ROWS - 1) / DIRS_ROWS), dim3(64), 0, g.stream, reqs_d,
+                           n, rows);
+        HIP_TRY(hipGetLastError());
+        if (!graph) HIP_TRY(hipEventRecord(g.ev[0], g.stream));
+        HIP_TRY(mark());
+        hipLaunchKernelGGL(k_primary, dim3(blocks), dim3(BLOCK_THREADS), lds, g.stream, (const StDScene *)sc_d,
+                           (const KReq *)reqs_d, n, tiles, g.tile_counter, (uint32_t)persistent, xcd_blocks,
+                           s_entries, m_entries, w);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(mark());
+        /* the depth loop: the queue kernels read their queue lengths on the
+           device, so the sequence needs no host round trip */
         for (uint32_t lv = 0; lv < levels; lv++) {
             if (lv == 0)
                 hipLaunchKernelGGL(k_shadow<true>, dim3(b_shadow0), dim3(BLOCK_THREADS), lds, g.stream,
@@ -1241,12 +1256,58 @@ int st_rt_launch_prepared(StSceneHandle *const *scenes, StHostView *hv, uint32_t
                 HIP_TRY(mark());
             }
         }
+        if (levels && count_blocks)
+            hipLaunchKernelGGL(k_counts, dim3(count_blocks), dim3(256), 0, g.stream, (const KReq *)reqs_d, n,
+                               (const uint32_t *)w.counts);
+        HIP_TRY(hipGetLastError());
+        if (!graph) HIP_TRY(hipEventRecord(g.ev[1], g.stream));
+        return ST_OK;
+    };
+
+    if (!graph) {
+        if ((rc = enqueue())) return rc;
+    } else {
+        /* everything a kernel argument or copy depends on: the buffers are
+           at fixed arena offsets, so an equal signature means the captured
+           graph does exactly this launch's work (the per-frame data is in
+           the upload block, copied from the same pinned staging) */
+        std::vector<uint64_t> sig = {n, up, rows, tiles, levels, lmax, count_blocks, blocks, xcd_blocks,
+                                     (uint64_t)persistent, s_entries, m_entries, b_shadow0, b_shadow, b_shade0,
+                                     b_shade, b_bounce, (uint64_t)(uintptr_t)blk, (uint64_t)(uintptr_t)st,
+                                     (uint64_t)(uintptr_t)cnt_d, (uint64_t)(uintptr_t)w.st, w.n};
+        for (uint32_t i = 0; i < n; i++) {
+            const StDView &dv = reqs_h[i].v;
+            sig.insert(sig.end(), {dv.width1, dv.height1, dv.mode, dv.depth, dv.tile_shift,
+                                   (uint64_t)(uintptr_t)dv.dirs, (uint64_t)(uintptr_t)dv.pixels,
+                                   (uint64_t)(uintptr_t)dv.hits, (uint64_t)(uintptr_t)reqs_h[i].bgra});
+        }
+        if (!g.graph_exec || sig != g.graph_sig) {
+            if (g.graph_exec) {
+                (void)hipGraphExecDestroy(g.graph_exec);
+                g.graph_exec = nullptr;
+            }
+            HIP_TRY(hipStreamBeginCapture(g.stream, hipStreamCaptureModeRelaxed));
+            const int erc = enqueue();
+            hipGraph_t gr = nullptr;
+            const hipError_t ee = hipStreamEndCapture(g.stream, &gr);
+            if (erc || ee != hipSuccess) {
+                if (gr) (void)hipGraphDestroy(gr);
+                return erc ? erc : set_err(ST_ERR_HIP, "graph capture failed: %s", hipGetErrorString(ee));
+            }
+            const hipError_t ie = hipGraphInstantiate(&g.graph_exec, gr, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(gr);
+            if (ie != hipSuccess) {
+                g.graph_exec = nullptr;
+                return set_err(ST_ERR_HIP, "graph instantiation failed: %s", hipGetErrorString(ie));
+            }
+            g.graph_sig = sig;
+            g.graph_captures++;
+        }
+        HIP_TRY(hipEventRecord(g.ev[0], g.stream));
+        HIP_TRY(hipGraphLaunch(g.graph_exec, g.stream));
+        HIP_TRY(hipEventRecord(g.ev[1], g.stream));
+        g.graph_launches++;
     }
-    if (levels && count_blocks)
-        hipLaunchKernelGGL(k_counts, dim3(count_blocks), dim3(256), 0, g.stream, (const KReq *)reqs_d, n,
-                           (const uint32_t *)w.counts);
-    HIP_TRY(hipGetLastError());
-    HIP_TRY(hipEventRecord(g.ev[1], g.stream));
     if (bgra_host && out_px) {
         hipLaunchKernelGGL(k_resolve, dim3((uint32_t)((out_px + 255) / 256)), dim3(256), 0, g.stream,
                            (const KReq *)reqs_d, n, (uint32_t)out_px);
@@ -1309,10 +1370,25 @@ extern "C" int st_render(StSceneHandle *scene, const StView *view, StFloatPixel 
     return launch(&scene, view, 1, hits != NULL, (int)g.tune.persistent, counts, &pixels, &hits, NULL, NULL);
 }
 
+/* Graph mode for the duration of one API call (ST_BATCH_GRAPH). */
+struct GraphScope {
+    explicit GraphScope(uint32_t flags) { g.graph_mode = (flags & ST_BATCH_GRAPH) != 0; }
+    ~GraphScope() { g.graph_mode = 0; }
+};
+
+extern "C" int st_graph_stats(uint64_t *captures, uint64_t *launches) {
+    RT_LOCK;
+    if (captures) *captures = g.graph_captures;
+    if (launches) *launches = g.graph_launches;
+    return ST_OK;
+}
+
 extern "C" int st_render_batch(StSceneHandle *const *scenes, const StView *views, uint32_t n,
                                StFloatPixel *const *pixels, StHitRec *const *hits, StRayCounts *counts,
                                uint32_t flags) {
     if (!scenes || !views) return set_err(ST_ERR_ARG, "null argument");
+    RT_LOCK;
+    GraphScope gs(flags);
     int want_hits = 0;
     if (hits)
         for (uint32_t i = 0; i < n; i++) want_hits |= hits[i] != NULL;
@@ -1322,6 +1398,7 @@ extern "C" int st_render_batch(StSceneHandle *const *scenes, const StView *views
 extern "C" int st_render_timed(StSceneHandle *const *scenes, const StView *views, uint32_t n, uint32_t flags,
                                float *kernel_ms, StRayCounts *counts) {
     RT_LOCK;
+    GraphScope gs(flags);
     g.stage_timing = (flags & ST_BATCH_STAGE_TIMES) != 0;
     g.stage_n = 0;
     int rc = launch(scenes, views, n, 0, (flags & ST_BATCH_PERSISTENT) ? 1 : 0, counts, NULL, NULL, NULL, kernel_ms);
@@ -1340,6 +1417,8 @@ extern "C" int st_last_stage_ms(float *ms, uint32_t cap, uint32_t *count) {
 extern "C" int st_render_batch_bgra8(StSceneHandle *const *scenes, const StView *views, uint32_t n,
                                      uint32_t *const *bgra, StRayCounts *counts, uint32_t flags) {
     if (!scenes || !views || !bgra) return set_err(ST_ERR_ARG, "null argument");
+    RT_LOCK;
+    GraphScope gs(flags);
     return launch(scenes, views, n, 0, (flags & ST_BATCH_PERSISTENT) ? 1 : 0, counts, NULL, NULL, bgra, NULL);
 }
 
