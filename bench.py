@@ -421,6 +421,21 @@ def main():
     stages = {n: round(m, 4) for n, m in zip(names, stage_ms)}
     log("stages (ms):", stages)
 
+    # the same frames as HIP-graph replays (the launch captured once)
+    c0, l0 = st.graph_stats()
+    for _ in range(2):
+        st.render_timed([scene], view, graph=True)
+    g_ms = []
+    for _ in range(args.steps):
+        f0 = time.perf_counter()
+        st.render_timed([scene], view, graph=True)
+        g_ms.append((time.perf_counter() - f0) * 1e3)
+    c1, l1 = st.graph_stats()
+    graph = {"frame_ms_eager_p50": round(float(np.median(frame_ms)), 4),
+             "frame_ms_graph_p50": round(float(np.median(g_ms)), 4),
+             "captures": c1 - c0, "replays": l1 - l0 - (c1 - c0)}
+    log("graph:", graph)
+
     # PCIe-inclusive: a plain frame with the blocking download (renderSceneOnGPU)
     t1 = time.perf_counter()
     for _ in range(3):
@@ -447,6 +462,7 @@ def main():
         "frame_ms_with_download": round(pcie_frame_ms, 4),
         "render_ms": round(kernel_ms, 4),
         "stage_ms": stages,
+        "graph": graph,
         "launch_shape": tune.as_dict(),
     }
 

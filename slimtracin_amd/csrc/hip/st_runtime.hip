@@ -647,6 +647,7 @@ void st_rt_unlock(void) { g_mu.unlock(); }
 int st_rt_ensure_init(void) { return ensure_init(); }
 hipStream_t st_rt_stream(void) { return g.stream; }
 int st_rt_error(int code, const char *msg) { return set_err(code, "%s", msg); }
+extern "C" int st_host_error(int code, const char *msg) { return set_err(code, "%s", msg); }
 
 static int arena_reserve(Arena *a, size_t bytes) {
     if (a->cap >= bytes) { a->used = 0; return ST_OK; }

@@ -159,6 +159,9 @@ int st_view_prepare(const StScene *sc, const StView *v, StHostView *hv);
 int st_view_prepare_frame(const StScene *sc, const StSeamFrame *f, const int32_t *screen_bounds, StHostView *hv);
 void st_view_release(StHostView *hv);
 
+/* Record an error message for st_last_error; returns code. */
+int st_host_error(int code, const char *msg);
+
 #ifdef __cplusplus
 }
 #endif
