@@ -127,6 +127,20 @@ int st_get_tuning(StTuning *tuning);
 int st_autotune(StSceneHandle *const *scenes, const StView *views, uint32_t n, uint32_t reps, StTuning *best,
                 float *best_ms);
 
+/* The SSB pass of a view on the device (updateSceneSSB, render/SSB.h:137-173;
+   render launches run it themselves): per primitive the rect as computeSSB
+   leaves screen_bounds (min.x, min.y, max.x, max.y) and the flags with
+   IS_VISIBLE recomputed, the same values as st_host_view_info. */
+int st_ssb(StSceneHandle *scene, const StView *view, int32_t *rects, uint8_t *flags);
+
+/* Picking at n window points (x, y pairs, window pixels from the top
+   left): the closest primitive under each point, or a light's glow sphere
+   in front of it, with the world hit point and its view-space distance, as
+   manipulateSelection's first button press computes them
+   (viewport/manipulation.h:55-124), on the device, from a fresh hit state
+   per point. */
+int st_pick(StSceneHandle *scene, const StView *view, uint32_t n, const uint32_t *xy, StPick *out);
+
 /* Page-locked host memory for frame outputs: downloads into it run at the
    link's full rate (pageable destinations are staged by the runtime). */
 int st_host_alloc(size_t bytes, void **out);

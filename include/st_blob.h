@@ -34,7 +34,7 @@
 
 /* Primitive types, same values as `enum PrimitiveType` (core/types.h:127-135). */
 enum { ST_PRIM_NONE = 0, ST_PRIM_MESH = 1, ST_PRIM_BOX = 2, ST_PRIM_TETRAHEDRON = 3,
-       ST_PRIM_QUAD = 4, ST_PRIM_SPHERE = 5 };
+       ST_PRIM_QUAD = 4, ST_PRIM_SPHERE = 5, ST_PRIM_LIGHT = 6 /* picking only */ };
 
 /* Render modes, same values as `enum RenderMode` (core/types.h:137-142). */
 enum { ST_MODE_NORMALS = 0, ST_MODE_BEAUTY = 1, ST_MODE_DEPTH = 2, ST_MODE_UVS = 3 };
@@ -155,6 +155,15 @@ typedef struct StHitRec {
     int32_t triangle_id;    /* triangle index (BVH leaf order) for mesh hits, -1 otherwise */
     float distance;         /* world-space hit distance t */
 } StHitRec; /* 16 B */
+
+/* Result of picking one window point (the first-press branch of
+   manipulateSelection, viewport/manipulation.h:55-124). */
+typedef struct StPick {
+    int32_t object_type;    /* ST_PRIM_* of the primitive, ST_PRIM_LIGHT, or 0 when nothing is hit */
+    uint32_t object_id;     /* primitive index (scene order) or light index */
+    float position[3];      /* world-space hit point (the selection's transformation plane origin) */
+    float object_distance;  /* view-space z of the hit point (selection->object_distance) */
+} StPick; /* 24 B */
 
 /* Ray counters: every ray that reaches an intersection routine. */
 typedef struct StRayCounts {

@@ -23,6 +23,7 @@
 #include "app.h"
 #include "render/raytracer.h"
 #include "scene/xform.h"
+#include "viewport/manipulation.h"
 
 #include "st_blob.h"
 
@@ -590,6 +591,49 @@ int ref_ssb(const void *blob, size_t size, const StView *view, int32_t *rects, u
         rects[4 * i + 3] = (int32_t)p->screen_bounds.max.y;
         flags[i] = p->flags;
     }
+    refViewFree(rv);
+    refFree(rs);
+    return 0;
+}
+
+/* Picking: manipulateSelection (viewport/manipulation.h:33-240) called with
+   the left button freshly pressed at each window point, no modifier and no
+   mouse movement, so only its pick branch runs. Each pick starts from the
+   hit state initTrace leaves (zeroed hits, closest_mesh_hit typed as mesh). */
+int ref_pick(const void *blob, size_t size, const StView *view, uint32_t n, const uint32_t *xy, StPick *out) {
+    RefScene *rs = refLoad(blob, size);
+    if (!rs) return -1;
+    RefView *rv = refViewInit(rs, view);
+    Scene *scene = &rs->scene;
+    Selection selection;
+    Selection *saved = scene->selection;
+    scene->selection = &selection;
+    Controls controls;
+    Trace *t = &rv->viewport.trace;
+    for (uint32_t k = 0; k < n; k++) {
+        memset(&selection, 0, sizeof selection);
+        memset(&controls, 0, sizeof controls);
+        memset(&t->closest_hit, 0, sizeof t->closest_hit);
+        memset(&t->closest_mesh_hit, 0, sizeof t->closest_mesh_hit);
+        memset(&t->current_hit, 0, sizeof t->current_hit);
+        memset(&t->sphere_hit, 0, sizeof t->sphere_hit);
+        if (scene->settings.meshes) t->closest_mesh_hit.object_type = PrimitiveType_Mesh;
+        controls.mouse.pos.x = (i32)xy[2 * k] + rv->viewport.settings.position.x;
+        controls.mouse.pos.y = (i32)xy[2 * k + 1] + rv->viewport.settings.position.y;
+        controls.mouse.left_button.is_pressed = true;
+        manipulateSelection(scene, &rv->viewport, &controls);
+        StPick *o = out + k;
+        memset(o, 0, sizeof *o);
+        if (selection.object_type) {
+            o->object_type = (int32_t)selection.object_type;
+            o->object_id = selection.object_id;
+            o->position[0] = selection.transformation_plane_origin.x;
+            o->position[1] = selection.transformation_plane_origin.y;
+            o->position[2] = selection.transformation_plane_origin.z;
+            o->object_distance = selection.object_distance;
+        }
+    }
+    scene->selection = saved;
     refViewFree(rv);
     refFree(rs);
     return 0;

@@ -2057,6 +2057,68 @@ int oracle_ssb(const void *blob, size_t size, const StView *view, int32_t *rects
     return 0;
 }
 
+/* manipulateSelection's first press (viewport/manipulation.h:55-124):
+   the ray of setRayFromCoords (render/shaders/common.h:170-175, closed
+   form start + right*x + down*y), hitPrimitives over all primitives with the
+   SSB gate at the point, then every light's glow sphere (radius I/128) in
+   front of the hit; the view-space z of the hit through the inverted
+   rotation matrix (manipulation.h:117-118). */
+int oracle_pick(const void *blob, size_t size, const StView *view, uint32_t n, const uint32_t *xy, StPick *out) {
+    OScene *sc = scene_load(blob, size);
+    if (!sc) return -1;
+    OCamera cam;
+    ODims dims;
+    camera_init(&cam, view);
+    dims_init(&dims, view->width, view->height);
+    update_ssb(sc, &cam, &dims);
+    V3 start, right, down;
+    projection_plane(&cam, &dims, &start, &right, &down);
+    OTrace *tr = (OTrace *)calloc(1, sizeof(OTrace));
+    if (!tr) { scene_free(sc); return -1; }
+    for (uint32_t k = 0; k < n; k++) {
+        memset(tr, 0, sizeof *tr);
+        tr->closest_mesh_hit.object_type = ST_PRIM_MESH;
+        tr->depth = (uint8_t)view->depth;
+        tr->mesh_stack_size = mesh_stack_size_of(sc);
+        const int32_t x = (int32_t)xy[2 * k], y = (int32_t)xy[2 * k + 1];
+        ORay ray;
+        memset(&ray, 0, sizeof ray);
+        ray.origin = cam.position;
+        ray.direction = smad3(right, (float)x, start);
+        ray.direction = smad3(down, (float)y, ray.direction);
+        ray.direction = norm3(ray.direction);
+        ray.direction_reciprocal = rcp3(ray.direction);
+        OHit *hit = &tr->closest_hit;
+        hit->distance = hit->distance_squared = INFINITY;
+        int found = hit_primitives(&ray, tr, sc, sc->bvh_leaf_ids, sc->prim_count, 0, 1, (uint16_t)x, (uint16_t)y);
+        for (uint32_t i = 0; i < sc->light_count; i++) {
+            const OLight *l = sc->lights + i;
+            const float light_radius = l->intensity * (1.0f / (8.0f * 16.0f));
+            tr->sphere_hit.furthest = hit->distance / light_radius;
+            if (hit_sphere_simple(ray.origin, ray.direction, l->position, 1.0f / light_radius, &tr->sphere_hit)) {
+                hit->distance = tr->sphere_hit.t_near * light_radius;
+                hit->position = smad3(ray.direction, hit->distance, ray.origin);
+                hit->object_type = ST_PRIM_LIGHT;
+                hit->object_id = i;
+                found = 1;
+            }
+        }
+        StPick *o = out + k;
+        memset(o, 0, sizeof *o);
+        if (found) {
+            o->object_type = (int32_t)hit->object_type;
+            o->object_id = hit->object_id;
+            o->position[0] = hit->position.x;
+            o->position[1] = hit->position.y;
+            o->position[2] = hit->position.z;
+            o->object_distance = vmat3(sub3(hit->position, ray.origin), cam.rotation_matrix_inverted).z;
+        }
+    }
+    free(tr);
+    scene_free(sc);
+    return 0;
+}
+
 int oracle_camera(const StView *view, float *out) {
     OCamera cam;
     ODims dims;

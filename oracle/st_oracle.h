@@ -44,6 +44,10 @@ int oracle_render(const void *blob, size_t size, const StView *view, StFloatPixe
 int oracle_render_timed(const void *blob, size_t size, const StView *view, StFloatPixel *pixels,
                         int threads, int frames, double *seconds, StOracleStats *stats);
 
+/* Picking at n window points (x, y pairs) on a fresh trace state per point
+   (viewport/manipulation.h:55-124). */
+int oracle_pick(const void *blob, size_t size, const StView *view, uint32_t n, const uint32_t *xy, StPick *out);
+
 /* Scene BVH (SAH, leaves <= 2) as built for the blob. nodes: 2*prim_count. */
 int oracle_scene_bvh(const void *blob, size_t size, StBVHNodeRec *nodes, uint32_t *leaf_ids,
                      uint32_t *node_count, uint32_t *height);

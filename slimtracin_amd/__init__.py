@@ -228,6 +228,29 @@ def render_batch(scenes, views: np.ndarray, hits: bool = False, persistent: bool
     return pix, hr, list(counts)
 
 
+def device_ssb(scene, view: np.ndarray):
+    """The SSB pass on the device (st_ssb): (rects (P, 4) int32 as computeSSB
+    leaves them, flags (P,) uint8)."""
+    P = scene.info().prim_count
+    rects = np.zeros((P, 4), dtype=np.int32)
+    flags = np.zeros(P, dtype=np.uint8)
+    view = np.ascontiguousarray(view, dtype=VIEW_DTYPE)
+    _check(lib().st_ssb(scene.handle, _ptr(view), _ptr(rects), _ptr(flags)))
+    return rects, flags
+
+
+def pick(scene, view: np.ndarray, xy) -> np.ndarray:
+    """Pick at window points (an (n, 2) array of x, y): per point the closest
+    primitive or light glow sphere, its world hit point and view-space
+    distance (st_pick; scene.PICK_DTYPE records, object_type 0 = nothing)."""
+    from .scene import PICK_DTYPE
+    pts = np.ascontiguousarray(xy, dtype=np.uint32).reshape(-1, 2)
+    out = np.zeros(len(pts), dtype=PICK_DTYPE)
+    view = np.ascontiguousarray(view, dtype=VIEW_DTYPE)
+    _check(lib().st_pick(scene.handle, _ptr(view), C.c_uint32(len(pts)), _ptr(pts), _ptr(out)))
+    return out
+
+
 def graph_stats():
     """(graphs captured, graph launches) so far in this process."""
     c, l = C.c_uint64(), C.c_uint64()

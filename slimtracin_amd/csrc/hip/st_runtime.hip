@@ -36,6 +36,7 @@
 #include <vector>
 
 #include "st_wave.h"
+#include "../st_ssb.h"
 #include "st_lbvh.h"
 #include "st_mesh_lbvh.h"
 #include "../st_mesh.h"
@@ -90,6 +91,13 @@ struct KReq {
     uint32_t count_block;  /* first block of k_counts for this request */
     uint32_t *bgra;        /* window image (width1-1)*(height1-1), or NULL */
     uint64_t pad2;
+    /* device SSB pass (k_ssb) when device_ssb: camera rotation_matrix_inverted
+       (rows X, Y, Z) and focal length; ssb_raw / ssb_flags optionally receive
+       the rects as computeSSB leaves them and the flags */
+    float ssb_rmi[9], focal_length;
+    uint32_t device_ssb, pad3;
+    int32_t *ssb_raw;
+    uint8_t *ssb_flags;
 };
 
 __device__ static inline uint32_t find_req(const KReq *reqs, uint32_t n, uint32_t key, bool by_row) {
@@ -512,6 +520,107 @@ __global__ void __launch_bounds__(256) k_counts(const KReq *__restrict__ reqs, u
     }
 }
 
+/* Screen-space bounds on the device (updateSceneSSB, render/SSB.h:137-173),
+   one block per request, the same operations as the host pass (st_ssb.h):
+   the gate rect of every primitive into the request's rect array. */
+__global__ void __launch_bounds__(256) k_ssb(const StDScene *__restrict__ scenes, const KReq *__restrict__ reqs) {
+    const KReq &q = reqs[blockIdx.x];
+    if (!q.device_ssb) return;
+    const StDView &v = q.v;
+    const StDScene sc = scenes[v.scene];
+    const StSsbDims d = st_ssb_dims(v.width1, v.height1);
+    sm3 rmi;
+    rmi.X = V3(q.ssb_rmi[0], q.ssb_rmi[1], q.ssb_rmi[2]);
+    rmi.Y = V3(q.ssb_rmi[3], q.ssb_rmi[4], q.ssb_rmi[5]);
+    rmi.Z = V3(q.ssb_rmi[6], q.ssb_rmi[7], q.ssb_rmi[8]);
+    const sv3 cam = V3(v.cam_pos[0], v.cam_pos[1], v.cam_pos[2]);
+    for (uint32_t i = threadIdx.x; i < sc.prim_count; i += blockDim.x) {
+        const DPrim p = load_prim(sc, i);
+        uint8_t flags = (uint8_t)p.flags;
+        float mb[6] = {0, 0, 0, 0, 0, 0};
+        if (p.type == ST_PRIM_MESH) {
+            const float *m = sc.mesh_aabbs + 6 * (size_t)p.mesh_id;
+            for (int k = 0; k < 6; k++) mb[k] = ((const ST_GLOBAL float *)m)[k];
+        }
+        int32_t gate[4], raw[4];
+        st_ssb_prim(p.type, &flags, p.pos, p.scale, mb, cam, rmi, q.focal_length, &d, gate, raw);
+        stgi4(const_cast<int32_t *>(v.rects), i, gate[0], gate[1], gate[2], gate[3]);
+        if (q.ssb_raw) stgi4(q.ssb_raw, i, raw[0], raw[1], raw[2], raw[3]);
+        if (q.ssb_flags) ((ST_GLOBAL uint8_t *)q.ssb_flags)[i] = flags;
+    }
+}
+
+/* Picking (the first-press branch of manipulateSelection,
+   viewport/manipulation.h:55-124), one lane per window point: the closed-form
+   ray of setRayFromCoords (render/shaders/common.h:170-175), the closest
+   primitive over all primitives with the SSB gate at the point, then each
+   light's glow sphere (radius intensity/128) in front of it; the view-space
+   z of the hit through the inverted camera rotation. Fresh hit state per
+   point. */
+struct KPick {
+    float start[3], right[3], down[3];
+    float rmi[9];                  /* camera rotation_matrix_inverted, rows X, Y, Z */
+    uint32_t n;
+    const uint32_t *xy;            /* n (x, y) pairs */
+    StPick *out;
+};
+
+__global__ void __launch_bounds__(64) k_pick(const StDScene *__restrict__ scenes, const KReq *__restrict__ reqs,
+                                             KPick pk, uint32_t mesh_entries) {
+    ST_LDS_STACKS(1u, mesh_entries);
+    const uint32_t i = blockIdx.x * 64u + threadIdx.x;
+    if (i >= pk.n) return;
+    const StDView v = reqs[0].v;
+    const StDScene sc = scenes[0];
+    const int32_t x = (int32_t)ldgu(pk.xy, 2 * (size_t)i), y = (int32_t)ldgu(pk.xy, 2 * (size_t)i + 1);
+    DTrace tr;
+    trace_init(tr, sc, scene_stack, mesh_stack, 0.0f);
+    DRay ray;
+    ray.origin = V3(v.cam_pos[0], v.cam_pos[1], v.cam_pos[2]);
+    ray.direction = v_smad(V3(pk.right[0], pk.right[1], pk.right[2]), (float)x, V3(pk.start[0], pk.start[1], pk.start[2]));
+    ray.direction = v_smad(V3(pk.down[0], pk.down[1], pk.down[2]), (float)y, ray.direction);
+    ray.direction = v_norm(ray.direction);
+    ray.direction_reciprocal = v_rcp(ray.direction);
+    DHit &hit = tr.closest_hit;
+    hit.distance = hit.distance_squared = INFINITY;
+    uint32_t type = 0, id = 0;
+    bool found = hit_primitives(ray, tr, sc, sc.scene_leaf_ids, sc.prim_count, false, v.rects, (uint16_t)x,
+                                (uint16_t)y);
+    if (found) {
+        type = hit.object_type;
+        id = hit.object_id;
+    }
+    for (uint32_t l = 0; l < sc.light_count; l++) {
+        const float4 la = ldg4(sc.lights + l, 0);
+        const float light_radius = la.w * (1.0f / (8.0f * 16.0f));
+        tr.sphere_hit.furthest = hit.distance / light_radius;
+        if (hit_sphere_simple(ray.origin, ray.direction, V3(la.x, la.y, la.z), 1.0f / light_radius, tr.sphere_hit)) {
+            hit.distance = tr.sphere_hit.t_near * light_radius;
+            hit.position = v_smad(ray.direction, hit.distance, ray.origin);
+            type = ST_PRIM_LIGHT;
+            id = l;
+            found = true;
+        }
+    }
+    float out[6] = {0, 0, 0, 0, 0, 0};
+    if (found) {
+        sm3 m;
+        m.X = V3(pk.rmi[0], pk.rmi[1], pk.rmi[2]);
+        m.Y = V3(pk.rmi[3], pk.rmi[4], pk.rmi[5]);
+        m.Z = V3(pk.rmi[6], pk.rmi[7], pk.rmi[8]);
+        out[0] = __uint_as_float(type);
+        out[1] = __uint_as_float(id);
+        out[2] = hit.position.x;
+        out[3] = hit.position.y;
+        out[4] = hit.position.z;
+        out[5] = v_mat(v_sub(hit.position, ray.origin), m).z;
+    }
+    void *o = pk.out + i;
+    stg2(o, 0, __float_as_uint(out[0]), __float_as_uint(out[1]));
+    stg2(o, 1, __float_as_uint(out[2]), __float_as_uint(out[3]));
+    stg2(o, 2, __float_as_uint(out[4]), __float_as_uint(out[5]));
+}
+
 /* Display resolve of one window pixel (drawViewportToWindowContent,
    viewport/viewport.h:12-66, QCAA branch): the 4 traced corners are averaged
    as (TL.c*TL.o + (TR.c*TR.o + (BL.c*BL.o + BR.c*BR.o))) * 0.25 and each
@@ -885,6 +994,7 @@ extern "C" int st_scene_upload(StSceneHandle *h) {
     h->bvh_on_device_only = 0;
     StDScene &d = h->dev;
     memset(&d, 0, sizeof d);
+    d.mesh_aabbs = h->d_mesh_aabbs;
     d.prims = (const StDPrim *)(b + o_prims);
     d.mats = (const StDMat *)(b + o_mats);
     d.lights = (const StDLight *)(b + o_lights);
@@ -1129,14 +1239,14 @@ int st_rt_launch_prepared(StSceneHandle *const *scenes, StHostView *hv, uint32_t
     StDScene *sc_d = (StDScene *)(blk + o_scenes);
     unsigned long long *cnt_d = (unsigned long long *)arena_take(&g.work, n * 24ull);
     for (uint32_t k = 0; k < nuniq; k++) sc_h[k] = uniq[k]->dev;
-    uint32_t row_base = 0, out_base = 0, count_blocks = 0;
+    uint32_t row_base = 0, out_base = 0, count_blocks = 0, any_device_ssb = 0;
     for (uint32_t i = 0; i < n; i++) {
         size_t px = (size_t)hv[i].dv.width1 * hv[i].dv.height1;
         StDView &dv = hv[i].dv;
         dv.scene = scene_idx[i];
         uint32_t P = scenes[i]->host->prim_count;
         memcpy(st + rs_off[i], hv[i].row_starts, dv.height1 * 16ull);
-        memcpy(st + rect_off[i], hv[i].rects, P * 16ull);
+        if (!hv[i].device_ssb) memcpy(st + rect_off[i], hv[i].rects, P * 16ull);
         dv.row_starts = (const float *)(blk + rs_off[i]);
         dv.rects = (const int32_t *)(blk + rect_off[i]);
         dv.dirs = (float *)arena_take(&g.work, px * 16);
@@ -1152,6 +1262,14 @@ int st_rt_launch_prepared(StSceneHandle *const *scenes, StHostView *hv, uint32_t
         count_blocks += (reqs_h[i].tile_count + COUNT_TILES - 1) / COUNT_TILES;
         uint32_t wpx = (dv.width1 - 1) * (dv.height1 - 1);
         reqs_h[i].bgra = (bgra_host && bgra_host[i]) ? (uint32_t *)arena_take(&g.work, (size_t)wpx * 4) : NULL;
+        if (hv[i].device_ssb) {
+            const sm3 &m = hv[i].rotation_matrix_inverted;
+            const float rmi[9] = {m.X.x, m.X.y, m.X.z, m.Y.x, m.Y.y, m.Y.z, m.Z.x, m.Z.y, m.Z.z};
+            memcpy(reqs_h[i].ssb_rmi, rmi, sizeof rmi);
+            reqs_h[i].focal_length = hv[i].focal_length;
+            reqs_h[i].device_ssb = 1;
+            any_device_ssb = 1;
+        }
         row_base += dv.height1;
         out_base += wpx;
     }
@@ -1221,6 +1339,8 @@ int st_rt_launch_prepared(StSceneHandle *const *scenes, StHostView *hv, uint32_t
         HIP_TRY(hipMemcpyAsync(blk, st, up, hipMemcpyHostToDevice, g.stream));
         hipLaunchKernelGGL(k_zero_u64, dim3((3 * n + 255) / 256), dim3(256), 0, g.stream, cnt_d, 3 * n);
         hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, g.stream, g.tile_counter, 64u);
+        if (any_device_ssb)
+            hipLaunchKernelGGL(k_ssb, dim3(n), dim3(256), 0, g.stream, (const StDScene *)sc_d, (const KReq *)reqs_d);
         if (levels)
             hipLaunchKernelGGL(k_zero_u32, dim3((levels * WC_PER_LEVEL + 255) / 256), dim3(256), 0, g.stream, w.ctr,
                                levels * WC_PER_LEVEL);
@@ -1272,7 +1392,7 @@ int st_rt_launch_prepared(StSceneHandle *const *scenes, StHostView *hv, uint32_t
            at fixed arena offsets, so an equal signature means the captured
            graph does exactly this launch's work (the per-frame data is in
            the upload block, copied from the same pinned staging) */
-        std::vector<uint64_t> sig = {n, up, rows, tiles, levels, lmax, count_blocks, blocks, xcd_blocks,
+        std::vector<uint64_t> sig = {n, up, rows, tiles, levels, lmax, count_blocks, blocks, xcd_blocks, any_device_ssb,
                                      (uint64_t)persistent, s_entries, m_entries, b_shadow0, b_shadow, b_shade0,
                                      b_shade, b_bounce, (uint64_t)(uintptr_t)blk, (uint64_t)(uintptr_t)st,
                                      (uint64_t)(uintptr_t)cnt_d, (uint64_t)(uintptr_t)w.st, w.n};
@@ -1351,7 +1471,7 @@ static int launch(StSceneHandle *const *scenes, const StView *views, uint32_t n,
     if (!hv) return set_err(ST_ERR_MEMORY, "out of host memory");
     for (uint32_t i = 0; i < n && !rc; i++) {
         if (!scenes[i] || !scenes[i]->host) rc = set_err(ST_ERR_ARG, "request %u has no resident scene", i);
-        else if (st_view_prepare(scenes[i]->host, views + i, hv + i))
+        else if (st_view_prepare_ex(scenes[i]->host, views + i, hv + i, 0))
             rc = set_err(ST_ERR_ARG, "request %u: invalid view (%ux%u, mode %u)", i, views[i].width, views[i].height,
                          views[i].render_mode);
     }
@@ -1421,6 +1541,95 @@ extern "C" int st_render_batch_bgra8(StSceneHandle *const *scenes, const StView 
     RT_LOCK;
     GraphScope gs(flags);
     return launch(scenes, views, n, 0, (flags & ST_BATCH_PERSISTENT) ? 1 : 0, counts, NULL, NULL, bgra, NULL);
+}
+
+extern "C" int st_ssb(StSceneHandle *scene, const StView *view, int32_t *rects, uint8_t *flags) {
+    RT_LOCK;
+    if (!scene || !view) return set_err(ST_ERR_ARG, "null argument");
+    int rc = ensure_init();
+    if (rc) return rc;
+    if (!scene->host || !scene->dbuf) return set_err(ST_ERR_ARG, "scene not resident");
+    StHostView hv;
+    if (st_view_prepare_ex(scene->host, view, &hv, 0)) return set_err(ST_ERR_ARG, "invalid view");
+    const uint32_t P = scene->host->prim_count;
+    const size_t o_sc = round256(sizeof(KReq)), up = o_sc + round256(sizeof(StDScene));
+    const size_t gate_b = round256((P + 1) * 16ull);
+    if ((rc = arena_reserve(&g.work, up + 2 * gate_b + round256(P + 1) + 512)) || (rc = staging_reserve(up))) {
+        st_view_release(&hv);
+        return rc;
+    }
+    char *blk = (char *)arena_take(&g.work, up);
+    int32_t *gate_d = (int32_t *)arena_take(&g.work, gate_b);
+    int32_t *raw_d = (int32_t *)arena_take(&g.work, gate_b);
+    uint8_t *flags_d = (uint8_t *)arena_take(&g.work, P + 1);
+    KReq *rq = (KReq *)g.staging;
+    memset(rq, 0, sizeof(KReq));
+    rq->v = hv.dv;
+    rq->v.rects = gate_d;
+    const sm3 &m = hv.rotation_matrix_inverted;
+    const float rmi[9] = {m.X.x, m.X.y, m.X.z, m.Y.x, m.Y.y, m.Y.z, m.Z.x, m.Z.y, m.Z.z};
+    memcpy(rq->ssb_rmi, rmi, sizeof rmi);
+    rq->focal_length = hv.focal_length;
+    rq->device_ssb = 1;
+    rq->ssb_raw = raw_d;
+    rq->ssb_flags = flags_d;
+    memcpy(g.staging + o_sc, &scene->dev, sizeof(StDScene));
+    st_view_release(&hv);
+    if (P == 0) return ST_OK;
+    HIP_TRY(hipMemcpyAsync(blk, g.staging, up, hipMemcpyHostToDevice, g.stream));
+    hipLaunchKernelGGL(k_ssb, dim3(1), dim3(256), 0, g.stream, (const StDScene *)(blk + o_sc), (const KReq *)blk);
+    HIP_TRY(hipGetLastError());
+    if (rects) HIP_TRY(hipMemcpyAsync(rects, raw_d, P * 16ull, hipMemcpyDeviceToHost, g.stream));
+    if (flags) HIP_TRY(hipMemcpyAsync(flags, flags_d, P, hipMemcpyDeviceToHost, g.stream));
+    HIP_TRY(hipStreamSynchronize(g.stream));
+    return ST_OK;
+}
+
+extern "C" int st_pick(StSceneHandle *scene, const StView *view, uint32_t n, const uint32_t *xy, StPick *out) {
+    RT_LOCK;
+    if (!scene || !view || (n && (!xy || !out))) return set_err(ST_ERR_ARG, "null argument");
+    int rc = ensure_init();
+    if (rc) return rc;
+    if (!scene->host || !scene->dbuf) return set_err(ST_ERR_ARG, "scene not resident");
+    if (n == 0) return ST_OK;
+    StHostView hv;
+    if (st_view_prepare(scene->host, view, &hv)) return set_err(ST_ERR_ARG, "invalid view");
+    const uint32_t P = scene->host->prim_count;
+    /* one upload block: request, scene descriptor, rects, points */
+    const size_t o_sc = round256(sizeof(KReq)), o_rect = o_sc + round256(sizeof(StDScene)),
+                 o_xy = o_rect + round256((P + 1) * 16ull), up = o_xy + round256(n * 8ull);
+    if ((rc = arena_reserve(&g.work, up + round256(n * sizeof(StPick)) + 512)) || (rc = staging_reserve(up))) {
+        st_view_release(&hv);
+        return rc;
+    }
+    char *blk = (char *)arena_take(&g.work, up);
+    StPick *out_d = (StPick *)arena_take(&g.work, n * sizeof(StPick));
+    KReq *rq = (KReq *)g.staging;
+    memset(rq, 0, sizeof(KReq));
+    rq->v = hv.dv;
+    rq->v.rects = (const int32_t *)(blk + o_rect);
+    memcpy(g.staging + o_sc, &scene->dev, sizeof(StDScene));
+    memcpy(g.staging + o_rect, hv.rects, P * 16ull);
+    memcpy(g.staging + o_xy, xy, n * 8ull);
+    KPick pk;
+    memcpy(pk.start, hv.start, 12);
+    memcpy(pk.right, hv.right, 12);
+    memcpy(pk.down, hv.down, 12);
+    const sm3 &m = hv.rotation_matrix_inverted;
+    const float rmi[9] = {m.X.x, m.X.y, m.X.z, m.Y.x, m.Y.y, m.Y.z, m.Z.x, m.Z.y, m.Z.z};
+    memcpy(pk.rmi, rmi, sizeof rmi);
+    pk.n = n;
+    pk.xy = (const uint32_t *)(blk + o_xy);
+    pk.out = out_d;
+    st_view_release(&hv);
+    const uint32_t m_entries = scene->dev.mesh_stack_size > 1 ? scene->dev.mesh_stack_size : 1;
+    HIP_TRY(hipMemcpyAsync(blk, g.staging, up, hipMemcpyHostToDevice, g.stream));
+    hipLaunchKernelGGL(k_pick, dim3((n + 63) / 64), dim3(64), (size_t)64 * (1 + m_entries) * 4, g.stream,
+                       (const StDScene *)(blk + o_sc), (const KReq *)blk, pk, m_entries);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(out, out_d, n * sizeof(StPick), hipMemcpyDeviceToHost, g.stream));
+    HIP_TRY(hipStreamSynchronize(g.stream));
+    return ST_OK;
 }
 
 extern "C" int st_resolve_bgra8(const StFloatPixel *grid, uint32_t grid_w, uint32_t grid_h, uint32_t *out) {

@@ -145,7 +145,7 @@ static void worker_loop(StServer *srv) {
         std::vector<uint32_t> live;
         for (size_t i = 0; i < n; i++) {
             if (!bufs[i]) status[i] = fail(ST_ERR_MEMORY, "no pinned output buffer");
-            else if (st_view_prepare(handles[i]->host, &views[i], &hv[i]))
+            else if (st_view_prepare_ex(handles[i]->host, &views[i], &hv[i], 0))
                 status[i] = fail(ST_ERR_ARG, "invalid view");
             else live.push_back((uint32_t)i);
         }

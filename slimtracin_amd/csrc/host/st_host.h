@@ -149,9 +149,14 @@ typedef struct StHostView {
     int32_t *rects;        /* 4 ints per primitive, empty when not visible */
     uint8_t *flags;        /* primitive flags after the SSB pass */
     int32_t *raw_rects;    /* screen_bounds as computeSSB left them */
+    float focal_length;
+    int device_ssb;        /* rects not computed on the host: the device SSB pass fills them */
 } StHostView;
 
 int st_view_prepare(const StScene *sc, const StView *v, StHostView *hv);
+/* with_ssb 0: no SSB pass (rects, raw_rects and flags are left unset; the
+   device computes the gate rects) */
+int st_view_prepare_ex(const StScene *sc, const StView *v, StHostView *hv, int with_ssb);
 
 /* Set-up for a frame described the way renderSceneOnGPU receives it: the
    viewport's projection plane and camera, and per-primitive screen bounds

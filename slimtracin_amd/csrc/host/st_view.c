@@ -17,6 +17,7 @@
 
 #include "st_host.h"
 #include "slimtracin.h"
+#include "../st_ssb.h"
 
 typedef struct {
     uint16_t width, height;
@@ -69,61 +70,9 @@ static sm3 rotation_matrix(float yaw_amt, float pitch_amt, float roll_amt) {
     return m_mul(m_mul(pitch, yaw), roll);
 }
 
-/* computeSSB (render/SSB.h:5-135); b = min.x, min.y, max.x, max.y */
-static int compute_ssb(int32_t *b, sv3 pos, float r, float fl, const Dims *d) {
-    b[0] = d->width + 1;
-    b[2] = d->width + 1;
-    b[1] = d->height + 1;
-    b[3] = d->height + 1;
-    if (pos.z <= -r) return 0;
-    float x = pos.x, y = pos.y, z = pos.z;
-    float left, right, top, bottom, factor;
-    if (z <= r) {
-        float h = fabsf(z) / fl;
-        top = y + r;
-        bottom = y - r;
-        if (h < bottom || top < -h) return 0;
-        float w = h * d->width_over_height;
-        right = x + r;
-        left = x - r;
-        if (w < left || right < -w) return 0;
-        b[0] = -w < left ? (uint16_t)(int32_t)(d->f_width * (left + w) / (2 * w)) : 0;
-        b[2] = right < w ? (uint16_t)(int32_t)(d->f_width * (right + w) / (2 * w)) : d->width;
-        b[1] = top < h ? (uint16_t)(int32_t)(d->f_height * (h - top) / (2 * h)) : 0;
-        b[3] = -h < bottom ? (uint16_t)(int32_t)(d->f_height * (h - bottom) / (2 * h)) : d->height;
-        return 1;
-    }
-    float den = z * z - r * r;
-    factor = fl / den;
-    float yz = y * z;
-    float sqr = y * y + den;
-    sqr = r * sqrtf(sqr);
-    bottom = factor * (yz - sqr);
-    top = factor * (yz + sqr);
-    if (bottom < 1 && top > -1) {
-        factor *= d->height_over_width;
-        float xz = x * z;
-        sqr = r * sqrtf(x * x + den);
-        left = factor * (xz - sqr);
-        right = factor * (xz + sqr);
-        if (left < 1 && right > -1) {
-            bottom = bottom > -1 ? bottom : -1; bottom += 1;
-            top = top < 1 ? top : 1; top += 1;
-            left = left > -1 ? left : -1; left += 1;
-            right = right < 1 ? right : 1; right += 1;
-            top = 2 - top;
-            bottom = 2 - bottom;
-            b[0] = (uint16_t)(int32_t)(d->h_width * left);
-            b[2] = (uint16_t)(int32_t)(d->h_width * right);
-            b[3] = (uint16_t)(int32_t)(d->h_height * bottom);
-            b[1] = (uint16_t)(int32_t)(d->h_height * top);
-            return 1;
-        }
-    }
-    return 0;
-}
+int st_view_prepare(const StScene *sc, const StView *v, StHostView *hv) { return st_view_prepare_ex(sc, v, hv, 1); }
 
-int st_view_prepare(const StScene *sc, const StView *v, StHostView *hv) {
+int st_view_prepare_ex(const StScene *sc, const StView *v, StHostView *hv, int with_ssb) {
     memset(hv, 0, sizeof *hv);
     if (v->width == 0 || v->height == 0 || v->width >= 0xFFFF || v->height >= 0xFFFF) return -1;
     if (v->render_mode > ST_MODE_UVS) return -1;
@@ -159,38 +108,14 @@ int st_view_prepare(const StScene *sc, const StView *v, StHostView *hv) {
     }
 
     sv3 cam = V3(v->position[0], v->position[1], v->position[2]);
-    for (uint32_t i = 0; i < sc->prim_count; i++) {
+    const StSsbDims sd = st_ssb_dims(d.width, d.height);
+    for (uint32_t i = 0; with_ssb && i < sc->prim_count; i++) {
         const StDPrim *p = sc->prims + i;
         uint8_t flags = (uint8_t)(p->flags_mat & 0xFFu);
-        int32_t *rc = hv->rects + 4 * i;
-        sv3 pos = V3(p->pos[0], p->pos[1], p->pos[2]);
-        sv3 scale = V3(p->scale[0], p->scale[1], p->scale[2]);
-        sv3 vsp = v_mat(v_sub(pos, cam), rmi);
-        float radius;
-        int known = 1;
-        switch (p->type) {
-            case ST_PRIM_QUAD:
-            case ST_PRIM_BOX: radius = v_len(scale); break;
-            case ST_PRIM_TETRAHEDRON: radius = v_maxc(scale) * 1.41421356237f; break;
-            case ST_PRIM_SPHERE: radius = v_maxc(scale); break;
-            case ST_PRIM_MESH: {
-                sbox bb = sc->mesh_aabbs[p->mesh_id];
-                float min_r = v_len(v_mul(scale, bb.min));
-                float max_r = v_len(v_mul(scale, bb.max));
-                radius = max_r > min_r ? max_r : min_r;
-            } break;
-            default: known = 0; radius = 0; break;
-        }
-        if (known) {
-            flags &= (uint8_t)~ST_IS_VISIBLE;
-            if (compute_ssb(rc, vsp, radius, v->focal_length, &d)) flags |= ST_IS_VISIBLE;
-        } else {
-            /* updatePrimitiveSSB returns early: flags and (zero) bounds stay */
-            rc[0] = rc[1] = rc[2] = rc[3] = 0;
-        }
+        const float *mb = p->type == ST_PRIM_MESH ? &sc->mesh_aabbs[p->mesh_id].min.x : NULL;
+        st_ssb_prim(p->type, &flags, V3(p->pos[0], p->pos[1], p->pos[2]), V3(p->scale[0], p->scale[1], p->scale[2]),
+                    mb, cam, rmi, v->focal_length, &sd, hv->rects + 4 * i, hv->raw_rects + 4 * i);
         hv->flags[i] = flags;
-        memcpy(hv->raw_rects + 4 * i, rc, 16);
-        if (!(flags & ST_IS_VISIBLE)) { rc[0] = rc[1] = 1; rc[2] = rc[3] = 0; }
     }
 
     StDView *dv = &hv->dv;
@@ -213,6 +138,8 @@ int st_view_prepare(const StScene *sc, const StView *v, StHostView *hv) {
     hv->rot[0] = rot.v.x; hv->rot[1] = rot.v.y; hv->rot[2] = rot.v.z; hv->rot[3] = rot.w;
     memcpy(hv->rot_inv, dv->cam_rot_inv, 16);
     hv->rotation_matrix_inverted = rmi;
+    hv->focal_length = v->focal_length;
+    hv->device_ssb = !with_ssb;
     return 0;
 }
 

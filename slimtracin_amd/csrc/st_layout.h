@@ -95,7 +95,7 @@ typedef struct StDScene {
     uint32_t mesh_stack_size;      /* (u8)(max mesh BVH height + 2), core/init.h:523-527 */
     uint32_t has_emissive_quads;
     uint32_t scene_stack_need;     /* entries the scene-BVH traversal stack can reach */
-    uint32_t pad[2];
+    const float *mesh_aabbs;       /* 6 floats per mesh: the bound the SSB pass uses */
 } StDScene;
 
 /* One render request on the device. */

@@ -36,6 +36,9 @@ VIEW_DTYPE = np.dtype([("position", "<f4", 3), ("yaw", "<f4"), ("pitch", "<f4"),
                        ("render_mode", "<u4"), ("depth", "<u4"), ("reserved", "<u4")])
 PIXEL_DTYPE = np.dtype([("r", "<f4"), ("g", "<f4"), ("b", "<f4"), ("opacity", "<f4"), ("depth", "<f8")])
 HIT_DTYPE = np.dtype([("prim_id", "<i4"), ("object_type", "<i4"), ("triangle_id", "<i4"), ("distance", "<f4")])
+PICK_DTYPE = np.dtype([("object_type", "<i4"), ("object_id", "<u4"), ("position", "<f4", 3),
+                       ("object_distance", "<f4")])
+PRIM_LIGHT = 6    # pick result: a light's glow sphere
 NODE_DTYPE = np.dtype([("min", "<f4", 3), ("max", "<f4", 3), ("first_child_id", "<u4"),
                        ("child_count", "<u2"), ("depth", "<u2")])
 TRIANGLE_DTYPE = np.dtype([("world_to_tangent", "<f4", 9), ("position", "<f4", 3), ("normal", "<f4", 3),

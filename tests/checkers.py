@@ -184,6 +184,23 @@ def ref_ssb(blob, view, prim_count):
     return _ssb(_load(REF_SO), "ref_ssb", blob, view, prim_count)
 
 
+def _pick(lib, fn, blob, view, xy):
+    from slimtracin_amd.scene import PICK_DTYPE
+    xy = np.ascontiguousarray(xy, dtype=np.uint32).reshape(-1, 2)
+    out = np.zeros(len(xy), dtype=PICK_DTYPE)
+    bp, bs, keep = _blob(blob)
+    assert getattr(lib, fn)(bp, bs, _ptr(view), C.c_uint32(len(xy)), _ptr(xy), _ptr(out)) == 0
+    return out
+
+
+def oracle_pick(blob, view, xy):
+    return _pick(_load(ORACLE_SO), "oracle_pick", blob, view, xy)
+
+
+def ref_pick(blob, view, xy):
+    return _pick(_load(REF_SO), "ref_pick", blob, view, xy)
+
+
 def oracle_camera(view):
     out = np.zeros(17, dtype=np.float32)
     _load(ORACLE_SO).oracle_camera(_ptr(view), _ptr(out))
