@@ -198,23 +198,36 @@ __device__ static inline __attribute__((always_inline)) void primary_tile(const 
     const StDScene sc = scenes[v.scene];
     uint32_t x, y;
     const bool valid = tile_pixel(v, tile, lane, x, y);
+    /* Every lane traces, pixels outside the traced grid included (they read
+       pixel 0's direction and no rect admits them), so that the whole wave
+       takes part in culling the primitives against the tile. */
+    const uint32_t pix = valid ? y * v.width1 + x : 0u;
+    const float4 d4 = ldg4(v.dirs, pix);
+    DTrace tr;
+    trace_init(tr, sc, scene_stack, mesh_stack, 0.0f);
+    DRay ray;
+    ray.origin = V3(v.cam_pos[0], v.cam_pos[1], v.cam_pos[2]);
+    /* dir = normVec3(current) (render/raytracer.h:159) */
+    ray.direction = v_norm(V3(d4.x, d4.y, d4.z));
+    ray.direction_reciprocal = v_rcp(ray.direction);
+    DHit &hit = tr.closest_hit;
+    hit.distance = hit.distance_squared = INFINITY;
+    hit.cone_angle = v.cone_angle;
+    hit.cone_width = 0;
+    tr.primary_triangle = -1;
+    DTileBox box;
+    {
+        const uint32_t local = tile - v.tile_base;
+        const uint32_t tw = 1u << v.tile_shift, th = 64u >> v.tile_shift;
+        box.x0 = (int32_t)((local % v.tiles_x) * tw);
+        box.y0 = (int32_t)((local / v.tiles_x) * th);
+        box.x1 = box.x0 + (int32_t)tw;
+        box.y1 = box.y0 + (int32_t)th;
+    }
+    const bool found = hit_primitives(ray, tr, sc, sc.scene_leaf_ids, sc.prim_count, false, v.rects,
+                                      valid ? x : ~0u, valid ? y : ~0u, &box);
     bool entered = false;   /* entered the Beauty depth loop */
     if (valid) {
-        const uint32_t pix = y * v.width1 + x;
-        const float4 d4 = ldg4(v.dirs, pix);
-        DTrace tr;
-        trace_init(tr, sc, scene_stack, mesh_stack, 0.0f);
-        DRay ray;
-        ray.origin = V3(v.cam_pos[0], v.cam_pos[1], v.cam_pos[2]);
-        /* dir = normVec3(current) (render/raytracer.h:159) */
-        ray.direction = v_norm(V3(d4.x, d4.y, d4.z));
-        ray.direction_reciprocal = v_rcp(ray.direction);
-        DHit &hit = tr.closest_hit;
-        hit.distance = hit.distance_squared = INFINITY;
-        hit.cone_angle = v.cone_angle;
-        hit.cone_width = 0;
-        tr.primary_triangle = -1;
-        const bool found = hit_primitives(ray, tr, sc, sc.scene_leaf_ids, sc.prim_count, false, v.rects, x, y);
         if (v.hits)
             stgi4(v.hits, pix, found ? (int)hit.object_id : -1, found ? (int)hit.object_type : 0,
                   found ? tr.primary_triangle : -1, __float_as_int(found ? hit.distance : INFINITY));
@@ -1156,6 +1169,31 @@ static uint32_t resident_blocks(const void *kernel, size_t lds) {
     return (uint32_t)per_cu * (uint32_t)g.cus;
 }
 
+/* LDS per CU on gfx950 */
+#define CU_LDS_BYTES (160u * 1024u)
+/* Waves per SIMD at most for the shadow stages (a block holds one wave per
+   SIMD, so this is also blocks per CU). The shadow kernel fits in 128 VGPRs,
+   which would allow 4, but on the 200k-triangle 1080p frame the first shadow
+   level took 1.45 ms at 4 waves per SIMD, 1.0 ms at 3 and 1.55 ms at 2. */
+#ifndef ST_SHADOW_WAVES_PER_SIMD
+#define ST_SHADOW_WAVES_PER_SIMD 3u
+#endif
+
+/* resident_blocks with at most `cap` blocks per CU (0: no cap): the block's
+   LDS request is raised until no more than `cap` fit; *lds_launch is the
+   request to launch with */
+static uint32_t capped_blocks(const void *kernel, size_t lds, uint32_t cap, size_t *lds_launch) {
+    size_t want = lds;
+    if (cap) {
+        /* just over a (cap+1)-th of the CU's LDS: cap+1 blocks do not fit,
+           and cap blocks fit with room for the allocation granularity */
+        const size_t pad = (((size_t)CU_LDS_BYTES / (cap + 1)) & ~(size_t)1023) + 2048;
+        if (pad > want) want = pad;
+    }
+    *lds_launch = want;
+    return resident_blocks(kernel, want);
+}
+
 struct LaunchPlan {
     uint32_t nreq, total_rows, total_tiles, nscenes;
     size_t work_bytes, stage_bytes;
@@ -1314,9 +1352,12 @@ int st_rt_launch_prepared(StSceneHandle *const *scenes, StHostView *hv, uint32_t
        order, so a wave that had to wait for a slot would run its whole
        share late */
     uint32_t b_shadow0 = 0, b_shadow = 0, b_shade0 = 0, b_shade = 0, b_bounce = 0;
+    size_t lds_shadow = lds;
     if (levels) {
-        b_shadow0 = resident_blocks(reinterpret_cast<const void *>(k_shadow<true>), lds);
-        b_shadow = resident_blocks(reinterpret_cast<const void *>(k_shadow<false>), lds);
+        b_shadow0 = capped_blocks(reinterpret_cast<const void *>(k_shadow<true>), lds, ST_SHADOW_WAVES_PER_SIMD,
+                                  &lds_shadow);
+        b_shadow = capped_blocks(reinterpret_cast<const void *>(k_shadow<false>), lds, ST_SHADOW_WAVES_PER_SIMD,
+                                 &lds_shadow);
         b_shade0 = resident_blocks(reinterpret_cast<const void *>(k_shade<true>), 0);
         b_shade = resident_blocks(reinterpret_cast<const void *>(k_shade<false>), 0);
         b_bounce = resident_blocks(reinterpret_cast<const void *>(k_bounce), lds);
@@ -1358,10 +1399,10 @@ int st_rt_launch_prepared(StSceneHandle *const *scenes, StHostView *hv, uint32_t
            device, so the sequence needs no host round trip */
         for (uint32_t lv = 0; lv < levels; lv++) {
             if (lv == 0)
-                hipLaunchKernelGGL(k_shadow<true>, dim3(b_shadow0), dim3(BLOCK_THREADS), lds, g.stream,
+                hipLaunchKernelGGL(k_shadow<true>, dim3(b_shadow0), dim3(BLOCK_THREADS), lds_shadow, g.stream,
                                    (const StDScene *)sc_d, (const KReq *)reqs_d, n, w, lv, s_entries, m_entries);
             else
-                hipLaunchKernelGGL(k_shadow<false>, dim3(b_shadow), dim3(BLOCK_THREADS), lds, g.stream,
+                hipLaunchKernelGGL(k_shadow<false>, dim3(b_shadow), dim3(BLOCK_THREADS), lds_shadow, g.stream,
                                    (const StDScene *)sc_d, (const KReq *)reqs_d, n, w, lv, s_entries, m_entries);
             HIP_TRY(mark());
             if (lv == 0)

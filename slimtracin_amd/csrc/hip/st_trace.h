@@ -113,6 +113,14 @@ DFN int4 ldgi4(const void *base, size_t i) {
     return make_int4(v.x, v.y, v.z, v.w);
 }
 DFN uint32_t ldgu(const uint32_t *base, size_t i) { return ((const ST_GLOBAL uint32_t *)base)[i]; }
+/* Loads through the constant address space: scalar loads when the index is
+   wave-uniform (scene data is not written while a tracing kernel runs). */
+#define ST_CONST __attribute__((address_space(4)))
+DFN uint32_t ldcu(const uint32_t *base, size_t i) { return ((const ST_CONST uint32_t *)base)[i]; }
+DFN float4 ldc4(const void *base, size_t i) {
+    st_vf4 v = ((const ST_CONST st_vf4 *)base)[i];
+    return make_float4(v.x, v.y, v.z, v.w);
+}
 typedef uint32_t st_vu2 __attribute__((ext_vector_type(2)));
 /* i-th 8-byte element at base */
 DFN void stg2(void *base, size_t i, uint32_t a, uint32_t b) {
@@ -153,9 +161,7 @@ DFN StDMip load_mip(const StDScene &sc, uint32_t i) {
     return m;
 }
 
-DFN DPrim load_prim(const StDScene &sc, uint32_t i) {
-    const void *p = sc.prims + i;
-    float4 a = ldg4(p, 0), b = ldg4(p, 1), c = ldg4(p, 2), d = ldg4(p, 3);
+DFN DPrim prim_of(float4 a, float4 b, float4 c, float4 d) {
     DPrim o;
     o.rot.v = V3(a.x, a.y, a.z);
     o.rot.w = a.w;
@@ -168,6 +174,15 @@ DFN DPrim load_prim(const StDScene &sc, uint32_t i) {
     o.inv_scale = V3(d.x, d.y, d.z);
     o.mesh_id = __float_as_uint(d.w);
     return o;
+}
+DFN DPrim load_prim(const StDScene &sc, uint32_t i) {
+    const void *p = sc.prims + i;
+    return prim_of(ldg4(p, 0), ldg4(p, 1), ldg4(p, 2), ldg4(p, 3));
+}
+/* load_prim for a wave-uniform index */
+DFN DPrim load_prim_c(const StDScene &sc, uint32_t i) {
+    const void *p = sc.prims + i;
+    return prim_of(ldc4(p, 0), ldc4(p, 1), ldc4(p, 2), ldc4(p, 3));
 }
 
 DFN DMat load_mat(const StDScene &sc, uint32_t i) {
@@ -516,14 +531,22 @@ DFN bool hit_tetrahedron(DHit &closest_hit, sv3 Ro, sv3 Rd, uint32_t flags) {
     return found;
 }
 
-/* intersection/mesh.h:5-37 over the 64-byte hot triangle records */
-DFN bool hit_triangles(DTrace &tr, const StDTriHot *tris, uint32_t first, uint32_t count, bool any_hit) {
+/* intersection/mesh.h:5-37 over the 64-byte hot triangle records of the
+   leaves one traversal step reached: the left leaf's range [f0, f0 + c0),
+   then the right leaf's [f1, f1 + c1), the order in which the reference
+   calls its per-leaf test. One loop covers both so that the test is emitted
+   once per kernel; object_id is the triangle's leaf-order index. */
+DFN bool hit_triangles(DTrace &tr, const StDTriHot *tris, uint32_t f0, uint32_t c0, uint32_t f1, uint32_t c1,
+                       bool any_hit) {
     DRay &ray = tr.local;
     DHit &hit = tr.current_hit;
     DHit &closest = tr.closest_mesh_hit;
     bool found = false;
-    for (uint32_t i = 0; i < count; i++) {
-        const void *p = tris + first + i;
+    const uint32_t n = c0 + c1;
+#pragma clang loop unroll(disable)
+    for (uint32_t j = 0; j < n; j++) {
+        const uint32_t i = j < c0 ? f0 + j : f1 + (j - c0);
+        const void *p = tris + i;
         float4 a = ldg4(p, 0), b = ldg4(p, 1), c = ldg4(p, 2), d = ldg4(p, 3);
         sv3 tpos = V3(c.y, c.z, c.w);
         sv3 tnrm = V3(d.x, d.y, d.z);
@@ -552,7 +575,11 @@ DFN bool hit_triangles(DTrace &tr, const StDTriHot *tris, uint32_t first, uint32
     return found;
 }
 
-/* intersection/mesh.h:38-111 */
+/* intersection/mesh.h:38-111. A leaf root holds every triangle; otherwise
+   each step tests both children's boxes, then the triangles of the children
+   that are leaves (left first), then descends. The leaf-root case runs
+   through the same loop body (one step, no children) so that the triangle
+   test and the traversal are emitted once. */
 DNOINLINE bool trace_mesh(DTrace &tr, const StDScene &sc, uint32_t mesh_id, bool any_hit) {
     const StDMesh mesh = load_mesh(sc, mesh_id);
     const uint32_t *nodes = sc.mesh_nodes + 8 * (size_t)mesh.node_offset;
@@ -566,32 +593,26 @@ DNOINLINE bool trace_mesh(DTrace &tr, const StDScene &sc, uint32_t mesh_id, bool
     uint32_t root_first, root_count;
     load_node(nodes, 0, root, root_first, root_count);
     if (!hit_aabb(root, ray, closest.distance, ld)) return false;
-    if (root_count) return hit_triangles(tr, tris, 0, mesh.triangle_count, any_hit);
+    const bool root_leaf = root_count != 0;
     uint32_t left = root_first;
-    bool has_left, has_right;
     uint32_t stack_size = 0;
     for (;;) {
-        DNodePair np = load_pair(nodes, left);
-        bool hl = hit_aabb(np.lb, ray, closest.distance, ld);
-        bool hr = hit_aabb(np.rb, ray, closest.distance, rd);
-        has_left = hl;
-        has_right = hr;
-        uint32_t lnext = np.lfirst, rnext = np.rfirst;
-        if (hl && np.lcount) {
-            if (hit_triangles(tr, tris, np.lfirst, np.lcount, any_hit)) {
-                closest.object_id += np.lfirst;
-                found = true;
-                if (any_hit) break;
-            }
-            has_left = false;
+        uint32_t f0 = 0, c0 = 0, f1 = 0, c1 = 0, lnext = 0, rnext = 0;
+        bool has_left = false, has_right = false;
+        if (root_leaf) {
+            c0 = mesh.triangle_count;
+        } else {
+            DNodePair np = load_pair(nodes, left);
+            const bool hl = hit_aabb(np.lb, ray, closest.distance, ld);
+            const bool hr = hit_aabb(np.rb, ray, closest.distance, rd);
+            lnext = np.lfirst;
+            rnext = np.rfirst;
+            if (hl && np.lcount) { f0 = np.lfirst; c0 = np.lcount; } else has_left = hl;
+            if (hr && np.rcount) { f1 = np.rfirst; c1 = np.rcount; } else has_right = hr;
         }
-        if (hr && np.rcount) {
-            if (hit_triangles(tr, tris, np.rfirst, np.rcount, any_hit)) {
-                closest.object_id += np.rfirst;
-                found = true;
-                if (any_hit) break;
-            }
-            has_right = false;
+        if (c0 + c1 && hit_triangles(tr, tris, f0, c0, f1, c1, any_hit)) {
+            found = true;
+            if (any_hit) break;
         }
         if (has_left) {
             if (has_right) {
@@ -610,23 +631,33 @@ DNOINLINE bool trace_mesh(DTrace &tr, const StDScene &sc, uint32_t mesh_id, bool
     return found;
 }
 
-/* intersection/primitives.h:9-111. `ids` are scene-BVH leaf ids. */
+/* Pixel box [x0, x1) x [y0, y1) of one primary-ray tile. */
+struct DTileBox { int32_t x0, y0, x1, y1; };
+
+/* intersection/primitives.h:9-111. `ids` are scene-BVH leaf ids, visited in
+   that order. With `rects` (the primary pass) a primitive is tested only at
+   pixels inside its screen-space rect. With `box` as well (called by the
+   whole wave, every lane active, one tile per wave) the wave first tests 64
+   primitives at a time against the tile's box and visits only those whose
+   rect meets it, in the same order: the per-pixel test is unchanged, so
+   the result is too, and a visited primitive's record is one scalar load
+   per wave instead of a load per pixel for every primitive. */
 DNOINLINE bool hit_primitives(const DRay &ray, DTrace &tr, const StDScene &sc, const uint32_t *ids, uint32_t count,
-                              bool any_hit, const int32_t *rects, uint32_t x, uint32_t y) {
+                              bool any_hit, const int32_t *rects, uint32_t x, uint32_t y,
+                              const DTileBox *box = nullptr) {
     bool found = false, cur = false;
     DHit &hit = tr.current_hit;
     DHit &closest = tr.closest_hit;
     float cone_angle = closest.cone_angle;
     uint32_t hit_prim_id = 0;
-    for (uint32_t i = 0; i < count; i++) {
-        uint32_t pid = ldgu(ids, i);
-        DPrim p = load_prim(sc, pid);
-        if (any_hit && !(p.flags & ST_IS_SHADOWING)) continue;
+    /* one primitive; true ends the walk (any-hit search satisfied) */
+    auto visit = [&](uint32_t pid, const DPrim &p) -> bool {
+        if (any_hit && !(p.flags & ST_IS_SHADOWING)) return false;
         if (rects) {
             /* IS_VISIBLE is folded into the per-view rect: an invisible
                primitive has an empty rect. */
             int4 r = ldgi4(rects, pid);
-            if ((int32_t)x < r.x || (int32_t)x >= r.z || (int32_t)y < r.y || (int32_t)y >= r.w) continue;
+            if ((int32_t)x < r.x || (int32_t)x >= r.z || (int32_t)y < r.y || (int32_t)y >= r.w) return false;
         }
         sv3 Ro, Rd;
         to_object(ray.origin, ray.direction, p, Ro, Rd);
@@ -646,7 +677,7 @@ DNOINLINE bool hit_primitives(const DRay &ray, DTrace &tr, const StDScene &sc, c
                 cur = trace_mesh(tr, sc, p.mesh_id, any_hit);
                 if (cur) hit = tr.closest_mesh_hit;
                 break;
-            default: continue;
+            default: return false;
         }
         if (cur) {
             hit.position = pos_to_world(hit.position, p);
@@ -657,8 +688,29 @@ DNOINLINE bool hit_primitives(const DRay &ray, DTrace &tr, const StDScene &sc, c
                 closest.object_type = p.type;
                 closest.material_id = p.material_id;
                 found = true;
-                if (any_hit) break;
+                if (any_hit) return true;
             }
+        }
+        return false;
+    };
+    if (box) {
+        bool done = false;
+        for (uint32_t base = 0; base < count && !done; base += 64) {
+            const uint32_t j = base + __lane_id();
+            bool meets = false;
+            if (j < count) {
+                const int4 r = ldgi4(rects, ldgu(ids, j));
+                meets = r.x < r.z && r.y < r.w && r.x < box->x1 && box->x0 < r.z && r.y < box->y1 && box->y0 < r.w;
+            }
+            for (uint64_t cand = __ballot(meets); cand; cand &= cand - 1) {
+                const uint32_t pid = ldcu(ids, base + (uint32_t)__builtin_ctzll(cand));
+                if (visit(pid, load_prim_c(sc, pid))) { done = true; break; }
+            }
+        }
+    } else {
+        for (uint32_t i = 0; i < count; i++) {
+            const uint32_t pid = ldgu(ids, i);
+            if (visit(pid, load_prim(sc, pid))) break;
         }
     }
     if (found) {
@@ -702,7 +754,10 @@ DNOINLINE bool hit_primitives(const DRay &ray, DTrace &tr, const StDScene &sc, c
     return found;
 }
 
-/* trace.h:8-76 */
+/* trace.h:8-76. Same step structure as trace_mesh: both children's boxes,
+   then the leaves among them (left first, each one primitive-test call as
+   the reference makes it), then descent; the leaf-root case is one step of
+   the same loop, so the primitive tests are emitted once. */
 DNOINLINE bool trace_scene(DRay &ray, DTrace &tr, const StDScene &sc, bool any_hit) {
     ray.direction_reciprocal = v_rcp(ray.direction);
     prep_ray(ray);
@@ -712,30 +767,38 @@ DNOINLINE bool trace_scene(DRay &ray, DTrace &tr, const StDScene &sc, bool any_h
     uint32_t root_first, root_count;
     load_node(sc.scene_nodes, 0, root, root_first, root_count);
     if (!hit_aabb(root, ray, tr.closest_hit.distance, ld)) return false;
-    if (root_count) return hit_primitives(ray, tr, sc, sc.scene_leaf_ids, sc.prim_count, any_hit, nullptr, 0, 0);
+    const bool root_leaf = root_count != 0;
     uint32_t left = root_first;
     uint32_t *stack = tr.scene_stack;
     uint32_t stack_size = 0;
     for (;;) {
-        DNodePair np = load_pair(sc.scene_nodes, left);
-        bool hl = hit_aabb(np.lb, ray, tr.closest_hit.distance, ld);
-        bool hr = hit_aabb(np.rb, ray, tr.closest_hit.distance, rd);
-        bool has_left = hl, has_right = hr;
-        uint32_t lnext = np.lfirst, rnext = np.rfirst;
-        if (hl && np.lcount) {
-            if (hit_primitives(ray, tr, sc, sc.scene_leaf_ids + np.lfirst, np.lcount, any_hit, nullptr, 0, 0)) {
-                found = true;
-                if (any_hit) break;
-            }
-            has_left = false;
+        uint32_t f0 = 0, c0 = 0, f1 = 0, c1 = 0, nleaf = 0, lnext = 0, rnext = 0;
+        bool has_left = false, has_right = false;
+        if (root_leaf) {
+            c0 = sc.prim_count;
+            nleaf = 1;
+        } else {
+            DNodePair np = load_pair(sc.scene_nodes, left);
+            const bool hl = hit_aabb(np.lb, ray, tr.closest_hit.distance, ld);
+            const bool hr = hit_aabb(np.rb, ray, tr.closest_hit.distance, rd);
+            lnext = np.lfirst;
+            rnext = np.rfirst;
+            if (hl && np.lcount) { f0 = np.lfirst; c0 = np.lcount; nleaf = 1; } else has_left = hl;
+            if (hr && np.rcount) {
+                if (nleaf) { f1 = np.rfirst; c1 = np.rcount; } else { f0 = np.rfirst; c0 = np.rcount; }
+                nleaf++;
+            } else has_right = hr;
         }
-        if (hr && np.rcount) {
-            if (hit_primitives(ray, tr, sc, sc.scene_leaf_ids + np.rfirst, np.rcount, any_hit, nullptr, 0, 0)) {
+        bool stop = false;
+#pragma clang loop unroll(disable)
+        for (uint32_t k = 0; k < nleaf; k++) {
+            const uint32_t first = k ? f1 : f0, count = k ? c1 : c0;
+            if (hit_primitives(ray, tr, sc, sc.scene_leaf_ids + first, count, any_hit, nullptr, 0, 0)) {
                 found = true;
-                if (any_hit) break;
+                if (any_hit) { stop = true; break; }
             }
-            has_right = false;
         }
+        if (stop) break;
         if (has_left) {
             if (has_right) {
                 if (!any_hit && ld > rd) { uint32_t t = lnext; lnext = rnext; rnext = t; }
