@@ -116,7 +116,8 @@ int st_render_batch_bgra8(StSceneHandle *const *scenes, const StView *views, uin
 /* Launch shape of the render kernel (process-wide). */
 typedef struct StTuning {
     uint32_t tile_shift;     /* tile = (1 << tile_shift) x (64 >> tile_shift) pixels, one wavefront per tile */
-    uint32_t blocks_per_cu;  /* persistent grid: blocks per compute unit (4 wavefronts each) */
+    uint32_t blocks_per_cu;  /* persistent grid: blocks per compute unit (4 wavefronts each); one wavefront
+                                per tile: at most this many blocks resident per compute unit (4 or more: no cap) */
     uint32_t persistent;     /* 1: persistent tile queue (st_render); 0: one wavefront per tile */
 } StTuning;
 

@@ -1352,7 +1352,11 @@ int st_rt_launch_prepared(StSceneHandle *const *scenes, StHostView *hv, uint32_t
        order, so a wave that had to wait for a slot would run its whole
        share late */
     uint32_t b_shadow0 = 0, b_shadow = 0, b_shade0 = 0, b_shade = 0, b_bounce = 0;
-    size_t lds_shadow = lds;
+    size_t lds_shadow = lds, lds_primary = lds;
+    /* one wave per tile: at most blocks_per_cu of its blocks per CU (fewer
+       tiles in flight share the caches better) */
+    if (!persistent && g.tune.blocks_per_cu < 4)
+        (void)capped_blocks(reinterpret_cast<const void *>(k_primary), lds, g.tune.blocks_per_cu, &lds_primary);
     if (levels) {
         b_shadow0 = capped_blocks(reinterpret_cast<const void *>(k_shadow<true>), lds, ST_SHADOW_WAVES_PER_SIMD,
                                   &lds_shadow);
@@ -1390,7 +1394,7 @@ int st_rt_launch_prepared(StSceneHandle *const *scenes, StHostView *hv, uint32_t
         HIP_TRY(hipGetLastError());
         if (!graph) HIP_TRY(hipEventRecord(g.ev[0], g.stream));
         HIP_TRY(mark());
-        hipLaunchKernelGGL(k_primary, dim3(blocks), dim3(BLOCK_THREADS), lds, g.stream, (const StDScene *)sc_d,
+        hipLaunchKernelGGL(k_primary, dim3(blocks), dim3(BLOCK_THREADS), lds_primary, g.stream, (const StDScene *)sc_d,
                            (const KReq *)reqs_d, n, tiles, g.tile_counter, (uint32_t)persistent, xcd_blocks,
                            s_entries, m_entries, w);
         HIP_TRY(hipGetLastError());
@@ -1435,7 +1439,8 @@ int st_rt_launch_prepared(StSceneHandle *const *scenes, StHostView *hv, uint32_t
            the upload block, copied from the same pinned staging) */
         std::vector<uint64_t> sig = {n, up, rows, tiles, levels, lmax, count_blocks, blocks, xcd_blocks, any_device_ssb,
                                      (uint64_t)persistent, s_entries, m_entries, b_shadow0, b_shadow, b_shade0,
-                                     b_shade, b_bounce, (uint64_t)(uintptr_t)blk, (uint64_t)(uintptr_t)st,
+                                     b_shade, b_bounce, lds_primary, lds_shadow, (uint64_t)(uintptr_t)blk,
+                                     (uint64_t)(uintptr_t)st,
                                      (uint64_t)(uintptr_t)cnt_d, (uint64_t)(uintptr_t)w.st, w.n};
         for (uint32_t i = 0; i < n; i++) {
             const StDView &dv = reqs_h[i].v;
@@ -1741,7 +1746,8 @@ extern "C" int st_get_tuning(StTuning *t) {
 /* Time every candidate launch shape on the caller's requests (median of
    `reps` render-kernel times from HIP events) and keep the fastest. The
    candidates: 8x8, 16x4 and 32x2 tiles; persistent grids of 2, 4 or 8
-   blocks per CU, or one block per 4 tiles in XCD-aware order. */
+   blocks per CU, or one block per 4 tiles in XCD-aware order with at most
+   2, 3 or any number of blocks per CU resident. */
 extern "C" int st_autotune(StSceneHandle *const *scenes, const StView *views, uint32_t n, uint32_t reps,
                            StTuning *best, float *best_ms) {
     RT_LOCK;
@@ -1751,12 +1757,12 @@ extern "C" int st_autotune(StSceneHandle *const *scenes, const StView *views, ui
     if (reps == 0) reps = 3;
     if (reps > 15) reps = 15;
     const StTuning saved = g.tune;
-    StTuning cand[12];
+    StTuning cand[18];
     uint32_t nc = 0;
     for (uint32_t sh = 3; sh <= 5; sh++) {
-        const uint32_t bpc[3] = {2, 4, 8};
+        const uint32_t bpc[3] = {2, 4, 8}, cap[3] = {2, 3, 8};
         for (int k = 0; k < 3; k++) cand[nc++] = StTuning{sh, bpc[k], 1u};
-        cand[nc++] = StTuning{sh, 8u, 0u};
+        for (int k = 0; k < 3; k++) cand[nc++] = StTuning{sh, cap[k], 0u};
     }
     float best_t = INFINITY;
     StTuning best_c = saved;

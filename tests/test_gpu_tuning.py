@@ -22,7 +22,7 @@ def test_all_launch_shapes_give_identical_frames():
                                 make_view(width=64, height=200, mode=MODE_NORMALS, **cam)])
         ref = None
         for shift in (3, 4, 5):
-            for bpc in (2, 8):
+            for bpc in (2, 3, 8):
                 for persistent in (True, False):
                     st.set_tuning(shift, bpc, persistent)
                     pix, hits, counts = st.render_batch([ds, ds], views, hits=True)
