@@ -182,10 +182,17 @@ __global__ void __launch_bounds__(64) k_primary_dirs(const KReq *reqs, uint32_t 
     }
 }
 
+/* Tile coordinates of launch tile `tile` within its request's tile grid. */
+__device__ static inline void tile_coords(const StDView &v, uint32_t tile, uint32_t &tx, uint32_t &ty) {
+    const uint32_t local = tile - v.tile_base;
+    tx = local % v.tiles_x;
+    ty = local / v.tiles_x;
+}
+
 /* Launch tile `tile`'s pixel for lane `lane`; false outside the traced grid. */
 __device__ static inline bool tile_pixel(const StDView &v, uint32_t tile, uint32_t lane, uint32_t &x, uint32_t &y) {
-    const uint32_t local = tile - v.tile_base;
-    const uint32_t tx = local % v.tiles_x, ty = local / v.tiles_x;
+    uint32_t tx, ty;
+    tile_coords(v, tile, tx, ty);
     const uint32_t tw = 1u << v.tile_shift, th = 64u >> v.tile_shift;
     x = tx * tw + (lane & (tw - 1u));
     y = ty * th + (lane >> v.tile_shift);
@@ -226,10 +233,11 @@ __device__ static inline __attribute__((always_inline)) void primary_tile(const 
     tr.primary_triangle = -1;
     DTileBox box;
     {
-        const uint32_t local = tile - v.tile_base;
+        uint32_t tx, ty;
+        tile_coords(v, tile, tx, ty);
         const uint32_t tw = 1u << v.tile_shift, th = 64u >> v.tile_shift;
-        box.x0 = (int32_t)((local % v.tiles_x) * tw);
-        box.y0 = (int32_t)((local / v.tiles_x) * th);
+        box.x0 = (int32_t)(tx * tw);
+        box.y0 = (int32_t)(ty * th);
         box.x1 = box.x0 + (int32_t)tw;
         box.y1 = box.y0 + (int32_t)th;
     }
