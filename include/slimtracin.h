@@ -113,7 +113,8 @@ int st_render_batch(StSceneHandle *const *scenes, const StView *views, uint32_t 
 int st_render_batch_bgra8(StSceneHandle *const *scenes, const StView *views, uint32_t n, uint32_t *const *bgra,
                           StRayCounts *counts, uint32_t flags);
 
-/* Launch shape of the render kernel (process-wide). */
+/* Launch shape of the primary stage (process-wide): tiles per wavefront and
+   grid; the depth loop's stages run at their own fixed shapes. */
 typedef struct StTuning {
     uint32_t tile_shift;     /* tile = (1 << tile_shift) x (64 >> tile_shift) pixels, one wavefront per tile */
     uint32_t blocks_per_cu;  /* persistent grid: blocks per compute unit (4 wavefronts each); one wavefront
@@ -151,8 +152,9 @@ void st_host_free(void *p);
    FloatPixels, host memory) into (grid_w-1) x (grid_h-1) BGRA8 pixels. */
 int st_resolve_bgra8(const StFloatPixel *grid, uint32_t grid_w, uint32_t grid_h, uint32_t *out);
 
-/* As st_render_batch without downloads; returns the render kernel's
-   duration measured with HIP events on the launch stream. */
+/* As st_render_batch without downloads; returns the duration of the
+   launch's device work (all stages) measured with HIP events on the launch
+   stream. */
 int st_render_timed(StSceneHandle *const *scenes, const StView *views, uint32_t n, uint32_t flags, float *kernel_ms,
                     StRayCounts *counts);
 

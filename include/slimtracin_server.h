@@ -48,7 +48,7 @@ typedef struct StServerStats {
     double latency_p50_ms, latency_p99_ms, latency_max_ms;  /* submit -> frame ready, completed requests */
     double mean_batch;                                       /* requests per launch */
     double prep_ms, launch_ms, kernel_ms, alloc_ms;          /* totals: host set-up, launch wall (uploads,
-                                                                kernels, downloads), render kernel, output
+                                                                kernels, downloads), device work of the launches, output
                                                                 buffer allocation at submit */
 } StServerStats;
 

@@ -39,7 +39,7 @@
 #define D_TET_A 0.577350259f
 #define D_TET_B 0.288675159f
 
-/* Everything is inlined into the render kernel so the per-ray state stays in
+/* Everything is inlined into the stage kernels so the per-ray state stays in
    registers (a call boundary would force the shared trace state to scratch). */
 #define DFN __device__ static inline __attribute__((always_inline))
 #ifndef DNOINLINE
