@@ -7,8 +7,10 @@ noise-displaced geodesic icosphere with 200,000 fp32 triangles (frequency
 and 2 emissive quads; 1920x1080 window (1921x1081 traced QCAA grid), 1 spp,
 trace depth 4, Beauty mode. The scene is resident in HBM before timing;
 a step is the per-frame host set-up (camera basis, SSB rects, ~20 KB of
-uploads), the primary-direction kernel and the render stages (primary,
-then shadow / shade / bounce per depth-loop level); the frame stays in HBM
+uploads), the primary-direction kernel and the render stages (primary
+with the first level's shadow rays, then shade / bounce per depth-loop
+level, each bounce stage with the next level's shadow rays); the frame stays
+in HBM
 (the PCIe-inclusive rate is reported separately).
 
 Also measured in the same run:
@@ -32,13 +34,14 @@ Also measured in the same run:
     admitted against a 200 GB HBM budget), requests of the same mix arriving
     as a Poisson process at a stated rate for a stated time, served by the
     continuous-batching loop; Mrays/s, p50/p99 submit-to-image latency;
-  - roofline of the dominant stage launch (the first level's shadow stage,
-    k_shadow): the algorithmic bytes of that level's shadow rays from the CPU
-    oracle's per-ray-class event counters on the same 1920x1080 frame (bytes
-    per event at the reference's record sizes), over the launch's duration
-    from HIP events recorded around each stage on the launch stream; the
-    same for the primary stage and for the whole frame ("stages",
-    "roofline_frame"); the measured HBM traffic per launch comes from the
+  - roofline of the dominant stage launch (the primary stage, k_primary,
+    which traces the primary rays and the first level's shadow rays): the
+    algorithmic bytes of those rays' traversals from the CPU oracle's
+    per-ray-class event counters on the same 1920x1080 frame (bytes per
+    event at the reference's record sizes), over the launch's duration from
+    HIP events recorded around each stage on the launch stream; the same
+    for the whole frame ("roofline_frame"); the measured HBM traffic per
+    launch comes from the
     rocprofv3 FETCH_SIZE / WRITE_SIZE summary kept in profiles/ (calibrated
     against a copy kernel of known size, --copy-probe);
   - the reference CPU renderer (compiled from the reference's own sources)
@@ -115,13 +118,12 @@ def cpu_baseline(blob, view, threads, frames=1):
                        f"{threads} threads, median of {frames} frame(s), {rays} rays, {t:.2f} s/frame"), stats, hits
 
 
-TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "k_shadow_level0_pbr200k_1920x1080.txt")
+TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "k_primary_pbr200k_1920x1080.txt")
 
 
 def measured_traffic():
-    """HBM bytes per first-level shadow-stage launch from the committed
-    rocprofv3 PMC summary (line 'traffic_bytes_per_launch: N'), or None when
-    absent."""
+    """HBM bytes per primary-stage launch from the committed rocprofv3 PMC
+    summary (line 'traffic_bytes_per_launch: N'), or None when absent."""
     try:
         with open(TRAFFIC_SUMMARY) as f:
             for line in f:
@@ -490,14 +492,12 @@ def main():
             return {"achieved": round(a, 2), "frac": round(a / HBM_PEAK_GBS, 5),
                     "algorithmic_bytes_per_launch": int(nbytes), "ms": round(ms, 4)}
 
-        shadow0 = line(stats.traversal_bytes("shadow0"), stages["shadow0"])
-        result["roofline"] = {"bound": "hbm", "achieved": shadow0["achieved"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                              "frac": shadow0["frac"], "traffic": measured_traffic(),
-                              "kernel": "k_shadow, first depth-loop level",
-                              "algorithmic_bytes_per_launch": shadow0["algorithmic_bytes_per_launch"],
-                              "launch_ms": shadow0["ms"]}
-        result["roofline_stages"] = {"primary": line(stats.traversal_bytes("primary"), stages["primary"]),
-                                     "shadow0": shadow0}
+        primary = line(stats.traversal_bytes("primary") + stats.traversal_bytes("shadow0"), stages["primary"])
+        result["roofline"] = {"bound": "hbm", "achieved": primary["achieved"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                              "frac": primary["frac"], "traffic": measured_traffic(),
+                              "kernel": "k_primary (primary rays and the first level's shadow rays)",
+                              "algorithmic_bytes_per_launch": primary["algorithmic_bytes_per_launch"],
+                              "launch_ms": primary["ms"]}
         result["roofline_frame"] = dict(line(frame_bytes, kernel_ms), bytes_per_ray=round(frame_bytes / oracle_rays, 2))
         gp, gh, gc = st.render(scene, view)
         same = bool((gh["prim_id"] == ohits["prim_id"]).all() and (gh["triangle_id"] == ohits["triangle_id"]).all()

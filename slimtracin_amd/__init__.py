@@ -280,10 +280,12 @@ def render_timed(scenes, views: np.ndarray, persistent: bool = True, stages: boo
 
 def stage_names(levels: int):
     """Names of the stage launches of a frame whose Beauty depth loop has
-    `levels` levels (the order render_timed(stages=True) reports)."""
+    `levels` levels (the order render_timed(stages=True) reports). "primary"
+    includes the first level's shadow rays and "bounce{lv}" the shadow rays
+    of level lv + 1."""
     names = ["primary"]
     for lv in range(levels):
-        names += [f"shadow{lv}", f"shade{lv}"] + ([f"bounce{lv}"] if lv + 1 < levels else [])
+        names += [f"shade{lv}"] + ([f"bounce{lv}"] if lv + 1 < levels else [])
     return names
 
 

@@ -9,8 +9,9 @@
  *                   unnormalised `current` vector (the primary stage
  *                   normalises it). Rows of every request of a launch are
  *                   flattened into one grid.
- *   k_primary       one wavefront per 8x8 pixel tile: primary ray and the
- *                   Beauty shading entry (st_wave.h). Tiles of all requests
+ *   k_primary       one wavefront per 8x8 pixel tile: primary ray, the
+ *                   Beauty shading entry and the first level's shadow rays
+ *                   (st_wave.h). Tiles of all requests
  *                   of a launch form one list; in persistent mode each
  *                   wavefront keeps pulling the next tile from a global
  *                   counter until the list is exhausted (the loop's exit
@@ -18,9 +19,10 @@
  *                   wave id with an XCD-aware block order. A single request
  *                   is just a list with one entry, so plain and batched
  *                   launches run the same code.
- *   k_shadow, k_shade, k_bounce
+ *   k_shade, k_bounce
  *                   the Beauty depth loop as queue stages, one launch per
- *                   stage and level (st_wave.h).
+ *                   stage and level (st_wave.h); k_bounce also traces the
+ *                   next level's shadow rays.
  *   k_counts        per-request shadow/bounce ray totals.
  */
 #include <hip/hip_runtime.h>
@@ -49,9 +51,6 @@
 /* Launch attributes of the primary stage (occupancy experiments). */
 #ifndef K_PRIMARY_ATTR
 #define K_PRIMARY_ATTR
-#endif
-#ifndef K_SHADOW_ATTR
-#define K_SHADOW_ATTR
 #endif
 /* The shade stage at 3 waves per SIMD (168 VGPRs, a few spilled) instead of
    the 2 its 176 VGPRs allow: 0.27 ms instead of 0.35 ms for the first level
@@ -314,6 +313,13 @@ __device__ static inline __attribute__((always_inline)) void primary_tile(const 
         pix_store_idle(w, tile * 64u + lane);
         record_counts(w, tile * 64u + lane, 0, 0);
     }
+    /* the first level's shadow rays start at the surfaces this wave just
+       found, whose nodes and triangles are still in its caches */
+    if (entered) {
+        const uint32_t slot = tile * 64u + lane;
+        trace_level_shadows(w, sc, scene_stack, mesh_stack, slot, f4v3(pl_ld(w, P_POS, slot)),
+                            f4v3(pl_ld(w, P_NRM, slot)));
+    }
 }
 
 /* Traversal stacks live in LDS, one column per lane (no bank conflicts):
@@ -384,36 +390,6 @@ __device__ static inline __attribute__((always_inline)) void drain_queue(const K
 
 __device__ static inline uint32_t queue_count(const KWave &w, uint32_t level, uint32_t which) {
     return __builtin_amdgcn_readfirstlane(ldgu(w.ctr, (size_t)level * WC_PER_LEVEL + which));
-}
-
-/* Shadow stage: any-hit traversal of the level's shadow rays. LEVEL0 (the
-   first level, its own kernel): every (light, slot) pair, light-major, so a
-   wave traces one tile's rays to one light (coherent); the pixel's light
-   mask says which exist. Later levels: the compacted queue. */
-template <bool LEVEL0>
-__global__ void __launch_bounds__(BLOCK_THREADS) K_SHADOW_ATTR k_shadow(const StDScene *__restrict__ scenes,
-                                                          const KReq *__restrict__ reqs, uint32_t nreq, KWave w,
-                                                          uint32_t level, uint32_t scene_entries,
-                                                          uint32_t mesh_entries) {
-    ST_LDS_STACKS(scene_entries, mesh_entries);
-    const uint32_t count = LEVEL0 ? w.lmax * w.n : queue_count(w, level, WC_SHADOW);
-    drain_queue(reqs, nreq, count,
-                [&](uint32_t qi, uint32_t &slot, uint32_t &light) {
-                    if (!LEVEL0) {
-                        slot = ldgu(w.shadow_q, (size_t)qi * 2);
-                        light = ldgu(w.shadow_q, (size_t)qi * 2 + 1);
-                        return true;
-                    }
-                    light = qi / w.n;
-                    slot = qi - light * w.n;
-                    return ((ldgu(w.mask, (size_t)(light >> 5) * w.n + slot) >> (light & 31u)) & 1u) != 0;
-                },
-                [&](uint32_t r, uint32_t slot, uint32_t light) {
-                    const StDScene sc = scenes[reqs[r].v.scene];
-                    DTrace tr;
-                    trace_init(tr, sc, scene_stack, mesh_stack, __uint_as_float(ST_DIST_UNSET));
-                    shadow_ray(w, sc, tr, slot, light);
-                });
 }
 
 /* Shade stage: lights, emissive quads and glow of the level's pixels; each
@@ -503,6 +479,7 @@ __global__ void __launch_bounds__(BLOCK_THREADS) K_BOUNCE_ATTR k_bounce(const St
                             p.material_id = hit.material_id;
                             p.hit_d2 = hit.distance_squared;
                             begin_level(w, level + 1, sc, v, M, hit, p, slot);
+                            trace_level_shadows(w, sc, scene_stack, mesh_stack, slot, p.pos, p.nrm);
                         }
                     }
                     if (done) {
@@ -1188,14 +1165,6 @@ static uint32_t resident_blocks(const void *kernel, size_t lds) {
 
 /* LDS per CU on gfx950 */
 #define CU_LDS_BYTES (160u * 1024u)
-/* Waves per SIMD at most for the shadow stages (a block holds one wave per
-   SIMD, so this is also blocks per CU). The shadow kernel fits in 128 VGPRs,
-   which would allow 4, but on the 200k-triangle 1080p frame the first shadow
-   level took 1.45 ms at 4 waves per SIMD, 1.0 ms at 3 and 1.55 ms at 2. */
-#ifndef ST_SHADOW_WAVES_PER_SIMD
-#define ST_SHADOW_WAVES_PER_SIMD 3u
-#endif
-
 /* resident_blocks with at most `cap` blocks per CU (0: no cap): the block's
    LDS request is raised until no more than `cap` fit; *lds_launch is the
    request to launch with */
@@ -1279,9 +1248,9 @@ int st_rt_launch_prepared(StSceneHandle *const *scenes, StHostView *hv, uint32_t
     size_t wave_bytes = 0;
     if (levels) {
         if (slots > 0xFFFFFFC0ull || slots * lmax > 0xFFFFFFC0ull) return set_err(ST_ERR_ARG, "batch too large");
-        wave_bytes = round256(slots * 16 * ST_PLANES) + 2 * round256(slots * lmax * 8) + round256(slots * 4) +
+        wave_bytes = round256(slots * 16 * ST_PLANES) + round256(slots * lmax * 8) + round256(slots * 4) +
                      3 * round256(slots * 4) + round256((size_t)levels * WC_PER_LEVEL * 4) + round256(slots * 8) +
-                     round256(slots * 4 * ((lmax + 31) / 32)) + 9 * 256;
+                     7 * 256;
     }
     size_t work = up + round256(n * 24ull) + per_req_out + wave_bytes + 256;
     if ((rc = arena_reserve(&g.work, work))) return rc;
@@ -1334,7 +1303,6 @@ int st_rt_launch_prepared(StSceneHandle *const *scenes, StHostView *hv, uint32_t
         w.n = (uint32_t)slots;
         w.lmax = lmax;
         w.st = (float *)arena_take(&g.work, slots * 16 * ST_PLANES);
-        w.shadow_q = (uint32_t *)arena_take(&g.work, slots * lmax * 8);
         w.shadow_res = (uint32_t *)arena_take(&g.work, slots * lmax * 8);
         w.occ_d2 = (float *)arena_take(&g.work, slots * 4);
         w.shade_q[0] = (uint32_t *)arena_take(&g.work, slots * 4);
@@ -1342,7 +1310,6 @@ int st_rt_launch_prepared(StSceneHandle *const *scenes, StHostView *hv, uint32_t
         w.bounce_q = (uint32_t *)arena_take(&g.work, slots * 4);
         w.ctr = (uint32_t *)arena_take(&g.work, (size_t)levels * WC_PER_LEVEL * 4);
         w.counts = (uint32_t *)arena_take(&g.work, slots * 8);
-        w.mask = (uint32_t *)arena_take(&g.work, slots * 4 * ((lmax + 31) / 32));
     }
     /* launch shape, fixed before anything is enqueued so that a captured
        graph of this launch can be matched against the next one */
@@ -1368,17 +1335,13 @@ int st_rt_launch_prepared(StSceneHandle *const *scenes, StHostView *hv, uint32_t
        the device at once: they take queue chunks in a fixed interleaved
        order, so a wave that had to wait for a slot would run its whole
        share late */
-    uint32_t b_shadow0 = 0, b_shadow = 0, b_shade0 = 0, b_shade = 0, b_bounce = 0;
-    size_t lds_shadow = lds, lds_primary = lds;
+    uint32_t b_shade0 = 0, b_shade = 0, b_bounce = 0;
+    size_t lds_primary = lds;
     /* one wave per tile: at most blocks_per_cu of its blocks per CU (fewer
        tiles in flight share the caches better) */
     if (!persistent && g.tune.blocks_per_cu < 4)
         (void)capped_blocks(reinterpret_cast<const void *>(k_primary), lds, g.tune.blocks_per_cu, &lds_primary);
     if (levels) {
-        b_shadow0 = capped_blocks(reinterpret_cast<const void *>(k_shadow<true>), lds, ST_SHADOW_WAVES_PER_SIMD,
-                                  &lds_shadow);
-        b_shadow = capped_blocks(reinterpret_cast<const void *>(k_shadow<false>), lds, ST_SHADOW_WAVES_PER_SIMD,
-                                 &lds_shadow);
         b_shade0 = resident_blocks(reinterpret_cast<const void *>(k_shade<true>), 0);
         b_shade = resident_blocks(reinterpret_cast<const void *>(k_shade<false>), 0);
         b_bounce = resident_blocks(reinterpret_cast<const void *>(k_bounce), lds);
@@ -1420,13 +1383,6 @@ int st_rt_launch_prepared(StSceneHandle *const *scenes, StHostView *hv, uint32_t
            device, so the sequence needs no host round trip */
         for (uint32_t lv = 0; lv < levels; lv++) {
             if (lv == 0)
-                hipLaunchKernelGGL(k_shadow<true>, dim3(b_shadow0), dim3(BLOCK_THREADS), lds_shadow, g.stream,
-                                   (const StDScene *)sc_d, (const KReq *)reqs_d, n, w, lv, s_entries, m_entries);
-            else
-                hipLaunchKernelGGL(k_shadow<false>, dim3(b_shadow), dim3(BLOCK_THREADS), lds_shadow, g.stream,
-                                   (const StDScene *)sc_d, (const KReq *)reqs_d, n, w, lv, s_entries, m_entries);
-            HIP_TRY(mark());
-            if (lv == 0)
                 hipLaunchKernelGGL(k_shade<true>, dim3(b_shade0), dim3(BLOCK_THREADS), 0, g.stream,
                                    (const StDScene *)sc_d, (const KReq *)reqs_d, n, w, lv);
             else
@@ -1455,8 +1411,8 @@ int st_rt_launch_prepared(StSceneHandle *const *scenes, StHostView *hv, uint32_t
            graph does exactly this launch's work (the per-frame data is in
            the upload block, copied from the same pinned staging) */
         std::vector<uint64_t> sig = {n, up, rows, tiles, levels, lmax, count_blocks, blocks, xcd_blocks, any_device_ssb,
-                                     (uint64_t)persistent, s_entries, m_entries, b_shadow0, b_shadow, b_shade0,
-                                     b_shade, b_bounce, lds_primary, lds_shadow, (uint64_t)(uintptr_t)blk,
+                                     (uint64_t)persistent, s_entries, m_entries, b_shade0, b_shade, b_bounce,
+                                     lds_primary, (uint64_t)(uintptr_t)blk,
                                      (uint64_t)(uintptr_t)st,
                                      (uint64_t)(uintptr_t)cnt_d, (uint64_t)(uintptr_t)w.st, w.n};
         for (uint32_t i = 0; i < n; i++) {

@@ -7,16 +7,18 @@
  * direct lights with one shadow ray each, emissive quads, light glow, and a
  * reflection/refraction bounce ray. Run as one GPU thread per pixel, that
  * loop keeps the shading state of a pixel live across three different
- * traversals and needs far more registers than a lane has. Here each
- * traversal kind runs in its own kernel over a compacted queue:
+ * traversals and needs far more registers than a lane has. Here the loop is
+ * cut where the shading state is largest, and each stage runs over a
+ * compacted queue:
  *
  *   primary   one wave per 8x8 tile: primary ray, Depth/Normals/UVs output,
- *             Beauty shading entry and the first level's shadow rays
- *   shadow    any-hit traversal of every queued shadow ray
+ *             Beauty shading entry, then the first level's shadow rays,
+ *             traced at once by the same wave (their origins are the hit
+ *             points just found, so the tile's nodes are still in cache)
  *   shade     per queued pixel: direct light from the shadow results,
  *             emissive quads, glow, and the bounce decision
  *   bounce    closest-hit traversal of every queued bounce ray, then the
- *             next level's shading entry and shadow rays
+ *             next level's shading entry and its shadow rays
  *
  * Between stages a pixel's state lives in HBM as planes of float4 (one plane
  * per four state values, slot = tile * 64 + lane), so that every state read
@@ -43,21 +45,18 @@ struct KWave {
     float *st;             /* ST_PLANES planes of n float4 */
     uint32_t n;            /* state slots: 64 per tile of the launch */
     uint32_t lmax;         /* planes of shadow_res (largest light count) */
-    uint32_t *shadow_q;    /* 2 words per shadow ray: slot, light */
     uint32_t *shadow_res;  /* [light][slot] 2 words: occluded, current-hit distance bits */
     float *occ_d2;         /* [slot] occluder's squared distance, last light's shadow ray */
     uint32_t *shade_q[2];  /* pixels entering a level's shading (ping-pong by level) */
     uint32_t *bounce_q;    /* pixels with a bounce ray */
     uint32_t *ctr;         /* WC_PER_LEVEL queue lengths per level */
     uint32_t *counts;      /* [slot] 2 words: shadow rays, bounce rays of the pixel */
-    uint32_t *mask;        /* [light / 32][slot] level 0: bit set = the light's shadow ray exists */
 };
 
-/* Queue lengths per level. Level 0 has no queues: its shade stage runs over
-   every slot (depth_left = 0 marks the pixels that did not enter the loop)
-   and its shadow stage over every (light, slot) pair, with the pixel's
-   light mask marking the rays that exist. */
-enum { WC_SHADOW = 0, WC_SHADE = 1, WC_BOUNCE = 2, WC_PER_LEVEL = 4 };
+/* Queue lengths per level. Level 0 has no shade queue: its shade stage runs
+   over every slot (depth_left = 0 marks the pixels that did not enter the
+   loop). */
+enum { WC_SHADE = 0, WC_BOUNCE = 1, WC_PER_LEVEL = 2 };
 
 /* state planes */
 enum { P_COLOR, P_THRU, P_VO, P_VD, P_POS, P_NRM, P_REF, P_ALB, P_BDIR, P_MISC, ST_PLANES };
@@ -121,7 +120,6 @@ DFN void pix_store_all(const KWave &w, uint32_t slot, const DPix &p) {
 /* a slot that does not enter the depth loop: no level-0 shading, no shadow rays */
 DFN void pix_store_idle(const KWave &w, uint32_t slot) {
     stgi4(w.st, (size_t)P_MISC * w.n + slot, 0, 0, 0, 0);
-    for (uint32_t wi = 0; wi * 32u < w.lmax; wi++) ((ST_GLOBAL uint32_t *)w.mask)[(size_t)wi * w.n + slot] = 0;
 }
 DFN uint32_t pix_depth_left(const KWave &w, uint32_t slot) { return ldgu((const uint32_t *)w.st, 4 * ((size_t)P_MISC * w.n + slot) + 1); }
 
@@ -140,21 +138,6 @@ DFN uint32_t wave_append(uint32_t *ctr, bool want) {
         base = __builtin_amdgcn_readlane(base, leader);
     }
     return base + __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
-}
-
-/* Reserve `total` (wave-uniform) entries of a queue whose length is *ctr
-   with one atomic from the first active lane; returns the first index. */
-DFN uint32_t wave_reserve(uint32_t *ctr, uint32_t total) {
-    uint32_t base = 0;
-    if (total) {
-        const uint32_t leader = (uint32_t)__builtin_ctzll(__ballot(1));
-        if (lane_id() == leader) base = atomicAdd(ctr, total);
-        base = __builtin_amdgcn_readlane(base, leader);
-    }
-    return base;
-}
-DFN uint32_t lanes_below(unsigned long long m) {
-    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
 }
 
 /* ---- shared shading steps ------------------------------------------------ */
@@ -245,10 +228,10 @@ DFN void finish_pixel(const StDScene &sc, const StDView &v, uint32_t pix, sv3 co
 }
 
 /* Top of the reference's depth loop (surface.h:190-200) for the hit in
-   tr.closest_hit, then its light loop's shadow rays (surface.h:9-35): one
-   per light that faces the surface, queued for the shadow stage; the pixel
-   itself is queued for the level's shade stage. Called by the active lanes
-   only; counts the level's shadow rays on the request. */
+   tr.closest_hit: the surface frame and reflection into the pixel state,
+   the number of shadow rays the light loop (surface.h:9-35) will cast (one
+   per light facing the surface; trace_level_shadows casts them), and, from
+   the second level on, the pixel queued for the level's shade stage. */
 DFN void begin_level(const KWave &w, uint32_t level, const StDScene &sc, const StDView &v, const DMat &M,
                      const DHit &hit, DPix &p, uint32_t slot) {
     p.pos = hit.position;
@@ -259,36 +242,10 @@ DFN void begin_level(const KWave &w, uint32_t level, const StDScene &sc, const S
         p.NdotRd = -inv_dot(p.nrm, p.vd);
         p.ref = reflect_with_dot(p.vd, p.nrm, p.NdotRd);
     }
-    if (level == 0) {
-        /* light mask words (all of the launch's, zero past this scene's
-           lights); the shadow stage covers every (light, slot) */
-        for (uint32_t wi = 0; wi * 32u < w.lmax; wi++) {
-            uint32_t word = 0;
-            for (uint32_t i = wi * 32u; i < sc.light_count && i < wi * 32u + 32u; i++)
-                if (v_dot(p.nrm, light_dir(sc, i, p.pos).dir) > 0) {
-                    word |= 1u << (i & 31u);
-                    p.n_shadow++;
-                }
-            ((ST_GLOBAL uint32_t *)w.mask)[(size_t)wi * w.n + slot] = word;
-        }
-    } else {
-        uint32_t *ctr = w.ctr + (size_t)level * WC_PER_LEVEL;
-        /* one reservation per wave for all its shadow rays; within it the
-           rays are light-major, so consecutive rays share a light */
-        uint32_t total = 0;
-        for (uint32_t i = 0; i < sc.light_count; i++)
-            total += (uint32_t)__builtin_popcountll(__ballot(v_dot(p.nrm, light_dir(sc, i, p.pos).dir) > 0));
-        uint32_t q = wave_reserve(ctr + WC_SHADOW, total);
-        for (uint32_t i = 0; i < sc.light_count; i++) {
-            const bool want = v_dot(p.nrm, light_dir(sc, i, p.pos).dir) > 0;
-            const unsigned long long m = __ballot(want);
-            if (want) {
-                stg2(w.shadow_q, q + lanes_below(m), slot, i);
-                p.n_shadow++;
-            }
-            q += (uint32_t)__builtin_popcountll(m);
-        }
-        const uint32_t e = wave_append(ctr + WC_SHADE, true);
+    for (uint32_t i = 0; i < sc.light_count; i++)
+        if (v_dot(p.nrm, light_dir(sc, i, p.pos).dir) > 0) p.n_shadow++;
+    if (level) {
+        const uint32_t e = wave_append(w.ctr + (size_t)level * WC_PER_LEVEL + WC_SHADE, true);
         ((ST_GLOBAL uint32_t *)w.shade_q[level & 1])[e] = slot;
     }
     pix_store_all(w, slot, p);
@@ -298,22 +255,37 @@ DFN void record_counts(const KWave &w, uint32_t slot, uint32_t n_shadow, uint32_
     stg2(w.counts, slot, n_shadow, n_bounce);
 }
 
-/* Shadow stage for one queued ray (surface.h:13-25 and trace.h:8-76 any-hit):
-   records whether the light is occluded and the last current-hit distance
-   the traversal wrote. */
-DFN void shadow_ray(const KWave &w, const StDScene &sc, DTrace &tr, uint32_t slot, uint32_t light) {
-    const float4 p4 = pl_ld(w, P_POS, slot);
-    const DLightDir L = light_dir(sc, light, f4v3(p4));
+/* One shadow ray (surface.h:13-25 and trace.h:8-76 any-hit): records
+   whether the light is occluded and the last current-hit distance the
+   traversal wrote. */
+DFN void shadow_ray(const KWave &w, const StDScene &sc, DTrace &tr, uint32_t slot, uint32_t light, sv3 pos,
+                    const DLightDir &L) {
     tr.closest_hit.position = L.pos;
     tr.closest_hit.distance_squared = L.d2;
     tr.closest_hit.distance = L.dist;
     DRay ray;
-    ray.origin = f4v3(p4);
+    ray.origin = pos;
     ray.direction = L.dir;
     const bool occluded = trace_scene(ray, tr, sc, true);
     stg2(w.shadow_res, (size_t)light * w.n + slot, occluded ? 1u : 0u, __float_as_uint(tr.current_hit.distance));
     if (occluded && light + 1 == sc.light_count)
         ((ST_GLOBAL float *)w.occ_d2)[slot] = tr.closest_hit.distance_squared;
+}
+
+/* The shadow rays of a pixel whose level just began (surface point pos,
+   normal nrm), traced by the lane that found the surface: one per light
+   that faces it, each from a fresh hit state. The traversal reads none of
+   the pixel's shading state, so the rays need no order among themselves;
+   the shade stage replays their current-hit writes in light order. */
+DFN void trace_level_shadows(const KWave &w, const StDScene &sc, uint32_t *scene_stack, uint32_t *mesh_stack,
+                             uint32_t slot, sv3 pos, sv3 nrm) {
+    for (uint32_t i = 0; i < sc.light_count; i++) {
+        const DLightDir L = light_dir(sc, i, pos);
+        if (!(v_dot(nrm, L.dir) > 0)) continue;
+        DTrace tr;
+        trace_init(tr, sc, scene_stack, mesh_stack, __uint_as_float(ST_DIST_UNSET));
+        shadow_ray(w, sc, tr, slot, i, pos, L);
+    }
 }
 
 /* Shade stage for one queued pixel (surface.h:200-243): direct light from
