@@ -7,10 +7,10 @@ noise-displaced geodesic icosphere with 200,000 fp32 triangles (frequency
 and 2 emissive quads; 1920x1080 window (1921x1081 traced QCAA grid), 1 spp,
 trace depth 4, Beauty mode. The scene is resident in HBM before timing;
 a step is the per-frame host set-up (camera basis, SSB rects, ~20 KB of
-uploads), the primary-direction kernel and the render stages (primary
-with the first level's shadow rays, then shade / bounce per depth-loop
-level, each bounce stage with the next level's shadow rays); the frame stays
-in HBM
+uploads), the primary-direction kernel and the render stages (primary,
+which also runs the first depth-loop level's shading entry, shadow rays and
+shading, then one bounce stage per further level, which runs the same for
+the new surfaces); the frame stays in HBM
 (the PCIe-inclusive rate is reported separately).
 
 Also measured in the same run:

@@ -281,12 +281,9 @@ def render_timed(scenes, views: np.ndarray, persistent: bool = True, stages: boo
 def stage_names(levels: int):
     """Names of the stage launches of a frame whose Beauty depth loop has
     `levels` levels (the order render_timed(stages=True) reports). "primary"
-    includes the first level's shadow rays and "bounce{lv}" the shadow rays
-    of level lv + 1."""
-    names = ["primary"]
-    for lv in range(levels):
-        names += [f"shade{lv}"] + ([f"bounce{lv}"] if lv + 1 < levels else [])
-    return names
+    includes the first level's shading entry, shadow rays and shading, and
+    "bounce{lv}" those of level lv + 1."""
+    return ["primary"] + [f"bounce{lv}" for lv in range(max(levels - 1, 0))]
 
 
 def render_batch_bgra8(scenes, views: np.ndarray, persistent: bool = True, outs=None):

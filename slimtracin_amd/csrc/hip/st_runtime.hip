@@ -19,10 +19,10 @@
  *                   wave id with an XCD-aware block order. A single request
  *                   is just a list with one entry, so plain and batched
  *                   launches run the same code.
- *   k_shade, k_bounce
- *                   the Beauty depth loop as queue stages, one launch per
- *                   stage and level (st_wave.h); k_bounce also traces the
- *                   next level's shadow rays.
+ *   k_bounce        the Beauty depth loop after its first level, one launch
+ *                   per level: the level's queued bounce rays, then each
+ *                   new surface's shading entry, shadow rays and shading
+ *                   (st_wave.h), as k_primary does for the first level.
  *   k_counts        per-request shadow/bounce ray totals.
  */
 #include <hip/hip_runtime.h>
@@ -55,9 +55,6 @@
 /* The shade stage at 3 waves per SIMD (168 VGPRs, a few spilled) instead of
    the 2 its 176 VGPRs allow: 0.27 ms instead of 0.35 ms for the first level
    of the 200k-triangle 1080p frame. */
-#ifndef K_SHADE_ATTR
-#define K_SHADE_ATTR __attribute__((amdgpu_waves_per_eu(3)))
-#endif
 #ifndef K_BOUNCE_ATTR
 #define K_BOUNCE_ATTR
 #endif
@@ -198,6 +195,31 @@ __device__ static inline bool tile_pixel(const StDView &v, uint32_t tile, uint32
     return x < v.width1 && y < v.height1;
 }
 
+/* Shading of one pixel at `level` (surface.h:200-243) once its shadow rays
+   are traced: direct light from the shadow results, emissive quads, glow;
+   then either its bounce ray is queued for the level's bounce stage or the
+   pixel is finished. */
+__device__ static inline __attribute__((always_inline)) void shade_slot(const KWave &w, const StDScene &sc,
+                                                                       const StDView &v, uint32_t level,
+                                                                       uint32_t slot) {
+    DPix p = pix_load(w, slot);
+    DTrace tr;
+    trace_init(tr, sc, nullptr, nullptr, p.cur_dist);
+    const bool bounce = shade_level(w, sc, tr, slot, p);
+    const uint32_t q = wave_append(w.ctr + level, bounce);
+    if (bounce) {
+        p.n_bounce++;
+        ((ST_GLOBAL uint32_t *)w.bounce_q[level & 1])[q] = slot;
+        pix_store_all(w, slot, p);
+    } else {
+        uint32_t x, y;
+        tile_pixel(v, slot >> 6, slot & 63u, x, y);
+        finish_pixel(sc, v, y * v.width1 + x, p.color, true, (p.flags & PF_LIGHTS_SHADED) != 0, p.z,
+                     p.closest_distance);
+        record_counts(w, slot, p.n_shadow, p.n_bounce);
+    }
+}
+
 /* Primary stage of one tile (raytracer.h:81-134 up to the Beauty depth
    loop): the primary ray over all primitives with the screen-space gate, the
    hit record, and then either the finished pixel (Depth/Normals/UVs,
@@ -287,7 +309,7 @@ __device__ static inline __attribute__((always_inline)) void primary_tile(const 
                     p.depth_left = v.depth;
                     p.flags = 0;
                     p.n_shadow = p.n_bounce = 0;
-                    begin_level(w, 0, sc, v, M, hit, p, tile * 64u + lane);
+                    begin_level(w, sc, M, hit, p, tile * 64u + lane);
                     entered = true;
                 }
             } else if (v.mode == ST_MODE_DEPTH) {
@@ -309,16 +331,15 @@ __device__ static inline __attribute__((always_inline)) void primary_tile(const 
         }
         if (!entered) finish_pixel(sc, v, pix, color, found, false, z, closest_distance);
     }
-    if (w.n && !entered) {
-        pix_store_idle(w, tile * 64u + lane);
-        record_counts(w, tile * 64u + lane, 0, 0);
-    }
+    if (w.n && !entered) record_counts(w, tile * 64u + lane, 0, 0);
     /* the first level's shadow rays start at the surfaces this wave just
-       found, whose nodes and triangles are still in its caches */
+       found, whose nodes and triangles are still in its caches; the
+       shading that reads their results follows in the same lane */
     if (entered) {
         const uint32_t slot = tile * 64u + lane;
         trace_level_shadows(w, sc, scene_stack, mesh_stack, slot, f4v3(pl_ld(w, P_POS, slot)),
                             f4v3(pl_ld(w, P_NRM, slot)));
+        shade_slot(w, sc, v, 0, slot);
     }
 }
 
@@ -388,61 +409,23 @@ __device__ static inline __attribute__((always_inline)) void drain_queue(const K
     }
 }
 
-__device__ static inline uint32_t queue_count(const KWave &w, uint32_t level, uint32_t which) {
-    return __builtin_amdgcn_readfirstlane(ldgu(w.ctr, (size_t)level * WC_PER_LEVEL + which));
-}
-
-/* Shade stage: lights, emissive quads and glow of the level's pixels; each
-   either queues its bounce ray or is finished. LEVEL0 (own kernel): every
-   slot, skipping those that did not enter the loop; later levels: the
-   level's shade queue. */
-template <bool LEVEL0>
-__global__ void __launch_bounds__(BLOCK_THREADS) K_SHADE_ATTR k_shade(const StDScene *__restrict__ scenes,
-                                                         const KReq *__restrict__ reqs, uint32_t nreq, KWave w,
-                                                         uint32_t level) {
-    const uint32_t count = LEVEL0 ? w.n : queue_count(w, level, WC_SHADE);
-    drain_queue(reqs, nreq, count,
-                [&](uint32_t qi, uint32_t &slot, uint32_t &) {
-                    if (!LEVEL0) {
-                        slot = ldgu(w.shade_q[level & 1], qi);
-                        return true;
-                    }
-                    slot = qi;
-                    return pix_depth_left(w, slot) != 0;
-                },
-                [&](uint32_t r, uint32_t slot, uint32_t) {
-                    const StDView v = reqs[r].v;
-                    const StDScene sc = scenes[v.scene];
-                    DPix p = pix_load(w, slot);
-                    DTrace tr;
-                    trace_init(tr, sc, nullptr, nullptr, p.cur_dist);
-                    const bool bounce = shade_level(w, sc, tr, slot, p);
-                    const uint32_t q = wave_append(w.ctr + (size_t)level * WC_PER_LEVEL + WC_BOUNCE, bounce);
-                    if (bounce) {
-                        p.n_bounce++;
-                        ((ST_GLOBAL uint32_t *)w.bounce_q)[q] = slot;
-                        pix_store_all(w, slot, p);
-                    } else {
-                        uint32_t x, y;
-                        tile_pixel(v, slot >> 6, slot & 63u, x, y);
-                        finish_pixel(sc, v, y * v.width1 + x, p.color, true, (p.flags & PF_LIGHTS_SHADED) != 0,
-                                     p.z, p.closest_distance);
-                        record_counts(w, slot, p.n_shadow, p.n_bounce);
-                    }
-                });
+/* length of level `level`'s bounce queue, read on the device */
+__device__ static inline uint32_t bounce_count(const KWave &w, uint32_t level) {
+    return __builtin_amdgcn_readfirstlane(ldgu(w.ctr, level));
 }
 
 /* Bounce stage (surface.h:221-240): closest-hit traversal of the level's
-   bounce rays; a hit that is not an emissive quad starts the next level. */
+   bounce rays; a hit that is not an emissive quad starts the next level,
+   whose shading entry, shadow rays and shading the same lane runs. */
 __global__ void __launch_bounds__(BLOCK_THREADS) K_BOUNCE_ATTR k_bounce(const StDScene *__restrict__ scenes,
                                                           const KReq *__restrict__ reqs, uint32_t nreq, KWave w,
                                                           uint32_t level, uint32_t scene_entries,
                                                           uint32_t mesh_entries) {
     ST_LDS_STACKS(scene_entries, mesh_entries);
-    const uint32_t count = queue_count(w, level, WC_BOUNCE);
+    const uint32_t count = bounce_count(w, level);
     drain_queue(reqs, nreq, count,
                 [&](uint32_t qi, uint32_t &slot, uint32_t &) {
-                    slot = ldgu(w.bounce_q, qi);
+                    slot = ldgu(w.bounce_q[level & 1], qi);
                     return true;
                 },
                 [&](uint32_t r, uint32_t slot, uint32_t) {
@@ -478,8 +461,9 @@ __global__ void __launch_bounds__(BLOCK_THREADS) K_BOUNCE_ATTR k_bounce(const St
                             p.vd = ray.direction;
                             p.material_id = hit.material_id;
                             p.hit_d2 = hit.distance_squared;
-                            begin_level(w, level + 1, sc, v, M, hit, p, slot);
+                            begin_level(w, sc, M, hit, p, slot);
                             trace_level_shadows(w, sc, scene_stack, mesh_stack, slot, p.pos, p.nrm);
+                            shade_slot(w, sc, v, level + 1, slot);
                         }
                     }
                     if (done) {
@@ -1249,8 +1233,7 @@ int st_rt_launch_prepared(StSceneHandle *const *scenes, StHostView *hv, uint32_t
     if (levels) {
         if (slots > 0xFFFFFFC0ull || slots * lmax > 0xFFFFFFC0ull) return set_err(ST_ERR_ARG, "batch too large");
         wave_bytes = round256(slots * 16 * ST_PLANES) + round256(slots * lmax * 8) + round256(slots * 4) +
-                     3 * round256(slots * 4) + round256((size_t)levels * WC_PER_LEVEL * 4) + round256(slots * 8) +
-                     7 * 256;
+                     2 * round256(slots * 4) + round256((size_t)levels * 4) + round256(slots * 8) + 6 * 256;
     }
     size_t work = up + round256(n * 24ull) + per_req_out + wave_bytes + 256;
     if ((rc = arena_reserve(&g.work, work))) return rc;
@@ -1305,10 +1288,9 @@ int st_rt_launch_prepared(StSceneHandle *const *scenes, StHostView *hv, uint32_t
         w.st = (float *)arena_take(&g.work, slots * 16 * ST_PLANES);
         w.shadow_res = (uint32_t *)arena_take(&g.work, slots * lmax * 8);
         w.occ_d2 = (float *)arena_take(&g.work, slots * 4);
-        w.shade_q[0] = (uint32_t *)arena_take(&g.work, slots * 4);
-        w.shade_q[1] = (uint32_t *)arena_take(&g.work, slots * 4);
-        w.bounce_q = (uint32_t *)arena_take(&g.work, slots * 4);
-        w.ctr = (uint32_t *)arena_take(&g.work, (size_t)levels * WC_PER_LEVEL * 4);
+        w.bounce_q[0] = (uint32_t *)arena_take(&g.work, slots * 4);
+        w.bounce_q[1] = (uint32_t *)arena_take(&g.work, slots * 4);
+        w.ctr = (uint32_t *)arena_take(&g.work, (size_t)levels * 4);
         w.counts = (uint32_t *)arena_take(&g.work, slots * 8);
     }
     /* launch shape, fixed before anything is enqueued so that a captured
@@ -1335,15 +1317,13 @@ int st_rt_launch_prepared(StSceneHandle *const *scenes, StHostView *hv, uint32_t
        the device at once: they take queue chunks in a fixed interleaved
        order, so a wave that had to wait for a slot would run its whole
        share late */
-    uint32_t b_shade0 = 0, b_shade = 0, b_bounce = 0;
+    uint32_t b_bounce = 0;
     size_t lds_primary = lds;
     /* one wave per tile: at most blocks_per_cu of its blocks per CU (fewer
        tiles in flight share the caches better) */
     if (!persistent && g.tune.blocks_per_cu < 4)
         (void)capped_blocks(reinterpret_cast<const void *>(k_primary), lds, g.tune.blocks_per_cu, &lds_primary);
     if (levels) {
-        b_shade0 = resident_blocks(reinterpret_cast<const void *>(k_shade<true>), 0);
-        b_shade = resident_blocks(reinterpret_cast<const void *>(k_shade<false>), 0);
         b_bounce = resident_blocks(reinterpret_cast<const void *>(k_bounce), lds);
     }
 
@@ -1367,8 +1347,7 @@ int st_rt_launch_prepared(StSceneHandle *const *scenes, StHostView *hv, uint32_t
         if (any_device_ssb)
             hipLaunchKernelGGL(k_ssb, dim3(n), dim3(256), 0, g.stream, (const StDScene *)sc_d, (const KReq *)reqs_d);
         if (levels)
-            hipLaunchKernelGGL(k_zero_u32, dim3((levels * WC_PER_LEVEL + 255) / 256), dim3(256), 0, g.stream, w.ctr,
-                               levels * WC_PER_LEVEL);
+            hipLaunchKernelGGL(k_zero_u32, dim3((levels + 255) / 256), dim3(256), 0, g.stream, w.ctr, levels);
         hipLaunchKernelGGL(k_primary_dirs, dim3((rows + DIRS_ROWS - 1) / DIRS_ROWS), dim3(64), 0, g.stream, reqs_d,
                            n, rows);
         HIP_TRY(hipGetLastError());
@@ -1379,21 +1358,12 @@ int st_rt_launch_prepared(StSceneHandle *const *scenes, StHostView *hv, uint32_t
                            s_entries, m_entries, w);
         HIP_TRY(hipGetLastError());
         HIP_TRY(mark());
-        /* the depth loop: the queue kernels read their queue lengths on the
-           device, so the sequence needs no host round trip */
-        for (uint32_t lv = 0; lv < levels; lv++) {
-            if (lv == 0)
-                hipLaunchKernelGGL(k_shade<true>, dim3(b_shade0), dim3(BLOCK_THREADS), 0, g.stream,
-                                   (const StDScene *)sc_d, (const KReq *)reqs_d, n, w, lv);
-            else
-                hipLaunchKernelGGL(k_shade<false>, dim3(b_shade), dim3(BLOCK_THREADS), 0, g.stream,
-                                   (const StDScene *)sc_d, (const KReq *)reqs_d, n, w, lv);
+        /* the rest of the depth loop: each bounce stage reads its queue
+           length on the device, so the sequence needs no host round trip */
+        for (uint32_t lv = 0; lv + 1 < levels; lv++) {
+            hipLaunchKernelGGL(k_bounce, dim3(b_bounce), dim3(BLOCK_THREADS), lds, g.stream,
+                               (const StDScene *)sc_d, (const KReq *)reqs_d, n, w, lv, s_entries, m_entries);
             HIP_TRY(mark());
-            if (lv + 1 < levels) {
-                hipLaunchKernelGGL(k_bounce, dim3(b_bounce), dim3(BLOCK_THREADS), lds, g.stream,
-                                   (const StDScene *)sc_d, (const KReq *)reqs_d, n, w, lv, s_entries, m_entries);
-                HIP_TRY(mark());
-            }
         }
         if (levels && count_blocks)
             hipLaunchKernelGGL(k_counts, dim3(count_blocks), dim3(256), 0, g.stream, (const KReq *)reqs_d, n,
@@ -1411,7 +1381,7 @@ int st_rt_launch_prepared(StSceneHandle *const *scenes, StHostView *hv, uint32_t
            graph does exactly this launch's work (the per-frame data is in
            the upload block, copied from the same pinned staging) */
         std::vector<uint64_t> sig = {n, up, rows, tiles, levels, lmax, count_blocks, blocks, xcd_blocks, any_device_ssb,
-                                     (uint64_t)persistent, s_entries, m_entries, b_shade0, b_shade, b_bounce,
+                                     (uint64_t)persistent, s_entries, m_entries, b_bounce,
                                      lds_primary, (uint64_t)(uintptr_t)blk,
                                      (uint64_t)(uintptr_t)st,
                                      (uint64_t)(uintptr_t)cnt_d, (uint64_t)(uintptr_t)w.st, w.n};

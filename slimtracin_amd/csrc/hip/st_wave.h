@@ -47,16 +47,11 @@ struct KWave {
     uint32_t lmax;         /* planes of shadow_res (largest light count) */
     uint32_t *shadow_res;  /* [light][slot] 2 words: occluded, current-hit distance bits */
     float *occ_d2;         /* [slot] occluder's squared distance, last light's shadow ray */
-    uint32_t *shade_q[2];  /* pixels entering a level's shading (ping-pong by level) */
-    uint32_t *bounce_q;    /* pixels with a bounce ray */
-    uint32_t *ctr;         /* WC_PER_LEVEL queue lengths per level */
+    uint32_t *bounce_q[2]; /* pixels with a bounce ray (ping-pong by level) */
+    uint32_t *ctr;         /* bounce-queue length per level */
     uint32_t *counts;      /* [slot] 2 words: shadow rays, bounce rays of the pixel */
 };
 
-/* Queue lengths per level. Level 0 has no shade queue: its shade stage runs
-   over every slot (depth_left = 0 marks the pixels that did not enter the
-   loop). */
-enum { WC_SHADE = 0, WC_BOUNCE = 1, WC_PER_LEVEL = 2 };
 
 /* state planes */
 enum { P_COLOR, P_THRU, P_VO, P_VD, P_POS, P_NRM, P_REF, P_ALB, P_BDIR, P_MISC, ST_PLANES };
@@ -117,11 +112,6 @@ DFN void pix_store_all(const KWave &w, uint32_t slot, const DPix &p) {
     stgi4(w.st, (size_t)P_MISC * w.n + slot, (int)p.material_id, (int)p.depth_left, (int)(p.flags | (p.n_bounce << 8)),
           (int)p.n_shadow);
 }
-/* a slot that does not enter the depth loop: no level-0 shading, no shadow rays */
-DFN void pix_store_idle(const KWave &w, uint32_t slot) {
-    stgi4(w.st, (size_t)P_MISC * w.n + slot, 0, 0, 0, 0);
-}
-DFN uint32_t pix_depth_left(const KWave &w, uint32_t slot) { return ldgu((const uint32_t *)w.st, 4 * ((size_t)P_MISC * w.n + slot) + 1); }
 
 /* ---- wave-level queue helpers ------------------------------------------ */
 
@@ -228,12 +218,10 @@ DFN void finish_pixel(const StDScene &sc, const StDView &v, uint32_t pix, sv3 co
 }
 
 /* Top of the reference's depth loop (surface.h:190-200) for the hit in
-   tr.closest_hit: the surface frame and reflection into the pixel state,
+   tr.closest_hit: the surface frame and reflection into the pixel state and
    the number of shadow rays the light loop (surface.h:9-35) will cast (one
-   per light facing the surface; trace_level_shadows casts them), and, from
-   the second level on, the pixel queued for the level's shade stage. */
-DFN void begin_level(const KWave &w, uint32_t level, const StDScene &sc, const StDView &v, const DMat &M,
-                     const DHit &hit, DPix &p, uint32_t slot) {
+   per light facing the surface; trace_level_shadows casts them). */
+DFN void begin_level(const KWave &w, const StDScene &sc, const DMat &M, const DHit &hit, DPix &p, uint32_t slot) {
     p.pos = hit.position;
     p.nrm = hit.from_behind ? v_neg(hit.normal) : hit.normal;
     p.ior = hit.from_behind ? M.n2_over_n1 : M.n1_over_n2;
@@ -244,10 +232,6 @@ DFN void begin_level(const KWave &w, uint32_t level, const StDScene &sc, const S
     }
     for (uint32_t i = 0; i < sc.light_count; i++)
         if (v_dot(p.nrm, light_dir(sc, i, p.pos).dir) > 0) p.n_shadow++;
-    if (level) {
-        const uint32_t e = wave_append(w.ctr + (size_t)level * WC_PER_LEVEL + WC_SHADE, true);
-        ((ST_GLOBAL uint32_t *)w.shade_q[level & 1])[e] = slot;
-    }
     pix_store_all(w, slot, p);
 }
 
