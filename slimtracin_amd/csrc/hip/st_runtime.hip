@@ -195,14 +195,18 @@ __device__ static inline bool tile_pixel(const StDView &v, uint32_t tile, uint32
     return x < v.width1 && y < v.height1;
 }
 
-/* Shading of one pixel at `level` (surface.h:200-243) once its shadow rays
-   are traced: direct light from the shadow results, emissive quads, glow;
-   then either its bounce ray is queued for the level's bounce stage or the
-   pixel is finished. */
-__device__ static inline __attribute__((always_inline)) void shade_slot(const KWave &w, const StDScene &sc,
-                                                                       const StDView &v, uint32_t level,
-                                                                       uint32_t slot) {
-    DPix p = pix_load(w, slot);
+/* A pixel's depth-loop level (surface.h:190-243) from its surface hit
+   `hit`: shading entry, the light loop's shadow rays, then the
+   shading from their results (direct light, emissive quads, glow); either
+   the pixel's state is stored and its bounce ray queued for the level's
+   bounce stage, or the pixel is finished. */
+__device__ static inline __attribute__((always_inline)) void run_level(const KWave &w, const StDScene &sc,
+                                                                      const StDView &v, uint32_t level,
+                                                                      uint32_t slot, const DMat &M, const DHit &hit,
+                                                                      DPix &p, uint32_t *scene_stack,
+                                                                      uint32_t *mesh_stack) {
+    begin_level(sc, M, hit, p);
+    trace_level_shadows(w, sc, scene_stack, mesh_stack, slot, p.pos, p.nrm);
     DTrace tr;
     trace_init(tr, sc, nullptr, nullptr, p.cur_dist);
     const bool bounce = shade_level(w, sc, tr, slot, p);
@@ -309,7 +313,7 @@ __device__ static inline __attribute__((always_inline)) void primary_tile(const 
                     p.depth_left = v.depth;
                     p.flags = 0;
                     p.n_shadow = p.n_bounce = 0;
-                    begin_level(w, sc, M, hit, p, tile * 64u + lane);
+                    run_level(w, sc, v, 0, tile * 64u + lane, M, hit, p, scene_stack, mesh_stack);
                     entered = true;
                 }
             } else if (v.mode == ST_MODE_DEPTH) {
@@ -332,15 +336,6 @@ __device__ static inline __attribute__((always_inline)) void primary_tile(const 
         if (!entered) finish_pixel(sc, v, pix, color, found, false, z, closest_distance);
     }
     if (w.n && !entered) record_counts(w, tile * 64u + lane, 0, 0);
-    /* the first level's shadow rays start at the surfaces this wave just
-       found, whose nodes and triangles are still in its caches; the
-       shading that reads their results follows in the same lane */
-    if (entered) {
-        const uint32_t slot = tile * 64u + lane;
-        trace_level_shadows(w, sc, scene_stack, mesh_stack, slot, f4v3(pl_ld(w, P_POS, slot)),
-                            f4v3(pl_ld(w, P_NRM, slot)));
-        shade_slot(w, sc, v, 0, slot);
-    }
 }
 
 /* Traversal stacks live in LDS, one column per lane (no bank conflicts):
@@ -461,9 +456,7 @@ __global__ void __launch_bounds__(BLOCK_THREADS) K_BOUNCE_ATTR k_bounce(const St
                             p.vd = ray.direction;
                             p.material_id = hit.material_id;
                             p.hit_d2 = hit.distance_squared;
-                            begin_level(w, sc, M, hit, p, slot);
-                            trace_level_shadows(w, sc, scene_stack, mesh_stack, slot, p.pos, p.nrm);
-                            shade_slot(w, sc, v, level + 1, slot);
+                            run_level(w, sc, v, level + 1, slot, M, hit, p, scene_stack, mesh_stack);
                         }
                     }
                     if (done) {

@@ -220,8 +220,9 @@ DFN void finish_pixel(const StDScene &sc, const StDView &v, uint32_t pix, sv3 co
 /* Top of the reference's depth loop (surface.h:190-200) for the hit in
    tr.closest_hit: the surface frame and reflection into the pixel state and
    the number of shadow rays the light loop (surface.h:9-35) will cast (one
-   per light facing the surface; trace_level_shadows casts them). */
-DFN void begin_level(const KWave &w, const StDScene &sc, const DMat &M, const DHit &hit, DPix &p, uint32_t slot) {
+   per light facing the surface; trace_level_shadows casts them). The state
+   stays in registers: the same lane traces the shadow rays and shades. */
+DFN void begin_level(const StDScene &sc, const DMat &M, const DHit &hit, DPix &p) {
     p.pos = hit.position;
     p.nrm = hit.from_behind ? v_neg(hit.normal) : hit.normal;
     p.ior = hit.from_behind ? M.n2_over_n1 : M.n1_over_n2;
@@ -232,7 +233,6 @@ DFN void begin_level(const KWave &w, const StDScene &sc, const DMat &M, const DH
     }
     for (uint32_t i = 0; i < sc.light_count; i++)
         if (v_dot(p.nrm, light_dir(sc, i, p.pos).dir) > 0) p.n_shadow++;
-    pix_store_all(w, slot, p);
 }
 
 DFN void record_counts(const KWave &w, uint32_t slot, uint32_t n_shadow, uint32_t n_bounce) {
