@@ -52,9 +52,7 @@
 #ifndef K_PRIMARY_ATTR
 #define K_PRIMARY_ATTR
 #endif
-/* The shade stage at 3 waves per SIMD (168 VGPRs, a few spilled) instead of
-   the 2 its 176 VGPRs allow: 0.27 ms instead of 0.35 ms for the first level
-   of the 200k-triangle 1080p frame. */
+/* Launch attributes of the bounce stage (occupancy experiments). */
 #ifndef K_BOUNCE_ATTR
 #define K_BOUNCE_ATTR
 #endif

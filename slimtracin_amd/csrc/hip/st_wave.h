@@ -31,9 +31,9 @@
  * carried explicitly, and so is the one field of the reference's
  * `current_hit` that flows from one traversal into a later one (its
  * `distance`, which a box hit leaves unwritten and copies into the closest
- * hit). A shadow ray's traversal does not depend on that value, so shadow
- * rays of one pixel run in parallel and the shade stage replays their
- * writes to it in light order.
+ * hit). A shadow ray's traversal does not depend on that value, so each
+ * shadow ray of a pixel starts from a fresh hit state and the shading
+ * replays their writes to it in light order.
  */
 #ifndef ST_WAVE_H
 #define ST_WAVE_H
@@ -260,7 +260,7 @@ DFN void shadow_ray(const KWave &w, const StDScene &sc, DTrace &tr, uint32_t slo
    normal nrm), traced by the lane that found the surface: one per light
    that faces it, each from a fresh hit state. The traversal reads none of
    the pixel's shading state, so the rays need no order among themselves;
-   the shade stage replays their current-hit writes in light order. */
+   the shading replays their current-hit writes in light order. */
 DFN void trace_level_shadows(const KWave &w, const StDScene &sc, uint32_t *scene_stack, uint32_t *mesh_stack,
                              uint32_t slot, sv3 pos, sv3 nrm) {
     for (uint32_t i = 0; i < sc.light_count; i++) {
