@@ -113,66 +113,29 @@ __device__ static inline uint32_t find_req(const KReq *reqs, uint32_t n, uint32_
     return lo;
 }
 
-/* One wave per DIRS_ROWS rows, one lane per row. The sum along a row is
-   sequential, so each row lane walks DIRS_STEP pixels at a time into an
-   LDS tile, and then the whole wave stores the tile one row at a time, 64
-   consecutive pixels per store (coalesced), instead of one scattered store
-   per row and pixel. */
-#define DIRS_ROWS 16u
-#define DIRS_STEP 64u
+/* One lane per traced row: walks the row's sequential `current += right`
+   sum and stores the value at every ST_DIR_MARK-th pixel. A pixel's
+   direction is then its checkpoint plus at most ST_DIR_MARK - 1 further
+   additions, the same fp32 operations in the same order (primary_dir). */
 __global__ void __launch_bounds__(64) k_primary_dirs(const KReq *reqs, uint32_t nreq, uint32_t total_rows) {
-    __shared__ float4 tile[DIRS_ROWS][DIRS_STEP + 1];
-    __shared__ uint32_t row_w[DIRS_ROWS];
-    __shared__ float4 *row_out[DIRS_ROWS];
-    const uint32_t lane = threadIdx.x;
-    const uint32_t row = blockIdx.x * DIRS_ROWS + lane;
-    uint32_t width = 0;
-    float4 *out = nullptr;
-    sv3 current = V3s(0), right = V3s(0);
-    if (lane < DIRS_ROWS && row < total_rows) {
-        const uint32_t r = find_req(reqs, nreq, row, true);
-        const StDView &v = reqs[r].v;
-        const uint32_t y = row - reqs[r].row_base;
-        if (y < v.height1) {
-            const float4 rs = ldg4(v.row_starts, y);
-            current = V3(rs.x, rs.y, rs.z);
-            right = V3(v.right[0], v.right[1], v.right[2]);
-            width = v.width1;
-            out = reinterpret_cast<float4 *>(v.dirs) + (size_t)y * v.width1;
-        }
-    }
-    if (lane < DIRS_ROWS) {
-        row_w[lane] = width;
-        row_out[lane] = out;
-    }
-    uint32_t max_width = width;
-    for (int off = 32; off > 0; off >>= 1) max_width = max(max_width, (uint32_t)__shfl_xor((int)max_width, off, 64));
-    __syncthreads();
-    uint32_t rw[DIRS_ROWS];
-    float4 *ro[DIRS_ROWS];
-    for (uint32_t j = 0; j < DIRS_ROWS; j++) {
-        rw[j] = row_w[j];
-        ro[j] = row_out[j];
-    }
-    for (uint32_t x0 = 0; x0 < max_width; x0 += DIRS_STEP) {
-        /* sums past the row's end are computed but never stored */
-        if (lane < DIRS_ROWS) {
+    const uint32_t row = blockIdx.x * 64u + threadIdx.x;
+    if (row >= total_rows) return;
+    const uint32_t r = find_req(reqs, nreq, row, true);
+    const StDView &v = reqs[r].v;
+    const uint32_t y = row - reqs[r].row_base;
+    if (y >= v.height1) return;
+    const float4 rs = ldg4(v.row_starts, y);
+    sv3 current = V3(rs.x, rs.y, rs.z);
+    const sv3 right = V3(v.right[0], v.right[1], v.right[2]);
+    const uint32_t marks = ST_DIR_MARKS(v.width1);
+    ST_GLOBAL st_vf4 *out = (ST_GLOBAL st_vf4 *)v.dir_marks + (size_t)y * marks;
+    for (uint32_t m = 0; m < marks; m++) {
+        st_vf4 val = {current.x, current.y, current.z, 0.0f};
+        out[m] = val;
+        if (m + 1 < marks) {
 #pragma unroll
-            for (uint32_t k = 0; k < DIRS_STEP; k++) {
-                tile[lane][k] = make_float4(current.x, current.y, current.z, 0.0f);
-                current = v_add(current, right);
-            }
+            for (uint32_t k = 0; k < ST_DIR_MARK; k++) current = v_add(current, right);
         }
-        __syncthreads();
-#pragma unroll
-        for (uint32_t j = 0; j < DIRS_ROWS; j++) {
-            if (x0 + lane < rw[j]) {
-                const float4 t = tile[j][lane];
-                st_vf4 val = {t.x, t.y, t.z, t.w};
-                ((ST_GLOBAL st_vf4 *)ro[j])[x0 + lane] = val;
-            }
-        }
-        __syncthreads();
     }
 }
 
@@ -241,13 +204,13 @@ __device__ static inline __attribute__((always_inline)) void primary_tile(const 
        pixel 0's direction and no rect admits them), so that the whole wave
        takes part in culling the primitives against the tile. */
     const uint32_t pix = valid ? y * v.width1 + x : 0u;
-    const float4 d4 = ldg4(v.dirs, pix);
+    const sv3 cur = primary_dir(v, valid ? x : 0u, valid ? y : 0u);
     DTrace tr;
     trace_init(tr, sc, scene_stack, mesh_stack, 0.0f);
     DRay ray;
     ray.origin = V3(v.cam_pos[0], v.cam_pos[1], v.cam_pos[2]);
     /* dir = normVec3(current) (render/raytracer.h:159) */
-    ray.direction = v_norm(V3(d4.x, d4.y, d4.z));
+    ray.direction = v_norm(cur);
     ray.direction_reciprocal = v_rcp(ray.direction);
     DHit &hit = tr.closest_hit;
     hit.distance = hit.distance_squared = INFINITY;
@@ -1188,7 +1151,8 @@ int st_rt_launch_prepared(StSceneHandle *const *scenes, StHostView *hv, uint32_t
         if (k == nuniq) uniq[nuniq++] = s;
         scene_idx[i] = k;
         size_t px = (size_t)hv[i].dv.width1 * hv[i].dv.height1;
-        per_req_out += round256(px * 16) + round256(px * 24) + (want_hits ? round256(px * 16) : 0);
+        per_req_out += round256((size_t)hv[i].dv.height1 * ST_DIR_MARKS(hv[i].dv.width1) * 16) + round256(px * 24) +
+                       (want_hits ? round256(px * 16) : 0);
         if (bgra_host && bgra_host[i]) {
             if (hv[i].dv.width1 < 2 || hv[i].dv.height1 < 2) return set_err(ST_ERR_ARG, "request %u: empty window", i);
             per_req_out += round256((size_t)(hv[i].dv.width1 - 1) * (hv[i].dv.height1 - 1) * 4);
@@ -1247,7 +1211,7 @@ int st_rt_launch_prepared(StSceneHandle *const *scenes, StHostView *hv, uint32_t
         if (!hv[i].device_ssb) memcpy(st + rect_off[i], hv[i].rects, P * 16ull);
         dv.row_starts = (const float *)(blk + rs_off[i]);
         dv.rects = (const int32_t *)(blk + rect_off[i]);
-        dv.dirs = (float *)arena_take(&g.work, px * 16);
+        dv.dir_marks = (float *)arena_take(&g.work, (size_t)dv.height1 * ST_DIR_MARKS(dv.width1) * 16);
         dv.pixels = arena_take(&g.work, px * 24);
         dv.hits = want_hits ? arena_take(&g.work, px * 16) : NULL;
         dv.counters = cnt_d + 3 * i;
@@ -1339,8 +1303,7 @@ int st_rt_launch_prepared(StSceneHandle *const *scenes, StHostView *hv, uint32_t
             hipLaunchKernelGGL(k_ssb, dim3(n), dim3(256), 0, g.stream, (const StDScene *)sc_d, (const KReq *)reqs_d);
         if (levels)
             hipLaunchKernelGGL(k_zero_u32, dim3((levels + 255) / 256), dim3(256), 0, g.stream, w.ctr, levels);
-        hipLaunchKernelGGL(k_primary_dirs, dim3((rows + DIRS_ROWS - 1) / DIRS_ROWS), dim3(64), 0, g.stream, reqs_d,
-                           n, rows);
+        hipLaunchKernelGGL(k_primary_dirs, dim3((rows + 63) / 64), dim3(64), 0, g.stream, reqs_d, n, rows);
         HIP_TRY(hipGetLastError());
         if (!graph) HIP_TRY(hipEventRecord(g.ev[0], g.stream));
         HIP_TRY(mark());
@@ -1379,7 +1342,7 @@ int st_rt_launch_prepared(StSceneHandle *const *scenes, StHostView *hv, uint32_t
         for (uint32_t i = 0; i < n; i++) {
             const StDView &dv = reqs_h[i].v;
             sig.insert(sig.end(), {dv.width1, dv.height1, dv.mode, dv.depth, dv.tile_shift,
-                                   (uint64_t)(uintptr_t)dv.dirs, (uint64_t)(uintptr_t)dv.pixels,
+                                   (uint64_t)(uintptr_t)dv.dir_marks, (uint64_t)(uintptr_t)dv.pixels,
                                    (uint64_t)(uintptr_t)dv.hits, (uint64_t)(uintptr_t)reqs_h[i].bgra});
         }
         if (!g.graph_exec || sig != g.graph_sig) {

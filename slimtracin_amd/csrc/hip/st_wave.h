@@ -178,14 +178,24 @@ DFN void trace_init(DTrace &tr, const StDScene &sc, uint32_t *scene_stack, uint3
     tr.mesh_stack = mesh_stack;
 }
 
+/* The unnormalised primary direction `current` of traced pixel (x, y): the
+   row's sequential sum (render/raytracer.h:156-170) continued from its
+   checkpoint at the previous multiple of ST_DIR_MARK. */
+DFN sv3 primary_dir(const StDView &v, uint32_t x, uint32_t y) {
+    const float4 m = ldg4(v.dir_marks, (size_t)y * ST_DIR_MARKS(v.width1) + x / ST_DIR_MARK);
+    sv3 current = V3(m.x, m.y, m.z);
+    const sv3 right = V3(v.right[0], v.right[1], v.right[2]);
+    for (uint32_t k = x % ST_DIR_MARK; k; k--) current = v_add(current, right);
+    return current;
+}
+
 /* The window pixel's final value (raytracer.h:105-134): light glow along
    the camera ray when no shading pass hit a light, tone mapping, and the
    squared 0-255 colour with the camera-space depth. */
 DFN void finish_pixel(const StDScene &sc, const StDView &v, uint32_t pix, sv3 color, bool hit_found,
                       bool lights_shaded, float z, float closest_distance) {
     if (v.mode == ST_MODE_BEAUTY && sc.light_count && !lights_shaded) {
-        const float4 d4 = ldg4(v.dirs, pix);
-        const sv3 Rd = v_norm(V3(d4.x, d4.y, d4.z));
+        const sv3 Rd = v_norm(primary_dir(v, pix % v.width1, pix / v.width1));
         const sv3 Ro = V3(v.cam_pos[0], v.cam_pos[1], v.cam_pos[2]);
         DSphereHit sh = {};
         if (shade_light_glow(sc, Ro, Rd, closest_distance, sh, color)) {

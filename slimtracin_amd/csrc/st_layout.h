@@ -19,6 +19,10 @@
 #define ST_SCENE_STACK_MAX 64   /* scene BVH traversal stack entries per ray */
 #define ST_MESH_STACK_MAX 64    /* mesh BVH traversal stack entries per ray */
 
+/* Checkpoint spacing of the primary directions along a row (pixels). */
+#define ST_DIR_MARK 8u
+#define ST_DIR_MARKS(width1) (((width1) + ST_DIR_MARK - 1u) / ST_DIR_MARK)
+
 typedef struct StDPrim {
     float rot[4];          /* axis xyz, amount */
     float pos[3];
@@ -110,7 +114,8 @@ typedef struct StDView {
     uint32_t tiles_x;              /* tiles per row */
     uint32_t tile_shift;           /* tile width = 1 << tile_shift lanes, height = 64 >> tile_shift */
     const float *row_starts;       /* 4 floats per row: sequential start vectors */
-    float *dirs;                   /* 4 floats per pixel: the sequential `current` vectors */
+    float *dir_marks;              /* 4 floats per ST_DIR_MARK pixels of a row: the row's sequential
+                                      `current` vector at x = 0, 8, 16, ... (rows of ST_DIR_MARKS(width1)) */
     const int32_t *rects;          /* 4 ints per primitive: SSB min.x, min.y, max.x, max.y */
     void *pixels;                  /* StFloatPixel per pixel */
     void *hits;                    /* StHitRec per pixel, or NULL */
