@@ -5,22 +5,21 @@
  * The reference traces one pixel start to finish (render/raytracer.h:81-134,
  * closest_hit/surface.h:146-255): primary ray, then a depth loop that shades
  * direct lights with one shadow ray each, emissive quads, light glow, and a
- * reflection/refraction bounce ray. Run as one GPU thread per pixel, that
- * loop keeps the shading state of a pixel live across three different
- * traversals and needs far more registers than a lane has. Here the loop is
- * cut where the shading state is largest, and each stage runs over a
- * compacted queue:
+ * reflection/refraction bounce ray. Run as one GPU thread per pixel start to
+ * finish, the lanes of a wave whose pixels stop early would idle while the
+ * others bounce. Here the loop is cut at each bounce: one level of it
+ * (shading entry, the shadow rays, the shading from their results) runs in
+ * the lane that found the level's surface, with the pixel's state in
+ * registers, and the pixels that continue are compacted into a queue for
+ * the next level's kernel:
  *
- *   primary   one wave per 8x8 tile: primary ray, Depth/Normals/UVs output,
- *             Beauty shading entry, then the first level's shadow rays,
- *             traced at once by the same wave (their origins are the hit
- *             points just found, so the tile's nodes are still in cache)
- *   shade     per queued pixel: direct light from the shadow results,
- *             emissive quads, glow, and the bounce decision
- *   bounce    closest-hit traversal of every queued bounce ray, then the
- *             next level's shading entry and its shadow rays
+ *   primary   one wave per tile: primary ray, Depth/Normals/UVs output,
+ *             and for Beauty the first level (the shadow rays start at the
+ *             hit points just found, so the tile's nodes are still in cache)
+ *   bounce    one launch per further level: closest-hit traversal of every
+ *             queued bounce ray, then the next level in the same lane
  *
- * Between stages a pixel's state lives in HBM as planes of float4 (one plane
+ * Between levels a pixel's state lives in HBM as planes of float4 (one plane
  * per four state values, slot = tile * 64 + lane), so that every state read
  * and write of a wave is one contiguous 1 KiB access.
  *
@@ -282,8 +281,8 @@ DFN void trace_level_shadows(const KWave &w, const StDScene &sc, uint32_t *scene
     }
 }
 
-/* Shade stage for one queued pixel (surface.h:200-243): direct light from
-   the shadow results, emissive quads and glow. Returns true when the pixel
+/* The shading of one pixel's level (surface.h:200-243): direct light from
+   the level's shadow results, emissive quads and glow. Returns true when the pixel
    continues with a bounce ray (direction in p.bdir), false when it is done. */
 DFN bool shade_level(const KWave &w, const StDScene &sc, DTrace &tr, uint32_t slot, DPix &p) {
     const DMat M = load_mat(sc, p.material_id);
