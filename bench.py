@@ -9,8 +9,8 @@ trace depth 4, Beauty mode. The scene is resident in HBM before timing;
 a step is the per-frame host set-up (camera basis, SSB rects, ~20 KB of
 uploads), the primary-direction kernel and the render stages (primary,
 which also runs the first depth-loop level's shading entry, shadow rays and
-shading, then one bounce stage per further level, which runs the same for
-the new surfaces); the frame stays in HBM
+shading, then the bounce stage, which traces each bounce ray and runs its
+pixel's further levels in the same lane); the frame stays in HBM
 (the PCIe-inclusive rate is reported separately).
 
 Also measured in the same run:

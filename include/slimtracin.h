@@ -161,10 +161,9 @@ int st_render_timed(StSceneHandle *const *scenes, const StView *views, uint32_t 
 /* Per-stage durations (ms, HIP events on the launch stream) of the last
    st_render_timed call made with ST_BATCH_STAGE_TIMES, in launch order: the
    primary stage (with the first level of the Beauty depth loop: shading
-   entry, shadow rays, shading), then one bounce stage per further level
-   (the bounce rays, then that level's entry, shadow rays and shading).
-   *count = number of
-   stages; up to cap values are written. Not in the reference (bench
+   entry, shadow rays, shading), then, for a depth above 1, the bounce
+   stage (every bounce ray and every further level's entry, shadow rays and
+   shading). *count = number of stages; up to cap values are written. Not in the reference (bench
    instrumentation). */
 int st_last_stage_ms(float *ms, uint32_t cap, uint32_t *count);
 

@@ -282,8 +282,8 @@ def stage_names(levels: int):
     """Names of the stage launches of a frame whose Beauty depth loop has
     `levels` levels (the order render_timed(stages=True) reports). "primary"
     includes the first level's shading entry, shadow rays and shading, and
-    "bounce{lv}" those of level lv + 1."""
-    return ["primary"] + [f"bounce{lv}" for lv in range(max(levels - 1, 0))]
+    "bounce" every bounce ray and every further level."""
+    return ["primary"] + (["bounce"] if levels > 1 else [])
 
 
 def render_batch_bgra8(scenes, views: np.ndarray, persistent: bool = True, outs=None):

@@ -9,9 +9,9 @@
  *                   unnormalised `current` vector (the primary stage
  *                   normalises it). Rows of every request of a launch are
  *                   flattened into one grid.
- *   k_primary       one wavefront per 8x8 pixel tile: primary ray, the
- *                   Beauty shading entry and the first level's shadow rays
- *                   (st_wave.h). Tiles of all requests
+ *   k_primary       one wavefront per pixel tile: primary ray, and for
+ *                   Beauty the first level of the depth loop (shading
+ *                   entry, shadow rays, shading; st_wave.h). Tiles of all requests
  *                   of a launch form one list; in persistent mode each
  *                   wavefront keeps pulling the next tile from a global
  *                   counter until the list is exhausted (the loop's exit
@@ -19,10 +19,10 @@
  *                   wave id with an XCD-aware block order. A single request
  *                   is just a list with one entry, so plain and batched
  *                   launches run the same code.
- *   k_bounce        the Beauty depth loop after its first level, one launch
- *                   per level: the level's queued bounce rays, then each
- *                   new surface's shading entry, shadow rays and shading
- *                   (st_wave.h), as k_primary does for the first level.
+ *   k_bounce        the Beauty depth loop after its first level, one launch:
+ *                   per queued pixel, in one lane, its bounce ray, then the
+ *                   new surface's level as k_primary runs the first, and so
+ *                   on while the pixel bounces.
  *   k_counts        per-request shadow/bounce ray totals.
  */
 #include <hip/hip_runtime.h>
@@ -156,13 +156,11 @@ __device__ static inline bool tile_pixel(const StDView &v, uint32_t tile, uint32
     return x < v.width1 && y < v.height1;
 }
 
-/* A pixel's depth-loop level (surface.h:190-243) from its surface hit
-   `hit`: shading entry, the light loop's shadow rays, then the
-   shading from their results (direct light, emissive quads, glow); either
-   the pixel's state is stored and its bounce ray queued for the level's
-   bounce stage, or the pixel is finished. */
-__device__ static inline __attribute__((always_inline)) void run_level(const KWave &w, const StDScene &sc,
-                                                                      const StDView &v, uint32_t level,
+/* One depth-loop level of a pixel (surface.h:190-243) from its surface hit
+   `hit`: shading entry, the light loop's shadow rays, then the shading from
+   their results (direct light, emissive quads, glow). Returns true when the
+   pixel continues with a bounce ray (direction in p.bdir). */
+__device__ static inline __attribute__((always_inline)) bool run_level(const KWave &w, const StDScene &sc,
                                                                       uint32_t slot, const DMat &M, const DHit &hit,
                                                                       DPix &p, uint32_t *scene_stack,
                                                                       uint32_t *mesh_stack) {
@@ -171,18 +169,18 @@ __device__ static inline __attribute__((always_inline)) void run_level(const KWa
     DTrace tr;
     trace_init(tr, sc, nullptr, nullptr, p.cur_dist);
     const bool bounce = shade_level(w, sc, tr, slot, p);
-    const uint32_t q = wave_append(w.ctr + level, bounce);
-    if (bounce) {
-        p.n_bounce++;
-        ((ST_GLOBAL uint32_t *)w.bounce_q[level & 1])[q] = slot;
-        pix_store_all(w, slot, p);
-    } else {
-        uint32_t x, y;
-        tile_pixel(v, slot >> 6, slot & 63u, x, y);
-        finish_pixel(sc, v, y * v.width1 + x, p.color, true, (p.flags & PF_LIGHTS_SHADED) != 0, p.z,
-                     p.closest_distance);
-        record_counts(w, slot, p.n_shadow, p.n_bounce);
-    }
+    if (bounce) p.n_bounce++;
+    return bounce;
+}
+
+/* A pixel whose depth loop has ended: its frame value and ray counts. */
+__device__ static inline void end_pixel(const KWave &w, const StDScene &sc, const StDView &v, uint32_t slot,
+                                        const DPix &p) {
+    uint32_t x, y;
+    tile_pixel(v, slot >> 6, slot & 63u, x, y);
+    finish_pixel(sc, v, y * v.width1 + x, p.color, true, (p.flags & PF_LIGHTS_SHADED) != 0, p.z,
+                 p.closest_distance);
+    record_counts(w, slot, p.n_shadow, p.n_bounce);
 }
 
 /* Primary stage of one tile (raytracer.h:81-134 up to the Beauty depth
@@ -274,7 +272,15 @@ __device__ static inline __attribute__((always_inline)) void primary_tile(const 
                     p.depth_left = v.depth;
                     p.flags = 0;
                     p.n_shadow = p.n_bounce = 0;
-                    run_level(w, sc, v, 0, tile * 64u + lane, M, hit, p, scene_stack, mesh_stack);
+                    const uint32_t slot = tile * 64u + lane;
+                    const bool bounce = run_level(w, sc, slot, M, hit, p, scene_stack, mesh_stack);
+                    const uint32_t q = wave_append(w.ctr, bounce);
+                    if (bounce) {
+                        ((ST_GLOBAL uint32_t *)w.bounce_q)[q] = slot;
+                        pix_store_all(w, slot, p);
+                    } else {
+                        end_pixel(w, sc, v, slot, p);
+                    }
                     entered = true;
                 }
             } else if (v.mode == ST_MODE_DEPTH) {
@@ -365,68 +371,62 @@ __device__ static inline __attribute__((always_inline)) void drain_queue(const K
     }
 }
 
-/* length of level `level`'s bounce queue, read on the device */
-__device__ static inline uint32_t bounce_count(const KWave &w, uint32_t level) {
-    return __builtin_amdgcn_readfirstlane(ldgu(w.ctr, level));
-}
-
-/* Bounce stage (surface.h:221-240): closest-hit traversal of the level's
-   bounce rays; a hit that is not an emissive quad starts the next level,
-   whose shading entry, shadow rays and shading the same lane runs. */
+/* Bounce stage (surface.h:221-240), one launch for every further level:
+   closest-hit traversal of each queued bounce ray; a hit that is not an
+   emissive quad starts the pixel's next level (shading entry, shadow rays,
+   shading) in the same lane, and a pixel that bounces again goes straight
+   on to its next bounce ray with its state in registers. No level waits for
+   the previous level's slowest wave, which at these depths, with a few
+   rays per SIMD, sets a level's time. */
 __global__ void __launch_bounds__(BLOCK_THREADS) K_BOUNCE_ATTR k_bounce(const StDScene *__restrict__ scenes,
                                                           const KReq *__restrict__ reqs, uint32_t nreq, KWave w,
-                                                          uint32_t level, uint32_t scene_entries,
-                                                          uint32_t mesh_entries) {
+                                                          uint32_t scene_entries, uint32_t mesh_entries) {
     ST_LDS_STACKS(scene_entries, mesh_entries);
-    const uint32_t count = bounce_count(w, level);
+    const uint32_t count = __builtin_amdgcn_readfirstlane(ldgu(w.ctr, 0));
     drain_queue(reqs, nreq, count,
                 [&](uint32_t qi, uint32_t &slot, uint32_t &) {
-                    slot = ldgu(w.bounce_q[level & 1], qi);
+                    slot = ldgu(w.bounce_q, qi);
                     return true;
                 },
                 [&](uint32_t r, uint32_t slot, uint32_t) {
                     const StDView v = reqs[r].v;
                     const StDScene sc = scenes[v.scene];
                     DPix p = pix_load(w, slot);
-                    DTrace tr;
-                    trace_init(tr, sc, scene_stack, mesh_stack, p.cur_dist);
-                    DRay ray;
-                    ray.origin = p.pos;
-                    ray.direction = p.bdir;
-                    DHit &hit = tr.closest_hit;
-                    hit.distance = hit.distance_squared = INFINITY;
-                    const bool found = trace_scene(ray, tr, sc, false);
-                    p.cur_dist = tr.current_hit.distance;
-                    bool done = !found;
-                    if (found) {
-                        p.prim_index = hit.object_id;
-                        const DMat M = load_mat(sc, hit.material_id);
-                        p.alb = M.albedo;
-                        if (M.use && M.texture_count) {
-                            DShaded s = shaded_of(p);
-                            sv2 dUV = V2(0, p.duv_y);
-                            apply_textures(tr, sc, s, M, dUV, false);
-                            p.alb = s.albedo;
+                    bool again;
+                    do {
+                        DTrace tr;
+                        trace_init(tr, sc, scene_stack, mesh_stack, p.cur_dist);
+                        DRay ray;
+                        ray.origin = p.pos;
+                        ray.direction = p.bdir;
+                        DHit &hit = tr.closest_hit;
+                        hit.distance = hit.distance_squared = INFINITY;
+                        const bool found = trace_scene(ray, tr, sc, false);
+                        p.cur_dist = tr.current_hit.distance;
+                        again = false;
+                        if (found) {
+                            p.prim_index = hit.object_id;
+                            const DMat M = load_mat(sc, hit.material_id);
+                            p.alb = M.albedo;
+                            if (M.use && M.texture_count) {
+                                DShaded s = shaded_of(p);
+                                sv2 dUV = V2(0, p.duv_y);
+                                apply_textures(tr, sc, s, M, dUV, false);
+                                p.alb = s.albedo;
+                            }
+                            if (hit.object_type == ST_PRIM_QUAD && (M.is & ST_EMISSIVE)) {
+                                p.color = hit.from_behind ? V3s(0) : M.emission;
+                            } else {
+                                if (M.brdf != ST_BRDF_COOK_TORRANCE) p.throughput = v_mul(p.throughput, M.reflectivity);
+                                p.vo = ray.origin;
+                                p.vd = ray.direction;
+                                p.material_id = hit.material_id;
+                                p.hit_d2 = hit.distance_squared;
+                                again = run_level(w, sc, slot, M, hit, p, scene_stack, mesh_stack);
+                            }
                         }
-                        if (hit.object_type == ST_PRIM_QUAD && (M.is & ST_EMISSIVE)) {
-                            p.color = hit.from_behind ? V3s(0) : M.emission;
-                            done = true;
-                        } else {
-                            if (M.brdf != ST_BRDF_COOK_TORRANCE) p.throughput = v_mul(p.throughput, M.reflectivity);
-                            p.vo = ray.origin;
-                            p.vd = ray.direction;
-                            p.material_id = hit.material_id;
-                            p.hit_d2 = hit.distance_squared;
-                            run_level(w, sc, v, level + 1, slot, M, hit, p, scene_stack, mesh_stack);
-                        }
-                    }
-                    if (done) {
-                        uint32_t x, y;
-                        tile_pixel(v, slot >> 6, slot & 63u, x, y);
-                        finish_pixel(sc, v, y * v.width1 + x, p.color, true, (p.flags & PF_LIGHTS_SHADED) != 0,
-                                     p.z, p.closest_distance);
-                        record_counts(w, slot, p.n_shadow, p.n_bounce);
-                    }
+                    } while (again);
+                    end_pixel(w, sc, v, slot, p);
                 });
 }
 
@@ -1188,7 +1188,7 @@ int st_rt_launch_prepared(StSceneHandle *const *scenes, StHostView *hv, uint32_t
     if (levels) {
         if (slots > 0xFFFFFFC0ull || slots * lmax > 0xFFFFFFC0ull) return set_err(ST_ERR_ARG, "batch too large");
         wave_bytes = round256(slots * 16 * ST_PLANES) + round256(slots * lmax * 8) + round256(slots * 4) +
-                     2 * round256(slots * 4) + round256((size_t)levels * 4) + round256(slots * 8) + 6 * 256;
+                     round256(slots * 4) + round256(4) + round256(slots * 8) + 6 * 256;
     }
     size_t work = up + round256(n * 24ull) + per_req_out + wave_bytes + 256;
     if ((rc = arena_reserve(&g.work, work))) return rc;
@@ -1243,9 +1243,8 @@ int st_rt_launch_prepared(StSceneHandle *const *scenes, StHostView *hv, uint32_t
         w.st = (float *)arena_take(&g.work, slots * 16 * ST_PLANES);
         w.shadow_res = (uint32_t *)arena_take(&g.work, slots * lmax * 8);
         w.occ_d2 = (float *)arena_take(&g.work, slots * 4);
-        w.bounce_q[0] = (uint32_t *)arena_take(&g.work, slots * 4);
-        w.bounce_q[1] = (uint32_t *)arena_take(&g.work, slots * 4);
-        w.ctr = (uint32_t *)arena_take(&g.work, (size_t)levels * 4);
+        w.bounce_q = (uint32_t *)arena_take(&g.work, slots * 4);
+        w.ctr = (uint32_t *)arena_take(&g.work, 4);
         w.counts = (uint32_t *)arena_take(&g.work, slots * 8);
     }
     /* launch shape, fixed before anything is enqueued so that a captured
@@ -1302,7 +1301,7 @@ int st_rt_launch_prepared(StSceneHandle *const *scenes, StHostView *hv, uint32_t
         if (any_device_ssb)
             hipLaunchKernelGGL(k_ssb, dim3(n), dim3(256), 0, g.stream, (const StDScene *)sc_d, (const KReq *)reqs_d);
         if (levels)
-            hipLaunchKernelGGL(k_zero_u32, dim3((levels + 255) / 256), dim3(256), 0, g.stream, w.ctr, levels);
+            hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, g.stream, w.ctr, 1u);
         hipLaunchKernelGGL(k_primary_dirs, dim3((rows + 63) / 64), dim3(64), 0, g.stream, reqs_d, n, rows);
         HIP_TRY(hipGetLastError());
         if (!graph) HIP_TRY(hipEventRecord(g.ev[0], g.stream));
@@ -1312,11 +1311,11 @@ int st_rt_launch_prepared(StSceneHandle *const *scenes, StHostView *hv, uint32_t
                            s_entries, m_entries, w);
         HIP_TRY(hipGetLastError());
         HIP_TRY(mark());
-        /* the rest of the depth loop: each bounce stage reads its queue
-           length on the device, so the sequence needs no host round trip */
-        for (uint32_t lv = 0; lv + 1 < levels; lv++) {
+        /* the rest of the depth loop: the bounce stage reads its queue
+           length on the device, so it needs no host round trip */
+        if (levels > 1) {
             hipLaunchKernelGGL(k_bounce, dim3(b_bounce), dim3(BLOCK_THREADS), lds, g.stream,
-                               (const StDScene *)sc_d, (const KReq *)reqs_d, n, w, lv, s_entries, m_entries);
+                               (const StDScene *)sc_d, (const KReq *)reqs_d, n, w, s_entries, m_entries);
             HIP_TRY(mark());
         }
         if (levels && count_blocks)

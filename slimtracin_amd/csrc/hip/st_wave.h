@@ -7,21 +7,21 @@
  * direct lights with one shadow ray each, emissive quads, light glow, and a
  * reflection/refraction bounce ray. Run as one GPU thread per pixel start to
  * finish, the lanes of a wave whose pixels stop early would idle while the
- * others bounce. Here the loop is cut at each bounce: one level of it
+ * others bounce. Here the loop is cut at the first bounce: a level of it
  * (shading entry, the shadow rays, the shading from their results) runs in
  * the lane that found the level's surface, with the pixel's state in
- * registers, and the pixels that continue are compacted into a queue for
- * the next level's kernel:
+ * registers, and the pixels that bounce after the first level are
+ * compacted into a queue:
  *
  *   primary   one wave per tile: primary ray, Depth/Normals/UVs output,
  *             and for Beauty the first level (the shadow rays start at the
  *             hit points just found, so the tile's nodes are still in cache)
- *   bounce    one launch per further level: closest-hit traversal of every
- *             queued bounce ray, then the next level in the same lane
+ *   bounce    per queued pixel: its bounce ray, the next level, and so on
+ *             down the depth loop, in one lane
  *
- * Between levels a pixel's state lives in HBM as planes of float4 (one plane
- * per four state values, slot = tile * 64 + lane), so that every state read
- * and write of a wave is one contiguous 1 KiB access.
+ * Across the queue a pixel's state lives in HBM as planes of float4 (one
+ * plane per four state values, slot = tile * 64 + lane), so that every state
+ * read and write of a wave is one contiguous 1 KiB access.
  *
  * Results equal the sequential reference bit for bit, including its shared
  * hit state: the shading values are recomputed with the reference's
@@ -46,8 +46,8 @@ struct KWave {
     uint32_t lmax;         /* planes of shadow_res (largest light count) */
     uint32_t *shadow_res;  /* [light][slot] 2 words: occluded, current-hit distance bits */
     float *occ_d2;         /* [slot] occluder's squared distance, last light's shadow ray */
-    uint32_t *bounce_q[2]; /* pixels with a bounce ray (ping-pong by level) */
-    uint32_t *ctr;         /* bounce-queue length per level */
+    uint32_t *bounce_q;    /* slots of the pixels with a first-level bounce ray */
+    uint32_t *ctr;         /* [1] length of bounce_q */
     uint32_t *counts;      /* [slot] 2 words: shadow rays, bounce rays of the pixel */
 };
 
