@@ -3,12 +3,13 @@
  * HIP tracing library (gfx950, wave64).
  *
  * Kernels
- *   k_primary_dirs  one lane per traced row: walks the row with the
- *                   reference's sequential `current += right` fp32 sum
- *                   (render/raytracer.h:156-170) and stores every pixel's
- *                   unnormalised `current` vector (the primary stage
- *                   normalises it). Rows of every request of a launch are
- *                   flattened into one grid.
+ *   k_primary_dirs  one lane per traced row and vector component: walks
+ *                   the row with the reference's sequential
+ *                   `current += right` fp32 sum (render/raytracer.h:156-170)
+ *                   and stores the unnormalised `current` vector at every
+ *                   ST_DIR_MARK-th pixel (the primary stage finishes the sum
+ *                   and normalises it). Rows of every request of a launch
+ *                   are flattened into one grid.
  *   k_primary       one wavefront per pixel tile: primary ray, and for
  *                   Beauty the first level of the depth loop (shading
  *                   entry, shadow rays, shading; st_wave.h). Tiles of all requests
@@ -113,28 +114,33 @@ __device__ static inline uint32_t find_req(const KReq *reqs, uint32_t n, uint32_
     return lo;
 }
 
-/* One lane per traced row: walks the row's sequential `current += right`
-   sum and stores the value at every ST_DIR_MARK-th pixel. A pixel's
-   direction is then its checkpoint plus at most ST_DIR_MARK - 1 further
-   additions, the same fp32 operations in the same order (primary_dir). */
-__global__ void __launch_bounds__(64) k_primary_dirs(const KReq *reqs, uint32_t nreq, uint32_t total_rows) {
-    const uint32_t row = blockIdx.x * 64u + threadIdx.x;
+/* One lane per traced row and component of the direction: walks that
+   component of the row's sequential `current += right` sum (a vector sum's
+   components are independent fp32 chains, so three lanes run a row's
+   chain a third as long each) and stores the value at every
+   ST_DIR_MARK-th pixel, into the x, y or z word of the pixel's float4
+   checkpoint. A pixel's direction is then its checkpoint plus at most
+   ST_DIR_MARK - 1 further additions, the same fp32 operations in the same
+   order (primary_dir). */
+#define DIRS_BLOCK 192u
+__global__ void __launch_bounds__(DIRS_BLOCK) k_primary_dirs(const KReq *reqs, uint32_t nreq, uint32_t total_rows) {
+    const uint32_t t = blockIdx.x * DIRS_BLOCK + threadIdx.x;
+    const uint32_t row = t / 3u, c = t - 3u * row;
     if (row >= total_rows) return;
     const uint32_t r = find_req(reqs, nreq, row, true);
     const StDView &v = reqs[r].v;
     const uint32_t y = row - reqs[r].row_base;
     if (y >= v.height1) return;
     const float4 rs = ldg4(v.row_starts, y);
-    sv3 current = V3(rs.x, rs.y, rs.z);
-    const sv3 right = V3(v.right[0], v.right[1], v.right[2]);
+    float current = c == 0 ? rs.x : (c == 1 ? rs.y : rs.z);
+    const float right = v.right[c];
     const uint32_t marks = ST_DIR_MARKS(v.width1);
-    ST_GLOBAL st_vf4 *out = (ST_GLOBAL st_vf4 *)v.dir_marks + (size_t)y * marks;
+    ST_GLOBAL float *out = (ST_GLOBAL float *)v.dir_marks + (size_t)y * marks * 4 + c;
     for (uint32_t m = 0; m < marks; m++) {
-        st_vf4 val = {current.x, current.y, current.z, 0.0f};
-        out[m] = val;
+        out[(size_t)m * 4] = current;
         if (m + 1 < marks) {
 #pragma unroll
-            for (uint32_t k = 0; k < ST_DIR_MARK; k++) current = v_add(current, right);
+            for (uint32_t k = 0; k < ST_DIR_MARK; k++) current = current + right;
         }
     }
 }
@@ -446,9 +452,16 @@ __global__ void __launch_bounds__(256) k_counts(const KReq *__restrict__ reqs, u
     const uint32_t t0 = q.v.tile_base + (blockIdx.x - q.count_block) * COUNT_TILES;
     const uint32_t t1 = min(t0 + COUNT_TILES, q.v.tile_base + q.tile_count);
     unsigned long long sh = 0, bo = 0;
-    for (size_t s = (size_t)t0 * 64 + threadIdx.x; s < (size_t)t1 * 64; s += 256) {
-        sh += ldgu(counts, 2 * s);
-        bo += ldgu(counts, 2 * s + 1);
+    /* one 8-byte load per slot, unrolled so that a lane's loads are all in
+       flight together */
+    typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+    const ST_GLOBAL u32x2 *c2 = (const ST_GLOBAL u32x2 *)counts + (size_t)t0 * 64;
+    const uint32_t n_slots = (t1 - t0) * 64u;
+#pragma unroll 16
+    for (uint32_t i = threadIdx.x; i < n_slots; i += 256) {
+        const u32x2 c = c2[i];
+        sh += c.x;
+        bo += c.y;
     }
     for (int off = 32; off > 0; off >>= 1) {
         sh += __shfl_xor(sh, off, 64);
@@ -1166,7 +1179,8 @@ int st_rt_launch_prepared(StSceneHandle *const *scenes, StHostView *hv, uint32_t
         rows += hv[i].dv.height1;
         out_px += (uint64_t)(hv[i].dv.width1 - 1) * (hv[i].dv.height1 - 1);
     }
-    if (out_px > 0xFFFFFFF0ull || tiles > 0x7FFFFFF0u) return set_err(ST_ERR_ARG, "batch too large");
+    if (out_px > 0xFFFFFFF0ull || tiles > 0x7FFFFFF0u || (uint64_t)rows * 3 > 0xFFFFFFF0ull)
+        return set_err(ST_ERR_ARG, "batch too large");
     size_t o_scenes = up;
     up += round256(nuniq * sizeof(StDScene));
     for (uint32_t i = 0; i < n; i++) {
@@ -1302,7 +1316,8 @@ int st_rt_launch_prepared(StSceneHandle *const *scenes, StHostView *hv, uint32_t
             hipLaunchKernelGGL(k_ssb, dim3(n), dim3(256), 0, g.stream, (const StDScene *)sc_d, (const KReq *)reqs_d);
         if (levels)
             hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, g.stream, w.ctr, 1u);
-        hipLaunchKernelGGL(k_primary_dirs, dim3((rows + 63) / 64), dim3(64), 0, g.stream, reqs_d, n, rows);
+        hipLaunchKernelGGL(k_primary_dirs, dim3((uint32_t)(((uint64_t)rows * 3 + DIRS_BLOCK - 1) / DIRS_BLOCK)),
+                           dim3(DIRS_BLOCK), 0, g.stream, reqs_d, n, rows);
         HIP_TRY(hipGetLastError());
         if (!graph) HIP_TRY(hipEventRecord(g.ev[0], g.stream));
         HIP_TRY(mark());
