@@ -348,19 +348,27 @@ k_primary(const StDScene *__restrict__ scenes, const KReq *__restrict__ reqs, ui
     }
 }
 
+/* word of the per-launch counters holding the bounce stage's chunk claims */
+#define BOUNCE_CLAIM 32u
+
 /* Wave loop over `count` queue entries. entry(i, slot, aux) decodes entry i
    into its state slot (and an extra word) and says whether it is live.
-   Waves take 64-entry chunks in a fixed interleaved order (no shared
-   counter: same-address atomics serialise). Lanes are grouped by request
-   and f(request, slot, aux) runs once per group with a wave-uniform
-   request, so the view and scene records load into scalar registers. */
+   Each wave's first 64-entry chunk is its wave id; after that a wave claims
+   the next chunk from *claim (zero at launch; one atomic per chunk, so
+   few: same-address atomics serialise). A wave whose chunk runs long thus
+   takes no further chunks, which shortens the launch's tail, and every
+   wave leaves after its first claim past the end. Lanes are grouped by
+   request and f(request, slot, aux) runs once per group with a
+   wave-uniform request, so the view and scene records load into scalar
+   registers. */
 template <class E, class F>
 __device__ static inline __attribute__((always_inline)) void drain_queue(const KReq *reqs, uint32_t nreq,
-                                                                        uint32_t count, E &&entry, F &&f) {
+                                                                        uint32_t count, uint32_t *claim,
+                                                                        E &&entry, F &&f) {
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t waves = gridDim.x * WAVES_PER_BLOCK;
-    for (uint32_t chunk = __builtin_amdgcn_readfirstlane(blockIdx.x * WAVES_PER_BLOCK + (threadIdx.x >> 6));
-         (size_t)chunk * 64 < count; chunk += waves) {
+    uint32_t chunk = __builtin_amdgcn_readfirstlane(blockIdx.x * WAVES_PER_BLOCK + (threadIdx.x >> 6));
+    while ((size_t)chunk * 64 < count) {
         const uint32_t qi = chunk * 64 + lane;
         uint32_t slot = 0, aux = 0;
         bool pending = qi < count && entry(qi, slot, aux);
@@ -374,6 +382,9 @@ __device__ static inline __attribute__((always_inline)) void drain_queue(const K
                 }
             }
         }
+        uint32_t c = 0;
+        if (lane == 0) c = atomicAdd(claim, 1u);
+        chunk = waves + __builtin_amdgcn_readfirstlane(c);
     }
 }
 
@@ -386,10 +397,11 @@ __device__ static inline __attribute__((always_inline)) void drain_queue(const K
    rays per SIMD, sets a level's time. */
 __global__ void __launch_bounds__(BLOCK_THREADS) K_BOUNCE_ATTR k_bounce(const StDScene *__restrict__ scenes,
                                                           const KReq *__restrict__ reqs, uint32_t nreq, KWave w,
-                                                          uint32_t scene_entries, uint32_t mesh_entries) {
+                                                          uint32_t *claim, uint32_t scene_entries,
+                                                          uint32_t mesh_entries) {
     ST_LDS_STACKS(scene_entries, mesh_entries);
     const uint32_t count = __builtin_amdgcn_readfirstlane(ldgu(w.ctr, 0));
-    drain_queue(reqs, nreq, count,
+    drain_queue(reqs, nreq, count, claim,
                 [&](uint32_t qi, uint32_t &slot, uint32_t &) {
                     slot = ldgu(w.bounce_q, qi);
                     return true;
@@ -662,7 +674,8 @@ static struct {
     Arena work;            /* per-launch device buffers (grow-only) */
     char *staging;         /* pinned host staging for per-launch uploads */
     size_t staging_cap;
-    uint32_t *tile_counter;
+    uint32_t *tile_counter; /* 64 words zeroed per launch: [0] the persistent primary stage's tile
+                               counter, [BOUNCE_CLAIM] the bounce stage's chunk claims (128 B apart) */
     StDScene *scene_table; /* device copy of the scene descriptors of a launch */
     uint32_t scene_table_cap;
     hipEvent_t ev[4];
@@ -1312,6 +1325,7 @@ int st_rt_launch_prepared(StSceneHandle *const *scenes, StHostView *hv, uint32_t
         HIP_TRY(hipMemcpyAsync(blk, st, up, hipMemcpyHostToDevice, g.stream));
         hipLaunchKernelGGL(k_zero_u64, dim3((3 * n + 255) / 256), dim3(256), 0, g.stream, cnt_d, 3 * n);
         hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, g.stream, g.tile_counter, 64u);
+        static_assert(BOUNCE_CLAIM < 64, "claim counters share the zeroed 64 words");
         if (any_device_ssb)
             hipLaunchKernelGGL(k_ssb, dim3(n), dim3(256), 0, g.stream, (const StDScene *)sc_d, (const KReq *)reqs_d);
         if (levels)
@@ -1330,7 +1344,8 @@ int st_rt_launch_prepared(StSceneHandle *const *scenes, StHostView *hv, uint32_t
            length on the device, so it needs no host round trip */
         if (levels > 1) {
             hipLaunchKernelGGL(k_bounce, dim3(b_bounce), dim3(BLOCK_THREADS), lds, g.stream,
-                               (const StDScene *)sc_d, (const KReq *)reqs_d, n, w, s_entries, m_entries);
+                               (const StDScene *)sc_d, (const KReq *)reqs_d, n, w, g.tile_counter + BOUNCE_CLAIM,
+                               s_entries, m_entries);
             HIP_TRY(mark());
         }
         if (levels && count_blocks)
