@@ -280,9 +280,14 @@ __device__ static inline __attribute__((always_inline)) void primary_tile(const 
                     p.n_shadow = p.n_bounce = 0;
                     const uint32_t slot = tile * 64u + lane;
                     const bool bounce = run_level(w, sc, slot, M, hit, p, scene_stack, mesh_stack);
-                    const uint32_t q = wave_append(w.ctr, bounce);
+                    /* refracting pixels, which go on for the most levels,
+                       fill the queue from the front, the others from the
+                       back; the bounce stage starts with the front */
+                    const bool deep = (M.is & ST_REFRACTIVE) != 0;
+                    const uint32_t qf = wave_append(w.ctr, bounce && deep);
+                    const uint32_t qb = wave_append(w.ctr + 1, bounce && !deep);
                     if (bounce) {
-                        ((ST_GLOBAL uint32_t *)w.bounce_q)[q] = slot;
+                        ((ST_GLOBAL uint32_t *)w.bounce_q)[deep ? qf : w.n - 1u - qb] = slot;
                         pix_store_all(w, slot, p);
                     } else {
                         end_pixel(w, sc, v, slot, p);
@@ -351,9 +356,9 @@ k_primary(const StDScene *__restrict__ scenes, const KReq *__restrict__ reqs, ui
 /* word of the per-launch counters holding the bounce stage's chunk claims */
 #define BOUNCE_CLAIM 32u
 
-/* Wave loop over `count` queue entries. entry(i, slot, aux) decodes entry i
-   into its state slot (and an extra word) and says whether it is live.
-   Each wave's first 64-entry chunk is its wave id; after that a wave claims
+/* Wave loop over `chunks` chunks of 64 queue entries. entry(chunk, lane,
+   slot, aux) decodes one entry into its state slot (and an extra word) and
+   says whether it is live. Each wave's first chunk is its wave id; after that a wave claims
    the next chunk from *claim (zero at launch; one atomic per chunk, so
    few: same-address atomics serialise). A wave whose chunk runs long thus
    takes no further chunks, which shortens the launch's tail, and every
@@ -363,15 +368,14 @@ k_primary(const StDScene *__restrict__ scenes, const KReq *__restrict__ reqs, ui
    registers. */
 template <class E, class F>
 __device__ static inline __attribute__((always_inline)) void drain_queue(const KReq *reqs, uint32_t nreq,
-                                                                        uint32_t count, uint32_t *claim,
+                                                                        uint32_t chunks, uint32_t *claim,
                                                                         E &&entry, F &&f) {
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t waves = gridDim.x * WAVES_PER_BLOCK;
     uint32_t chunk = __builtin_amdgcn_readfirstlane(blockIdx.x * WAVES_PER_BLOCK + (threadIdx.x >> 6));
-    while ((size_t)chunk * 64 < count) {
-        const uint32_t qi = chunk * 64 + lane;
+    while (chunk < chunks) {
         uint32_t slot = 0, aux = 0;
-        bool pending = qi < count && entry(qi, slot, aux);
+        bool pending = entry(chunk, lane, slot, aux);
         const uint32_t r = pending ? find_req(reqs, nreq, slot >> 6, false) : 0u;
         while (__ballot(pending)) {
             if (pending) {
@@ -400,10 +404,22 @@ __global__ void __launch_bounds__(BLOCK_THREADS) K_BOUNCE_ATTR k_bounce(const St
                                                           uint32_t *claim, uint32_t scene_entries,
                                                           uint32_t mesh_entries) {
     ST_LDS_STACKS(scene_entries, mesh_entries);
-    const uint32_t count = __builtin_amdgcn_readfirstlane(ldgu(w.ctr, 0));
-    drain_queue(reqs, nreq, count, claim,
-                [&](uint32_t qi, uint32_t &slot, uint32_t &) {
-                    slot = ldgu(w.bounce_q, qi);
+    /* the queue's front part (ctr[0] entries) first, then its back part
+       (ctr[1] entries, stored from the end of the buffer down) */
+    const uint32_t n_front = __builtin_amdgcn_readfirstlane(ldgu(w.ctr, 0));
+    const uint32_t n_back = __builtin_amdgcn_readfirstlane(ldgu(w.ctr, 1));
+    const uint32_t c_front = (n_front + 63u) / 64u;
+    drain_queue(reqs, nreq, c_front + (n_back + 63u) / 64u, claim,
+                [&](uint32_t chunk, uint32_t lane, uint32_t &slot, uint32_t &) {
+                    if (chunk < c_front) {
+                        const uint32_t qi = chunk * 64u + lane;
+                        if (qi >= n_front) return false;
+                        slot = ldgu(w.bounce_q, qi);
+                    } else {
+                        const uint32_t qi = (chunk - c_front) * 64u + lane;
+                        if (qi >= n_back) return false;
+                        slot = ldgu(w.bounce_q, w.n - 1u - qi);
+                    }
                     return true;
                 },
                 [&](uint32_t r, uint32_t slot, uint32_t) {
@@ -1215,7 +1231,7 @@ int st_rt_launch_prepared(StSceneHandle *const *scenes, StHostView *hv, uint32_t
     if (levels) {
         if (slots > 0xFFFFFFC0ull || slots * lmax > 0xFFFFFFC0ull) return set_err(ST_ERR_ARG, "batch too large");
         wave_bytes = round256(slots * 16 * ST_PLANES) + round256(slots * lmax * 8) + round256(slots * 4) +
-                     round256(slots * 4) + round256(4) + round256(slots * 8) + 6 * 256;
+                     round256(slots * 4) + round256(8) + round256(slots * 8) + 6 * 256;
     }
     size_t work = up + round256(n * 24ull) + per_req_out + wave_bytes + 256;
     if ((rc = arena_reserve(&g.work, work))) return rc;
@@ -1271,7 +1287,7 @@ int st_rt_launch_prepared(StSceneHandle *const *scenes, StHostView *hv, uint32_t
         w.shadow_res = (uint32_t *)arena_take(&g.work, slots * lmax * 8);
         w.occ_d2 = (float *)arena_take(&g.work, slots * 4);
         w.bounce_q = (uint32_t *)arena_take(&g.work, slots * 4);
-        w.ctr = (uint32_t *)arena_take(&g.work, 4);
+        w.ctr = (uint32_t *)arena_take(&g.work, 8);
         w.counts = (uint32_t *)arena_take(&g.work, slots * 8);
     }
     /* launch shape, fixed before anything is enqueued so that a captured
@@ -1329,7 +1345,7 @@ int st_rt_launch_prepared(StSceneHandle *const *scenes, StHostView *hv, uint32_t
         if (any_device_ssb)
             hipLaunchKernelGGL(k_ssb, dim3(n), dim3(256), 0, g.stream, (const StDScene *)sc_d, (const KReq *)reqs_d);
         if (levels)
-            hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, g.stream, w.ctr, 1u);
+            hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, g.stream, w.ctr, 2u);
         hipLaunchKernelGGL(k_primary_dirs, dim3((uint32_t)(((uint64_t)rows * 3 + DIRS_BLOCK - 1) / DIRS_BLOCK)),
                            dim3(DIRS_BLOCK), 0, g.stream, reqs_d, n, rows);
         HIP_TRY(hipGetLastError());

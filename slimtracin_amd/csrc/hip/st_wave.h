@@ -46,8 +46,9 @@ struct KWave {
     uint32_t lmax;         /* planes of shadow_res (largest light count) */
     uint32_t *shadow_res;  /* [light][slot] 2 words: occluded, current-hit distance bits */
     float *occ_d2;         /* [slot] occluder's squared distance, last light's shadow ray */
-    uint32_t *bounce_q;    /* slots of the pixels with a first-level bounce ray */
-    uint32_t *ctr;         /* [1] length of bounce_q */
+    uint32_t *bounce_q;    /* [n] slots of the pixels with a first-level bounce ray: ctr[0] from the
+                              front (refracting surfaces), ctr[1] from the back (the others) */
+    uint32_t *ctr;         /* [2] lengths of bounce_q's front and back parts */
     uint32_t *counts;      /* [slot] 2 words: shadow rays, bounce rays of the pixel */
 };
 
