@@ -328,7 +328,7 @@ __device__ static inline __attribute__((always_inline)) void primary_tile(const 
 __global__ void __launch_bounds__(BLOCK_THREADS) K_PRIMARY_ATTR
 k_primary(const StDScene *__restrict__ scenes, const KReq *__restrict__ reqs, uint32_t nreq, uint32_t total_tiles,
           uint32_t *tile_counter, uint32_t persistent, uint32_t xcd_blocks, uint32_t scene_entries,
-          uint32_t mesh_entries, KWave w) {
+          uint32_t mesh_entries, KWave w, const uint32_t *__restrict__ tile_order, uint32_t *__restrict__ tile_cost) {
     ST_LDS_STACKS(scene_entries, mesh_entries);
     /* One call site, so primary_tile is inlined once. Persistent: waves
        claim tiles from the global counter until it runs past the end.
@@ -349,7 +349,57 @@ k_primary(const StDScene *__restrict__ scenes, const KReq *__restrict__ reqs, ui
            scene records load into scalar registers */
         t = __builtin_amdgcn_readfirstlane(t);
         if (t >= total_tiles) break;
+        /* claim order -> tile (k_tile_order) */
+        t = __builtin_amdgcn_readfirstlane(ldgu(tile_order, t));
+        const uint64_t t0 = __builtin_amdgcn_s_memrealtime();
         primary_tile(scenes, reqs, nreq, t, scene_stack, mesh_stack, w);
+        /* the tile's time (100 MHz ticks), which orders the next launch */
+        if ((threadIdx.x & 63u) == 0)
+            ((ST_GLOBAL uint32_t *)tile_cost)[t] = (uint32_t)(__builtin_amdgcn_s_memrealtime() - t0);
+    }
+}
+
+/* Claim order of the primary stage's tiles, from each tile's time in the
+   previous launch (written by k_primary): tiles that took longer than the
+   mean first, then the others, each group in tile order. A launch ends
+   when its slowest tiles end, so those should start first and the cheap
+   ones fill the end (on the 200k-triangle frame the tiles over the mesh
+   take 200-600 us, the floor's a few tens). Any cost values give a
+   permutation of the tiles, so the order never changes a result; the
+   first launch of a shape takes whatever the buffer holds. One block. */
+#define ORDER_THREADS 1024u
+__global__ void __launch_bounds__(ORDER_THREADS) k_tile_order(const uint32_t *__restrict__ tile_cost,
+                                                              uint32_t *__restrict__ tile_order, uint32_t tiles) {
+    __shared__ unsigned long long s_sum[ORDER_THREADS / 64];
+    __shared__ uint32_t s_scan[ORDER_THREADS];
+    const uint32_t tid = threadIdx.x;
+    const uint32_t per = (tiles + ORDER_THREADS - 1) / ORDER_THREADS;
+    const uint32_t lo = min(tid * per, tiles), hi = min(lo + per, tiles);
+    unsigned long long sum = 0;
+    for (uint32_t i = lo; i < hi; i++) sum += ldgu(tile_cost, i);
+    for (int off = 32; off > 0; off >>= 1) sum += __shfl_xor(sum, off, 64);
+    if ((tid & 63u) == 0) s_sum[tid >> 6] = sum;
+    __syncthreads();
+    unsigned long long total = 0;
+    for (uint32_t k = 0; k < ORDER_THREADS / 64; k++) total += s_sum[k];
+    /* heavy: cost * tiles > total (cost above the mean, no division) */
+    uint32_t heavy = 0;
+    for (uint32_t i = lo; i < hi; i++) heavy += (unsigned long long)ldgu(tile_cost, i) * tiles > total;
+    /* inclusive scan of the per-thread heavy counts */
+    s_scan[tid] = heavy;
+    __syncthreads();
+    for (uint32_t off = 1; off < ORDER_THREADS; off <<= 1) {
+        const uint32_t v = tid >= off ? s_scan[tid - off] : 0u;
+        __syncthreads();
+        s_scan[tid] += v;
+        __syncthreads();
+    }
+    const uint32_t n_heavy = s_scan[ORDER_THREADS - 1];
+    uint32_t hpos = s_scan[tid] - heavy;            /* heavy tiles before lo */
+    uint32_t lpos = n_heavy + (lo - hpos);          /* light tiles go after all heavy ones */
+    for (uint32_t i = lo; i < hi; i++) {
+        const bool h = (unsigned long long)ldgu(tile_cost, i) * tiles > total;
+        ((ST_GLOBAL uint32_t *)tile_order)[h ? hpos++ : lpos++] = i;
     }
 }
 
@@ -690,6 +740,9 @@ static struct {
     Arena work;            /* per-launch device buffers (grow-only) */
     char *staging;         /* pinned host staging for per-launch uploads */
     size_t staging_cap;
+    uint32_t *tile_cost;   /* [tile_cap] each tile's time in the last primary stage */
+    uint32_t *tile_order;  /* [tile_cap] the primary stage's claim order (k_tile_order) */
+    uint32_t tile_cap;
     uint32_t *tile_counter; /* 64 words zeroed per launch: [0] the persistent primary stage's tile
                                counter, [BOUNCE_CLAIM] the bounce stage's chunk claims (128 B apart) */
     StDScene *scene_table; /* device copy of the scene descriptors of a launch */
@@ -1236,6 +1289,15 @@ int st_rt_launch_prepared(StSceneHandle *const *scenes, StHostView *hv, uint32_t
     size_t work = up + round256(n * 24ull) + per_req_out + wave_bytes + 256;
     if ((rc = arena_reserve(&g.work, work))) return rc;
     if ((rc = staging_reserve(up))) return rc;
+    if (tiles > g.tile_cap) {   /* per-tile times and claim order, kept across launches */
+        if (g.tile_cost) HIP_TRY(hipFree(g.tile_cost));
+        g.tile_cost = g.tile_order = NULL;
+        g.tile_cap = 0;
+        HIP_TRY(hipMalloc(&g.tile_cost, (size_t)tiles * 8));
+        HIP_TRY(hipMemset(g.tile_cost, 0, (size_t)tiles * 8));
+        g.tile_order = g.tile_cost + tiles;
+        g.tile_cap = tiles;
+    }
     char *st = g.staging;
     char *blk = (char *)arena_take(&g.work, up);
     KReq *reqs_h = (KReq *)st;
@@ -1349,11 +1411,14 @@ int st_rt_launch_prepared(StSceneHandle *const *scenes, StHostView *hv, uint32_t
         hipLaunchKernelGGL(k_primary_dirs, dim3((uint32_t)(((uint64_t)rows * 3 + DIRS_BLOCK - 1) / DIRS_BLOCK)),
                            dim3(DIRS_BLOCK), 0, g.stream, reqs_d, n, rows);
         HIP_TRY(hipGetLastError());
+        hipLaunchKernelGGL(k_tile_order, dim3(1), dim3(ORDER_THREADS), 0, g.stream, (const uint32_t *)g.tile_cost,
+                           g.tile_order, tiles);
+        HIP_TRY(hipGetLastError());
         if (!graph) HIP_TRY(hipEventRecord(g.ev[0], g.stream));
         HIP_TRY(mark());
         hipLaunchKernelGGL(k_primary, dim3(blocks), dim3(BLOCK_THREADS), lds_primary, g.stream, (const StDScene *)sc_d,
                            (const KReq *)reqs_d, n, tiles, g.tile_counter, (uint32_t)persistent, xcd_blocks,
-                           s_entries, m_entries, w);
+                           s_entries, m_entries, w, (const uint32_t *)g.tile_order, g.tile_cost);
         HIP_TRY(hipGetLastError());
         HIP_TRY(mark());
         /* the rest of the depth loop: the bounce stage reads its queue
@@ -1383,7 +1448,8 @@ int st_rt_launch_prepared(StSceneHandle *const *scenes, StHostView *hv, uint32_t
                                      (uint64_t)persistent, s_entries, m_entries, b_bounce,
                                      lds_primary, (uint64_t)(uintptr_t)blk,
                                      (uint64_t)(uintptr_t)st,
-                                     (uint64_t)(uintptr_t)cnt_d, (uint64_t)(uintptr_t)w.st, w.n};
+                                     (uint64_t)(uintptr_t)cnt_d, (uint64_t)(uintptr_t)w.st, w.n,
+                                     (uint64_t)(uintptr_t)g.tile_cost};
         for (uint32_t i = 0; i < n; i++) {
             const StDView &dv = reqs_h[i].v;
             sig.insert(sig.end(), {dv.width1, dv.height1, dv.mode, dv.depth, dv.tile_shift,
