@@ -10,6 +10,8 @@
  *                   ST_DIR_MARK-th pixel (the primary stage finishes the sum
  *                   and normalises it). Rows of every request of a launch
  *                   are flattened into one grid.
+ *   k_tile_order    the primary stage's tile claim order: tiles slower than
+ *                   the mean in the previous launch first.
  *   k_primary       one wavefront per pixel tile: primary ray, and for
  *                   Beauty the first level of the depth loop (shading
  *                   entry, shadow rays, shading; st_wave.h). Tiles of all requests
@@ -328,8 +330,11 @@ __device__ static inline __attribute__((always_inline)) void primary_tile(const 
 __global__ void __launch_bounds__(BLOCK_THREADS) K_PRIMARY_ATTR
 k_primary(const StDScene *__restrict__ scenes, const KReq *__restrict__ reqs, uint32_t nreq, uint32_t total_tiles,
           uint32_t *tile_counter, uint32_t persistent, uint32_t xcd_blocks, uint32_t scene_entries,
-          uint32_t mesh_entries, KWave w, const uint32_t *__restrict__ tile_order, uint32_t *__restrict__ tile_cost) {
+          uint32_t mesh_entries, KWave w, const uint32_t *__restrict__ tile_order, uint32_t *__restrict__ tile_cost,
+          unsigned long long *cost_sum) {
     ST_LDS_STACKS(scene_entries, mesh_entries);
+    /* k_tile_order has read the previous launch's sum; k_counts adds this one's */
+    if (blockIdx.x == 0 && threadIdx.x == 0) *(volatile unsigned long long *)cost_sum = 0;
     /* One call site, so primary_tile is inlined once. Persistent: waves
        claim tiles from the global counter until it runs past the end.
        Otherwise each wave renders one tile: blocks are dealt to the 8 XCDs
@@ -361,50 +366,52 @@ k_primary(const StDScene *__restrict__ scenes, const KReq *__restrict__ reqs, ui
 
 /* Claim order of the primary stage's tiles, from each tile's time in the
    previous launch (written by k_primary): tiles that took longer than the
-   mean first, then the others, each group in tile order. A launch ends
-   when its slowest tiles end, so those should start first and the cheap
-   ones fill the end (on the 200k-triangle frame the tiles over the mesh
-   take 200-600 us, the floor's a few tens). Any cost values give a
-   permutation of the tiles, so the order never changes a result; the
-   first launch of a shape takes whatever the buffer holds. One block. */
+   mean (cost_sum[0] / tiles, summed by that launch's k_counts) first, then
+   the others. A launch ends when its slowest tiles end, so those should
+   start first and the cheap ones fill the end (on the 200k-triangle frame
+   the tiles over the mesh take 200-600 us, the floor's a few tens). One
+   thread per tile; each block keeps its tiles in order within each group
+   and claims its ranges with one atomic per group (ctr[0] from the front,
+   ctr[1] from the back, both zero at launch). Any cost values give a
+   permutation of the tiles, so the order never changes a result.
+   k_primary then clears cost_sum for its own launch's k_counts. */
 #define ORDER_THREADS 1024u
 __global__ void __launch_bounds__(ORDER_THREADS) k_tile_order(const uint32_t *__restrict__ tile_cost,
-                                                              uint32_t *__restrict__ tile_order, uint32_t tiles) {
-    __shared__ unsigned long long s_sum[ORDER_THREADS / 64];
-    __shared__ uint32_t s_scan[ORDER_THREADS];
-    const uint32_t tid = threadIdx.x;
-    const uint32_t per = (tiles + ORDER_THREADS - 1) / ORDER_THREADS;
-    const uint32_t lo = min(tid * per, tiles), hi = min(lo + per, tiles);
-    unsigned long long sum = 0;
-    for (uint32_t i = lo; i < hi; i++) sum += ldgu(tile_cost, i);
-    for (int off = 32; off > 0; off >>= 1) sum += __shfl_xor(sum, off, 64);
-    if ((tid & 63u) == 0) s_sum[tid >> 6] = sum;
-    __syncthreads();
-    unsigned long long total = 0;
-    for (uint32_t k = 0; k < ORDER_THREADS / 64; k++) total += s_sum[k];
-    /* heavy: cost * tiles > total (cost above the mean, no division) */
-    uint32_t heavy = 0;
-    for (uint32_t i = lo; i < hi; i++) heavy += (unsigned long long)ldgu(tile_cost, i) * tiles > total;
-    /* inclusive scan of the per-thread heavy counts */
-    s_scan[tid] = heavy;
-    __syncthreads();
-    for (uint32_t off = 1; off < ORDER_THREADS; off <<= 1) {
-        const uint32_t v = tid >= off ? s_scan[tid - off] : 0u;
-        __syncthreads();
-        s_scan[tid] += v;
-        __syncthreads();
+                                                              uint32_t *__restrict__ tile_order, uint32_t tiles,
+                                                              unsigned long long *cost_sum, uint32_t *ctr) {
+    __shared__ uint32_t s_h[ORDER_THREADS / 64], s_l[ORDER_THREADS / 64], s_base[2];
+    const uint32_t i = blockIdx.x * ORDER_THREADS + threadIdx.x;
+    const uint32_t wv = threadIdx.x >> 6;
+    const unsigned long long total = *(volatile unsigned long long *)cost_sum;
+    const bool live = i < tiles;
+    const bool heavy = live && (unsigned long long)ldgu(tile_cost, i) * tiles > total;
+    const unsigned long long mh = __ballot(heavy), ml = __ballot(live && !heavy);
+    const uint32_t rh = __builtin_amdgcn_mbcnt_hi((uint32_t)(mh >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mh, 0u));
+    const uint32_t rl = __builtin_amdgcn_mbcnt_hi((uint32_t)(ml >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)ml, 0u));
+    if ((threadIdx.x & 63u) == 0) {
+        s_h[wv] = (uint32_t)__builtin_popcountll(mh);
+        s_l[wv] = (uint32_t)__builtin_popcountll(ml);
     }
-    const uint32_t n_heavy = s_scan[ORDER_THREADS - 1];
-    uint32_t hpos = s_scan[tid] - heavy;            /* heavy tiles before lo */
-    uint32_t lpos = n_heavy + (lo - hpos);          /* light tiles go after all heavy ones */
-    for (uint32_t i = lo; i < hi; i++) {
-        const bool h = (unsigned long long)ldgu(tile_cost, i) * tiles > total;
-        ((ST_GLOBAL uint32_t *)tile_order)[h ? hpos++ : lpos++] = i;
+    __syncthreads();
+    uint32_t bh = 0, bl = 0, nh = 0, nl = 0;
+    for (uint32_t k = 0; k < ORDER_THREADS / 64; k++) {
+        if (k < wv) { bh += s_h[k]; bl += s_l[k]; }
+        nh += s_h[k];
+        nl += s_l[k];
     }
+    if (threadIdx.x == 0) {
+        s_base[0] = atomicAdd(ctr, nh);
+        s_base[1] = atomicAdd(ctr + 1, nl);
+    }
+    __syncthreads();
+    if (heavy) ((ST_GLOBAL uint32_t *)tile_order)[s_base[0] + bh + rh] = i;
+    else if (live) ((ST_GLOBAL uint32_t *)tile_order)[tiles - 1u - (s_base[1] + bl + rl)] = i;
 }
 
 /* word of the per-launch counters holding the bounce stage's chunk claims */
 #define BOUNCE_CLAIM 32u
+/* words of the per-launch counters holding k_tile_order's two claims */
+#define ORDER_CLAIM 40u
 
 /* Wave loop over `chunks` chunks of 64 queue entries. entry(chunk, lane,
    slot, aux) decodes one entry into its state slot (and an extra word) and
@@ -518,8 +525,11 @@ __global__ void __launch_bounds__(BLOCK_THREADS) K_BOUNCE_ATTR k_bounce(const St
    sums COUNT_TILES tiles of one request (reqs[r].count_block = its first) and
    adds them with one atomic per counter. */
 #define COUNT_TILES 64u
+static_assert(COUNT_TILES <= 64, "one lane per tile sums a block's tile times");
 __global__ void __launch_bounds__(256) k_counts(const KReq *__restrict__ reqs, uint32_t nreq,
-                                                const uint32_t *__restrict__ counts) {
+                                                const uint32_t *__restrict__ counts,
+                                                const uint32_t *__restrict__ tile_cost,
+                                                unsigned long long *cost_sum) {
     __shared__ unsigned long long part[2][4];
     uint32_t lo = 0, hi = nreq - 1;
     while (lo < hi) {
@@ -548,6 +558,12 @@ __global__ void __launch_bounds__(256) k_counts(const KReq *__restrict__ reqs, u
     if ((threadIdx.x & 63u) == 0) {
         part[0][threadIdx.x >> 6] = sh;
         part[1][threadIdx.x >> 6] = bo;
+    }
+    /* the same tiles' primary-stage times, for the next launch's order */
+    if (threadIdx.x < 64) {
+        unsigned long long c = t0 + threadIdx.x < t1 ? ldgu(tile_cost, t0 + threadIdx.x) : 0u;
+        for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off, 64);
+        if (threadIdx.x == 0) atomicAdd(cost_sum, c);
     }
     __syncthreads();
     if (threadIdx.x == 0 && q.v.counters) {
@@ -743,6 +759,7 @@ static struct {
     uint32_t *tile_cost;   /* [tile_cap] each tile's time in the last primary stage */
     uint32_t *tile_order;  /* [tile_cap] the primary stage's claim order (k_tile_order) */
     uint32_t tile_cap;
+    unsigned long long *cost_sum; /* sum of tile_cost over the last primary stage (k_counts) */
     uint32_t *tile_counter; /* 64 words zeroed per launch: [0] the persistent primary stage's tile
                                counter, [BOUNCE_CLAIM] the bounce stage's chunk claims (128 B apart) */
     StDScene *scene_table; /* device copy of the scene descriptors of a launch */
@@ -777,6 +794,8 @@ extern "C" int st_init(int device) {
     HIP_TRY(hipSetDevice(device));
     HIP_TRY(hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking));
     HIP_TRY(hipMalloc(&g.tile_counter, 256));
+    HIP_TRY(hipMalloc(&g.cost_sum, 256));
+    HIP_TRY(hipMemset(g.cost_sum, 0, 256));
     for (int i = 0; i < 4; i++) HIP_TRY(hipEventCreate(&g.ev[i]));
     g.tune.tile_shift = 3;          /* 8 x 8 tiles */
     g.tune.blocks_per_cu = 8;
@@ -1403,7 +1422,7 @@ int st_rt_launch_prepared(StSceneHandle *const *scenes, StHostView *hv, uint32_t
         HIP_TRY(hipMemcpyAsync(blk, st, up, hipMemcpyHostToDevice, g.stream));
         hipLaunchKernelGGL(k_zero_u64, dim3((3 * n + 255) / 256), dim3(256), 0, g.stream, cnt_d, 3 * n);
         hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, g.stream, g.tile_counter, 64u);
-        static_assert(BOUNCE_CLAIM < 64, "claim counters share the zeroed 64 words");
+        static_assert(BOUNCE_CLAIM < 64 && ORDER_CLAIM + 1 < 64, "claim counters share the zeroed 64 words");
         if (any_device_ssb)
             hipLaunchKernelGGL(k_ssb, dim3(n), dim3(256), 0, g.stream, (const StDScene *)sc_d, (const KReq *)reqs_d);
         if (levels)
@@ -1411,14 +1430,15 @@ int st_rt_launch_prepared(StSceneHandle *const *scenes, StHostView *hv, uint32_t
         hipLaunchKernelGGL(k_primary_dirs, dim3((uint32_t)(((uint64_t)rows * 3 + DIRS_BLOCK - 1) / DIRS_BLOCK)),
                            dim3(DIRS_BLOCK), 0, g.stream, reqs_d, n, rows);
         HIP_TRY(hipGetLastError());
-        hipLaunchKernelGGL(k_tile_order, dim3(1), dim3(ORDER_THREADS), 0, g.stream, (const uint32_t *)g.tile_cost,
-                           g.tile_order, tiles);
+        hipLaunchKernelGGL(k_tile_order, dim3((tiles + ORDER_THREADS - 1) / ORDER_THREADS), dim3(ORDER_THREADS), 0,
+                           g.stream, (const uint32_t *)g.tile_cost, g.tile_order, tiles, g.cost_sum,
+                           g.tile_counter + ORDER_CLAIM);
         HIP_TRY(hipGetLastError());
         if (!graph) HIP_TRY(hipEventRecord(g.ev[0], g.stream));
         HIP_TRY(mark());
         hipLaunchKernelGGL(k_primary, dim3(blocks), dim3(BLOCK_THREADS), lds_primary, g.stream, (const StDScene *)sc_d,
                            (const KReq *)reqs_d, n, tiles, g.tile_counter, (uint32_t)persistent, xcd_blocks,
-                           s_entries, m_entries, w, (const uint32_t *)g.tile_order, g.tile_cost);
+                           s_entries, m_entries, w, (const uint32_t *)g.tile_order, g.tile_cost, g.cost_sum);
         HIP_TRY(hipGetLastError());
         HIP_TRY(mark());
         /* the rest of the depth loop: the bounce stage reads its queue
@@ -1431,7 +1451,7 @@ int st_rt_launch_prepared(StSceneHandle *const *scenes, StHostView *hv, uint32_t
         }
         if (levels && count_blocks)
             hipLaunchKernelGGL(k_counts, dim3(count_blocks), dim3(256), 0, g.stream, (const KReq *)reqs_d, n,
-                               (const uint32_t *)w.counts);
+                               (const uint32_t *)w.counts, (const uint32_t *)g.tile_cost, g.cost_sum);
         HIP_TRY(hipGetLastError());
         if (!graph) HIP_TRY(hipEventRecord(g.ev[1], g.stream));
         return ST_OK;
