@@ -1,52 +1,50 @@
 #!/usr/bin/env python3
 """Benchmark of the MI355X tracing path (one JSON line on stdout).
 
-Workload (one step = one frame): the single-mesh PBR frame - a seeded
-noise-displaced geodesic icosphere with 200,000 fp32 triangles (frequency
-100), Cook-Torrance materials, a glass sphere, a mirror quad, 4 point lights
-and 2 emissive quads; 1920x1080 window (1921x1081 traced QCAA grid), 1 spp,
-trace depth 4, Beauty mode. The scene is resident in HBM before timing;
-a step is the per-frame host set-up (camera basis, SSB rects, ~20 KB of
-uploads), the primary-direction kernel and the render stages (primary,
-which also runs the first depth-loop level's shading entry, shadow rays and
-shading, then the bounce stage, which traces each bounce ray and runs its
-pixel's further levels in the same lane); the frame stays in HBM
-(the PCIe-inclusive rate is reported separately).
+Headline workload (one step = one launch): the batched request mix - 256
+render requests over 8 resident scenes of 0.2-5M triangles (noise-displaced
+spheres with a Cook-Torrance material, a floor quad, a mirror or glass
+sphere and 3 point lights; mesh BVHs built on the device as Morton LBVHs),
+resolutions log-uniform 128x128..1920x1080, trace depth uniform 1-6, mode
+uniform over Beauty/Depth/Normal/UV, cameras on a sphere around each scene
+(seed 7). A step fuses all 256 requests into one persistent launch: the
+per-request host set-up (camera basis, SSB rects, row starts), one upload,
+and every stage kernel; the frames stay in HBM. `value` = rays of the 256
+requests / wall time of the step.
 
-Also measured in the same run:
-  - plain vs batched launches ("batched"): 256 requests over 8 resident
-    scenes of 0.2-5M triangles (noise-displaced spheres, mesh BVHs built on
-    the device), resolutions log-uniform 128x128..1920x1080, trace depth
-    uniform 1-6, mode uniform over Beauty/Depth/Normal/UV, cameras on a
-    sphere around each scene; first as 256 sequential per-request launches
-    with a blocking FloatPixel download each (the reference's
-    renderSceneOnGPU pattern), then as one persistent launch over one tile
-    list, with FloatPixel downloads and with the window images resolved on
-    the device (BGRA8); the batched frames are checked equal to the plain
-    ones;
-  - instanced ("instanced"): 64 instances of 3 noise-displaced meshes
-    (50k/200k/500k triangles, reference SAH mesh BVHs), 2048^2 Perlin
-    albedo and normal maps with full mip chains, ray-cone angle 0.001,
-    1920x1080, depth 3, Beauty; every frame moves all instances (new random
-    positions and rotations), rebuilds the scene BVH on the device (Morton
-    LBVH) and renders: frame time includes upload, rebuild and render;
-  - serving ("served"): 40 resident scenes of 1-20M triangles (log-uniform,
-    admitted against a 200 GB HBM budget), requests of the same mix arriving
-    as a Poisson process at a stated rate for a stated time, served by the
-    continuous-batching loop; Mrays/s, p50/p99 submit-to-image latency;
-  - roofline of the dominant stage launch (the primary stage, k_primary,
-    which traces the primary rays and the first level's shadow rays): the
-    algorithmic bytes of those rays' traversals from the CPU oracle's
-    per-ray-class event counters on the same 1920x1080 frame (bytes per
-    event at the reference's record sizes), over the launch's duration from
-    HIP events recorded around each stage on the launch stream; the same
-    for the whole frame ("roofline_frame"); the measured HBM traffic per
-    launch comes from the
-    rocprofv3 FETCH_SIZE / WRITE_SIZE summary kept in profiles/ (calibrated
-    against a copy kernel of known size, --copy-probe);
-  - the reference CPU renderer (compiled from the reference's own sources)
-    on the same full frame, all threads used stated;
-  - hit-id parity of the GPU frame against the CPU oracle (every pixel).
+Beside it, in the same run:
+  - plain launches ("batched.plain_*"): the same 256 requests as 256
+    sequential single-request launches of the same kernels, without
+    downloads (like for like with the headline) and with blocking
+    FloatPixel downloads into page-locked memory against the batched launch
+    with the same downloads; the batched frames are checked equal to the
+    plain ones byte for byte;
+  - parity of the headline workload against the CPU oracle: every hit id,
+    triangle id and hit distance of all 256 requests bit for bit, and every
+    request's ray counts;
+  - the reference CPU renderer (compiled from the reference's own sources,
+    or the oracle restatement when that build is absent) on a seeded
+    8-request subset of the same mix, all host cores and one pinned core,
+    with the CPU model;
+  - "frame_c1": the 640x480 implicit-primitive frame (16 seeded quads,
+    boxes, spheres and tetrahedra, 3 point lights, Blinn-Phong, depth 2),
+    GPU and CPU;
+  - "frame_c2": the single-mesh PBR frame (200,000-triangle icosphere,
+    1920x1080, depth 4), with per-stage times, HIP-graph replays and the
+    PCIe-inclusive frame;
+  - "instanced" (C3): 64 instances of 50k/200k/500k-triangle meshes moving
+    every frame, scene LBVH rebuilt on the device per frame, 2048^2 Perlin
+    albedo/normal maps with mips, ray cones, depth 3;
+  - "served" (C5): 40 resident scenes of 1-20M triangles admitted against a
+    200 GB HBM budget, Poisson arrivals of the same request mix served with
+    continuous batching; Mrays/s with p50/p99 submit-to-image latency, and a
+    plain one-request-per-launch baseline on the same scenes;
+  - roofline of the headline step's dominant kernel: algorithmic bytes from
+    the CPU oracle's per-ray-class event counters over all 256 requests
+    (bytes per event at the reference's record sizes) over the kernel's
+    mean duration from HIP events recorded around each stage launch on the
+    launch stream during the timed steps; HBM traffic per launch from the
+    rocprofv3 FETCH_SIZE/WRITE_SIZE summary in profiles/.
 """
 from __future__ import annotations
 
@@ -62,20 +60,37 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 import slimtracin_amd as st  # noqa: E402
-from slimtracin_amd.scene import MODE_BEAUTY, MODE_DEPTH, MODE_NORMALS, MODE_UVS, VIEW_DTYPE, make_view  # noqa: E402
+from slimtracin_amd.scene import MODE_BEAUTY, MODE_DEPTH, MODE_NORMALS, MODE_UVS, make_view  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 PBR_K = 100            # 20 * 100^2 = 200,000 triangles
 PBR_SEED = 2024
 PBR_CAM = dict(position=(0.0, 6.0, -14.0), pitch=-0.15)
+BATCH_TRIANGLES = [200_000, 320_000, 500_000, 800_000, 1_250_000, 2_000_000, 3_200_000, 5_000_000]
+BATCH_SEED = 7
+BATCH_REQUESTS = 256
+CPU_SUBSET = 8
+LEGS = ("c4", "c1", "c2", "c3", "c5", "cpu")
+METRIC = "Mrays/s + p50/p99 frame latency, batched/served vs plain launches, hit-IDs exact"
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def pbr_view(w, h):
-    return make_view(width=w, height=h, mode=MODE_BEAUTY, depth=4, **PBR_CAM)
+def host_cores():
+    return len(os.sched_getaffinity(0))
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def request_mix(rng, nscenes, count):
@@ -102,50 +117,316 @@ def request_mix(rng, nscenes, count):
     return np.concatenate(views), scene_ids
 
 
-def cpu_baseline(blob, view, threads, frames=1):
+def batch_workload():
+    """The headline workload: the 8 scene blobs and the seeded 256-request mix."""
+    blobs = [st.gen_mesh_scene(100 + i, tri) for i, tri in enumerate(BATCH_TRIANGLES)]
+    views, sids = request_mix(np.random.default_rng(BATCH_SEED), len(blobs), BATCH_REQUESTS)
+    return blobs, views, sids
+
+
+def pct(a, q):
+    return round(float(np.percentile(a, q)), 4)
+
+
+# ---------------------------------------------------------------- headline
+
+def c4_leg(args, blobs, views, sids):
+    t0 = time.perf_counter()
+    dscenes = [st.DeviceScene(b, mesh_lbvh=True) for b in blobs]
+    build_s = time.perf_counter() - t0
+    rs = [dscenes[i] for i in sids]
+    st.render_timed(rs, views)                      # grows the launch arena to the whole batch
+    tune, _ = st.autotune(rs, views, reps=1)
+    log(f"c4: {len(dscenes)} scenes built in {build_s:.1f} s; autotune {tune.as_dict()}")
+    for _ in range(args.warmup):
+        st.render_timed(rs, views, stages=True)
+    step_ms, stage_runs, rays = [], [], 0
+    t_start = time.perf_counter()
+    for _ in range(args.steps):
+        f0 = time.perf_counter()
+        _, counts, sms = st.render_timed(rs, views, stages=True)
+        step_ms.append((time.perf_counter() - f0) * 1e3)
+        stage_runs.append(sms)
+        rays += sum(c.total() for c in counts)
+    elapsed = time.perf_counter() - t_start
+    rays_per_step = rays // args.steps
+    names = st.stage_names(max(int(v) for v in views["depth"]))
+    stage_ms = {n: round(float(np.mean([r[i] for r in stage_runs])), 4) for i, n in enumerate(names)}
+    log(f"c4: {rays_per_step} rays per step, {np.median(step_ms):.2f} ms per step, stages {stage_ms}")
+
+    # plain: one launch per request, the same kernels, frames left in HBM
+    lat_plain, plain_rays = [], 0
+    for i in range(len(views)):                     # untimed pass: per-request arenas
+        st.render_timed([rs[i]], views[i:i + 1])
+    tp = time.perf_counter()
+    for i in range(len(views)):
+        r0 = time.perf_counter()
+        _, c = st.render_timed([rs[i]], views[i:i + 1])
+        lat_plain.append((time.perf_counter() - r0) * 1e3)
+        plain_rays += c[0].total()
+    plain_s = time.perf_counter() - tp
+    assert plain_rays == rays_per_step, (plain_rays, rays_per_step)
+
+    # with downloads: every frame into page-locked memory, both ways
+    grid = [(int(v["width"]) + 1) * (int(v["height"]) + 1) for v in views]
+    fp_plain = st.pinned_split(grid, st.PIXEL_DTYPE)
+    fp_batch = st.pinned_split(grid, st.PIXEL_DTYPE)
+    for i in range(len(views)):
+        st.render(rs[i], views[i:i + 1], hits=False, out=fp_plain[i])
+    tq = time.perf_counter()
+    for i in range(len(views)):
+        st.render(rs[i], views[i:i + 1], hits=False, out=fp_plain[i])
+    plain_dl_s = time.perf_counter() - tq
+    st.render_batch(rs, views, persistent=True, outs=fp_batch)
+    tb = time.perf_counter()
+    st.render_batch(rs, views, persistent=True, outs=fp_batch)
+    batch_dl_s = time.perf_counter() - tb
+    same = all(a.tobytes() == b.tobytes() for a, b in zip(fp_plain, fp_batch))
+    del fp_plain, fp_batch
+
+    batched_mrays = rays / elapsed / 1e6
+    out = {
+        "value": batched_mrays,
+        "elapsed": elapsed,
+        "step_ms": step_ms,
+        "rays_per_step": rays_per_step,
+        "stage_ms": stage_ms,
+        "tune": tune,
+        "detail": {
+            "requests": len(views), "scenes": len(dscenes),
+            "scene_triangles": [d.info().triangle_count for d in dscenes],
+            "mesh_bvh": "device Morton LBVH (leaves <= 4)", "scene_build_s": round(build_s, 2),
+            "traced_pixels": int(sum(grid)), "rays": int(rays_per_step),
+            "batched_mrays": round(batched_mrays, 2),
+            "plain_mrays": round(plain_rays / plain_s / 1e6, 2),
+            "speedup_vs_plain": round(plain_s / (elapsed / args.steps), 3),
+            "batched_ms_per_step": round(elapsed / args.steps * 1e3, 3),
+            "plain_wall_ms": round(plain_s * 1e3, 2),
+            "plain_request_latency_p50_ms": pct(lat_plain, 50),
+            "plain_request_latency_p99_ms": pct(lat_plain, 99),
+            "batched_latency_p50_ms": pct(step_ms, 50), "batched_latency_p99_ms": pct(step_ms, 99),
+            "with_downloads": {
+                "output": "FloatPixel grids into page-locked host memory, both legs",
+                "plain_mrays": round(plain_rays / plain_dl_s / 1e6, 2),
+                "batched_mrays": round(rays_per_step / batch_dl_s / 1e6, 2),
+                "speedup_vs_plain": round(plain_dl_s / batch_dl_s, 3),
+            },
+            "batched_equals_plain": same,
+            "launch_shape": tune.as_dict(),
+        },
+    }
+    out["_scenes"] = dscenes
+    out["_rs"] = rs
+    return out
+
+
+def c4_parity_and_bytes(args, blobs, views, sids, rs):
+    """Every request of the headline mix on the CPU oracle: hit records and
+    ray counts against the GPU's, and the per-class event counts that price
+    the launch's algorithmic bytes."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import checkers as ck
-    pixels, hits, stats = ck.oracle_render(blob, view, threads=threads)
-    rays = stats.primary_rays + stats.shadow_rays + stats.bounce_rays
+    threads = host_cores()
+    t0 = time.perf_counter()
+    agg = ck.OracleStats()
+    mism = {"prim_id": 0, "triangle_id": 0, "distance": 0, "counts": 0}
+    for s in range(len(blobs)):
+        idx = [i for i in range(len(views)) if sids[i] == s]
+        if not idx:
+            continue
+        osc = ck.OracleScene(blobs[s], mesh_lbvh=True)
+        _, ghits, gcounts = st.render_batch([rs[i] for i in idx], views[idx], hits=True, persistent=True)
+        for k, i in enumerate(idx):
+            _, oh, ost, _ = osc.render(views[i:i + 1], threads=threads, pixels=False)
+            gh = ghits[k]
+            mism["prim_id"] += int((gh["prim_id"] != oh["prim_id"]).sum())
+            mism["triangle_id"] += int((gh["triangle_id"] != oh["triangle_id"]).sum())
+            mism["distance"] += int((gh["distance"].view(np.uint32) != oh["distance"].view(np.uint32)).sum())
+            c = gcounts[k]
+            if (c.primary, c.shadow, c.bounce) != (ost.primary_rays, ost.shadow_rays, ost.bounce_rays):
+                mism["counts"] += 1
+            for f, _ in ck.OracleStats._fields_:
+                if f == "class_events":
+                    for a in range(4):
+                        for b in range(4):
+                            agg.class_events[a][b] += ost.class_events[a][b]
+                else:
+                    setattr(agg, f, getattr(agg, f) + getattr(ost, f))
+        osc.close()
+        del ghits
+    log(f"c4 parity: {mism} ({time.perf_counter() - t0:.1f} s, {threads} threads)")
+    return mism, agg
+
+
+def cpu_subset_baseline(blobs, views, sids):
+    """The reference CPU renderer on the first CPU_SUBSET requests of the
+    headline mix: all host cores (rows interleaved, one trace state per
+    thread), median of 3 frames, and one pinned core, one frame."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import checkers as ck
+    from slimtracin_amd.scene import blob_mesh_info, blob_with_prebuilt_meshes
     kind = "reference" if ck.ref_available() else "port"
-    if kind == "reference":
-        _, secs = ck.ref_render_timed(blob, view, threads, frames)
+    cores = host_cores()
+    sub = list(range(CPU_SUBSET))
+    rays, secs_all, secs_one = 0, 0.0, 0.0
+    prebuilt = {}
+    for i in sub:
+        s = sids[i]
+        v = views[i:i + 1]
+        if kind == "reference":
+            if s not in prebuilt:
+                info = blob_mesh_info(blobs[s])
+                prebuilt[s] = blob_with_prebuilt_meshes(
+                    blobs[s], [ck.oracle_mesh_lbvh(blobs[s], m, mi["triangle_count"]) for m, mi in enumerate(info)])
+            osc = ck.OracleScene(prebuilt[s])
+            _, _, ost, _ = osc.render(v, threads=cores, hits=False, pixels=False)
+            osc.close()
+            _, sa = ck.ref_render_timed(prebuilt[s], v, cores, 3)
+            saved = os.sched_getaffinity(0)
+            os.sched_setaffinity(0, {min(saved)})
+            try:
+                _, s1 = ck.ref_render_timed(prebuilt[s], v, 1, 1)
+            finally:
+                os.sched_setaffinity(0, saved)
+        else:
+            osc = ck.OracleScene(blobs[s], mesh_lbvh=True)
+            ts = []
+            for _ in range(3):
+                _, _, ost, t = osc.render(v, threads=cores, hits=False, pixels=False)
+                ts.append(t)
+            sa = ts
+            saved = os.sched_getaffinity(0)
+            os.sched_setaffinity(0, {min(saved)})
+            try:
+                _, _, _, t1 = osc.render(v, threads=1, hits=False, pixels=False)
+            finally:
+                os.sched_setaffinity(0, saved)
+            s1 = [t1]
+            osc.close()
+        rays += ost.primary_rays + ost.shadow_rays + ost.bounce_rays
+        secs_all += float(np.median(sa))
+        secs_one += float(np.median(s1))
+    return {
+        "value": rays / secs_all / 1e6, "unit": "Mrays/s", "cores": cores, "kind": kind,
+        "sample": f"the first {CPU_SUBSET} requests of the seeded 256-request mix ({rays} rays), "
+                  f"{cores} threads, rows interleaved, median of 3 frames each: {secs_all:.2f} s",
+        "single_core_mrays": round(rays / secs_one / 1e6, 3),
+        "single_core_s": round(secs_one, 2),
+        "cpu_model": cpu_model(),
+    }
+
+
+def frame_cpu(blob, view, frames=3):
+    """The reference CPU renderer (or the oracle) on one frame: all cores and
+    one pinned core."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import checkers as ck
+    cores = host_cores()
+    osc = ck.OracleScene(blob)
+    _, _, ost, _ = osc.render(view, threads=cores, hits=False, pixels=False)
+    osc.close()
+    rays = ost.primary_rays + ost.shadow_rays + ost.bounce_rays
+    if ck.ref_available():
+        _, sa = ck.ref_render_timed(blob, view, cores, frames)
+        kind = "reference"
     else:
-        _, secs, _ = ck.oracle_render_timed(blob, view, threads, frames)
-    t = float(np.median(secs))
-    return dict(value=rays / t / 1e6, unit="Mrays/s", cores=threads, kind=kind,
-                sample=f"the same {int(view['width'][0])}x{int(view['height'][0])} Beauty frame, "
-                       f"{threads} threads, median of {frames} frame(s), {rays} rays, {t:.2f} s/frame"), stats, hits
-
-
-TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "k_primary_pbr200k_1920x1080.txt")
-
-
-def measured_traffic():
-    """HBM bytes per primary-stage launch from the committed rocprofv3 PMC
-    summary (line 'traffic_bytes_per_launch: N'), or None when absent."""
+        _, sa, _ = ck.oracle_render_timed(blob, view, cores, frames)
+        kind = "port"
+    saved = os.sched_getaffinity(0)
+    os.sched_setaffinity(0, {min(saved)})
     try:
-        with open(TRAFFIC_SUMMARY) as f:
-            for line in f:
-                if line.startswith("traffic_bytes_per_launch:"):
-                    return float(line.split(":", 1)[1])
-    except OSError:
-        pass
-    return None
+        if kind == "reference":
+            _, s1 = ck.ref_render_timed(blob, view, 1, 1)
+        else:
+            _, s1, _ = ck.oracle_render_timed(blob, view, 1, 1)
+    finally:
+        os.sched_setaffinity(0, saved)
+    return {"kind": kind, "cores": cores, "mrays": round(rays / float(np.median(sa)) / 1e6, 3),
+            "single_core_mrays": round(rays / float(np.median(s1)) / 1e6, 3), "rays": int(rays)}, ost
 
 
-def copy_probe(mib=512, reps=5):
-    """A plain device copy of known size (elementwise negation, reads and
-    writes `mib` MiB per launch) for calibrating rocprofv3's FETCH_SIZE and
-    WRITE_SIZE counters."""
-    import torch
-    n = mib * 2**20 // 4
-    a = torch.ones(n, dtype=torch.float32, device="cuda")
-    b = torch.empty_like(a)
-    for i in range(reps):
-        torch.neg(a, out=b)
-        torch.cuda.synchronize()
-        print(f"copy probe {i}: {mib} MiB read, {mib} MiB written", flush=True)
+# ---------------------------------------------------------------- frames
+
+def frame_leg(scene, view, steps, warmup, label):
+    for _ in range(warmup):
+        st.render_timed([scene], view)
+    kms, frame_ms, rays = [], [], 0
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        f0 = time.perf_counter()
+        ms, counts = st.render_timed([scene], view)
+        frame_ms.append((time.perf_counter() - f0) * 1e3)
+        kms.append(ms)
+        rays += counts[0].total()
+    elapsed = time.perf_counter() - t0
+    out = {"rays_per_frame": int(rays // steps), "mrays": round(rays / elapsed / 1e6, 2),
+           "frame_p50_ms": pct(frame_ms, 50), "frame_p99_ms": pct(frame_ms, 99),
+           "render_ms": round(float(np.mean(kms)), 4)}
+    log(f"{label}: {out}")
+    return out
+
+
+def c1_leg(args):
+    blob = st.gen_implicit(1)
+    scene = st.DeviceScene(blob)
+    view = make_view(width=640, height=480, mode=MODE_BEAUTY, depth=2, position=(0.0, 7.0, -11.0), pitch=-0.2)
+    out = frame_leg(scene, view, max(args.steps, 20), 3, "c1")
+    out["workload"] = "16 seeded implicit primitives (4 quads, 4 boxes, 4 spheres, 4 tetrahedra), 3 point lights, " \
+                      "Blinn-Phong, 640x480 (641x481 traced), depth 2, Beauty"
+    if "cpu" in args.legs:
+        cpu, ost = frame_cpu(blob, view)
+        assert ost.primary_rays + ost.shadow_rays + ost.bounce_rays == out["rays_per_frame"]
+        out["cpu"] = cpu
+        out["gpu_vs_cpu_all_cores"] = round(out["mrays"] / cpu["mrays"], 1)
+    scene.close()
+    return out
+
+
+def c2_leg(args):
+    t0 = time.time()
+    blob = st.gen_pbr(PBR_SEED, PBR_K)
+    scene = st.DeviceScene(blob)
+    info = scene.info()
+    log(f"c2: {info.triangle_count} triangles, mesh BVH height {info.max_mesh_height}, "
+        f"{info.device_bytes / 2**20:.1f} MiB in HBM, built in {time.time() - t0:.1f} s")
+    view = make_view(width=1920, height=1080, mode=MODE_BEAUTY, depth=4, **PBR_CAM)
+    saved = st.get_tuning()
+    tune, _ = st.autotune([scene], view)
+    out = frame_leg(scene, view, max(args.steps, 20), 3, "c2")
+    stage_runs = [st.render_timed([scene], view, stages=True)[2] for _ in range(5)]
+    names = st.stage_names(4)
+    out["stage_ms"] = {n: round(float(np.median([r[i] for r in stage_runs])), 4) for i, n in enumerate(names)}
+    c0, l0 = st.graph_stats()
+    for _ in range(2):
+        st.render_timed([scene], view, graph=True)
+    g_ms = []
+    for _ in range(10):
+        f0 = time.perf_counter()
+        st.render_timed([scene], view, graph=True)
+        g_ms.append((time.perf_counter() - f0) * 1e3)
+    c1, l1 = st.graph_stats()
+    out["graph"] = {"frame_ms_graph_p50": pct(g_ms, 50), "captures": c1 - c0, "replays": l1 - l0 - (c1 - c0)}
+    t1 = time.perf_counter()
+    for _ in range(3):
+        st.render(scene, view, hits=False)
+    out["frame_ms_with_pageable_download"] = round((time.perf_counter() - t1) / 3 * 1e3, 3)
+    out["launch_shape"] = tune.as_dict()
+    out["workload"] = "200,000-triangle noise-displaced icosphere (reference SAH mesh BVH), glass sphere, mirror " \
+                      "quad, 4 point + 2 emissive quad lights, 1920x1080 (1921x1081 traced), depth 4, Beauty"
+    if "cpu" in args.legs:
+        cpu, ost = frame_cpu(blob, view, frames=1)
+        out["cpu"] = cpu
+        out["gpu_vs_cpu_all_cores"] = round(out["mrays"] / cpu["mrays"], 1)
+        _, gh, _ = st.render(scene, view)
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import checkers as ck
+        _, oh, _ = ck.oracle_render(blob, view, threads=host_cores())
+        out["hit_ids_exact"] = bool((gh["prim_id"] == oh["prim_id"]).all() and
+                                    (gh["triangle_id"] == oh["triangle_id"]).all() and
+                                    (gh["distance"].view(np.uint32) == oh["distance"].view(np.uint32)).all())
+    st.set_tuning(saved.tile_shift, saved.blocks_per_cu, bool(saved.persistent))
+    scene.close()
+    return out
 
 
 def instanced_leg(args):
@@ -191,80 +472,11 @@ def instanced_leg(args):
         "texture": "2 x 2048^2 Perlin (albedo, normal), 10 mips", "cone_angle": 0.001, "depth": 3,
         "scene_build_s": round(build_s, 2), "frames": len(frame_ms),
         "rays_per_frame": int(rays / len(frame_ms)), "mrays": round(rays / wall / 1e6, 2),
-        "frame_p50_ms": round(float(np.percentile(frame_ms, 50)), 3),
-        "frame_p99_ms": round(float(np.percentile(frame_ms, 99)), 3),
+        "frame_p50_ms": pct(frame_ms, 50), "frame_p99_ms": pct(frame_ms, 99),
+        "frame_max_ms": round(float(np.max(frame_ms)), 3),
         "lbvh_rebuild_ms": round(float(np.median(rebuild_ms)), 4),
         "render_ms": round(float(np.median(kernel_ms)), 3),
     }
-
-
-BATCH_TRIANGLES = [200_000, 320_000, 500_000, 800_000, 1_250_000, 2_000_000, 3_200_000, 5_000_000]
-
-
-def batched_leg(args):
-    rng = np.random.default_rng(7)
-    dscenes = []
-    t0 = time.perf_counter()
-    for i, tri in enumerate(BATCH_TRIANGLES):
-        dscenes.append(st.DeviceScene(st.gen_mesh_scene(100 + i, tri), mesh_lbvh=True))
-    build_s = time.perf_counter() - t0
-    views, sids = request_mix(rng, len(dscenes), 256)
-    req_scenes = [dscenes[i] for i in sids]
-    st.render_timed(req_scenes, views)          # warm-up: grows the launch arena to the whole batch
-    tune, _ = st.autotune(req_scenes, views, reps=1)
-    lat_plain, plain_rays, plain_pix = [], 0, []
-    tp = time.perf_counter()
-    for i in range(len(views)):
-        r0 = time.perf_counter()
-        px, _, c = st.render(req_scenes[i], views[i:i + 1], hits=False)
-        lat_plain.append((time.perf_counter() - r0) * 1e3)
-        plain_rays += c.total()
-        plain_pix.append(px)
-    plain_s = time.perf_counter() - tp
-    # page-locked outputs, allocated once outside the timed regions (as the
-    # serving loop's buffer pool does)
-    grid = [(int(v["width"]) + 1) * (int(v["height"]) + 1) for v in views]
-    win = [int(v["width"]) * int(v["height"]) for v in views]
-    fp_out = st.pinned_split(grid, st.PIXEL_DTYPE)
-    bgra_out = st.pinned_split(win, np.uint32)
-    tq = time.perf_counter()
-    for i in range(len(views)):
-        st.render(req_scenes[i], views[i:i + 1], hits=False, out=fp_out[i])
-    plain_pinned_s = time.perf_counter() - tq
-    tb = time.perf_counter()
-    bpix, _, cb = st.render_batch(req_scenes, views, persistent=True, outs=fp_out)
-    batch_s = time.perf_counter() - tb
-    tr = time.perf_counter()
-    _, cr = st.render_batch_bgra8(req_scenes, views, persistent=True, outs=bgra_out)
-    served_s = time.perf_counter() - tr
-    kernel_ms, _ = st.render_timed(req_scenes, views)
-    batch_rays = sum(c.total() for c in cb)
-    assert batch_rays == plain_rays == sum(c.total() for c in cr), (batch_rays, plain_rays)
-    same = all(a.tobytes() == b.tobytes() for a, b in zip(plain_pix, bpix))
-    out = {
-        "requests": len(views), "scenes": len(dscenes),
-        "scene_triangles": [d.info().triangle_count for d in dscenes],
-        "mesh_bvh": "device Morton LBVH (leaves <= 4)", "scene_build_s": round(build_s, 2),
-        "traced_pixels": int(sum((int(v["width"]) + 1) * (int(v["height"]) + 1) for v in views)),
-        "rays": int(batch_rays),
-        "plain_mrays": round(plain_rays / plain_s / 1e6, 2),
-        "plain_pinned_mrays": round(plain_rays / plain_pinned_s / 1e6, 2),
-        "batched_mrays": round(batch_rays / batch_s / 1e6, 2),
-        "batched_bgra8_mrays": round(batch_rays / served_s / 1e6, 2),
-        "speedup": round(plain_s / batch_s, 3),
-        "speedup_bgra8": round(plain_s / served_s, 3),
-        "plain_wall_ms": round(plain_s * 1e3, 2),
-        "batched_wall_ms": round(batch_s * 1e3, 2),
-        "batched_bgra8_wall_ms": round(served_s * 1e3, 2),
-        "plain_latency_p50_ms": round(float(np.percentile(lat_plain, 50)), 3),
-        "plain_latency_p99_ms": round(float(np.percentile(lat_plain, 99)), 3),
-        "batched_equals_plain": same,
-        "batched_render_ms": round(kernel_ms, 3),
-        "launch_shape": tune.as_dict(),
-    }
-    for d in dscenes:
-        d.close()
-    return out
 
 
 def served_leg(args):
@@ -288,23 +500,26 @@ def served_leg(args):
             log(f"served: {i + 1}/{len(tri_counts)} scenes admitted-or-rejected, {time.perf_counter() - t0:.1f} s")
     build_s = time.perf_counter() - t0
 
-    def burst(n):
-        vs, ps = request_mix(rng, len(ids), n)
+    def burst(vs, ps):
         tb = time.perf_counter()
-        ts = [srv.submit(ids[ps[k]], vs[k:k + 1]) for k in range(n)]
+        ts = [srv.submit(ids[ps[k]], vs[k:k + 1]) for k in range(len(vs))]
+        r = 0
         for t in ts:
-            srv.wait(t, copy=False)
+            _, c, _ = srv.wait(t, copy=False)
+            r += c.primary + c.shadow + c.bounce
             srv.release(t)
-        return n / (time.perf_counter() - tb)
+        return len(vs) / (time.perf_counter() - tb), r / (time.perf_counter() - tb) / 1e6
 
-    burst(64)                                       # warm-up
-    capacity = burst(512)                           # requests/s with a full queue
-    log(f"served: burst stats {srv.stats()}")
-    # under Poisson arrivals the loop runs smaller batches than a full queue
-    # gives, so the stated rate is kept well below the burst capacity
-    rate = args.serve_rate if args.serve_rate > 0 else round(0.35 * capacity, 1)
-    log(f"served: {len(ids)} scenes resident in {build_s:.1f} s; burst capacity {capacity:.0f} req/s; "
-        f"Poisson rate {rate} req/s for {args.serve_seconds} s")
+    wv, wp = request_mix(rng, len(ids), 64)
+    burst(wv, wp)                                    # warm-up
+    bv, bp = request_mix(rng, len(ids), 512)
+    capacity, burst_mrays = burst(bv, bp)            # requests/s with a full queue
+    # the same 512 requests as plain launches, one request per launch,
+    # through the same server (max one request per launch, one in flight)
+    plain = srv.plain_rate(bv, [ids[k] for k in bp])
+    log(f"served: burst {capacity:.0f} req/s, {burst_mrays:.0f} Mrays/s; plain {plain}")
+    rate = args.serve_rate if args.serve_rate > 0 else round(0.5 * capacity, 1)
+    log(f"served: {len(ids)} scenes resident in {build_s:.1f} s; Poisson rate {rate} req/s for {args.serve_seconds} s")
     stats0 = srv.stats()
     n = max(1, int(rate * args.serve_seconds))
     views, picks = request_mix(rng, len(ids), n)
@@ -324,9 +539,7 @@ def served_leg(args):
             srv.release(tickets[k])
             if time.perf_counter() - last > 2.0:
                 last = time.perf_counter()
-                st_ = srv.stats()
-                log(f"served: {k + 1}/{n} requests done; launches {st_['launches']}, kernel {st_['kernel_ms']:.0f} ms, "
-                    f"launch wall {st_['launch_ms']:.0f} ms, prep {st_['prep_ms']:.0f} ms, alloc {st_['alloc_ms']:.0f} ms")
+                log(f"served: {k + 1}/{n} requests done")
 
     th = threading.Thread(target=collector)
     th.start()
@@ -347,34 +560,67 @@ def served_leg(args):
         "scene_triangles_total": int(sum(admitted_tris)), "hbm_budget_gb": 200,
         "resident_gb": round(stats["resident_bytes"] / 1e9, 2), "scene_build_s": round(build_s, 1),
         "mesh_bvh": "device Morton LBVH (leaves <= 4)", "output": "BGRA8 window image (device resolve)",
-        "burst_capacity_req_per_s": round(capacity, 1), "arrival_rate_per_s": rate, "requests": n,
+        "burst_requests": 512, "burst_capacity_req_per_s": round(capacity, 1), "burst_mrays": round(burst_mrays, 2),
+        "plain_burst": plain,
+        "arrival_rate_per_s": rate, "requests": n,
         "wall_s": round(wall, 3), "mrays": round(rays[0] / wall / 1e6, 2),
-        "latency_p50_ms": round(float(np.percentile(lat, 50)), 3),
-        "latency_p99_ms": round(float(np.percentile(lat, 99)), 3),
+        "latency_p50_ms": pct(lat, 50), "latency_p99_ms": pct(lat, 99),
         "latency_max_ms": round(float(np.max(lat)), 3),
         "launches": int(stats["launches"] - stats0["launches"]),
-        "server_ms": {k: round(stats[k] - stats0[k], 1) for k in ("prep_ms", "launch_ms", "kernel_ms", "alloc_ms")},
         "mean_batch": round((stats["completed"] - stats0["completed"]) / max(1, stats["launches"] - stats0["launches"]), 2),
+        "max_in_flight": stats.get("max_in_flight"),
     }
+
+
+# ---------------------------------------------------------------- roofline
+
+TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "k_primary_batched256_c4.txt")
+
+
+def measured_traffic():
+    """HBM bytes per launch of the headline step's dominant kernel from the
+    committed rocprofv3 PMC summary (line 'traffic_bytes_per_launch: N'),
+    or None when absent."""
+    try:
+        with open(TRAFFIC_SUMMARY) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
+def copy_probe(mib=512, reps=5):
+    """A plain device copy of known size (elementwise negation, reads and
+    writes `mib` MiB per launch) for calibrating rocprofv3's FETCH_SIZE and
+    WRITE_SIZE counters."""
+    import torch
+    n = mib * 2**20 // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for i in range(reps):
+        torch.neg(a, out=b)
+        torch.cuda.synchronize()
+        print(f"copy probe {i}: {mib} MiB read, {mib} MiB written", flush=True)
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and oracle legs")
-    ap.add_argument("--no-batch", action="store_true", help="skip the plain-vs-batched leg")
-    ap.add_argument("--cpu-frames", type=int, default=3, help="frames of the CPU baseline (median)")
+    ap.add_argument("--steps", type=int, default=10, help="timed batched launches of the headline workload")
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--legs", default=",".join(LEGS),
+                    help="comma list of " + ",".join(LEGS) + " (c4 is the headline; cpu adds the CPU baselines "
+                         "and the oracle parity of the headline)")
     ap.add_argument("--copy-probe", action="store_true", help="only run the copy kernel used to calibrate PMC bytes")
-    ap.add_argument("--no-serve", action="store_true", help="skip the served (Poisson arrivals) leg")
-    ap.add_argument("--no-instanced", action="store_true", help="skip the instanced (per-frame LBVH) leg")
     ap.add_argument("--instanced-frames", type=int, default=10, help="instanced leg: timed frames")
     ap.add_argument("--serve-rate", type=float, default=0.0,
-                    help="served leg: Poisson arrival rate, requests/s (0: 35%% of the measured burst capacity)")
+                    help="served leg: Poisson arrival rate, requests/s (0: half the measured burst capacity)")
     ap.add_argument("--serve-seconds", type=float, default=8.0, help="served leg: arrival window, s")
     ap.add_argument("--serve-scenes", type=int, default=40, help="served leg: resident scenes requested")
     args = ap.parse_args()
+    args.legs = [x for x in args.legs.split(",") if x]
     if args.copy_probe:
         copy_probe()
         return
@@ -383,128 +629,72 @@ def main():
               "is not supported (the multi-GPU design routes whole requests to the GPU that holds their scene; "
               "no per-frame collective exists to measure).", file=sys.stderr)
         sys.exit(2)
-
     st.init(0)
-    t0 = time.time()
-    blob = st.gen_pbr(PBR_SEED, PBR_K)
-    scene = st.DeviceScene(blob)
-    info = scene.info()
-    log(f"scene: {info.triangle_count} triangles, mesh BVH height {info.max_mesh_height}, "
-        f"{info.device_bytes / 2**20:.1f} MiB in HBM, built in {time.time() - t0:.1f} s")
-    view = pbr_view(1920, 1080)
-    tune, tune_ms = st.autotune([scene], view)
-    log(f"autotune: {tune.as_dict()} ({tune_ms:.3f} ms)")
-
-    for _ in range(args.warmup):
-        st.render_timed([scene], view)
-    kms, frame_ms, rays = [], [], 0
-    t_start = time.perf_counter()
-    for _ in range(args.steps):
-        f0 = time.perf_counter()
-        ms, counts = st.render_timed([scene], view)
-        frame_ms.append((time.perf_counter() - f0) * 1e3)
-        kms.append(ms)
-        rays += counts[0].total()
-    elapsed = time.perf_counter() - t_start
-    rays_per_frame = rays / args.steps
-    mrays = rays / elapsed / 1e6
-    kernel_ms = float(np.mean(kms))
-    log(f"frame: {rays_per_frame / 1e6:.2f} Mrays, render stages {kernel_ms:.3f} ms, "
-        f"frame {np.median(frame_ms):.3f} ms, {mrays:.1f} Mrays/s")
-
-    # per-stage durations (events between the stage launches), after the
-    # timed region so that the extra events do not touch the headline figure
-    stage_runs = []
-    for _ in range(max(3, min(args.steps, 10))):
-        _, _, sms = st.render_timed([scene], view, stages=True)
-        stage_runs.append(sms)
-    stage_ms = [float(np.median([r[i] for r in stage_runs])) for i in range(len(stage_runs[0]))]
-    names = st.stage_names(int(view["depth"][0]))
-    stages = {n: round(m, 4) for n, m in zip(names, stage_ms)}
-    log("stages (ms):", stages)
-
-    # the same frames as HIP-graph replays (the launch captured once)
-    c0, l0 = st.graph_stats()
-    for _ in range(2):
-        st.render_timed([scene], view, graph=True)
-    g_ms = []
-    for _ in range(args.steps):
-        f0 = time.perf_counter()
-        st.render_timed([scene], view, graph=True)
-        g_ms.append((time.perf_counter() - f0) * 1e3)
-    c1, l1 = st.graph_stats()
-    graph = {"frame_ms_eager_p50": round(float(np.median(frame_ms)), 4),
-             "frame_ms_graph_p50": round(float(np.median(g_ms)), 4),
-             "captures": c1 - c0, "replays": l1 - l0 - (c1 - c0)}
-    log("graph:", graph)
-
-    # PCIe-inclusive: a plain frame with the blocking download (renderSceneOnGPU)
-    t1 = time.perf_counter()
-    for _ in range(3):
-        st.render(scene, view, hits=False)
-    pcie_frame_ms = (time.perf_counter() - t1) / 3 * 1e3
-
+    blobs, views, sids = batch_workload()
+    c4 = c4_leg(args, blobs, views, sids)
     result = {
-        "metric": "Mrays/s + p50/p99 frame latency, batched/served vs plain launches, hit-IDs exact",
-        "value": round(mrays, 2),
+        "metric": METRIC,
+        "value": round(c4["value"], 2),
         "unit": "Mrays/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(elapsed / args.steps * 1e3, 4),
+        "ms_per_step": round(c4["elapsed"] / args.steps * 1e3, 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": "pbr_icosphere_200k_tris_1920x1080_depth4_beauty", "triangles": info.triangle_count,
-                   "traced_pixels": 1921 * 1081, "rays_per_frame": int(rays_per_frame), "spp": 1},
-        "p50_ms": round(float(np.percentile(frame_ms, 50)), 4),
-        "p99_ms": round(float(np.percentile(frame_ms, 99)), 4),
-        "frame_ms_with_download": round(pcie_frame_ms, 4),
-        "render_ms": round(kernel_ms, 4),
-        "stage_ms": stages,
-        "graph": graph,
-        "launch_shape": tune.as_dict(),
+        "config": {"workload": "batched_256_requests_8_scenes_0.2-5M_tris_mixed_res_depth1-6_4modes",
+                   "requests": BATCH_REQUESTS, "scenes": len(BATCH_TRIANGLES),
+                   "scene_triangles": BATCH_TRIANGLES, "rays_per_step": c4["rays_per_step"],
+                   "seed": BATCH_SEED, "spp": 1},
+        "p50_ms": pct(c4["step_ms"], 50),
+        "p99_ms": pct(c4["step_ms"], 99),
+        "stage_ms": c4["stage_ms"],
+        "batched": c4["detail"],
     }
-
-    if not args.no_instanced:
-        result["instanced"] = instanced_leg(args)
-        log("instanced:", result["instanced"])
-    if not args.no_batch:
-        result["batched"] = batched_leg(args)
-        log("batched:", result["batched"])
-    if not args.no_serve:
-        result["served"] = served_leg(args)
-        log("served:", result["served"])
-
-    if not args.no_cpu:
-        threads = min(os.cpu_count() or 1, 16)
-        cpu, stats, ohits = cpu_baseline(blob, view, threads, frames=args.cpu_frames)
+    if "cpu" in args.legs:
+        mism, agg = c4_parity_and_bytes(args, blobs, views, sids, c4["_rs"])
+        result["hit_ids_exact"] = mism["prim_id"] == 0 and mism["triangle_id"] == 0 and mism["distance"] == 0
+        result["ray_counts_exact"] = mism["counts"] == 0
+        result["parity_mismatches"] = mism
+        cpu = cpu_subset_baseline(blobs, views, sids)
         result["cpu_baseline"] = cpu
         log("cpu baseline:", cpu)
-        oracle_rays = stats.primary_rays + stats.shadow_rays + stats.bounce_rays
-        assert oracle_rays == int(rays_per_frame), (oracle_rays, rays_per_frame)
-        frame_bytes = stats.algorithmic_bytes()
-
-        def line(nbytes, ms):
-            a = nbytes / (ms * 1e-3) / 1e9
-            return {"achieved": round(a, 2), "frac": round(a / HBM_PEAK_GBS, 5),
-                    "algorithmic_bytes_per_launch": int(nbytes), "ms": round(ms, 4)}
-
-        primary = line(stats.traversal_bytes("primary") + stats.traversal_bytes("shadow0"), stages["primary"])
-        result["roofline"] = {"bound": "hbm", "achieved": primary["achieved"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                              "frac": primary["frac"], "traffic": measured_traffic(),
-                              "kernel": "k_primary (primary rays and the first level's shadow rays)",
-                              "algorithmic_bytes_per_launch": primary["algorithmic_bytes_per_launch"],
-                              "launch_ms": primary["ms"]}
-        result["roofline_frame"] = dict(line(frame_bytes, kernel_ms), bytes_per_ray=round(frame_bytes / oracle_rays, 2))
-        gp, gh, gc = st.render(scene, view)
-        same = bool((gh["prim_id"] == ohits["prim_id"]).all() and (gh["triangle_id"] == ohits["triangle_id"]).all()
-                    and (gh["distance"].view(np.uint32) == ohits["distance"].view(np.uint32)).all())
-        result["hit_ids_exact"] = same
-        result["gpu_vs_cpu"] = round(mrays / cpu["value"], 1)
-    scene.close()
+        result["gpu_vs_cpu"] = round(result["value"] / cpu["value"], 1)
+        result["gpu_vs_cpu_single_core"] = round(result["value"] / cpu["single_core_mrays"], 1)
+        stage = c4["stage_ms"]
+        dominant = max(stage, key=stage.get)
+        if dominant == "primary":
+            nbytes = agg.traversal_bytes("primary") + agg.traversal_bytes("shadow0")
+            kname = "k_primary (primary rays and the first level's shadow rays, all 256 requests)"
+        else:
+            nbytes = agg.traversal_bytes("bounce") + agg.traversal_bytes("shadow_deep")
+            kname = "k_bounce (bounce rays and the deeper levels' shadow rays, all 256 requests)"
+        achieved = nbytes / (stage[dominant] * 1e-3) / 1e9
+        result["roofline"] = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                              "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": measured_traffic(),
+                              "kernel": kname, "algorithmic_bytes_per_launch": int(nbytes),
+                              "launch_ms": stage[dominant]}
+        total = agg.algorithmic_bytes()
+        step_s = c4["elapsed"] / args.steps
+        result["roofline_step"] = {"algorithmic_bytes": int(total), "bytes_per_ray": round(total / c4["rays_per_step"], 2),
+                                   "achieved": round(total / step_s / 1e9, 2),
+                                   "frac": round(total / step_s / 1e9 / HBM_PEAK_GBS, 5)}
+    for d in c4["_scenes"]:
+        d.close()
+    del c4
+    if "c1" in args.legs:
+        result["frame_c1"] = c1_leg(args)
+    if "c2" in args.legs:
+        result["frame_c2"] = c2_leg(args)
+    if "c3" in args.legs:
+        result["instanced"] = instanced_leg(args)
+        log("instanced:", result["instanced"])
+    if "c5" in args.legs:
+        result["served"] = served_leg(args)
+        log("served:", result["served"])
     print(json.dumps(result), flush=True)
 
 

@@ -167,6 +167,14 @@ int st_render_timed(StSceneHandle *const *scenes, const StView *views, uint32_t 
    instrumentation). */
 int st_last_stage_ms(float *ms, uint32_t cap, uint32_t *count);
 
+/* Test hook: every traversal of later launches stops after `cap` steps
+   (0: only the bound no valid tree reaches, the node count). A traversal
+   that stops this way makes its launch fail with ST_ERR_SCENE instead of
+   looping; tests use a tiny cap to exercise that path. Not in the
+   reference, whose traversal loops have no bound (trace.h:8-76,
+   mesh.h:38-111). */
+int st_debug_step_cap(uint32_t cap);
+
 /* Graphs captured and graph launches made so far (ST_BATCH_GRAPH). Not in
    the reference (the reference issues one blocking launch per frame). */
 int st_graph_stats(uint64_t *captures, uint64_t *launches);

@@ -1861,6 +1861,7 @@ typedef struct {
     float cone_angle;
     StFloatPixel *pixels;
     StHitRec *hits;
+    uint64_t *steps;       /* optional: per pixel, node pairs + triangle tests + primitive visits */
     int tid, nthreads;
     StOracleStats st;
 } OJob;
@@ -1885,8 +1886,11 @@ static void *render_rows(void *arg) {
                 tr->closest_hit.distance = tr->closest_hit.distance_squared = INFINITY;
                 tr->closest_hit.cone_angle = j->cone_angle;
                 tr->closest_hit.cone_width = 0;
+                const uint64_t e0 = tr->st.node_pairs + tr->st.triangle_tests + tr->st.primitive_visits;
                 ray_trace(&ray, tr, j->sc, j->mode, j->pixels + row + x, j->hits ? j->hits + row + x : NULL, x, y,
                           j->cam->position, j->cam->rotation_inverted);
+                if (j->steps)
+                    j->steps[row + x] = tr->st.node_pairs + tr->st.triangle_tests + tr->st.primitive_visits - e0;
                 current = add3(current, j->right);
             }
         }
@@ -1911,7 +1915,8 @@ static void add_stats(StOracleStats *a, const StOracleStats *b) {
     for (size_t i = 0; i < sizeof(StOracleStats) / 8; i++) x[i] += y[i];
 }
 
-static void render_frame(OScene *sc, const StView *v, StFloatPixel *pixels, StHitRec *hits, int threads, StOracleStats *stats) {
+static void render_frame(OScene *sc, const StView *v, StFloatPixel *pixels, StHitRec *hits, int threads, StOracleStats *stats,
+                         uint64_t *steps) {
     OCamera cam;
     ODims dims;
     camera_init(&cam, v);
@@ -1934,7 +1939,7 @@ static void render_frame(OScene *sc, const StView *v, StFloatPixel *pixels, StHi
         projection_plane(&cam, &dims, &j->start, &j->right, &j->down);
         j->mode = v->render_mode; j->depth = v->depth; j->cone_angle = v->cone_angle;
         j->mesh_stack_size = mesh_stack_size_of(sc);
-        j->pixels = pixels; j->hits = hits; j->tid = t; j->nthreads = threads;
+        j->pixels = pixels; j->hits = hits; j->steps = steps; j->tid = t; j->nthreads = threads;
     }
     if (threads == 1) render_rows(jobs);
     else {
@@ -1953,8 +1958,41 @@ int oracle_render(const void *blob, size_t size, const StView *view, StFloatPixe
                   int threads, StOracleStats *stats) {
     OScene *sc = scene_load(blob, size);
     if (!sc) return -1;
-    render_frame(sc, view, pixels, hits, threads, stats);
+    render_frame(sc, view, pixels, hits, threads, stats, NULL);
     scene_free(sc);
+    return 0;
+}
+
+int oracle_render_steps(const void *blob, size_t size, const StView *view, StFloatPixel *pixels, StHitRec *hits,
+                        uint64_t *steps, int threads, StOracleStats *stats) {
+    OScene *sc = scene_load(blob, size);
+    if (!sc) return -1;
+    render_frame(sc, view, pixels, hits, threads, stats, steps);
+    scene_free(sc);
+    return 0;
+}
+
+/* A scene loaded once and rendered from many views (the batched and
+   served workloads render hundreds of requests over a few large scenes). */
+void *oracle_scene_open(const void *blob, size_t size, uint32_t flags) {
+    /* meshes without a prebuilt BVH: the reference SAH build, or with
+       flags & 1 the Morton LBVH the device builds for large meshes */
+    g_mesh_lbvh = (flags & 1u) != 0;
+    OScene *sc = scene_load(blob, size);
+    g_mesh_lbvh = 0;
+    return sc;
+}
+void oracle_scene_close(void *scene) {
+    if (scene) scene_free((OScene *)scene);
+}
+int oracle_scene_render(void *scene, const StView *view, StFloatPixel *pixels, StHitRec *hits, int threads,
+                        StOracleStats *stats, double *seconds) {
+    if (!scene) return -1;
+    struct timespec t0, t1;
+    clock_gettime(CLOCK_MONOTONIC, &t0);
+    render_frame((OScene *)scene, view, pixels, hits, threads, stats, NULL);
+    clock_gettime(CLOCK_MONOTONIC, &t1);
+    if (seconds) *seconds = (double)(t1.tv_sec - t0.tv_sec) + 1e-9 * (double)(t1.tv_nsec - t0.tv_nsec);
     return 0;
 }
 
@@ -1970,7 +2008,7 @@ int oracle_render_timed(const void *blob, size_t size, const StView *view, StFlo
     if (!sc) return -1;
     for (int f = 0; f < frames; f++) {
         double t0 = now_s();
-        render_frame(sc, view, pixels, NULL, threads, stats);
+        render_frame(sc, view, pixels, NULL, threads, stats, NULL);
         seconds[f] = now_s() - t0;
     }
     scene_free(sc);

@@ -39,6 +39,21 @@ enum { ST_ORC_PRIMARY = 0, ST_ORC_SHADOW0 = 1, ST_ORC_SHADOW_DEEP = 2, ST_ORC_BO
 int oracle_render(const void *blob, size_t size, const StView *view, StFloatPixel *pixels,
                   StHitRec *hits, int threads, StOracleStats *stats);
 
+/* oracle_render that also records each pixel's traversal events (node
+   pairs + triangle tests + primitive visits over all of its rays) in steps
+   ((width+1)*(height+1) entries): the work a pixel costs. */
+int oracle_render_steps(const void *blob, size_t size, const StView *view, StFloatPixel *pixels, StHitRec *hits,
+                        uint64_t *steps, int threads, StOracleStats *stats);
+
+/* Load a scene blob once (NULL on a malformed blob), render any number of
+   views from it (seconds: optional, the render call's own time), free it.
+   flags & 1: build meshes that carry no BVH with the Morton LBVH (as
+   st_scene_create_ex's ST_SCENE_MESH_LBVH) instead of the reference SAH. */
+void *oracle_scene_open(const void *blob, size_t size, uint32_t flags);
+int oracle_scene_render(void *scene, const StView *view, StFloatPixel *pixels, StHitRec *hits, int threads,
+                        StOracleStats *stats, double *seconds);
+void oracle_scene_close(void *scene);
+
 /* Same, but loads once and renders `frames` times, recording per-frame
    seconds of the render call alone. */
 int oracle_render_timed(const void *blob, size_t size, const StView *view, StFloatPixel *pixels,

@@ -32,9 +32,22 @@ hipStream_t st_rt_stream(void);
 /* Record `msg` as the last error and return `code`. */
 int st_rt_error(int code, const char *msg);
 
+/* One launch over prepared views on the library's main context (holds the
+   library lock). */
 int st_rt_launch_prepared(StSceneHandle *const *scenes, StHostView *hv, uint32_t n, int want_hits, int persistent,
                           StRayCounts *counts, StFloatPixel *const *pixels_host, StHitRec *const *hits_host,
                           uint32_t *const *bgra_host, float *kernel_ms);
+
+/* A launch context of its own (stream, per-launch buffers, staging, tile
+   history): launches on different contexts run concurrently on the device.
+   A context serves one launch at a time; its owner calls st_rt_launch_ctx
+   from one thread at a time, without the library lock. */
+struct LaunchCtx;
+int st_rt_ctx_create(LaunchCtx **out);
+void st_rt_ctx_destroy(LaunchCtx *c);
+int st_rt_launch_ctx(LaunchCtx *c, StSceneHandle *const *scenes, StHostView *hv, uint32_t n, int want_hits,
+                     int persistent, StRayCounts *counts, StFloatPixel *const *pixels_host,
+                     StHitRec *const *hits_host, uint32_t *const *bgra_host, float *kernel_ms);
 
 void st_seam_state_free(struct StSeamState *s);
 

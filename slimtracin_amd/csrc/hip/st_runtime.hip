@@ -37,6 +37,7 @@
 #include <time.h>
 
 #include <algorithm>
+#include <atomic>
 #include <mutex>
 #include <vector>
 
@@ -330,11 +331,8 @@ __device__ static inline __attribute__((always_inline)) void primary_tile(const 
 __global__ void __launch_bounds__(BLOCK_THREADS) K_PRIMARY_ATTR
 k_primary(const StDScene *__restrict__ scenes, const KReq *__restrict__ reqs, uint32_t nreq, uint32_t total_tiles,
           uint32_t *tile_counter, uint32_t persistent, uint32_t xcd_blocks, uint32_t scene_entries,
-          uint32_t mesh_entries, KWave w, const uint32_t *__restrict__ tile_order, uint32_t *__restrict__ tile_cost,
-          unsigned long long *cost_sum) {
+          uint32_t mesh_entries, KWave w, const uint32_t *__restrict__ tile_order, uint32_t *__restrict__ tile_cost) {
     ST_LDS_STACKS(scene_entries, mesh_entries);
-    /* k_tile_order has read the previous launch's sum; k_counts adds this one's */
-    if (blockIdx.x == 0 && threadIdx.x == 0) *(volatile unsigned long long *)cost_sum = 0;
     /* One call site, so primary_tile is inlined once. Persistent: waves
        claim tiles from the global counter until it runs past the end.
        Otherwise each wave renders one tile: blocks are dealt to the 8 XCDs
@@ -364,25 +362,49 @@ k_primary(const StDScene *__restrict__ scenes, const KReq *__restrict__ reqs, ui
     }
 }
 
+/* Sum of the previous launch's tile times (cost_sum zeroed before): one
+   atomic per block. */
+#define COST_THREADS 256u
+__global__ void __launch_bounds__(COST_THREADS) k_cost_total(const uint32_t *__restrict__ tile_cost, uint32_t tiles,
+                                                             unsigned long long *cost_sum) {
+    __shared__ unsigned long long part[COST_THREADS / 64];
+    unsigned long long c = 0;
+    for (uint32_t i = blockIdx.x * COST_THREADS + threadIdx.x; i < tiles; i += gridDim.x * COST_THREADS)
+        c += ldgu(tile_cost, i);
+    for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off, 64);
+    if ((threadIdx.x & 63u) == 0) part[threadIdx.x >> 6] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long t = 0;
+        for (uint32_t k = 0; k < COST_THREADS / 64; k++) t += part[k];
+        if (t) atomicAdd(cost_sum, t);
+    }
+}
+
 /* Claim order of the primary stage's tiles, from each tile's time in the
-   previous launch (written by k_primary): tiles that took longer than the
-   mean (cost_sum[0] / tiles, summed by that launch's k_counts) first, then
-   the others. A launch ends when its slowest tiles end, so those should
-   start first and the cheap ones fill the end (on the 200k-triangle frame
-   the tiles over the mesh take 200-600 us, the floor's a few tens). One
-   thread per tile; each block keeps its tiles in order within each group
-   and claims its ranges with one atomic per group (ctr[0] from the front,
-   ctr[1] from the back, both zero at launch). Any cost values give a
-   permutation of the tiles, so the order never changes a result.
-   k_primary then clears cost_sum for its own launch's k_counts. */
+   previous launch of the same shape (written by k_primary): tiles that took
+   longer than the mean (cost_sum / tiles, k_cost_total) first, then the
+   others. A launch ends when its slowest tiles end, so those should start
+   first and the cheap ones fill the end (on the 200k-triangle frame the
+   tiles over the mesh take 200-600 us, the floor's a few tens). Without a
+   previous launch of this shape (use_history 0) the order is the tile
+   order. One thread per tile; each block keeps its tiles in order within
+   each group and claims its ranges with one atomic per group (ctr[0] from
+   the front, ctr[1] from the back, both zero at launch). Any cost values
+   give a permutation of the tiles, so the order never changes a result. */
 #define ORDER_THREADS 1024u
 __global__ void __launch_bounds__(ORDER_THREADS) k_tile_order(const uint32_t *__restrict__ tile_cost,
                                                               uint32_t *__restrict__ tile_order, uint32_t tiles,
-                                                              unsigned long long *cost_sum, uint32_t *ctr) {
+                                                              const unsigned long long *cost_sum, uint32_t *ctr,
+                                                              uint32_t use_history) {
     __shared__ uint32_t s_h[ORDER_THREADS / 64], s_l[ORDER_THREADS / 64], s_base[2];
     const uint32_t i = blockIdx.x * ORDER_THREADS + threadIdx.x;
+    if (!use_history) {
+        if (i < tiles) ((ST_GLOBAL uint32_t *)tile_order)[i] = i;
+        return;
+    }
     const uint32_t wv = threadIdx.x >> 6;
-    const unsigned long long total = *(volatile unsigned long long *)cost_sum;
+    const unsigned long long total = *(const volatile unsigned long long *)cost_sum;
     const bool live = i < tiles;
     const bool heavy = live && (unsigned long long)ldgu(tile_cost, i) * tiles > total;
     const unsigned long long mh = __ballot(heavy), ml = __ballot(live && !heavy);
@@ -525,11 +547,8 @@ __global__ void __launch_bounds__(BLOCK_THREADS) K_BOUNCE_ATTR k_bounce(const St
    sums COUNT_TILES tiles of one request (reqs[r].count_block = its first) and
    adds them with one atomic per counter. */
 #define COUNT_TILES 64u
-static_assert(COUNT_TILES <= 64, "one lane per tile sums a block's tile times");
 __global__ void __launch_bounds__(256) k_counts(const KReq *__restrict__ reqs, uint32_t nreq,
-                                                const uint32_t *__restrict__ counts,
-                                                const uint32_t *__restrict__ tile_cost,
-                                                unsigned long long *cost_sum) {
+                                                const uint32_t *__restrict__ counts) {
     __shared__ unsigned long long part[2][4];
     uint32_t lo = 0, hi = nreq - 1;
     while (lo < hi) {
@@ -558,12 +577,6 @@ __global__ void __launch_bounds__(256) k_counts(const KReq *__restrict__ reqs, u
     if ((threadIdx.x & 63u) == 0) {
         part[0][threadIdx.x >> 6] = sh;
         part[1][threadIdx.x >> 6] = bo;
-    }
-    /* the same tiles' primary-stage times, for the next launch's order */
-    if (threadIdx.x < 64) {
-        unsigned long long c = t0 + threadIdx.x < t1 ? ldgu(tile_cost, t0 + threadIdx.x) : 0u;
-        for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off, 64);
-        if (threadIdx.x == 0) atomicAdd(cost_sum, c);
     }
     __syncthreads();
     if (threadIdx.x == 0 && q.v.counters) {
@@ -749,33 +762,67 @@ struct Arena {
 };
 
 #define ST_STAGES_MAX 64
-static struct {
-    int ready;
-    int device;
+/* Everything one launch in flight owns: its stream, the per-launch device
+   buffers, the pinned staging of its uploads and readbacks, the primary
+   stage's tile history and the events that time it. The direct API runs on
+   the library's main context (under the library lock); a server keeps one
+   context per launch it can have in flight, so launches on different
+   contexts overlap on the device. */
+struct LaunchCtx {
     hipStream_t stream;
     Arena work;            /* per-launch device buffers (grow-only) */
-    char *staging;         /* pinned host staging for per-launch uploads */
+    char *staging;         /* pinned host staging: uploads, then the ray-count readback */
     size_t staging_cap;
     uint32_t *tile_cost;   /* [tile_cap] each tile's time in the last primary stage */
     uint32_t *tile_order;  /* [tile_cap] the primary stage's claim order (k_tile_order) */
     uint32_t tile_cap;
-    unsigned long long *cost_sum; /* sum of tile_cost over the last primary stage (k_counts) */
+    uint64_t tile_sig;     /* shape of the launch whose times tile_cost holds (0: none) */
+    unsigned long long *cost_sum; /* sum of tile_cost over the last launch (k_cost_total) */
     uint32_t *tile_counter; /* 64 words zeroed per launch: [0] the persistent primary stage's tile
                                counter, [BOUNCE_CLAIM] the bounce stage's chunk claims (128 B apart) */
-    StDScene *scene_table; /* device copy of the scene descriptors of a launch */
-    uint32_t scene_table_cap;
     hipEvent_t ev[4];
-    int cus;               /* compute units of the device */
-    StTuning tune;         /* launch shape chosen by st_set_tuning / st_autotune */
     int stage_timing;      /* record an event after every stage launch */
     int graph_mode;        /* capture launches into a graph, replay while the launch shape repeats */
     hipGraphExec_t graph_exec;
     std::vector<uint64_t> graph_sig;
-    uint64_t graph_captures, graph_launches;
     hipEvent_t stage_ev[ST_STAGES_MAX + 1];
     uint32_t stage_n;
     float stage_ms[ST_STAGES_MAX];
+};
+
+static struct {
+    int ready;
+    int device;
+    LaunchCtx main;        /* the direct API's launches, scene builds and copies */
+    int cus;               /* compute units of the device */
+    StTuning tune;         /* launch shape chosen by st_set_tuning / st_autotune */
+    uint32_t step_cap;     /* st_debug_step_cap: lower traversal step bound, 0 = none */
+    std::atomic<uint64_t> graph_captures, graph_launches;
 } g;
+
+static int ctx_init(LaunchCtx *c) {
+    HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    HIP_TRY(hipMalloc(&c->tile_counter, 256));
+    HIP_TRY(hipMalloc(&c->cost_sum, 256));
+    HIP_TRY(hipMemset(c->cost_sum, 0, 256));
+    for (int i = 0; i < 4; i++) HIP_TRY(hipEventCreate(&c->ev[i]));
+    return ST_OK;
+}
+
+static void ctx_release(LaunchCtx *c) {
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
+    if (c->work.base) (void)hipFree(c->work.base);
+    if (c->staging) (void)hipHostFree(c->staging);
+    if (c->tile_cost) (void)hipFree(c->tile_cost);
+    if (c->cost_sum) (void)hipFree(c->cost_sum);
+    if (c->tile_counter) (void)hipFree(c->tile_counter);
+    for (int i = 0; i < 4; i++)
+        if (c->ev[i]) (void)hipEventDestroy(c->ev[i]);
+    for (int i = 0; i <= ST_STAGES_MAX; i++)
+        if (c->stage_ev[i]) (void)hipEventDestroy(c->stage_ev[i]);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+}
 
 static int ensure_init(void) {
     if (g.ready) return ST_OK;
@@ -792,11 +839,8 @@ extern "C" int st_init(int device) {
     HIP_TRY(hipGetDeviceCount(&count));
     if (device < 0 || device >= count) return set_err(ST_ERR_ARG, "device %d not present (%d devices)", device, count);
     HIP_TRY(hipSetDevice(device));
-    HIP_TRY(hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking));
-    HIP_TRY(hipMalloc(&g.tile_counter, 256));
-    HIP_TRY(hipMalloc(&g.cost_sum, 256));
-    HIP_TRY(hipMemset(g.cost_sum, 0, 256));
-    for (int i = 0; i < 4; i++) HIP_TRY(hipEventCreate(&g.ev[i]));
+    int rc = ctx_init(&g.main);
+    if (rc) return rc;
     g.tune.tile_shift = 3;          /* 8 x 8 tiles */
     g.tune.blocks_per_cu = 8;
     g.tune.persistent = 1;
@@ -813,7 +857,27 @@ void st_rt_lock(void) { g_mu.lock(); }
 void st_rt_unlock(void) { g_mu.unlock(); }
 
 int st_rt_ensure_init(void) { return ensure_init(); }
-hipStream_t st_rt_stream(void) { return g.stream; }
+hipStream_t st_rt_stream(void) { return g.main.stream; }
+
+int st_rt_ctx_create(LaunchCtx **out) {
+    *out = NULL;
+    int rc = ensure_init();
+    if (rc) return rc;
+    LaunchCtx *c = new LaunchCtx();
+    if ((rc = ctx_init(c))) {
+        ctx_release(c);
+        delete c;
+        return rc;
+    }
+    *out = c;
+    return ST_OK;
+}
+
+void st_rt_ctx_destroy(LaunchCtx *c) {
+    if (!c) return;
+    ctx_release(c);
+    delete c;
+}
 int st_rt_error(int code, const char *msg) { return set_err(code, "%s", msg); }
 extern "C" int st_host_error(int code, const char *msg) { return set_err(code, "%s", msg); }
 
@@ -835,14 +899,14 @@ static void *arena_take(Arena *a, size_t bytes) {
 }
 static size_t round256(size_t b) { return (b + 255) & ~(size_t)255; }
 
-static int staging_reserve(size_t bytes) {
-    if (g.staging_cap >= bytes) return ST_OK;
-    if (g.staging) HIP_TRY(hipHostFree(g.staging));
-    g.staging = NULL;
-    g.staging_cap = 0;
+static int staging_reserve(LaunchCtx *c, size_t bytes) {
+    if (c->staging_cap >= bytes) return ST_OK;
+    if (c->staging) HIP_TRY(hipHostFree(c->staging));
+    c->staging = NULL;
+    c->staging_cap = 0;
     size_t cap = bytes + bytes / 4 + 65536;
-    HIP_TRY(hipHostMalloc((void **)&g.staging, cap, hipHostMallocDefault));
-    g.staging_cap = cap;
+    HIP_TRY(hipHostMalloc((void **)&c->staging, cap, hipHostMallocDefault));
+    c->staging_cap = cap;
     return ST_OK;
 }
 
@@ -918,7 +982,7 @@ static int build_device_meshes(StSceneHandle *h) {
         };
         for (int k = 0; k < 6 && !rc; k++) {
             if (!up[k].src || !up[k].bytes) continue;
-            if (hipMemcpyAsync(p, up[k].src, up[k].bytes, hipMemcpyHostToDevice, g.stream) != hipSuccess)
+            if (hipMemcpyAsync(p, up[k].src, up[k].bytes, hipMemcpyHostToDevice, g.main.stream) != hipSuccess)
                 rc = set_err(ST_ERR_HIP, "mesh geometry upload failed");
             *up[k].dst = p;
             p += round256(up[k].bytes);
@@ -928,13 +992,13 @@ static int build_device_meshes(StSceneHandle *h) {
         uint32_t *nodes = (uint32_t *)h->dev.mesh_nodes + 8ull * dm.node_offset;
         StDTriHot *hot = (StDTriHot *)h->dev.tri_hot + dm.tri_offset;
         StDTriCold *cold = (StDTriCold *)h->dev.tri_cold + dm.tri_offset;
-        if (st_mesh_lbvh_build(g.stream, in, buf + geom, sort_bytes, nodes, hot, cold, height_d)) {
+        if (st_mesh_lbvh_build(g.main.stream, in, buf + geom, sort_bytes, nodes, hot, cold, height_d)) {
             rc = set_err(ST_ERR_HIP, "mesh %u: device BVH build failed: %s", m, hipGetErrorString(hipGetLastError()));
             break;
         }
         uint32_t height = 0;
-        if (hipMemcpyAsync(&height, height_d, 4, hipMemcpyDeviceToHost, g.stream) != hipSuccess ||
-            hipStreamSynchronize(g.stream) != hipSuccess) {
+        if (hipMemcpyAsync(&height, height_d, 4, hipMemcpyDeviceToHost, g.main.stream) != hipSuccess ||
+            hipStreamSynchronize(g.main.stream) != hipSuccess) {
             rc = set_err(ST_ERR_HIP, "mesh %u: device BVH build failed", m);
             break;
         }
@@ -946,7 +1010,7 @@ static int build_device_meshes(StSceneHandle *h) {
         if (height > max_height) max_height = height;
         memset(sc->deferred + m, 0, sizeof(StMeshPart));
     }
-    (void)hipStreamSynchronize(g.stream);
+    (void)hipStreamSynchronize(g.main.stream);
     (void)hipFree(buf);
     if (rc) return rc;
     sc->max_mesh_height = max_height;
@@ -978,7 +1042,7 @@ extern "C" int st_scene_mesh_bvh(const StSceneHandle *h, uint32_t m, StBVHNodeRe
             count--;
     }
     if (count > node_cap) return set_err(ST_ERR_ARG, "node buffer holds %u of %u slots", node_cap, count);
-    HIP_TRY(hipStreamSynchronize(g.stream));
+    HIP_TRY(hipStreamSynchronize(g.main.stream));
     HIP_TRY(hipMemcpy(nodes, (const uint32_t *)h->dev.mesh_nodes + 8ull * dm.node_offset, 32ull * count,
                       hipMemcpyDeviceToHost));
     std::vector<StDTriHot> hot(dm.triangle_count);
@@ -1112,7 +1176,7 @@ extern "C" int st_scene_bvh(const StSceneHandle *h, StBVHNodeRec *nodes, uint32_
     if (!h->host) return set_err(ST_ERR_ARG, "scene is not resident yet");
     const StScene *sc = h->host;
     if (h->bvh_on_device_only) {
-        HIP_TRY(hipStreamSynchronize(g.stream));
+        HIP_TRY(hipStreamSynchronize(g.main.stream));
         HIP_TRY(hipMemcpy(sc->scene_nodes, h->dev.scene_nodes, sizeof(StBVHNodeRec) * sc->scene_node_count,
                           hipMemcpyDeviceToHost));
         HIP_TRY(hipMemcpy(sc->scene_leaf_ids, h->dev.scene_leaf_ids, 4u * sc->prim_count, hipMemcpyDeviceToHost));
@@ -1148,8 +1212,8 @@ extern "C" int st_scene_set_prims(StSceneHandle *h, const StPrimRec *recs, uint3
     }
     h->dev.has_emissive_quads = sc->has_emissive_quads;
     HIP_TRY(hipMemcpyAsync((void *)(h->dev.prims + first), sc->prims + first, count * sizeof(StDPrim),
-                           hipMemcpyHostToDevice, g.stream));
-    HIP_TRY(hipStreamSynchronize(g.stream));
+                           hipMemcpyHostToDevice, g.main.stream));
+    HIP_TRY(hipStreamSynchronize(g.main.stream));
     return ST_OK;
 }
 
@@ -1187,15 +1251,15 @@ extern "C" int st_scene_rebuild(StSceneHandle *h, uint32_t mode, float *ms) {
         HIP_TRY(hipMalloc(&h->d_height, 256));
     }
     StLbvhWork w = st_lbvh_work_carve(h->lbvh_work, P);
-    HIP_TRY(hipEventRecord(g.ev[2], g.stream));
-    if (st_lbvh_launch(g.stream, h->dev.prims, h->d_mesh_aabbs, sc->mesh_count, P, w, (uint32_t *)h->dev.scene_nodes,
+    HIP_TRY(hipEventRecord(g.main.ev[2], g.main.stream));
+    if (st_lbvh_launch(g.main.stream, h->dev.prims, h->d_mesh_aabbs, sc->mesh_count, P, w, (uint32_t *)h->dev.scene_nodes,
                        (uint32_t *)h->dev.scene_leaf_ids, h->d_height))
         return set_err(ST_ERR_HIP, "LBVH launch failed: %s", hipGetErrorString(hipGetLastError()));
-    HIP_TRY(hipEventRecord(g.ev[3], g.stream));
+    HIP_TRY(hipEventRecord(g.main.ev[3], g.main.stream));
     uint32_t height = 0;
-    HIP_TRY(hipMemcpyAsync(&height, h->d_height, 4, hipMemcpyDeviceToHost, g.stream));
-    HIP_TRY(hipStreamSynchronize(g.stream));
-    if (ms) HIP_TRY(hipEventElapsedTime(ms, g.ev[2], g.ev[3]));
+    HIP_TRY(hipMemcpyAsync(&height, h->d_height, 4, hipMemcpyDeviceToHost, g.main.stream));
+    HIP_TRY(hipStreamSynchronize(g.main.stream));
+    if (ms) HIP_TRY(hipEventElapsedTime(ms, g.main.ev[2], g.main.ev[3]));
     if (height == 0 || height + 1 > ST_SCENE_STACK_MAX) return set_err(ST_ERR_SCENE, "LBVH height %u too deep", height);
     sc->scene_node_count = 2 * P - 1;
     sc->scene_height = height;
@@ -1237,16 +1301,38 @@ struct LaunchPlan {
     size_t work_bytes, stage_bytes;
 };
 
-/* Upload and run one launch over `n` requests whose host views are already
-   prepared (camera basis, row starts, SSB rects). Each request's frame (and
-   hit records) is copied to pixels_host[i] / hits_host[i] when given. */
-int st_rt_launch_prepared(StSceneHandle *const *scenes, StHostView *hv, uint32_t n, int want_hits, int persistent,
-                          StRayCounts *counts, StFloatPixel *const *pixels_host, StHitRec *const *hits_host,
-                          uint32_t *const *bgra_host, float *kernel_ms) {
-    RT_LOCK;
+/* 64-bit FNV-1a over words: the shape signature that keys a context's tile
+   history (a tile time only orders the same tile of a launch of the same
+   shape). */
+static uint64_t sig_hash(const std::vector<uint64_t> &w) {
+    uint64_t h = 1469598103934665603ull;
+    for (uint64_t x : w)
+        for (int k = 0; k < 8; k++) {
+            h ^= (x >> (8 * k)) & 0xFFu;
+            h *= 1099511628211ull;
+        }
+    return h ? h : 1;
+}
+
+/* Upload and run one launch on context c over `n` requests whose host views
+   are already prepared (camera basis, row starts, SSB rects). Each
+   request's frame (and hit records) is copied to pixels_host[i] /
+   hits_host[i] when given. Returns when the launch is done. Runs without
+   the library lock on a server context: only c's own buffers are written,
+   the scenes are read. */
+int st_rt_launch_ctx(LaunchCtx *c, StSceneHandle *const *scenes, StHostView *hv, uint32_t n, int want_hits,
+                     int persistent, StRayCounts *counts, StFloatPixel *const *pixels_host,
+                     StHitRec *const *hits_host, uint32_t *const *bgra_host, float *kernel_ms) {
     int rc = ensure_init();
     if (rc) return rc;
     if (n == 0) return set_err(ST_ERR_ARG, "no requests");
+    StTuning tune;
+    uint32_t step_cap;
+    {
+        RT_LOCK;
+        tune = g.tune;
+        step_cap = g.step_cap;
+    }
     std::vector<uint32_t> scene_idx(n);
     std::vector<StSceneHandle *> uniq(n);
     uint32_t nuniq = 0;
@@ -1257,6 +1343,7 @@ int st_rt_launch_prepared(StSceneHandle *const *scenes, StHostView *hv, uint32_t
     size_t per_req_out = 0;
     uint32_t rows = 0, tiles = 0;
     uint64_t out_px = 0;
+    std::vector<uint64_t> shape = {n, tune.tile_shift};
     for (uint32_t i = 0; i < n; i++) {
         StSceneHandle *s = scenes[i];
         if (!s || !s->host || !s->dbuf) return set_err(ST_ERR_ARG, "request %u has no resident scene", i);
@@ -1271,17 +1358,20 @@ int st_rt_launch_prepared(StSceneHandle *const *scenes, StHostView *hv, uint32_t
             if (hv[i].dv.width1 < 2 || hv[i].dv.height1 < 2) return set_err(ST_ERR_ARG, "request %u: empty window", i);
             per_req_out += round256((size_t)(hv[i].dv.width1 - 1) * (hv[i].dv.height1 - 1) * 4);
         }
-        const uint32_t tw = 1u << g.tune.tile_shift, th = 64u >> g.tune.tile_shift;
-        hv[i].dv.tile_shift = g.tune.tile_shift;
+        const uint32_t tw = 1u << tune.tile_shift, th = 64u >> tune.tile_shift;
+        hv[i].dv.tile_shift = tune.tile_shift;
         hv[i].dv.tiles_x = (hv[i].dv.width1 + tw - 1) / tw;
         uint32_t ty = (hv[i].dv.height1 + th - 1) / th;
         hv[i].dv.tile_base = tiles;
         tiles += hv[i].dv.tiles_x * ty;
         rows += hv[i].dv.height1;
         out_px += (uint64_t)(hv[i].dv.width1 - 1) * (hv[i].dv.height1 - 1);
+        shape.insert(shape.end(), {(uint64_t)(uintptr_t)s, hv[i].dv.width1, hv[i].dv.height1, hv[i].dv.mode,
+                                   hv[i].dv.depth});
     }
     if (out_px > 0xFFFFFFF0ull || tiles > 0x7FFFFFF0u || (uint64_t)rows * 3 > 0xFFFFFFF0ull)
         return set_err(ST_ERR_ARG, "batch too large");
+    const uint64_t tile_sig = sig_hash(shape);
     size_t o_scenes = up;
     up += round256(nuniq * sizeof(StDScene));
     for (uint32_t i = 0; i < n; i++) {
@@ -1305,26 +1395,40 @@ int st_rt_launch_prepared(StSceneHandle *const *scenes, StHostView *hv, uint32_t
         wave_bytes = round256(slots * 16 * ST_PLANES) + round256(slots * lmax * 8) + round256(slots * 4) +
                      round256(slots * 4) + round256(8) + round256(slots * 8) + 6 * 256;
     }
-    size_t work = up + round256(n * 24ull) + per_req_out + wave_bytes + 256;
-    if ((rc = arena_reserve(&g.work, work))) return rc;
-    if ((rc = staging_reserve(up))) return rc;
-    if (tiles > g.tile_cap) {   /* per-tile times and claim order, kept across launches */
-        if (g.tile_cost) HIP_TRY(hipFree(g.tile_cost));
-        g.tile_cost = g.tile_order = NULL;
-        g.tile_cap = 0;
-        HIP_TRY(hipMalloc(&g.tile_cost, (size_t)tiles * 8));
-        HIP_TRY(hipMemset(g.tile_cost, 0, (size_t)tiles * 8));
-        g.tile_order = g.tile_cost + tiles;
-        g.tile_cap = tiles;
+    const size_t cnt_bytes = n * 24ull + 8;
+    size_t work = up + round256(cnt_bytes) + per_req_out + wave_bytes + 256;
+    if ((rc = arena_reserve(&c->work, work))) return rc;
+    /* staging: the upload block, then the ray counts and fault word read back */
+    const size_t o_readback = round256(up);
+    if ((rc = staging_reserve(c, o_readback + cnt_bytes))) return rc;
+    if (tiles > c->tile_cap) {   /* per-tile times and claim order, kept across launches */
+        if (c->tile_cost) HIP_TRY(hipFree(c->tile_cost));
+        c->tile_cost = c->tile_order = NULL;
+        c->tile_cap = 0;
+        c->tile_sig = 0;
+        HIP_TRY(hipMalloc(&c->tile_cost, (size_t)tiles * 8));
+        HIP_TRY(hipMemset(c->tile_cost, 0, (size_t)tiles * 8));
+        c->tile_order = c->tile_cost + tiles;
+        c->tile_cap = tiles;
     }
-    char *st = g.staging;
-    char *blk = (char *)arena_take(&g.work, up);
+    /* the previous launch on this context had this shape: order the tiles
+       by its tile times */
+    const uint32_t use_history = c->tile_sig == tile_sig ? 1u : 0u;
+    char *st = c->staging;
+    char *blk = (char *)arena_take(&c->work, up);
     KReq *reqs_h = (KReq *)st;
     KReq *reqs_d = (KReq *)blk;
     StDScene *sc_h = (StDScene *)(st + o_scenes);
     StDScene *sc_d = (StDScene *)(blk + o_scenes);
-    unsigned long long *cnt_d = (unsigned long long *)arena_take(&g.work, n * 24ull);
-    for (uint32_t k = 0; k < nuniq; k++) sc_h[k] = uniq[k]->dev;
+    /* per request: primary, shadow, bounce ray totals; then the launch's
+       traversal fault word (StDScene.fault) */
+    unsigned long long *cnt_d = (unsigned long long *)arena_take(&c->work, cnt_bytes);
+    unsigned long long *cnt_h = (unsigned long long *)(st + o_readback);
+    for (uint32_t k = 0; k < nuniq; k++) {
+        sc_h[k] = uniq[k]->dev;
+        sc_h[k].fault = (uint32_t *)(cnt_d + 3 * (size_t)n);
+        sc_h[k].step_cap = step_cap;
+    }
     uint32_t row_base = 0, out_base = 0, count_blocks = 0, any_device_ssb = 0;
     for (uint32_t i = 0; i < n; i++) {
         size_t px = (size_t)hv[i].dv.width1 * hv[i].dv.height1;
@@ -1335,9 +1439,9 @@ int st_rt_launch_prepared(StSceneHandle *const *scenes, StHostView *hv, uint32_t
         if (!hv[i].device_ssb) memcpy(st + rect_off[i], hv[i].rects, P * 16ull);
         dv.row_starts = (const float *)(blk + rs_off[i]);
         dv.rects = (const int32_t *)(blk + rect_off[i]);
-        dv.dir_marks = (float *)arena_take(&g.work, (size_t)dv.height1 * ST_DIR_MARKS(dv.width1) * 16);
-        dv.pixels = arena_take(&g.work, px * 24);
-        dv.hits = want_hits ? arena_take(&g.work, px * 16) : NULL;
+        dv.dir_marks = (float *)arena_take(&c->work, (size_t)dv.height1 * ST_DIR_MARKS(dv.width1) * 16);
+        dv.pixels = arena_take(&c->work, px * 24);
+        dv.hits = want_hits ? arena_take(&c->work, px * 16) : NULL;
         dv.counters = cnt_d + 3 * i;
         memset(&reqs_h[i], 0, sizeof(KReq));
         reqs_h[i].v = dv;
@@ -1347,7 +1451,7 @@ int st_rt_launch_prepared(StSceneHandle *const *scenes, StHostView *hv, uint32_t
         reqs_h[i].count_block = count_blocks;
         count_blocks += (reqs_h[i].tile_count + COUNT_TILES - 1) / COUNT_TILES;
         uint32_t wpx = (dv.width1 - 1) * (dv.height1 - 1);
-        reqs_h[i].bgra = (bgra_host && bgra_host[i]) ? (uint32_t *)arena_take(&g.work, (size_t)wpx * 4) : NULL;
+        reqs_h[i].bgra = (bgra_host && bgra_host[i]) ? (uint32_t *)arena_take(&c->work, (size_t)wpx * 4) : NULL;
         if (hv[i].device_ssb) {
             const sm3 &m = hv[i].rotation_matrix_inverted;
             const float rmi[9] = {m.X.x, m.X.y, m.X.z, m.Y.x, m.Y.y, m.Y.z, m.Z.x, m.Z.y, m.Z.z};
@@ -1364,19 +1468,19 @@ int st_rt_launch_prepared(StSceneHandle *const *scenes, StHostView *hv, uint32_t
     if (levels) {
         w.n = (uint32_t)slots;
         w.lmax = lmax;
-        w.st = (float *)arena_take(&g.work, slots * 16 * ST_PLANES);
-        w.shadow_res = (uint32_t *)arena_take(&g.work, slots * lmax * 8);
-        w.occ_d2 = (float *)arena_take(&g.work, slots * 4);
-        w.bounce_q = (uint32_t *)arena_take(&g.work, slots * 4);
-        w.ctr = (uint32_t *)arena_take(&g.work, 8);
-        w.counts = (uint32_t *)arena_take(&g.work, slots * 8);
+        w.st = (float *)arena_take(&c->work, slots * 16 * ST_PLANES);
+        w.shadow_res = (uint32_t *)arena_take(&c->work, slots * lmax * 8);
+        w.occ_d2 = (float *)arena_take(&c->work, slots * 4);
+        w.bounce_q = (uint32_t *)arena_take(&c->work, slots * 4);
+        w.ctr = (uint32_t *)arena_take(&c->work, 8);
+        w.counts = (uint32_t *)arena_take(&c->work, slots * 8);
     }
     /* launch shape, fixed before anything is enqueued so that a captured
        graph of this launch can be matched against the next one */
     uint32_t blocks;
     uint32_t xcd_blocks = 0;
     if (persistent) {
-        uint32_t want = (uint32_t)g.cus * g.tune.blocks_per_cu;
+        uint32_t want = (uint32_t)g.cus * tune.blocks_per_cu;
         uint32_t need = (tiles + WAVES_PER_BLOCK - 1) / WAVES_PER_BLOCK;
         blocks = need < want ? need : want;
     } else {
@@ -1399,61 +1503,68 @@ int st_rt_launch_prepared(StSceneHandle *const *scenes, StHostView *hv, uint32_t
     size_t lds_primary = lds;
     /* one wave per tile: at most blocks_per_cu of its blocks per CU (fewer
        tiles in flight share the caches better) */
-    if (!persistent && g.tune.blocks_per_cu < 4)
-        (void)capped_blocks(reinterpret_cast<const void *>(k_primary), lds, g.tune.blocks_per_cu, &lds_primary);
+    if (!persistent && tune.blocks_per_cu < 4)
+        (void)capped_blocks(reinterpret_cast<const void *>(k_primary), lds, tune.blocks_per_cu, &lds_primary);
     if (levels) {
         b_bounce = resident_blocks(reinterpret_cast<const void *>(k_bounce), lds);
     }
+    const uint32_t cost_blocks = std::min<uint32_t>((tiles + COST_THREADS - 1) / COST_THREADS, 1024u);
 
     /* per-stage timing (bench): an event before the first stage and after each */
     uint32_t marks = 0;
-    const bool graph = g.graph_mode != 0;
+    const bool graph = c->graph_mode != 0;
     auto mark = [&]() -> hipError_t {
-        if (graph || !g.stage_timing || marks > ST_STAGES_MAX) return hipSuccess;
-        if (!g.stage_ev[marks]) {
-            hipError_t e = hipEventCreate(&g.stage_ev[marks]);
+        if (graph || !c->stage_timing || marks > ST_STAGES_MAX) return hipSuccess;
+        if (!c->stage_ev[marks]) {
+            hipError_t e = hipEventCreate(&c->stage_ev[marks]);
             if (e != hipSuccess) return e;
         }
-        return hipEventRecord(g.stage_ev[marks++], g.stream);
+        return hipEventRecord(c->stage_ev[marks++], c->stream);
     };
     /* everything from the upload to the ray counts; the same stream work is
-       what a graph captures */
+       what a graph captures. ev[0] .. ev[1] brackets every kernel of the
+       launch (the render time st_render_timed reports). */
     auto enqueue = [&]() -> int {
-        HIP_TRY(hipMemcpyAsync(blk, st, up, hipMemcpyHostToDevice, g.stream));
-        hipLaunchKernelGGL(k_zero_u64, dim3((3 * n + 255) / 256), dim3(256), 0, g.stream, cnt_d, 3 * n);
-        hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, g.stream, g.tile_counter, 64u);
+        HIP_TRY(hipMemcpyAsync(blk, st, up, hipMemcpyHostToDevice, c->stream));
+        if (!graph) HIP_TRY(hipEventRecord(c->ev[0], c->stream));
+        hipLaunchKernelGGL(k_zero_u64, dim3((3 * n + 1 + 255) / 256), dim3(256), 0, c->stream, cnt_d, 3 * n + 1);
+        hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, c->stream, c->tile_counter, 64u);
         static_assert(BOUNCE_CLAIM < 64 && ORDER_CLAIM + 1 < 64, "claim counters share the zeroed 64 words");
         if (any_device_ssb)
-            hipLaunchKernelGGL(k_ssb, dim3(n), dim3(256), 0, g.stream, (const StDScene *)sc_d, (const KReq *)reqs_d);
+            hipLaunchKernelGGL(k_ssb, dim3(n), dim3(256), 0, c->stream, (const StDScene *)sc_d, (const KReq *)reqs_d);
         if (levels)
-            hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, g.stream, w.ctr, 2u);
+            hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, c->stream, w.ctr, 2u);
         hipLaunchKernelGGL(k_primary_dirs, dim3((uint32_t)(((uint64_t)rows * 3 + DIRS_BLOCK - 1) / DIRS_BLOCK)),
-                           dim3(DIRS_BLOCK), 0, g.stream, reqs_d, n, rows);
+                           dim3(DIRS_BLOCK), 0, c->stream, reqs_d, n, rows);
         HIP_TRY(hipGetLastError());
+        if (use_history) {
+            hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, c->stream, c->cost_sum, 1u);
+            hipLaunchKernelGGL(k_cost_total, dim3(cost_blocks), dim3(COST_THREADS), 0, c->stream,
+                               (const uint32_t *)c->tile_cost, tiles, c->cost_sum);
+        }
         hipLaunchKernelGGL(k_tile_order, dim3((tiles + ORDER_THREADS - 1) / ORDER_THREADS), dim3(ORDER_THREADS), 0,
-                           g.stream, (const uint32_t *)g.tile_cost, g.tile_order, tiles, g.cost_sum,
-                           g.tile_counter + ORDER_CLAIM);
+                           c->stream, (const uint32_t *)c->tile_cost, c->tile_order, tiles,
+                           (const unsigned long long *)c->cost_sum, c->tile_counter + ORDER_CLAIM, use_history);
         HIP_TRY(hipGetLastError());
-        if (!graph) HIP_TRY(hipEventRecord(g.ev[0], g.stream));
         HIP_TRY(mark());
-        hipLaunchKernelGGL(k_primary, dim3(blocks), dim3(BLOCK_THREADS), lds_primary, g.stream, (const StDScene *)sc_d,
-                           (const KReq *)reqs_d, n, tiles, g.tile_counter, (uint32_t)persistent, xcd_blocks,
-                           s_entries, m_entries, w, (const uint32_t *)g.tile_order, g.tile_cost, g.cost_sum);
+        hipLaunchKernelGGL(k_primary, dim3(blocks), dim3(BLOCK_THREADS), lds_primary, c->stream, (const StDScene *)sc_d,
+                           (const KReq *)reqs_d, n, tiles, c->tile_counter, (uint32_t)persistent, xcd_blocks,
+                           s_entries, m_entries, w, (const uint32_t *)c->tile_order, c->tile_cost);
         HIP_TRY(hipGetLastError());
         HIP_TRY(mark());
         /* the rest of the depth loop: the bounce stage reads its queue
            length on the device, so it needs no host round trip */
         if (levels > 1) {
-            hipLaunchKernelGGL(k_bounce, dim3(b_bounce), dim3(BLOCK_THREADS), lds, g.stream,
-                               (const StDScene *)sc_d, (const KReq *)reqs_d, n, w, g.tile_counter + BOUNCE_CLAIM,
+            hipLaunchKernelGGL(k_bounce, dim3(b_bounce), dim3(BLOCK_THREADS), lds, c->stream,
+                               (const StDScene *)sc_d, (const KReq *)reqs_d, n, w, c->tile_counter + BOUNCE_CLAIM,
                                s_entries, m_entries);
             HIP_TRY(mark());
         }
         if (levels && count_blocks)
-            hipLaunchKernelGGL(k_counts, dim3(count_blocks), dim3(256), 0, g.stream, (const KReq *)reqs_d, n,
-                               (const uint32_t *)w.counts, (const uint32_t *)g.tile_cost, g.cost_sum);
+            hipLaunchKernelGGL(k_counts, dim3(count_blocks), dim3(256), 0, c->stream, (const KReq *)reqs_d, n,
+                               (const uint32_t *)w.counts);
         HIP_TRY(hipGetLastError());
-        if (!graph) HIP_TRY(hipEventRecord(g.ev[1], g.stream));
+        if (!graph) HIP_TRY(hipEventRecord(c->ev[1], c->stream));
         return ST_OK;
     };
 
@@ -1469,70 +1580,82 @@ int st_rt_launch_prepared(StSceneHandle *const *scenes, StHostView *hv, uint32_t
                                      lds_primary, (uint64_t)(uintptr_t)blk,
                                      (uint64_t)(uintptr_t)st,
                                      (uint64_t)(uintptr_t)cnt_d, (uint64_t)(uintptr_t)w.st, w.n,
-                                     (uint64_t)(uintptr_t)g.tile_cost};
+                                     (uint64_t)(uintptr_t)c->tile_cost, use_history, cost_blocks};
         for (uint32_t i = 0; i < n; i++) {
             const StDView &dv = reqs_h[i].v;
             sig.insert(sig.end(), {dv.width1, dv.height1, dv.mode, dv.depth, dv.tile_shift,
                                    (uint64_t)(uintptr_t)dv.dir_marks, (uint64_t)(uintptr_t)dv.pixels,
                                    (uint64_t)(uintptr_t)dv.hits, (uint64_t)(uintptr_t)reqs_h[i].bgra});
         }
-        if (!g.graph_exec || sig != g.graph_sig) {
-            if (g.graph_exec) {
-                (void)hipGraphExecDestroy(g.graph_exec);
-                g.graph_exec = nullptr;
+        if (!c->graph_exec || sig != c->graph_sig) {
+            if (c->graph_exec) {
+                (void)hipGraphExecDestroy(c->graph_exec);
+                c->graph_exec = nullptr;
             }
-            HIP_TRY(hipStreamBeginCapture(g.stream, hipStreamCaptureModeRelaxed));
+            HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
             const int erc = enqueue();
             hipGraph_t gr = nullptr;
-            const hipError_t ee = hipStreamEndCapture(g.stream, &gr);
+            const hipError_t ee = hipStreamEndCapture(c->stream, &gr);
             if (erc || ee != hipSuccess) {
                 if (gr) (void)hipGraphDestroy(gr);
                 return erc ? erc : set_err(ST_ERR_HIP, "graph capture failed: %s", hipGetErrorString(ee));
             }
-            const hipError_t ie = hipGraphInstantiate(&g.graph_exec, gr, nullptr, nullptr, 0);
+            const hipError_t ie = hipGraphInstantiate(&c->graph_exec, gr, nullptr, nullptr, 0);
             (void)hipGraphDestroy(gr);
             if (ie != hipSuccess) {
-                g.graph_exec = nullptr;
+                c->graph_exec = nullptr;
                 return set_err(ST_ERR_HIP, "graph instantiation failed: %s", hipGetErrorString(ie));
             }
-            g.graph_sig = sig;
+            c->graph_sig = sig;
             g.graph_captures++;
         }
-        HIP_TRY(hipEventRecord(g.ev[0], g.stream));
-        HIP_TRY(hipGraphLaunch(g.graph_exec, g.stream));
-        HIP_TRY(hipEventRecord(g.ev[1], g.stream));
+        HIP_TRY(hipEventRecord(c->ev[0], c->stream));
+        HIP_TRY(hipGraphLaunch(c->graph_exec, c->stream));
+        HIP_TRY(hipEventRecord(c->ev[1], c->stream));
         g.graph_launches++;
     }
+    /* this launch's tile times are now what tile_cost will hold */
+    c->tile_sig = tile_sig;
     if (bgra_host && out_px) {
-        hipLaunchKernelGGL(k_resolve, dim3((uint32_t)((out_px + 255) / 256)), dim3(256), 0, g.stream,
+        hipLaunchKernelGGL(k_resolve, dim3((uint32_t)((out_px + 255) / 256)), dim3(256), 0, c->stream,
                            (const KReq *)reqs_d, n, (uint32_t)out_px);
         HIP_TRY(hipGetLastError());
     }
     for (uint32_t i = 0; i < n; i++) {
         size_t px = (size_t)hv[i].dv.width1 * hv[i].dv.height1;
         if (pixels_host && pixels_host[i])
-            HIP_TRY(hipMemcpyAsync(pixels_host[i], reqs_h[i].v.pixels, px * 24, hipMemcpyDeviceToHost, g.stream));
+            HIP_TRY(hipMemcpyAsync(pixels_host[i], reqs_h[i].v.pixels, px * 24, hipMemcpyDeviceToHost, c->stream));
         if (hits_host && hits_host[i] && want_hits)
-            HIP_TRY(hipMemcpyAsync(hits_host[i], reqs_h[i].v.hits, px * 16, hipMemcpyDeviceToHost, g.stream));
+            HIP_TRY(hipMemcpyAsync(hits_host[i], reqs_h[i].v.hits, px * 16, hipMemcpyDeviceToHost, c->stream));
         if (reqs_h[i].bgra)
             HIP_TRY(hipMemcpyAsync(bgra_host[i], reqs_h[i].bgra,
                                    (size_t)(hv[i].dv.width1 - 1) * (hv[i].dv.height1 - 1) * 4, hipMemcpyDeviceToHost,
-                                   g.stream));
+                                   c->stream));
     }
-    std::vector<unsigned long long> cnt_h(3 * (size_t)n);
-    HIP_TRY(hipMemcpyAsync(cnt_h.data(), cnt_d, 24ull * n, hipMemcpyDeviceToHost, g.stream));
-    HIP_TRY(hipStreamSynchronize(g.stream));
+    HIP_TRY(hipMemcpyAsync(cnt_h, cnt_d, cnt_bytes, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    if (cnt_h[3 * (size_t)n])
+        return set_err(ST_ERR_SCENE, "a traversal ran past its bound (fault bits 0x%llx): malformed BVH",
+                       cnt_h[3 * (size_t)n]);
     if (counts)
         for (uint32_t i = 0; i < n; i++) {
             counts[i].primary = (uint64_t)hv[i].dv.width1 * hv[i].dv.height1;
             counts[i].shadow = cnt_h[3 * i + 1];
             counts[i].bounce = cnt_h[3 * i + 2];
         }
-    if (kernel_ms) HIP_TRY(hipEventElapsedTime(kernel_ms, g.ev[0], g.ev[1]));
-    g.stage_n = marks ? marks - 1 : 0;
-    for (uint32_t k = 0; k < g.stage_n; k++)
-        HIP_TRY(hipEventElapsedTime(&g.stage_ms[k], g.stage_ev[k], g.stage_ev[k + 1]));
+    if (kernel_ms) HIP_TRY(hipEventElapsedTime(kernel_ms, c->ev[0], c->ev[1]));
+    c->stage_n = marks ? marks - 1 : 0;
+    for (uint32_t k = 0; k < c->stage_n; k++)
+        HIP_TRY(hipEventElapsedTime(&c->stage_ms[k], c->stage_ev[k], c->stage_ev[k + 1]));
     return rc;
+}
+
+int st_rt_launch_prepared(StSceneHandle *const *scenes, StHostView *hv, uint32_t n, int want_hits, int persistent,
+                          StRayCounts *counts, StFloatPixel *const *pixels_host, StHitRec *const *hits_host,
+                          uint32_t *const *bgra_host, float *kernel_ms) {
+    RT_LOCK;
+    return st_rt_launch_ctx(&g.main, scenes, hv, n, want_hits, persistent, counts, pixels_host, hits_host, bgra_host,
+                            kernel_ms);
 }
 
 static int launch(StSceneHandle *const *scenes, const StView *views, uint32_t n, int want_hits, int persistent,
@@ -1567,9 +1690,15 @@ extern "C" int st_render(StSceneHandle *scene, const StView *view, StFloatPixel 
 
 /* Graph mode for the duration of one API call (ST_BATCH_GRAPH). */
 struct GraphScope {
-    explicit GraphScope(uint32_t flags) { g.graph_mode = (flags & ST_BATCH_GRAPH) != 0; }
-    ~GraphScope() { g.graph_mode = 0; }
+    explicit GraphScope(uint32_t flags) { g.main.graph_mode = (flags & ST_BATCH_GRAPH) != 0; }
+    ~GraphScope() { g.main.graph_mode = 0; }
 };
+
+extern "C" int st_debug_step_cap(uint32_t cap) {
+    RT_LOCK;
+    g.step_cap = cap;
+    return ST_OK;
+}
 
 extern "C" int st_graph_stats(uint64_t *captures, uint64_t *launches) {
     RT_LOCK;
@@ -1594,18 +1723,18 @@ extern "C" int st_render_timed(StSceneHandle *const *scenes, const StView *views
                                float *kernel_ms, StRayCounts *counts) {
     RT_LOCK;
     GraphScope gs(flags);
-    g.stage_timing = (flags & ST_BATCH_STAGE_TIMES) != 0;
-    g.stage_n = 0;
+    g.main.stage_timing = (flags & ST_BATCH_STAGE_TIMES) != 0;
+    g.main.stage_n = 0;
     int rc = launch(scenes, views, n, 0, (flags & ST_BATCH_PERSISTENT) ? 1 : 0, counts, NULL, NULL, NULL, kernel_ms);
-    g.stage_timing = 0;
+    g.main.stage_timing = 0;
     return rc;
 }
 
 extern "C" int st_last_stage_ms(float *ms, uint32_t cap, uint32_t *count) {
     RT_LOCK;
     if (!count) return set_err(ST_ERR_ARG, "null argument");
-    *count = g.stage_n;
-    for (uint32_t k = 0; k < g.stage_n && k < cap && ms; k++) ms[k] = g.stage_ms[k];
+    *count = g.main.stage_n;
+    for (uint32_t k = 0; k < g.main.stage_n && k < cap && ms; k++) ms[k] = g.main.stage_ms[k];
     return ST_OK;
 }
 
@@ -1628,15 +1757,15 @@ extern "C" int st_ssb(StSceneHandle *scene, const StView *view, int32_t *rects, 
     const uint32_t P = scene->host->prim_count;
     const size_t o_sc = round256(sizeof(KReq)), up = o_sc + round256(sizeof(StDScene));
     const size_t gate_b = round256((P + 1) * 16ull);
-    if ((rc = arena_reserve(&g.work, up + 2 * gate_b + round256(P + 1) + 512)) || (rc = staging_reserve(up))) {
+    if ((rc = arena_reserve(&g.main.work, up + 2 * gate_b + round256(P + 1) + 512)) || (rc = staging_reserve(&g.main, up))) {
         st_view_release(&hv);
         return rc;
     }
-    char *blk = (char *)arena_take(&g.work, up);
-    int32_t *gate_d = (int32_t *)arena_take(&g.work, gate_b);
-    int32_t *raw_d = (int32_t *)arena_take(&g.work, gate_b);
-    uint8_t *flags_d = (uint8_t *)arena_take(&g.work, P + 1);
-    KReq *rq = (KReq *)g.staging;
+    char *blk = (char *)arena_take(&g.main.work, up);
+    int32_t *gate_d = (int32_t *)arena_take(&g.main.work, gate_b);
+    int32_t *raw_d = (int32_t *)arena_take(&g.main.work, gate_b);
+    uint8_t *flags_d = (uint8_t *)arena_take(&g.main.work, P + 1);
+    KReq *rq = (KReq *)g.main.staging;
     memset(rq, 0, sizeof(KReq));
     rq->v = hv.dv;
     rq->v.rects = gate_d;
@@ -1647,15 +1776,15 @@ extern "C" int st_ssb(StSceneHandle *scene, const StView *view, int32_t *rects, 
     rq->device_ssb = 1;
     rq->ssb_raw = raw_d;
     rq->ssb_flags = flags_d;
-    memcpy(g.staging + o_sc, &scene->dev, sizeof(StDScene));
+    memcpy(g.main.staging + o_sc, &scene->dev, sizeof(StDScene));
     st_view_release(&hv);
     if (P == 0) return ST_OK;
-    HIP_TRY(hipMemcpyAsync(blk, g.staging, up, hipMemcpyHostToDevice, g.stream));
-    hipLaunchKernelGGL(k_ssb, dim3(1), dim3(256), 0, g.stream, (const StDScene *)(blk + o_sc), (const KReq *)blk);
+    HIP_TRY(hipMemcpyAsync(blk, g.main.staging, up, hipMemcpyHostToDevice, g.main.stream));
+    hipLaunchKernelGGL(k_ssb, dim3(1), dim3(256), 0, g.main.stream, (const StDScene *)(blk + o_sc), (const KReq *)blk);
     HIP_TRY(hipGetLastError());
-    if (rects) HIP_TRY(hipMemcpyAsync(rects, raw_d, P * 16ull, hipMemcpyDeviceToHost, g.stream));
-    if (flags) HIP_TRY(hipMemcpyAsync(flags, flags_d, P, hipMemcpyDeviceToHost, g.stream));
-    HIP_TRY(hipStreamSynchronize(g.stream));
+    if (rects) HIP_TRY(hipMemcpyAsync(rects, raw_d, P * 16ull, hipMemcpyDeviceToHost, g.main.stream));
+    if (flags) HIP_TRY(hipMemcpyAsync(flags, flags_d, P, hipMemcpyDeviceToHost, g.main.stream));
+    HIP_TRY(hipStreamSynchronize(g.main.stream));
     return ST_OK;
 }
 
@@ -1671,20 +1800,25 @@ extern "C" int st_pick(StSceneHandle *scene, const StView *view, uint32_t n, con
     const uint32_t P = scene->host->prim_count;
     /* one upload block: request, scene descriptor, rects, points */
     const size_t o_sc = round256(sizeof(KReq)), o_rect = o_sc + round256(sizeof(StDScene)),
-                 o_xy = o_rect + round256((P + 1) * 16ull), up = o_xy + round256(n * 8ull);
-    if ((rc = arena_reserve(&g.work, up + round256(n * sizeof(StPick)) + 512)) || (rc = staging_reserve(up))) {
+                 o_xy = o_rect + round256((P + 1) * 16ull), o_fault = o_xy + round256(n * 8ull),
+                 up = o_fault + 256;
+    if ((rc = arena_reserve(&g.main.work, up + round256(n * sizeof(StPick)) + 512)) || (rc = staging_reserve(&g.main, up))) {
         st_view_release(&hv);
         return rc;
     }
-    char *blk = (char *)arena_take(&g.work, up);
-    StPick *out_d = (StPick *)arena_take(&g.work, n * sizeof(StPick));
-    KReq *rq = (KReq *)g.staging;
+    char *blk = (char *)arena_take(&g.main.work, up);
+    StPick *out_d = (StPick *)arena_take(&g.main.work, n * sizeof(StPick));
+    KReq *rq = (KReq *)g.main.staging;
     memset(rq, 0, sizeof(KReq));
     rq->v = hv.dv;
     rq->v.rects = (const int32_t *)(blk + o_rect);
-    memcpy(g.staging + o_sc, &scene->dev, sizeof(StDScene));
-    memcpy(g.staging + o_rect, hv.rects, P * 16ull);
-    memcpy(g.staging + o_xy, xy, n * 8ull);
+    StDScene dsc = scene->dev;
+    dsc.fault = (uint32_t *)(blk + o_fault);
+    dsc.step_cap = g.step_cap;
+    memcpy(g.main.staging + o_sc, &dsc, sizeof(StDScene));
+    memset(g.main.staging + o_fault, 0, 4);
+    memcpy(g.main.staging + o_rect, hv.rects, P * 16ull);
+    memcpy(g.main.staging + o_xy, xy, n * 8ull);
     KPick pk;
     memcpy(pk.start, hv.start, 12);
     memcpy(pk.right, hv.right, 12);
@@ -1697,12 +1831,15 @@ extern "C" int st_pick(StSceneHandle *scene, const StView *view, uint32_t n, con
     pk.out = out_d;
     st_view_release(&hv);
     const uint32_t m_entries = scene->dev.mesh_stack_size > 1 ? scene->dev.mesh_stack_size : 1;
-    HIP_TRY(hipMemcpyAsync(blk, g.staging, up, hipMemcpyHostToDevice, g.stream));
-    hipLaunchKernelGGL(k_pick, dim3((n + 63) / 64), dim3(64), (size_t)64 * (1 + m_entries) * 4, g.stream,
+    HIP_TRY(hipMemcpyAsync(blk, g.main.staging, up, hipMemcpyHostToDevice, g.main.stream));
+    hipLaunchKernelGGL(k_pick, dim3((n + 63) / 64), dim3(64), (size_t)64 * (1 + m_entries) * 4, g.main.stream,
                        (const StDScene *)(blk + o_sc), (const KReq *)blk, pk, m_entries);
     HIP_TRY(hipGetLastError());
-    HIP_TRY(hipMemcpyAsync(out, out_d, n * sizeof(StPick), hipMemcpyDeviceToHost, g.stream));
-    HIP_TRY(hipStreamSynchronize(g.stream));
+    HIP_TRY(hipMemcpyAsync(out, out_d, n * sizeof(StPick), hipMemcpyDeviceToHost, g.main.stream));
+    uint32_t fault = 0;
+    HIP_TRY(hipMemcpyAsync(&fault, blk + o_fault, 4, hipMemcpyDeviceToHost, g.main.stream));
+    HIP_TRY(hipStreamSynchronize(g.main.stream));
+    if (fault) return set_err(ST_ERR_SCENE, "a traversal ran past its bound (fault bits 0x%x): malformed BVH", fault);
     return ST_OK;
 }
 
@@ -1713,24 +1850,24 @@ extern "C" int st_resolve_bgra8(const StFloatPixel *grid, uint32_t grid_w, uint3
     int rc = ensure_init();
     if (rc) return rc;
     size_t gbytes = (size_t)grid_w * grid_h * 24, opx = (size_t)(grid_w - 1) * (grid_h - 1);
-    if ((rc = arena_reserve(&g.work, round256(sizeof(KReq)) + round256(gbytes) + round256(opx * 4) + 256))) return rc;
-    if ((rc = staging_reserve(round256(sizeof(KReq))))) return rc;
-    KReq *req_d = (KReq *)arena_take(&g.work, sizeof(KReq));
-    void *grid_d = arena_take(&g.work, gbytes);
-    uint32_t *out_d = (uint32_t *)arena_take(&g.work, opx * 4);
-    KReq *req_h = (KReq *)g.staging;
+    if ((rc = arena_reserve(&g.main.work, round256(sizeof(KReq)) + round256(gbytes) + round256(opx * 4) + 256))) return rc;
+    if ((rc = staging_reserve(&g.main, round256(sizeof(KReq))))) return rc;
+    KReq *req_d = (KReq *)arena_take(&g.main.work, sizeof(KReq));
+    void *grid_d = arena_take(&g.main.work, gbytes);
+    uint32_t *out_d = (uint32_t *)arena_take(&g.main.work, opx * 4);
+    KReq *req_h = (KReq *)g.main.staging;
     memset(req_h, 0, sizeof(KReq));
     req_h->v.pixels = grid_d;
     req_h->v.width1 = grid_w;
     req_h->v.height1 = grid_h;
     req_h->bgra = out_d;
-    HIP_TRY(hipMemcpyAsync(grid_d, grid, gbytes, hipMemcpyHostToDevice, g.stream));
-    HIP_TRY(hipMemcpyAsync(req_d, req_h, sizeof(KReq), hipMemcpyHostToDevice, g.stream));
-    hipLaunchKernelGGL(k_resolve, dim3((uint32_t)((opx + 255) / 256)), dim3(256), 0, g.stream, (const KReq *)req_d, 1u,
+    HIP_TRY(hipMemcpyAsync(grid_d, grid, gbytes, hipMemcpyHostToDevice, g.main.stream));
+    HIP_TRY(hipMemcpyAsync(req_d, req_h, sizeof(KReq), hipMemcpyHostToDevice, g.main.stream));
+    hipLaunchKernelGGL(k_resolve, dim3((uint32_t)((opx + 255) / 256)), dim3(256), 0, g.main.stream, (const KReq *)req_d, 1u,
                        (uint32_t)opx);
     HIP_TRY(hipGetLastError());
-    HIP_TRY(hipMemcpyAsync(out, out_d, opx * 4, hipMemcpyDeviceToHost, g.stream));
-    HIP_TRY(hipStreamSynchronize(g.stream));
+    HIP_TRY(hipMemcpyAsync(out, out_d, opx * 4, hipMemcpyDeviceToHost, g.main.stream));
+    HIP_TRY(hipStreamSynchronize(g.main.stream));
     return ST_OK;
 }
 

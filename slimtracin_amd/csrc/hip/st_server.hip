@@ -1,13 +1,17 @@
 /*
  * st_server.hip - serving loop (include/slimtracin_server.h).
  *
- * One worker thread per server. Requests are queued FIFO with a ticket; the
- * worker takes everything queued (up to the batch limits), prepares each
- * request's camera set-up on the host, and runs the whole batch as one
- * persistent launch (st_rt_launch_prepared: one tile list over all
- * requests, one upload transfer, device display resolve, downloads into
- * pinned per-ticket buffers). Requests that arrive while a launch runs form
- * the next batch. Scenes are admitted against an HBM byte budget.
+ * Requests are queued FIFO with a ticket. The server runs max_in_flight
+ * workers, each with a launch context of its own (stream, per-launch
+ * buffers, pinned staging): a free worker takes everything queued (up to
+ * the batch limits), prepares each request's camera set-up on the host and
+ * runs the batch as one persistent launch on its context (one tile list over
+ * all requests, one upload transfer, device display resolve, downloads into
+ * pinned per-ticket buffers). Requests that arrive while launches run are
+ * taken by the next free worker and launched at once on its own stream, so
+ * their work overlaps on the device with the launches already there; a
+ * launch that runs long holds only its own requests. Scenes are admitted
+ * against an HBM byte budget.
  */
 #include <hip/hip_runtime.h>
 
@@ -98,10 +102,14 @@ struct StServer {
     uint64_t next_ticket = 1;
     std::vector<Resident> scenes;
     uint64_t resident = 0;
-    std::thread worker;
+    std::vector<std::thread> workers;
+    std::vector<LaunchCtx *> ctxs;    /* one per worker */
+    uint32_t batch_limit = 0;         /* st_server_set_limits: requests per launch */
+    uint32_t active_limit = 0;        /* st_server_set_limits: workers that take work */
+    uint32_t in_flight = 0, max_in_flight_seen = 0;
     bool stop = false;
     PinnedPool pool;
-    uint64_t submitted = 0, completed = 0, launches = 0, rays = 0, batched = 0;
+    uint64_t submitted = 0, completed = 0, launches = 0, rays = 0, batched = 0, overlapped = 0;
     double prep_ms = 0, launch_ms = 0, kernel_ms = 0, alloc_ms = 0;
     std::vector<float> latencies;     /* ms, completed requests */
 };
@@ -113,8 +121,8 @@ static size_t output_bytes(const StServerConfig &cfg, const StView &v) {
     return (size_t)v.width * v.height * 4;
 }
 
-static void worker_loop(StServer *srv) {
-    const uint32_t max_req = srv->cfg.max_batch_requests;
+static void worker_loop(StServer *srv, uint32_t index) {
+    LaunchCtx *ctx = srv->ctxs[index];
     const uint64_t max_px = srv->cfg.max_batch_pixels;
     for (;;) {
         std::vector<uint64_t> ids;
@@ -123,8 +131,12 @@ static void worker_loop(StServer *srv) {
         std::vector<void *> bufs;
         {
             std::unique_lock<std::mutex> lk(srv->mu);
-            srv->cv_work.wait(lk, [&] { return srv->stop || !srv->queue.empty(); });
-            if (srv->queue.empty()) break;  /* stop requested and drained */
+            srv->cv_work.wait(lk, [&] {
+                return (srv->stop && (srv->queue.empty() || index >= srv->active_limit)) ||
+                       (!srv->queue.empty() && index < srv->active_limit);
+            });
+            if (srv->queue.empty() || index >= srv->active_limit) break;  /* stop requested and drained */
+            const uint32_t max_req = srv->batch_limit;
             uint64_t px = 0;
             while (!srv->queue.empty() && ids.size() < max_req) {
                 Ticket &t = srv->tickets[srv->queue.front()];
@@ -137,7 +149,12 @@ static void worker_loop(StServer *srv) {
                 bufs.push_back(t.buf);
                 px += p;
             }
+            if (srv->in_flight) srv->overlapped++;
+            srv->in_flight++;
+            if (srv->in_flight > srv->max_in_flight_seen) srv->max_in_flight_seen = srv->in_flight;
         }
+        /* another worker may take what this one left queued */
+        srv->cv_work.notify_one();
         const size_t n = ids.size();
         const double t_prep = now_ms();
         std::vector<StHostView> hv(n);
@@ -166,9 +183,9 @@ static void worker_loop(StServer *srv) {
                 if (srv->cfg.output == ST_OUTPUT_FLOAT_PIXELS) fp[k] = (StFloatPixel *)bufs[live[k]];
                 else bp[k] = (uint32_t *)bufs[live[k]];
             }
-            rc = st_rt_launch_prepared(lh.data(), lv.data(), (uint32_t)m, 0, 1, lc.data(),
-                                       srv->cfg.output == ST_OUTPUT_FLOAT_PIXELS ? fp.data() : NULL, NULL,
-                                       srv->cfg.output == ST_OUTPUT_FLOAT_PIXELS ? NULL : bp.data(), &kms);
+            rc = st_rt_launch_ctx(ctx, lh.data(), lv.data(), (uint32_t)m, 0, 1, lc.data(),
+                                  srv->cfg.output == ST_OUTPUT_FLOAT_PIXELS ? fp.data() : NULL, NULL,
+                                  srv->cfg.output == ST_OUTPUT_FLOAT_PIXELS ? NULL : bp.data(), &kms);
             for (size_t k = 0; k < m; k++) {
                 counts[live[k]] = lc[k];
                 if (rc) status[live[k]] = rc;
@@ -178,6 +195,7 @@ static void worker_loop(StServer *srv) {
         const double t = now_ms();
         {
             std::lock_guard<std::mutex> lk(srv->mu);
+            srv->in_flight--;
             if (!live.empty()) {
                 srv->launches++;
                 srv->batched += live.size();
@@ -215,6 +233,13 @@ extern "C" int st_server_create(const StServerConfig *cfg, StServer **out) {
     }
     if (!srv->cfg.max_batch_requests) srv->cfg.max_batch_requests = 256;
     if (!srv->cfg.max_batch_pixels) srv->cfg.max_batch_pixels = 64ull << 20;
+    if (!srv->cfg.max_in_flight) srv->cfg.max_in_flight = 3;
+    if (srv->cfg.max_in_flight > ST_SERVER_MAX_IN_FLIGHT) {
+        delete srv;
+        return fail(ST_ERR_ARG, "max_in_flight above ST_SERVER_MAX_IN_FLIGHT");
+    }
+    srv->batch_limit = srv->cfg.max_batch_requests;
+    srv->active_limit = srv->cfg.max_in_flight;
     if (!srv->cfg.hbm_budget) {
         size_t free_b = 0, total_b = 0;
         if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) {
@@ -223,8 +248,29 @@ extern "C" int st_server_create(const StServerConfig *cfg, StServer **out) {
         }
         srv->cfg.hbm_budget = (uint64_t)(free_b * 0.85);
     }
-    srv->worker = std::thread(worker_loop, srv);
+    for (uint32_t i = 0; i < srv->cfg.max_in_flight; i++) {
+        LaunchCtx *c = NULL;
+        if ((rc = st_rt_ctx_create(&c))) {
+            for (LaunchCtx *x : srv->ctxs) st_rt_ctx_destroy(x);
+            delete srv;
+            return rc;
+        }
+        srv->ctxs.push_back(c);
+    }
+    for (uint32_t i = 0; i < srv->cfg.max_in_flight; i++) srv->workers.emplace_back(worker_loop, srv, i);
     *out = srv;
+    return ST_OK;
+}
+
+extern "C" int st_server_set_limits(StServer *srv, uint32_t max_batch_requests, uint32_t max_in_flight) {
+    if (!srv) return fail(ST_ERR_ARG, "null server");
+    if (max_in_flight > srv->cfg.max_in_flight) return fail(ST_ERR_ARG, "max_in_flight above the server's");
+    {
+        std::lock_guard<std::mutex> lk(srv->mu);
+        srv->batch_limit = max_batch_requests ? max_batch_requests : srv->cfg.max_batch_requests;
+        srv->active_limit = max_in_flight ? max_in_flight : srv->cfg.max_in_flight;
+    }
+    srv->cv_work.notify_all();
     return ST_OK;
 }
 
@@ -235,7 +281,9 @@ extern "C" void st_server_destroy(StServer *srv) {
         srv->stop = true;
     }
     srv->cv_work.notify_all();
-    if (srv->worker.joinable()) srv->worker.join();
+    for (auto &w : srv->workers)
+        if (w.joinable()) w.join();
+    for (LaunchCtx *c : srv->ctxs) st_rt_ctx_destroy(c);
     for (auto &kv : srv->tickets) srv->pool.put(kv.second.buf, kv.second.cls);
     for (auto &r : srv->scenes)
         if (r.h) st_scene_destroy(r.h);
@@ -394,6 +442,8 @@ extern "C" int st_server_stats(StServer *srv, StServerStats *s) {
         s->launch_ms = srv->launch_ms;
         s->kernel_ms = srv->kernel_ms;
         s->alloc_ms = srv->alloc_ms;
+        s->overlapped_launches = srv->overlapped;
+        s->max_in_flight = srv->max_in_flight_seen;
         lat = srv->latencies;
     }
     if (!lat.empty()) {

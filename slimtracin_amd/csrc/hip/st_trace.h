@@ -575,6 +575,12 @@ DFN bool hit_triangles(DTrace &tr, const StDTriHot *tris, uint32_t f0, uint32_t 
     return found;
 }
 
+/* A traversal ran past a bound that no valid tree reaches: record it for
+   the host (the launch then fails with an error) and stop. */
+DFN void trav_fault(const StDScene &sc, uint32_t code) {
+    if (sc.fault) atomicOr(sc.fault, code);
+}
+
 /* intersection/mesh.h:38-111. A leaf root holds every triangle; otherwise
    each step tests both children's boxes, then the triangles of the children
    that are leaves (left first), then descends. The leaf-root case runs
@@ -596,7 +602,17 @@ DNOINLINE bool trace_mesh(DTrace &tr, const StDScene &sc, uint32_t mesh_id, bool
     const bool root_leaf = root_count != 0;
     uint32_t left = root_first;
     uint32_t stack_size = 0;
+    /* each step loads one node pair, and a traversal loads a pair at most
+       once: more steps than the mesh has nodes (at most 2T - 1) means a
+       malformed tree, which would otherwise loop for ever */
+    uint32_t max_steps = 2u * mesh.triangle_count + 2u;
+    if (sc.step_cap && sc.step_cap < max_steps) max_steps = sc.step_cap;
+    uint32_t steps = 0;
     for (;;) {
+        if (++steps > max_steps) {
+            trav_fault(sc, ST_FAULT_MESH_STEPS);
+            return false;
+        }
         uint32_t f0 = 0, c0 = 0, f1 = 0, c1 = 0, lnext = 0, rnext = 0;
         bool has_left = false, has_right = false;
         if (root_leaf) {
@@ -771,7 +787,16 @@ DNOINLINE bool trace_scene(DRay &ray, DTrace &tr, const StDScene &sc, bool any_h
     uint32_t left = root_first;
     uint32_t *stack = tr.scene_stack;
     uint32_t stack_size = 0;
+    /* bounds no valid tree reaches (see trace_mesh); the stack holds at
+       most the tree's height, scene_stack_need - 1 entries */
+    uint32_t max_steps = 2u * sc.prim_count + 2u;
+    if (sc.step_cap && sc.step_cap < max_steps) max_steps = sc.step_cap;
+    uint32_t steps = 0;
     for (;;) {
+        if (++steps > max_steps) {
+            trav_fault(sc, ST_FAULT_SCENE_STEPS);
+            break;
+        }
         uint32_t f0 = 0, c0 = 0, f1 = 0, c1 = 0, nleaf = 0, lnext = 0, rnext = 0;
         bool has_left = false, has_right = false;
         if (root_leaf) {
@@ -802,6 +827,10 @@ DNOINLINE bool trace_scene(DRay &ray, DTrace &tr, const StDScene &sc, bool any_h
         if (has_left) {
             if (has_right) {
                 if (!any_hit && ld > rd) { uint32_t t = lnext; lnext = rnext; rnext = t; }
+                if (stack_size + 1u >= sc.scene_stack_need) {
+                    trav_fault(sc, ST_FAULT_SCENE_STACK);
+                    break;
+                }
                 stack[ST_STACK_STRIDE * stack_size++] = rnext;
             }
             left = lnext;

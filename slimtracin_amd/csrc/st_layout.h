@@ -100,7 +100,18 @@ typedef struct StDScene {
     uint32_t has_emissive_quads;
     uint32_t scene_stack_need;     /* entries the scene-BVH traversal stack can reach */
     const float *mesh_aabbs;       /* 6 floats per mesh: the bound the SSB pass uses */
+    uint32_t *fault;               /* per launch (set in the launch's scene table): a traversal that runs
+                                      past its step bound or stack ORs ST_FAULT_* in and stops */
+    uint32_t step_cap;             /* per launch: 0, or a lower step bound (st_debug_step_cap) */
+    uint32_t pad_[3];
 } StDScene;
+
+/* Fault bits of StDScene.fault. A traversal of a valid tree never sets them:
+   it loads each node pair at most once (so takes at most as many steps as the
+   tree has nodes) and pushes at most its height. */
+#define ST_FAULT_MESH_STEPS 1u
+#define ST_FAULT_SCENE_STEPS 2u
+#define ST_FAULT_SCENE_STACK 4u
 
 /* One render request on the device. */
 typedef struct StDView {

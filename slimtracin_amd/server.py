@@ -14,7 +14,7 @@ ST_ERR_BUSY = -5
 
 class ServerConfig(C.Structure):
     _fields_ = [("hbm_budget", C.c_uint64), ("max_batch_requests", C.c_uint32), ("output", C.c_uint32),
-                ("max_batch_pixels", C.c_uint64)]
+                ("max_batch_pixels", C.c_uint64), ("max_in_flight", C.c_uint32), ("reserved", C.c_uint32)]
 
 
 class ServerStats(C.Structure):
@@ -23,16 +23,21 @@ class ServerStats(C.Structure):
                 ("resident_scenes", C.c_uint32), ("queued", C.c_uint32), ("latency_p50_ms", C.c_double),
                 ("latency_p99_ms", C.c_double), ("latency_max_ms", C.c_double), ("mean_batch", C.c_double),
                 ("prep_ms", C.c_double), ("launch_ms", C.c_double), ("kernel_ms", C.c_double),
-                ("alloc_ms", C.c_double)]
+                ("alloc_ms", C.c_double), ("overlapped_launches", C.c_uint64), ("max_in_flight", C.c_uint32),
+                ("reserved", C.c_uint32)]
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}
 
 
 class Server:
-    def __init__(self, hbm_budget=0, max_batch_requests=0, output=OUTPUT_BGRA8, max_batch_pixels=0):
+    """Continuous-batching server (st_server_*): max_in_flight launches run at
+    once, each on its own stream; 0 = the library default (3)."""
+
+    def __init__(self, hbm_budget=0, max_batch_requests=0, output=OUTPUT_BGRA8, max_batch_pixels=0,
+                 max_in_flight=0):
         self.output = output
-        cfg = ServerConfig(hbm_budget, max_batch_requests, output, max_batch_pixels)
+        cfg = ServerConfig(hbm_budget, max_batch_requests, output, max_batch_pixels, max_in_flight, 0)
         h = C.c_void_p()
         _check(lib().st_server_create(C.byref(cfg), C.byref(h)))
         self.handle = h
@@ -93,6 +98,32 @@ class Server:
 
     def release(self, ticket: int):
         _check(lib().st_server_release(self.handle, C.c_uint64(ticket)))
+
+    def set_limits(self, max_batch_requests: int = 0, max_in_flight: int = 0):
+        """Requests per launch and launches in flight (0: the configured value)."""
+        _check(lib().st_server_set_limits(self.handle, C.c_uint32(max_batch_requests), C.c_uint32(max_in_flight)))
+
+    def plain_rate(self, views: np.ndarray, scene_ids) -> dict:
+        """The requests served as plain launches - one request per launch, one
+        launch at a time - over the same resident scenes; requests/s and
+        Mrays/s (the limits are restored afterwards)."""
+        import time
+        self.set_limits(1, 1)
+        try:
+            t0 = time.perf_counter()
+            ts = [self.submit(scene_ids[k], views[k:k + 1]) for k in range(len(views))]
+            rays = 0
+            lat = []
+            for t in ts:
+                _, c, l_ms = self.wait(t, copy=False)
+                rays += c.primary + c.shadow + c.bounce
+                lat.append(l_ms)
+                self.release(t)
+            dt = time.perf_counter() - t0
+        finally:
+            self.set_limits(0, 0)
+        return {"requests": len(views), "req_per_s": round(len(views) / dt, 1), "mrays": round(rays / dt / 1e6, 2),
+                "wall_s": round(dt, 3)}
 
     def stats(self) -> dict:
         s = ServerStats()

@@ -90,6 +90,60 @@ def oracle_render(blob, view, threads=1, hits=True):
     return pixels, hrec, st
 
 
+def oracle_render_steps(blob, view, threads=1):
+    """oracle_render plus each pixel's traversal events (node pairs +
+    triangle tests + primitive visits over all its rays)."""
+    lib = _load(ORACLE_SO)
+    n = grid_size(view)
+    pixels = np.zeros(n, dtype=PIXEL_DTYPE)
+    hrec = np.zeros(n, dtype=HIT_DTYPE)
+    steps = np.zeros(n, dtype=np.uint64)
+    st = OracleStats()
+    bp, bs, keep = _blob(blob)
+    rc = lib.oracle_render_steps(bp, bs, _ptr(view), _ptr(pixels), _ptr(hrec), _ptr(steps), C.c_int(threads),
+                                 C.byref(st))
+    assert rc == 0, "oracle_render_steps failed"
+    return pixels, hrec, steps, st
+
+
+class OracleScene:
+    """A scene blob loaded into the oracle once, rendered from many views.
+    mesh_lbvh: meshes without a prebuilt BVH get the device's Morton LBVH
+    (leaves <= 4), as DeviceScene(mesh_lbvh=True) builds them."""
+
+    def __init__(self, blob, mesh_lbvh=False):
+        lib = _load(ORACLE_SO)
+        lib.oracle_scene_open.restype = C.c_void_p
+        self._keep = np.frombuffer(blob, dtype=np.uint8)
+        self.handle = lib.oracle_scene_open(_ptr(self._keep), C.c_size_t(len(blob)), C.c_uint32(1 if mesh_lbvh else 0))
+        assert self.handle, "oracle_scene_open failed"
+        self._keep = None
+
+    def render(self, view, threads=1, hits=True, pixels=True):
+        """(pixels or None, hits or None, OracleStats, seconds of the render call)."""
+        n = grid_size(view)
+        px = np.zeros(n, dtype=PIXEL_DTYPE)
+        hr = np.zeros(n, dtype=HIT_DTYPE) if hits else None
+        st = OracleStats()
+        secs = C.c_double()
+        rc = _load(ORACLE_SO).oracle_scene_render(C.c_void_p(self.handle), _ptr(view), _ptr(px),
+                                                  _ptr(hr) if hits else None, C.c_int(threads), C.byref(st),
+                                                  C.byref(secs))
+        assert rc == 0
+        return (px if pixels else None), hr, st, secs.value
+
+    def close(self):
+        if self.handle:
+            _load(ORACLE_SO).oracle_scene_close(C.c_void_p(self.handle))
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def oracle_render_timed(blob, view, threads, frames):
     lib = _load(ORACLE_SO)
     pixels = np.zeros(grid_size(view), dtype=PIXEL_DTYPE)

@@ -172,3 +172,61 @@ def test_pinned_outputs_equal_pageable():
             assert (c[i] == d[i]).all()
     finally:
         ds.close()
+
+
+def test_streamed_arrivals_overlap_and_equal_plain():
+    """Requests arriving one by one while launches run: free workers launch
+    them at once on their own streams (launches overlap on the device) and
+    every frame still equals the plain launch's."""
+    import time
+    sc = _scene_set()
+    blobs = [s.to_blob() for s, _ in sc]
+    cams = [{k: v for k, v in c.items() if k != "depth"} for _, c in sc]
+    plain = [st.DeviceScene(b) for b in blobs]
+    srv = Server(output=OUTPUT_FLOAT_PIXELS, max_in_flight=3)
+    try:
+        ids = [srv.add_scene(b) for b in blobs]
+        reqs = _views(np.random.default_rng(23), cams, 60)
+        tickets = []
+        for sid, v in reqs:
+            tickets.append(srv.submit(ids[sid], v))
+            time.sleep(0.0005)
+        for (sid, v), t in zip(reqs, tickets):
+            img, counts, _ = srv.wait(t)
+            srv.release(t)
+            pix, _, c = st.render(plain[sid], v, hits=False)
+            assert img.tobytes() == pix.tobytes()
+            assert counts.primary + counts.shadow + counts.bounce == c.total()
+        stats = srv.stats()
+        print("streamed:", {k: stats[k] for k in ("launches", "overlapped_launches", "max_in_flight", "mean_batch")},
+              flush=True)
+        assert stats["completed"] == 60
+        assert 1 <= stats["max_in_flight"] <= 3
+    finally:
+        srv.close()
+        for d in plain:
+            d.close()
+
+
+def test_plain_limits_one_request_per_launch():
+    """set_limits(1, 1): one request per launch, one launch at a time (the
+    plain baseline over the same resident scenes), then back to batching."""
+    sc = _scene_set()[:2]
+    blobs = [s.to_blob() for s, _ in sc]
+    cams = [{k: v for k, v in c.items() if k != "depth"} for _, c in sc]
+    srv = Server(output=OUTPUT_BGRA8)
+    try:
+        ids = [srv.add_scene(b) for b in blobs]
+        reqs = _views(np.random.default_rng(4), cams, 12)
+        views = np.concatenate([v for _, v in reqs])
+        r = srv.plain_rate(views, [ids[sid] for sid, _ in reqs])
+        stats = srv.stats()
+        assert r["requests"] == 12 and stats["launches"] == 12 and stats["max_in_flight"] == 1
+        assert stats["overlapped_launches"] == 0
+        ts = [srv.submit(ids[sid], v) for sid, v in reqs]
+        for t in ts:
+            srv.wait(t, copy=False)
+            srv.release(t)
+        assert srv.stats()["completed"] == 24
+    finally:
+        srv.close()
