@@ -66,7 +66,16 @@ HBM_PEAK_GBS = 8000.0
 PBR_K = 100            # 20 * 100^2 = 200,000 triangles
 PBR_SEED = 2024
 PBR_CAM = dict(position=(0.0, 6.0, -14.0), pitch=-0.15)
-BATCH_TRIANGLES = [200_000, 320_000, 500_000, 800_000, 1_250_000, 2_000_000, 3_200_000, 5_000_000]
+# The 8 resident scenes of the batched workload: 4 of the single-mesh PBR
+# type (noise-displaced icosphere of 20k^2 triangles, GGX materials, glass
+# sphere, mirror quad, 4 point lights, 2 emissive quads) and 4 of the
+# instanced textured type (64 instances of 3 noise-displaced meshes, Perlin
+# albedo and normal maps with mip chains, sampled with ray cones).
+BATCH_PBR_K = [100, 200, 316, 500]                       # 200k, 800k, ~2.0M, 5.0M triangles
+BATCH_INSTANCED = [[50_000, 100_000, 150_000], [100_000, 200_000, 500_000],
+                   [200_000, 500_000, 1_000_000], [500_000, 1_000_000, 2_000_000]]
+BATCH_TEXTURE = 1024
+BATCH_CONE = 0.001                                       # ray-cone angle of requests on the textured scenes
 BATCH_SEED = 7
 BATCH_REQUESTS = 256
 CPU_SUBSET = 8
@@ -118,10 +127,18 @@ def request_mix(rng, nscenes, count):
 
 
 def batch_workload():
-    """The headline workload: the 8 scene blobs and the seeded 256-request mix."""
-    blobs = [st.gen_mesh_scene(100 + i, tri) for i, tri in enumerate(BATCH_TRIANGLES)]
+    """The headline workload: the 8 scene blobs (PBR type first, then the
+    instanced textured type) and the seeded 256-request mix; requests on the
+    textured scenes sample with ray cones."""
+    blobs = [st.gen_pbr(100 + i, k) for i, k in enumerate(BATCH_PBR_K)]
+    blobs += [st.gen_instanced(200 + i, 64, tris, BATCH_TEXTURE) for i, tris in enumerate(BATCH_INSTANCED)]
     views, sids = request_mix(np.random.default_rng(BATCH_SEED), len(blobs), BATCH_REQUESTS)
+    views["cone_angle"] = [BATCH_CONE if s >= len(BATCH_PBR_K) else 0.0 for s in sids]
     return blobs, views, sids
+
+
+def batch_scene_triangles():
+    return [20 * k * k for k in BATCH_PBR_K] + [sum(t) for t in BATCH_INSTANCED]
 
 
 def pct(a, q):
@@ -149,6 +166,7 @@ def c4_leg(args, blobs, views, sids):
         stage_runs.append(sms)
         rays += sum(c.total() for c in counts)
     elapsed = time.perf_counter() - t_start
+    redone = st.last_redo_count()
     rays_per_step = rays // args.steps
     names = st.stage_names(max(int(v) for v in views["depth"]))
     stage_ms = {n: round(float(np.mean([r[i] for r in stage_runs])), 4) for i, n in enumerate(names)}
@@ -212,6 +230,7 @@ def c4_leg(args, blobs, views, sids):
                 "speedup_vs_plain": round(plain_dl_s / batch_dl_s, 3),
             },
             "batched_equals_plain": same,
+            "redo_pixels": redone,
             "launch_shape": tune.as_dict(),
         },
     }
@@ -488,13 +507,14 @@ def served_leg(args):
     ids, admitted_tris, rejected = [], [], 0
     t0 = time.perf_counter()
     for i, tri in enumerate(tri_counts):
-        blob = st.gen_mesh_scene(500 + i, int(tri))
+        # the single-mesh PBR type: a 20k^2-triangle icosphere
+        blob = st.gen_pbr(500 + i, max(1, int(round(np.sqrt(tri / 20.0)))))
         sid = srv.try_add_scene(blob, mesh_lbvh=True)
         if sid is None:
             rejected += 1
         else:
             ids.append(sid)
-            admitted_tris.append(int(tri))
+            admitted_tris.append(20 * int(round(np.sqrt(tri / 20.0))) ** 2)
         del blob
         if (i + 1) % 8 == 0:
             log(f"served: {i + 1}/{len(tri_counts)} scenes admitted-or-rejected, {time.perf_counter() - t0:.1f} s")
@@ -645,9 +665,10 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": "batched_256_requests_8_scenes_0.2-5M_tris_mixed_res_depth1-6_4modes",
-                   "requests": BATCH_REQUESTS, "scenes": len(BATCH_TRIANGLES),
-                   "scene_triangles": BATCH_TRIANGLES, "rays_per_step": c4["rays_per_step"],
+        "config": {"workload": "batched_256_requests_8_scenes_pbr_and_instanced_textured_0.2-5M_tris_"
+                               "mixed_res_depth1-6_4modes",
+                   "requests": BATCH_REQUESTS, "scenes": len(BATCH_PBR_K) + len(BATCH_INSTANCED),
+                   "scene_unique_triangles": batch_scene_triangles(), "rays_per_step": c4["rays_per_step"],
                    "seed": BATCH_SEED, "spp": 1},
         "p50_ms": pct(c4["step_ms"], 50),
         "p99_ms": pct(c4["step_ms"], 99),
@@ -665,7 +686,7 @@ def main():
         result["gpu_vs_cpu"] = round(result["value"] / cpu["value"], 1)
         result["gpu_vs_cpu_single_core"] = round(result["value"] / cpu["single_core_mrays"], 1)
         stage = c4["stage_ms"]
-        dominant = max(stage, key=stage.get)
+        dominant = max(("primary", "bounce"), key=lambda k: stage.get(k, 0.0))
         if dominant == "primary":
             nbytes = agg.traversal_bytes("primary") + agg.traversal_bytes("shadow0")
             kname = "k_primary (primary rays and the first level's shadow rays, all 256 requests)"

@@ -4,14 +4,15 @@
  * Two groups of entry points:
  *
  * 1. The reference's host/device seam (SlimTracin core/init.h:36-126 and
- *    render/raytracer.h:258-309): allocateDeviceScene, uploadScene,
- *    uploadPrimitives, uploadMaterials, uploadLights, uploadSceneBVH,
- *    uploadMeshBVHs and renderSceneOnGPU take the reference's own `Scene` and
- *    `Viewport` structs, so a reference application links this library where
- *    it used to compile its CUDA device mirror. They keep the reference's
- *    `void` signatures; st_last_status() reports what the last one did.
- *    They are declared in slimtracin_seam.h, which needs the reference's
- *    core/types.h.
+ *    render/raytracer.h:258-299), in slimtracin_seam.h: one st_seam_*
+ *    function per function of the reference's device mirror
+ *    (allocateDeviceScene, uploadScene, uploadPrimitives, uploadMaterials,
+ *    uploadLights, uploadSceneBVH, uploadMeshBVHs) and renderSceneOnGPU,
+ *    with the same meaning, over the plain records of st_blob.h; each
+ *    returns an ST_* status. The reference-side binding that keeps the
+ *    reference's `void` functions over its own Scene / Viewport structs and
+ *    calls these is integration/init_amd.h (compiled with the reference's
+ *    headers).
  *
  * 2. The resident-scene API used by batching and serving: scenes are created
  *    once from a scene blob (st_blob.h) and stay in HBM; render requests
@@ -174,6 +175,17 @@ int st_last_stage_ms(float *ms, uint32_t cap, uint32_t *count);
    reference, whose traversal loops have no bound (trace.h:8-76,
    mesh.h:38-111). */
 int st_debug_step_cap(uint32_t cap);
+
+/* The redo stage: a mesh traversal of the main stages that takes more than
+   abandon_steps node-pair steps (0: ST_ABANDON_STEPS, 2048) gives up and its
+   pixel is traced again by a whole wave, which splits traversals of meshes
+   of at least coop_min_tris triangles (0: ST_COOP_MIN_TRIS, 4096) across its
+   lanes. Frames do not depend on either value; tests use tiny ones to send
+   every pixel through the redo stage. */
+int st_debug_redo(uint32_t abandon_steps, uint32_t coop_min_tris);
+/* Pixels the redo stage traced again in the last launch on the library's
+   main context (st_render, st_render_batch, st_render_timed). */
+int st_last_redo_count(uint32_t *count);
 
 /* Graphs captured and graph launches made so far (ST_BATCH_GRAPH). Not in
    the reference (the reference issues one blocking launch per frame). */

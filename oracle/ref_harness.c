@@ -233,6 +233,11 @@ static RefScene *refLoad(const void *blob, size_t size) {
 
     /* Builder sized like _initApp (app.h:277-279) then initBVHBuilder (builder_top_down.h:42-64). */
     rs->max_leaves = h->prim_count > max_tris ? h->prim_count : max_tris;
+#ifdef REF_HARNESS_BEFORE_BVH
+    /* a build with a device mirror: allocateDeviceScene and uploadScene come
+       before the BVH builds, in _initApp's order (app.h:160-199, 296-298) */
+    REF_HARNESS_BEFORE_BVH(s);
+#endif
     initBVHBuilder(&rs->builder, s, mem);
     initBVH(&s->bvh, h->prim_count, mem);
 

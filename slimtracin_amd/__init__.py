@@ -251,6 +251,19 @@ def pick(scene, view: np.ndarray, xy) -> np.ndarray:
     return out
 
 
+def debug_redo(abandon_steps: int = 0, coop_min_tris: int = 0):
+    """st_debug_redo: the main stages' give-up step count and the redo stage's
+    cooperative-traversal mesh size (0: the library defaults)."""
+    _check(lib().st_debug_redo(C.c_uint32(abandon_steps), C.c_uint32(coop_min_tris)))
+
+
+def last_redo_count() -> int:
+    """Pixels the redo stage traced again in the last launch (main context)."""
+    c = C.c_uint32()
+    _check(lib().st_last_redo_count(C.byref(c)))
+    return c.value
+
+
 def graph_stats():
     """(graphs captured, graph launches) so far in this process."""
     c, l = C.c_uint64(), C.c_uint64()
@@ -281,9 +294,10 @@ def render_timed(scenes, views: np.ndarray, persistent: bool = True, stages: boo
 def stage_names(levels: int):
     """Names of the stage launches of a frame whose Beauty depth loop has
     `levels` levels (the order render_timed(stages=True) reports). "primary"
-    includes the first level's shading entry, shadow rays and shading, and
-    "bounce" every bounce ray and every further level."""
-    return ["primary"] + (["bounce"] if levels > 1 else [])
+    includes the first level's shading entry, shadow rays and shading,
+    "bounce" every bounce ray and every further level, and "redo" the pixels
+    whose traversal gave up in those two, traced again by whole waves."""
+    return ["primary"] + (["bounce"] if levels > 1 else []) + ["redo"]
 
 
 def render_batch_bgra8(scenes, views: np.ndarray, persistent: bool = True, outs=None):

@@ -26,6 +26,10 @@
  *                   per queued pixel, in one lane, its bounce ray, then the
  *                   new surface's level as k_primary runs the first, and so
  *                   on while the pixel bounces.
+ *   k_redo          the pixels whose traversal gave up in the two stages
+ *                   above (a mesh traversal past ST_ABANDON_STEPS steps),
+ *                   one per wave, all lanes on one pixel: its whole path
+ *                   again with large mesh traversals split across the lanes.
  *   k_counts        per-request shadow/bounce ray totals.
  */
 #include <hip/hip_runtime.h>
@@ -58,7 +62,8 @@
 #endif
 /* Launch attributes of the bounce stage (occupancy experiments). */
 #ifndef K_BOUNCE_ATTR
-#define K_BOUNCE_ATTR
+/* 2 waves per SIMD: the register budget of 256 (a few values spill) */
+#define K_BOUNCE_ATTR __attribute__((amdgpu_waves_per_eu(2)))
 #endif
 
 /* Every use of the shared stream, arena and staging buffer holds this lock
@@ -168,13 +173,17 @@ __device__ static inline bool tile_pixel(const StDView &v, uint32_t tile, uint32
 /* One depth-loop level of a pixel (surface.h:190-243) from its surface hit
    `hit`: shading entry, the light loop's shadow rays, then the shading from
    their results (direct light, emissive quads, glow). Returns true when the
-   pixel continues with a bounce ray (direction in p.bdir). */
+   pixel continues with a bounce ray (direction in p.bdir). In the main
+   stages a pixel whose shadow ray gave up is left for the redo stage. */
+template <bool COOP = false>
 __device__ static inline __attribute__((always_inline)) bool run_level(const KWave &w, const StDScene &sc,
                                                                       uint32_t slot, const DMat &M, const DHit &hit,
-                                                                      DPix &p, uint32_t *scene_stack,
-                                                                      uint32_t *mesh_stack) {
+                                                                      DPix &p, const DStacks &stk) {
     begin_level(sc, M, hit, p);
-    trace_level_shadows(w, sc, scene_stack, mesh_stack, slot, p.pos, p.nrm);
+    if (trace_level_shadows<COOP>(w, sc, stk, slot, p.pos, p.nrm)) {
+        p.flags |= PF_REDO;
+        return false;
+    }
     DTrace tr;
     trace_init(tr, sc, nullptr, nullptr, p.cur_dist);
     const bool bounce = shade_level(w, sc, tr, slot, p);
@@ -192,28 +201,76 @@ __device__ static inline void end_pixel(const KWave &w, const StDScene &sc, cons
     record_counts(w, slot, p.n_shadow, p.n_bounce);
 }
 
-/* Primary stage of one tile (raytracer.h:81-134 up to the Beauty depth
+/* The depth loop after a level that ends in a bounce ray (surface.h:221-240):
+   closest-hit traversal of the bounce ray; a hit that is not an emissive
+   quad starts the pixel's next level (shading entry, shadow rays, shading)
+   with the state in registers, and so on while the pixel bounces. In the
+   main stages a traversal that gave up ends the loop with PF_REDO set. */
+template <bool COOP = false>
+__device__ static inline __attribute__((always_inline)) void bounce_chain(const KWave &w, const StDScene &sc,
+                                                                         uint32_t slot, DPix &p,
+                                                                         const DStacks &stk) {
+    bool again;
+    do {
+        DTrace tr;
+        trace_init(tr, sc, stk.scene, stk.mesh, p.cur_dist, stk.aux);
+        DRay ray;
+        ray.origin = p.pos;
+        ray.direction = p.bdir;
+        DHit &hit = tr.closest_hit;
+        hit.distance = hit.distance_squared = INFINITY;
+        const bool found = trace_scene<COOP>(ray, tr, sc, false);
+        if (tr.abandoned) {
+            p.flags |= PF_REDO;
+            break;
+        }
+        p.cur_dist = tr.current_hit.distance;
+        again = false;
+        if (found) {
+            p.prim_index = hit.object_id;
+            const DMat M = load_mat(sc, hit.material_id);
+            p.alb = M.albedo;
+            if (M.use && M.texture_count) {
+                DShaded s = shaded_of(p);
+                sv2 dUV = V2(0, p.duv_y);
+                apply_textures(tr, sc, s, M, dUV, false);
+                p.alb = s.albedo;
+            }
+            if (hit.object_type == ST_PRIM_QUAD && (M.is & ST_EMISSIVE)) {
+                p.color = hit.from_behind ? V3s(0) : M.emission;
+            } else {
+                if (M.brdf != ST_BRDF_COOK_TORRANCE) p.throughput = v_mul(p.throughput, M.reflectivity);
+                p.vo = ray.origin;
+                p.vd = ray.direction;
+                p.material_id = hit.material_id;
+                p.hit_d2 = hit.distance_squared;
+                again = run_level<COOP>(w, sc, slot, M, hit, p, stk);
+            }
+        }
+    } while (again);
+}
+
+/* Primary stage of one pixel (raytracer.h:81-134 up to the Beauty depth
    loop): the primary ray over all primitives with the screen-space gate, the
    hit record, and then either the finished pixel (Depth/Normals/UVs,
    misses, emissive hits, depth 0) or the Beauty shading entry
-   (surface.h:146-170) and the first level of its depth loop. */
-__device__ static inline __attribute__((always_inline)) void primary_tile(const StDScene *scenes, const KReq *reqs,
-                                                                         uint32_t nreq, uint32_t tile,
-                                                                         uint32_t *scene_stack, uint32_t *mesh_stack,
-                                                                         const KWave &w) {
-    const uint32_t lane = threadIdx.x & 63u;
-    const uint32_t r = find_req(reqs, nreq, tile, false);
-    const StDView v = reqs[r].v;
-    const StDScene sc = scenes[v.scene];
-    uint32_t x, y;
-    const bool valid = tile_pixel(v, tile, lane, x, y);
+   (surface.h:146-170) and the first level of its depth loop. The main stage
+   queues a pixel that bounces for the bounce stage and one whose traversal
+   gave up for the redo stage; the redo stage (COOP, the whole wave on one
+   pixel) runs the pixel's whole depth loop here. `slot` = tile * 64 + lane
+   indexes the pixel's depth-loop state. */
+template <bool COOP = false>
+__device__ static inline __attribute__((always_inline)) void primary_pixel(const StDScene &sc, const StDView &v,
+                                                                          uint32_t slot, uint32_t x, uint32_t y,
+                                                                          bool valid, const DTileBox &box,
+                                                                          const DStacks &stk, const KWave &w) {
     /* Every lane traces, pixels outside the traced grid included (they read
        pixel 0's direction and no rect admits them), so that the whole wave
        takes part in culling the primitives against the tile. */
     const uint32_t pix = valid ? y * v.width1 + x : 0u;
     const sv3 cur = primary_dir(v, valid ? x : 0u, valid ? y : 0u);
     DTrace tr;
-    trace_init(tr, sc, scene_stack, mesh_stack, 0.0f);
+    trace_init(tr, sc, stk.scene, stk.mesh, 0.0f, stk.aux);
     DRay ray;
     ray.origin = V3(v.cam_pos[0], v.cam_pos[1], v.cam_pos[2]);
     /* dir = normVec3(current) (render/raytracer.h:159) */
@@ -224,18 +281,9 @@ __device__ static inline __attribute__((always_inline)) void primary_tile(const 
     hit.cone_angle = v.cone_angle;
     hit.cone_width = 0;
     tr.primary_triangle = -1;
-    DTileBox box;
-    {
-        uint32_t tx, ty;
-        tile_coords(v, tile, tx, ty);
-        const uint32_t tw = 1u << v.tile_shift, th = 64u >> v.tile_shift;
-        box.x0 = (int32_t)(tx * tw);
-        box.y0 = (int32_t)(ty * th);
-        box.x1 = box.x0 + (int32_t)tw;
-        box.y1 = box.y0 + (int32_t)th;
-    }
-    const bool found = hit_primitives(ray, tr, sc, sc.scene_leaf_ids, sc.prim_count, false, v.rects,
-                                      valid ? x : ~0u, valid ? y : ~0u, &box);
+    const bool found = hit_primitives<COOP>(ray, tr, sc, sc.scene_leaf_ids, sc.prim_count, false, v.rects,
+                                            valid ? x : ~0u, valid ? y : ~0u, &box);
+    bool redo = tr.abandoned != 0;
     bool entered = false;   /* entered the Beauty depth loop */
     if (valid) {
         if (v.hits)
@@ -281,19 +329,28 @@ __device__ static inline __attribute__((always_inline)) void primary_tile(const 
                     p.depth_left = v.depth;
                     p.flags = 0;
                     p.n_shadow = p.n_bounce = 0;
-                    const uint32_t slot = tile * 64u + lane;
-                    const bool bounce = run_level(w, sc, slot, M, hit, p, scene_stack, mesh_stack);
-                    /* refracting pixels, which go on for the most levels,
-                       fill the queue from the front, the others from the
-                       back; the bounce stage starts with the front */
-                    const bool deep = (M.is & ST_REFRACTIVE) != 0;
-                    const uint32_t qf = wave_append(w.ctr, bounce && deep);
-                    const uint32_t qb = wave_append(w.ctr + 1, bounce && !deep);
-                    if (bounce) {
-                        ((ST_GLOBAL uint32_t *)w.bounce_q)[deep ? qf : w.n - 1u - qb] = slot;
-                        pix_store_all(w, slot, p);
-                    } else {
+                    bool bounce = false;
+                    if (!redo) {
+                        bounce = run_level<COOP>(w, sc, slot, M, hit, p, stk);
+                        redo = (p.flags & PF_REDO) != 0;
+                    }
+                    if (COOP) {
+                        if (bounce) bounce_chain<true>(w, sc, slot, p, stk);
                         end_pixel(w, sc, v, slot, p);
+                    } else {
+                        /* refracting pixels, which go on for the most levels,
+                           fill the queue from the front, the others from the
+                           back; the bounce stage starts with the front */
+                        const bool go = bounce && !redo;
+                        const bool deep = (M.is & ST_REFRACTIVE) != 0;
+                        const uint32_t qf = wave_append(w.ctr, go && deep);
+                        const uint32_t qb = wave_append(w.ctr + 1, go && !deep);
+                        if (go) {
+                            ((ST_GLOBAL uint32_t *)w.bounce_q)[deep ? qf : w.n - 1u - qb] = slot;
+                            pix_store_all(w, slot, p);
+                        } else {
+                            end_pixel(w, sc, v, slot, p);
+                        }
                     }
                     entered = true;
                 }
@@ -316,7 +373,35 @@ __device__ static inline __attribute__((always_inline)) void primary_tile(const 
         }
         if (!entered) finish_pixel(sc, v, pix, color, found, false, z, closest_distance);
     }
-    if (w.n && !entered) record_counts(w, tile * 64u + lane, 0, 0);
+    if (w.n && !entered) record_counts(w, slot, 0, 0);
+    if (!COOP) {
+        const bool r = valid && redo;
+        const uint32_t q = wave_append(w.redo_ctr, r);
+        if (r) ((ST_GLOBAL uint32_t *)w.redo_q)[q] = slot;
+    }
+}
+
+/* Primary stage of one tile: each lane its pixel (primary_pixel). */
+__device__ static inline __attribute__((always_inline)) void primary_tile(const StDScene *scenes, const KReq *reqs,
+                                                                         uint32_t nreq, uint32_t tile,
+                                                                         const DStacks &stk, const KWave &w) {
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t r = find_req(reqs, nreq, tile, false);
+    const StDView v = reqs[r].v;
+    const StDScene sc = scenes[v.scene];
+    uint32_t x, y;
+    const bool valid = tile_pixel(v, tile, lane, x, y);
+    DTileBox box;
+    {
+        uint32_t tx, ty;
+        tile_coords(v, tile, tx, ty);
+        const uint32_t tw = 1u << v.tile_shift, th = 64u >> v.tile_shift;
+        box.x0 = (int32_t)(tx * tw);
+        box.y0 = (int32_t)(ty * th);
+        box.x1 = box.x0 + (int32_t)tw;
+        box.y1 = box.y0 + (int32_t)th;
+    }
+    primary_pixel<false>(sc, v, tile * 64u + lane, x, y, valid, box, stk, w);
 }
 
 /* Traversal stacks live in LDS, one column per lane (no bank conflicts):
@@ -333,6 +418,7 @@ k_primary(const StDScene *__restrict__ scenes, const KReq *__restrict__ reqs, ui
           uint32_t *tile_counter, uint32_t persistent, uint32_t xcd_blocks, uint32_t scene_entries,
           uint32_t mesh_entries, KWave w, const uint32_t *__restrict__ tile_order, uint32_t *__restrict__ tile_cost) {
     ST_LDS_STACKS(scene_entries, mesh_entries);
+    const DStacks stk = {scene_stack, mesh_stack, nullptr};
     /* One call site, so primary_tile is inlined once. Persistent: waves
        claim tiles from the global counter until it runs past the end.
        Otherwise each wave renders one tile: blocks are dealt to the 8 XCDs
@@ -355,10 +441,34 @@ k_primary(const StDScene *__restrict__ scenes, const KReq *__restrict__ reqs, ui
         /* claim order -> tile (k_tile_order) */
         t = __builtin_amdgcn_readfirstlane(ldgu(tile_order, t));
         const uint64_t t0 = __builtin_amdgcn_s_memrealtime();
-        primary_tile(scenes, reqs, nreq, t, scene_stack, mesh_stack, w);
+        primary_tile(scenes, reqs, nreq, t, stk, w);
         /* the tile's time (100 MHz ticks), which orders the next launch */
         if ((threadIdx.x & 63u) == 0)
             ((ST_GLOBAL uint32_t *)tile_cost)[t] = (uint32_t)(__builtin_amdgcn_s_memrealtime() - t0);
+    }
+}
+
+/* Redo stage: the pixels whose traversal gave up in the main stages, one
+   per wave, all 64 lanes on the same pixel: its primary ray and whole depth
+   loop again, with every traversal of a large mesh split across the lanes
+   (coop_trace_mesh). The pixel's frame value, hit record and ray counts
+   replace what the main stages wrote for it. Blocks of one wave: each lane
+   has a scene, a mesh and a path-maximum stack column. */
+__global__ void __launch_bounds__(64) k_redo(const StDScene *__restrict__ scenes, const KReq *__restrict__ reqs,
+                                             uint32_t nreq, KWave w, uint32_t scene_entries, uint32_t mesh_entries) {
+    ST_LDS_STACKS(scene_entries, 2u * mesh_entries);
+    const DStacks stk = {scene_stack, mesh_stack, mesh_stack + 64u * mesh_entries};
+    const uint32_t count = __builtin_amdgcn_readfirstlane(ldgu(w.redo_ctr, 0));
+    for (uint32_t i = blockIdx.x; i < count; i += gridDim.x) {
+        const uint32_t slot = __builtin_amdgcn_readfirstlane(ldgu(w.redo_q, i));
+        const uint32_t tile = slot >> 6;
+        const uint32_t r = find_req(reqs, nreq, tile, false);
+        const StDView v = reqs[r].v;
+        const StDScene sc = scenes[v.scene];
+        uint32_t x, y;
+        if (!tile_pixel(v, tile, slot & 63u, x, y)) continue;
+        const DTileBox box = {(int32_t)x, (int32_t)y, (int32_t)x + 1, (int32_t)y + 1};
+        primary_pixel<true>(sc, v, slot, x, y, true, box, stk, w);
     }
 }
 
@@ -483,6 +593,7 @@ __global__ void __launch_bounds__(BLOCK_THREADS) K_BOUNCE_ATTR k_bounce(const St
                                                           uint32_t *claim, uint32_t scene_entries,
                                                           uint32_t mesh_entries) {
     ST_LDS_STACKS(scene_entries, mesh_entries);
+    const DStacks stk = {scene_stack, mesh_stack, nullptr};
     /* the queue's front part (ctr[0] entries) first, then its back part
        (ctr[1] entries, stored from the end of the buffer down) */
     const uint32_t n_front = __builtin_amdgcn_readfirstlane(ldgu(w.ctr, 0));
@@ -505,41 +616,11 @@ __global__ void __launch_bounds__(BLOCK_THREADS) K_BOUNCE_ATTR k_bounce(const St
                     const StDView v = reqs[r].v;
                     const StDScene sc = scenes[v.scene];
                     DPix p = pix_load(w, slot);
-                    bool again;
-                    do {
-                        DTrace tr;
-                        trace_init(tr, sc, scene_stack, mesh_stack, p.cur_dist);
-                        DRay ray;
-                        ray.origin = p.pos;
-                        ray.direction = p.bdir;
-                        DHit &hit = tr.closest_hit;
-                        hit.distance = hit.distance_squared = INFINITY;
-                        const bool found = trace_scene(ray, tr, sc, false);
-                        p.cur_dist = tr.current_hit.distance;
-                        again = false;
-                        if (found) {
-                            p.prim_index = hit.object_id;
-                            const DMat M = load_mat(sc, hit.material_id);
-                            p.alb = M.albedo;
-                            if (M.use && M.texture_count) {
-                                DShaded s = shaded_of(p);
-                                sv2 dUV = V2(0, p.duv_y);
-                                apply_textures(tr, sc, s, M, dUV, false);
-                                p.alb = s.albedo;
-                            }
-                            if (hit.object_type == ST_PRIM_QUAD && (M.is & ST_EMISSIVE)) {
-                                p.color = hit.from_behind ? V3s(0) : M.emission;
-                            } else {
-                                if (M.brdf != ST_BRDF_COOK_TORRANCE) p.throughput = v_mul(p.throughput, M.reflectivity);
-                                p.vo = ray.origin;
-                                p.vd = ray.direction;
-                                p.material_id = hit.material_id;
-                                p.hit_d2 = hit.distance_squared;
-                                again = run_level(w, sc, slot, M, hit, p, scene_stack, mesh_stack);
-                            }
-                        }
-                    } while (again);
+                    bounce_chain<false>(w, sc, slot, p, stk);
                     end_pixel(w, sc, v, slot, p);
+                    /* a traversal gave up: the redo stage traces the pixel again */
+                    if (p.flags & PF_REDO)
+                        ((ST_GLOBAL uint32_t *)w.redo_q)[atomicAdd(w.redo_ctr, 1u)] = slot;
                 });
 }
 
@@ -651,6 +732,7 @@ __global__ void __launch_bounds__(64) k_pick(const StDScene *__restrict__ scenes
     uint32_t type = 0, id = 0;
     bool found = hit_primitives(ray, tr, sc, sc.scene_leaf_ids, sc.prim_count, false, v.rects, (uint16_t)x,
                                 (uint16_t)y);
+    if (tr.abandoned) trav_fault(sc, ST_FAULT_MESH_STEPS);
     if (found) {
         type = hit.object_type;
         id = hit.object_id;
@@ -788,6 +870,7 @@ struct LaunchCtx {
     hipEvent_t stage_ev[ST_STAGES_MAX + 1];
     uint32_t stage_n;
     float stage_ms[ST_STAGES_MAX];
+    uint32_t last_redo;    /* pixels the redo stage traced again in the last launch */
 };
 
 static struct {
@@ -797,8 +880,18 @@ static struct {
     int cus;               /* compute units of the device */
     StTuning tune;         /* launch shape chosen by st_set_tuning / st_autotune */
     uint32_t step_cap;     /* st_debug_step_cap: lower traversal step bound, 0 = none */
+    uint32_t abandon_steps, coop_min_tris;   /* st_debug_redo (0: ST_ABANDON_STEPS, ST_COOP_MIN_TRIS) */
     std::atomic<uint64_t> graph_captures, graph_launches;
 } g;
+
+/* The per-launch traversal limits of a scene table entry. */
+static void launch_limits(StDScene &d, uint32_t step_cap, uint32_t abandon_steps, uint32_t coop_min_tris) {
+    d.step_cap = step_cap;
+    uint32_t lim = abandon_steps ? abandon_steps : ST_ABANDON_STEPS;
+    if (step_cap && step_cap < lim) lim = step_cap;
+    d.main_step_limit = lim;
+    d.coop_min_tris = coop_min_tris ? coop_min_tris : ST_COOP_MIN_TRIS;
+}
 
 static int ctx_init(LaunchCtx *c) {
     HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
@@ -1327,11 +1420,13 @@ int st_rt_launch_ctx(LaunchCtx *c, StSceneHandle *const *scenes, StHostView *hv,
     if (rc) return rc;
     if (n == 0) return set_err(ST_ERR_ARG, "no requests");
     StTuning tune;
-    uint32_t step_cap;
+    uint32_t step_cap, abandon_steps, coop_min_tris;
     {
         RT_LOCK;
         tune = g.tune;
         step_cap = g.step_cap;
+        abandon_steps = g.abandon_steps;
+        coop_min_tris = g.coop_min_tris;
     }
     std::vector<uint32_t> scene_idx(n);
     std::vector<StSceneHandle *> uniq(n);
@@ -1395,8 +1490,12 @@ int st_rt_launch_ctx(LaunchCtx *c, StSceneHandle *const *scenes, StHostView *hv,
         wave_bytes = round256(slots * 16 * ST_PLANES) + round256(slots * lmax * 8) + round256(slots * 4) +
                      round256(slots * 4) + round256(8) + round256(slots * 8) + 6 * 256;
     }
-    const size_t cnt_bytes = n * 24ull + 8;
-    size_t work = up + round256(cnt_bytes) + per_req_out + wave_bytes + 256;
+    /* per request: primary, shadow, bounce totals; then the fault word and
+       the redo stage's queue length */
+    const size_t cnt_bytes = n * 24ull + 16;
+    /* the redo stage's queue: every launch (a primary ray can give up too) */
+    const size_t redo_bytes = round256(slots * 4) + 256;
+    size_t work = up + round256(cnt_bytes) + per_req_out + wave_bytes + redo_bytes + 256;
     if ((rc = arena_reserve(&c->work, work))) return rc;
     /* staging: the upload block, then the ray counts and fault word read back */
     const size_t o_readback = round256(up);
@@ -1427,7 +1526,7 @@ int st_rt_launch_ctx(LaunchCtx *c, StSceneHandle *const *scenes, StHostView *hv,
     for (uint32_t k = 0; k < nuniq; k++) {
         sc_h[k] = uniq[k]->dev;
         sc_h[k].fault = (uint32_t *)(cnt_d + 3 * (size_t)n);
-        sc_h[k].step_cap = step_cap;
+        launch_limits(sc_h[k], step_cap, abandon_steps, coop_min_tris);
     }
     uint32_t row_base = 0, out_base = 0, count_blocks = 0, any_device_ssb = 0;
     for (uint32_t i = 0; i < n; i++) {
@@ -1475,6 +1574,8 @@ int st_rt_launch_ctx(LaunchCtx *c, StSceneHandle *const *scenes, StHostView *hv,
         w.ctr = (uint32_t *)arena_take(&c->work, 8);
         w.counts = (uint32_t *)arena_take(&c->work, slots * 8);
     }
+    w.redo_q = (uint32_t *)arena_take(&c->work, slots * 4);
+    w.redo_ctr = (uint32_t *)(cnt_d + 3 * (size_t)n + 1);
     /* launch shape, fixed before anything is enqueued so that a captured
        graph of this launch can be matched against the next one */
     uint32_t blocks;
@@ -1509,6 +1610,9 @@ int st_rt_launch_ctx(LaunchCtx *c, StSceneHandle *const *scenes, StHostView *hv,
         b_bounce = resident_blocks(reinterpret_cast<const void *>(k_bounce), lds);
     }
     const uint32_t cost_blocks = std::min<uint32_t>((tiles + COST_THREADS - 1) / COST_THREADS, 1024u);
+    /* redo stage: one-wave blocks, each with its scene, mesh and path-maximum stack columns */
+    const size_t lds_redo = (size_t)64 * (s_entries + 2 * m_entries) * sizeof(uint32_t);
+    const uint32_t redo_blocks = (uint32_t)g.cus * 8u;
 
     /* per-stage timing (bench): an event before the first stage and after each */
     uint32_t marks = 0;
@@ -1527,7 +1631,7 @@ int st_rt_launch_ctx(LaunchCtx *c, StSceneHandle *const *scenes, StHostView *hv,
     auto enqueue = [&]() -> int {
         HIP_TRY(hipMemcpyAsync(blk, st, up, hipMemcpyHostToDevice, c->stream));
         if (!graph) HIP_TRY(hipEventRecord(c->ev[0], c->stream));
-        hipLaunchKernelGGL(k_zero_u64, dim3((3 * n + 1 + 255) / 256), dim3(256), 0, c->stream, cnt_d, 3 * n + 1);
+        hipLaunchKernelGGL(k_zero_u64, dim3((3 * n + 2 + 255) / 256), dim3(256), 0, c->stream, cnt_d, 3 * n + 2);
         hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, c->stream, c->tile_counter, 64u);
         static_assert(BOUNCE_CLAIM < 64 && ORDER_CLAIM + 1 < 64, "claim counters share the zeroed 64 words");
         if (any_device_ssb)
@@ -1560,6 +1664,11 @@ int st_rt_launch_ctx(LaunchCtx *c, StSceneHandle *const *scenes, StHostView *hv,
                                s_entries, m_entries);
             HIP_TRY(mark());
         }
+        /* the pixels that gave up, traced again (it reads the queue's length
+           on the device: no round trip; nothing to do is one quick launch) */
+        hipLaunchKernelGGL(k_redo, dim3(redo_blocks), dim3(64), lds_redo, c->stream, (const StDScene *)sc_d,
+                           (const KReq *)reqs_d, n, w, s_entries, m_entries);
+        HIP_TRY(mark());
         if (levels && count_blocks)
             hipLaunchKernelGGL(k_counts, dim3(count_blocks), dim3(256), 0, c->stream, (const KReq *)reqs_d, n,
                                (const uint32_t *)w.counts);
@@ -1580,7 +1689,8 @@ int st_rt_launch_ctx(LaunchCtx *c, StSceneHandle *const *scenes, StHostView *hv,
                                      lds_primary, (uint64_t)(uintptr_t)blk,
                                      (uint64_t)(uintptr_t)st,
                                      (uint64_t)(uintptr_t)cnt_d, (uint64_t)(uintptr_t)w.st, w.n,
-                                     (uint64_t)(uintptr_t)c->tile_cost, use_history, cost_blocks};
+                                     (uint64_t)(uintptr_t)c->tile_cost, use_history, cost_blocks,
+                                     (uint64_t)(uintptr_t)w.redo_q, redo_blocks, lds_redo};
         for (uint32_t i = 0; i < n; i++) {
             const StDView &dv = reqs_h[i].v;
             sig.insert(sig.end(), {dv.width1, dv.height1, dv.mode, dv.depth, dv.tile_shift,
@@ -1634,6 +1744,7 @@ int st_rt_launch_ctx(LaunchCtx *c, StSceneHandle *const *scenes, StHostView *hv,
     }
     HIP_TRY(hipMemcpyAsync(cnt_h, cnt_d, cnt_bytes, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
+    c->last_redo = (uint32_t)cnt_h[3 * (size_t)n + 1];
     if (cnt_h[3 * (size_t)n])
         return set_err(ST_ERR_SCENE, "a traversal ran past its bound (fault bits 0x%llx): malformed BVH",
                        cnt_h[3 * (size_t)n]);
@@ -1697,6 +1808,20 @@ struct GraphScope {
 extern "C" int st_debug_step_cap(uint32_t cap) {
     RT_LOCK;
     g.step_cap = cap;
+    return ST_OK;
+}
+
+extern "C" int st_debug_redo(uint32_t abandon_steps, uint32_t coop_min_tris) {
+    RT_LOCK;
+    g.abandon_steps = abandon_steps;
+    g.coop_min_tris = coop_min_tris;
+    return ST_OK;
+}
+
+extern "C" int st_last_redo_count(uint32_t *count) {
+    RT_LOCK;
+    if (!count) return set_err(ST_ERR_ARG, "null argument");
+    *count = g.main.last_redo;
     return ST_OK;
 }
 
@@ -1815,6 +1940,13 @@ extern "C" int st_pick(StSceneHandle *scene, const StView *view, uint32_t n, con
     StDScene dsc = scene->dev;
     dsc.fault = (uint32_t *)(blk + o_fault);
     dsc.step_cap = g.step_cap;
+    /* picking traces sequentially: its mesh traversals stop (a fault) at the
+       bound no valid tree reaches, the largest mesh's node count */
+    uint32_t max_tris = 0;
+    for (uint32_t m = 0; m < scene->host->mesh_count; m++)
+        if (scene->host->meshes[m].triangle_count > max_tris) max_tris = scene->host->meshes[m].triangle_count;
+    dsc.main_step_limit = 2u * max_tris + 2u;
+    if (g.step_cap && g.step_cap < dsc.main_step_limit) dsc.main_step_limit = g.step_cap;
     memcpy(g.main.staging + o_sc, &dsc, sizeof(StDScene));
     memset(g.main.staging + o_fault, 0, 4);
     memcpy(g.main.staging + o_rect, hv.rects, P * 16ull);

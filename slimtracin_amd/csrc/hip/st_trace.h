@@ -74,6 +74,9 @@ struct DTrace {
     uint32_t mesh_stack_size;
     int32_t primary_triangle;
     uint32_t *scene_stack, *mesh_stack;   /* lane columns of LDS: entry i at [i * ST_STACK_STRIDE] */
+    uint32_t *aux_stack;                  /* wave-cooperative traversal: the path maxima beside mesh_stack */
+    uint32_t abandoned;                   /* a mesh traversal gave up (StDScene.main_step_limit): this ray's
+                                             result is not the reference's */
 };
 
 #define ST_STACK_STRIDE 64u
@@ -581,11 +584,241 @@ DFN void trav_fault(const StDScene &sc, uint32_t code) {
     if (sc.fault) atomicOr(sc.fault, code);
 }
 
+/* Levels of the top of a mesh tree that the wave splits between its lanes:
+   lane L takes the subtree at depth ST_COOP_LEVELS reached by the bits of L
+   (left = 0, from the root down). */
+#define ST_COOP_LEVELS 6u
+
+/* "not written" marker for a current-hit distance: a signalling-NaN
+   pattern, which no arithmetic result can have */
+#define ST_DIST_UNSET 0x7FA5A5A5u
+
+DFN float wave_min_f(float v) {
+    for (int o = 32; o > 0; o >>= 1) {
+        const float u = __shfl_xor(v, o, 64);
+        v = u < v ? u : v;
+    }
+    return v;
+}
+DFN uint32_t wave_min_u(uint32_t v) {
+    for (int o = 32; o > 0; o >>= 1) {
+        const uint32_t u = __shfl_xor(v, o, 64);
+        v = u < v ? u : v;
+    }
+    return v;
+}
+DFN uint32_t wave_max_u(uint32_t v) {
+    for (int o = 32; o > 0; o >>= 1) {
+        const uint32_t u = __shfl_xor(v, o, 64);
+        v = u > v ? u : v;
+    }
+    return v;
+}
+DFN sv3 bcast3(sv3 v, int src) { return V3(__shfl(v.x, src, 64), __shfl(v.y, src, 64), __shfl(v.z, src, 64)); }
+DFN DHit bcast_hit(const DHit &h, int src) {
+    DHit o;
+    o.position = bcast3(h.position, src);
+    o.normal = bcast3(h.normal, src);
+    o.uv = V2(__shfl(h.uv.x, src, 64), __shfl(h.uv.y, src, 64));
+    o.dUV = V2(__shfl(h.dUV.x, src, 64), __shfl(h.dUV.y, src, 64));
+    o.distance = __shfl(h.distance, src, 64);
+    o.distance_squared = __shfl(h.distance_squared, src, 64);
+    o.cone_angle = __shfl(h.cone_angle, src, 64);
+    o.cone_width = __shfl(h.cone_width, src, 64);
+    o.NdotV = __shfl(h.NdotV, src, 64);
+    o.area = __shfl(h.area, src, 64);
+    o.uv_area = __shfl(h.uv_area, src, 64);
+    o.material_id = (uint32_t)__shfl((int)h.material_id, src, 64);
+    o.object_id = (uint32_t)__shfl((int)h.object_id, src, 64);
+    o.object_type = (uint32_t)__shfl((int)h.object_type, src, 64);
+    o.from_behind = (uint32_t)__shfl((int)h.from_behind, src, 64);
+    return o;
+}
+
+/* One lane's share of a cooperative mesh traversal: the reference's
+   traversal loop (mesh.h:38-111, same step structure and child order as
+   trace_mesh) below node pair `left`, with the box bound fixed at `bound`
+   (the traversal's starting bound: it visits every box the reference could
+   admit, as boxes are only pruned when the bound shrinks) and the triangle
+   test as the reference makes it. `pm` is the largest entry distance of the
+   boxes on the path down to `left`; *best_pm receives that of the leaf of
+   the last triangle taken. Returns 0, 1 when the stack would reach the
+   reference's limit (never for a valid tree), 2 past the step bound. */
+DFN int coop_dfs(DTrace &tr, const uint32_t *nodes, const StDTriHot *tris, uint32_t left, float bound, float pm,
+                 bool any_hit, uint32_t max_steps, bool &found, float &best_pm) {
+    DRay &ray = tr.local;
+    uint32_t *stack = tr.mesh_stack, *pstack = tr.aux_stack;
+    uint32_t sp = 0, steps = 0;
+    for (;;) {
+        if (++steps > max_steps) return 2;
+        const DNodePair np = load_pair(nodes, left);
+        float ld, rd;
+        const bool hl = hit_aabb(np.lb, ray, bound, ld);
+        const bool hr = hit_aabb(np.rb, ray, bound, rd);
+        const float pml = ld > pm ? ld : pm, pmr = rd > pm ? rd : pm;
+        if (hl && np.lcount && hit_triangles(tr, tris, np.lfirst, np.lcount, 0, 0, any_hit)) {
+            found = true;
+            best_pm = pml;
+            if (any_hit) return 0;
+        }
+        if (hr && np.rcount && hit_triangles(tr, tris, np.rfirst, np.rcount, 0, 0, any_hit)) {
+            found = true;
+            best_pm = pmr;
+            if (any_hit) return 0;
+        }
+        const bool il = hl && !np.lcount, ir = hr && !np.rcount;
+        if (il && ir) {
+            uint32_t a = np.lfirst, b = np.rfirst;
+            float pa = pml, pb = pmr;
+            if (!any_hit && ld > rd) {
+                a = np.rfirst; b = np.lfirst;
+                pa = pmr; pb = pml;
+            }
+            if (sp + 1u >= tr.mesh_stack_size) return 1;
+            stack[ST_STACK_STRIDE * sp] = b;
+            pstack[ST_STACK_STRIDE * sp] = __float_as_uint(pb);
+            sp++;
+            left = a;
+            pm = pa;
+        } else if (il) {
+            left = np.lfirst;
+            pm = pml;
+        } else if (ir) {
+            left = np.rfirst;
+            pm = pmr;
+        } else {
+            if (sp == 0) break;
+            sp--;
+            left = stack[ST_STACK_STRIDE * sp];
+            pm = __uint_as_float(pstack[ST_STACK_STRIDE * sp]);
+        }
+    }
+    return 0;
+}
+
+/* The reference's mesh traversal of one ray (mesh.h:38-111), run by all 64
+   lanes of a wave that hold the same ray (the redo stage). The top
+   ST_COOP_LEVELS levels of the tree split it into units in the reference's
+   visiting order: lane L descends along the bits of L and takes the subtree
+   it reaches, or a leaf its path ends at (the lane whose lower bits are zero
+   owns it). Each lane runs the reference's loop over its unit with the box
+   bound fixed at the traversal's start (coop_dfs), a superset of the boxes
+   the sequential traversal admits. The units' order key is, per pair on the
+   path, where the reference takes that child: leaf children in that step
+   (left 0, right 1), then the inner child it enters first (2, the nearer one
+   for a closest hit) and last (3).
+     any hit: the sequential traversal's bound stays fixed until its first
+       hit, so it visits exactly these units in key order: its hit is the
+       first hit of the unit with the smallest key;
+     closest hit: t* = the least hit distance; among the hits at t*, the one
+       of the smallest key is first in the reference's order. The reference
+       reaches it if every box on its path admits it at bound t* (its bound
+       is still above t* there), i.e. every entry distance on the path is at
+       most t* (the boxes were admitted at the larger starting bound), and it
+       then keeps it: nothing is nearer, and ties lose to the first.
+   Nothing hit: the bound never moved, so the units are exactly what the
+   reference visits, and its current-hit distance is the last plane hit of
+   the unit latest in key order that has one.
+   Returns 1 (hit in tr.closest_mesh_hit), 0 (none), or -1 when the check
+   fails and the caller traces sequentially (tr restored). */
+DFN int coop_trace_mesh(DTrace &tr, const StDScene &sc, const StDMesh &mesh, const uint32_t *nodes,
+                        const StDTriHot *tris, uint32_t root_first, bool any_hit) {
+    DRay &ray = tr.local;
+    DHit &closest = tr.closest_mesh_hit;
+    const float c0 = closest.distance;
+    const DHit saved_closest = closest, saved_current = tr.current_hit;
+    const uint32_t lane = __lane_id();
+    const uint32_t D = ST_COOP_LEVELS;
+    uint32_t max_steps = 2u * mesh.triangle_count + 2u;
+    if (sc.step_cap && sc.step_cap < max_steps) max_steps = sc.step_cap;
+    uint32_t left = root_first, key = 0, unit = 2u, lfirst = 0, lcount = 0;
+    float pm = 0.0f;
+    for (uint32_t d = 0; d < D; d++) {
+        const DNodePair np = load_pair(nodes, left);
+        float ld, rd;
+        const bool hl = hit_aabb(np.lb, ray, c0, ld);
+        const bool hr = hit_aabb(np.rb, ray, c0, rd);
+        const uint32_t bit = (lane >> (D - 1u - d)) & 1u;
+        const uint32_t cnt = bit ? np.rcount : np.lcount, first = bit ? np.rfirst : np.lfirst;
+        if (!(bit ? hr : hl)) {
+            unit = 0;
+            break;
+        }
+        const uint32_t first_inner = (!any_hit && ld > rd) ? 1u : 0u;
+        key = key * 4u + (cnt ? bit : (bit == first_inner ? 2u : 3u));
+        const float l = bit ? rd : ld;
+        pm = l > pm ? l : pm;
+        if (cnt) {
+            const uint32_t below = D - 1u - d;
+            unit = (lane & ((1u << below) - 1u)) == 0 ? 1u : 0u;
+            key <<= 2u * below;
+            lfirst = first;
+            lcount = cnt;
+            break;
+        }
+        left = first;
+    }
+    tr.current_hit.distance = __uint_as_float(ST_DIST_UNSET);
+    bool found = false;
+    float best_pm = 0.0f;
+    int status = 0;
+    if (unit == 1u) {
+        if (hit_triangles(tr, tris, lfirst, lcount, 0, 0, any_hit)) {
+            found = true;
+            best_pm = pm;
+        }
+    } else if (unit == 2u) {
+        status = coop_dfs(tr, nodes, tris, left, c0, pm, any_hit, max_steps, found, best_pm);
+    }
+    if (__ballot(status != 0)) {
+        closest = saved_closest;
+        tr.current_hit = saved_current;
+        if (__ballot(status == 2)) {
+            trav_fault(sc, ST_FAULT_MESH_STEPS);
+            return 0;
+        }
+        return -1;
+    }
+    if (__ballot(found)) {
+        int win;
+        if (any_hit) {
+            const uint32_t k = wave_min_u(found ? key : 0xFFFFFFFFu);
+            win = __builtin_ctzll(__ballot(found && key == k));
+        } else {
+            const float t = wave_min_f(found ? closest.distance : INFINITY);
+            const bool cand = found && closest.distance == t;
+            const uint32_t k = wave_min_u(cand ? key : 0xFFFFFFFFu);
+            win = __builtin_ctzll(__ballot(cand && key == k));
+            if (!(__shfl(best_pm, win, 64) <= t)) {
+                closest = saved_closest;
+                tr.current_hit = saved_current;
+                return -1;
+            }
+        }
+        closest = bcast_hit(closest, win);
+        tr.current_hit = bcast_hit(tr.current_hit, win);
+        return 1;
+    }
+    closest = saved_closest;
+    const bool plane = __float_as_uint(tr.current_hit.distance) != ST_DIST_UNSET;
+    if (__ballot(plane)) {
+        const uint32_t k = wave_max_u(plane ? key + 1u : 0u);
+        const int win = __builtin_ctzll(__ballot(plane && key + 1u == k));
+        tr.current_hit = bcast_hit(tr.current_hit, win);
+    } else {
+        tr.current_hit = saved_current;
+    }
+    return 0;
+}
+
 /* intersection/mesh.h:38-111. A leaf root holds every triangle; otherwise
    each step tests both children's boxes, then the triangles of the children
    that are leaves (left first), then descends. The leaf-root case runs
    through the same loop body (one step, no children) so that the triangle
-   test and the traversal are emitted once. */
+   test and the traversal are emitted once. COOP (the redo stage, every lane
+   of the wave on the same ray): large meshes are traversed by the whole
+   wave (coop_trace_mesh), with the same result. */
+template <bool COOP = false>
 DNOINLINE bool trace_mesh(DTrace &tr, const StDScene &sc, uint32_t mesh_id, bool any_hit) {
     const StDMesh mesh = load_mesh(sc, mesh_id);
     const uint32_t *nodes = sc.mesh_nodes + 8 * (size_t)mesh.node_offset;
@@ -600,18 +833,34 @@ DNOINLINE bool trace_mesh(DTrace &tr, const StDScene &sc, uint32_t mesh_id, bool
     load_node(nodes, 0, root, root_first, root_count);
     if (!hit_aabb(root, ray, closest.distance, ld)) return false;
     const bool root_leaf = root_count != 0;
+    if (COOP && !root_leaf && mesh.triangle_count >= sc.coop_min_tris) {
+        const int r = coop_trace_mesh(tr, sc, mesh, nodes, tris, root_first, any_hit);
+        if (r >= 0) return r != 0;
+        /* r < 0: the cooperative result could not be shown equal to the
+           sequential one; trace sequentially */
+    }
     uint32_t left = root_first;
     uint32_t stack_size = 0;
     /* each step loads one node pair, and a traversal loads a pair at most
        once: more steps than the mesh has nodes (at most 2T - 1) means a
        malformed tree, which would otherwise loop for ever */
-    uint32_t max_steps = 2u * mesh.triangle_count + 2u;
-    if (sc.step_cap && sc.step_cap < max_steps) max_steps = sc.step_cap;
+    /* The main stages give up after main_step_limit steps (the pixel is
+       redone). Each step loads one node pair, and a traversal loads a pair
+       at most once, so the redo stage's bound, the mesh's node count, is
+       never reached in a valid tree: a malformed one fails the launch
+       instead of looping for ever. */
+    uint32_t limit = sc.main_step_limit;
+    if (COOP) {
+        limit = 2u * mesh.triangle_count + 2u;
+        if (sc.step_cap && sc.step_cap < limit) limit = sc.step_cap;
+    }
     uint32_t steps = 0;
     for (;;) {
-        if (++steps > max_steps) {
-            trav_fault(sc, ST_FAULT_MESH_STEPS);
-            return false;
+        if (++steps > limit) {
+            if (COOP) trav_fault(sc, ST_FAULT_MESH_STEPS);
+            else tr.abandoned = 1;
+            found = false;
+            break;
         }
         uint32_t f0 = 0, c0 = 0, f1 = 0, c1 = 0, lnext = 0, rnext = 0;
         bool has_left = false, has_right = false;
@@ -658,6 +907,7 @@ struct DTileBox { int32_t x0, y0, x1, y1; };
    rect meets it, in the same order: the per-pixel test is unchanged, so
    the result is too, and a visited primitive's record is one scalar load
    per wave instead of a load per pixel for every primitive. */
+template <bool COOP = false>
 DNOINLINE bool hit_primitives(const DRay &ray, DTrace &tr, const StDScene &sc, const uint32_t *ids, uint32_t count,
                               bool any_hit, const int32_t *rects, uint32_t x, uint32_t y,
                               const DTileBox *box = nullptr) {
@@ -690,7 +940,7 @@ DNOINLINE bool hit_primitives(const DRay &ray, DTrace &tr, const StDScene &sc, c
                 tr.closest_mesh_hit.distance = closest.distance == INFINITY
                     ? INFINITY : v_len(v_sub(pos_to_object(closest.position, p), Ro));
                 prep_ray(tr.local);
-                cur = trace_mesh(tr, sc, p.mesh_id, any_hit);
+                cur = trace_mesh<COOP>(tr, sc, p.mesh_id, any_hit);
                 if (cur) hit = tr.closest_mesh_hit;
                 break;
             default: return false;
@@ -774,6 +1024,7 @@ DNOINLINE bool hit_primitives(const DRay &ray, DTrace &tr, const StDScene &sc, c
    then the leaves among them (left first, each one primitive-test call as
    the reference makes it), then descent; the leaf-root case is one step of
    the same loop, so the primitive tests are emitted once. */
+template <bool COOP = false>
 DNOINLINE bool trace_scene(DRay &ray, DTrace &tr, const StDScene &sc, bool any_hit) {
     ray.direction_reciprocal = v_rcp(ray.direction);
     prep_ray(ray);
@@ -818,7 +1069,7 @@ DNOINLINE bool trace_scene(DRay &ray, DTrace &tr, const StDScene &sc, bool any_h
 #pragma clang loop unroll(disable)
         for (uint32_t k = 0; k < nleaf; k++) {
             const uint32_t first = k ? f1 : f0, count = k ? c1 : c0;
-            if (hit_primitives(ray, tr, sc, sc.scene_leaf_ids + first, count, any_hit, nullptr, 0, 0)) {
+            if (hit_primitives<COOP>(ray, tr, sc, sc.scene_leaf_ids + first, count, any_hit, nullptr, 0, 0)) {
                 found = true;
                 if (any_hit) { stop = true; break; }
             }

@@ -50,6 +50,8 @@ struct KWave {
                               front (refracting surfaces), ctr[1] from the back (the others) */
     uint32_t *ctr;         /* [2] lengths of bounce_q's front and back parts */
     uint32_t *counts;      /* [slot] 2 words: shadow rays, bounce rays of the pixel */
+    uint32_t *redo_q;      /* [slots of the launch] pixels to redo in the redo stage (every launch) */
+    uint32_t *redo_ctr;    /* [1] length of redo_q */
 };
 
 
@@ -57,10 +59,8 @@ struct KWave {
 enum { P_COLOR, P_THRU, P_VO, P_VD, P_POS, P_NRM, P_REF, P_ALB, P_BDIR, P_MISC, ST_PLANES };
 
 #define PF_LIGHTS_SHADED 1u
+#define PF_REDO 2u         /* a traversal of this pixel gave up: the redo stage traces it again */
 
-/* "not written" marker for the current-hit distance of a shadow ray: a
-   signalling-NaN pattern, which no arithmetic result can have */
-#define ST_DIST_UNSET 0x7FA5A5A5u
 
 /* Shading state of one pixel between stages (surface.h:146-255 locals). */
 struct DPix {
@@ -169,14 +169,22 @@ DFN DShaded shaded_of(const DPix &p) {
 
 /* Fresh traversal state around one ray; `cur_dist` is the reference's
    current_hit.distance carried over from the pixel's earlier traversals. */
-DFN void trace_init(DTrace &tr, const StDScene &sc, uint32_t *scene_stack, uint32_t *mesh_stack, float cur_dist) {
+DFN void trace_init(DTrace &tr, const StDScene &sc, uint32_t *scene_stack, uint32_t *mesh_stack, float cur_dist,
+                    uint32_t *aux_stack = nullptr) {
     tr = DTrace{};
     tr.closest_mesh_hit.object_type = ST_PRIM_MESH;
     tr.current_hit.distance = cur_dist;
     tr.mesh_stack_size = sc.mesh_stack_size;
     tr.scene_stack = scene_stack;
     tr.mesh_stack = mesh_stack;
+    tr.aux_stack = aux_stack;
 }
+
+/* Per-lane traversal stacks of a stage kernel: scene and mesh columns, and
+   for the redo stage the cooperative traversal's path maxima. */
+struct DStacks {
+    uint32_t *scene, *mesh, *aux;
+};
 
 /* The unnormalised primary direction `current` of traced pixel (x, y): the
    row's sequential sum (render/raytracer.h:156-170) continued from its
@@ -252,6 +260,7 @@ DFN void record_counts(const KWave &w, uint32_t slot, uint32_t n_shadow, uint32_
 /* One shadow ray (surface.h:13-25 and trace.h:8-76 any-hit): records
    whether the light is occluded and the last current-hit distance the
    traversal wrote. */
+template <bool COOP = false>
 DFN void shadow_ray(const KWave &w, const StDScene &sc, DTrace &tr, uint32_t slot, uint32_t light, sv3 pos,
                     const DLightDir &L) {
     tr.closest_hit.position = L.pos;
@@ -260,7 +269,7 @@ DFN void shadow_ray(const KWave &w, const StDScene &sc, DTrace &tr, uint32_t slo
     DRay ray;
     ray.origin = pos;
     ray.direction = L.dir;
-    const bool occluded = trace_scene(ray, tr, sc, true);
+    const bool occluded = trace_scene<COOP>(ray, tr, sc, true);
     stg2(w.shadow_res, (size_t)light * w.n + slot, occluded ? 1u : 0u, __float_as_uint(tr.current_hit.distance));
     if (occluded && light + 1 == sc.light_count)
         ((ST_GLOBAL float *)w.occ_d2)[slot] = tr.closest_hit.distance_squared;
@@ -271,15 +280,19 @@ DFN void shadow_ray(const KWave &w, const StDScene &sc, DTrace &tr, uint32_t slo
    that faces it, each from a fresh hit state. The traversal reads none of
    the pixel's shading state, so the rays need no order among themselves;
    the shading replays their current-hit writes in light order. */
-DFN void trace_level_shadows(const KWave &w, const StDScene &sc, uint32_t *scene_stack, uint32_t *mesh_stack,
-                             uint32_t slot, sv3 pos, sv3 nrm) {
+template <bool COOP = false>
+DFN bool trace_level_shadows(const KWave &w, const StDScene &sc, const DStacks &stk, uint32_t slot, sv3 pos,
+                             sv3 nrm) {
+    uint32_t abandoned = 0;
     for (uint32_t i = 0; i < sc.light_count; i++) {
         const DLightDir L = light_dir(sc, i, pos);
         if (!(v_dot(nrm, L.dir) > 0)) continue;
         DTrace tr;
-        trace_init(tr, sc, scene_stack, mesh_stack, __uint_as_float(ST_DIST_UNSET));
-        shadow_ray(w, sc, tr, slot, i, pos, L);
+        trace_init(tr, sc, stk.scene, stk.mesh, __uint_as_float(ST_DIST_UNSET), stk.aux);
+        shadow_ray<COOP>(w, sc, tr, slot, i, pos, L);
+        abandoned |= tr.abandoned;
     }
+    return abandoned != 0;
 }
 
 /* The shading of one pixel's level (surface.h:200-243): direct light from

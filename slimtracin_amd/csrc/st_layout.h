@@ -103,8 +103,23 @@ typedef struct StDScene {
     uint32_t *fault;               /* per launch (set in the launch's scene table): a traversal that runs
                                       past its step bound or stack ORs ST_FAULT_* in and stops */
     uint32_t step_cap;             /* per launch: 0, or a lower step bound (st_debug_step_cap) */
-    uint32_t pad_[3];
+    uint32_t main_step_limit;      /* per launch: mesh traversals of the main stages give up after this
+                                      many steps and their pixel is redone (ST_ABANDON_STEPS) */
+    uint32_t coop_min_tris;        /* per launch: the redo stage splits traversals of meshes of at least
+                                      this many triangles across the wave (ST_COOP_MIN_TRIS) */
+    uint32_t pad_;
 } StDScene;
+
+/* Mesh traversals longer than this (node-pair steps) give up in the main
+   stages: the lane would hold its wave for the whole chain of dependent
+   fetches (a ray with an exactly zero object-space direction component
+   makes the reference's slab test admit every box on one side of the mesh,
+   hundreds of thousands of steps). Its pixel is redone from the start by
+   the redo stage, where the whole wave traces it together. */
+#define ST_ABANDON_STEPS 2048u
+/* Meshes from this size are traversed by the whole wave in the redo stage;
+   smaller ones sequentially (a few hundred steps at most). */
+#define ST_COOP_MIN_TRIS 4096u
 
 /* Fault bits of StDScene.fault. A traversal of a valid tree never sets them:
    it loads each node pair at most once (so takes at most as many steps as the

@@ -1,0 +1,125 @@
+"""GPU: the redo stage. A mesh traversal of the main stages that runs past
+its step limit gives up, and the redo stage traces its pixel again from the
+primary ray with all 64 lanes of a wave, splitting each traversal of a large
+mesh across the lanes (coop_trace_mesh: the reference's traversal order
+rebuilt from the units the lanes take, st_trace.h). Frames must not depend
+on it: with a give-up limit of one step and a cooperative-mesh minimum of 16
+triangles (st_debug_redo), nearly every pixel goes through the redo stage and
+every mesh traversal in it through the cooperative path, and frames, hit
+records and ray counts still equal the CPU oracle's, closest and any-hit,
+over host SAH and device LBVH mesh BVHs, with and without textures."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import checkers as ck
+import scenes
+import slimtracin_amd as st
+from parity import compare_hits, compare_pixels
+from slimtracin_amd.scene import (IS_ROTATED, MODE_BEAUTY, MODE_DEPTH, MODE_NORMALS, MODE_UVS, blob_mesh_info,
+                                  blob_prims, blob_with_prebuilt_meshes, blob_with_prims, make_view)
+
+pytestmark = pytest.mark.gpu
+MODES = [MODE_BEAUTY, MODE_DEPTH, MODE_NORMALS, MODE_UVS]
+
+
+@pytest.fixture
+def redo_everything():
+    st.debug_redo(1, 16)
+    try:
+        yield
+    finally:
+        st.debug_redo(0, 0)
+
+
+def _check(blob, view, label, lbvh=False, want_blob=None):
+    ds = st.DeviceScene(blob, mesh_lbvh=lbvh)
+    try:
+        gp, gh, gc = st.render(ds, view)
+        redone = st.last_redo_count()
+    finally:
+        ds.close()
+    op, oh, oc = ck.oracle_render(want_blob or blob, view)
+    mode = int(view["render_mode"][0])
+    hs = compare_hits(gh, oh)
+    ps = compare_pixels(gp, op, mode)
+    print(label, "redone", redone, "hits", hs, "pixels", ps, flush=True)
+    assert hs["id_mismatch"] == 0 and hs["t_mismatch"] == 0, (label, hs)
+    assert ps["ok"], (label, ps)
+    assert (gc.primary, gc.shadow, gc.bounce) == (oc.primary_rays, oc.shadow_rays, oc.bounce_rays), label
+    return redone
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_monkey_all_redone(redo_everything, mode):
+    s, cam = scenes.ex07_monkey()
+    view = make_view(width=120, height=90, mode=mode, **dict(cam, depth=4))
+    assert _check(s.to_blob(), view, f"monkey/m{mode}") > 1000
+
+
+@pytest.mark.parametrize("lbvh", [False, True])
+@pytest.mark.parametrize("mode", [MODE_BEAUTY, MODE_NORMALS])
+def test_mesh_scene_all_redone(redo_everything, mode, lbvh):
+    blob = st.gen_mesh_scene(31, 20000)
+    want = blob
+    if lbvh:
+        T = blob_mesh_info(blob)[0]["triangle_count"]
+        want = blob_with_prebuilt_meshes(blob, [ck.oracle_mesh_lbvh(blob, 0, T)])
+    view = make_view(width=96, height=72, mode=mode, depth=5, position=(4.0, 7.0, -12.0), yaw=0.15, pitch=-0.25)
+    assert _check(blob, view, f"mesh20k/m{mode}/lbvh{lbvh}", lbvh=lbvh, want_blob=want) > 1000
+
+
+def test_pbr_and_crowd_all_redone(redo_everything):
+    blob = st.gen_pbr(5, 12)
+    view = make_view(width=128, height=72, mode=MODE_BEAUTY, depth=4, position=(0.0, 6.0, -14.0), pitch=-0.15)
+    _check(blob, view, "pbr")
+    s, cam = scenes.crowd(3, 60)
+    view = make_view(width=128, height=96, mode=MODE_BEAUTY, **dict(cam, depth=3))
+    _check(s.to_blob(), view, "crowd")
+
+
+def test_textured_all_redone(redo_everything):
+    s, cam = scenes.ex09_textures()
+    for mode in (MODE_BEAUTY, MODE_NORMALS):
+        view = make_view(width=120, height=90, mode=mode, cone_angle=0.002, **dict(cam, depth=3))
+        _check(s.to_blob(), view, f"textures/m{mode}")
+
+
+def test_degenerate_rays_take_the_redo_stage():
+    """Rays with an exactly zero direction component through an unrotated
+    mesh (the slab test admits every box on one side) give up at the default
+    limit and are traced by whole waves; the frame equals the oracle's."""
+    blob = st.gen_mesh_scene(5, 60000)
+    prims = blob_prims(blob)
+    prims[0]["rotation"] = (0.0, 0.0, 0.0, 1.0)
+    prims[0]["flags"] &= ~np.uint32(IS_ROTATED)
+    blob = blob_with_prims(blob, prims)
+    T = blob_mesh_info(blob)[0]["triangle_count"]
+    want = blob_with_prebuilt_meshes(blob, [ck.oracle_mesh_lbvh(blob, 0, T)])
+    for mode in (MODE_NORMALS, MODE_BEAUTY):
+        view = make_view(width=65, height=49, mode=mode, depth=3, position=(0.0, 3.0, -10.0))
+        assert _check(blob, view, f"degenerate/m{mode}", lbvh=True, want_blob=want) > 0
+
+
+def test_batched_redo_equals_plain(redo_everything):
+    """A batch whose pixels go through the redo stage equals the plain
+    launches of its requests."""
+    sc = [scenes.ex07_monkey(), scenes.crowd(2, 40)]
+    ds = [st.DeviceScene(s.to_blob()) for s, _ in sc]
+    try:
+        views, owners = [], []
+        for i, (w, h, mode) in enumerate([(64, 48, MODE_BEAUTY), (50, 70, MODE_NORMALS), (90, 40, MODE_UVS),
+                                          (33, 33, MODE_BEAUTY)]):
+            views.append(make_view(width=w, height=h, mode=mode, **dict(sc[i % 2][1], depth=3)))
+            owners.append(ds[i % 2])
+        views = np.concatenate(views)
+        bp, bh, bc = st.render_batch(owners, views, hits=True)
+        assert st.last_redo_count() > 100
+        for i in range(len(views)):
+            p, h, c = st.render(owners[i], views[i:i + 1])
+            assert p.tobytes() == bp[i].tobytes() and h.tobytes() == bh[i].tobytes()
+            assert (c.primary, c.shadow, c.bounce) == (bc[i].primary, bc[i].shadow, bc[i].bounce)
+    finally:
+        for d in ds:
+            d.close()
