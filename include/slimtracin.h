@@ -103,6 +103,23 @@ int st_scene_rebuild(StSceneHandle *scene, uint32_t mode, float *ms);
    NULL. */
 int st_render(StSceneHandle *scene, const StView *view, StFloatPixel *pixels, StHitRec *hits, StRayCounts *counts);
 
+/* st_render without any allocation: no hipMalloc, hipFree, page-locked or
+   heap allocation in the call. No counterpart in the reference; for callers
+   that cannot allocate (code under HIP graph capture, tight serving loops,
+   harnesses that forbid allocation). Every per-launch device buffer comes
+   from a file-scope __device__ scratch array of ST_NOALLOC_SCRATCH_BYTES
+   (128 MiB) per GPU, held until the process ends, whose address st_init
+   takes with hipGetSymbolAddress; a request whose buffers do not fit (about
+   500k traced pixels in Beauty mode with 3 lights, more in the other modes)
+   fails with ST_ERR_MEMORY. pixels ((width+1)*(height+1) FloatPixels) is
+   required, hits and counts may be NULL. Every call on a GPU shares that
+   GPU's array and its contents stay from the last call: calls are
+   serialised by the library lock within a process, and calls that overlap on
+   one GPU from several processes race. Results equal st_render's (tiles are
+   claimed in tile order: the array keeps no tile history). */
+int st_render_noalloc(StSceneHandle *scene, const StView *view, StFloatPixel *pixels, StHitRec *hits,
+                      StRayCounts *counts);
+
 /* n requests fused into one launch. pixels[i] / hits[i] may be NULL. */
 int st_render_batch(StSceneHandle *const *scenes, const StView *views, uint32_t n, StFloatPixel *const *pixels,
                     StHitRec *const *hits, StRayCounts *counts, uint32_t flags);

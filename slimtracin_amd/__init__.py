@@ -210,6 +210,19 @@ def render(scene: DeviceScene, view: np.ndarray, hits: bool = True, out: np.ndar
     return pixels, hrec, counts
 
 
+def render_noalloc(scene: DeviceScene, view: np.ndarray, hits: bool = True, out: np.ndarray | None = None):
+    """st_render_noalloc: the plain launch with every device buffer taken from
+    the library's fixed scratch array (no allocation in the call)."""
+    n = grid_size(view)
+    pixels = out if out is not None else np.zeros(n, dtype=PIXEL_DTYPE)
+    assert len(pixels) >= n and pixels.dtype == PIXEL_DTYPE
+    hrec = np.zeros(n, dtype=HIT_DTYPE) if hits else None
+    counts = RayCounts()
+    _check(lib().st_render_noalloc(scene.handle, _ptr(view), _ptr(pixels), _ptr(hrec) if hits else None,
+                                   C.byref(counts)))
+    return pixels, hrec, counts
+
+
 def render_batch(scenes, views: np.ndarray, hits: bool = False, persistent: bool = True, outs=None,
                  graph: bool = False):
     """Fuse len(views) requests into one launch; returns per-request outputs

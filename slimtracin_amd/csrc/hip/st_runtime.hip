@@ -893,6 +893,19 @@ static void launch_limits(StDScene &d, uint32_t step_cap, uint32_t abandon_steps
     d.coop_min_tris = coop_min_tris ? coop_min_tris : ST_COOP_MIN_TRIS;
 }
 
+/* Device buffers of st_render_noalloc: one array per GPU, part of the code
+   object, held until the process ends. In host code `scratch` names a host
+   stand-in, not device memory: host code uses only the address
+   hipGetSymbolAddress returns for the current device (g.noalloc_scratch). */
+static __device__ __align__(256) char scratch[ST_NOALLOC_SCRATCH_BYTES];
+#define ST_MAX_DEVICES 64
+static char *g_noalloc_scratch[ST_MAX_DEVICES];
+/* host side of a st_render_noalloc launch: the upload block (request,
+   scene record, row starts) and the ray counts read back */
+#define ST_NOALLOC_HOST_BYTES (1u << 21)
+static char g_noalloc_host[ST_NOALLOC_HOST_BYTES] __attribute__((aligned(256)));
+static unsigned long long g_noalloc_counts[5];
+
 static int ctx_init(LaunchCtx *c) {
     HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     HIP_TRY(hipMalloc(&c->tile_counter, 256));
@@ -941,6 +954,11 @@ extern "C" int st_init(int device) {
     g.cus = 256;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
         g.cus = prop.multiProcessorCount;
+    /* st_render_noalloc's buffers: this device's copy of the scratch array */
+    if (device >= ST_MAX_DEVICES) return set_err(ST_ERR_ARG, "device %d above %d", device, ST_MAX_DEVICES);
+    void *sp = NULL;
+    HIP_TRY(hipGetSymbolAddress(&sp, HIP_SYMBOL(scratch)));
+    g_noalloc_scratch[device] = (char *)sp;
     g.device = device;
     g.ready = 1;
     return ST_OK;
@@ -1797,6 +1815,167 @@ extern "C" int st_render(StSceneHandle *scene, const StView *view, StFloatPixel 
     int rc = ensure_init();
     if (rc) return rc;
     return launch(&scene, view, 1, hits != NULL, (int)g.tune.persistent, counts, &pixels, &hits, NULL, NULL);
+}
+
+/* st_render without allocating: every per-launch device buffer is carved
+   from this device's scratch array, the host side uses static buffers, and
+   the launch runs on the main stream. The same kernels, in the same order,
+   as a single-request st_render (tiles in tile order: the array keeps no
+   tile history). */
+extern "C" int st_render_noalloc(StSceneHandle *scene, const StView *view, StFloatPixel *pixels, StHitRec *hits,
+                                 StRayCounts *counts) {
+    if (!scene || !view || !pixels) return set_err(ST_ERR_ARG, "null argument");
+    RT_LOCK;
+    if (!g.ready) return set_err(ST_ERR_ARG, "st_init has not run: the scratch array's address is taken there");
+    char *const base = g_noalloc_scratch[g.device];
+    if (!base) return set_err(ST_ERR_HIP, "no scratch array address for device %d", g.device);
+    if (!scene->host || !scene->dbuf) return set_err(ST_ERR_ARG, "scene is not resident");
+    const StScene *hs = scene->host;
+    if (view->width == 0 || view->height == 0 || view->width >= 0xFFFF || view->height >= 0xFFFF ||
+        view->render_mode > ST_MODE_UVS)
+        return set_err(ST_ERR_ARG, "invalid view (%ux%u, mode %u)", view->width, view->height, view->render_mode);
+    const StTuning tune = g.tune;
+    const uint32_t width1 = view->width + 1u, height1 = view->height + 1u;
+    const size_t px = (size_t)width1 * height1;
+    const uint32_t tw = 1u << tune.tile_shift, th = 64u >> tune.tile_shift;
+    const uint32_t tiles_x = (width1 + tw - 1) / tw, tiles = tiles_x * ((height1 + th - 1) / th);
+    const uint32_t P = hs->prim_count;
+    const bool beauty = view->render_mode == ST_MODE_BEAUTY && (uint8_t)view->depth;
+    const uint32_t lmax = hs->light_count > 1 ? hs->light_count : 1;
+    const size_t slots = (size_t)tiles * 64;
+    /* layout: upload block (request, scene record, row starts, rects), ray
+       counts + fault word, direction checkpoints, frame, hit records, tile
+       claim order, tile times, counters, depth-loop state */
+    const size_t o_scene = round256(sizeof(KReq)), o_rows = o_scene + round256(sizeof(StDScene));
+    const size_t o_rects = o_rows + round256((size_t)height1 * 16), up_host = o_rects;
+    size_t off = o_rects + round256(((size_t)P + 1) * 16);
+    auto take = [&](size_t bytes) {
+        const size_t o = off;
+        off += round256(bytes);
+        return o;
+    };
+    const size_t o_cnt = take(40), o_dirs = take((size_t)height1 * ST_DIR_MARKS(width1) * 16), o_pix = take(px * 24);
+    const size_t o_hits = hits ? take(px * 16) : 0;
+    const size_t o_order = take((size_t)tiles * 4), o_cost = take((size_t)tiles * 4), o_ctr = take(256);
+    const size_t o_redo = take(slots * 4);
+    size_t o_st = 0, o_sres = 0, o_occ = 0, o_bq = 0, o_wctr = 0, o_wcnt = 0;
+    if (beauty) {
+        o_st = take(slots * 16 * ST_PLANES);
+        o_sres = take(slots * lmax * 8);
+        o_occ = take(slots * 4);
+        o_bq = take(slots * 4);
+        o_wctr = take(8);
+        o_wcnt = take(slots * 8);
+    }
+    if (off > ST_NOALLOC_SCRATCH_BYTES)
+        return set_err(ST_ERR_MEMORY, "request needs %zu bytes of scratch; st_render_noalloc has %u", off,
+                       (unsigned)ST_NOALLOC_SCRATCH_BYTES);
+    if (up_host > ST_NOALLOC_HOST_BYTES) return set_err(ST_ERR_ARG, "view too tall for st_render_noalloc");
+    StHostView hv;
+    if (st_view_prepare_rows(hs, view, &hv, (float *)(g_noalloc_host + o_rows)))
+        return set_err(ST_ERR_ARG, "invalid view");
+    KReq *rq = (KReq *)g_noalloc_host;
+    memset(rq, 0, sizeof(KReq));
+    StDView &dv = rq->v;
+    dv = hv.dv;
+    dv.scene = 0;
+    dv.tile_base = 0;
+    dv.tiles_x = tiles_x;
+    dv.tile_shift = tune.tile_shift;
+    dv.row_starts = (const float *)(base + o_rows);
+    dv.rects = (const int32_t *)(base + o_rects);
+    dv.dir_marks = (float *)(base + o_dirs);
+    dv.pixels = base + o_pix;
+    dv.hits = hits ? base + o_hits : NULL;
+    unsigned long long *cnt_d = (unsigned long long *)(base + o_cnt);
+    dv.counters = cnt_d;
+    rq->row_base = 0;
+    rq->out_base = 0;
+    rq->tile_count = tiles;
+    rq->count_block = 0;
+    const sm3 &m = hv.rotation_matrix_inverted;
+    const float rmi[9] = {m.X.x, m.X.y, m.X.z, m.Y.x, m.Y.y, m.Y.z, m.Z.x, m.Z.y, m.Z.z};
+    memcpy(rq->ssb_rmi, rmi, sizeof rmi);
+    rq->focal_length = hv.focal_length;
+    rq->device_ssb = 1;
+    StDScene dsc = scene->dev;
+    dsc.fault = (uint32_t *)(cnt_d + 3);
+    launch_limits(dsc, g.step_cap, g.abandon_steps, g.coop_min_tris);
+    memcpy(g_noalloc_host + o_scene, &dsc, sizeof dsc);
+    KWave w;
+    memset(&w, 0, sizeof w);
+    if (beauty) {
+        w.n = (uint32_t)slots;
+        w.lmax = lmax;
+        w.st = (float *)(base + o_st);
+        w.shadow_res = (uint32_t *)(base + o_sres);
+        w.occ_d2 = (float *)(base + o_occ);
+        w.bounce_q = (uint32_t *)(base + o_bq);
+        w.ctr = (uint32_t *)(base + o_wctr);
+        w.counts = (uint32_t *)(base + o_wcnt);
+    }
+    w.redo_q = (uint32_t *)(base + o_redo);
+    w.redo_ctr = (uint32_t *)(cnt_d + 4);
+    const uint32_t s_entries = dsc.scene_stack_need > 1 ? dsc.scene_stack_need : 1;
+    const uint32_t m_entries = dsc.mesh_stack_size > 1 ? dsc.mesh_stack_size : 1;
+    const size_t lds = (size_t)BLOCK_THREADS * (s_entries + m_entries) * sizeof(uint32_t);
+    if (lds > 160 * 1024) return set_err(ST_ERR_SCENE, "traversal stacks need %zu B of LDS", lds);
+    uint32_t blocks, xcd_blocks = 0;
+    size_t lds_primary = lds;
+    if (tune.persistent) {
+        const uint32_t want = (uint32_t)g.cus * tune.blocks_per_cu, need = (tiles + WAVES_PER_BLOCK - 1) / WAVES_PER_BLOCK;
+        blocks = need < want ? need : want;
+    } else {
+        blocks = ((tiles + WAVES_PER_BLOCK - 1) / WAVES_PER_BLOCK + 7u) & ~7u;
+        xcd_blocks = blocks;
+        if (tune.blocks_per_cu < 4)
+            (void)capped_blocks(reinterpret_cast<const void *>(k_primary), lds, tune.blocks_per_cu, &lds_primary);
+    }
+    const KReq *reqs_d = (const KReq *)base;
+    const StDScene *sc_d = (const StDScene *)(base + o_scene);
+    uint32_t *ctr = (uint32_t *)(base + o_ctr);
+    hipStream_t stream = g.main.stream;
+    /* the request, scene record and row starts; the rects region is filled
+       by the device SSB pass */
+    HIP_TRY(hipMemcpyAsync(base, g_noalloc_host, up_host, hipMemcpyHostToDevice, stream));
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, stream, cnt_d, 5u);
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, stream, ctr, 64u);
+    hipLaunchKernelGGL(k_ssb, dim3(1), dim3(256), 0, stream, sc_d, reqs_d);
+    if (beauty) hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, stream, w.ctr, 2u);
+    hipLaunchKernelGGL(k_primary_dirs, dim3((height1 * 3 + DIRS_BLOCK - 1) / DIRS_BLOCK), dim3(DIRS_BLOCK), 0, stream,
+                       reqs_d, 1u, height1);
+    hipLaunchKernelGGL(k_tile_order, dim3((tiles + ORDER_THREADS - 1) / ORDER_THREADS), dim3(ORDER_THREADS), 0, stream,
+                       (const uint32_t *)(base + o_cost), (uint32_t *)(base + o_order), tiles,
+                       (const unsigned long long *)cnt_d, ctr + ORDER_CLAIM, 0u);
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(k_primary, dim3(blocks), dim3(BLOCK_THREADS), lds_primary, stream, sc_d, reqs_d, 1u, tiles, ctr,
+                       (uint32_t)tune.persistent, xcd_blocks, s_entries, m_entries, w,
+                       (const uint32_t *)(base + o_order), (uint32_t *)(base + o_cost));
+    HIP_TRY(hipGetLastError());
+    if (beauty && dv.depth > 1) {
+        const uint32_t b_bounce = resident_blocks(reinterpret_cast<const void *>(k_bounce), lds);
+        hipLaunchKernelGGL(k_bounce, dim3(b_bounce), dim3(BLOCK_THREADS), lds, stream, sc_d, reqs_d, 1u, w,
+                           ctr + BOUNCE_CLAIM, s_entries, m_entries);
+    }
+    hipLaunchKernelGGL(k_redo, dim3((uint32_t)g.cus * 8u), dim3(64), (size_t)64 * (s_entries + 2 * m_entries) * 4,
+                       stream, sc_d, reqs_d, 1u, w, s_entries, m_entries);
+    if (beauty)
+        hipLaunchKernelGGL(k_counts, dim3((tiles + COUNT_TILES - 1) / COUNT_TILES), dim3(256), 0, stream, reqs_d, 1u,
+                           (const uint32_t *)w.counts);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(pixels, base + o_pix, px * 24, hipMemcpyDeviceToHost, stream));
+    if (hits) HIP_TRY(hipMemcpyAsync(hits, base + o_hits, px * 16, hipMemcpyDeviceToHost, stream));
+    HIP_TRY(hipMemcpyAsync(g_noalloc_counts, cnt_d, 40, hipMemcpyDeviceToHost, stream));
+    HIP_TRY(hipStreamSynchronize(stream));
+    if (g_noalloc_counts[3])
+        return set_err(ST_ERR_SCENE, "a traversal ran past its bound (fault bits 0x%llx): malformed BVH",
+                       g_noalloc_counts[3]);
+    if (counts) {
+        counts->primary = px;
+        counts->shadow = g_noalloc_counts[1];
+        counts->bounce = g_noalloc_counts[2];
+    }
+    return ST_OK;
 }
 
 /* Graph mode for the duration of one API call (ST_BATCH_GRAPH). */

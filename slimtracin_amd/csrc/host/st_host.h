@@ -157,6 +157,10 @@ int st_view_prepare(const StScene *sc, const StView *v, StHostView *hv);
 /* with_ssb 0: no SSB pass (rects, raw_rects and flags are left unset; the
    device computes the gate rects) */
 int st_view_prepare_ex(const StScene *sc, const StView *v, StHostView *hv, int with_ssb);
+/* The same without allocating: row starts into the caller's buffer (4 floats
+   per traced row, (height + 1) rows), no host SSB pass (the device computes
+   the gate rects). Not to be passed to st_view_release. */
+int st_view_prepare_rows(const StScene *sc, const StView *v, StHostView *hv, float *row_starts);
 
 /* Set-up for a frame described the way renderSceneOnGPU receives it: the
    viewport's projection plane and camera, and per-primitive screen bounds

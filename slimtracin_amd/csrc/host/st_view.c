@@ -72,7 +72,20 @@ static sm3 rotation_matrix(float yaw_amt, float pitch_amt, float roll_amt) {
 
 int st_view_prepare(const StScene *sc, const StView *v, StHostView *hv) { return st_view_prepare_ex(sc, v, hv, 1); }
 
+static int view_setup(const StScene *sc, const StView *v, StHostView *hv, int with_ssb, float *rows_buf);
+
 int st_view_prepare_ex(const StScene *sc, const StView *v, StHostView *hv, int with_ssb) {
+    return view_setup(sc, v, hv, with_ssb, NULL);
+}
+
+int st_view_prepare_rows(const StScene *sc, const StView *v, StHostView *hv, float *row_starts) {
+    if (!row_starts) return -1;
+    return view_setup(sc, v, hv, 0, row_starts);
+}
+
+/* rows_buf: the caller's row-start buffer (4 floats per traced row); then
+   nothing is allocated and the gate rects are left to the device SSB pass. */
+static int view_setup(const StScene *sc, const StView *v, StHostView *hv, int with_ssb, float *rows_buf) {
     memset(hv, 0, sizeof *hv);
     if (v->width == 0 || v->height == 0 || v->width >= 0xFFFF || v->height >= 0xFFFF) return -1;
     if (v->render_mode > ST_MODE_UVS) return -1;
@@ -93,11 +106,16 @@ int st_view_prepare_ex(const StScene *sc, const StView *v, StHostView *hv, int w
     right = v_scale(right_dir, 2);
     down = v_scale(up, -2);
 
-    hv->row_starts = (float *)malloc(4 * sizeof(float) * d.height);
-    hv->rects = (int32_t *)malloc(4 * sizeof(int32_t) * (sc->prim_count + 1));
-    hv->flags = (uint8_t *)malloc(sc->prim_count + 1);
-    hv->raw_rects = (int32_t *)malloc(4 * sizeof(int32_t) * (sc->prim_count + 1));
-    if (!hv->row_starts || !hv->rects || !hv->flags || !hv->raw_rects) { st_view_release(hv); return -2; }
+    if (rows_buf) {
+        with_ssb = 0;
+        hv->row_starts = rows_buf;
+    } else {
+        hv->row_starts = (float *)malloc(4 * sizeof(float) * d.height);
+        hv->rects = (int32_t *)malloc(4 * sizeof(int32_t) * (sc->prim_count + 1));
+        hv->flags = (uint8_t *)malloc(sc->prim_count + 1);
+        hv->raw_rects = (int32_t *)malloc(4 * sizeof(int32_t) * (sc->prim_count + 1));
+        if (!hv->row_starts || !hv->rects || !hv->flags || !hv->raw_rects) { st_view_release(hv); return -2; }
+    }
     sv3 s = start;
     for (uint32_t y = 0; y < d.height; y++) {
         hv->row_starts[4 * y] = s.x;

@@ -19,6 +19,9 @@
 #define ST_SCENE_STACK_MAX 64   /* scene BVH traversal stack entries per ray */
 #define ST_MESH_STACK_MAX 64    /* mesh BVH traversal stack entries per ray */
 
+/* Device scratch of st_render_noalloc (bytes, one array per GPU). */
+#define ST_NOALLOC_SCRATCH_BYTES (128u << 20)
+
 /* Checkpoint spacing of the primary directions along a row (pixels). */
 #define ST_DIR_MARK 8u
 #define ST_DIR_MARKS(width1) (((width1) + ST_DIR_MARK - 1u) / ST_DIR_MARK)
