@@ -256,7 +256,7 @@ __device__ static inline __attribute__((always_inline)) void bounce_chain(const 
    misses, emissive hits, depth 0) or the Beauty shading entry
    (surface.h:146-170) and the first level of its depth loop. The main stage
    queues a pixel that bounces for the bounce stage and one whose traversal
-   gave up for the redo stage; the redo stage (COOP, the whole wave on one
+   gave up for the redo stage; the redo stage (COOP, a whole block on one
    pixel) runs the pixel's whole depth loop here. `slot` = tile * 64 + lane
    indexes the pixel's depth-loop state. */
 template <bool COOP = false>
@@ -449,13 +449,17 @@ k_primary(const StDScene *__restrict__ scenes, const KReq *__restrict__ reqs, ui
 }
 
 /* Redo stage: the pixels whose traversal gave up in the main stages, one
-   per wave, all 64 lanes on the same pixel: its primary ray and whole depth
-   loop again, with every traversal of a large mesh split across the lanes
-   (coop_trace_mesh). The pixel's frame value, hit record and ray counts
-   replace what the main stages wrote for it. Blocks of one wave: each lane
-   has a scene, a mesh and a path-maximum stack column. */
-__global__ void __launch_bounds__(64) k_redo(const StDScene *__restrict__ scenes, const KReq *__restrict__ reqs,
-                                             uint32_t nreq, KWave w, uint32_t scene_entries, uint32_t mesh_entries) {
+   per block, every thread of the block on the same pixel: its primary ray
+   and whole depth loop again, with every traversal of a large mesh split
+   across the block's threads (coop_trace_mesh). The pixel's frame value,
+   hit record and ray counts replace what the main stages wrote for it.
+   Each thread has a scene, a mesh and a path-maximum stack column; every
+   thread computes the same values, so the block's barriers are reached by
+   all of its threads. */
+#define REDO_THREADS 256u
+__global__ void __launch_bounds__(REDO_THREADS) k_redo(const StDScene *__restrict__ scenes,
+                                                       const KReq *__restrict__ reqs, uint32_t nreq, KWave w,
+                                                       uint32_t scene_entries, uint32_t mesh_entries) {
     ST_LDS_STACKS(scene_entries, 2u * mesh_entries);
     const DStacks stk = {scene_stack, mesh_stack, mesh_stack + 64u * mesh_entries};
     const uint32_t count = __builtin_amdgcn_readfirstlane(ldgu(w.redo_ctr, 0));
@@ -470,6 +474,17 @@ __global__ void __launch_bounds__(64) k_redo(const StDScene *__restrict__ scenes
         const DTileBox box = {(int32_t)x, (int32_t)y, (int32_t)x + 1, (int32_t)y + 1};
         primary_pixel<true>(sc, v, slot, x, y, true, box, stk, w);
     }
+}
+
+/* Redo stage launch shape: blocks of up to REDO_THREADS threads, as many
+   waves as the LDS stack columns (scene, mesh and path maxima per thread)
+   leave room for beside the cooperative traversal's block state. */
+static void redo_shape(uint32_t s_entries, uint32_t m_entries, uint32_t &threads, size_t &lds) {
+    const size_t per_wave = (size_t)64 * (s_entries + 2 * m_entries) * sizeof(uint32_t);
+    uint32_t waves = REDO_THREADS / 64u;
+    while (waves > 1 && per_wave * waves > 160 * 1024 - 256) waves >>= 1;
+    threads = 64u * waves;
+    lds = per_wave * waves;
 }
 
 /* Sum of the previous launch's tile times (cost_sum zeroed before): one
@@ -1628,9 +1643,11 @@ int st_rt_launch_ctx(LaunchCtx *c, StSceneHandle *const *scenes, StHostView *hv,
         b_bounce = resident_blocks(reinterpret_cast<const void *>(k_bounce), lds);
     }
     const uint32_t cost_blocks = std::min<uint32_t>((tiles + COST_THREADS - 1) / COST_THREADS, 1024u);
-    /* redo stage: one-wave blocks, each with its scene, mesh and path-maximum stack columns */
-    const size_t lds_redo = (size_t)64 * (s_entries + 2 * m_entries) * sizeof(uint32_t);
-    const uint32_t redo_blocks = (uint32_t)g.cus * 8u;
+    /* redo stage: one block per CU, each on one pixel at a time */
+    uint32_t redo_threads;
+    size_t lds_redo;
+    redo_shape(s_entries, m_entries, redo_threads, lds_redo);
+    const uint32_t redo_blocks = (uint32_t)g.cus;
 
     /* per-stage timing (bench): an event before the first stage and after each */
     uint32_t marks = 0;
@@ -1684,7 +1701,7 @@ int st_rt_launch_ctx(LaunchCtx *c, StSceneHandle *const *scenes, StHostView *hv,
         }
         /* the pixels that gave up, traced again (it reads the queue's length
            on the device: no round trip; nothing to do is one quick launch) */
-        hipLaunchKernelGGL(k_redo, dim3(redo_blocks), dim3(64), lds_redo, c->stream, (const StDScene *)sc_d,
+        hipLaunchKernelGGL(k_redo, dim3(redo_blocks), dim3(redo_threads), lds_redo, c->stream, (const StDScene *)sc_d,
                            (const KReq *)reqs_d, n, w, s_entries, m_entries);
         HIP_TRY(mark());
         if (levels && count_blocks)
@@ -1708,7 +1725,7 @@ int st_rt_launch_ctx(LaunchCtx *c, StSceneHandle *const *scenes, StHostView *hv,
                                      (uint64_t)(uintptr_t)st,
                                      (uint64_t)(uintptr_t)cnt_d, (uint64_t)(uintptr_t)w.st, w.n,
                                      (uint64_t)(uintptr_t)c->tile_cost, use_history, cost_blocks,
-                                     (uint64_t)(uintptr_t)w.redo_q, redo_blocks, lds_redo};
+                                     (uint64_t)(uintptr_t)w.redo_q, redo_blocks, redo_threads, lds_redo};
         for (uint32_t i = 0; i < n; i++) {
             const StDView &dv = reqs_h[i].v;
             sig.insert(sig.end(), {dv.width1, dv.height1, dv.mode, dv.depth, dv.tile_shift,
@@ -1957,8 +1974,11 @@ extern "C" int st_render_noalloc(StSceneHandle *scene, const StView *view, StFlo
         hipLaunchKernelGGL(k_bounce, dim3(b_bounce), dim3(BLOCK_THREADS), lds, stream, sc_d, reqs_d, 1u, w,
                            ctr + BOUNCE_CLAIM, s_entries, m_entries);
     }
-    hipLaunchKernelGGL(k_redo, dim3((uint32_t)g.cus * 8u), dim3(64), (size_t)64 * (s_entries + 2 * m_entries) * 4,
-                       stream, sc_d, reqs_d, 1u, w, s_entries, m_entries);
+    uint32_t redo_threads;
+    size_t lds_redo;
+    redo_shape(s_entries, m_entries, redo_threads, lds_redo);
+    hipLaunchKernelGGL(k_redo, dim3((uint32_t)g.cus), dim3(redo_threads), lds_redo, stream, sc_d, reqs_d, 1u, w,
+                       s_entries, m_entries);
     if (beauty)
         hipLaunchKernelGGL(k_counts, dim3((tiles + COUNT_TILES - 1) / COUNT_TILES), dim3(256), 0, stream, reqs_d, 1u,
                            (const uint32_t *)w.counts);

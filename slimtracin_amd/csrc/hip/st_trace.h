@@ -584,73 +584,30 @@ DFN void trav_fault(const StDScene &sc, uint32_t code) {
     if (sc.fault) atomicOr(sc.fault, code);
 }
 
-/* Levels of the top of a mesh tree that the wave splits between its lanes:
-   lane L takes the subtree at depth ST_COOP_LEVELS reached by the bits of L
-   (left = 0, from the root down). */
-#define ST_COOP_LEVELS 6u
-
 /* "not written" marker for a current-hit distance: a signalling-NaN
    pattern, which no arithmetic result can have */
 #define ST_DIST_UNSET 0x7FA5A5A5u
 
-DFN float wave_min_f(float v) {
-    for (int o = 32; o > 0; o >>= 1) {
-        const float u = __shfl_xor(v, o, 64);
-        v = u < v ? u : v;
-    }
-    return v;
-}
-DFN uint32_t wave_min_u(uint32_t v) {
-    for (int o = 32; o > 0; o >>= 1) {
-        const uint32_t u = __shfl_xor(v, o, 64);
-        v = u < v ? u : v;
-    }
-    return v;
-}
-DFN uint32_t wave_max_u(uint32_t v) {
-    for (int o = 32; o > 0; o >>= 1) {
-        const uint32_t u = __shfl_xor(v, o, 64);
-        v = u > v ? u : v;
-    }
-    return v;
-}
-DFN sv3 bcast3(sv3 v, int src) { return V3(__shfl(v.x, src, 64), __shfl(v.y, src, 64), __shfl(v.z, src, 64)); }
-DFN DHit bcast_hit(const DHit &h, int src) {
-    DHit o;
-    o.position = bcast3(h.position, src);
-    o.normal = bcast3(h.normal, src);
-    o.uv = V2(__shfl(h.uv.x, src, 64), __shfl(h.uv.y, src, 64));
-    o.dUV = V2(__shfl(h.dUV.x, src, 64), __shfl(h.dUV.y, src, 64));
-    o.distance = __shfl(h.distance, src, 64);
-    o.distance_squared = __shfl(h.distance_squared, src, 64);
-    o.cone_angle = __shfl(h.cone_angle, src, 64);
-    o.cone_width = __shfl(h.cone_width, src, 64);
-    o.NdotV = __shfl(h.NdotV, src, 64);
-    o.area = __shfl(h.area, src, 64);
-    o.uv_area = __shfl(h.uv_area, src, 64);
-    o.material_id = (uint32_t)__shfl((int)h.material_id, src, 64);
-    o.object_id = (uint32_t)__shfl((int)h.object_id, src, 64);
-    o.object_type = (uint32_t)__shfl((int)h.object_type, src, 64);
-    o.from_behind = (uint32_t)__shfl((int)h.from_behind, src, 64);
-    return o;
-}
-
-/* One lane's share of a cooperative mesh traversal: the reference's
+/* One thread's unit of a cooperative mesh traversal: the reference's
    traversal loop (mesh.h:38-111, same step structure and child order as
    trace_mesh) below node pair `left`, with the box bound fixed at `bound`
    (the traversal's starting bound: it visits every box the reference could
    admit, as boxes are only pruned when the bound shrinks) and the triangle
    test as the reference makes it. `pm` is the largest entry distance of the
-   boxes on the path down to `left`; *best_pm receives that of the leaf of
+   boxes on the path down to `left`; best_pm receives that of the leaf of
    the last triangle taken. Returns 0, 1 when the stack would reach the
-   reference's limit (never for a valid tree), 2 past the step bound. */
+   reference's limit (never for a valid tree), 2 past the step bound, 3
+   when `stop` (any hit: the least key of a unit with a hit, checked every
+   32 steps) falls below this unit's `key`, so that its result is moot. */
 DFN int coop_dfs(DTrace &tr, const uint32_t *nodes, const StDTriHot *tris, uint32_t left, float bound, float pm,
-                 bool any_hit, uint32_t max_steps, bool &found, float &best_pm) {
+                 bool any_hit, uint32_t max_steps, bool &found, float &best_pm, const volatile uint32_t *stop,
+                 uint32_t key) {
     DRay &ray = tr.local;
     uint32_t *stack = tr.mesh_stack, *pstack = tr.aux_stack;
     uint32_t sp = 0, steps = 0;
     for (;;) {
         if (++steps > max_steps) return 2;
+        if (stop && (steps & 31u) == 0 && *stop < key) return 3;
         const DNodePair np = load_pair(nodes, left);
         float ld, rd;
         const bool hl = hit_aabb(np.lb, ray, bound, ld);
@@ -696,118 +653,180 @@ DFN int coop_dfs(DTrace &tr, const uint32_t *nodes, const StDTriHot *tris, uint3
     return 0;
 }
 
-/* The reference's mesh traversal of one ray (mesh.h:38-111), run by all 64
-   lanes of a wave that hold the same ray (the redo stage). The top
-   ST_COOP_LEVELS levels of the tree split it into units in the reference's
-   visiting order: lane L descends along the bits of L and takes the subtree
-   it reaches, or a leaf its path ends at (the lane whose lower bits are zero
-   owns it). Each lane runs the reference's loop over its unit with the box
-   bound fixed at the traversal's start (coop_dfs), a superset of the boxes
-   the sequential traversal admits. The units' order key is, per pair on the
-   path, where the reference takes that child: leaf children in that step
-   (left 0, right 1), then the inner child it enters first (2, the nearer one
-   for a closest hit) and last (3).
+/* Block-shared state of one cooperative mesh traversal. */
+struct DCoopShared {
+    unsigned long long best;   /* closest hit: least (distance bits << 32 | key) over the units' hits;
+                                  any hit: least key */
+    uint32_t next;             /* next unit to claim */
+    uint32_t status;           /* OR of the units' failures: 1 stack limit, 2 step bound */
+    uint32_t first_key;        /* any hit: least key of a unit with a hit so far */
+    uint32_t last_key;         /* nothing hit: greatest key + 1 of a unit with a plane hit */
+    uint32_t tri;              /* the winning unit's triangle */
+    float pm;                  /* its path maximum */
+    float last_dist;           /* distance of the last plane hit of the latest unit with one */
+};
+
+/* The reference's mesh traversal of one ray (mesh.h:38-111), run by every
+   thread of a block, all on the same ray (the redo stage). The top D levels
+   of the tree (D = log2(triangles) - 6, clamped to 6..12) split it into
+   2^D units: unit u descends along the bits of u (left = 0, from the root)
+   and is the subtree it reaches, or a leaf its path ends at (then only the
+   unit whose lower bits are zero holds it). Threads claim units from a
+   block counter and run the reference's loop over each (coop_dfs) with the
+   box bound fixed at the traversal's start, a superset of the boxes the
+   sequential traversal admits, and a triangle test that starts afresh in
+   each unit. A unit's order key is, per pair on its path, where the
+   reference takes that child: leaf children in that step (left 0, right
+   1), then the inner child it enters first (2, the nearer one for a
+   closest hit) and last (3); the reference visits the units in key order.
      any hit: the sequential traversal's bound stays fixed until its first
-       hit, so it visits exactly these units in key order: its hit is the
-       first hit of the unit with the smallest key;
+       hit, so its hit is the first hit of the unit with the least key
+       (units whose key exceeds a key already hit are skipped);
      closest hit: t* = the least hit distance; among the hits at t*, the one
-       of the smallest key is first in the reference's order. The reference
+       of the least key is first in the reference's order. The reference
        reaches it if every box on its path admits it at bound t* (its bound
        is still above t* there), i.e. every entry distance on the path is at
        most t* (the boxes were admitted at the larger starting bound), and it
        then keeps it: nothing is nearer, and ties lose to the first.
-   Nothing hit: the bound never moved, so the units are exactly what the
-   reference visits, and its current-hit distance is the last plane hit of
-   the unit latest in key order that has one.
+   The winner's hit record is recomputed from its triangle with the same
+   test. Nothing hit: the bound never moved, so the units are exactly what
+   the reference visits, and its current-hit distance (the only field of
+   that record a later result reads) is that of the last plane hit of the
+   unit latest in key order that has one.
    Returns 1 (hit in tr.closest_mesh_hit), 0 (none), or -1 when the check
-   fails and the caller traces sequentially (tr restored). */
+   fails and the caller traces sequentially (start state restored). */
 DFN int coop_trace_mesh(DTrace &tr, const StDScene &sc, const StDMesh &mesh, const uint32_t *nodes,
                         const StDTriHot *tris, uint32_t root_first, bool any_hit) {
+    __shared__ DCoopShared cs;
+    volatile DCoopShared &vs = cs;
     DRay &ray = tr.local;
     DHit &closest = tr.closest_mesh_hit;
-    const float c0 = closest.distance;
-    const DHit saved_closest = closest, saved_current = tr.current_hit;
-    const uint32_t lane = __lane_id();
-    const uint32_t D = ST_COOP_LEVELS;
-    uint32_t max_steps = 2u * mesh.triangle_count + 2u;
+    const float c0 = closest.distance, cur0 = tr.current_hit.distance;
+    const uint32_t T = mesh.triangle_count;
+    const uint32_t lg = 31u - (uint32_t)__builtin_clz(T);
+    const uint32_t D = lg < 12u ? 6u : (lg > 18u ? 12u : lg - 6u);
+    const uint32_t units = 1u << D;
+    uint32_t max_steps = 2u * T + 2u;
     if (sc.step_cap && sc.step_cap < max_steps) max_steps = sc.step_cap;
-    uint32_t left = root_first, key = 0, unit = 2u, lfirst = 0, lcount = 0;
-    float pm = 0.0f;
-    for (uint32_t d = 0; d < D; d++) {
-        const DNodePair np = load_pair(nodes, left);
-        float ld, rd;
-        const bool hl = hit_aabb(np.lb, ray, c0, ld);
-        const bool hr = hit_aabb(np.rb, ray, c0, rd);
-        const uint32_t bit = (lane >> (D - 1u - d)) & 1u;
-        const uint32_t cnt = bit ? np.rcount : np.lcount, first = bit ? np.rfirst : np.lfirst;
-        if (!(bit ? hr : hl)) {
-            unit = 0;
-            break;
-        }
-        const uint32_t first_inner = (!any_hit && ld > rd) ? 1u : 0u;
-        key = key * 4u + (cnt ? bit : (bit == first_inner ? 2u : 3u));
-        const float l = bit ? rd : ld;
-        pm = l > pm ? l : pm;
-        if (cnt) {
-            const uint32_t below = D - 1u - d;
-            unit = (lane & ((1u << below) - 1u)) == 0 ? 1u : 0u;
-            key <<= 2u * below;
-            lfirst = first;
-            lcount = cnt;
-            break;
-        }
-        left = first;
+    __syncthreads();  /* the previous traversal's readers of cs are done */
+    if (threadIdx.x == 0) {
+        cs.best = ~0ull;
+        cs.next = 0;
+        cs.status = 0;
+        cs.first_key = 0xFFFFFFFFu;
+        cs.last_key = 0;
     }
-    tr.current_hit.distance = __uint_as_float(ST_DIST_UNSET);
+    __syncthreads();
     bool found = false;
-    float best_pm = 0.0f;
-    int status = 0;
-    if (unit == 1u) {
-        if (hit_triangles(tr, tris, lfirst, lcount, 0, 0, any_hit)) {
-            found = true;
-            best_pm = pm;
+    uint32_t b_key = 0xFFFFFFFFu, b_tri = 0, l_key = 0;
+    float b_t = INFINITY, b_pm = 0.0f, l_dist = 0.0f;
+    for (;;) {
+        const uint32_t u = atomicAdd(&cs.next, 1u);
+        if (u >= units || vs.status) break;
+        uint32_t left = root_first, key = 0, kind = 2u, lfirst = 0, lcount = 0;
+        float pm = 0.0f;
+        for (uint32_t d = 0; d < D; d++) {
+            const DNodePair np = load_pair(nodes, left);
+            float ld, rd;
+            const bool hl = hit_aabb(np.lb, ray, c0, ld);
+            const bool hr = hit_aabb(np.rb, ray, c0, rd);
+            const uint32_t bit = (u >> (D - 1u - d)) & 1u;
+            const uint32_t cnt = bit ? np.rcount : np.lcount, first = bit ? np.rfirst : np.lfirst;
+            if (!(bit ? hr : hl)) {
+                kind = 0;
+                break;
+            }
+            const uint32_t first_inner = (!any_hit && ld > rd) ? 1u : 0u;
+            key = key * 4u + (cnt ? bit : (bit == first_inner ? 2u : 3u));
+            const float l = bit ? rd : ld;
+            pm = l > pm ? l : pm;
+            if (cnt) {
+                const uint32_t below = D - 1u - d;
+                kind = (u & ((1u << below) - 1u)) == 0 ? 1u : 0u;
+                key <<= 2u * below;
+                lfirst = first;
+                lcount = cnt;
+                break;
+            }
+            left = first;
         }
-    } else if (unit == 2u) {
-        status = coop_dfs(tr, nodes, tris, left, c0, pm, any_hit, max_steps, found, best_pm);
+        if (!kind || (any_hit && key > vs.first_key)) continue;
+        closest.distance = c0;
+        tr.current_hit.distance = __uint_as_float(ST_DIST_UNSET);
+        bool f = false;
+        float f_pm = 0.0f;
+        int st = 0;
+        if (kind == 1u) {
+            if (hit_triangles(tr, tris, lfirst, lcount, 0, 0, any_hit)) {
+                f = true;
+                f_pm = pm;
+            }
+        } else {
+            st = coop_dfs(tr, nodes, tris, left, c0, pm, any_hit, max_steps, f, f_pm,
+                          any_hit ? &vs.first_key : nullptr, key);
+        }
+        if (st == 1 || st == 2) {
+            atomicOr(&cs.status, (uint32_t)st);
+            break;
+        }
+        if (f) {
+            const float t = closest.distance;
+            if (any_hit) {
+                if (key < b_key) {
+                    b_key = key;
+                    b_tri = closest.object_id;
+                }
+                atomicMin(&cs.first_key, key);
+            } else if (t < b_t || (t == b_t && key < b_key)) {
+                b_t = t;
+                b_key = key;
+                b_tri = closest.object_id;
+                b_pm = f_pm;
+            }
+            found = true;
+        } else if (st == 0) {
+            const float cd = tr.current_hit.distance;
+            if (__float_as_uint(cd) != ST_DIST_UNSET && key + 1u > l_key) {
+                l_key = key + 1u;
+                l_dist = cd;
+            }
+        }
     }
-    if (__ballot(status != 0)) {
-        closest = saved_closest;
-        tr.current_hit = saved_current;
-        if (__ballot(status == 2)) {
+    __syncthreads();
+    closest.distance = c0;
+    tr.current_hit.distance = cur0;
+    const uint32_t status = vs.status;
+    if (status) {
+        if (status & 2u) {
             trav_fault(sc, ST_FAULT_MESH_STEPS);
             return 0;
         }
         return -1;
     }
-    if (__ballot(found)) {
-        int win;
-        if (any_hit) {
-            const uint32_t k = wave_min_u(found ? key : 0xFFFFFFFFu);
-            win = __builtin_ctzll(__ballot(found && key == k));
-        } else {
-            const float t = wave_min_f(found ? closest.distance : INFINITY);
-            const bool cand = found && closest.distance == t;
-            const uint32_t k = wave_min_u(cand ? key : 0xFFFFFFFFu);
-            win = __builtin_ctzll(__ballot(cand && key == k));
-            if (!(__shfl(best_pm, win, 64) <= t)) {
-                closest = saved_closest;
-                tr.current_hit = saved_current;
-                return -1;
-            }
-        }
-        closest = bcast_hit(closest, win);
-        tr.current_hit = bcast_hit(tr.current_hit, win);
-        return 1;
+    const unsigned long long mine =
+        any_hit ? (unsigned long long)b_key : ((unsigned long long)__float_as_uint(b_t) << 32) | b_key;
+    if (found) atomicMin(&cs.best, mine);
+    if (l_key) atomicMax(&cs.last_key, l_key);
+    __syncthreads();
+    const unsigned long long best = vs.best;
+    const uint32_t lk = vs.last_key;
+    if (found && mine == best) {
+        cs.tri = b_tri;
+        cs.pm = b_pm;
+    } else if (best == ~0ull && lk && l_key == lk) {
+        cs.last_dist = l_dist;
     }
-    closest = saved_closest;
-    const bool plane = __float_as_uint(tr.current_hit.distance) != ST_DIST_UNSET;
-    if (__ballot(plane)) {
-        const uint32_t k = wave_max_u(plane ? key + 1u : 0u);
-        const int win = __builtin_ctzll(__ballot(plane && key + 1u == k));
-        tr.current_hit = bcast_hit(tr.current_hit, win);
-    } else {
-        tr.current_hit = saved_current;
+    __syncthreads();
+    if (best != ~0ull) {
+        const uint32_t tri = vs.tri;
+        const float pm = vs.pm;
+        if (!any_hit && !(pm <= __uint_as_float((uint32_t)(best >> 32)))) return -1;
+        if (hit_triangles(tr, tris, tri, 1u, 0, 0, any_hit)) return 1;
+        closest.distance = c0;
+        tr.current_hit.distance = cur0;
+        return -1;
     }
+    if (lk) tr.current_hit.distance = vs.last_dist;
     return 0;
 }
 
@@ -815,9 +834,9 @@ DFN int coop_trace_mesh(DTrace &tr, const StDScene &sc, const StDMesh &mesh, con
    each step tests both children's boxes, then the triangles of the children
    that are leaves (left first), then descends. The leaf-root case runs
    through the same loop body (one step, no children) so that the triangle
-   test and the traversal are emitted once. COOP (the redo stage, every lane
-   of the wave on the same ray): large meshes are traversed by the whole
-   wave (coop_trace_mesh), with the same result. */
+   test and the traversal are emitted once. COOP (the redo stage, every
+   thread of the block on the same ray): large meshes are traversed by the
+   whole block (coop_trace_mesh), with the same result. */
 template <bool COOP = false>
 DNOINLINE bool trace_mesh(DTrace &tr, const StDScene &sc, uint32_t mesh_id, bool any_hit) {
     const StDMesh mesh = load_mesh(sc, mesh_id);

@@ -1,8 +1,8 @@
 """GPU: the redo stage. A mesh traversal of the main stages that runs past
 its step limit gives up, and the redo stage traces its pixel again from the
-primary ray with all 64 lanes of a wave, splitting each traversal of a large
-mesh across the lanes (coop_trace_mesh: the reference's traversal order
-rebuilt from the units the lanes take, st_trace.h). Frames must not depend
+primary ray with every thread of a block, splitting each traversal of a
+large mesh across the threads (coop_trace_mesh: the reference's traversal
+order rebuilt from the units the threads claim, st_trace.h). Frames must not depend
 on it: with a give-up limit of one step and a cooperative-mesh minimum of 16
 triangles (st_debug_redo), nearly every pixel goes through the redo stage and
 every mesh traversal in it through the cooperative path, and frames, hit
