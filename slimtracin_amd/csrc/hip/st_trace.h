@@ -534,45 +534,88 @@ DFN bool hit_tetrahedron(DHit &closest_hit, sv3 Ro, sv3 Rd, uint32_t flags) {
     return found;
 }
 
-/* intersection/mesh.h:5-37 over the 64-byte hot triangle records of the
-   leaves one traversal step reached: the left leaf's range [f0, f0 + c0),
-   then the right leaf's [f1, f1 + c1), the order in which the reference
-   calls its per-leaf test. One loop covers both so that the test is emitted
-   once per kernel; object_id is the triangle's leaf-order index. */
-DFN bool hit_triangles(DTrace &tr, const StDTriHot *tris, uint32_t f0, uint32_t c0, uint32_t f1, uint32_t c1,
-                       bool any_hit) {
+/* intersection/mesh.h:5-37 for one 64-byte hot triangle record a, b, c, d
+   (leaf-order index i): the plane hit into tr.current_hit, then, when it is
+   nearer than the closest mesh hit and inside the triangle, the closest
+   mesh hit. True when taken. */
+DFN bool tri_test(DTrace &tr, float4 a, float4 b, float4 c, float4 d, uint32_t i) {
     DRay &ray = tr.local;
     DHit &hit = tr.current_hit;
     DHit &closest = tr.closest_mesh_hit;
+    sv3 tpos = V3(c.y, c.z, c.w);
+    sv3 tnrm = V3(d.x, d.y, d.z);
+    if (!(hit_plane(tpos, tnrm, ray.origin, ray.direction, hit) && hit.distance < closest.distance)) return false;
+    sm3 w;
+    w.X = V3(a.x, a.y, a.z);
+    w.Y = V3(a.w, b.x, b.y);
+    w.Z = V3(b.z, b.w, c.x);
+    sv3 UV = v_sub(hit.position, tpos);
+    UV = v_mat(UV, w);
+    if (UV.x < 0 || UV.y < 0 || (UV.x + UV.y) > 1) return false;
+    closest.NdotV = hit.NdotV;
+    closest.normal = hit.normal;
+    closest.uv.x = UV.x;
+    closest.uv.y = UV.y;
+    closest.object_id = i;
+    closest.from_behind = hit.from_behind;
+    closest.distance = hit.distance;
+    closest.position = hit.position;
+    closest.area = 0;       /* areas are fetched from the cold record once the hit is final */
+    closest.uv_area = 0;
+    return true;
+}
+
+/* The triangles of the leaves one traversal step reached: the left leaf's
+   range [f0, f0 + c0), then the right leaf's [f1, f1 + c1), the order in
+   which the reference calls its per-leaf test (mesh.h:5-37). One loop covers
+   both so that the test is emitted once per kernel; object_id is the
+   triangle's leaf-order index. */
+DFN bool hit_triangles(DTrace &tr, const StDTriHot *tris, uint32_t f0, uint32_t c0, uint32_t f1, uint32_t c1,
+                       bool any_hit) {
     bool found = false;
     const uint32_t n = c0 + c1;
 #pragma clang loop unroll(disable)
     for (uint32_t j = 0; j < n; j++) {
         const uint32_t i = j < c0 ? f0 + j : f1 + (j - c0);
         const void *p = tris + i;
-        float4 a = ldg4(p, 0), b = ldg4(p, 1), c = ldg4(p, 2), d = ldg4(p, 3);
-        sv3 tpos = V3(c.y, c.z, c.w);
-        sv3 tnrm = V3(d.x, d.y, d.z);
-        if (hit_plane(tpos, tnrm, ray.origin, ray.direction, hit) && hit.distance < closest.distance) {
-            sm3 w;
-            w.X = V3(a.x, a.y, a.z);
-            w.Y = V3(a.w, b.x, b.y);
-            w.Z = V3(b.z, b.w, c.x);
-            sv3 UV = v_sub(hit.position, tpos);
-            UV = v_mat(UV, w);
-            if (UV.x < 0 || UV.y < 0 || (UV.x + UV.y) > 1) continue;
-            closest.NdotV = hit.NdotV;
-            closest.normal = hit.normal;
-            closest.uv.x = UV.x;
-            closest.uv.y = UV.y;
-            closest.object_id = i;
-            closest.from_behind = hit.from_behind;
-            closest.distance = hit.distance;
-            closest.position = hit.position;
-            closest.area = 0;       /* areas are fetched from the cold record once the hit is final */
-            closest.uv_area = 0;
+        if (tri_test(tr, ldg4(p, 0), ldg4(p, 1), ldg4(p, 2), ldg4(p, 3), i)) {
             found = true;
             if (any_hit) break;
+        }
+    }
+    return found;
+}
+
+/* hit_triangles with the records of up to four triangles loaded together
+   before they are tested in order (the cooperative traversal, where a
+   thread's memory latency is not hidden by other waves). */
+#define ST_TRI_BATCH 4
+DFN bool hit_triangles_batched(DTrace &tr, const StDTriHot *tris, uint32_t f0, uint32_t c0, uint32_t f1, uint32_t c1,
+                               bool any_hit) {
+    bool found = false;
+    const uint32_t n = c0 + c1;
+    for (uint32_t j = 0; j < n; j += ST_TRI_BATCH) {
+        float4 r[ST_TRI_BATCH][4];
+        uint32_t id[ST_TRI_BATCH];
+#pragma unroll
+        for (int k = 0; k < ST_TRI_BATCH; k++) {
+            const uint32_t jj = j + (uint32_t)k;
+            id[k] = jj < c0 ? f0 + jj : f1 + (jj - c0);
+            if (jj < n) {
+                const void *p = tris + id[k];
+                r[k][0] = ldg4(p, 0);
+                r[k][1] = ldg4(p, 1);
+                r[k][2] = ldg4(p, 2);
+                r[k][3] = ldg4(p, 3);
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < ST_TRI_BATCH; k++) {
+            if (j + (uint32_t)k >= n) break;
+            if (tri_test(tr, r[k][0], r[k][1], r[k][2], r[k][3], id[k])) {
+                found = true;
+                if (any_hit) return true;
+            }
         }
     }
     return found;
@@ -587,71 +630,6 @@ DFN void trav_fault(const StDScene &sc, uint32_t code) {
 /* "not written" marker for a current-hit distance: a signalling-NaN
    pattern, which no arithmetic result can have */
 #define ST_DIST_UNSET 0x7FA5A5A5u
-
-/* One thread's unit of a cooperative mesh traversal: the reference's
-   traversal loop (mesh.h:38-111, same step structure and child order as
-   trace_mesh) below node pair `left`, with the box bound fixed at `bound`
-   (the traversal's starting bound: it visits every box the reference could
-   admit, as boxes are only pruned when the bound shrinks) and the triangle
-   test as the reference makes it. `pm` is the largest entry distance of the
-   boxes on the path down to `left`; best_pm receives that of the leaf of
-   the last triangle taken. Returns 0, 1 when the stack would reach the
-   reference's limit (never for a valid tree), 2 past the step bound, 3
-   when `stop` (any hit: the least key of a unit with a hit, checked every
-   32 steps) falls below this unit's `key`, so that its result is moot. */
-DFN int coop_dfs(DTrace &tr, const uint32_t *nodes, const StDTriHot *tris, uint32_t left, float bound, float pm,
-                 bool any_hit, uint32_t max_steps, bool &found, float &best_pm, const volatile uint32_t *stop,
-                 uint32_t key) {
-    DRay &ray = tr.local;
-    uint32_t *stack = tr.mesh_stack, *pstack = tr.aux_stack;
-    uint32_t sp = 0, steps = 0;
-    for (;;) {
-        if (++steps > max_steps) return 2;
-        if (stop && (steps & 31u) == 0 && *stop < key) return 3;
-        const DNodePair np = load_pair(nodes, left);
-        float ld, rd;
-        const bool hl = hit_aabb(np.lb, ray, bound, ld);
-        const bool hr = hit_aabb(np.rb, ray, bound, rd);
-        const float pml = ld > pm ? ld : pm, pmr = rd > pm ? rd : pm;
-        if (hl && np.lcount && hit_triangles(tr, tris, np.lfirst, np.lcount, 0, 0, any_hit)) {
-            found = true;
-            best_pm = pml;
-            if (any_hit) return 0;
-        }
-        if (hr && np.rcount && hit_triangles(tr, tris, np.rfirst, np.rcount, 0, 0, any_hit)) {
-            found = true;
-            best_pm = pmr;
-            if (any_hit) return 0;
-        }
-        const bool il = hl && !np.lcount, ir = hr && !np.rcount;
-        if (il && ir) {
-            uint32_t a = np.lfirst, b = np.rfirst;
-            float pa = pml, pb = pmr;
-            if (!any_hit && ld > rd) {
-                a = np.rfirst; b = np.lfirst;
-                pa = pmr; pb = pml;
-            }
-            if (sp + 1u >= tr.mesh_stack_size) return 1;
-            stack[ST_STACK_STRIDE * sp] = b;
-            pstack[ST_STACK_STRIDE * sp] = __float_as_uint(pb);
-            sp++;
-            left = a;
-            pm = pa;
-        } else if (il) {
-            left = np.lfirst;
-            pm = pml;
-        } else if (ir) {
-            left = np.rfirst;
-            pm = pmr;
-        } else {
-            if (sp == 0) break;
-            sp--;
-            left = stack[ST_STACK_STRIDE * sp];
-            pm = __uint_as_float(pstack[ST_STACK_STRIDE * sp]);
-        }
-    }
-    return 0;
-}
 
 /* Block-shared state of one cooperative mesh traversal. */
 struct DCoopShared {
@@ -668,20 +646,29 @@ struct DCoopShared {
 
 /* The reference's mesh traversal of one ray (mesh.h:38-111), run by every
    thread of a block, all on the same ray (the redo stage). The top D levels
-   of the tree (D = log2(triangles) - 6, clamped to 6..12) split it into
+   of the tree (D = log2(triangles) - 5, clamped to 6..15) split it into
    2^D units: unit u descends along the bits of u (left = 0, from the root)
    and is the subtree it reaches, or a leaf its path ends at (then only the
    unit whose lower bits are zero holds it). Threads claim units from a
-   block counter and run the reference's loop over each (coop_dfs) with the
-   box bound fixed at the traversal's start, a superset of the boxes the
-   sequential traversal admits, and a triangle test that starts afresh in
-   each unit. A unit's order key is, per pair on its path, where the
-   reference takes that child: leaf children in that step (left 0, right
-   1), then the inner child it enters first (2, the nearer one for a
-   closest hit) and last (3); the reference visits the units in key order.
+   block counter, which skips the rest of a range of units found empty (a
+   box on their common path not admitted, or the leaf their path ends at),
+   and run the reference's loop over each unit's subtree (same step
+   structure and child order as trace_mesh) with the box bound fixed at the
+   traversal's start, a superset of the boxes the sequential traversal
+   admits, and a triangle test that starts afresh in each unit.
+     Each loop iteration handles one node pair per thread, of a descent to a
+   unit or of a unit's traversal, and a thread that ends its unit claims the
+   next one in the same loop, so the threads of a wave do not wait for each
+   other's units. With the bound fixed, a step's successor follows from its
+   box tests alone: its node pair is loaded before the step's triangles are
+   tested, and those are loaded together (hit_triangles_batched).
+     A unit's order key is, per pair on its path, where the reference takes
+   that child: leaf children in that step (left 0, right 1), then the inner
+   child it enters first (2, the nearer one for a closest hit) and last (3);
+   the reference visits the units in key order.
      any hit: the sequential traversal's bound stays fixed until its first
        hit, so its hit is the first hit of the unit with the least key
-       (units whose key exceeds a key already hit are skipped);
+       (units whose key exceeds a key already hit are skipped or left);
      closest hit: t* = the least hit distance; among the hits at t*, the one
        of the least key is first in the reference's order. The reference
        reaches it if every box on its path admits it at bound t* (its bound
@@ -694,17 +681,19 @@ struct DCoopShared {
    that record a later result reads) is that of the last plane hit of the
    unit latest in key order that has one.
    Returns 1 (hit in tr.closest_mesh_hit), 0 (none), or -1 when the check
-   fails and the caller traces sequentially (start state restored). */
+   fails, or a unit's stack would reach the reference's limit (never for a
+   valid tree), and the caller traces sequentially (start state restored). */
 DFN int coop_trace_mesh(DTrace &tr, const StDScene &sc, const StDMesh &mesh, const uint32_t *nodes,
                         const StDTriHot *tris, uint32_t root_first, bool any_hit) {
     __shared__ DCoopShared cs;
     volatile DCoopShared &vs = cs;
     DRay &ray = tr.local;
     DHit &closest = tr.closest_mesh_hit;
+    uint32_t *stack = tr.mesh_stack, *pstack = tr.aux_stack;
     const float c0 = closest.distance, cur0 = tr.current_hit.distance;
     const uint32_t T = mesh.triangle_count;
     const uint32_t lg = 31u - (uint32_t)__builtin_clz(T);
-    const uint32_t D = lg < 12u ? 6u : (lg > 18u ? 12u : lg - 6u);
+    const uint32_t D = lg < 11u ? 6u : (lg > 20u ? 15u : lg - 5u);
     const uint32_t units = 1u << D;
     uint32_t max_steps = 2u * T + 2u;
     if (sc.step_cap && sc.step_cap < max_steps) max_steps = sc.step_cap;
@@ -717,79 +706,148 @@ DFN int coop_trace_mesh(DTrace &tr, const StDScene &sc, const StDMesh &mesh, con
         cs.last_key = 0;
     }
     __syncthreads();
+    /* this thread's best unit result */
     bool found = false;
     uint32_t b_key = 0xFFFFFFFFu, b_tri = 0, l_key = 0;
     float b_t = INFINITY, b_pm = 0.0f, l_dist = 0.0f;
+    /* its current unit: claiming (0), descending to it (1), traversing it (2) */
+    enum { CLAIM, DESCEND, TRAVERSE };
+    uint32_t phase = CLAIM, u = 0, d = 0, key = 0, left = 0, sp = 0, steps = 0;
+    float pm = 0.0f, u_pm = 0.0f;
+    bool u_found = false;
+    DNodePair np = {};
     for (;;) {
-        const uint32_t u = atomicAdd(&cs.next, 1u);
-        if (u >= units || vs.status) break;
-        uint32_t left = root_first, key = 0, kind = 2u, lfirst = 0, lcount = 0;
-        float pm = 0.0f;
-        for (uint32_t d = 0; d < D; d++) {
-            const DNodePair np = load_pair(nodes, left);
-            float ld, rd;
-            const bool hl = hit_aabb(np.lb, ray, c0, ld);
-            const bool hr = hit_aabb(np.rb, ray, c0, rd);
-            const uint32_t bit = (u >> (D - 1u - d)) & 1u;
+        if (phase == CLAIM) {
+            u = atomicAdd(&cs.next, 1u);
+            if (u >= units || vs.status) break;
+            phase = DESCEND;
+            d = key = 0;
+            pm = 0.0f;
+            left = root_first;
+            np = load_pair(nodes, left);
+        }
+        float ld, rd;
+        const bool hl = hit_aabb(np.lb, ray, c0, ld);
+        const bool hr = hit_aabb(np.rb, ray, c0, rd);
+        const float pml = ld > pm ? ld : pm, pmr = rd > pm ? rd : pm;
+        uint32_t f0 = 0, n0 = 0, f1 = 0, n1 = 0;
+        bool next = false, unit_end = false;
+        if (phase == DESCEND) {
+            const uint32_t below = D - 1u - d;
+            const uint32_t bit = (u >> below) & 1u;
             const uint32_t cnt = bit ? np.rcount : np.lcount, first = bit ? np.rfirst : np.lfirst;
             if (!(bit ? hr : hl)) {
-                kind = 0;
-                break;
+                /* every unit below this box is empty */
+                atomicMax(&cs.next, ((u >> below) + 1u) << below);
+                phase = CLAIM;
+                continue;
             }
             const uint32_t first_inner = (!any_hit && ld > rd) ? 1u : 0u;
             key = key * 4u + (cnt ? bit : (bit == first_inner ? 2u : 3u));
-            const float l = bit ? rd : ld;
-            pm = l > pm ? l : pm;
+            pm = bit ? pmr : pml;
             if (cnt) {
-                const uint32_t below = D - 1u - d;
-                kind = (u & ((1u << below) - 1u)) == 0 ? 1u : 0u;
+                /* the leaf belongs to the first unit of its range */
+                atomicMax(&cs.next, ((u >> below) + 1u) << below);
+                if (u & ((1u << below) - 1u)) {
+                    phase = CLAIM;
+                    continue;
+                }
                 key <<= 2u * below;
-                lfirst = first;
-                lcount = cnt;
-                break;
-            }
-            left = first;
-        }
-        if (!kind || (any_hit && key > vs.first_key)) continue;
-        closest.distance = c0;
-        tr.current_hit.distance = __uint_as_float(ST_DIST_UNSET);
-        bool f = false;
-        float f_pm = 0.0f;
-        int st = 0;
-        if (kind == 1u) {
-            if (hit_triangles(tr, tris, lfirst, lcount, 0, 0, any_hit)) {
-                f = true;
-                f_pm = pm;
+                closest.distance = c0;
+                tr.current_hit.distance = __uint_as_float(ST_DIST_UNSET);
+                u_found = false;
+                f0 = first;
+                n0 = cnt;
+                unit_end = true;
+            } else {
+                left = first;
+                next = true;
+                if (++d == D) {
+                    phase = TRAVERSE;
+                    sp = steps = 0;
+                    closest.distance = c0;
+                    tr.current_hit.distance = __uint_as_float(ST_DIST_UNSET);
+                    u_found = false;
+                }
             }
         } else {
-            st = coop_dfs(tr, nodes, tris, left, c0, pm, any_hit, max_steps, f, f_pm,
-                          any_hit ? &vs.first_key : nullptr, key);
+            if (++steps > max_steps) {
+                atomicOr(&cs.status, 2u);
+                break;
+            }
+            if (any_hit && (steps & 31u) == 0 && vs.first_key < key) {
+                /* a unit earlier in key order has a hit: this one is moot */
+                phase = CLAIM;
+                continue;
+            }
+            if (hl && np.lcount) { f0 = np.lfirst; n0 = np.lcount; }
+            if (hr && np.rcount) { f1 = np.rfirst; n1 = np.rcount; }
+            const bool il = hl && !np.lcount, ir = hr && !np.rcount;
+            if (il && ir) {
+                uint32_t a = np.lfirst, b = np.rfirst;
+                float pa = pml, pb = pmr;
+                if (!any_hit && ld > rd) {
+                    a = np.rfirst; b = np.lfirst;
+                    pa = pmr; pb = pml;
+                }
+                if (sp + 1u >= tr.mesh_stack_size) {
+                    atomicOr(&cs.status, 1u);
+                    break;
+                }
+                stack[ST_STACK_STRIDE * sp] = b;
+                pstack[ST_STACK_STRIDE * sp] = __float_as_uint(pb);
+                sp++;
+                left = a;
+                pm = pa;
+                next = true;
+            } else if (il) {
+                left = np.lfirst;
+                pm = pml;
+                next = true;
+            } else if (ir) {
+                left = np.rfirst;
+                pm = pmr;
+                next = true;
+            } else if (sp) {
+                sp--;
+                left = stack[ST_STACK_STRIDE * sp];
+                pm = __uint_as_float(pstack[ST_STACK_STRIDE * sp]);
+                next = true;
+            } else {
+                unit_end = true;
+            }
         }
-        if (st == 1 || st == 2) {
-            atomicOr(&cs.status, (uint32_t)st);
-            break;
+        /* the successor's pair is in flight while this step's triangles are tested */
+        if (next) np = load_pair(nodes, left);
+        if (n0 + n1 && hit_triangles_batched(tr, tris, f0, n0, f1, n1, any_hit)) {
+            u_found = true;
+            u_pm = phase == DESCEND ? pm : (closest.object_id - f0 < n0 ? pml : pmr);
+            if (any_hit) unit_end = true;
         }
-        if (f) {
-            const float t = closest.distance;
-            if (any_hit) {
-                if (key < b_key) {
+        if (unit_end) {
+            if (u_found) {
+                const float t = closest.distance;
+                if (any_hit) {
+                    if (key < b_key) {
+                        b_key = key;
+                        b_tri = closest.object_id;
+                    }
+                    atomicMin(&cs.first_key, key);
+                } else if (t < b_t || (t == b_t && key < b_key)) {
+                    b_t = t;
                     b_key = key;
                     b_tri = closest.object_id;
+                    b_pm = u_pm;
                 }
-                atomicMin(&cs.first_key, key);
-            } else if (t < b_t || (t == b_t && key < b_key)) {
-                b_t = t;
-                b_key = key;
-                b_tri = closest.object_id;
-                b_pm = f_pm;
+                found = true;
+            } else {
+                const float cd = tr.current_hit.distance;
+                if (__float_as_uint(cd) != ST_DIST_UNSET && key + 1u > l_key) {
+                    l_key = key + 1u;
+                    l_dist = cd;
+                }
             }
-            found = true;
-        } else if (st == 0) {
-            const float cd = tr.current_hit.distance;
-            if (__float_as_uint(cd) != ST_DIST_UNSET && key + 1u > l_key) {
-                l_key = key + 1u;
-                l_dist = cd;
-            }
+            phase = CLAIM;
         }
     }
     __syncthreads();
