@@ -147,31 +147,10 @@ def pct(a, q):
 
 # ---------------------------------------------------------------- headline
 
-def c4_leg(args, blobs, views, sids):
-    t0 = time.perf_counter()
-    dscenes = [st.DeviceScene(b, mesh_lbvh=True) for b in blobs]
-    build_s = time.perf_counter() - t0
-    rs = [dscenes[i] for i in sids]
-    st.render_timed(rs, views)                      # grows the launch arena to the whole batch
-    tune, _ = st.autotune(rs, views, reps=1)
-    log(f"c4: {len(dscenes)} scenes built in {build_s:.1f} s; autotune {tune.as_dict()}")
-    for _ in range(args.warmup):
-        st.render_timed(rs, views, stages=True)
-    step_ms, stage_runs, rays = [], [], 0
-    t_start = time.perf_counter()
-    for _ in range(args.steps):
-        f0 = time.perf_counter()
-        _, counts, sms = st.render_timed(rs, views, stages=True)
-        step_ms.append((time.perf_counter() - f0) * 1e3)
-        stage_runs.append(sms)
-        rays += sum(c.total() for c in counts)
-    elapsed = time.perf_counter() - t_start
-    redone = st.last_redo_count()
-    rays_per_step = rays // args.steps
-    names = st.stage_names(max(int(v) for v in views["depth"]))
-    stage_ms = {n: round(float(np.mean([r[i] for r in stage_runs])), 4) for i, n in enumerate(names)}
-    log(f"c4: {rays_per_step} rays per step, {np.median(step_ms):.2f} ms per step, stages {stage_ms}")
-
+def plain_and_download_legs(rs, views, rays_per_step):
+    """The headline requests as plain launches (one request per launch, the
+    same kernels, frames left in HBM), and both ways with every frame
+    downloaded into page-locked host memory."""
     # plain: one launch per request, the same kernels, frames left in HBM
     lat_plain, plain_rays = [], 0
     for i in range(len(views)):                     # untimed pass: per-request arenas
@@ -202,7 +181,67 @@ def c4_leg(args, blobs, views, sids):
     same = all(a.tobytes() == b.tobytes() for a, b in zip(fp_plain, fp_batch))
     del fp_plain, fp_batch
 
+    return {"plain_s": plain_s, "plain_rays": plain_rays, "lat_plain": lat_plain, "plain_dl_s": plain_dl_s,
+            "batch_dl_s": batch_dl_s, "same": same, "grid": grid}
+
+
+def c4_leg(args, blobs, views, sids):
+    t0 = time.perf_counter()
+    dscenes = [st.DeviceScene(b, mesh_lbvh=True) for b in blobs]
+    build_s = time.perf_counter() - t0
+    rs = [dscenes[i] for i in sids]
+    st.render_timed(rs, views)                      # grows the launch arena to the whole batch
+    tune, _ = st.autotune(rs, views, reps=1)
+    log(f"c4: {len(dscenes)} scenes built in {build_s:.1f} s; autotune {tune.as_dict()}")
+    for _ in range(args.warmup):
+        st.render_timed(rs, views, stages=True)
+    step_ms, stage_runs, rays = [], [], 0
+    t_start = time.perf_counter()
+    for _ in range(args.steps):
+        f0 = time.perf_counter()
+        _, counts, sms = st.render_timed(rs, views, stages=True)
+        step_ms.append((time.perf_counter() - f0) * 1e3)
+        stage_runs.append(sms)
+        rays += sum(c.total() for c in counts)
+    elapsed = time.perf_counter() - t_start
+    redone = st.last_redo_count()
+    rays_per_step = rays // args.steps
+    names = st.stage_names(max(int(v) for v in views["depth"]))
+    stage_ms = {n: round(float(np.mean([r[i] for r in stage_runs])), 4) for i, n in enumerate(names)}
+    log(f"c4: {rays_per_step} rays per step, {np.median(step_ms):.2f} ms per step, stages {stage_ms}")
+
+    plain = None
+    if not args.batched_only:
+        plain = plain_and_download_legs(rs, views, rays_per_step)
+
     batched_mrays = rays / elapsed / 1e6
+    grid = [(int(v["width"]) + 1) * (int(v["height"]) + 1) for v in views]
+    detail = {
+        "requests": len(views), "scenes": len(dscenes),
+        "scene_triangles": [d.info().triangle_count for d in dscenes],
+        "mesh_bvh": "device Morton LBVH (leaves <= 4)", "scene_build_s": round(build_s, 2),
+        "traced_pixels": int(sum(grid)), "rays": int(rays_per_step),
+        "batched_mrays": round(batched_mrays, 2),
+        "batched_ms_per_step": round(elapsed / args.steps * 1e3, 3),
+        "batched_latency_p50_ms": pct(step_ms, 50), "batched_latency_p99_ms": pct(step_ms, 99),
+        "redo_pixels": redone,
+        "launch_shape": tune.as_dict(),
+    }
+    if plain is not None:
+        detail.update({
+            "plain_mrays": round(plain["plain_rays"] / plain["plain_s"] / 1e6, 2),
+            "speedup_vs_plain": round(plain["plain_s"] / (elapsed / args.steps), 3),
+            "plain_wall_ms": round(plain["plain_s"] * 1e3, 2),
+            "plain_request_latency_p50_ms": pct(plain["lat_plain"], 50),
+            "plain_request_latency_p99_ms": pct(plain["lat_plain"], 99),
+            "with_downloads": {
+                "output": "FloatPixel grids into page-locked host memory, both legs",
+                "plain_mrays": round(plain["plain_rays"] / plain["plain_dl_s"] / 1e6, 2),
+                "batched_mrays": round(rays_per_step / plain["batch_dl_s"] / 1e6, 2),
+                "speedup_vs_plain": round(plain["plain_dl_s"] / plain["batch_dl_s"], 3),
+            },
+            "batched_equals_plain": plain["same"],
+        })
     out = {
         "value": batched_mrays,
         "elapsed": elapsed,
@@ -210,29 +249,7 @@ def c4_leg(args, blobs, views, sids):
         "rays_per_step": rays_per_step,
         "stage_ms": stage_ms,
         "tune": tune,
-        "detail": {
-            "requests": len(views), "scenes": len(dscenes),
-            "scene_triangles": [d.info().triangle_count for d in dscenes],
-            "mesh_bvh": "device Morton LBVH (leaves <= 4)", "scene_build_s": round(build_s, 2),
-            "traced_pixels": int(sum(grid)), "rays": int(rays_per_step),
-            "batched_mrays": round(batched_mrays, 2),
-            "plain_mrays": round(plain_rays / plain_s / 1e6, 2),
-            "speedup_vs_plain": round(plain_s / (elapsed / args.steps), 3),
-            "batched_ms_per_step": round(elapsed / args.steps * 1e3, 3),
-            "plain_wall_ms": round(plain_s * 1e3, 2),
-            "plain_request_latency_p50_ms": pct(lat_plain, 50),
-            "plain_request_latency_p99_ms": pct(lat_plain, 99),
-            "batched_latency_p50_ms": pct(step_ms, 50), "batched_latency_p99_ms": pct(step_ms, 99),
-            "with_downloads": {
-                "output": "FloatPixel grids into page-locked host memory, both legs",
-                "plain_mrays": round(plain_rays / plain_dl_s / 1e6, 2),
-                "batched_mrays": round(rays_per_step / batch_dl_s / 1e6, 2),
-                "speedup_vs_plain": round(plain_dl_s / batch_dl_s, 3),
-            },
-            "batched_equals_plain": same,
-            "redo_pixels": redone,
-            "launch_shape": tune.as_dict(),
-        },
+        "detail": detail,
     }
     out["_scenes"] = dscenes
     out["_rs"] = rs
@@ -634,6 +651,8 @@ def main():
                     help="comma list of " + ",".join(LEGS) + " (c4 is the headline; cpu adds the CPU baselines "
                          "and the oracle parity of the headline)")
     ap.add_argument("--copy-probe", action="store_true", help="only run the copy kernel used to calibrate PMC bytes")
+    ap.add_argument("--batched-only", action="store_true",
+                    help="headline leg without its plain-launch and download comparisons (for profiling runs)")
     ap.add_argument("--instanced-frames", type=int, default=10, help="instanced leg: timed frames")
     ap.add_argument("--serve-rate", type=float, default=0.0,
                     help="served leg: Poisson arrival rate, requests/s (0: half the measured burst capacity)")
