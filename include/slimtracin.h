@@ -138,12 +138,15 @@ typedef struct StTuning {
     uint32_t blocks_per_cu;  /* persistent grid: blocks per compute unit (4 wavefronts each); one wavefront
                                 per tile: at most this many blocks resident per compute unit (4 or more: no cap) */
     uint32_t persistent;     /* 1: persistent tile queue (st_render); 0: one wavefront per tile */
+    uint32_t cost_order;     /* 1: tiles claimed slowest first, from their times in the previous launch of the
+                                same shape on the same stream; 0: in tile order (request by request) */
 } StTuning;
 
 int st_set_tuning(const StTuning *tuning);
 int st_get_tuning(StTuning *tuning);
-/* Autotuner: times each candidate shape on these requests (median of reps
-   render-kernel times) and keeps the fastest; returns it and its time. */
+/* Autotuner: times each candidate shape and claim order on these requests
+   (median of reps render-kernel times) and keeps the fastest; returns it and
+   its time. */
 int st_autotune(StSceneHandle *const *scenes, const StView *views, uint32_t n, uint32_t reps, StTuning *best,
                 float *best_ms);
 

@@ -965,6 +965,7 @@ extern "C" int st_init(int device) {
     g.tune.tile_shift = 3;          /* 8 x 8 tiles */
     g.tune.blocks_per_cu = 8;
     g.tune.persistent = 1;
+    g.tune.cost_order = 1;
     hipDeviceProp_t prop;
     g.cus = 256;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
@@ -1544,8 +1545,8 @@ int st_rt_launch_ctx(LaunchCtx *c, StSceneHandle *const *scenes, StHostView *hv,
         c->tile_cap = tiles;
     }
     /* the previous launch on this context had this shape: order the tiles
-       by its tile times */
-    const uint32_t use_history = c->tile_sig == tile_sig ? 1u : 0u;
+       by its tile times (when the tuning asks for it) */
+    const uint32_t use_history = (c->tile_sig == tile_sig && g.tune.cost_order) ? 1u : 0u;
     char *st = c->staging;
     char *blk = (char *)arena_take(&c->work, up);
     KReq *reqs_h = (KReq *)st;
@@ -2218,7 +2219,8 @@ extern "C" void st_host_free(void *p) {
 /* ---- launch tuning ------------------------------------------------------ */
 
 static int tuning_valid(const StTuning *t) {
-    return t->tile_shift <= 6 && t->blocks_per_cu >= 1 && t->blocks_per_cu <= 64 && t->persistent <= 1;
+    return t->tile_shift <= 6 && t->blocks_per_cu >= 1 && t->blocks_per_cu <= 64 && t->persistent <= 1 &&
+           t->cost_order <= 1;
 }
 
 extern "C" int st_set_tuning(const StTuning *t) {
@@ -2239,11 +2241,13 @@ extern "C" int st_get_tuning(StTuning *t) {
     return ST_OK;
 }
 
-/* Time every candidate launch shape on the caller's requests (median of
-   `reps` render-kernel times from HIP events) and keep the fastest. The
-   candidates: 8x8, 16x4 and 32x2 tiles; persistent grids of 2, 4 or 8
-   blocks per CU, or one block per 4 tiles in XCD-aware order with at most
-   2, 3 or any number of blocks per CU resident. */
+/* Time every candidate launch shape and claim order on the caller's
+   requests (median of `reps` render-kernel times from HIP events) and keep
+   the fastest. The candidates: 8x8, 16x4 and 32x2 tiles; persistent grids
+   of 2, 4 or 8 blocks per CU, or one block per 4 tiles in XCD-aware order
+   with at most 2, 3 or any number of blocks per CU resident; tiles claimed
+   in tile order or slowest first (that order needs the previous launch's
+   tile times: one untimed launch of the shape comes first). */
 extern "C" int st_autotune(StSceneHandle *const *scenes, const StView *views, uint32_t n, uint32_t reps,
                            StTuning *best, float *best_ms) {
     RT_LOCK;
@@ -2253,18 +2257,22 @@ extern "C" int st_autotune(StSceneHandle *const *scenes, const StView *views, ui
     if (reps == 0) reps = 3;
     if (reps > 15) reps = 15;
     const StTuning saved = g.tune;
-    StTuning cand[18];
+    StTuning cand[36];
     uint32_t nc = 0;
     for (uint32_t sh = 3; sh <= 5; sh++) {
         const uint32_t bpc[3] = {2, 4, 8}, cap[3] = {2, 3, 8};
-        for (int k = 0; k < 3; k++) cand[nc++] = StTuning{sh, bpc[k], 1u};
-        for (int k = 0; k < 3; k++) cand[nc++] = StTuning{sh, cap[k], 0u};
+        for (uint32_t order = 0; order <= 1; order++) {
+            for (int k = 0; k < 3; k++) cand[nc++] = StTuning{sh, bpc[k], 1u, order};
+            for (int k = 0; k < 3; k++) cand[nc++] = StTuning{sh, cap[k], 0u, order};
+        }
     }
     float best_t = INFINITY;
     StTuning best_c = saved;
     for (uint32_t c = 0; c < nc; c++) {
         g.tune = cand[c];
         float times[15];
+        if (cand[c].cost_order)
+            rc = launch(scenes, views, n, 0, (int)cand[c].persistent, NULL, NULL, NULL, NULL, &times[0]);
         for (uint32_t r = 0; r < reps && !rc; r++)
             rc = launch(scenes, views, n, 0, (int)cand[c].persistent, NULL, NULL, NULL, NULL, &times[r]);
         if (rc) break;

@@ -34,10 +34,11 @@ def test_frames_do_not_depend_on_the_previous_launch():
         saved = st.get_tuning()
         try:
             for persistent in (True, False):
-                st.set_tuning(tile_shift=3, blocks_per_cu=4 if persistent else 8, persistent=persistent)
+                st.set_tuning(tile_shift=3, blocks_per_cu=4 if persistent else 8, persistent=persistent,
+                              slowest_first=True)
                 runs.append(_frame(ds, a))
         finally:
-            st.set_tuning(saved.tile_shift, saved.blocks_per_cu, bool(saved.persistent))
+            st.set_tuning(saved.tile_shift, saved.blocks_per_cu, bool(saved.persistent), bool(saved.cost_order))
         for k, (p, h, c) in enumerate(runs):
             assert p.tobytes() == p0.tobytes(), k
             assert h.tobytes() == h0.tobytes(), k
