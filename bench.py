@@ -555,41 +555,63 @@ def served_leg(args):
     # through the same server (max one request per launch, one in flight)
     plain = srv.plain_rate(bv, [ids[k] for k in bp])
     log(f"served: burst {capacity:.0f} req/s, {burst_mrays:.0f} Mrays/s; plain {plain}")
-    rate = args.serve_rate if args.serve_rate > 0 else round(0.5 * capacity, 1)
+    # Poisson arrivals at 80% of the rate the plain loop sustains, served
+    # both ways: through the server (continuous batching, several launches
+    # in flight) and as plain launches (one request per launch, one launch
+    # at a time), the same requests at the same arrival times
+    rate = args.serve_rate if args.serve_rate > 0 else round(0.8 * plain["req_per_s"], 1)
     log(f"served: {len(ids)} scenes resident in {build_s:.1f} s; Poisson rate {rate} req/s for {args.serve_seconds} s")
-    stats0 = srv.stats()
     n = max(1, int(rate * args.serve_seconds))
     views, picks = request_mix(rng, len(ids), n)
     arrivals = np.cumsum(rng.exponential(1.0 / rate, n))
-    tickets = [None] * n
-    lat = []
-    rays = [0]
 
-    def collector():
-        last = time.perf_counter()
+    def poisson(label):
+        stats0 = srv.stats()
+        tickets = [None] * n
+        lat = []
+        rays = [0]
+
+        def collector():
+            last = time.perf_counter()
+            for k in range(n):
+                while tickets[k] is None:
+                    time.sleep(0.0002)
+                _, c, l_ms = srv.wait(tickets[k], copy=False)
+                rays[0] += c.primary + c.shadow + c.bounce
+                lat.append(l_ms)
+                srv.release(tickets[k])
+                if time.perf_counter() - last > 2.0:
+                    last = time.perf_counter()
+                    log(f"served ({label}): {k + 1}/{n} requests done")
+
+        th = threading.Thread(target=collector)
+        th.start()
+        start = time.perf_counter()
         for k in range(n):
-            while tickets[k] is None:
-                time.sleep(0.0002)
-            _, c, l_ms = srv.wait(tickets[k], copy=False)
-            rays[0] += c.primary + c.shadow + c.bounce
-            lat.append(l_ms)
-            srv.release(tickets[k])
-            if time.perf_counter() - last > 2.0:
-                last = time.perf_counter()
-                log(f"served: {k + 1}/{n} requests done")
+            while True:
+                gap = arrivals[k] - (time.perf_counter() - start)
+                if gap <= 0:
+                    break
+                time.sleep(gap if gap > 0.001 else 0)
+            tickets[k] = srv.submit(ids[picks[k]], views[k:k + 1])
+        th.join()
+        wall = time.perf_counter() - start
+        stats = srv.stats()
+        launches = int(stats["launches"] - stats0["launches"])
+        return {"wall_s": round(wall, 3), "mrays": round(rays[0] / wall / 1e6, 2),
+                "latency_p50_ms": pct(lat, 50), "latency_p99_ms": pct(lat, 99),
+                "latency_max_ms": round(float(np.max(lat)), 3), "launches": launches,
+                "mean_batch": round((stats["completed"] - stats0["completed"]) / max(1, launches), 2),
+                "max_in_flight": stats.get("max_in_flight")}
 
-    th = threading.Thread(target=collector)
-    th.start()
-    start = time.perf_counter()
-    for k in range(n):
-        while True:
-            gap = arrivals[k] - (time.perf_counter() - start)
-            if gap <= 0:
-                break
-            time.sleep(gap if gap > 0.001 else 0)
-        tickets[k] = srv.submit(ids[picks[k]], views[k:k + 1])
-    th.join()
-    wall = time.perf_counter() - start
+    served = poisson("server")
+    log(f"served: {served}")
+    srv.set_limits(1, 1)
+    try:
+        plain_poisson = poisson("plain")
+    finally:
+        srv.set_limits(0, 0)
+    log(f"served: plain launches at the same arrivals {plain_poisson}")
     stats = srv.stats()
     srv.close()
     return {
@@ -599,13 +621,9 @@ def served_leg(args):
         "mesh_bvh": "device Morton LBVH (leaves <= 4)", "output": "BGRA8 window image (device resolve)",
         "burst_requests": 512, "burst_capacity_req_per_s": round(capacity, 1), "burst_mrays": round(burst_mrays, 2),
         "plain_burst": plain,
-        "arrival_rate_per_s": rate, "requests": n,
-        "wall_s": round(wall, 3), "mrays": round(rays[0] / wall / 1e6, 2),
-        "latency_p50_ms": pct(lat, 50), "latency_p99_ms": pct(lat, 99),
-        "latency_max_ms": round(float(np.max(lat)), 3),
-        "launches": int(stats["launches"] - stats0["launches"]),
-        "mean_batch": round((stats["completed"] - stats0["completed"]) / max(1, stats["launches"] - stats0["launches"]), 2),
-        "max_in_flight": stats.get("max_in_flight"),
+        "arrival_rate_per_s": rate, "arrival_rate": "80% of the plain loop's request rate", "requests": n,
+        **served,
+        "plain_at_same_arrivals": plain_poisson,
     }
 
 
@@ -655,8 +673,8 @@ def main():
                     help="headline leg without its plain-launch and download comparisons (for profiling runs)")
     ap.add_argument("--instanced-frames", type=int, default=10, help="instanced leg: timed frames")
     ap.add_argument("--serve-rate", type=float, default=0.0,
-                    help="served leg: Poisson arrival rate, requests/s (0: half the measured burst capacity)")
-    ap.add_argument("--serve-seconds", type=float, default=8.0, help="served leg: arrival window, s")
+                    help="served leg: Poisson arrival rate, requests/s (0: 80%% of the plain loop's rate)")
+    ap.add_argument("--serve-seconds", type=float, default=6.0, help="served leg: arrival window, s")
     ap.add_argument("--serve-scenes", type=int, default=40, help="served leg: resident scenes requested")
     args = ap.parse_args()
     args.legs = [x for x in args.legs.split(",") if x]

@@ -198,9 +198,9 @@ int st_debug_step_cap(uint32_t cap);
 
 /* The redo stage: a mesh traversal of the main stages that takes more than
    abandon_steps node-pair steps (0: ST_ABANDON_STEPS, 2048) gives up and its
-   pixel is traced again by a whole wave, which splits traversals of meshes
-   of at least coop_min_tris triangles (0: ST_COOP_MIN_TRIS, 4096) across its
-   lanes. Frames do not depend on either value; tests use tiny ones to send
+   pixel is traced again by a group of blocks, which splits traversals of
+   meshes of at least coop_min_tris triangles (0: ST_COOP_MIN_TRIS, 4096)
+   across its threads. Frames do not depend on either value; tests use tiny ones to send
    every pixel through the redo stage. */
 int st_debug_redo(uint32_t abandon_steps, uint32_t coop_min_tris);
 /* Pixels the redo stage traced again in the last launch on the library's

@@ -27,9 +27,9 @@
  *                   new surface's level as k_primary runs the first, and so
  *                   on while the pixel bounces.
  *   k_redo          the pixels whose traversal gave up in the two stages
- *                   above (a mesh traversal past ST_ABANDON_STEPS steps),
- *                   one per wave, all lanes on one pixel: its whole path
- *                   again with large mesh traversals split across the lanes.
+ *                   above (a mesh traversal past ST_ABANDON_STEPS steps):
+ *                   each pixel's whole path again by a group of blocks,
+ *                   with large mesh traversals split across the group.
  *   k_counts        per-request shadow/bounce ray totals.
  */
 #include <hip/hip_runtime.h>
@@ -448,22 +448,68 @@ k_primary(const StDScene *__restrict__ scenes, const KReq *__restrict__ reqs, ui
     }
 }
 
-/* Redo stage: the pixels whose traversal gave up in the main stages, one
-   per block, every thread of the block on the same pixel: its primary ray
-   and whole depth loop again, with every traversal of a large mesh split
-   across the block's threads (coop_trace_mesh). The pixel's frame value,
-   hit record and ray counts replace what the main stages wrote for it.
-   Each thread has a scene, a mesh and a path-maximum stack column; every
-   thread computes the same values, so the block's barriers are reached by
-   all of its threads. */
+/* Redo stage: the pixels whose traversal gave up in the main stages, each
+   traced again by a group of blocks: its leader block runs the pixel's
+   primary ray and whole depth loop again (every thread computing the same
+   values, so the block's barriers are reached by all of its threads), with
+   every traversal of a large mesh split across the group's blocks
+   (coop_trace_mesh; the other blocks of the group run coop_helper). With
+   fewer pixels than blocks each pixel gets a group of gridDim / pixels
+   blocks; with more, each block is a group of one and takes every
+   gridDim-th pixel. The pixel's frame value, hit record and ray counts
+   replace what the main stages wrote for it. Each thread has a scene, a
+   mesh and a path-maximum stack column in LDS. */
 #define REDO_THREADS 256u
+struct KCoop {
+    char *base;            /* group records: group_stride bytes each (CO_HEADER, then the task queue) */
+    DCoopSlot *slots;      /* one per block of the launch */
+    uint32_t group_stride, q_cap;
+    const uint32_t *epoch; /* launch number (k_tile_order) */
+};
 __global__ void __launch_bounds__(REDO_THREADS) k_redo(const StDScene *__restrict__ scenes,
                                                        const KReq *__restrict__ reqs, uint32_t nreq, KWave w,
-                                                       uint32_t scene_entries, uint32_t mesh_entries) {
+                                                       uint32_t scene_entries, uint32_t mesh_entries, KCoop co) {
     ST_LDS_STACKS(scene_entries, 2u * mesh_entries);
     const DStacks stk = {scene_stack, mesh_stack, mesh_stack + 64u * mesh_entries};
     const uint32_t count = __builtin_amdgcn_readfirstlane(ldgu(w.redo_ctr, 0));
-    for (uint32_t i = blockIdx.x; i < count; i += gridDim.x) {
+    if (count == 0) return;
+    const uint32_t blocks = gridDim.x;
+    const uint32_t ngroups = count < blocks ? count : blocks;
+    const uint32_t gsize = blocks / ngroups;
+    const uint32_t grp = blockIdx.x / gsize, rank = blockIdx.x - grp * gsize;
+    if (grp >= ngroups) return;
+    DCoopGroup *rec = (DCoopGroup *)(co.base + (size_t)grp * co.group_stride);
+    if (threadIdx.x == 0) {
+        co_blk.group = rec;
+        co_blk.queue = (u64 *)((char *)rec + CO_HEADER);
+        co_blk.slots = co.slots + (size_t)grp * gsize;
+        co_blk.gsize = gsize;
+        co_blk.rank = rank;
+        co_blk.q_cap = co.q_cap;
+        co_blk.base = co_ld(co.epoch) << 12;
+        co_blk.jobs = 0;
+    }
+    __syncthreads();
+    if (rank) {
+        coop_helper(stk.mesh, stk.aux);
+        return;
+    }
+    /* the leader: a clean queue and record before its first job */
+    for (uint32_t i = threadIdx.x; i < 4u * co.q_cap; i += blockDim.x) co_st64(co_blk.queue + i, 0);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        co_st(&rec->inflight, 0);
+        co_st(&rec->status, 0);
+        co_st(&rec->head, 0);
+        co_st(&rec->tail, 0);
+        co_st(&rec->busy, 0);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        co_st64(&rec->claim, ((u64)co_blk.base << 32) | CO_CLOSED);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    }
+    __syncthreads();
+    for (uint32_t i = grp; i < count; i += ngroups) {
         const uint32_t slot = __builtin_amdgcn_readfirstlane(ldgu(w.redo_q, i));
         const uint32_t tile = slot >> 6;
         const uint32_t r = find_req(reqs, nreq, tile, false);
@@ -472,8 +518,18 @@ __global__ void __launch_bounds__(REDO_THREADS) k_redo(const StDScene *__restric
         uint32_t x, y;
         if (!tile_pixel(v, tile, slot & 63u, x, y)) continue;
         const DTileBox box = {(int32_t)x, (int32_t)y, (int32_t)x + 1, (int32_t)y + 1};
+#ifdef CO_STATS
+        const u64 st_t0 = __builtin_amdgcn_s_memrealtime();
+#endif
         primary_pixel<true>(sc, v, slot, x, y, true, box, stk, w);
+#ifdef CO_STATS
+        if (threadIdx.x == 0)
+            co_stat(2ull | ((u64)blockIdx.x << 8) | ((u64)gsize << 32), st_t0, __builtin_amdgcn_s_memrealtime(),
+                    (u64)slot | ((u64)v.mode << 32) | ((u64)v.depth << 40));
+#endif
     }
+    __syncthreads();
+    if (threadIdx.x == 0) co_st(&rec->quit, co_blk.base);
 }
 
 /* Redo stage launch shape: blocks of up to REDO_THREADS threads, as many
@@ -482,9 +538,21 @@ __global__ void __launch_bounds__(REDO_THREADS) k_redo(const StDScene *__restric
 static void redo_shape(uint32_t s_entries, uint32_t m_entries, uint32_t &threads, size_t &lds) {
     const size_t per_wave = (size_t)64 * (s_entries + 2 * m_entries) * sizeof(uint32_t);
     uint32_t waves = REDO_THREADS / 64u;
-    while (waves > 1 && per_wave * waves > 160 * 1024 - 256) waves >>= 1;
+    while (waves > 1 && per_wave * waves > 160 * 1024 - sizeof(DCoopBlk) - 256) waves >>= 1;
     threads = 64u * waves;
     lds = per_wave * waves;
+}
+
+/* queued tasks per group (main launches; st_render_noalloc keeps fewer) */
+#define REDO_QUEUE 2048u
+#define REDO_QUEUE_NOALLOC 256u
+static size_t redo_group_stride(uint32_t q_cap) { return CO_HEADER + (size_t)q_cap * 32u; }
+/* device bytes of the redo stage's group records and slots */
+static size_t redo_slots_offset(uint32_t blocks, uint32_t q_cap) {
+    return ((size_t)blocks * redo_group_stride(q_cap) + 255) & ~(size_t)255;
+}
+static size_t redo_coop_bytes(uint32_t blocks, uint32_t q_cap) {
+    return redo_slots_offset(blocks, q_cap) + (((size_t)blocks * sizeof(DCoopSlot) + 255) & ~(size_t)255);
 }
 
 /* Sum of the previous launch's tile times (cost_sum zeroed before): one
@@ -516,14 +584,18 @@ __global__ void __launch_bounds__(COST_THREADS) k_cost_total(const uint32_t *__r
    order. One thread per tile; each block keeps its tiles in order within
    each group and claims its ranges with one atomic per group (ctr[0] from
    the front, ctr[1] from the back, both zero at launch). Any cost values
-   give a permutation of the tiles, so the order never changes a result. */
+   give a permutation of the tiles, so the order never changes a result.
+   Thread 0 also advances the launch number `epoch` the redo stage tags its
+   cooperative jobs with. */
 #define ORDER_THREADS 1024u
 __global__ void __launch_bounds__(ORDER_THREADS) k_tile_order(const uint32_t *__restrict__ tile_cost,
                                                               uint32_t *__restrict__ tile_order, uint32_t tiles,
                                                               const unsigned long long *cost_sum, uint32_t *ctr,
-                                                              uint32_t use_history) {
+                                                              uint32_t use_history, uint32_t *epoch) {
     __shared__ uint32_t s_h[ORDER_THREADS / 64], s_l[ORDER_THREADS / 64], s_base[2];
     const uint32_t i = blockIdx.x * ORDER_THREADS + threadIdx.x;
+    /* a new launch number for the redo stage's group records */
+    if (i == 0) ((ST_GLOBAL uint32_t *)epoch)[0] += 1u;
     if (!use_history) {
         if (i < tiles) ((ST_GLOBAL uint32_t *)tile_order)[i] = i;
         return;
@@ -1408,6 +1480,16 @@ static uint32_t resident_blocks(const void *kernel, size_t lds) {
 
 /* LDS per CU on gfx950 */
 #define CU_LDS_BYTES (160u * 1024u)
+/* the redo stage's grid: as many blocks as are resident at once */
+static uint32_t redo_blocks_for(uint32_t threads, size_t lds) {
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void *>(k_redo), (int)threads,
+                                                     lds) != hipSuccess ||
+        per_cu < 1)
+        per_cu = 1;
+    return (uint32_t)per_cu * (uint32_t)g.cus;
+}
+
 /* resident_blocks with at most `cap` blocks per CU (0: no cap): the block's
    LDS request is raised until no more than `cap` fit; *lds_launch is the
    request to launch with */
@@ -1528,7 +1610,18 @@ int st_rt_launch_ctx(LaunchCtx *c, StSceneHandle *const *scenes, StHostView *hv,
        the redo stage's queue length */
     const size_t cnt_bytes = n * 24ull + 16;
     /* the redo stage's queue: every launch (a primary ray can give up too) */
-    const size_t redo_bytes = round256(slots * 4) + 256;
+    uint32_t s_entries = 1, m_entries = 1;
+    for (uint32_t i = 0; i < n; i++) {
+        const StDScene &d = scenes[i]->dev;
+        if (d.scene_stack_need > s_entries) s_entries = d.scene_stack_need;
+        if (d.mesh_stack_size > m_entries) m_entries = d.mesh_stack_size;
+    }
+    /* redo stage: as many blocks as fit at once, grouped per redone pixel */
+    uint32_t redo_threads;
+    size_t lds_redo;
+    redo_shape(s_entries, m_entries, redo_threads, lds_redo);
+    const uint32_t redo_blocks = redo_blocks_for(redo_threads, lds_redo);
+    const size_t redo_bytes = round256(slots * 4) + 256 + redo_coop_bytes(redo_blocks, REDO_QUEUE) + 256;
     size_t work = up + round256(cnt_bytes) + per_req_out + wave_bytes + redo_bytes + 256;
     if ((rc = arena_reserve(&c->work, work))) return rc;
     /* staging: the upload block, then the ray counts and fault word read back */
@@ -1610,6 +1703,12 @@ int st_rt_launch_ctx(LaunchCtx *c, StSceneHandle *const *scenes, StHostView *hv,
     }
     w.redo_q = (uint32_t *)arena_take(&c->work, slots * 4);
     w.redo_ctr = (uint32_t *)(cnt_d + 3 * (size_t)n + 1);
+    KCoop coop;
+    coop.base = (char *)arena_take(&c->work, redo_coop_bytes(redo_blocks, REDO_QUEUE));
+    coop.slots = (DCoopSlot *)(coop.base + redo_slots_offset(redo_blocks, REDO_QUEUE));
+    coop.group_stride = (uint32_t)redo_group_stride(REDO_QUEUE);
+    coop.q_cap = REDO_QUEUE;
+    coop.epoch = (const uint32_t *)(c->cost_sum + 1);
     /* launch shape, fixed before anything is enqueued so that a captured
        graph of this launch can be matched against the next one */
     uint32_t blocks;
@@ -1622,11 +1721,6 @@ int st_rt_launch_ctx(LaunchCtx *c, StSceneHandle *const *scenes, StHostView *hv,
         blocks = (tiles + WAVES_PER_BLOCK - 1) / WAVES_PER_BLOCK;
         blocks = (blocks + 7u) & ~7u;
         xcd_blocks = blocks;
-    }
-    uint32_t s_entries = 1, m_entries = 1;
-    for (uint32_t k = 0; k < nuniq; k++) {
-        if (sc_h[k].scene_stack_need > s_entries) s_entries = sc_h[k].scene_stack_need;
-        if (sc_h[k].mesh_stack_size > m_entries) m_entries = sc_h[k].mesh_stack_size;
     }
     size_t lds = (size_t)BLOCK_THREADS * (s_entries + m_entries) * sizeof(uint32_t);
     if (lds > 160 * 1024) return set_err(ST_ERR_SCENE, "traversal stacks need %zu B of LDS", lds);
@@ -1644,11 +1738,6 @@ int st_rt_launch_ctx(LaunchCtx *c, StSceneHandle *const *scenes, StHostView *hv,
         b_bounce = resident_blocks(reinterpret_cast<const void *>(k_bounce), lds);
     }
     const uint32_t cost_blocks = std::min<uint32_t>((tiles + COST_THREADS - 1) / COST_THREADS, 1024u);
-    /* redo stage: one block per CU, each on one pixel at a time */
-    uint32_t redo_threads;
-    size_t lds_redo;
-    redo_shape(s_entries, m_entries, redo_threads, lds_redo);
-    const uint32_t redo_blocks = (uint32_t)g.cus;
 
     /* per-stage timing (bench): an event before the first stage and after each */
     uint32_t marks = 0;
@@ -1684,7 +1773,8 @@ int st_rt_launch_ctx(LaunchCtx *c, StSceneHandle *const *scenes, StHostView *hv,
         }
         hipLaunchKernelGGL(k_tile_order, dim3((tiles + ORDER_THREADS - 1) / ORDER_THREADS), dim3(ORDER_THREADS), 0,
                            c->stream, (const uint32_t *)c->tile_cost, c->tile_order, tiles,
-                           (const unsigned long long *)c->cost_sum, c->tile_counter + ORDER_CLAIM, use_history);
+                           (const unsigned long long *)c->cost_sum, c->tile_counter + ORDER_CLAIM, use_history,
+                           (uint32_t *)(c->cost_sum + 1));
         HIP_TRY(hipGetLastError());
         HIP_TRY(mark());
         hipLaunchKernelGGL(k_primary, dim3(blocks), dim3(BLOCK_THREADS), lds_primary, c->stream, (const StDScene *)sc_d,
@@ -1703,7 +1793,7 @@ int st_rt_launch_ctx(LaunchCtx *c, StSceneHandle *const *scenes, StHostView *hv,
         /* the pixels that gave up, traced again (it reads the queue's length
            on the device: no round trip; nothing to do is one quick launch) */
         hipLaunchKernelGGL(k_redo, dim3(redo_blocks), dim3(redo_threads), lds_redo, c->stream, (const StDScene *)sc_d,
-                           (const KReq *)reqs_d, n, w, s_entries, m_entries);
+                           (const KReq *)reqs_d, n, w, s_entries, m_entries, coop);
         HIP_TRY(mark());
         if (levels && count_blocks)
             hipLaunchKernelGGL(k_counts, dim3(count_blocks), dim3(256), 0, c->stream, (const KReq *)reqs_d, n,
@@ -1726,7 +1816,8 @@ int st_rt_launch_ctx(LaunchCtx *c, StSceneHandle *const *scenes, StHostView *hv,
                                      (uint64_t)(uintptr_t)st,
                                      (uint64_t)(uintptr_t)cnt_d, (uint64_t)(uintptr_t)w.st, w.n,
                                      (uint64_t)(uintptr_t)c->tile_cost, use_history, cost_blocks,
-                                     (uint64_t)(uintptr_t)w.redo_q, redo_blocks, redo_threads, lds_redo};
+                                     (uint64_t)(uintptr_t)w.redo_q, redo_blocks, redo_threads, lds_redo,
+                                     (uint64_t)(uintptr_t)coop.base};
         for (uint32_t i = 0; i < n; i++) {
             const StDView &dv = reqs_h[i].v;
             sig.insert(sig.end(), {dv.width1, dv.height1, dv.mode, dv.depth, dv.tile_shift,
@@ -1876,6 +1967,13 @@ extern "C" int st_render_noalloc(StSceneHandle *scene, const StView *view, StFlo
     const size_t o_hits = hits ? take(px * 16) : 0;
     const size_t o_order = take((size_t)tiles * 4), o_cost = take((size_t)tiles * 4), o_ctr = take(256);
     const size_t o_redo = take(slots * 4);
+    const uint32_t s_entries = scene->dev.scene_stack_need > 1 ? scene->dev.scene_stack_need : 1;
+    const uint32_t m_entries = scene->dev.mesh_stack_size > 1 ? scene->dev.mesh_stack_size : 1;
+    uint32_t redo_threads;
+    size_t lds_redo;
+    redo_shape(s_entries, m_entries, redo_threads, lds_redo);
+    const uint32_t redo_blocks = redo_blocks_for(redo_threads, lds_redo);
+    const size_t o_coop = take(redo_coop_bytes(redo_blocks, REDO_QUEUE_NOALLOC));
     size_t o_st = 0, o_sres = 0, o_occ = 0, o_bq = 0, o_wctr = 0, o_wcnt = 0;
     if (beauty) {
         o_st = take(slots * 16 * ST_PLANES);
@@ -1885,9 +1983,11 @@ extern "C" int st_render_noalloc(StSceneHandle *scene, const StView *view, StFlo
         o_wctr = take(8);
         o_wcnt = take(slots * 8);
     }
-    if (off > ST_NOALLOC_SCRATCH_BYTES)
-        return set_err(ST_ERR_MEMORY, "request needs %zu bytes of scratch; st_render_noalloc has %u", off,
+    /* the last 256 bytes hold the redo stage's launch number (kept across calls) */
+    if (off > ST_NOALLOC_SCRATCH_BYTES - 256)
+        return set_err(ST_ERR_MEMORY, "request needs %zu bytes of scratch; st_render_noalloc has %u", off + 256,
                        (unsigned)ST_NOALLOC_SCRATCH_BYTES);
+    uint32_t *const epoch = (uint32_t *)(base + ST_NOALLOC_SCRATCH_BYTES - 256);
     if (up_host > ST_NOALLOC_HOST_BYTES) return set_err(ST_ERR_ARG, "view too tall for st_render_noalloc");
     StHostView hv;
     if (st_view_prepare_rows(hs, view, &hv, (float *)(g_noalloc_host + o_rows)))
@@ -1934,8 +2034,12 @@ extern "C" int st_render_noalloc(StSceneHandle *scene, const StView *view, StFlo
     }
     w.redo_q = (uint32_t *)(base + o_redo);
     w.redo_ctr = (uint32_t *)(cnt_d + 4);
-    const uint32_t s_entries = dsc.scene_stack_need > 1 ? dsc.scene_stack_need : 1;
-    const uint32_t m_entries = dsc.mesh_stack_size > 1 ? dsc.mesh_stack_size : 1;
+    KCoop coop;
+    coop.base = base + o_coop;
+    coop.slots = (DCoopSlot *)(coop.base + redo_slots_offset(redo_blocks, REDO_QUEUE_NOALLOC));
+    coop.group_stride = (uint32_t)redo_group_stride(REDO_QUEUE_NOALLOC);
+    coop.q_cap = REDO_QUEUE_NOALLOC;
+    coop.epoch = epoch;
     const size_t lds = (size_t)BLOCK_THREADS * (s_entries + m_entries) * sizeof(uint32_t);
     if (lds > 160 * 1024) return set_err(ST_ERR_SCENE, "traversal stacks need %zu B of LDS", lds);
     uint32_t blocks, xcd_blocks = 0;
@@ -1964,7 +2068,7 @@ extern "C" int st_render_noalloc(StSceneHandle *scene, const StView *view, StFlo
                        reqs_d, 1u, height1);
     hipLaunchKernelGGL(k_tile_order, dim3((tiles + ORDER_THREADS - 1) / ORDER_THREADS), dim3(ORDER_THREADS), 0, stream,
                        (const uint32_t *)(base + o_cost), (uint32_t *)(base + o_order), tiles,
-                       (const unsigned long long *)cnt_d, ctr + ORDER_CLAIM, 0u);
+                       (const unsigned long long *)cnt_d, ctr + ORDER_CLAIM, 0u, epoch);
     HIP_TRY(hipGetLastError());
     hipLaunchKernelGGL(k_primary, dim3(blocks), dim3(BLOCK_THREADS), lds_primary, stream, sc_d, reqs_d, 1u, tiles, ctr,
                        (uint32_t)tune.persistent, xcd_blocks, s_entries, m_entries, w,
@@ -1975,11 +2079,8 @@ extern "C" int st_render_noalloc(StSceneHandle *scene, const StView *view, StFlo
         hipLaunchKernelGGL(k_bounce, dim3(b_bounce), dim3(BLOCK_THREADS), lds, stream, sc_d, reqs_d, 1u, w,
                            ctr + BOUNCE_CLAIM, s_entries, m_entries);
     }
-    uint32_t redo_threads;
-    size_t lds_redo;
-    redo_shape(s_entries, m_entries, redo_threads, lds_redo);
-    hipLaunchKernelGGL(k_redo, dim3((uint32_t)g.cus), dim3(redo_threads), lds_redo, stream, sc_d, reqs_d, 1u, w,
-                       s_entries, m_entries);
+    hipLaunchKernelGGL(k_redo, dim3(redo_blocks), dim3(redo_threads), lds_redo, stream, sc_d, reqs_d, 1u, w,
+                       s_entries, m_entries, coop);
     if (beauty)
         hipLaunchKernelGGL(k_counts, dim3((tiles + COUNT_TILES - 1) / COUNT_TILES), dim3(256), 0, stream, reqs_d, 1u,
                            (const uint32_t *)w.counts);
@@ -2010,6 +2111,23 @@ extern "C" int st_debug_step_cap(uint32_t cap) {
     g.step_cap = cap;
     return ST_OK;
 }
+
+#ifdef CO_STATS
+/* measurement builds: the cooperative traversal's records since the last call */
+extern "C" int st_debug_coop_stats(unsigned long long *out, uint32_t cap, uint32_t *n) {
+    RT_LOCK;
+    HIP_TRY(hipDeviceSynchronize());
+    unsigned cnt = 0;
+    HIP_TRY(hipMemcpyFromSymbol(&cnt, HIP_SYMBOL(co_stats_n), sizeof cnt));
+    if (cnt > 32768u) cnt = 32768u;
+    if (cnt > cap) cnt = cap;
+    if (cnt) HIP_TRY(hipMemcpyFromSymbol(out, HIP_SYMBOL(co_stats), (size_t)cnt * 32));
+    const unsigned zero = 0;
+    HIP_TRY(hipMemcpyToSymbol(HIP_SYMBOL(co_stats_n), &zero, sizeof zero));
+    *n = cnt;
+    return ST_OK;
+}
+#endif
 
 extern "C" int st_debug_redo(uint32_t abandon_steps, uint32_t coop_min_tris) {
     RT_LOCK;

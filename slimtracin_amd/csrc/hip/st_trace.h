@@ -631,261 +631,833 @@ DFN void trav_fault(const StDScene &sc, uint32_t code) {
    pattern, which no arithmetic result can have */
 #define ST_DIST_UNSET 0x7FA5A5A5u
 
-/* Block-shared state of one cooperative mesh traversal. */
-struct DCoopShared {
-    unsigned long long best;   /* closest hit: least (distance bits << 32 | key) over the units' hits;
-                                  any hit: least key */
-    uint32_t next;             /* next unit to claim */
-    uint32_t status;           /* OR of the units' failures: 1 stack limit, 2 step bound */
-    uint32_t first_key;        /* any hit: least key of a unit with a hit so far */
-    uint32_t last_key;         /* nothing hit: greatest key + 1 of a unit with a plane hit */
-    uint32_t tri;              /* the winning unit's triangle */
-    float pm;                  /* its path maximum */
-    float last_dist;           /* distance of the last plane hit of the latest unit with one */
+/* ---- cooperative traversal of one ray by the blocks of a redo group ------
+
+   The redo stage traces each of its pixels with a group of blocks: the
+   group's leader block runs the pixel's path (every thread of it computing
+   the same values), and every traversal of a large mesh on that path is a
+   job that the leader posts to the group's record in device memory; the
+   group's other blocks (helpers) wait for jobs and take part in them. No
+   block ever waits for a block that has not started: a helper only joins a
+   posted job, and the leader waits only for helpers that have joined, so a
+   group whose helpers never get a CU still finishes, on the leader alone.
+
+   Work of a job: tasks, each a subtree of the mesh tree that one lane
+   traverses with the reference's loop (mesh.h:38-111) with the box bound
+   fixed at the traversal's start. The leader's first lane starts with the
+   root; a lane with no task takes the oldest entry of another lane's stack
+   (the largest subtree that lane has left, visited last), within its wave
+   or through the group's queue (co_work). A task's order key is its path
+   from the root, two bits per level, most significant first: leaf children
+   of a step 0 (left) and 1 (right), the inner child entered first 2 and
+   last 3 (the nearer first for a closest hit). The reference visits the
+   subtrees in key order, and each task's visited set is a contiguous run
+   of that order (an entry taken from a stack is visited after everything
+   its lane still visits), so:
+     any hit: the reference's hit is the first hit of the task with the
+       least key (tasks beyond a key already hit stop early);
+     closest hit: t* = the least hit distance; among hits at t*, the one of
+       the least key is first in the reference's order. The reference
+       reaches it if every box on its path admits it at bound t* (every
+       entry distance on the path is at most t*), and then keeps it;
+     nothing hit: the bound never moved, so the tasks are exactly what the
+       reference visits, and its current-hit distance (the only field of
+       that record a later result reads) is that of the last plane hit of
+       the task latest in key order that has one.
+   All shared words are read and written with device-scope atomics (they
+   bypass the non-coherent caches: the group's blocks sit on different
+   XCDs); queued tasks carry a tag in each 64-bit word, so a reader never
+   needs a fence to know a task is whole. */
+
+#define CO_CLOSED 0x80000000u     /* job word: the leader has ended the job */
+#define CO_MAX_JOBS 4095u         /* jobs per group and launch (the job's sequence number's low 12 bits) */
+#define CO_MAX_DIGITS 64u         /* order-key digits (tree levels) */
+#define CO_NODE_BITS 25u          /* stack entry: node pair index | depth << CO_NODE_BITS */
+#define CO_EXPORT_LOW 128u        /* waves hand entries to the queue while it holds fewer than this */
+#define CO_SPIN_LIMIT (1u << 22)  /* polls of any wait before it gives up (seconds) */
+#define CO_HELPER_IDLE_TICKS 20000u  /* 200 us of the 100 MHz clock without a job: a helper leaves */
+/* status bits of a job */
+#define CO_ST_DEEP 1u             /* a path deeper than the order key or a stack limit: trace sequentially */
+#define CO_ST_STEPS 2u            /* a traversal ran past its step bound (malformed tree) */
+#define CO_ST_LOST 4u             /* a wait gave up: the job's result is not known */
+
+typedef unsigned long long u64;
+#define CO_G __attribute__((address_space(1)))
+
+DFN uint32_t co_ld(const uint32_t *p) {
+    return __hip_atomic_load((const CO_G uint32_t *)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+DFN u64 co_ld64(const u64 *p) {
+    return __hip_atomic_load((const CO_G u64 *)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+DFN void co_st(uint32_t *p, uint32_t v) {
+    __hip_atomic_store((CO_G uint32_t *)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+DFN void co_st64(u64 *p, u64 v) { __hip_atomic_store((CO_G u64 *)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+DFN uint32_t co_add(uint32_t *p, uint32_t v) {
+    return __hip_atomic_fetch_add((CO_G uint32_t *)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+DFN u64 co_add64(u64 *p, u64 v) {
+    return __hip_atomic_fetch_add((CO_G u64 *)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+DFN void co_max64(u64 *p, u64 v) {
+    (void)__hip_atomic_fetch_max((CO_G u64 *)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+DFN void co_min64(u64 *p, u64 v) {
+    (void)__hip_atomic_fetch_min((CO_G u64 *)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+DFN void co_or(uint32_t *p, uint32_t v) {
+    (void)__hip_atomic_fetch_or((CO_G uint32_t *)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+DFN bool co_cas(uint32_t *p, uint32_t expect, uint32_t want) {
+    return __hip_atomic_compare_exchange_strong((CO_G uint32_t *)p, &expect, want, __ATOMIC_RELAXED,
+                                                __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+DFN void co_sleep() { __builtin_amdgcn_s_sleep(2); }
+
+/* Path order key: 2-bit digits, most significant first (digit i at bit
+   126 - 2i of hi:lo). */
+struct DKey { u64 hi, lo; };
+DFN DKey key_digit(DKey k, uint32_t i, uint32_t dg) {
+    if (i < 32u) k.hi |= (u64)dg << (62u - 2u * i);
+    else k.lo |= (u64)dg << (62u - 2u * (i - 32u));
+    return k;
+}
+DFN DKey key_trunc(DKey k, uint32_t n) {
+    if (n == 0u) return DKey{0, 0};
+    if (n < 32u) {
+        k.hi &= ~0ull << (64u - 2u * n);
+        k.lo = 0;
+    } else if (n == 32u) {
+        k.lo = 0;
+    } else if (n < 64u) {
+        k.lo &= ~0ull << (64u - 2u * (n - 32u));
+    }
+    return k;
+}
+DFN bool key_lt(DKey a, DKey b) { return a.hi < b.hi || (a.hi == b.hi && a.lo < b.lo); }
+
+/* One job: a mesh traversal posted by the leader (26 words). */
+struct DCoopJob {
+    uint32_t ro[3], rd[3], rr[3], rs[3];   /* local ray: origin, direction, reciprocal, scaled origin (bits) */
+    uint32_t neg, c0, nodes_lo, nodes_hi, tris_lo, tris_hi;
+    uint32_t T, pad0, pad1, root_first, any_hit, max_steps, stack_size, seq;
+};
+/* A group's record (CO_HEADER bytes), then its queue of tasks
+   (4 words of 64 bits each). */
+struct DCoopGroup {
+    u64 claim;      /* job word: seq << 32, | CO_CLOSED once the leader has ended the job */
+    u64 first_hi;   /* any hit: least key (high word) of a task with a hit so far */
+    uint32_t inflight;           /* helper blocks taking part in the job */
+    uint32_t status;             /* CO_ST_* */
+    uint32_t head, tail;         /* queue: next item to take, next free */
+    uint32_t busy;               /* waves holding tasks */
+    uint32_t quit;               /* the leader's launch base once it has no more pixels */
+    uint32_t pad[6];
+    DCoopJob job;
+};
+#define CO_HEADER 256u
+static_assert(sizeof(DCoopGroup) <= CO_HEADER, "group record");
+/* a block's result for a job (16 words): seq tag, hit, last plane hit */
+struct DCoopSlot {
+    uint32_t seq, found, t, tri, pm, k0, k1, k2, k3, lfound, l0, l1, l2, l3, ldist, pad;
 };
 
-/* The reference's mesh traversal of one ray (mesh.h:38-111), run by every
-   thread of a block, all on the same ray (the redo stage). The top D levels
-   of the tree (D = log2(triangles) - 5, clamped to 6..15) split it into
-   2^D units: unit u descends along the bits of u (left = 0, from the root)
-   and is the subtree it reaches, or a leaf its path ends at (then only the
-   unit whose lower bits are zero holds it). Threads claim units from a
-   block counter, which skips the rest of a range of units found empty (a
-   box on their common path not admitted, or the leaf their path ends at),
-   and run the reference's loop over each unit's subtree (same step
-   structure and child order as trace_mesh) with the box bound fixed at the
-   traversal's start, a superset of the boxes the sequential traversal
-   admits, and a triangle test that starts afresh in each unit.
-     Each loop iteration handles one node pair per thread, of a descent to a
-   unit or of a unit's traversal, and a thread that ends its unit claims the
-   next one in the same loop, so the threads of a wave do not wait for each
-   other's units. With the bound fixed, a step's successor follows from its
-   box tests alone: its node pair is loaded before the step's triangles are
-   tested, and those are loaded together (hit_triangles_batched).
-     A unit's order key is, per pair on its path, where the reference takes
-   that child: leaf children in that step (left 0, right 1), then the inner
-   child it enters first (2, the nearer one for a closest hit) and last (3);
-   the reference visits the units in key order.
-     any hit: the sequential traversal's bound stays fixed until its first
-       hit, so its hit is the first hit of the unit with the least key
-       (units whose key exceeds a key already hit are skipped or left);
-     closest hit: t* = the least hit distance; among the hits at t*, the one
-       of the least key is first in the reference's order. The reference
-       reaches it if every box on its path admits it at bound t* (its bound
-       is still above t* there), i.e. every entry distance on the path is at
-       most t* (the boxes were admitted at the larger starting bound), and it
-       then keeps it: nothing is nearer, and ties lose to the first.
-   The winner's hit record is recomputed from its triangle with the same
-   test. Nothing hit: the bound never moved, so the units are exactly what
-   the reference visits, and its current-hit distance (the only field of
-   that record a later result reads) is that of the last plane hit of the
-   unit latest in key order that has one.
-   Returns 1 (hit in tr.closest_mesh_hit), 0 (none), or -1 when the check
-   fails, or a unit's stack would reach the reference's limit (never for a
-   valid tree), and the caller traces sequentially (start state restored). */
-DFN int coop_trace_mesh(DTrace &tr, const StDScene &sc, const StDMesh &mesh, const uint32_t *nodes,
-                        const StDTriHot *tris, uint32_t root_first, bool any_hit) {
-    __shared__ DCoopShared cs;
-    volatile DCoopShared &vs = cs;
-    DRay &ray = tr.local;
-    DHit &closest = tr.closest_mesh_hit;
-    uint32_t *stack = tr.mesh_stack, *pstack = tr.aux_stack;
-    const float c0 = closest.distance, cur0 = tr.current_hit.distance;
-    const uint32_t T = mesh.triangle_count;
-    const uint32_t lg = 31u - (uint32_t)__builtin_clz(T);
-    const uint32_t D = lg < 11u ? 6u : (lg > 20u ? 15u : lg - 5u);
-    const uint32_t units = 1u << D;
-    uint32_t max_steps = 2u * T + 2u;
-    if (sc.step_cap && sc.step_cap < max_steps) max_steps = sc.step_cap;
-    __syncthreads();  /* the previous traversal's readers of cs are done */
+/* a job as one thread holds it */
+struct DCoopJobR {
+    const uint32_t *nodes;
+    const StDTriHot *tris;
+    float c0;
+    uint32_t T, root_first, any_hit, max_steps, stack_size, seq, valid;
+};
+
+/* one thread's (or block's) result so far */
+struct DCoopRes {
+    uint32_t found, tri, lfound;
+    float t, pm, ldist;
+    DKey k, lk;
+};
+DFN void res_merge(DCoopRes &r, const DCoopRes &o, bool any_hit) {
+    if (o.found) {
+        bool better = !r.found;
+        if (!better) {
+            if (!any_hit && o.t != r.t) better = o.t < r.t;
+            else better = key_lt(o.k, r.k);
+        }
+        if (better) {
+            r.found = 1;
+            r.t = o.t;
+            r.tri = o.tri;
+            r.pm = o.pm;
+            r.k = o.k;
+        }
+    }
+    if (o.lfound && (!r.lfound || key_lt(r.lk, o.lk))) {
+        r.lfound = 1;
+        r.lk = o.lk;
+        r.ldist = o.ldist;
+    }
+}
+DFN uint32_t shfl_u(uint32_t v, uint32_t m) { return (uint32_t)__shfl_xor((int)v, (int)m, 64); }
+DFN u64 shfl_u64(u64 v, uint32_t m) {
+    return (u64)shfl_u((uint32_t)v, m) | ((u64)shfl_u((uint32_t)(v >> 32), m) << 32);
+}
+DFN void res_wave_reduce(DCoopRes &r, bool any_hit) {
+#pragma unroll
+    for (uint32_t m = 32; m > 0; m >>= 1) {
+        DCoopRes o;
+        o.found = shfl_u(r.found, m);
+        o.t = __uint_as_float(shfl_u(__float_as_uint(r.t), m));
+        o.tri = shfl_u(r.tri, m);
+        o.pm = __uint_as_float(shfl_u(__float_as_uint(r.pm), m));
+        o.k.hi = shfl_u64(r.k.hi, m);
+        o.k.lo = shfl_u64(r.k.lo, m);
+        o.lfound = shfl_u(r.lfound, m);
+        o.lk.hi = shfl_u64(r.lk.hi, m);
+        o.lk.lo = shfl_u64(r.lk.lo, m);
+        o.ldist = __uint_as_float(shfl_u(__float_as_uint(r.ldist), m));
+        res_merge(r, o, any_hit);
+    }
+}
+
+#define CO_MAX_WAVES 4u   /* waves per redo block (REDO_THREADS / 64) */
+/* Per block state of the redo stage (LDS). */
+struct DCoopBlk {
+    DCoopGroup *group;
+    u64 *queue;
+    DCoopSlot *slots;      /* the group's blocks' slots, by rank */
+    uint32_t gsize, rank, q_cap, base, jobs, seq, cmd, status;
+    uint32_t jw[26];       /* a helper: the job it takes part in */
+    DCoopRes red[CO_MAX_WAVES];
+    uint32_t ex[CO_MAX_WAVES][64 * 6];   /* per wave: tasks handed between lanes */
+};
+static __shared__ DCoopBlk co_blk;
+
+#define T_STAT(m) ((u64)(m).triangle_count)
+#ifdef CO_STATS
+/* measurement builds: one record per cooperative job, redone pixel and
+   sequential retry (4 words: kind | group << 8 | aux << 32, start clock,
+   end clock, detail) */
+static __device__ unsigned long long co_stats[32768][4];
+static __device__ unsigned int co_stats_n;
+DFN void co_stat(u64 a, u64 b, u64 c, u64 d) {
+    const unsigned i = atomicAdd(&co_stats_n, 1u);
+    if (i < 32768u) {
+        co_stats[i][0] = a;
+        co_stats[i][1] = b;
+        co_stats[i][2] = c;
+        co_stats[i][3] = d;
+    }
+}
+#endif
+
+/* Merge r over the block: every thread returns the block's result (all
+   threads call it). */
+DFN DCoopRes res_block_reduce(DCoopRes r, bool any_hit) {
+    res_wave_reduce(r, any_hit);
+    const uint32_t wv = threadIdx.x >> 6, nw = (blockDim.x + 63u) >> 6;
+    __syncthreads();
+    if ((threadIdx.x & 63u) == 0) co_blk.red[wv] = r;
+    __syncthreads();
     if (threadIdx.x == 0) {
-        cs.best = ~0ull;
-        cs.next = 0;
-        cs.status = 0;
-        cs.first_key = 0xFFFFFFFFu;
-        cs.last_key = 0;
+        DCoopRes a = co_blk.red[0];
+        for (uint32_t i = 1; i < nw; i++) res_merge(a, co_blk.red[i], any_hit);
+        co_blk.red[0] = a;
     }
     __syncthreads();
-    /* this thread's best unit result */
-    bool found = false;
-    uint32_t b_key = 0xFFFFFFFFu, b_tri = 0, l_key = 0;
-    float b_t = INFINITY, b_pm = 0.0f, l_dist = 0.0f;
-    /* its current unit: claiming (0), descending to it (1), traversing it (2) */
-    enum { CLAIM, DESCEND, TRAVERSE };
-    uint32_t phase = CLAIM, u = 0, d = 0, key = 0, left = 0, sp = 0, steps = 0;
-    float pm = 0.0f, u_pm = 0.0f;
-    bool u_found = false;
+    const DCoopRes out = co_blk.red[0];
+    __syncthreads();
+    return out;
+}
+
+DFN void co_put_slot(DCoopSlot *s, uint32_t seq, const DCoopRes &r) {
+    uint32_t *p = (uint32_t *)s;
+    co_st(p + 1, r.found);
+    co_st(p + 2, __float_as_uint(r.t));
+    co_st(p + 3, r.tri);
+    co_st(p + 4, __float_as_uint(r.pm));
+    co_st(p + 5, (uint32_t)(r.k.hi >> 32));
+    co_st(p + 6, (uint32_t)r.k.hi);
+    co_st(p + 7, (uint32_t)(r.k.lo >> 32));
+    co_st(p + 8, (uint32_t)r.k.lo);
+    co_st(p + 9, r.lfound);
+    co_st(p + 10, (uint32_t)(r.lk.hi >> 32));
+    co_st(p + 11, (uint32_t)r.lk.hi);
+    co_st(p + 12, (uint32_t)(r.lk.lo >> 32));
+    co_st(p + 13, (uint32_t)r.lk.lo);
+    co_st(p + 14, __float_as_uint(r.ldist));
+    co_st(p + 0, seq);
+}
+DFN bool co_get_slot(const DCoopSlot *s, uint32_t seq, DCoopRes &r) {
+    const uint32_t *p = (const uint32_t *)s;
+    if (co_ld(p) != seq) return false;
+    r.found = co_ld(p + 1);
+    r.t = __uint_as_float(co_ld(p + 2));
+    r.tri = co_ld(p + 3);
+    r.pm = __uint_as_float(co_ld(p + 4));
+    r.k.hi = ((u64)co_ld(p + 5) << 32) | co_ld(p + 6);
+    r.k.lo = ((u64)co_ld(p + 7) << 32) | co_ld(p + 8);
+    r.lfound = co_ld(p + 9);
+    r.lk.hi = ((u64)co_ld(p + 10) << 32) | co_ld(p + 11);
+    r.lk.lo = ((u64)co_ld(p + 12) << 32) | co_ld(p + 13);
+    r.ldist = __uint_as_float(co_ld(p + 14));
+    return true;
+}
+
+/* Queued tasks: node | depth << CO_NODE_BITS, path maximum, key; four
+   64-bit words in ring slot (index % q_cap), each with the task's 16-bit tag
+   (from the job and the task's index) in its top bits, so that a reader
+   tells a whole task from a slot still being written or holding an older
+   one. The reader of a task zeroes its first word once it has read it; a
+   writer waits for that before it reuses the slot. */
+DFN u64 co_tag(uint32_t seq, uint32_t idx) { return (u64)((seq * 4099u + idx) % 0xFFFFu + 1u) << 48; }
+/* false: the slot never came free */
+DFN bool co_push_item(u64 *slot, uint32_t seq, uint32_t idx, uint32_t nd, float pm, DKey k) {
+    for (uint32_t spins = 0; co_ld64(slot) != 0ull; spins++) {
+        if (spins > CO_SPIN_LIMIT) return false;
+        co_sleep();
+    }
+    const u64 tg = co_tag(seq, idx);
+    const uint32_t pb = __float_as_uint(pm);
+    co_st64(slot + 1, tg | ((u64)(pb & 0xFFFFu) << 32) | (k.hi >> 32));
+    co_st64(slot + 2, tg | ((k.hi & 0xFFFFFFFFull) << 16) | (k.lo >> 48));
+    co_st64(slot + 3, tg | (k.lo & 0xFFFFFFFFFFFFull));
+    co_st64(slot + 0, tg | ((u64)nd << 16) | (pb >> 16));
+    return true;
+}
+/* Item idx of the queue (a slot this wave has claimed): waits for its words,
+   frees the slot; 1, or -1 when its words never arrived or do not decode
+   to a node pair of the mesh. */
+DFN int co_pop_item(u64 *slot, uint32_t seq, uint32_t idx, uint32_t T, uint32_t &nd, float &pm, DKey &k) {
+    const u64 tg = co_tag(seq, idx), m = 0xFFFFull << 48;
+    u64 w0, w1, w2, w3;
+    for (uint32_t spins = 0;; spins++) {
+        w0 = co_ld64(slot + 0);
+        w1 = co_ld64(slot + 1);
+        w2 = co_ld64(slot + 2);
+        w3 = co_ld64(slot + 3);
+        if ((w0 & m) == tg && (w1 & m) == tg && (w2 & m) == tg && (w3 & m) == tg) break;
+        if (spins > CO_SPIN_LIMIT) return -1;
+        co_sleep();
+    }
+    co_st64(slot + 0, 0);
+    nd = (uint32_t)(w0 >> 16);
+    pm = __uint_as_float((uint32_t)(((w0 & 0xFFFFull) << 16) | ((w1 >> 32) & 0xFFFFull)));
+    k.hi = ((w1 & 0xFFFFFFFFull) << 32) | ((w2 >> 16) & 0xFFFFFFFFull);
+    k.lo = ((w2 & 0xFFFFull) << 48) | (w3 & 0xFFFFFFFFFFFFull);
+    const uint32_t node = nd & ((1u << CO_NODE_BITS) - 1u), depth = nd >> CO_NODE_BITS;
+    if (node + 1u >= 2u * T || depth == 0u || depth > CO_MAX_DIGITS) return -1;
+    return 1;
+}
+
+/* The job in LDS as 26 words (a helper's thread 0 copies it from the
+   record) -> one thread's copy. */
+DFN void co_job_from_words(const uint32_t *v, uint32_t seq, DCoopJobR &J, DTrace &tr) {
+    const DCoopJob &j = *(const DCoopJob *)v;
+    tr.local.origin = V3(__uint_as_float(j.ro[0]), __uint_as_float(j.ro[1]), __uint_as_float(j.ro[2]));
+    tr.local.direction = V3(__uint_as_float(j.rd[0]), __uint_as_float(j.rd[1]), __uint_as_float(j.rd[2]));
+    tr.local.direction_reciprocal = V3(__uint_as_float(j.rr[0]), __uint_as_float(j.rr[1]), __uint_as_float(j.rr[2]));
+    tr.local.scaled_origin = V3(__uint_as_float(j.rs[0]), __uint_as_float(j.rs[1]), __uint_as_float(j.rs[2]));
+    tr.local.neg_x = j.neg & 1u;
+    tr.local.neg_y = (j.neg >> 1) & 1u;
+    tr.local.neg_z = (j.neg >> 2) & 1u;
+    J.nodes = (const uint32_t *)(uintptr_t)(((u64)j.nodes_hi << 32) | j.nodes_lo);
+    J.tris = (const StDTriHot *)(uintptr_t)(((u64)j.tris_hi << 32) | j.tris_lo);
+    J.c0 = __uint_as_float(j.c0);
+    J.T = j.T;
+    J.root_first = j.root_first;
+    J.any_hit = j.any_hit;
+    J.max_steps = j.max_steps;
+    J.stack_size = j.stack_size;
+    J.seq = seq;
+    J.valid = 1;
+    tr.mesh_stack_size = j.stack_size;
+}
+
+/* Work of a job, per wave. Each lane holds at most one task: a subtree
+   (node pair, path key, path maximum, depth) that it traverses with the
+   reference's loop, one step per iteration. Every CO_BAL iterations the
+   wave rebalances: lanes without a task take the oldest stack entry (the
+   largest subtree left, visited last) of lanes that have one, through LDS;
+   a wave that still has many idle lanes takes a batch of tasks from the
+   group's queue, and a wave with no idle lane hands a batch of oldest
+   entries to the queue while it is short. The queue's shared words change
+   once per batch. The leader's first wave starts with the root. */
+#define CO_BAL 4u                 /* iterations between rebalances */
+#define CO_POP_MIN 16u            /* idle lanes that make a wave take tasks from the queue */
+
+/* a task: node | depth << CO_NODE_BITS, path maximum, key */
+struct DTask {
+    uint32_t nd;
+    float pm;
+    DKey k;
+};
+/* a wave's task exchange area in LDS (6 words per lane) */
+DFN volatile uint32_t *co_ex_slot(uint32_t i) { return co_blk.ex[threadIdx.x >> 6] + 6u * i; }
+
+/* One wave's share of a job (every lane of the wave calls it; the wave
+   leaves together). Results are merged into `res`; returns true when the
+   wave took part in the job. */
+DFN bool co_work(DTrace &tr, DCoopGroup *G, u64 *queue, uint32_t q_cap, const DCoopJobR &J, DCoopRes &res,
+                 bool root) {
+    DHit &closest = tr.closest_mesh_hit;
+    uint32_t *stack = tr.mesh_stack, *pstack = tr.aux_stack;
+    const uint32_t lane = threadIdx.x & 63u;
+    bool has = false, it_found = false, err = false;
+    bool wave_busy = root;            /* counted in the record's busy waves */
+    bool took = root;
+    uint32_t left = 0, sp = 0, steps = 0, dep = 0, idle_rounds = 0;
+    float pm = 0.0f, it_pm = 0.0f;
+    DKey key = {0, 0}, ks = {0, 0};
     DNodePair np = {};
-    for (;;) {
-        if (phase == CLAIM) {
-            u = atomicAdd(&cs.next, 1u);
-            if (u >= units || vs.status) break;
-            phase = DESCEND;
-            d = key = 0;
-            pm = 0.0f;
-            left = root_first;
-            np = load_pair(nodes, left);
+    auto start = [&](uint32_t nd, float p, DKey k) {
+        has = true;
+        left = nd & ((1u << CO_NODE_BITS) - 1u);
+        dep = nd >> CO_NODE_BITS;
+        pm = p;
+        key = ks = k;
+        sp = steps = 0;
+        it_found = false;
+        closest.distance = J.c0;
+        tr.current_hit.distance = __uint_as_float(ST_DIST_UNSET);
+        np = load_pair(J.nodes, left);
+    };
+    auto finish = [&]() {
+#ifdef CO_STATS
+        {
+            uint32_t *sw = G->pad;
+            co_add(sw + 0, steps);
+            co_add(sw + 1, 1u);
+            (void)__hip_atomic_fetch_max((CO_G uint32_t *)(sw + 2), steps, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
-        float ld, rd;
-        const bool hl = hit_aabb(np.lb, ray, c0, ld);
-        const bool hr = hit_aabb(np.rb, ray, c0, rd);
-        const float pml = ld > pm ? ld : pm, pmr = rd > pm ? rd : pm;
-        uint32_t f0 = 0, n0 = 0, f1 = 0, n1 = 0;
-        bool next = false, unit_end = false;
-        if (phase == DESCEND) {
-            const uint32_t below = D - 1u - d;
-            const uint32_t bit = (u >> below) & 1u;
-            const uint32_t cnt = bit ? np.rcount : np.lcount, first = bit ? np.rfirst : np.lfirst;
-            if (!(bit ? hr : hl)) {
-                /* every unit below this box is empty */
-                atomicMax(&cs.next, ((u >> below) + 1u) << below);
-                phase = CLAIM;
-                continue;
-            }
-            const uint32_t first_inner = (!any_hit && ld > rd) ? 1u : 0u;
-            key = key * 4u + (cnt ? bit : (bit == first_inner ? 2u : 3u));
-            pm = bit ? pmr : pml;
-            if (cnt) {
-                /* the leaf belongs to the first unit of its range */
-                atomicMax(&cs.next, ((u >> below) + 1u) << below);
-                if (u & ((1u << below) - 1u)) {
-                    phase = CLAIM;
-                    continue;
-                }
-                key <<= 2u * below;
-                closest.distance = c0;
-                tr.current_hit.distance = __uint_as_float(ST_DIST_UNSET);
-                u_found = false;
-                f0 = first;
-                n0 = cnt;
-                unit_end = true;
-            } else {
-                left = first;
-                next = true;
-                if (++d == D) {
-                    phase = TRAVERSE;
-                    sp = steps = 0;
-                    closest.distance = c0;
-                    tr.current_hit.distance = __uint_as_float(ST_DIST_UNSET);
-                    u_found = false;
-                }
-            }
+#endif
+        DCoopRes o = {};
+        if (it_found) {
+            o.found = 1;
+            o.t = closest.distance;
+            o.tri = closest.object_id;
+            o.pm = it_pm;
+            o.k = ks;
+            if (J.any_hit) co_min64(&G->first_hi, ks.hi);
         } else {
-            if (++steps > max_steps) {
-                atomicOr(&cs.status, 2u);
-                break;
+            const float cd = tr.current_hit.distance;
+            if (__float_as_uint(cd) != ST_DIST_UNSET) {
+                o.lfound = 1;
+                o.lk = ks;
+                o.ldist = cd;
             }
-            if (any_hit && (steps & 31u) == 0 && vs.first_key < key) {
-                /* a unit earlier in key order has a hit: this one is moot */
-                phase = CLAIM;
-                continue;
+        }
+        res_merge(res, o, J.any_hit != 0);
+        has = false;
+    };
+    /* the oldest stack entry as a task of its own (removed from the stack) */
+    auto take_oldest = [&](DTask &t) {
+        const uint32_t e = stack[0];
+        const uint32_t de = e >> CO_NODE_BITS;
+        t.nd = e;
+        t.pm = __uint_as_float(pstack[0]);
+        t.k = key_digit(key_trunc(key, de - 1u), de - 1u, 3u);
+        for (uint32_t i = 1; i < sp; i++) {
+            stack[ST_STACK_STRIDE * (i - 1u)] = stack[ST_STACK_STRIDE * i];
+            pstack[ST_STACK_STRIDE * (i - 1u)] = pstack[ST_STACK_STRIDE * i];
+        }
+        sp--;
+    };
+    if (root && lane == 0) start(J.root_first, 0.0f, DKey{0, 0});
+    for (uint32_t it = 1;; it++) {
+        if (has) {
+            /* one step of the reference's loop (mesh.h:38-111): both
+               children's boxes, the leaf children's triangles, then descent */
+            float ld, rd;
+            const bool hl = hit_aabb(np.lb, tr.local, J.c0, ld);
+            const bool hr = hit_aabb(np.rb, tr.local, J.c0, rd);
+            const float pml = ld > pm ? ld : pm, pmr = rd > pm ? rd : pm;
+            uint32_t f0 = 0, n0 = 0, f1 = 0, n1 = 0;
+            bool next = false, end = false;
+            if (++steps > J.max_steps) {
+                co_or(&G->status, CO_ST_STEPS);
+                err = true;
             }
             if (hl && np.lcount) { f0 = np.lfirst; n0 = np.lcount; }
             if (hr && np.rcount) { f1 = np.rfirst; n1 = np.rcount; }
             const bool il = hl && !np.lcount, ir = hr && !np.rcount;
-            if (il && ir) {
-                uint32_t a = np.lfirst, b = np.rfirst;
-                float pa = pml, pb = pmr;
-                if (!any_hit && ld > rd) {
-                    a = np.rfirst; b = np.lfirst;
-                    pa = pmr; pb = pml;
+            if (il || ir) {
+                if (dep + 1u >= CO_MAX_DIGITS) {
+                    co_or(&G->status, CO_ST_DEEP);
+                    err = true;
                 }
-                if (sp + 1u >= tr.mesh_stack_size) {
-                    atomicOr(&cs.status, 1u);
-                    break;
+                if (il && ir) {
+                    uint32_t a = np.lfirst, b = np.rfirst;
+                    float pa = pml, pb = pmr;
+                    if (!J.any_hit && ld > rd) {
+                        a = np.rfirst; b = np.lfirst;
+                        pa = pmr; pb = pml;
+                    }
+                    if (sp + 1u >= J.stack_size || b >= (1u << CO_NODE_BITS)) {
+                        co_or(&G->status, CO_ST_DEEP);
+                        err = true;
+                    } else {
+                        stack[ST_STACK_STRIDE * sp] = b | ((dep + 1u) << CO_NODE_BITS);
+                        pstack[ST_STACK_STRIDE * sp] = __float_as_uint(pb);
+                        sp++;
+                    }
+                    left = a;
+                    pm = pa;
+                } else {
+                    left = il ? np.lfirst : np.rfirst;
+                    pm = il ? pml : pmr;
                 }
-                stack[ST_STACK_STRIDE * sp] = b;
-                pstack[ST_STACK_STRIDE * sp] = __float_as_uint(pb);
-                sp++;
-                left = a;
-                pm = pa;
-                next = true;
-            } else if (il) {
-                left = np.lfirst;
-                pm = pml;
-                next = true;
-            } else if (ir) {
-                left = np.rfirst;
-                pm = pmr;
+                key = key_digit(key, dep, 2u);
+                dep++;
                 next = true;
             } else if (sp) {
                 sp--;
-                left = stack[ST_STACK_STRIDE * sp];
+                const uint32_t e = stack[ST_STACK_STRIDE * sp];
+                const uint32_t de = e >> CO_NODE_BITS;
+                left = e & ((1u << CO_NODE_BITS) - 1u);
                 pm = __uint_as_float(pstack[ST_STACK_STRIDE * sp]);
+                key = key_digit(key_trunc(key, de - 1u), de - 1u, 3u);
+                dep = de;
                 next = true;
             } else {
-                unit_end = true;
+                end = true;
+            }
+            /* the successor's pair is in flight while this step's triangles are tested */
+            if (next && !err) np = load_pair(J.nodes, left);
+            if (!err && n0 + n1 && hit_triangles_batched(tr, J.tris, f0, n0, f1, n1, J.any_hit)) {
+                it_found = true;
+                it_pm = closest.object_id - f0 < n0 ? pml : pmr;
+                if (J.any_hit) end = true;
+            }
+            if (err) has = false;
+            else if (end) finish();
+        }
+        if (__ballot(err)) break;
+        if (it % CO_BAL) continue;
+        /* any hit: a task after a task that has a hit is moot */
+        if (J.any_hit && (it & 31u) == 0 && has && co_ld64(&G->first_hi) < ks.hi) has = false;
+        /* within the wave: idle lanes take busy lanes' oldest entries */
+        const u64 idle = __ballot(!has);
+        if (idle) {
+            const u64 donors = __ballot(has && sp > 0);
+            const uint32_t ni = (uint32_t)__builtin_popcountll(idle), nd = (uint32_t)__builtin_popcountll(donors);
+            const uint32_t n = ni < nd ? ni : nd;
+            if (n) {
+                const uint32_t rd = __builtin_amdgcn_mbcnt_hi((uint32_t)(donors >> 32),
+                                                              __builtin_amdgcn_mbcnt_lo((uint32_t)donors, 0u));
+                const uint32_t ri = __builtin_amdgcn_mbcnt_hi((uint32_t)(idle >> 32),
+                                                              __builtin_amdgcn_mbcnt_lo((uint32_t)idle, 0u));
+                if (has && sp > 0 && rd < n) {
+                    DTask t;
+                    take_oldest(t);
+                    volatile uint32_t *x = co_ex_slot(rd);
+                    x[0] = t.nd;
+                    x[1] = __float_as_uint(t.pm);
+                    x[2] = (uint32_t)(t.k.hi >> 32);
+                    x[3] = (uint32_t)t.k.hi;
+                    x[4] = (uint32_t)(t.k.lo >> 32);
+                    x[5] = (uint32_t)t.k.lo;
+                }
+                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+                __builtin_amdgcn_wave_barrier();
+                if (!has && ri < n) {
+                    volatile uint32_t *x = co_ex_slot(ri);
+                    const uint32_t a0 = x[0], a1 = x[1], a2 = x[2], a3 = x[3], a4 = x[4], a5 = x[5];
+                    start(a0, __uint_as_float(a1), DKey{((u64)a2 << 32) | a3, ((u64)a4 << 32) | a5});
+                }
+                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+                __builtin_amdgcn_wave_barrier();
             }
         }
-        /* the successor's pair is in flight while this step's triangles are tested */
-        if (next) np = load_pair(nodes, left);
-        if (n0 + n1 && hit_triangles_batched(tr, tris, f0, n0, f1, n1, any_hit)) {
-            u_found = true;
-            u_pm = phase == DESCEND ? pm : (closest.object_id - f0 < n0 ? pml : pmr);
-            if (any_hit) unit_end = true;
-        }
-        if (unit_end) {
-            if (u_found) {
-                const float t = closest.distance;
-                if (any_hit) {
-                    if (key < b_key) {
-                        b_key = key;
-                        b_tri = closest.object_id;
+        const u64 idle2 = __ballot(!has);
+        const uint32_t ni2 = (uint32_t)__builtin_popcountll(idle2);
+        if (ni2 >= CO_POP_MIN) {
+            /* take a batch from the group's queue */
+            uint32_t h = 0, n = 0;
+            if (lane == 0) {
+                const uint32_t hh = co_ld(&G->head), tt = co_ld(&G->tail);
+                if (tt > hh) {
+                    n = tt - hh < ni2 ? tt - hh : ni2;
+                    if (co_cas(&G->head, hh, hh + n)) {
+                        h = hh;
+                        /* counted once it holds tasks (a wave that sees no
+                           busy wave and an empty queue in between leaves
+                           early: this wave does their share) */
+                        if (!wave_busy) co_add(&G->busy, 1u);
+                    } else {
+                        n = 0;
                     }
-                    atomicMin(&cs.first_key, key);
-                } else if (t < b_t || (t == b_t && key < b_key)) {
-                    b_t = t;
-                    b_key = key;
-                    b_tri = closest.object_id;
-                    b_pm = u_pm;
-                }
-                found = true;
-            } else {
-                const float cd = tr.current_hit.distance;
-                if (__float_as_uint(cd) != ST_DIST_UNSET && key + 1u > l_key) {
-                    l_key = key + 1u;
-                    l_dist = cd;
                 }
             }
-            phase = CLAIM;
+            h = (uint32_t)__shfl((int)h, 0, 64);
+            n = (uint32_t)__shfl((int)n, 0, 64);
+            if (n) {
+                wave_busy = took = true;
+                idle_rounds = 0;
+                const uint32_t ri = __builtin_amdgcn_mbcnt_hi((uint32_t)(idle2 >> 32),
+                                                              __builtin_amdgcn_mbcnt_lo((uint32_t)idle2, 0u));
+                if (!has && ri < n) {
+                    uint32_t tnd;
+                    float tpm;
+                    DKey tk;
+                    if (co_pop_item(queue + 4 * (size_t)((h + ri) % q_cap), J.seq, h + ri, J.T, tnd, tpm, tk) > 0) {
+                        start(tnd, tpm, tk);
+                    } else {
+                        co_or(&G->status, CO_ST_LOST);
+                        err = true;
+                    }
+                }
+                if (__ballot(err)) break;
+                continue;
+            }
+        } else if (ni2 == 0 && (it % (4u * CO_BAL)) == 0) {
+            /* no idle lane: hand oldest entries to the queue while it is short */
+            const u64 donors = __ballot(sp > 0);
+            const uint32_t nd = (uint32_t)__builtin_popcountll(donors);
+            if (nd) {
+                uint32_t t = 0, n = 0;
+                if (lane == 0) {
+                    const uint32_t q = co_ld(&G->tail) - co_ld(&G->head);
+                    if (q < CO_EXPORT_LOW) {
+                        n = CO_EXPORT_LOW - q < nd ? CO_EXPORT_LOW - q : nd;
+                        t = co_add(&G->tail, n);
+                    }
+                }
+                t = (uint32_t)__shfl((int)t, 0, 64);
+                n = (uint32_t)__shfl((int)n, 0, 64);
+                const uint32_t rd = __builtin_amdgcn_mbcnt_hi((uint32_t)(donors >> 32),
+                                                              __builtin_amdgcn_mbcnt_lo((uint32_t)donors, 0u));
+                if (sp > 0 && rd < n) {
+                    DTask tk;
+                    take_oldest(tk);
+                    if (!co_push_item(queue + 4 * (size_t)((t + rd) % q_cap), J.seq, t + rd, tk.nd, tk.pm, tk.k)) {
+                        co_or(&G->status, CO_ST_LOST);
+                        err = true;
+                    }
+                }
+                if (__ballot(err)) break;
+            }
+        }
+        if (ni2 == 64u) {
+            /* the whole wave is idle: leave once no wave is busy and nothing
+               is queued (the busy count is read before the queue: a wave
+               queues its entries before it stops being busy) */
+            uint32_t stop = 0;
+            if (lane == 0) {
+                if (wave_busy) co_add(&G->busy, 0xFFFFFFFFu);
+                stop = co_ld(&G->busy) == 0u && co_ld(&G->head) >= co_ld(&G->tail);
+            }
+            wave_busy = false;
+            if (__shfl((int)stop, 0, 64)) break;
+            if (++idle_rounds > CO_SPIN_LIMIT / 16u) {
+                co_or(&G->status, CO_ST_LOST);
+                break;
+            }
+            for (uint32_t k = idle_rounds < 8u ? idle_rounds : 8u; k; k--) __builtin_amdgcn_s_sleep(8);
         }
     }
+    if (wave_busy && lane == 0) co_add(&G->busy, 0xFFFFFFFFu);
+    return took;
+}
+
+/* The leader's side of one cooperative mesh traversal (every thread of the
+   leader block, all on the same ray): posts the job, takes part in it,
+   waits for the helpers that joined, and merges the blocks' results.
+   Returns 1 (hit in tr.closest_mesh_hit), 0 (none), or -1 when the result
+   could not be shown equal to the sequential one (a path deeper than the
+   order key, a stack limit, a lost wait, or a winner whose path is not
+   admitted at t*): the caller then traces sequentially (start state
+   restored). */
+DFN int coop_trace_mesh(DTrace &tr, const StDScene &sc, const StDMesh &mesh, const uint32_t *nodes,
+                        const StDTriHot *tris, uint32_t root_first, bool any_hit) {
+    DHit &closest = tr.closest_mesh_hit;
+    DCoopGroup *G = co_blk.group;
+    const float c0 = closest.distance, cur0 = tr.current_hit.distance;
+    const uint32_t T = mesh.triangle_count;
+    uint32_t max_steps = 2u * T + 2u;
+    if (sc.step_cap && sc.step_cap < max_steps) max_steps = sc.step_cap;
+#ifdef CO_STATS
+    const u64 st_t0 = __builtin_amdgcn_s_memrealtime();
+#endif
+    __syncthreads();  /* the previous job's readers of co_blk are done */
+    if (co_blk.jobs >= CO_MAX_JOBS) return -1;
+    if (threadIdx.x == 0) {
+        const uint32_t seq = co_blk.base + ++co_blk.jobs;
+        uint32_t v[26];
+        DCoopJob &j = *(DCoopJob *)v;
+        const DRay &r = tr.local;
+        j.ro[0] = __float_as_uint(r.origin.x); j.ro[1] = __float_as_uint(r.origin.y); j.ro[2] = __float_as_uint(r.origin.z);
+        j.rd[0] = __float_as_uint(r.direction.x); j.rd[1] = __float_as_uint(r.direction.y);
+        j.rd[2] = __float_as_uint(r.direction.z);
+        j.rr[0] = __float_as_uint(r.direction_reciprocal.x); j.rr[1] = __float_as_uint(r.direction_reciprocal.y);
+        j.rr[2] = __float_as_uint(r.direction_reciprocal.z);
+        j.rs[0] = __float_as_uint(r.scaled_origin.x); j.rs[1] = __float_as_uint(r.scaled_origin.y);
+        j.rs[2] = __float_as_uint(r.scaled_origin.z);
+        j.neg = r.neg_x | (r.neg_y << 1) | (r.neg_z << 2);
+        j.c0 = __float_as_uint(c0);
+        j.nodes_lo = (uint32_t)(uintptr_t)nodes;
+        j.nodes_hi = (uint32_t)((u64)(uintptr_t)nodes >> 32);
+        j.tris_lo = (uint32_t)(uintptr_t)tris;
+        j.tris_hi = (uint32_t)((u64)(uintptr_t)tris >> 32);
+        j.T = T;
+        j.pad0 = j.pad1 = 0;
+        j.root_first = root_first;
+        j.any_hit = any_hit ? 1u : 0u;
+        j.max_steps = max_steps;
+        j.stack_size = tr.mesh_stack_size;
+        j.seq = seq;
+        uint32_t *p = (uint32_t *)&G->job;
+#pragma unroll
+        for (int i = 0; i < 26; i++) co_st(p + i, v[i]);
+        co_st(&G->status, 0);
+        co_st(&G->head, 0);
+        co_st(&G->tail, 0);
+        co_st(&G->busy, 1u);   /* the leader's first wave starts with the root */
+        co_st64(&G->first_hi, ~0ull);
+#ifdef CO_STATS
+        for (int i = 0; i < 6; i++) co_st(G->pad + i, 0);
+#endif
+        /* the job and the reset words are in place before the claim word opens it */
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        co_st64(&G->claim, (u64)seq << 32);
+        /* ... and the claim word is written before the block's threads claim from it */
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        co_blk.seq = seq;
+    }
     __syncthreads();
+    DCoopJobR J;
+    J.nodes = nodes;
+    J.tris = tris;
+    J.c0 = c0;
+    J.T = T;
+    J.root_first = root_first;
+    J.any_hit = any_hit ? 1u : 0u;
+    J.max_steps = max_steps;
+    J.stack_size = tr.mesh_stack_size;
+    J.seq = co_blk.seq;
+    J.valid = 1;
+    DCoopRes res = {};
+    (void)co_work(tr, G, co_blk.queue, co_blk.q_cap, J, res, threadIdx.x < 64u);
+    const DCoopRes mine = res_block_reduce(res, any_hit);
+    if (threadIdx.x == 0) {
+        /* no unit can be claimed from here on; wait for the helpers that
+           joined (each of them is running) */
+        co_max64(&G->claim, ((u64)J.seq << 32) | CO_CLOSED);
+        uint32_t st = 0;
+        for (uint32_t spins = 0; co_ld(&G->inflight) != 0u; spins++) {
+            if (spins > CO_SPIN_LIMIT) {
+                st |= CO_ST_LOST;
+                break;
+            }
+            co_sleep();
+        }
+        if (co_ld(&G->head) < co_ld(&G->tail)) st |= CO_ST_LOST;   /* a task nobody took */
+        co_blk.status = st | co_ld(&G->status);
+    }
+    __syncthreads();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    /* the helpers' results */
+    DCoopRes all = {};
+    if (threadIdx.x == 0) all = mine;
+    for (uint32_t b = 1u + threadIdx.x; b < co_blk.gsize; b += blockDim.x) {
+        DCoopRes o;
+        if (co_get_slot(co_blk.slots + b, J.seq, o)) res_merge(all, o, any_hit);
+    }
+    const DCoopRes best = res_block_reduce(all, any_hit);
+    const uint32_t status = co_blk.status;
+#ifdef CO_STATS
+    if (threadIdx.x == 0) {
+        uint32_t joined = 0;
+        for (uint32_t b = 1u; b < co_blk.gsize; b++) joined += co_ld((const uint32_t *)(co_blk.slots + b)) == J.seq;
+        co_stat(1ull | ((u64)(blockIdx.x & 0xFFFFFFu) << 8) | ((u64)co_blk.jobs << 32), st_t0,
+                __builtin_amdgcn_s_memrealtime(),
+                (u64)T | ((u64)status << 32) | ((u64)best.found << 36) | ((u64)any_hit << 37) |
+                    ((u64)(co_ld(&G->tail) & 0xFFFFu) << 40) | ((u64)(joined & 0xFFu) << 56));
+        co_stat(4ull | ((u64)(blockIdx.x & 0xFFFFFFu) << 8) | ((u64)co_blk.jobs << 32),
+                (u64)co_ld(G->pad + 0) | ((u64)co_ld(G->pad + 1) << 32), (u64)co_ld(G->pad + 2) | ((u64)co_ld(G->pad + 3) << 32),
+                (u64)best.pm | 0);
+    }
+#endif
     closest.distance = c0;
     tr.current_hit.distance = cur0;
-    const uint32_t status = vs.status;
-    if (status) {
-        if (status & 2u) {
-            trav_fault(sc, ST_FAULT_MESH_STEPS);
-            return 0;
-        }
-        return -1;
+    if (status & CO_ST_STEPS) {
+        trav_fault(sc, ST_FAULT_MESH_STEPS);
+        return 0;
     }
-    const unsigned long long mine =
-        any_hit ? (unsigned long long)b_key : ((unsigned long long)__float_as_uint(b_t) << 32) | b_key;
-    if (found) atomicMin(&cs.best, mine);
-    if (l_key) atomicMax(&cs.last_key, l_key);
-    __syncthreads();
-    const unsigned long long best = vs.best;
-    const uint32_t lk = vs.last_key;
-    if (found && mine == best) {
-        cs.tri = b_tri;
-        cs.pm = b_pm;
-    } else if (best == ~0ull && lk && l_key == lk) {
-        cs.last_dist = l_dist;
-    }
-    __syncthreads();
-    if (best != ~0ull) {
-        const uint32_t tri = vs.tri;
-        const float pm = vs.pm;
-        if (!any_hit && !(pm <= __uint_as_float((uint32_t)(best >> 32)))) return -1;
-        if (hit_triangles(tr, tris, tri, 1u, 0, 0, any_hit)) return 1;
+    if (status) return -1;
+    if (best.found) {
+        if (!any_hit && !(best.pm <= best.t)) return -1;
+        if (hit_triangles(tr, tris, best.tri, 1u, 0, 0, any_hit)) return 1;
         closest.distance = c0;
         tr.current_hit.distance = cur0;
         return -1;
     }
-    if (lk) tr.current_hit.distance = vs.last_dist;
+    if (best.lfound) tr.current_hit.distance = best.ldist;
     return 0;
+}
+
+/* A helper block of a redo group (every thread): waits for the leader's
+   jobs, takes part in each, and leaves its result in its slot; returns
+   once the leader has no more pixels (or no job came for a long time). */
+DFN void coop_helper(uint32_t *mesh_stack, uint32_t *aux_stack) {
+    DCoopGroup *G = co_blk.group;
+    uint32_t last = co_blk.base;   /* thread 0: the latest job seen */
+    for (;;) {
+        if (threadIdx.x == 0) {
+            uint32_t cmd = 2;
+            /* a helper that sees no job for CO_HELPER_IDLE_TICKS leaves, so
+               that its CU goes to other work (the leader needs no helper) */
+            const u64 t_wait = __builtin_amdgcn_s_memrealtime();
+            for (uint32_t spins = 0; spins <= CO_SPIN_LIMIT; spins++) {
+                if (co_ld(&G->quit) == co_blk.base) break;
+                if (__builtin_amdgcn_s_memrealtime() - t_wait > CO_HELPER_IDLE_TICKS) break;
+                const u64 c = co_ld64(&G->claim);
+                const uint32_t s = (uint32_t)(c >> 32);
+                if (s - co_blk.base - 1u < CO_MAX_JOBS && s > last && (uint32_t)c < CO_CLOSED) {
+                    /* join, then check that the job is still open: if so, the
+                       leader's wait at its end counts this block and the
+                       record holds this job until then; if not, the block
+                       leaves the job's words alone */
+                    co_add(&G->inflight, 1u);
+                    const u64 c2 = co_ld64(&G->claim);
+                    last = s;
+                    if ((uint32_t)(c2 >> 32) == s && (uint32_t)c2 < CO_CLOSED) {
+                        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+                        const uint32_t *p = (const uint32_t *)&G->job;
+                        for (int i = 0; i < 26; i++) co_blk.jw[i] = co_ld(p + i);
+                        if (co_blk.jw[25] == s) {
+                            co_blk.seq = s;
+                            cmd = 1;
+                            break;
+                        }
+                        co_or(&G->status, CO_ST_LOST);
+                    }
+                    co_add(&G->inflight, 0xFFFFFFFFu);
+                }
+                co_sleep();
+            }
+            co_blk.cmd = cmd;
+        }
+        __syncthreads();
+        if (co_blk.cmd != 1u) break;
+        DTrace tr = DTrace{};
+        tr.closest_mesh_hit.object_type = ST_PRIM_MESH;
+        tr.mesh_stack = mesh_stack;
+        tr.aux_stack = aux_stack;
+        DCoopJobR J;
+        co_job_from_words(co_blk.jw, co_blk.seq, J, tr);
+        DCoopRes res = {};
+        const bool worked = co_work(tr, G, co_blk.queue, co_blk.q_cap, J, res, false);
+        const uint32_t any_worked = __syncthreads_or(worked);
+        const DCoopRes r = res_block_reduce(res, J.any_hit != 0u);
+        if (threadIdx.x == 0) {
+            if (any_worked) co_put_slot(co_blk.slots + co_blk.rank, J.seq, r);
+            /* the slot is written before the leader can see this block leave */
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+            co_add(&G->inflight, 0xFFFFFFFFu);
+        }
+        __syncthreads();
+    }
 }
 
 /* intersection/mesh.h:38-111. A leaf root holds every triangle; otherwise
@@ -913,6 +1485,9 @@ DNOINLINE bool trace_mesh(DTrace &tr, const StDScene &sc, uint32_t mesh_id, bool
     if (COOP && !root_leaf && mesh.triangle_count >= sc.coop_min_tris) {
         const int r = coop_trace_mesh(tr, sc, mesh, nodes, tris, root_first, any_hit);
         if (r >= 0) return r != 0;
+#ifdef CO_STATS
+        if (threadIdx.x == 0) co_stat(3ull | ((u64)blockIdx.x << 8), __builtin_amdgcn_s_memrealtime(), 0, T_STAT(mesh));
+#endif
         /* r < 0: the cooperative result could not be shown equal to the
            sequential one; trace sequentially */
     }

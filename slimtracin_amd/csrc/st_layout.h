@@ -109,7 +109,7 @@ typedef struct StDScene {
     uint32_t main_step_limit;      /* per launch: mesh traversals of the main stages give up after this
                                       many steps and their pixel is redone (ST_ABANDON_STEPS) */
     uint32_t coop_min_tris;        /* per launch: the redo stage splits traversals of meshes of at least
-                                      this many triangles across the wave (ST_COOP_MIN_TRIS) */
+                                      this many triangles across a group of blocks (ST_COOP_MIN_TRIS) */
     uint32_t pad_;
 } StDScene;
 
@@ -118,10 +118,10 @@ typedef struct StDScene {
    fetches (a ray with an exactly zero object-space direction component
    makes the reference's slab test admit every box on one side of the mesh,
    hundreds of thousands of steps). Its pixel is redone from the start by
-   the redo stage, where the whole wave traces it together. */
+   the redo stage, where a group of blocks traces it together. */
 #define ST_ABANDON_STEPS 2048u
-/* Meshes from this size are traversed by the whole wave in the redo stage;
-   smaller ones sequentially (a few hundred steps at most). */
+/* Meshes from this size are traversed by the pixel's group of blocks in the
+   redo stage; smaller ones sequentially (a few hundred steps at most). */
 #define ST_COOP_MIN_TRIS 4096u
 
 /* Fault bits of StDScene.fault. A traversal of a valid tree never sets them:

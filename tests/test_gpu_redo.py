@@ -1,9 +1,9 @@
 """GPU: the redo stage. A mesh traversal of the main stages that runs past
 its step limit gives up, and the redo stage traces its pixel again from the
-primary ray with every thread of a block, splitting each traversal of a
-large mesh across the threads (coop_trace_mesh: the reference's traversal
-order rebuilt from the units the threads claim, st_trace.h). Frames must not depend
-on it: with a give-up limit of one step and a cooperative-mesh minimum of 16
+primary ray with a group of blocks, splitting each traversal of a large
+mesh across the group's threads (coop_trace_mesh and coop_helper: the
+reference's traversal order rebuilt from the units and exported subtrees the
+threads claim, st_trace.h). Frames must not depend on it: with a give-up limit of one step and a cooperative-mesh minimum of 16
 triangles (st_debug_redo), nearly every pixel goes through the redo stage and
 every mesh traversal in it through the cooperative path, and frames, hit
 records and ray counts still equal the CPU oracle's, closest and any-hit,
@@ -70,6 +70,26 @@ def test_mesh_scene_all_redone(redo_everything, mode, lbvh):
     assert _check(blob, view, f"mesh20k/m{mode}/lbvh{lbvh}", lbvh=lbvh, want_blob=want) > 1000
 
 
+@pytest.mark.parametrize("lbvh", [False, True])
+def test_small_views_redone_by_block_groups(redo_everything, lbvh):
+    """Views of a few pixels: every pixel is redone by a group of many
+    blocks, so each large-mesh traversal is shared by the leader block and
+    helper blocks (static units, exported subtrees), closest and any-hit."""
+    blob = st.gen_mesh_scene(31, 20000)
+    want = blob
+    if lbvh:
+        T = blob_mesh_info(blob)[0]["triangle_count"]
+        want = blob_with_prebuilt_meshes(blob, [ck.oracle_mesh_lbvh(blob, 0, T)])
+    for w, h, mode in [(1, 1, MODE_BEAUTY), (3, 2, MODE_BEAUTY), (4, 4, MODE_NORMALS), (7, 3, MODE_DEPTH),
+                       (2, 5, MODE_UVS)]:
+        view = make_view(width=w, height=h, mode=mode, depth=5, position=(4.0, 7.0, -12.0), yaw=0.15, pitch=-0.25)
+        assert _check(blob, view, f"small{w}x{h}/m{mode}/lbvh{lbvh}", lbvh=lbvh, want_blob=want) > 0
+    s, cam = scenes.ex07_monkey()
+    for w, h, mode in [(2, 2, MODE_BEAUTY), (5, 3, MODE_NORMALS)]:
+        view = make_view(width=w, height=h, mode=mode, **dict(cam, depth=4))
+        assert _check(s.to_blob(), view, f"monkey{w}x{h}/m{mode}") > 0
+
+
 def test_pbr_and_crowd_all_redone(redo_everything):
     blob = st.gen_pbr(5, 12)
     view = make_view(width=128, height=72, mode=MODE_BEAUTY, depth=4, position=(0.0, 6.0, -14.0), pitch=-0.15)
@@ -89,7 +109,7 @@ def test_textured_all_redone(redo_everything):
 def test_degenerate_rays_take_the_redo_stage():
     """Rays with an exactly zero direction component through an unrotated
     mesh (the slab test admits every box on one side) give up at the default
-    limit and are traced by whole waves; the frame equals the oracle's."""
+    limit and are traced by groups of blocks; the frame equals the oracle's."""
     blob = st.gen_mesh_scene(5, 60000)
     prims = blob_prims(blob)
     prims[0]["rotation"] = (0.0, 0.0, 0.0, 1.0)
