@@ -19,6 +19,7 @@ struct StSceneHandle {
     uint64_t dbytes;
     float *d_mesh_aabbs;       /* 6 floats per mesh, for device BVH rebuilds */
     char *lbvh_work;           /* scratch of device LBVH rebuilds */
+    size_t lbvh_work_bytes;    /* its size */
     uint32_t *d_height;        /* height written by the device builder */
     int bvh_on_device_only;    /* host copy of the scene BVH is stale */
     struct StSeamState *seam;  /* host copies of a scene fed through slimtracin_seam.h */

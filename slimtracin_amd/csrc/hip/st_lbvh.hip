@@ -4,39 +4,31 @@
  * The reference rebuilds the scene BVH on the CPU with its SAH builder
  * whenever a primitive moves (updateSceneBVH, builder_top_down.h:366-381).
  * Here the rebuild runs on the GPU as a linear BVH: 30-bit Morton codes of
- * the primitives' world-box centroids, a sort of the unique 64-bit keys
- * (code << 32 | original index), the binary radix tree of Karras 2012
- * ("Maximizing parallelism in the construction of BVHs, octrees and k-d
- * trees") built with one lane per internal node, and output in the
- * reference's node layout so the traversal is unchanged:
+ * the primitives' world-box centroids, a stable radix sort of (code,
+ * primitive index) pairs (st_sort.hip, so equal codes keep index order: the
+ * order of the unique keys code << 32 | index), the binary radix tree of
+ * Karras 2012 ("Maximizing parallelism in the construction of BVHs, octrees
+ * and k-d trees") with one lane per internal node, boxes bottom-up, and
+ * output in the reference's node layout so the traversal is unchanged:
  *   node 0 = root; the children of radix-tree internal node k are nodes
  *   2k+1 and 2k+2 (adjacent, as the traversal requires); a leaf holds one
  *   primitive (child_count 1, first_child_id = its sorted position, and
  *   leaf_ids[sorted position] = primitive index); internal boxes are
- *   merge(left, right), filled level by level from the deepest level up;
- *   depth = tree depth.
+ *   merge(left, right); depth = tree depth.
  * The primitive boxes are the reference's leaf boxes (getPrimitiveAABB or
- * the mesh AABB, transformed by transformAABB, render/AABB.h:44-84).
- *
- * One workgroup of 1024 lanes builds scene BVHs of up to ST_LBVH_MAX
- * primitives with all sorting in LDS. The CPU oracle restates the same
+ * the mesh AABB, transformed by transformAABB, render/AABB.h:44-84). Steps
+ * after the boxes are the mesh LBVH's (st_lbvh_tree, st_mesh_lbvh.hip) with
+ * leaves of one primitive; every kernel is multi-block, so the primitive
+ * count is not limited by a workgroup. The CPU oracle restates the same
  * specification with a top-down construction; tests compare the nodes bit
  * for bit.
  */
 #include <hip/hip_runtime.h>
 
 #include "st_lbvh.h"
+#include "st_mesh_lbvh.h"
 
-#define LBVH_THREADS 1024
-
-__device__ static inline uint32_t expand10(uint32_t v) {
-    v &= 0x3FFu;
-    v = (v | (v << 16)) & 0x030000FFu;
-    v = (v | (v << 8)) & 0x0300F00Fu;
-    v = (v | (v << 4)) & 0x030C30C3u;
-    v = (v | (v << 2)) & 0x09249249u;
-    return v;
-}
+#define LBVH_THREADS 256u
 
 /* transformAABB of the primitive's object-space box (render/AABB.h:44-84) */
 __device__ static sbox prim_world_box(const StDPrim &p, const float *mesh_aabbs, uint32_t mesh_count) {
@@ -76,39 +68,25 @@ __device__ static sbox prim_world_box(const StDPrim &p, const float *mesh_aabbs,
     return bx;
 }
 
-/* common-prefix length of the unique keys at sorted positions i and j;
-   -1 outside [0, n) */
-__device__ static inline int lcp(const uint32_t *codes, int n, int i, int j) {
-    if (j < 0 || j >= n) return -1;
-    uint32_t a = codes[i], b = codes[j];
-    if (a == b) return 32 + __clz((uint32_t)i ^ (uint32_t)j);
-    return __clz(a ^ b);
-}
-
-__global__ void __launch_bounds__(LBVH_THREADS) k_lbvh_small(const StDPrim *prims, const float *mesh_aabbs,
-                                                               uint32_t mesh_count, uint32_t n, StLbvhWork work,
-                                                               uint32_t *nodes_out, uint32_t *leaf_ids_out,
-                                                               uint32_t *height_out) {
-    __shared__ unsigned long long keys[ST_LBVH_MAX];
+/* world boxes, centroids (-0 folded to +0) and per-block centroid bounds */
+__global__ void __launch_bounds__(LBVH_THREADS) k_sl_boxes(const StDPrim *prims, const float *mesh_aabbs,
+                                                           uint32_t mesh_count, uint32_t n, sbox *boxes, float *cent,
+                                                           float *partials) {
     __shared__ float red[6][LBVH_THREADS];
-    __shared__ uint32_t s_max_depth;
-    const uint32_t tid = threadIdx.x;
-    sbox *boxes = reinterpret_cast<sbox *>(work.boxes);
-
-    /* 1. world boxes, centroids (-0 folded to +0) and their bounds */
-    float mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
-    for (uint32_t i = tid; i < n; i += LBVH_THREADS) {
-        sbox b = prim_world_box(prims[i], mesh_aabbs, mesh_count);
+    const uint32_t tid = threadIdx.x, i = blockIdx.x * LBVH_THREADS + tid;
+    float c[3] = {INFINITY, INFINITY, INFINITY}, C[3] = {-INFINITY, -INFINITY, -INFINITY};
+    if (i < n) {
+        const sbox b = prim_world_box(prims[i], mesh_aabbs, mesh_count);
         boxes[i] = b;
-        float c[3] = {(b.min.x + b.max.x) * 0.5f + 0.0f, (b.min.y + b.max.y) * 0.5f + 0.0f,
-                      (b.min.z + b.max.z) * 0.5f + 0.0f};
+        const float m[3] = {(b.min.x + b.max.x) * 0.5f + 0.0f, (b.min.y + b.max.y) * 0.5f + 0.0f,
+                            (b.min.z + b.max.z) * 0.5f + 0.0f};
         for (int a = 0; a < 3; a++) {
-            work.cent[3 * i + a] = c[a];
-            mn[a] = c[a] < mn[a] ? c[a] : mn[a];
-            mx[a] = c[a] > mx[a] ? c[a] : mx[a];
+            cent[3 * (size_t)i + a] = m[a];
+            c[a] = m[a];
+            C[a] = m[a];
         }
     }
-    for (int a = 0; a < 3; a++) { red[a][tid] = mn[a]; red[3 + a][tid] = mx[a]; }
+    for (int a = 0; a < 3; a++) { red[a][tid] = c[a]; red[3 + a][tid] = C[a]; }
     __syncthreads();
     for (uint32_t s = LBVH_THREADS / 2; s > 0; s >>= 1) {
         if (tid < s)
@@ -120,130 +98,28 @@ __global__ void __launch_bounds__(LBVH_THREADS) k_lbvh_small(const StDPrim *prim
             }
         __syncthreads();
     }
-    const float cmin[3] = {red[0][0], red[1][0], red[2][0]};
-    float scale[3];
-    for (int a = 0; a < 3; a++) {
-        float e = red[3 + a][0] - red[a][0];
-        scale[a] = e > 0 ? 1023.0f / e : 0.0f;
-    }
+    if (tid < 6) partials[6 * (size_t)blockIdx.x + tid] = red[tid][0];
+}
 
-    /* 2. keys = code << 32 | index, padded to a power of two */
-    uint32_t m = 1;
-    while (m < n) m <<= 1;
-    for (uint32_t i = tid; i < m; i += LBVH_THREADS) {
-        if (i < n) {
-            uint32_t q[3];
-            for (int a = 0; a < 3; a++) {
-                float t = (work.cent[3 * i + a] - cmin[a]) * scale[a];
-                t = t < 0 ? 0 : t;
-                t = t > 1023.0f ? 1023.0f : t;
-                q[a] = (uint32_t)t;
-            }
-            uint32_t code = (expand10(q[0]) << 2) | (expand10(q[1]) << 1) | expand10(q[2]);
-            keys[i] = ((unsigned long long)code << 32) | i;
-        } else
-            keys[i] = ~0ull;
-    }
-    __syncthreads();
+static size_t r256(size_t b) { return (b + 255) & ~(size_t)255; }
 
-    /* 3. bitonic sort of the unique keys */
-    for (uint32_t size = 2; size <= m; size <<= 1)
-        for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
-            for (uint32_t i = tid; i < m; i += LBVH_THREADS) {
-                uint32_t j = i ^ stride;
-                if (j > i) {
-                    bool up = (i & size) == 0;
-                    unsigned long long a = keys[i], b = keys[j];
-                    if ((a > b) == up) { keys[i] = b; keys[j] = a; }
-                }
-            }
-            __syncthreads();
-        }
-    for (uint32_t i = tid; i < n; i += LBVH_THREADS) {
-        work.codes[i] = (uint32_t)(keys[i] >> 32);
-        leaf_ids_out[i] = (uint32_t)keys[i];
-    }
-    if (tid == 0) s_max_depth = 0;
-    __syncthreads();
-
-    if (n == 1) {
-        if (tid == 0) {
-            sbox b = boxes[0];
-            uint32_t *o = nodes_out;
-            o[0] = __float_as_uint(b.min.x); o[1] = __float_as_uint(b.min.y); o[2] = __float_as_uint(b.min.z);
-            o[3] = __float_as_uint(b.max.x); o[4] = __float_as_uint(b.max.y); o[5] = __float_as_uint(b.max.z);
-            o[6] = 0; o[7] = 1;
-            *height_out = 1;
-        }
-        return;
-    }
-
-    /* 4. radix tree: one lane per internal node (Karras 2012, section 4) */
-    const int N = (int)n;
-    for (int k = (int)tid; k < N - 1; k += LBVH_THREADS) {
-        int d = (lcp(work.codes, N, k, k + 1) - lcp(work.codes, N, k, k - 1)) >= 0 ? 1 : -1;
-        int dmin = lcp(work.codes, N, k, k - d);
-        int lmax = 2;
-        while (lcp(work.codes, N, k, k + lmax * d) > dmin) lmax <<= 1;
-        int l = 0;
-        for (int t = lmax >> 1; t >= 1; t >>= 1)
-            if (lcp(work.codes, N, k, k + (l + t) * d) > dmin) l += t;
-        int j = k + l * d;
-        int dnode = lcp(work.codes, N, k, j);
-        int s = 0, t = l;
-        do {
-            t = (t + 1) >> 1;
-            if (lcp(work.codes, N, k, k + (s + t) * d) > dnode) s += t;
-        } while (t > 1);
-        int g = k + s * d + (d < 0 ? -1 : 0);
-        int first = k < j ? k : j, last = k < j ? j : k;
-        /* children -> output slots 2k+1 (left) and 2k+2 (right) */
-        uint32_t lo = 2u * k + 1u, ro = 2u * k + 2u;
-        work.parent[lo] = (uint32_t)k;
-        work.parent[ro] = (uint32_t)k;
-        if (first == g) { work.kind[lo] = 1u; work.first[lo] = (uint32_t)g; }
-        else { work.kind[lo] = 0u; work.first[lo] = 2u * g + 1u; work.slot[g] = lo; }
-        if (g + 1 == last) { work.kind[ro] = 1u; work.first[ro] = (uint32_t)(g + 1); }
-        else { work.kind[ro] = 0u; work.first[ro] = 2u * (g + 1) + 1u; work.slot[g + 1] = ro; }
-    }
-    if (tid == 0) { work.slot[0] = 0; work.kind[0] = 0; work.first[0] = 1; work.parent[0] = 0xFFFFFFFFu; }
-    __syncthreads();
-
-    /* 5. depths (walk to the root) and the deepest level */
-    const uint32_t count = 2u * n - 1u;
-    for (uint32_t o = tid; o < count; o += LBVH_THREADS) {
-        uint32_t dpt = 0, x = o;
-        while (x != 0) { x = work.slot[work.parent[x]]; dpt++; }
-        work.depth[o] = dpt;
-        atomicMax(&s_max_depth, dpt);
-    }
-    __syncthreads();
-    const uint32_t max_depth = s_max_depth;
-
-    /* 6. boxes, deepest level first */
-    for (int level = (int)max_depth; level >= 0; level--) {
-        for (uint32_t o = tid; o < count; o += LBVH_THREADS) {
-            if (work.depth[o] != (uint32_t)level) continue;
-            sbox b;
-            if (work.kind[o]) b = boxes[leaf_ids_out[work.first[o]]];
-            else b = box_merge(work.node_boxes[work.first[o]], work.node_boxes[work.first[o] + 1]);
-            work.node_boxes[o] = b;
-            uint32_t *w = nodes_out + 8 * o;
-            w[0] = __float_as_uint(b.min.x); w[1] = __float_as_uint(b.min.y); w[2] = __float_as_uint(b.min.z);
-            w[3] = __float_as_uint(b.max.x); w[4] = __float_as_uint(b.max.y); w[5] = __float_as_uint(b.max.z);
-            w[6] = work.first[o];
-            w[7] = (work.kind[o] ? 1u : 0u) | (work.depth[o] << 16);
-        }
-        __syncthreads();
-    }
-    if (tid == 0) *height_out = max_depth + 1;
+size_t st_lbvh_work_bytes(uint32_t n) {
+    const size_t nb = (n + LBVH_THREADS - 1) / LBVH_THREADS;
+    return r256(24ull * n) + r256(12ull * n) + r256(24ull * nb) + r256(4ull * n) + st_lbvh_tree_work_bytes(n);
 }
 
 extern "C" int st_lbvh_launch(hipStream_t stream, const StDPrim *prims, const float *mesh_aabbs, uint32_t mesh_count,
-                              uint32_t n, StLbvhWork work, uint32_t *nodes_out, uint32_t *leaf_ids_out,
+                              uint32_t n, char *work, uint32_t *nodes_out, uint32_t *leaf_ids_out,
                               uint32_t *height_out) {
-    if (n == 0 || n > ST_LBVH_MAX) return -1;
-    hipLaunchKernelGGL(k_lbvh_small, dim3(1), dim3(LBVH_THREADS), 0, stream, prims, mesh_aabbs, mesh_count, n, work,
-                       nodes_out, leaf_ids_out, height_out);
-    return hipGetLastError() == hipSuccess ? 0 : -2;
+    if (n == 0) return -1;
+    const uint32_t nb = (n + LBVH_THREADS - 1) / LBVH_THREADS;
+    char *p = work;
+    sbox *boxes = (sbox *)p; p += r256(24ull * n);
+    float *cent = (float *)p; p += r256(12ull * n);
+    float *partials = (float *)p; p += r256(24ull * nb);
+    uint32_t *keys = (uint32_t *)p; p += r256(4ull * n);
+    hipLaunchKernelGGL(k_sl_boxes, dim3(nb), dim3(LBVH_THREADS), 0, stream, prims, mesh_aabbs, mesh_count, n, boxes, cent,
+                       partials);
+    if (hipGetLastError() != hipSuccess) return -2;
+    return st_lbvh_tree(stream, boxes, cent, partials, nb, n, 1u, p, keys, leaf_ids_out, nodes_out, height_out) ? -2 : 0;
 }

@@ -23,7 +23,18 @@ typedef struct StMeshLbvhWork {
     float *ibox;
 } StMeshLbvhWork;
 
-/* Device scratch bytes for a T-triangle build (sort_bytes: the radix sort's share). */
+/* Steps 2-6 of a build over T boxes (centroids `cent`, 3 floats each, and
+   nb per-256 partial centroid bounds, 6 floats each): Morton codes, the
+   stable radix sort, the radix tree, bottom-up boxes and the node output
+   with leaves of up to max_leaf boxes. `keys` and `order` (T words each)
+   receive the sorted codes and the box index at each sorted position;
+   `work` holds st_lbvh_tree_work_bytes(T). */
+size_t st_lbvh_tree_work_bytes(uint32_t T);
+int st_lbvh_tree(hipStream_t stream, const sbox *boxes, const float *cent, const float *partials, uint32_t nb,
+                 uint32_t T, uint32_t max_leaf, char *work, uint32_t *keys, uint32_t *order, uint32_t *nodes,
+                 uint32_t *height_dev);
+
+/* Device scratch bytes for a T-triangle build (sort_bytes: 0, kept for the caller's layout). */
 size_t st_mesh_lbvh_work_bytes(uint32_t T, size_t *sort_bytes);
 
 /* Build into `nodes` (2T-1 slots of 8 words, reference layout) and the hot /

@@ -11,7 +11,7 @@
  *   2. 30-bit Morton codes of the centroids in the centroid bounds
  *      (10 bits per axis, x highest);
  *   3. a stable device radix sort of (code, triangle index) pairs, so equal
- *      codes keep triangle order (rocPRIM radix_sort_pairs);
+ *      codes keep triangle order (st_sort.hip: LDS ranking, 8-bit digits);
  *   4. the binary radix tree over the sorted keys made unique by their sorted
  *      position (Karras 2012), one lane per internal node;
  *   5. boxes bottom-up: each leaf walks to the root and the second child to
@@ -20,6 +20,8 @@
  *      2k+2; a radix node covering at most 4 triangles becomes one leaf
  *      (child_count = its size, first_child_id = its first sorted position)
  *      and the slots below it stay unused; depth = tree depth;
+ * Steps 2-6 also build the scene LBVH over primitive boxes with leaves of
+ * one primitive (st_lbvh_tree, called from st_lbvh.hip).
  *   7. Triangle records in sorted order (st_mesh.h, the host builder's own
  *      per-triangle code), split into the device's hot and cold records.
  * The CPU oracle restates the same specification top-down; tests compare
@@ -27,13 +29,12 @@
  */
 #include <hip/hip_runtime.h>
 
-#include <rocprim/device/device_radix_sort.hpp>
-
 #include "../st_mesh.h"
 #include "st_mesh_lbvh.h"
+#include "st_sort.h"
 
 #define MB_THREADS 256u
-#define MB_MAX_LEAF 4u
+#define MB_MAX_LEAF 4u   /* triangles per mesh leaf */
 
 __device__ static inline uint32_t mb_expand10(uint32_t v) {
     v &= 0x3FFu;
@@ -211,14 +212,15 @@ __device__ static inline void mb_write_node(uint32_t *nodes, uint32_t slot, sbox
 }
 
 __global__ void __launch_bounds__(MB_THREADS) k_mb_emit(const sbox *boxes, const uint32_t *order, uint32_t T,
-                                                        StMeshLbvhWork w, uint32_t *nodes, uint32_t *height) {
+                                                        StMeshLbvhWork w, uint32_t *nodes, uint32_t *height,
+                                                        uint32_t max_leaf) {
     const uint32_t k = blockIdx.x * MB_THREADS + threadIdx.x;
     if (k + 1 >= T) return;
     /* depth = ancestors; not emitted when an ancestor became one leaf */
     uint32_t dep = 0, x = k;
     for (uint32_t guard = 0; x != 0 && guard < T; guard++) {
         const uint32_t p = w.iparent[x];
-        if (w.ilast[p] - w.ifirst[p] + 1u <= MB_MAX_LEAF) return;
+        if (w.ilast[p] - w.ifirst[p] + 1u <= max_leaf) return;
         dep++;
         x = p;
     }
@@ -229,7 +231,7 @@ __global__ void __launch_bounds__(MB_THREADS) k_mb_emit(const sbox *boxes, const
     }
     const sbox b = mb_box_load(w.ibox + 6 * (size_t)k);
     const uint32_t size = w.ilast[k] - w.ifirst[k] + 1u;
-    if (size <= MB_MAX_LEAF) {
+    if (size <= max_leaf) {
         mb_write_node(nodes, slot, b, w.ifirst[k], size, dep);
         atomicMax(height, dep + 1u);
         return;
@@ -277,31 +279,17 @@ __global__ void k_mb_zero(uint32_t *p, size_t n) {
 
 static size_t r256(size_t b) { return (b + 255) & ~(size_t)255; }
 
-size_t st_mesh_lbvh_work_bytes(uint32_t T, size_t *sort_bytes) {
-    size_t temp = 0;
-    if (rocprim::radix_sort_pairs((void *)NULL, temp, (uint32_t *)NULL, (uint32_t *)NULL, (uint32_t *)NULL,
-                                  (uint32_t *)NULL, T, 0, 30) != hipSuccess)
-        temp = 0;
-    *sort_bytes = temp;
-    size_t nb = (T + MB_THREADS - 1) / MB_THREADS;
-    return r256(24ull * T) + r256(12ull * T) + r256(24ull * nb) + 256 + 4 * r256(4ull * T) + 6 * r256(4ull * T) +
-           r256(24ull * T) + r256(temp) + 256;
+size_t st_lbvh_tree_work_bytes(uint32_t T) {
+    return 2 * r256(4ull * T) + 6 * r256(4ull * T) + r256(24ull * T) + r256(st_radix_sort_scratch_bytes(T)) + 256;
 }
 
-int st_mesh_lbvh_build(hipStream_t stream, StMeshGeomDev in, char *work, size_t sort_bytes, uint32_t *nodes,
-                       StDTriHot *hot, StDTriCold *cold, uint32_t *height_dev) {
-    const uint32_t T = in.T;
-    if (T == 0) return -1;
-    const uint32_t nb = (T + MB_THREADS - 1) / MB_THREADS;
+int st_lbvh_tree(hipStream_t stream, const sbox *boxes, const float *cent, const float *partials, uint32_t nb,
+                 uint32_t T, uint32_t max_leaf, char *work, uint32_t *keys, uint32_t *order, uint32_t *nodes,
+                 uint32_t *height_dev) {
+    const uint32_t nbt = (T + MB_THREADS - 1) / MB_THREADS;
     char *p = work;
-    sbox *boxes = (sbox *)p; p += r256(24ull * T);
-    float *cent = (float *)p; p += r256(12ull * T);
-    float *partials = (float *)p; p += r256(24ull * nb);
-    float *bounds = (float *)p; p += 256;
-    uint32_t *keys = (uint32_t *)p; p += r256(4ull * T);
-    uint32_t *vals = (uint32_t *)p; p += r256(4ull * T);
-    uint32_t *keys_s = (uint32_t *)p; p += r256(4ull * T);
-    uint32_t *order = (uint32_t *)p; p += r256(4ull * T);
+    uint32_t *keys_tmp = (uint32_t *)p; p += r256(4ull * T);
+    uint32_t *vals_tmp = (uint32_t *)p; p += r256(4ull * T);
     StMeshLbvhWork w;
     w.ifirst = (uint32_t *)p; p += r256(4ull * T);
     w.ilast = (uint32_t *)p; p += r256(4ull * T);
@@ -311,26 +299,47 @@ int st_mesh_lbvh_build(hipStream_t stream, StMeshGeomDev in, char *work, size_t 
     w.flags = (uint32_t *)p; p += r256(4ull * T);
     w.ibox = (float *)p; p += r256(24ull * T);
     void *sort_tmp = p;
-
+    float *bounds = (float *)((char *)sort_tmp + r256(st_radix_sort_scratch_bytes(T)));
     hipLaunchKernelGGL(k_mb_zero, dim3(1), dim3(64), 0, stream, height_dev, (size_t)1);
-    hipLaunchKernelGGL(k_mb_boxes, dim3(nb), dim3(MB_THREADS), 0, stream, in, boxes, cent, partials);
-    if (hipGetLastError() != hipSuccess) return -2;
     if (T == 1) {
         hipLaunchKernelGGL(k_mb_single, dim3(1), dim3(64), 0, stream, boxes, nodes, height_dev);
-    } else {
-        hipLaunchKernelGGL(k_mb_bounds, dim3(1), dim3(MB_THREADS), 0, stream, partials, nb, bounds);
-        hipLaunchKernelGGL(k_mb_codes, dim3(nb), dim3(MB_THREADS), 0, stream, cent, bounds, T, keys, vals);
-        size_t temp = sort_bytes;
-        if (rocprim::radix_sort_pairs(sort_tmp, temp, keys, keys_s, vals, order, T, 0, 30, stream) != hipSuccess)
-            return -3;
-        hipLaunchKernelGGL(k_mb_karras, dim3(nb), dim3(MB_THREADS), 0, stream, keys_s, T, w);
-        hipLaunchKernelGGL(k_mb_refit, dim3(nb), dim3(MB_THREADS), 0, stream, boxes, order, T, w);
-        hipLaunchKernelGGL(k_mb_emit, dim3(nb), dim3(MB_THREADS), 0, stream, boxes, order, T, w, nodes, height_dev);
-    }
-    if (T == 1) {
-        /* the single triangle is its own order */
         hipLaunchKernelGGL(k_mb_zero, dim3(1), dim3(64), 0, stream, order, (size_t)1);
+        return hipGetLastError() == hipSuccess ? 0 : -2;
     }
-    hipLaunchKernelGGL(k_mb_tris, dim3(nb), dim3(MB_THREADS), 0, stream, in, order, hot, cold);
+    hipLaunchKernelGGL(k_mb_bounds, dim3(1), dim3(MB_THREADS), 0, stream, partials, nb, bounds);
+    hipLaunchKernelGGL(k_mb_codes, dim3(nbt), dim3(MB_THREADS), 0, stream, cent, (const float *)bounds, T, keys,
+                       order);
+    if (st_radix_sort_pairs(stream, keys, order, keys_tmp, vals_tmp, T, 30, sort_tmp)) return -3;
+    hipLaunchKernelGGL(k_mb_karras, dim3(nbt), dim3(MB_THREADS), 0, stream, (const uint32_t *)keys, T, w);
+    hipLaunchKernelGGL(k_mb_refit, dim3(nbt), dim3(MB_THREADS), 0, stream, boxes, (const uint32_t *)order, T, w);
+    hipLaunchKernelGGL(k_mb_emit, dim3(nbt), dim3(MB_THREADS), 0, stream, boxes, (const uint32_t *)order, T, w, nodes,
+                       height_dev, max_leaf);
+    return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+size_t st_mesh_lbvh_work_bytes(uint32_t T, size_t *sort_bytes) {
+    *sort_bytes = 0;
+    const size_t nb = (T + MB_THREADS - 1) / MB_THREADS;
+    return r256(24ull * T) + r256(12ull * T) + r256(24ull * nb) + 2 * r256(4ull * T) + st_lbvh_tree_work_bytes(T) +
+           256;
+}
+
+int st_mesh_lbvh_build(hipStream_t stream, StMeshGeomDev in, char *work, size_t sort_bytes, uint32_t *nodes,
+                       StDTriHot *hot, StDTriCold *cold, uint32_t *height_dev) {
+    (void)sort_bytes;
+    const uint32_t T = in.T;
+    if (T == 0) return -1;
+    const uint32_t nb = (T + MB_THREADS - 1) / MB_THREADS;
+    char *p = work;
+    sbox *boxes = (sbox *)p; p += r256(24ull * T);
+    float *cent = (float *)p; p += r256(12ull * T);
+    float *partials = (float *)p; p += r256(24ull * nb);
+    uint32_t *keys = (uint32_t *)p; p += r256(4ull * T);
+    uint32_t *order = (uint32_t *)p; p += r256(4ull * T);
+    hipLaunchKernelGGL(k_mb_boxes, dim3(nb), dim3(MB_THREADS), 0, stream, in, boxes, cent, partials);
+    if (hipGetLastError() != hipSuccess) return -2;
+    const int rc = st_lbvh_tree(stream, boxes, cent, partials, nb, T, MB_MAX_LEAF, p, keys, order, nodes, height_dev);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_mb_tris, dim3(nb), dim3(MB_THREADS), 0, stream, in, (const uint32_t *)order, hot, cold);
     return hipGetLastError() == hipSuccess ? 0 : -2;
 }

@@ -1444,14 +1444,18 @@ extern "C" int st_scene_rebuild(StSceneHandle *h, uint32_t mode, float *ms) {
         return ST_OK;
     }
     if (mode != ST_REBUILD_LBVH) return set_err(ST_ERR_ARG, "unknown rebuild mode %u", mode);
-    if (P > ST_LBVH_MAX) return set_err(ST_ERR_ARG, "device LBVH supports up to %u primitives", ST_LBVH_MAX);
-    if (!h->lbvh_work) {
-        HIP_TRY(hipMalloc(&h->lbvh_work, st_lbvh_work_bytes(ST_LBVH_MAX)));
-        HIP_TRY(hipMalloc(&h->d_height, 256));
+    const size_t wb = st_lbvh_work_bytes(P);
+    if (h->lbvh_work_bytes < wb) {
+        if (h->lbvh_work) HIP_TRY(hipFree(h->lbvh_work));
+        h->lbvh_work = NULL;
+        h->lbvh_work_bytes = 0;
+        HIP_TRY(hipMalloc(&h->lbvh_work, wb));
+        h->lbvh_work_bytes = wb;
     }
-    StLbvhWork w = st_lbvh_work_carve(h->lbvh_work, P);
+    if (!h->d_height) HIP_TRY(hipMalloc(&h->d_height, 256));
     HIP_TRY(hipEventRecord(g.main.ev[2], g.main.stream));
-    if (st_lbvh_launch(g.main.stream, h->dev.prims, h->d_mesh_aabbs, sc->mesh_count, P, w, (uint32_t *)h->dev.scene_nodes,
+    if (st_lbvh_launch(g.main.stream, h->dev.prims, h->d_mesh_aabbs, sc->mesh_count, P, h->lbvh_work,
+                       (uint32_t *)h->dev.scene_nodes,
                        (uint32_t *)h->dev.scene_leaf_ids, h->d_height))
         return set_err(ST_ERR_HIP, "LBVH launch failed: %s", hipGetErrorString(hipGetLastError()));
     HIP_TRY(hipEventRecord(g.main.ev[3], g.main.stream));
