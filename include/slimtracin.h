@@ -140,13 +140,15 @@ typedef struct StTuning {
     uint32_t persistent;     /* 1: persistent tile queue (st_render); 0: one wavefront per tile */
     uint32_t cost_order;     /* 1: tiles claimed slowest first, from their times in the previous launch of the
                                 same shape on the same stream; 0: in tile order (request by request) */
+    uint32_t ray_sort;       /* 1: the bounce stage takes its rays binned by request and direction octant
+                                (a counting sort after the primary stage); 0: in the order they were queued */
 } StTuning;
 
 int st_set_tuning(const StTuning *tuning);
 int st_get_tuning(StTuning *tuning);
 /* Autotuner: times each candidate shape and claim order on these requests
-   (median of reps render-kernel times) and keeps the fastest; returns it and
-   its time. */
+   (median of reps render-kernel times), then the fastest with and without
+   the bounce-ray sort, and keeps the fastest; returns it and its time. */
 int st_autotune(StSceneHandle *const *scenes, const StView *views, uint32_t n, uint32_t reps, StTuning *best,
                 float *best_ms);
 

@@ -53,6 +53,7 @@
 #include "../host/st_host.h"
 #include "slimtracin.h"
 #include "st_internal.h"
+#include "st_sort.h"
 
 #define WAVES_PER_BLOCK 4
 #define BLOCK_THREADS (64 * WAVES_PER_BLOCK)
@@ -686,9 +687,15 @@ __global__ void __launch_bounds__(BLOCK_THREADS) K_BOUNCE_ATTR k_bounce(const St
     const uint32_t n_front = __builtin_amdgcn_readfirstlane(ldgu(w.ctr, 0));
     const uint32_t n_back = __builtin_amdgcn_readfirstlane(ldgu(w.ctr, 1));
     const uint32_t c_front = (n_front + 63u) / 64u;
-    drain_queue(reqs, nreq, c_front + (n_back + 63u) / 64u, claim,
+    const uint32_t chunks = w.sorted_q ? (n_front + n_back + 63u) / 64u : c_front + (n_back + 63u) / 64u;
+    drain_queue(reqs, nreq, chunks, claim,
                 [&](uint32_t chunk, uint32_t lane, uint32_t &slot, uint32_t &) {
-                    if (chunk < c_front) {
+                    if (w.sorted_q) {
+                        /* binned (k_bin): one list */
+                        const uint32_t qi = chunk * 64u + lane;
+                        if (qi >= n_front + n_back) return false;
+                        slot = ldgu(w.sorted_q, qi);
+                    } else if (chunk < c_front) {
                         const uint32_t qi = chunk * 64u + lane;
                         if (qi >= n_front) return false;
                         slot = ldgu(w.bounce_q, qi);
@@ -709,6 +716,31 @@ __global__ void __launch_bounds__(BLOCK_THREADS) K_BOUNCE_ATTR k_bounce(const St
                     if (p.flags & PF_REDO)
                         ((ST_GLOBAL uint32_t *)w.redo_q)[atomicAdd(w.redo_ctr, 1u)] = slot;
                 });
+}
+
+/* Ray sort of the bounce queue (autotuned, StTuning.ray_sort): a counting
+   sort of its entries into bins by part (the refracting front, then the
+   back), request and bounce direction octant (sign bits). A wave of the
+   bounce stage then takes rays of one request leaving in one octant, which
+   keeps its traversals together in the tree and its view and scene records
+   scalar. pass 0 counts the bins, pass 1 (after an exclusive scan of the
+   counts) places each entry; entries of one bin land in any order (a
+   pixel's result does not depend on its place in the queue). */
+#define BIN_THREADS 256u
+__global__ void __launch_bounds__(BIN_THREADS) k_bin(const KReq *__restrict__ reqs, uint32_t nreq, KWave w,
+                                                     uint32_t *__restrict__ bins, uint32_t pass) {
+    const uint32_t n_front = ldgu(w.ctr, 0), total = n_front + ldgu(w.ctr, 1);
+    for (uint32_t i = blockIdx.x * BIN_THREADS + threadIdx.x; i < total; i += gridDim.x * BIN_THREADS) {
+        const uint32_t part = i < n_front ? 0u : 1u;
+        const uint32_t slot = ldgu(w.bounce_q, part ? w.n - 1u - (i - n_front) : i);
+        const uint32_t r = find_req(reqs, nreq, slot >> 6, false);
+        const float4 bd = pl_ld(w, P_BDIR, slot);
+        const uint32_t oct = (__builtin_signbit(bd.x) ? 1u : 0u) | (__builtin_signbit(bd.y) ? 2u : 0u) |
+                             (__builtin_signbit(bd.z) ? 4u : 0u);
+        const uint32_t b = (part * nreq + r) * 8u + oct;
+        if (pass == 0) atomicAdd(&bins[b], 1u);
+        else ((ST_GLOBAL uint32_t *)w.sorted_q)[atomicAdd(&bins[b], 1u)] = slot;
+    }
 }
 
 /* Per-request shadow and bounce ray totals from the per-slot counts: block b
@@ -1605,10 +1637,14 @@ int st_rt_launch_ctx(LaunchCtx *c, StSceneHandle *const *scenes, StHostView *hv,
         }
     const size_t slots = (size_t)tiles * 64;
     size_t wave_bytes = 0;
+    /* the bounce-ray sort: its bins and the binned queue */
+    const bool ray_sort = levels > 1 && tune.ray_sort;
+    const uint32_t nbins = 2u * n * 8u;
     if (levels) {
         if (slots > 0xFFFFFFC0ull || slots * lmax > 0xFFFFFFC0ull) return set_err(ST_ERR_ARG, "batch too large");
         wave_bytes = round256(slots * 16 * ST_PLANES) + round256(slots * lmax * 8) + round256(slots * 4) +
                      round256(slots * 4) + round256(8) + round256(slots * 8) + 6 * 256;
+        if (ray_sort) wave_bytes += round256(slots * 4) + 2 * round256(nbins * 4ull) + 512;
     }
     /* per request: primary, shadow, bounce totals; then the fault word and
        the redo stage's queue length */
@@ -1705,6 +1741,12 @@ int st_rt_launch_ctx(LaunchCtx *c, StSceneHandle *const *scenes, StHostView *hv,
         w.ctr = (uint32_t *)arena_take(&c->work, 8);
         w.counts = (uint32_t *)arena_take(&c->work, slots * 8);
     }
+    uint32_t *bins = NULL, *bin_sums = NULL;
+    if (ray_sort) {
+        w.sorted_q = (uint32_t *)arena_take(&c->work, slots * 4);
+        bins = (uint32_t *)arena_take(&c->work, nbins * 4ull);
+        bin_sums = (uint32_t *)arena_take(&c->work, nbins * 4ull);
+    }
     w.redo_q = (uint32_t *)arena_take(&c->work, slots * 4);
     w.redo_ctr = (uint32_t *)(cnt_d + 3 * (size_t)n + 1);
     KCoop coop;
@@ -1789,6 +1831,18 @@ int st_rt_launch_ctx(LaunchCtx *c, StSceneHandle *const *scenes, StHostView *hv,
         /* the rest of the depth loop: the bounce stage reads its queue
            length on the device, so it needs no host round trip */
         if (levels > 1) {
+            if (ray_sort) {
+                const uint32_t bb = std::min<uint32_t>((uint32_t)((slots + BIN_THREADS - 1) / BIN_THREADS),
+                                                       4u * (uint32_t)g.cus);
+                hipLaunchKernelGGL(k_zero_u32, dim3((nbins + 255) / 256), dim3(256), 0, c->stream, bins, nbins);
+                hipLaunchKernelGGL(k_bin, dim3(bb), dim3(BIN_THREADS), 0, c->stream, (const KReq *)reqs_d, n, w, bins,
+                                   0u);
+                if (st_scan_exclusive(c->stream, bins, nbins, bin_sums))
+                    return set_err(ST_ERR_HIP, "bin scan failed: %s", hipGetErrorString(hipGetLastError()));
+                hipLaunchKernelGGL(k_bin, dim3(bb), dim3(BIN_THREADS), 0, c->stream, (const KReq *)reqs_d, n, w, bins,
+                                   1u);
+                HIP_TRY(hipGetLastError());
+            }
             hipLaunchKernelGGL(k_bounce, dim3(b_bounce), dim3(BLOCK_THREADS), lds, c->stream,
                                (const StDScene *)sc_d, (const KReq *)reqs_d, n, w, c->tile_counter + BOUNCE_CLAIM,
                                s_entries, m_entries);
@@ -1821,7 +1875,8 @@ int st_rt_launch_ctx(LaunchCtx *c, StSceneHandle *const *scenes, StHostView *hv,
                                      (uint64_t)(uintptr_t)cnt_d, (uint64_t)(uintptr_t)w.st, w.n,
                                      (uint64_t)(uintptr_t)c->tile_cost, use_history, cost_blocks,
                                      (uint64_t)(uintptr_t)w.redo_q, redo_blocks, redo_threads, lds_redo,
-                                     (uint64_t)(uintptr_t)coop.base};
+                                     (uint64_t)(uintptr_t)coop.base, (uint64_t)ray_sort,
+                                     (uint64_t)(uintptr_t)w.sorted_q};
         for (uint32_t i = 0; i < n; i++) {
             const StDView &dv = reqs_h[i].v;
             sig.insert(sig.end(), {dv.width1, dv.height1, dv.mode, dv.depth, dv.tile_shift,
@@ -2342,7 +2397,7 @@ extern "C" void st_host_free(void *p) {
 
 static int tuning_valid(const StTuning *t) {
     return t->tile_shift <= 6 && t->blocks_per_cu >= 1 && t->blocks_per_cu <= 64 && t->persistent <= 1 &&
-           t->cost_order <= 1;
+           t->cost_order <= 1 && t->ray_sort <= 1;
 }
 
 extern "C" int st_set_tuning(const StTuning *t) {
@@ -2379,18 +2434,24 @@ extern "C" int st_autotune(StSceneHandle *const *scenes, const StView *views, ui
     if (reps == 0) reps = 3;
     if (reps > 15) reps = 15;
     const StTuning saved = g.tune;
-    StTuning cand[36];
+    StTuning cand[37];
     uint32_t nc = 0;
     for (uint32_t sh = 3; sh <= 5; sh++) {
         const uint32_t bpc[3] = {2, 4, 8}, cap[3] = {2, 3, 8};
         for (uint32_t order = 0; order <= 1; order++) {
-            for (int k = 0; k < 3; k++) cand[nc++] = StTuning{sh, bpc[k], 1u, order};
-            for (int k = 0; k < 3; k++) cand[nc++] = StTuning{sh, cap[k], 0u, order};
+            for (int k = 0; k < 3; k++) cand[nc++] = StTuning{sh, bpc[k], 1u, order, saved.ray_sort};
+            for (int k = 0; k < 3; k++) cand[nc++] = StTuning{sh, cap[k], 0u, order, saved.ray_sort};
         }
     }
     float best_t = INFINITY;
     StTuning best_c = saved;
-    for (uint32_t c = 0; c < nc; c++) {
+    /* the shapes and claim orders, then the best of them with the bounce-ray
+       sort toggled */
+    for (uint32_t c = 0; c < nc + 1u; c++) {
+        if (c == nc) {
+            cand[c] = best_c;
+            cand[c].ray_sort ^= 1u;
+        }
         g.tune = cand[c];
         float times[15];
         if (cand[c].cost_order)

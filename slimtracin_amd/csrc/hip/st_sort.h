@@ -19,4 +19,8 @@ size_t st_radix_sort_scratch_bytes(uint32_t n);
 int st_radix_sort_pairs(hipStream_t stream, uint32_t *keys, uint32_t *vals, uint32_t *keys_tmp, uint32_t *vals_tmp,
                         uint32_t n, uint32_t key_bits, void *scratch);
 
+/* Exclusive prefix sum of a[0, len) in place (sums: (len + 2047) / 2048
+   words of scratch; len at most 2048 * 2048). Returns 0 on success. */
+int st_scan_exclusive(hipStream_t stream, uint32_t *a, uint32_t len, uint32_t *sums);
+
 #endif

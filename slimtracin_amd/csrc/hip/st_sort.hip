@@ -138,6 +138,18 @@ size_t st_radix_sort_scratch_bytes(uint32_t n) {
     return r256(4 * len) + r256(4 * b1) + 256;
 }
 
+int st_scan_exclusive(hipStream_t stream, uint32_t *a, uint32_t len, uint32_t *sums) {
+    const uint32_t b1 = (len + SC_BLOCK - 1) / SC_BLOCK;
+    if (len == 0) return 0;
+    if (b1 > SC_BLOCK) return -1;
+    hipLaunchKernelGGL(k_scan_blocks, dim3(b1), dim3(SC_THREADS), 0, stream, a, len, b1 > 1 ? sums : nullptr);
+    if (b1 > 1) {
+        hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(SC_THREADS), 0, stream, sums, b1, nullptr);
+        hipLaunchKernelGGL(k_scan_add, dim3(b1), dim3(SC_THREADS), 0, stream, a, len, (const uint32_t *)sums);
+    }
+    return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
 int st_radix_sort_pairs(hipStream_t stream, uint32_t *keys, uint32_t *vals, uint32_t *keys_tmp, uint32_t *vals_tmp,
                         uint32_t n, uint32_t key_bits, void *scratch) {
     if (n <= 1) return 0;

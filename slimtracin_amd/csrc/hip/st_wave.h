@@ -50,6 +50,8 @@ struct KWave {
                               front (refracting surfaces), ctr[1] from the back (the others) */
     uint32_t *ctr;         /* [2] lengths of bounce_q's front and back parts */
     uint32_t *counts;      /* [slot] 2 words: shadow rays, bounce rays of the pixel */
+    uint32_t *sorted_q;    /* [n] or NULL: the bounce queue binned by request and direction octant
+                              (k_bin), front part then back part */
     uint32_t *redo_q;      /* [slots of the launch] pixels to redo in the redo stage (every launch) */
     uint32_t *redo_ctr;    /* [1] length of redo_q */
 };
