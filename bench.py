@@ -460,8 +460,7 @@ def c2_leg(args):
         out["hit_ids_exact"] = bool((gh["prim_id"] == oh["prim_id"]).all() and
                                     (gh["triangle_id"] == oh["triangle_id"]).all() and
                                     (gh["distance"].view(np.uint32) == oh["distance"].view(np.uint32)).all())
-    st.set_tuning(saved.tile_shift, saved.blocks_per_cu, bool(saved.persistent), bool(saved.cost_order),
-                  bool(saved.ray_sort))
+    st.restore_tuning(saved)
     scene.close()
     return out
 

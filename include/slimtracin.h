@@ -142,13 +142,20 @@ typedef struct StTuning {
                                 same shape on the same stream; 0: in tile order (request by request) */
     uint32_t ray_sort;       /* 1: the bounce stage takes its rays binned by request and direction octant
                                 (a counting sort after the primary stage); 0: in the order they were queued */
+    uint32_t stack_entries;  /* LDS slots per traversal stack and lane in the primary and bounce stages (a
+                                power of two, 4-64): deeper entries go to the lane's private memory; 0: the
+                                whole stack the launch's scenes need */
+    uint32_t primary_waves;  /* 4: the primary stage held to 4 waves per SIMD (when its stacks leave LDS for
+                                them); 3 or 0: the occupancy its registers allow (3) */
 } StTuning;
 
 int st_set_tuning(const StTuning *tuning);
 int st_get_tuning(StTuning *tuning);
 /* Autotuner: times each candidate shape and claim order on these requests
    (median of reps render-kernel times), then the fastest with and without
-   the bounce-ray sort, and keeps the fastest; returns it and its time. */
+   the bounce-ray sort, then with traversal-stack windows (whole, 8, 16, 32
+   slots) and primary-stage occupancies (3, 4 waves per SIMD), and keeps
+   the fastest; returns it and its time. */
 int st_autotune(StSceneHandle *const *scenes, const StView *views, uint32_t n, uint32_t reps, StTuning *best,
                 float *best_ms);
 

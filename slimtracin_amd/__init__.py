@@ -339,12 +339,14 @@ def resolve_bgra8(grid: np.ndarray, grid_w: int, grid_h: int) -> np.ndarray:
 
 class Tuning(C.Structure):
     _fields_ = [("tile_shift", C.c_uint32), ("blocks_per_cu", C.c_uint32), ("persistent", C.c_uint32),
-                ("cost_order", C.c_uint32), ("ray_sort", C.c_uint32)]
+                ("cost_order", C.c_uint32), ("ray_sort", C.c_uint32), ("stack_entries", C.c_uint32),
+                ("primary_waves", C.c_uint32)]
 
     def as_dict(self):
         return {"tile": f"{1 << self.tile_shift}x{64 >> self.tile_shift}", "blocks_per_cu": self.blocks_per_cu,
                 "persistent": bool(self.persistent), "slowest_first": bool(self.cost_order),
-                "ray_sort": bool(self.ray_sort)}
+                "ray_sort": bool(self.ray_sort), "stack_entries": self.stack_entries or "whole",
+                "primary_waves_per_simd": self.primary_waves or 3}
 
 
 def get_tuning() -> Tuning:
@@ -354,8 +356,14 @@ def get_tuning() -> Tuning:
 
 
 def set_tuning(tile_shift: int = 3, blocks_per_cu: int = 8, persistent: bool = True, slowest_first: bool = True,
-               ray_sort: bool = False):
-    t = Tuning(tile_shift, blocks_per_cu, 1 if persistent else 0, 1 if slowest_first else 0, 1 if ray_sort else 0)
+               ray_sort: bool = False, stack_entries: int = 0, primary_waves: int = 3):
+    t = Tuning(tile_shift, blocks_per_cu, 1 if persistent else 0, 1 if slowest_first else 0, 1 if ray_sort else 0,
+               stack_entries, primary_waves)
+    _check(lib().st_set_tuning(C.byref(t)))
+
+
+def restore_tuning(t: Tuning):
+    """Set a Tuning read earlier with get_tuning (every field)."""
     _check(lib().st_set_tuning(C.byref(t)))
 
 

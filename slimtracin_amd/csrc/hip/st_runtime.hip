@@ -414,10 +414,11 @@ __device__ static inline __attribute__((always_inline)) void primary_tile(const 
     uint32_t *scene_stack = wave_base_;                                                                         \
     uint32_t *mesh_stack = wave_base_ + 64u * (scene_entries)
 
-__global__ void __launch_bounds__(BLOCK_THREADS) K_PRIMARY_ATTR
-k_primary(const StDScene *__restrict__ scenes, const KReq *__restrict__ reqs, uint32_t nreq, uint32_t total_tiles,
-          uint32_t *tile_counter, uint32_t persistent, uint32_t xcd_blocks, uint32_t scene_entries,
-          uint32_t mesh_entries, KWave w, const uint32_t *__restrict__ tile_order, uint32_t *__restrict__ tile_cost) {
+__device__ static inline __attribute__((always_inline)) void
+primary_stage(const StDScene *__restrict__ scenes, const KReq *__restrict__ reqs, uint32_t nreq, uint32_t total_tiles,
+              uint32_t *tile_counter, uint32_t persistent, uint32_t xcd_blocks, uint32_t scene_entries,
+              uint32_t mesh_entries, const KWave &w, const uint32_t *__restrict__ tile_order,
+              uint32_t *__restrict__ tile_cost) {
     ST_LDS_STACKS(scene_entries, mesh_entries);
     const DStacks stk = {scene_stack, mesh_stack, nullptr};
     /* One call site, so primary_tile is inlined once. Persistent: waves
@@ -447,6 +448,24 @@ k_primary(const StDScene *__restrict__ scenes, const KReq *__restrict__ reqs, ui
         if ((threadIdx.x & 63u) == 0)
             ((ST_GLOBAL uint32_t *)tile_cost)[t] = (uint32_t)(__builtin_amdgcn_s_memrealtime() - t0);
     }
+}
+
+#define K_PRIMARY_ARGS                                                                                                \
+    const StDScene *__restrict__ scenes, const KReq *__restrict__ reqs, uint32_t nreq, uint32_t total_tiles,         \
+        uint32_t *tile_counter, uint32_t persistent, uint32_t xcd_blocks, uint32_t scene_entries,                     \
+        uint32_t mesh_entries, KWave w, const uint32_t *__restrict__ tile_order, uint32_t *__restrict__ tile_cost
+/* The primary stage at the occupancy its registers allow (3 waves per
+   SIMD), and held to 4 waves per SIMD (128 registers, a few values in
+   scratch) for launches whose traversal stacks leave LDS for 4 waves
+   (StTuning.primary_waves): with 16-slot stack windows the headline batch's
+   primary stage took 30.0 ms at 4 waves against 34.2 ms at 3. */
+__global__ void __launch_bounds__(BLOCK_THREADS) K_PRIMARY_ATTR k_primary(K_PRIMARY_ARGS) {
+    primary_stage(scenes, reqs, nreq, total_tiles, tile_counter, persistent, xcd_blocks, scene_entries, mesh_entries,
+                  w, tile_order, tile_cost);
+}
+__global__ void __launch_bounds__(BLOCK_THREADS) __attribute__((amdgpu_waves_per_eu(4))) k_primary4(K_PRIMARY_ARGS) {
+    primary_stage(scenes, reqs, nreq, total_tiles, tile_counter, persistent, xcd_blocks, scene_entries, mesh_entries,
+                  w, tile_order, tile_cost);
 }
 
 /* Redo stage: the pixels whose traversal gave up in the main stages, each
@@ -1004,12 +1023,27 @@ static struct {
 } g;
 
 /* The per-launch traversal limits of a scene table entry. */
-static void launch_limits(StDScene &d, uint32_t step_cap, uint32_t abandon_steps, uint32_t coop_min_tris) {
+static void launch_limits(StDScene &d, uint32_t step_cap, uint32_t abandon_steps, uint32_t coop_min_tris,
+                          uint32_t mask_s = ~0u, uint32_t mask_m = ~0u) {
+    d.stack_mask_s = mask_s;
+    d.stack_mask_m = mask_m;
     d.step_cap = step_cap;
     uint32_t lim = abandon_steps ? abandon_steps : ST_ABANDON_STEPS;
     if (step_cap && step_cap < lim) lim = step_cap;
     d.main_step_limit = lim;
     d.coop_min_tris = coop_min_tris ? coop_min_tris : ST_COOP_MIN_TRIS;
+}
+
+/* LDS slots and ring mask of a traversal stack of `need` entries kept in a
+   window of `win` slots (0, or a window at least the stack: all of it) */
+static void stack_window(uint32_t need, uint32_t win, uint32_t &slots, uint32_t &mask) {
+    if (win && win < need) {
+        slots = win;
+        mask = win - 1u;
+    } else {
+        slots = need;
+        mask = ~0u;
+    }
 }
 
 /* Device buffers of st_render_noalloc: one array per GPU, part of the code
@@ -1067,9 +1101,12 @@ extern "C" int st_init(int device) {
     int rc = ctx_init(&g.main);
     if (rc) return rc;
     g.tune.tile_shift = 3;          /* 8 x 8 tiles */
-    g.tune.blocks_per_cu = 8;
+    g.tune.blocks_per_cu = 4;
     g.tune.persistent = 1;
     g.tune.cost_order = 1;
+    g.tune.ray_sort = 0;
+    g.tune.stack_entries = 16;      /* 16-slot LDS stack windows ... */
+    g.tune.primary_waves = 4;       /* ... leave room for 4 waves per SIMD in the primary stage */
     hipDeviceProp_t prop;
     g.cus = 256;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
@@ -1369,6 +1406,9 @@ extern "C" int st_scene_upload(StSceneHandle *h) {
     d.mesh_stack_size = sc->mesh_stack_size;
     d.has_emissive_quads = sc->has_emissive_quads;
     d.scene_stack_need = sc->scene_real_height + 1;
+    d.stack_mask_s = d.stack_mask_m = ~0u;   /* whole stacks in LDS unless a launch sets a window */
+    d.spill = NULL;
+    d.spill_s = d.spill_m = 0;
     return ST_OK;
 }
 
@@ -1656,13 +1696,34 @@ int st_rt_launch_ctx(LaunchCtx *c, StSceneHandle *const *scenes, StHostView *hv,
         if (d.scene_stack_need > s_entries) s_entries = d.scene_stack_need;
         if (d.mesh_stack_size > m_entries) m_entries = d.mesh_stack_size;
     }
+    /* a persistent launch keeps each traversal stack's newest entries in an
+       LDS window of the primary and bounce stages (StTuning.stack_entries)
+       and their older entries in a spill buffer with a region per wave of a
+       stage kernel (persistent grids and resident-only grids bound the wave
+       ids); one wavefront per tile keeps whole stacks */
+    const uint32_t win = persistent ? tune.stack_entries : 0u;
+    uint32_t ks, km, mask_s, mask_m;
+    stack_window(s_entries, win, ks, mask_s);
+    stack_window(m_entries, win, km, mask_m);
+    if (persistent && (size_t)BLOCK_THREADS * (ks + km) * sizeof(uint32_t) > 160 * 1024) {
+        stack_window(s_entries, 32u, ks, mask_s);
+        stack_window(m_entries, 32u, km, mask_m);
+    }
+    const uint32_t spill_s = mask_s != ~0u ? s_entries : 0u, spill_m = mask_m != ~0u ? m_entries : 0u;
+    uint32_t spill_waves = 0;
+    if (spill_s + spill_m) {
+        const uint32_t pk = std::min<uint32_t>((tiles + WAVES_PER_BLOCK - 1) / WAVES_PER_BLOCK,
+                                               (uint32_t)g.cus * tune.blocks_per_cu);
+        spill_waves = std::max<uint32_t>(pk * WAVES_PER_BLOCK, (uint32_t)g.cus * 32u);
+    }
+    const size_t spill_bytes = (size_t)spill_waves * 64u * (spill_s + spill_m) * 4u;
     /* redo stage: as many blocks as fit at once, grouped per redone pixel */
     uint32_t redo_threads;
     size_t lds_redo;
     redo_shape(s_entries, m_entries, redo_threads, lds_redo);
     const uint32_t redo_blocks = redo_blocks_for(redo_threads, lds_redo);
     const size_t redo_bytes = round256(slots * 4) + 256 + redo_coop_bytes(redo_blocks, REDO_QUEUE) + 256;
-    size_t work = up + round256(cnt_bytes) + per_req_out + wave_bytes + redo_bytes + 256;
+    size_t work = up + round256(cnt_bytes) + per_req_out + wave_bytes + redo_bytes + round256(spill_bytes) + 512;
     if ((rc = arena_reserve(&c->work, work))) return rc;
     /* staging: the upload block, then the ray counts and fault word read back */
     const size_t o_readback = round256(up);
@@ -1693,7 +1754,9 @@ int st_rt_launch_ctx(LaunchCtx *c, StSceneHandle *const *scenes, StHostView *hv,
     for (uint32_t k = 0; k < nuniq; k++) {
         sc_h[k] = uniq[k]->dev;
         sc_h[k].fault = (uint32_t *)(cnt_d + 3 * (size_t)n);
-        launch_limits(sc_h[k], step_cap, abandon_steps, coop_min_tris);
+        launch_limits(sc_h[k], step_cap, abandon_steps, coop_min_tris, mask_s, mask_m);
+        sc_h[k].spill_s = spill_s;
+        sc_h[k].spill_m = spill_m;
     }
     uint32_t row_base = 0, out_base = 0, count_blocks = 0, any_device_ssb = 0;
     for (uint32_t i = 0; i < n; i++) {
@@ -1755,6 +1818,8 @@ int st_rt_launch_ctx(LaunchCtx *c, StSceneHandle *const *scenes, StHostView *hv,
     coop.group_stride = (uint32_t)redo_group_stride(REDO_QUEUE);
     coop.q_cap = REDO_QUEUE;
     coop.epoch = (const uint32_t *)(c->cost_sum + 1);
+    uint32_t *spill = spill_bytes ? (uint32_t *)arena_take(&c->work, spill_bytes) : NULL;
+    for (uint32_t k = 0; k < nuniq; k++) sc_h[k].spill = spill;
     /* launch shape, fixed before anything is enqueued so that a captured
        graph of this launch can be matched against the next one */
     uint32_t blocks;
@@ -1768,8 +1833,11 @@ int st_rt_launch_ctx(LaunchCtx *c, StSceneHandle *const *scenes, StHostView *hv,
         blocks = (blocks + 7u) & ~7u;
         xcd_blocks = blocks;
     }
-    size_t lds = (size_t)BLOCK_THREADS * (s_entries + m_entries) * sizeof(uint32_t);
+    size_t lds = (size_t)BLOCK_THREADS * (ks + km) * sizeof(uint32_t);
     if (lds > 160 * 1024) return set_err(ST_ERR_SCENE, "traversal stacks need %zu B of LDS", lds);
+    /* the primary stage at 4 waves per SIMD when asked for and the stacks
+       leave LDS for 4 blocks of 4 waves per CU */
+    const bool waves4 = tune.primary_waves == 4 && 4 * lds <= 160 * 1024;
     /* the depth loop's queue kernels run exactly as many waves as fit on
        the device at once: they take queue chunks in a fixed interleaved
        order, so a wave that had to wait for a slot would run its whole
@@ -1779,7 +1847,8 @@ int st_rt_launch_ctx(LaunchCtx *c, StSceneHandle *const *scenes, StHostView *hv,
     /* one wave per tile: at most blocks_per_cu of its blocks per CU (fewer
        tiles in flight share the caches better) */
     if (!persistent && tune.blocks_per_cu < 4)
-        (void)capped_blocks(reinterpret_cast<const void *>(k_primary), lds, tune.blocks_per_cu, &lds_primary);
+        (void)capped_blocks(waves4 ? reinterpret_cast<const void *>(k_primary4) : reinterpret_cast<const void *>(k_primary),
+                            lds, tune.blocks_per_cu, &lds_primary);
     if (levels) {
         b_bounce = resident_blocks(reinterpret_cast<const void *>(k_bounce), lds);
     }
@@ -1823,9 +1892,16 @@ int st_rt_launch_ctx(LaunchCtx *c, StSceneHandle *const *scenes, StHostView *hv,
                            (uint32_t *)(c->cost_sum + 1));
         HIP_TRY(hipGetLastError());
         HIP_TRY(mark());
-        hipLaunchKernelGGL(k_primary, dim3(blocks), dim3(BLOCK_THREADS), lds_primary, c->stream, (const StDScene *)sc_d,
-                           (const KReq *)reqs_d, n, tiles, c->tile_counter, (uint32_t)persistent, xcd_blocks,
-                           s_entries, m_entries, w, (const uint32_t *)c->tile_order, c->tile_cost);
+        if (waves4)
+            hipLaunchKernelGGL(k_primary4, dim3(blocks), dim3(BLOCK_THREADS), lds_primary, c->stream,
+                               (const StDScene *)sc_d, (const KReq *)reqs_d, n, tiles, c->tile_counter,
+                               (uint32_t)persistent, xcd_blocks, ks, km, w, (const uint32_t *)c->tile_order,
+                               c->tile_cost);
+        else
+            hipLaunchKernelGGL(k_primary, dim3(blocks), dim3(BLOCK_THREADS), lds_primary, c->stream,
+                               (const StDScene *)sc_d, (const KReq *)reqs_d, n, tiles, c->tile_counter,
+                               (uint32_t)persistent, xcd_blocks, ks, km, w, (const uint32_t *)c->tile_order,
+                               c->tile_cost);
         HIP_TRY(hipGetLastError());
         HIP_TRY(mark());
         /* the rest of the depth loop: the bounce stage reads its queue
@@ -1845,7 +1921,7 @@ int st_rt_launch_ctx(LaunchCtx *c, StSceneHandle *const *scenes, StHostView *hv,
             }
             hipLaunchKernelGGL(k_bounce, dim3(b_bounce), dim3(BLOCK_THREADS), lds, c->stream,
                                (const StDScene *)sc_d, (const KReq *)reqs_d, n, w, c->tile_counter + BOUNCE_CLAIM,
-                               s_entries, m_entries);
+                               ks, km);
             HIP_TRY(mark());
         }
         /* the pixels that gave up, traced again (it reads the queue's length
@@ -1869,14 +1945,14 @@ int st_rt_launch_ctx(LaunchCtx *c, StSceneHandle *const *scenes, StHostView *hv,
            graph does exactly this launch's work (the per-frame data is in
            the upload block, copied from the same pinned staging) */
         std::vector<uint64_t> sig = {n, up, rows, tiles, levels, lmax, count_blocks, blocks, xcd_blocks, any_device_ssb,
-                                     (uint64_t)persistent, s_entries, m_entries, b_bounce,
+                                     (uint64_t)persistent, s_entries, m_entries, ks, km, (uint64_t)waves4, b_bounce,
                                      lds_primary, (uint64_t)(uintptr_t)blk,
                                      (uint64_t)(uintptr_t)st,
                                      (uint64_t)(uintptr_t)cnt_d, (uint64_t)(uintptr_t)w.st, w.n,
                                      (uint64_t)(uintptr_t)c->tile_cost, use_history, cost_blocks,
                                      (uint64_t)(uintptr_t)w.redo_q, redo_blocks, redo_threads, lds_redo,
                                      (uint64_t)(uintptr_t)coop.base, (uint64_t)ray_sort,
-                                     (uint64_t)(uintptr_t)w.sorted_q};
+                                     (uint64_t)(uintptr_t)w.sorted_q, (uint64_t)(uintptr_t)spill, spill_s, spill_m};
         for (uint32_t i = 0; i < n; i++) {
             const StDView &dv = reqs_h[i].v;
             sig.insert(sig.end(), {dv.width1, dv.height1, dv.mode, dv.depth, dv.tile_shift,
@@ -2033,6 +2109,8 @@ extern "C" int st_render_noalloc(StSceneHandle *scene, const StView *view, StFlo
     redo_shape(s_entries, m_entries, redo_threads, lds_redo);
     const uint32_t redo_blocks = redo_blocks_for(redo_threads, lds_redo);
     const size_t o_coop = take(redo_coop_bytes(redo_blocks, REDO_QUEUE_NOALLOC));
+    /* whole traversal stacks in LDS (a spill buffer would not fit the array) */
+    const uint32_t ks = s_entries, km = m_entries, mask_s = ~0u, mask_m = ~0u;
     size_t o_st = 0, o_sres = 0, o_occ = 0, o_bq = 0, o_wctr = 0, o_wcnt = 0;
     if (beauty) {
         o_st = take(slots * 16 * ST_PLANES);
@@ -2077,7 +2155,9 @@ extern "C" int st_render_noalloc(StSceneHandle *scene, const StView *view, StFlo
     rq->device_ssb = 1;
     StDScene dsc = scene->dev;
     dsc.fault = (uint32_t *)(cnt_d + 3);
-    launch_limits(dsc, g.step_cap, g.abandon_steps, g.coop_min_tris);
+    launch_limits(dsc, g.step_cap, g.abandon_steps, g.coop_min_tris, mask_s, mask_m);
+    dsc.spill = NULL;
+    dsc.spill_s = dsc.spill_m = 0;
     memcpy(g_noalloc_host + o_scene, &dsc, sizeof dsc);
     KWave w;
     memset(&w, 0, sizeof w);
@@ -2099,7 +2179,7 @@ extern "C" int st_render_noalloc(StSceneHandle *scene, const StView *view, StFlo
     coop.group_stride = (uint32_t)redo_group_stride(REDO_QUEUE_NOALLOC);
     coop.q_cap = REDO_QUEUE_NOALLOC;
     coop.epoch = epoch;
-    const size_t lds = (size_t)BLOCK_THREADS * (s_entries + m_entries) * sizeof(uint32_t);
+    const size_t lds = (size_t)BLOCK_THREADS * (ks + km) * sizeof(uint32_t);
     if (lds > 160 * 1024) return set_err(ST_ERR_SCENE, "traversal stacks need %zu B of LDS", lds);
     uint32_t blocks, xcd_blocks = 0;
     size_t lds_primary = lds;
@@ -2130,13 +2210,13 @@ extern "C" int st_render_noalloc(StSceneHandle *scene, const StView *view, StFlo
                        (const unsigned long long *)cnt_d, ctr + ORDER_CLAIM, 0u, epoch);
     HIP_TRY(hipGetLastError());
     hipLaunchKernelGGL(k_primary, dim3(blocks), dim3(BLOCK_THREADS), lds_primary, stream, sc_d, reqs_d, 1u, tiles, ctr,
-                       (uint32_t)tune.persistent, xcd_blocks, s_entries, m_entries, w,
+                       (uint32_t)tune.persistent, xcd_blocks, ks, km, w,
                        (const uint32_t *)(base + o_order), (uint32_t *)(base + o_cost));
     HIP_TRY(hipGetLastError());
     if (beauty && dv.depth > 1) {
         const uint32_t b_bounce = resident_blocks(reinterpret_cast<const void *>(k_bounce), lds);
         hipLaunchKernelGGL(k_bounce, dim3(b_bounce), dim3(BLOCK_THREADS), lds, stream, sc_d, reqs_d, 1u, w,
-                           ctr + BOUNCE_CLAIM, s_entries, m_entries);
+                           ctr + BOUNCE_CLAIM, ks, km);
     }
     hipLaunchKernelGGL(k_redo, dim3(redo_blocks), dim3(redo_threads), lds_redo, stream, sc_d, reqs_d, 1u, w,
                        s_entries, m_entries, coop);
@@ -2397,7 +2477,10 @@ extern "C" void st_host_free(void *p) {
 
 static int tuning_valid(const StTuning *t) {
     return t->tile_shift <= 6 && t->blocks_per_cu >= 1 && t->blocks_per_cu <= 64 && t->persistent <= 1 &&
-           t->cost_order <= 1 && t->ray_sort <= 1;
+           t->cost_order <= 1 && t->ray_sort <= 1 &&
+           (t->stack_entries == 0 || (t->stack_entries >= 4 && t->stack_entries <= 64 &&
+                                      (t->stack_entries & (t->stack_entries - 1)) == 0)) &&
+           (t->primary_waves == 0 || t->primary_waves == 3 || t->primary_waves == 4);
 }
 
 extern "C" int st_set_tuning(const StTuning *t) {
@@ -2434,23 +2517,33 @@ extern "C" int st_autotune(StSceneHandle *const *scenes, const StView *views, ui
     if (reps == 0) reps = 3;
     if (reps > 15) reps = 15;
     const StTuning saved = g.tune;
-    StTuning cand[37];
+    StTuning cand[44];
     uint32_t nc = 0;
     for (uint32_t sh = 3; sh <= 5; sh++) {
         const uint32_t bpc[3] = {2, 4, 8}, cap[3] = {2, 3, 8};
         for (uint32_t order = 0; order <= 1; order++) {
-            for (int k = 0; k < 3; k++) cand[nc++] = StTuning{sh, bpc[k], 1u, order, saved.ray_sort};
-            for (int k = 0; k < 3; k++) cand[nc++] = StTuning{sh, cap[k], 0u, order, saved.ray_sort};
+            for (int k = 0; k < 3; k++)
+                cand[nc++] = StTuning{sh, bpc[k], 1u, order, saved.ray_sort, saved.stack_entries, saved.primary_waves};
+            for (int k = 0; k < 3; k++)
+                cand[nc++] = StTuning{sh, cap[k], 0u, order, saved.ray_sort, saved.stack_entries, saved.primary_waves};
         }
     }
     float best_t = INFINITY;
     StTuning best_c = saved;
-    /* the shapes and claim orders, then the best of them with the bounce-ray
-       sort toggled */
-    for (uint32_t c = 0; c < nc + 1u; c++) {
+    /* the shapes and claim orders; then the best of them with the bounce-ray
+       sort toggled; then the best with each traversal-stack window and
+       primary-stage occupancy */
+    const uint32_t windows[5][2] = {{0u, 3u}, {16u, 3u}, {32u, 3u}, {8u, 4u}, {16u, 4u}};
+    for (uint32_t c = 0; c < nc + 6u; c++) {
         if (c == nc) {
             cand[c] = best_c;
             cand[c].ray_sort ^= 1u;
+        } else if (c > nc) {
+            cand[c] = best_c;
+            cand[c].stack_entries = windows[c - nc - 1u][0];
+            cand[c].primary_waves = windows[c - nc - 1u][1];
+            if (cand[c].stack_entries == best_c.stack_entries && cand[c].primary_waves == best_c.primary_waves)
+                continue;
         }
         g.tune = cand[c];
         float times[15];

@@ -81,6 +81,7 @@ struct DTrace {
 
 #define ST_STACK_STRIDE 64u
 
+
 struct DShaded {
     sv3 albedo, position, normal, viewing_direction, viewing_origin, reflected_direction, light_direction;
     sv3 quad_vertices[4];
@@ -135,6 +136,39 @@ DFN void stgi4(void *base, size_t i, int a, int b, int c, int d) {
     st_vi4 v = {a, b, c, d};
     ((ST_GLOBAL st_vi4 *)base)[i] = v;
 }
+/* A traversal stack: its newest entries in the lane's LDS column, used as a
+   ring of mask + 1 slots, the older ones in the launch's spill buffer
+   (StDScene.spill: per wave of the kernel, 64 lanes of spill_s scene then
+   spill_m mesh entries; `first` is 0 for the scene stack, spill_s for the
+   mesh stack). Entries [lo, sp) are in LDS. With mask all ones the column
+   holds the whole stack and nothing spills. The spill address is computed
+   only when an entry moves, so the traversal keeps its registers. */
+struct DStk {
+    uint32_t *lds;
+    uint32_t mask, lo;
+};
+DFN ST_GLOBAL uint32_t *stk_spill(const StDScene &sc, uint32_t first, uint32_t i) {
+    const uint32_t wave = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const size_t per_wave = 64u * (size_t)(sc.spill_s + sc.spill_m);
+    return (ST_GLOBAL uint32_t *)sc.spill + wave * per_wave + 64u * (size_t)(first + i) + (threadIdx.x & 63u);
+}
+/* push v as entry sp */
+DFN void stk_push(DStk &s, const StDScene &sc, uint32_t first, uint32_t sp, uint32_t v) {
+    if (sp - s.lo > s.mask) {
+        *stk_spill(sc, first, s.lo) = s.lds[ST_STACK_STRIDE * (s.lo & s.mask)];
+        s.lo++;
+    }
+    s.lds[ST_STACK_STRIDE * (sp & s.mask)] = v;
+}
+/* entry sp (the stack's size after the pop) */
+DFN uint32_t stk_pop(DStk &s, const StDScene &sc, uint32_t first, uint32_t sp) {
+    if (sp < s.lo) {
+        s.lo = sp;
+        return *stk_spill(sc, first, sp);
+    }
+    return s.lds[ST_STACK_STRIDE * (sp & s.mask)];
+}
+
 DFN StDMesh load_mesh(const StDScene &sc, uint32_t i) {
     int4 a = ldgi4(sc.meshes + i, 0);
     StDMesh m = {};
@@ -1493,9 +1527,7 @@ DNOINLINE bool trace_mesh(DTrace &tr, const StDScene &sc, uint32_t mesh_id, bool
     }
     uint32_t left = root_first;
     uint32_t stack_size = 0;
-    /* each step loads one node pair, and a traversal loads a pair at most
-       once: more steps than the mesh has nodes (at most 2T - 1) means a
-       malformed tree, which would otherwise loop for ever */
+    DStk stk = {stack, sc.stack_mask_m, 0};
     /* The main stages give up after main_step_limit steps (the pixel is
        redone). Each step loads one node pair, and a traversal loads a pair
        at most once, so the redo stage's bound, the mesh's node count, is
@@ -1534,7 +1566,7 @@ DNOINLINE bool trace_mesh(DTrace &tr, const StDScene &sc, uint32_t mesh_id, bool
         if (has_left) {
             if (has_right) {
                 if (!any_hit && ld > rd) { uint32_t t = lnext; lnext = rnext; rnext = t; }
-                stack[ST_STACK_STRIDE * stack_size++] = rnext;
+                stk_push(stk, sc, sc.spill_s, stack_size++, rnext);
                 if (stack_size == tr.mesh_stack_size) return false;
             }
             left = lnext;
@@ -1542,7 +1574,7 @@ DNOINLINE bool trace_mesh(DTrace &tr, const StDScene &sc, uint32_t mesh_id, bool
             left = rnext;
         } else {
             if (stack_size == 0) break;
-            left = stack[ST_STACK_STRIDE * --stack_size];
+            left = stk_pop(stk, sc, sc.spill_s, --stack_size);
         }
     }
     return found;
@@ -1688,8 +1720,8 @@ DNOINLINE bool trace_scene(DRay &ray, DTrace &tr, const StDScene &sc, bool any_h
     if (!hit_aabb(root, ray, tr.closest_hit.distance, ld)) return false;
     const bool root_leaf = root_count != 0;
     uint32_t left = root_first;
-    uint32_t *stack = tr.scene_stack;
     uint32_t stack_size = 0;
+    DStk stk = {tr.scene_stack, sc.stack_mask_s, 0};
     /* bounds no valid tree reaches (see trace_mesh); the stack holds at
        most the tree's height, scene_stack_need - 1 entries */
     uint32_t max_steps = 2u * sc.prim_count + 2u;
@@ -1734,14 +1766,14 @@ DNOINLINE bool trace_scene(DRay &ray, DTrace &tr, const StDScene &sc, bool any_h
                     trav_fault(sc, ST_FAULT_SCENE_STACK);
                     break;
                 }
-                stack[ST_STACK_STRIDE * stack_size++] = rnext;
+                stk_push(stk, sc, 0u, stack_size++, rnext);
             }
             left = lnext;
         } else if (has_right) {
             left = rnext;
         } else {
             if (stack_size == 0) break;
-            left = stack[ST_STACK_STRIDE * --stack_size];
+            left = stk_pop(stk, sc, 0u, --stack_size);
         }
     }
     return found;

@@ -16,8 +16,10 @@
 
 #include <stdint.h>
 
-#define ST_SCENE_STACK_MAX 64   /* scene BVH traversal stack entries per ray */
-#define ST_MESH_STACK_MAX 64    /* mesh BVH traversal stack entries per ray */
+/* Traversal stack entries per ray (a launch keeps each stack's newest entries
+   in LDS and the rest in the lane's private memory) */
+#define ST_SCENE_STACK_MAX 64   /* scene BVH */
+#define ST_MESH_STACK_MAX 128   /* mesh BVH */
 
 /* Device scratch of st_render_noalloc (bytes, one array per GPU). */
 #define ST_NOALLOC_SCRATCH_BYTES (128u << 20)
@@ -110,7 +112,11 @@ typedef struct StDScene {
                                       many steps and their pixel is redone (ST_ABANDON_STEPS) */
     uint32_t coop_min_tris;        /* per launch: the redo stage splits traversals of meshes of at least
                                       this many triangles across a group of blocks (ST_COOP_MIN_TRIS) */
-    uint32_t pad_;
+    uint32_t stack_mask_s, stack_mask_m; /* per launch: LDS slots of the scene / mesh traversal stack - 1
+                                      (a power of two less one; all ones: the whole stack is in LDS) */
+    uint32_t *spill;               /* per launch, with a window: the stacks' older entries, per wave of a
+                                      stage kernel (64 lanes x (spill_s + spill_m) words) */
+    uint32_t spill_s, spill_m;     /* entries per lane of the scene and mesh stacks in `spill` */
 } StDScene;
 
 /* Mesh traversals longer than this (node-pair steps) give up in the main

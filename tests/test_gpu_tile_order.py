@@ -38,8 +38,7 @@ def test_frames_do_not_depend_on_the_previous_launch():
                               slowest_first=True)
                 runs.append(_frame(ds, a))
         finally:
-            st.set_tuning(saved.tile_shift, saved.blocks_per_cu, bool(saved.persistent), bool(saved.cost_order),
-                          bool(saved.ray_sort))
+            st.restore_tuning(saved)
         for k, (p, h, c) in enumerate(runs):
             assert p.tobytes() == p0.tobytes(), k
             assert h.tobytes() == h0.tobytes(), k

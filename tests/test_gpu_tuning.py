@@ -39,6 +39,5 @@ def test_all_launch_shapes_give_identical_frames():
         assert best.tile_shift in (3, 4, 5) and ms > 0
         print("autotune:", best.as_dict(), f"{ms:.3f} ms", flush=True)
     finally:
-        st.set_tuning(saved.tile_shift, saved.blocks_per_cu, bool(saved.persistent), bool(saved.cost_order),
-                      bool(saved.ray_sort))
+        st.restore_tuning(saved)
         ds.close()
