@@ -137,14 +137,19 @@ typedef struct StTuning {
     uint32_t tile_shift;     /* tile = (1 << tile_shift) x (64 >> tile_shift) pixels, one wavefront per tile */
     uint32_t blocks_per_cu;  /* persistent grid: blocks per compute unit (4 wavefronts each); one wavefront
                                 per tile: at most this many blocks resident per compute unit (4 or more: no cap) */
-    uint32_t persistent;     /* 1: persistent tile queue (st_render); 0: one wavefront per tile */
+    uint32_t persistent;     /* 1: persistent tile queue (st_render), tiles claimed from one counter; 2: the
+                                same, each XCD's blocks first claiming its own eighth of the claim order (a
+                                contiguous run of tiles, so its L2 serves a compact part of the frames), then
+                                the other eighths as they run out; 0: one wavefront per tile */
     uint32_t cost_order;     /* 1: tiles claimed slowest first, from their times in the previous launch of the
-                                same shape on the same stream; 0: in tile order (request by request) */
+                                same shape on the same stream; 0: in tile order (request by request, row by
+                                row); 2: request by request in super-tiles of 8 x 8 tiles, so that a run of
+                                claims covers a compact part of a frame */
     uint32_t ray_sort;       /* 1: the bounce stage takes its rays binned by request and direction octant
                                 (a counting sort after the primary stage); 0: in the order they were queued */
-    uint32_t stack_entries;  /* LDS slots per traversal stack and lane in the primary and bounce stages (a
-                                power of two, 4-64): deeper entries go to the lane's private memory; 0: the
-                                whole stack the launch's scenes need */
+    uint32_t stack_entries;  /* LDS slots per traversal stack and lane in the primary and bounce stages of a
+                                persistent launch (a power of two, 4-64): older entries go to the launch's
+                                spill buffer in device memory; 0: the whole stack the launch's scenes need */
     uint32_t primary_waves;  /* 4: the primary stage held to 4 waves per SIMD (when its stacks leave LDS for
                                 them); 3 or 0: the occupancy its registers allow (3) */
 } StTuning;
@@ -153,9 +158,11 @@ int st_set_tuning(const StTuning *tuning);
 int st_get_tuning(StTuning *tuning);
 /* Autotuner: times each candidate shape and claim order on these requests
    (median of reps render-kernel times), then the fastest with and without
-   the bounce-ray sort, then with traversal-stack windows (whole, 8, 16, 32
-   slots) and primary-stage occupancies (3, 4 waves per SIMD), and keeps
-   the fastest; returns it and its time. */
+   the bounce-ray sort, with either persistent tile claim (1 or 2), in
+   super-tile claim order (cost_order 2), then with
+   traversal-stack windows (whole, 8, 16, 32 slots) and primary-stage
+   occupancies (3, 4 waves per SIMD), and keeps the fastest; returns it and
+   its time. */
 int st_autotune(StSceneHandle *const *scenes, const StView *views, uint32_t n, uint32_t reps, StTuning *best,
                 float *best_ms);
 

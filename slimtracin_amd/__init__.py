@@ -344,7 +344,8 @@ class Tuning(C.Structure):
 
     def as_dict(self):
         return {"tile": f"{1 << self.tile_shift}x{64 >> self.tile_shift}", "blocks_per_cu": self.blocks_per_cu,
-                "persistent": bool(self.persistent), "slowest_first": bool(self.cost_order),
+                "persistent": bool(self.persistent), "xcd_claim": self.persistent == 2,
+                "slowest_first": self.cost_order == 1, "super_tiles": self.cost_order == 2,
                 "ray_sort": bool(self.ray_sort), "stack_entries": self.stack_entries or "whole",
                 "primary_waves_per_simd": self.primary_waves or 3}
 
@@ -355,10 +356,16 @@ def get_tuning() -> Tuning:
     return t
 
 
-def set_tuning(tile_shift: int = 3, blocks_per_cu: int = 8, persistent: bool = True, slowest_first: bool = True,
-               ray_sort: bool = False, stack_entries: int = 0, primary_waves: int = 3):
-    t = Tuning(tile_shift, blocks_per_cu, 1 if persistent else 0, 1 if slowest_first else 0, 1 if ray_sort else 0,
-               stack_entries, primary_waves)
+def set_tuning(tile_shift: int = 3, blocks_per_cu: int = 4, persistent: bool = True, slowest_first: bool = False,
+               ray_sort: bool = False, stack_entries: int = 16, primary_waves: int = 4, xcd_claim: bool = True,
+               super_tiles: bool = False):
+    """Launch shape (StTuning); the defaults are the library's. xcd_claim:
+    a persistent launch's tiles claimed by each XCD from its own eighth of
+    the claim order first; super_tiles (when not slowest_first): tiles
+    claimed request by request in blocks of 8 x 8 tiles."""
+    mode = (2 if xcd_claim else 1) if persistent else 0
+    order = 1 if slowest_first else 2 if super_tiles else 0
+    t = Tuning(tile_shift, blocks_per_cu, mode, order, 1 if ray_sort else 0, stack_entries, primary_waves)
     _check(lib().st_set_tuning(C.byref(t)))
 
 

@@ -414,6 +414,35 @@ __device__ static inline __attribute__((always_inline)) void primary_tile(const 
     uint32_t *scene_stack = wave_base_;                                                                         \
     uint32_t *mesh_stack = wave_base_ + 64u * (scene_entries)
 
+/* words of the per-launch counters (zeroed at launch): the XCD-aware
+   claim's 8 chunk counters from XCD_CLAIM on, each on a 128-byte line of
+   its own (atomics to one line serialise) */
+#define CLAIM_WORDS 1024u
+#define XCD_MIN_TILES 32u   /* tiles per wave below which a launch claims from one counter */
+#define XCD_CLAIM 64u
+#define XCD_CLAIM_STRIDE 32u
+
+/* XCD-aware tile claim (StTuning.persistent 2): the claim order in 8
+   contiguous chunks. The blocks of XCD k (a launch's workgroups are dealt
+   to the 8 XCDs round-robin) take chunk k's tiles first, so each XCD's L2
+   serves a compact part of the frames instead of every 8th tile of a
+   region 8 times as large, then chunk k+1's, k+2's, ... as each runs out.
+   chunk and tried are the wave's state (first chunk blockIdx.x mod 8, none
+   tried); every failed claim moves to the next chunk, so a wave makes at
+   most 8 of them and the counters stay below tiles + 8 x waves. Returns
+   total once all 8 chunks are done. */
+__device__ static inline uint32_t claim_xcd(uint32_t *ctr, uint32_t total, uint32_t &chunk, uint32_t &tried) {
+    while (tried < 8u) {
+        const uint32_t lo = (uint32_t)((uint64_t)total * chunk / 8u);
+        const uint32_t hi = (uint32_t)((uint64_t)total * (chunk + 1u) / 8u);
+        const uint32_t p = atomicAdd(ctr + XCD_CLAIM_STRIDE * chunk, 1u);
+        if (p < hi - lo) return lo + p;
+        chunk = (chunk + 1u) & 7u;
+        tried++;
+    }
+    return total;
+}
+
 __device__ static inline __attribute__((always_inline)) void
 primary_stage(const StDScene *__restrict__ scenes, const KReq *__restrict__ reqs, uint32_t nreq, uint32_t total_tiles,
               uint32_t *tile_counter, uint32_t persistent, uint32_t xcd_blocks, uint32_t scene_entries,
@@ -421,17 +450,21 @@ primary_stage(const StDScene *__restrict__ scenes, const KReq *__restrict__ reqs
               uint32_t *__restrict__ tile_cost) {
     ST_LDS_STACKS(scene_entries, mesh_entries);
     const DStacks stk = {scene_stack, mesh_stack, nullptr};
-    /* One call site, so primary_tile is inlined once. Persistent: waves
-       claim tiles from the global counter until it runs past the end.
+    /* One call site, so primary_tile is inlined once. Persistent (1): waves
+       claim tiles from the global counter until it runs past the end; 2:
+       from their XCD's chunk of the claim order first (claim_xcd).
        Otherwise each wave renders one tile: blocks are dealt to the 8 XCDs
        round-robin, so each XCD gets a contiguous run of tiles and
        neighbouring tiles share its L2. */
     const uint32_t b = blockIdx.x;
     uint32_t t = ((b % 8u) * (xcd_blocks / 8u) + b / 8u) * WAVES_PER_BLOCK + (threadIdx.x >> 6);
+    uint32_t chunk = b & 7u, tried = 0;
     for (uint32_t round = 0;; round++) {
         if (persistent) {
             uint32_t c = 0;
-            if ((threadIdx.x & 63u) == 0) c = atomicAdd(tile_counter, 1u);
+            if ((threadIdx.x & 63u) == 0)
+                c = persistent == 2u ? claim_xcd(tile_counter + XCD_CLAIM, total_tiles, chunk, tried)
+                                     : atomicAdd(tile_counter, 1u);
             t = c;
         } else if (round > 0) {
             break;
@@ -600,24 +633,46 @@ __global__ void __launch_bounds__(COST_THREADS) k_cost_total(const uint32_t *__r
    others. A launch ends when its slowest tiles end, so those should start
    first and the cheap ones fill the end (on the 200k-triangle frame the
    tiles over the mesh take 200-600 us, the floor's a few tens). Without a
-   previous launch of this shape (use_history 0) the order is the tile
-   order. One thread per tile; each block keeps its tiles in order within
-   each group and claims its ranges with one atomic per group (ctr[0] from
-   the front, ctr[1] from the back, both zero at launch). Any cost values
-   give a permutation of the tiles, so the order never changes a result.
-   Thread 0 also advances the launch number `epoch` the redo stage tags its
-   cooperative jobs with. */
+   previous launch of this shape the order is the tile order (mode 0), or
+   (mode 2) the super-tile order of super_tile. One thread per tile; each
+   block keeps its tiles in order within each group and claims its ranges
+   with one atomic per group (ctr[0] from the front, ctr[1] from the back,
+   both zero at launch). Any cost values give a permutation of the tiles,
+   so the order never changes a result. Thread 0 also advances the launch
+   number `epoch` the redo stage tags its cooperative jobs with. */
 #define ORDER_THREADS 1024u
+#define ORDER_TILE 0u      /* tile order: each request's tiles row by row */
+#define ORDER_HISTORY 1u   /* slowest first, from the previous launch's tile times */
+#define ORDER_SUPER 2u     /* super-tile order */
+#define ST_SUPER 8u        /* super-tile: ST_SUPER x ST_SUPER tiles */
+
+/* Launch tile at claim position i in super-tile order: each request's tile
+   rows in bands of ST_SUPER, each band in blocks of ST_SUPER tile columns
+   (narrower at the right edge, lower at the bottom), the tiles of a block
+   row by row. A bijection on each request's tiles, so that a run of claims
+   covers a compact part of the frame rather than a few full-width rows. */
+__device__ static inline uint32_t super_tile(const KReq *reqs, uint32_t nreq, uint32_t i) {
+    const StDView &v = reqs[find_req(reqs, nreq, i, false)].v;
+    const uint32_t w = v.tiles_x, th = 64u >> v.tile_shift, h = (v.height1 + th - 1u) / th;
+    const uint32_t local = i - v.tile_base;
+    const uint32_t band = local / (ST_SUPER * w), q = local - band * ST_SUPER * w;
+    const uint32_t hb = min(ST_SUPER, h - band * ST_SUPER);
+    const uint32_t col = q / (ST_SUPER * hb), q2 = q - col * ST_SUPER * hb;
+    const uint32_t wc = min(ST_SUPER, w - col * ST_SUPER);
+    return v.tile_base + (band * ST_SUPER + q2 / wc) * w + col * ST_SUPER + q2 % wc;
+}
+
 __global__ void __launch_bounds__(ORDER_THREADS) k_tile_order(const uint32_t *__restrict__ tile_cost,
                                                               uint32_t *__restrict__ tile_order, uint32_t tiles,
                                                               const unsigned long long *cost_sum, uint32_t *ctr,
-                                                              uint32_t use_history, uint32_t *epoch) {
+                                                              uint32_t mode, uint32_t *epoch,
+                                                              const KReq *__restrict__ reqs, uint32_t nreq) {
     __shared__ uint32_t s_h[ORDER_THREADS / 64], s_l[ORDER_THREADS / 64], s_base[2];
     const uint32_t i = blockIdx.x * ORDER_THREADS + threadIdx.x;
     /* a new launch number for the redo stage's group records */
     if (i == 0) ((ST_GLOBAL uint32_t *)epoch)[0] += 1u;
-    if (!use_history) {
-        if (i < tiles) ((ST_GLOBAL uint32_t *)tile_order)[i] = i;
+    if (mode != ORDER_HISTORY) {
+        if (i < tiles) ((ST_GLOBAL uint32_t *)tile_order)[i] = mode == ORDER_SUPER ? super_tile(reqs, nreq, i) : i;
         return;
     }
     const uint32_t wv = threadIdx.x >> 6;
@@ -998,8 +1053,9 @@ struct LaunchCtx {
     uint32_t tile_cap;
     uint64_t tile_sig;     /* shape of the launch whose times tile_cost holds (0: none) */
     unsigned long long *cost_sum; /* sum of tile_cost over the last launch (k_cost_total) */
-    uint32_t *tile_counter; /* 64 words zeroed per launch: [0] the persistent primary stage's tile
-                               counter, [BOUNCE_CLAIM] the bounce stage's chunk claims (128 B apart) */
+    uint32_t *tile_counter; /* CLAIM_WORDS words zeroed per launch: [0] the persistent primary stage's
+                               tile counter, [BOUNCE_CLAIM] the bounce stage's chunk claims (128 B
+                               apart), [ORDER_CLAIM] k_tile_order's, [XCD_CLAIM + 32k] chunk k's */
     hipEvent_t ev[4];
     int stage_timing;      /* record an event after every stage launch */
     int graph_mode;        /* capture launches into a graph, replay while the launch shape repeats */
@@ -1061,7 +1117,7 @@ static unsigned long long g_noalloc_counts[5];
 
 static int ctx_init(LaunchCtx *c) {
     HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
-    HIP_TRY(hipMalloc(&c->tile_counter, 256));
+    HIP_TRY(hipMalloc(&c->tile_counter, CLAIM_WORDS * sizeof(uint32_t)));
     HIP_TRY(hipMalloc(&c->cost_sum, 256));
     HIP_TRY(hipMemset(c->cost_sum, 0, 256));
     for (int i = 0; i < 4; i++) HIP_TRY(hipEventCreate(&c->ev[i]));
@@ -1102,8 +1158,8 @@ extern "C" int st_init(int device) {
     if (rc) return rc;
     g.tune.tile_shift = 3;          /* 8 x 8 tiles */
     g.tune.blocks_per_cu = 4;
-    g.tune.persistent = 1;
-    g.tune.cost_order = 1;
+    g.tune.persistent = 2;          /* persistent, each XCD claiming from its own chunk first ... */
+    g.tune.cost_order = 0;          /* ... of the claim order in tile order */
     g.tune.ray_sort = 0;
     g.tune.stack_entries = 16;      /* 16-slot LDS stack windows ... */
     g.tune.primary_waves = 4;       /* ... leave room for 4 waves per SIMD in the primary stage */
@@ -1740,7 +1796,8 @@ int st_rt_launch_ctx(LaunchCtx *c, StSceneHandle *const *scenes, StHostView *hv,
     }
     /* the previous launch on this context had this shape: order the tiles
        by its tile times (when the tuning asks for it) */
-    const uint32_t use_history = (c->tile_sig == tile_sig && g.tune.cost_order) ? 1u : 0u;
+    const uint32_t use_history = (c->tile_sig == tile_sig && g.tune.cost_order == 1u) ? 1u : 0u;
+    const uint32_t order_mode = use_history ? ORDER_HISTORY : g.tune.cost_order == 2u ? ORDER_SUPER : ORDER_TILE;
     char *st = c->staging;
     char *blk = (char *)arena_take(&c->work, up);
     KReq *reqs_h = (KReq *)st;
@@ -1824,10 +1881,16 @@ int st_rt_launch_ctx(LaunchCtx *c, StSceneHandle *const *scenes, StHostView *hv,
        graph of this launch can be matched against the next one */
     uint32_t blocks;
     uint32_t xcd_blocks = 0;
+    uint32_t claim_mode = 0;
     if (persistent) {
         uint32_t want = (uint32_t)g.cus * tune.blocks_per_cu;
         uint32_t need = (tiles + WAVES_PER_BLOCK - 1) / WAVES_PER_BLOCK;
         blocks = need < want ? need : want;
+        /* the primary stage's tile claim: the global counter (1), or XCD
+           chunks first (2, when tuned so and the launch has at least
+           XCD_MIN_TILES tiles per wave: with a few tiles per wave the
+           chunks' claims, all made at once, only add contention) */
+        claim_mode = (tune.persistent == 2u && tiles >= XCD_MIN_TILES * blocks * WAVES_PER_BLOCK) ? 2u : 1u;
     } else {
         blocks = (tiles + WAVES_PER_BLOCK - 1) / WAVES_PER_BLOCK;
         blocks = (blocks + 7u) & ~7u;
@@ -1872,8 +1935,11 @@ int st_rt_launch_ctx(LaunchCtx *c, StSceneHandle *const *scenes, StHostView *hv,
         HIP_TRY(hipMemcpyAsync(blk, st, up, hipMemcpyHostToDevice, c->stream));
         if (!graph) HIP_TRY(hipEventRecord(c->ev[0], c->stream));
         hipLaunchKernelGGL(k_zero_u64, dim3((3 * n + 2 + 255) / 256), dim3(256), 0, c->stream, cnt_d, 3 * n + 2);
-        hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, c->stream, c->tile_counter, 64u);
-        static_assert(BOUNCE_CLAIM < 64 && ORDER_CLAIM + 1 < 64, "claim counters share the zeroed 64 words");
+        hipLaunchKernelGGL(k_zero_u32, dim3(CLAIM_WORDS / 256u), dim3(256), 0, c->stream, c->tile_counter,
+                           CLAIM_WORDS);
+        static_assert(BOUNCE_CLAIM < ORDER_CLAIM && ORDER_CLAIM + 1 < XCD_CLAIM &&
+                          XCD_CLAIM + 8 * XCD_CLAIM_STRIDE <= CLAIM_WORDS && CLAIM_WORDS % 256u == 0,
+                      "claim counters share the zeroed words");
         if (any_device_ssb)
             hipLaunchKernelGGL(k_ssb, dim3(n), dim3(256), 0, c->stream, (const StDScene *)sc_d, (const KReq *)reqs_d);
         if (levels)
@@ -1888,19 +1954,19 @@ int st_rt_launch_ctx(LaunchCtx *c, StSceneHandle *const *scenes, StHostView *hv,
         }
         hipLaunchKernelGGL(k_tile_order, dim3((tiles + ORDER_THREADS - 1) / ORDER_THREADS), dim3(ORDER_THREADS), 0,
                            c->stream, (const uint32_t *)c->tile_cost, c->tile_order, tiles,
-                           (const unsigned long long *)c->cost_sum, c->tile_counter + ORDER_CLAIM, use_history,
-                           (uint32_t *)(c->cost_sum + 1));
+                           (const unsigned long long *)c->cost_sum, c->tile_counter + ORDER_CLAIM, order_mode,
+                           (uint32_t *)(c->cost_sum + 1), (const KReq *)reqs_d, n);
         HIP_TRY(hipGetLastError());
         HIP_TRY(mark());
         if (waves4)
             hipLaunchKernelGGL(k_primary4, dim3(blocks), dim3(BLOCK_THREADS), lds_primary, c->stream,
                                (const StDScene *)sc_d, (const KReq *)reqs_d, n, tiles, c->tile_counter,
-                               (uint32_t)persistent, xcd_blocks, ks, km, w, (const uint32_t *)c->tile_order,
+                               claim_mode, xcd_blocks, ks, km, w, (const uint32_t *)c->tile_order,
                                c->tile_cost);
         else
             hipLaunchKernelGGL(k_primary, dim3(blocks), dim3(BLOCK_THREADS), lds_primary, c->stream,
                                (const StDScene *)sc_d, (const KReq *)reqs_d, n, tiles, c->tile_counter,
-                               (uint32_t)persistent, xcd_blocks, ks, km, w, (const uint32_t *)c->tile_order,
+                               claim_mode, xcd_blocks, ks, km, w, (const uint32_t *)c->tile_order,
                                c->tile_cost);
         HIP_TRY(hipGetLastError());
         HIP_TRY(mark());
@@ -1945,11 +2011,11 @@ int st_rt_launch_ctx(LaunchCtx *c, StSceneHandle *const *scenes, StHostView *hv,
            graph does exactly this launch's work (the per-frame data is in
            the upload block, copied from the same pinned staging) */
         std::vector<uint64_t> sig = {n, up, rows, tiles, levels, lmax, count_blocks, blocks, xcd_blocks, any_device_ssb,
-                                     (uint64_t)persistent, s_entries, m_entries, ks, km, (uint64_t)waves4, b_bounce,
+                                     (uint64_t)claim_mode, s_entries, m_entries, ks, km, (uint64_t)waves4, b_bounce,
                                      lds_primary, (uint64_t)(uintptr_t)blk,
                                      (uint64_t)(uintptr_t)st,
                                      (uint64_t)(uintptr_t)cnt_d, (uint64_t)(uintptr_t)w.st, w.n,
-                                     (uint64_t)(uintptr_t)c->tile_cost, use_history, cost_blocks,
+                                     (uint64_t)(uintptr_t)c->tile_cost, order_mode, cost_blocks,
                                      (uint64_t)(uintptr_t)w.redo_q, redo_blocks, redo_threads, lds_redo,
                                      (uint64_t)(uintptr_t)coop.base, (uint64_t)ray_sort,
                                      (uint64_t)(uintptr_t)w.sorted_q, (uint64_t)(uintptr_t)spill, spill_s, spill_m};
@@ -2100,7 +2166,7 @@ extern "C" int st_render_noalloc(StSceneHandle *scene, const StView *view, StFlo
     };
     const size_t o_cnt = take(40), o_dirs = take((size_t)height1 * ST_DIR_MARKS(width1) * 16), o_pix = take(px * 24);
     const size_t o_hits = hits ? take(px * 16) : 0;
-    const size_t o_order = take((size_t)tiles * 4), o_cost = take((size_t)tiles * 4), o_ctr = take(256);
+    const size_t o_order = take((size_t)tiles * 4), o_cost = take((size_t)tiles * 4), o_ctr = take(CLAIM_WORDS * 4);
     const size_t o_redo = take(slots * 4);
     const uint32_t s_entries = scene->dev.scene_stack_need > 1 ? scene->dev.scene_stack_need : 1;
     const uint32_t m_entries = scene->dev.mesh_stack_size > 1 ? scene->dev.mesh_stack_size : 1;
@@ -2183,9 +2249,11 @@ extern "C" int st_render_noalloc(StSceneHandle *scene, const StView *view, StFlo
     if (lds > 160 * 1024) return set_err(ST_ERR_SCENE, "traversal stacks need %zu B of LDS", lds);
     uint32_t blocks, xcd_blocks = 0;
     size_t lds_primary = lds;
+    uint32_t claim_mode = 0;
     if (tune.persistent) {
         const uint32_t want = (uint32_t)g.cus * tune.blocks_per_cu, need = (tiles + WAVES_PER_BLOCK - 1) / WAVES_PER_BLOCK;
         blocks = need < want ? need : want;
+        claim_mode = (tune.persistent == 2u && tiles >= XCD_MIN_TILES * blocks * WAVES_PER_BLOCK) ? 2u : 1u;
     } else {
         blocks = ((tiles + WAVES_PER_BLOCK - 1) / WAVES_PER_BLOCK + 7u) & ~7u;
         xcd_blocks = blocks;
@@ -2200,17 +2268,18 @@ extern "C" int st_render_noalloc(StSceneHandle *scene, const StView *view, StFlo
        by the device SSB pass */
     HIP_TRY(hipMemcpyAsync(base, g_noalloc_host, up_host, hipMemcpyHostToDevice, stream));
     hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, stream, cnt_d, 5u);
-    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, stream, ctr, 64u);
+    hipLaunchKernelGGL(k_zero_u32, dim3(CLAIM_WORDS / 256u), dim3(256), 0, stream, ctr, CLAIM_WORDS);
     hipLaunchKernelGGL(k_ssb, dim3(1), dim3(256), 0, stream, sc_d, reqs_d);
     if (beauty) hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, stream, w.ctr, 2u);
     hipLaunchKernelGGL(k_primary_dirs, dim3((height1 * 3 + DIRS_BLOCK - 1) / DIRS_BLOCK), dim3(DIRS_BLOCK), 0, stream,
                        reqs_d, 1u, height1);
     hipLaunchKernelGGL(k_tile_order, dim3((tiles + ORDER_THREADS - 1) / ORDER_THREADS), dim3(ORDER_THREADS), 0, stream,
                        (const uint32_t *)(base + o_cost), (uint32_t *)(base + o_order), tiles,
-                       (const unsigned long long *)cnt_d, ctr + ORDER_CLAIM, 0u, epoch);
+                       (const unsigned long long *)cnt_d, ctr + ORDER_CLAIM,
+                       tune.cost_order == 2u ? ORDER_SUPER : ORDER_TILE, epoch, reqs_d, 1u);
     HIP_TRY(hipGetLastError());
     hipLaunchKernelGGL(k_primary, dim3(blocks), dim3(BLOCK_THREADS), lds_primary, stream, sc_d, reqs_d, 1u, tiles, ctr,
-                       (uint32_t)tune.persistent, xcd_blocks, ks, km, w,
+                       claim_mode, xcd_blocks, ks, km, w,
                        (const uint32_t *)(base + o_order), (uint32_t *)(base + o_cost));
     HIP_TRY(hipGetLastError());
     if (beauty && dv.depth > 1) {
@@ -2476,8 +2545,8 @@ extern "C" void st_host_free(void *p) {
 /* ---- launch tuning ------------------------------------------------------ */
 
 static int tuning_valid(const StTuning *t) {
-    return t->tile_shift <= 6 && t->blocks_per_cu >= 1 && t->blocks_per_cu <= 64 && t->persistent <= 1 &&
-           t->cost_order <= 1 && t->ray_sort <= 1 &&
+    return t->tile_shift <= 6 && t->blocks_per_cu >= 1 && t->blocks_per_cu <= 64 && t->persistent <= 2 &&
+           t->cost_order <= 2 && t->ray_sort <= 1 &&
            (t->stack_entries == 0 || (t->stack_entries >= 4 && t->stack_entries <= 64 &&
                                       (t->stack_entries & (t->stack_entries - 1)) == 0)) &&
            (t->primary_waves == 0 || t->primary_waves == 3 || t->primary_waves == 4);
@@ -2507,7 +2576,10 @@ extern "C" int st_get_tuning(StTuning *t) {
    of 2, 4 or 8 blocks per CU, or one block per 4 tiles in XCD-aware order
    with at most 2, 3 or any number of blocks per CU resident; tiles claimed
    in tile order or slowest first (that order needs the previous launch's
-   tile times: one untimed launch of the shape comes first). */
+   tile times: one untimed launch of the shape comes first). Then the best
+   with the bounce-ray sort toggled, with the other persistent tile claim
+   (global counter or XCD chunks), in super-tile claim order, and with each
+   traversal-stack window and primary-stage occupancy. */
 extern "C" int st_autotune(StSceneHandle *const *scenes, const StView *views, uint32_t n, uint32_t reps,
                            StTuning *best, float *best_ms) {
     RT_LOCK;
@@ -2517,7 +2589,7 @@ extern "C" int st_autotune(StSceneHandle *const *scenes, const StView *views, ui
     if (reps == 0) reps = 3;
     if (reps > 15) reps = 15;
     const StTuning saved = g.tune;
-    StTuning cand[44];
+    StTuning cand[36 + 8];
     uint32_t nc = 0;
     for (uint32_t sh = 3; sh <= 5; sh++) {
         const uint32_t bpc[3] = {2, 4, 8}, cap[3] = {2, 3, 8};
@@ -2531,17 +2603,26 @@ extern "C" int st_autotune(StSceneHandle *const *scenes, const StView *views, ui
     float best_t = INFINITY;
     StTuning best_c = saved;
     /* the shapes and claim orders; then the best of them with the bounce-ray
-       sort toggled; then the best with each traversal-stack window and
-       primary-stage occupancy */
+       sort toggled; then, when persistent, with the other tile claim (global
+       counter or XCD chunks); then in super-tile order; then the best with
+       each traversal-stack window and primary-stage occupancy */
     const uint32_t windows[5][2] = {{0u, 3u}, {16u, 3u}, {32u, 3u}, {8u, 4u}, {16u, 4u}};
-    for (uint32_t c = 0; c < nc + 6u; c++) {
+    for (uint32_t c = 0; c < nc + 8u; c++) {
         if (c == nc) {
             cand[c] = best_c;
             cand[c].ray_sort ^= 1u;
-        } else if (c > nc) {
+        } else if (c == nc + 1u) {
+            if (!best_c.persistent) continue;
             cand[c] = best_c;
-            cand[c].stack_entries = windows[c - nc - 1u][0];
-            cand[c].primary_waves = windows[c - nc - 1u][1];
+            cand[c].persistent = best_c.persistent == 2u ? 1u : 2u;
+        } else if (c == nc + 2u) {
+            if (best_c.cost_order == 2u) continue;
+            cand[c] = best_c;
+            cand[c].cost_order = 2u;
+        } else if (c > nc + 2u) {
+            cand[c] = best_c;
+            cand[c].stack_entries = windows[c - nc - 3u][0];
+            cand[c].primary_waves = windows[c - nc - 3u][1];
             if (cand[c].stack_entries == best_c.stack_entries && cand[c].primary_waves == best_c.primary_waves)
                 continue;
         }

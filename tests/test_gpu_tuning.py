@@ -1,7 +1,8 @@
-"""GPU: every launch shape, tile claim order and bounce-ray order the
-autotuner may pick renders the same frames bit for bit (tiles only change
-which wavefront traces a pixel, the claim order and the bounce-ray sort only
-when), and the autotuner keeps one of its candidates."""
+"""GPU: every launch shape, tile claim (one counter or XCD chunks), claim
+order and bounce-ray order the autotuner may pick renders the same frames
+bit for bit (tiles only change which wavefront traces a pixel, the claims,
+the claim order and the bounce-ray sort only when), and the autotuner keeps
+one of its candidates."""
 from __future__ import annotations
 
 import numpy as np
@@ -24,17 +25,18 @@ def test_all_launch_shapes_give_identical_frames():
         ref = None
         for shift in (3, 4, 5):
             for bpc in (2, 3, 8):
-                for persistent in (True, False):
-                    for slowest_first in (False, True):
+                for persistent, xcd in ((True, False), (True, True), (False, False)):
+                    for order in ("tiles", "slowest_first", "super_tiles"):
                         for ray_sort in (False, True):
-                            st.set_tuning(shift, bpc, persistent, slowest_first, ray_sort)
+                            st.set_tuning(shift, bpc, persistent, order == "slowest_first", ray_sort, xcd_claim=xcd,
+                                          super_tiles=order == "super_tiles")
                             for _ in range(2):   # the second launch has its predecessor's tile times
                                 pix, hits, counts = st.render_batch([ds, ds], views, hits=True)
                                 got = b"".join(p.tobytes() for p in pix) + b"".join(h.tobytes() for h in hits)
                                 got += b"".join(np.array([c.shadow, c.bounce], np.uint64).tobytes() for c in counts)
                                 if ref is None:
                                     ref = got
-                                assert got == ref, (shift, bpc, persistent, slowest_first, ray_sort)
+                                assert got == ref, (shift, bpc, persistent, xcd, order, ray_sort)
         best, ms = st.autotune([ds, ds], views, reps=3)
         assert best.tile_shift in (3, 4, 5) and ms > 0
         print("autotune:", best.as_dict(), f"{ms:.3f} ms", flush=True)
