@@ -191,8 +191,12 @@ def c4_leg(args, blobs, views, sids):
     build_s = time.perf_counter() - t0
     rs = [dscenes[i] for i in sids]
     st.render_timed(rs, views)                      # grows the launch arena to the whole batch
-    tune, _ = st.autotune(rs, views, reps=1)
-    log(f"c4: {len(dscenes)} scenes built in {build_s:.1f} s; autotune {tune.as_dict()}")
+    if args.fixed_shape:
+        tune = st.get_tuning()
+    else:
+        tune, _ = st.autotune(rs, views, reps=1)
+    log(f"c4: {len(dscenes)} scenes built in {build_s:.1f} s; {'fixed' if args.fixed_shape else 'autotune'} "
+        f"{tune.as_dict()}")
     for _ in range(args.warmup):
         st.render_timed(rs, views, stages=True)
     step_ms, stage_runs, rays = [], [], 0
@@ -629,7 +633,7 @@ def served_leg(args):
 
 # ---------------------------------------------------------------- roofline
 
-TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "k_primary_batched256_c4.txt")
+TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "k_primary4_batched256_c4.txt")
 
 
 def measured_traffic():
@@ -669,6 +673,9 @@ def main():
                     help="comma list of " + ",".join(LEGS) + " (c4 is the headline; cpu adds the CPU baselines "
                          "and the oracle parity of the headline)")
     ap.add_argument("--copy-probe", action="store_true", help="only run the copy kernel used to calibrate PMC bytes")
+    ap.add_argument("--fixed-shape", action="store_true",
+                    help="headline leg at the library's default launch shape instead of the autotuner's choice "
+                         "(so that several profiling runs time the same launches)")
     ap.add_argument("--batched-only", action="store_true",
                     help="headline leg without its plain-launch and download comparisons (for profiling runs)")
     ap.add_argument("--instanced-frames", type=int, default=10, help="instanced leg: timed frames")
@@ -726,7 +733,9 @@ def main():
         dominant = max(("primary", "bounce"), key=lambda k: stage.get(k, 0.0))
         if dominant == "primary":
             nbytes = agg.traversal_bytes("primary") + agg.traversal_bytes("shadow0")
-            kname = "k_primary (primary rays and the first level's shadow rays, all 256 requests)"
+            # the 4-waves-per-SIMD build of the same stage when the launch shape asks for it
+            kname = ("k_primary4" if c4["tune"].primary_waves == 4 else "k_primary") + \
+                " (primary rays and the first level's shadow rays, all 256 requests)"
         else:
             nbytes = agg.traversal_bytes("bounce") + agg.traversal_bytes("shadow_deep")
             kname = "k_bounce (bounce rays and the deeper levels' shadow rays, all 256 requests)"

@@ -57,9 +57,10 @@
 
 #define WAVES_PER_BLOCK 4
 #define BLOCK_THREADS (64 * WAVES_PER_BLOCK)
-/* Launch attributes of the primary stage (occupancy experiments). */
+/* Launch attributes of the primary stage (occupancy experiments): 3 waves
+   per SIMD, a register budget of 168 */
 #ifndef K_PRIMARY_ATTR
-#define K_PRIMARY_ATTR
+#define K_PRIMARY_ATTR __attribute__((amdgpu_waves_per_eu(3)))
 #endif
 /* Launch attributes of the bounce stage (occupancy experiments). */
 #ifndef K_BOUNCE_ATTR
