@@ -61,6 +61,20 @@ typedef struct StSceneInfo {
 int st_init(int device);
 const char *st_last_error(void);
 
+/* The device the library runs on (initialising it with device 0 if no
+   st_init came first): name and architecture as the runtime reports them,
+   compute units, HBM bytes (name "AMD GPU <arch>" when the runtime reports
+   none). No counterpart in the reference (its GPU build
+   takes the first CUDA device); the tuning store keys launch shapes by it. */
+typedef struct StDeviceInfo {
+    char name[64];
+    char arch[32];        /* e.g. "gfx950:sramecc+:xnack-" */
+    int32_t device;
+    uint32_t compute_units;
+    uint64_t hbm_bytes;
+} StDeviceInfo;
+int st_device_info(StDeviceInfo *info);
+
 /* Parse a scene blob, build its BVHs (the reference's SAH builder unless the
    blob carries prebuilt ones) and copy it to HBM. */
 int st_scene_create(const void *blob, size_t size, StSceneHandle **out);

@@ -70,6 +70,20 @@ def init(device: int = 0):
     _check(lib().st_init(C.c_int(device)))
 
 
+class DeviceInfo(C.Structure):
+    _fields_ = [("name", C.c_char * 64), ("arch", C.c_char * 32), ("device", C.c_int32),
+                ("compute_units", C.c_uint32), ("hbm_bytes", C.c_uint64)]
+
+
+def device_info() -> dict:
+    """The GPU the library runs on (st_device_info): name, architecture,
+    device index, compute units, HBM bytes."""
+    d = DeviceInfo()
+    _check(lib().st_device_info(C.byref(d)))
+    return {"name": d.name.decode(errors="replace"), "arch": d.arch.decode(errors="replace"), "device": d.device,
+            "compute_units": d.compute_units, "hbm_bytes": d.hbm_bytes}
+
+
 def _take_blob(rc, out, size):
     _check(rc)
     try:

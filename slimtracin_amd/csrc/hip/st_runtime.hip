@@ -1145,6 +1145,26 @@ static int ensure_init(void) {
     return st_init(0);
 }
 
+extern "C" int st_device_info(StDeviceInfo *info) {
+    RT_LOCK;
+    if (!info) return set_err(ST_ERR_ARG, "null info");
+    int rc = ensure_init();
+    if (rc) return rc;
+    hipDeviceProp_t prop;
+    HIP_TRY(hipGetDeviceProperties(&prop, g.device));
+    memset(info, 0, sizeof *info);
+    snprintf(info->arch, sizeof info->arch, "%s", prop.gcnArchName);
+    /* some runtimes report no marketing name: the architecture stands in */
+    if (prop.name[0])
+        snprintf(info->name, sizeof info->name, "%s", prop.name);
+    else
+        snprintf(info->name, sizeof info->name, "AMD GPU %.*s", (int)strcspn(info->arch, ":"), info->arch);
+    info->device = g.device;
+    info->compute_units = (uint32_t)g.cus;
+    info->hbm_bytes = (uint64_t)prop.totalGlobalMem;
+    return ST_OK;
+}
+
 extern "C" int st_init(int device) {
     RT_LOCK;
     if (g.ready) {

@@ -43,3 +43,37 @@ def test_all_launch_shapes_give_identical_frames():
     finally:
         st.restore_tuning(saved)
         ds.close()
+
+
+def test_xcd_claim_gives_identical_frames():
+    """A batch over three scenes with enough tiles per wave (one block per
+    CU) that the XCD-aware claim is used: the frames equal those of the
+    one-counter claim, in tile and super-tile order."""
+    built = [scenes.ex04_glass_mirror(), scenes.ex09_textures(), scenes.ex07_monkey()]
+    ds = [st.DeviceScene(s.to_blob()) for s, _ in built]
+    saved = st.get_tuning()
+    try:
+        rs, views = [], []
+        for k in range(6):   # scenes interleaved in batch order: 0 1 2 0 1 2
+            s, cam = built[k % 3]
+            rs.append(ds[k % 3])
+            views.append(make_view(width=1024 - 64 * k, height=640 + 16 * k, mode=MODE_BEAUTY, **cam))
+        views = np.concatenate(views)
+        # 8x8 tiles over the traced (width + 1) x (height + 1) grids: at least
+        # 32 per wave of a one-block-per-CU grid on a 256-CU MI355X
+        tiles = sum((int(v["width"]) + 8) // 8 * ((int(v["height"]) + 8) // 8) for v in views)
+        assert tiles >= 32 * 4 * 256, tiles
+        ref = None
+        for xcd in (False, True):
+            for super_tiles in (False, True):
+                st.set_tuning(3, 1, True, False, False, 16, 4, xcd_claim=xcd, super_tiles=super_tiles)
+                pix, hits, counts = st.render_batch(rs, views, hits=True)
+                got = b"".join(p.tobytes() for p in pix) + b"".join(h.tobytes() for h in hits)
+                got += b"".join(np.array([c.shadow, c.bounce], np.uint64).tobytes() for c in counts)
+                if ref is None:
+                    ref = got
+                assert got == ref, (xcd, super_tiles)
+    finally:
+        st.restore_tuning(saved)
+        for d in ds:
+            d.close()
