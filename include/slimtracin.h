@@ -165,7 +165,9 @@ typedef struct StTuning {
                                 persistent launch (a power of two, 4-64): older entries go to the launch's
                                 spill buffer in device memory; 0: the whole stack the launch's scenes need */
     uint32_t primary_waves;  /* 4: the primary stage held to 4 waves per SIMD (when its stacks leave LDS for
-                                them); 3 or 0: the occupancy its registers allow (3) */
+                                them); 3 or 0: 3 waves per SIMD */
+    uint32_t bounce_order;   /* which pixels' depth loops the bounce stage starts first: 0 the refracting
+                                ones, 1 the others (the stage ends with its longest chains) */
 } StTuning;
 
 int st_set_tuning(const StTuning *tuning);
@@ -175,8 +177,8 @@ int st_get_tuning(StTuning *tuning);
    the bounce-ray sort, with either persistent tile claim (1 or 2), in
    super-tile claim order (cost_order 2), then with
    traversal-stack windows (whole, 8, 16, 32 slots) and primary-stage
-   occupancies (3, 4 waves per SIMD), and keeps the fastest; returns it and
-   its time. */
+   occupancies (3, 4 waves per SIMD), then with the other bounce_order, and
+   keeps the fastest; returns it and its time. */
 int st_autotune(StSceneHandle *const *scenes, const StView *views, uint32_t n, uint32_t reps, StTuning *best,
                 float *best_ms);
 

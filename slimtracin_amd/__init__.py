@@ -354,14 +354,15 @@ def resolve_bgra8(grid: np.ndarray, grid_w: int, grid_h: int) -> np.ndarray:
 class Tuning(C.Structure):
     _fields_ = [("tile_shift", C.c_uint32), ("blocks_per_cu", C.c_uint32), ("persistent", C.c_uint32),
                 ("cost_order", C.c_uint32), ("ray_sort", C.c_uint32), ("stack_entries", C.c_uint32),
-                ("primary_waves", C.c_uint32)]
+                ("primary_waves", C.c_uint32), ("bounce_order", C.c_uint32)]
 
     def as_dict(self):
         return {"tile": f"{1 << self.tile_shift}x{64 >> self.tile_shift}", "blocks_per_cu": self.blocks_per_cu,
                 "persistent": bool(self.persistent), "xcd_claim": self.persistent == 2,
                 "slowest_first": self.cost_order == 1, "super_tiles": self.cost_order == 2,
                 "ray_sort": bool(self.ray_sort), "stack_entries": self.stack_entries or "whole",
-                "primary_waves_per_simd": self.primary_waves or 3}
+                "primary_waves_per_simd": self.primary_waves or 3,
+                "bounce_first": "others" if self.bounce_order else "refracting"}
 
 
 def get_tuning() -> Tuning:
@@ -372,14 +373,17 @@ def get_tuning() -> Tuning:
 
 def set_tuning(tile_shift: int = 3, blocks_per_cu: int = 4, persistent: bool = True, slowest_first: bool = False,
                ray_sort: bool = False, stack_entries: int = 16, primary_waves: int = 4, xcd_claim: bool = True,
-               super_tiles: bool = False):
+               super_tiles: bool = False, refracting_first: bool = False):
     """Launch shape (StTuning); the defaults are the library's. xcd_claim:
     a persistent launch's tiles claimed by each XCD from its own eighth of
     the claim order first; super_tiles (when not slowest_first): tiles
-    claimed request by request in blocks of 8 x 8 tiles."""
+    claimed request by request in blocks of 8 x 8 tiles; refracting_first:
+    the bounce stage starts with the refracting pixels' depth loops rather
+    than the others'."""
     mode = (2 if xcd_claim else 1) if persistent else 0
     order = 1 if slowest_first else 2 if super_tiles else 0
-    t = Tuning(tile_shift, blocks_per_cu, mode, order, 1 if ray_sort else 0, stack_entries, primary_waves)
+    t = Tuning(tile_shift, blocks_per_cu, mode, order, 1 if ray_sort else 0, stack_entries, primary_waves,
+               0 if refracting_first else 1)
     _check(lib().st_set_tuning(C.byref(t)))
 
 

@@ -750,19 +750,25 @@ __device__ static inline __attribute__((always_inline)) void drain_queue(const K
    shading) in the same lane, and a pixel that bounces again goes straight
    on to its next bounce ray with its state in registers. No level waits for
    the previous level's slowest wave, which at these depths, with a few
-   rays per SIMD, sets a level's time. */
+   rays per SIMD, sets a level's time. The stage ends with its longest
+   chains (a chunk whose 64 pixels all bounce to the full depth over large
+   meshes), so the part of the queue that holds them should start first:
+   back_first 0 takes the refracting pixels (the queue's front part) first,
+   1 the others (its back part; on the headline batch the reflecting
+   pixels' chains are the long ones: 4.9 against 5.3 ms). */
 __global__ void __launch_bounds__(BLOCK_THREADS) K_BOUNCE_ATTR k_bounce(const StDScene *__restrict__ scenes,
                                                           const KReq *__restrict__ reqs, uint32_t nreq, KWave w,
                                                           uint32_t *claim, uint32_t scene_entries,
-                                                          uint32_t mesh_entries) {
+                                                          uint32_t mesh_entries, uint32_t back_first) {
     ST_LDS_STACKS(scene_entries, mesh_entries);
     const DStacks stk = {scene_stack, mesh_stack, nullptr};
-    /* the queue's front part (ctr[0] entries) first, then its back part
-       (ctr[1] entries, stored from the end of the buffer down) */
+    /* the queue's front part (ctr[0] entries) and its back part (ctr[1]
+       entries, stored from the end of the buffer down), in back_first's order */
     const uint32_t n_front = __builtin_amdgcn_readfirstlane(ldgu(w.ctr, 0));
     const uint32_t n_back = __builtin_amdgcn_readfirstlane(ldgu(w.ctr, 1));
-    const uint32_t c_front = (n_front + 63u) / 64u;
-    const uint32_t chunks = w.sorted_q ? (n_front + n_back + 63u) / 64u : c_front + (n_back + 63u) / 64u;
+    const uint32_t c_front = (n_front + 63u) / 64u, c_back = (n_back + 63u) / 64u;
+    const uint32_t c_first = back_first ? c_back : c_front;
+    const uint32_t chunks = w.sorted_q ? (n_front + n_back + 63u) / 64u : c_front + c_back;
     drain_queue(reqs, nreq, chunks, claim,
                 [&](uint32_t chunk, uint32_t lane, uint32_t &slot, uint32_t &) {
                     if (w.sorted_q) {
@@ -770,15 +776,12 @@ __global__ void __launch_bounds__(BLOCK_THREADS) K_BOUNCE_ATTR k_bounce(const St
                         const uint32_t qi = chunk * 64u + lane;
                         if (qi >= n_front + n_back) return false;
                         slot = ldgu(w.sorted_q, qi);
-                    } else if (chunk < c_front) {
-                        const uint32_t qi = chunk * 64u + lane;
-                        if (qi >= n_front) return false;
-                        slot = ldgu(w.bounce_q, qi);
-                    } else {
-                        const uint32_t qi = (chunk - c_front) * 64u + lane;
-                        if (qi >= n_back) return false;
-                        slot = ldgu(w.bounce_q, w.n - 1u - qi);
+                        return true;
                     }
+                    const bool back = (chunk < c_first) == (back_first != 0u);
+                    const uint32_t qi = (chunk < c_first ? chunk : chunk - c_first) * 64u + lane;
+                    if (qi >= (back ? n_back : n_front)) return false;
+                    slot = ldgu(w.bounce_q, back ? w.n - 1u - qi : qi);
                     return true;
                 },
                 [&](uint32_t r, uint32_t slot, uint32_t) {
@@ -1184,6 +1187,7 @@ extern "C" int st_init(int device) {
     g.tune.ray_sort = 0;
     g.tune.stack_entries = 16;      /* 16-slot LDS stack windows ... */
     g.tune.primary_waves = 4;       /* ... leave room for 4 waves per SIMD in the primary stage */
+    g.tune.bounce_order = 1;        /* the bounce stage starts with the pixels that do not refract */
     hipDeviceProp_t prop;
     g.cus = 256;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
@@ -2008,7 +2012,7 @@ int st_rt_launch_ctx(LaunchCtx *c, StSceneHandle *const *scenes, StHostView *hv,
             }
             hipLaunchKernelGGL(k_bounce, dim3(b_bounce), dim3(BLOCK_THREADS), lds, c->stream,
                                (const StDScene *)sc_d, (const KReq *)reqs_d, n, w, c->tile_counter + BOUNCE_CLAIM,
-                               ks, km);
+                               ks, km, tune.bounce_order);
             HIP_TRY(mark());
         }
         /* the pixels that gave up, traced again (it reads the queue's length
@@ -2032,7 +2036,7 @@ int st_rt_launch_ctx(LaunchCtx *c, StSceneHandle *const *scenes, StHostView *hv,
            graph does exactly this launch's work (the per-frame data is in
            the upload block, copied from the same pinned staging) */
         std::vector<uint64_t> sig = {n, up, rows, tiles, levels, lmax, count_blocks, blocks, xcd_blocks, any_device_ssb,
-                                     (uint64_t)claim_mode, s_entries, m_entries, ks, km, (uint64_t)waves4, b_bounce,
+                                     (uint64_t)claim_mode, s_entries, m_entries, ks, km, (uint64_t)waves4, b_bounce, tune.bounce_order,
                                      lds_primary, (uint64_t)(uintptr_t)blk,
                                      (uint64_t)(uintptr_t)st,
                                      (uint64_t)(uintptr_t)cnt_d, (uint64_t)(uintptr_t)w.st, w.n,
@@ -2306,7 +2310,7 @@ extern "C" int st_render_noalloc(StSceneHandle *scene, const StView *view, StFlo
     if (beauty && dv.depth > 1) {
         const uint32_t b_bounce = resident_blocks(reinterpret_cast<const void *>(k_bounce), lds);
         hipLaunchKernelGGL(k_bounce, dim3(b_bounce), dim3(BLOCK_THREADS), lds, stream, sc_d, reqs_d, 1u, w,
-                           ctr + BOUNCE_CLAIM, ks, km);
+                           ctr + BOUNCE_CLAIM, ks, km, tune.bounce_order);
     }
     hipLaunchKernelGGL(k_redo, dim3(redo_blocks), dim3(redo_threads), lds_redo, stream, sc_d, reqs_d, 1u, w,
                        s_entries, m_entries, coop);
@@ -2567,7 +2571,7 @@ extern "C" void st_host_free(void *p) {
 
 static int tuning_valid(const StTuning *t) {
     return t->tile_shift <= 6 && t->blocks_per_cu >= 1 && t->blocks_per_cu <= 64 && t->persistent <= 2 &&
-           t->cost_order <= 2 && t->ray_sort <= 1 &&
+           t->cost_order <= 2 && t->ray_sort <= 1 && t->bounce_order <= 1 &&
            (t->stack_entries == 0 || (t->stack_entries >= 4 && t->stack_entries <= 64 &&
                                       (t->stack_entries & (t->stack_entries - 1)) == 0)) &&
            (t->primary_waves == 0 || t->primary_waves == 3 || t->primary_waves == 4);
@@ -2599,8 +2603,9 @@ extern "C" int st_get_tuning(StTuning *t) {
    in tile order or slowest first (that order needs the previous launch's
    tile times: one untimed launch of the shape comes first). Then the best
    with the bounce-ray sort toggled, with the other persistent tile claim
-   (global counter or XCD chunks), in super-tile claim order, and with each
-   traversal-stack window and primary-stage occupancy. */
+   (global counter or XCD chunks), in super-tile claim order, with each
+   traversal-stack window and primary-stage occupancy, and with the bounce
+   queue's parts in the other order. */
 extern "C" int st_autotune(StSceneHandle *const *scenes, const StView *views, uint32_t n, uint32_t reps,
                            StTuning *best, float *best_ms) {
     RT_LOCK;
@@ -2610,15 +2615,17 @@ extern "C" int st_autotune(StSceneHandle *const *scenes, const StView *views, ui
     if (reps == 0) reps = 3;
     if (reps > 15) reps = 15;
     const StTuning saved = g.tune;
-    StTuning cand[36 + 8];
+    StTuning cand[36 + 9];
     uint32_t nc = 0;
     for (uint32_t sh = 3; sh <= 5; sh++) {
         const uint32_t bpc[3] = {2, 4, 8}, cap[3] = {2, 3, 8};
         for (uint32_t order = 0; order <= 1; order++) {
             for (int k = 0; k < 3; k++)
-                cand[nc++] = StTuning{sh, bpc[k], 1u, order, saved.ray_sort, saved.stack_entries, saved.primary_waves};
+                cand[nc++] = StTuning{sh, bpc[k], 1u, order, saved.ray_sort, saved.stack_entries,
+                                      saved.primary_waves, saved.bounce_order};
             for (int k = 0; k < 3; k++)
-                cand[nc++] = StTuning{sh, cap[k], 0u, order, saved.ray_sort, saved.stack_entries, saved.primary_waves};
+                cand[nc++] = StTuning{sh, cap[k], 0u, order, saved.ray_sort, saved.stack_entries,
+                                      saved.primary_waves, saved.bounce_order};
         }
     }
     float best_t = INFINITY;
@@ -2626,12 +2633,16 @@ extern "C" int st_autotune(StSceneHandle *const *scenes, const StView *views, ui
     /* the shapes and claim orders; then the best of them with the bounce-ray
        sort toggled; then, when persistent, with the other tile claim (global
        counter or XCD chunks); then in super-tile order; then the best with
-       each traversal-stack window and primary-stage occupancy */
+       each traversal-stack window and primary-stage occupancy; then with
+       the bounce queue's parts in the other order */
     const uint32_t windows[5][2] = {{0u, 3u}, {16u, 3u}, {32u, 3u}, {8u, 4u}, {16u, 4u}};
-    for (uint32_t c = 0; c < nc + 8u; c++) {
+    for (uint32_t c = 0; c < nc + 9u; c++) {
         if (c == nc) {
             cand[c] = best_c;
             cand[c].ray_sort ^= 1u;
+        } else if (c == nc + 8u) {
+            cand[c] = best_c;
+            cand[c].bounce_order ^= 1u;
         } else if (c == nc + 1u) {
             if (!best_c.persistent) continue;
             cand[c] = best_c;
@@ -2640,7 +2651,7 @@ extern "C" int st_autotune(StSceneHandle *const *scenes, const StView *views, ui
             if (best_c.cost_order == 2u) continue;
             cand[c] = best_c;
             cand[c].cost_order = 2u;
-        } else if (c > nc + 2u) {
+        } else if (c > nc + 2u && c < nc + 8u) {
             cand[c] = best_c;
             cand[c].stack_entries = windows[c - nc - 3u][0];
             cand[c].primary_waves = windows[c - nc - 3u][1];
@@ -2649,7 +2660,7 @@ extern "C" int st_autotune(StSceneHandle *const *scenes, const StView *views, ui
         }
         g.tune = cand[c];
         float times[15];
-        if (cand[c].cost_order)
+        if (cand[c].cost_order == 1u)   /* slowest first: the shape's tile times first */
             rc = launch(scenes, views, n, 0, (int)cand[c].persistent, NULL, NULL, NULL, NULL, &times[0]);
         for (uint32_t r = 0; r < reps && !rc; r++)
             rc = launch(scenes, views, n, 0, (int)cand[c].persistent, NULL, NULL, NULL, NULL, &times[r]);

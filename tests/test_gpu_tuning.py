@@ -28,8 +28,9 @@ def test_all_launch_shapes_give_identical_frames():
                 for persistent, xcd in ((True, False), (True, True), (False, False)):
                     for order in ("tiles", "slowest_first", "super_tiles"):
                         for ray_sort in (False, True):
+                            # the bounce queue's parts in either order (the sort replaces them)
                             st.set_tuning(shift, bpc, persistent, order == "slowest_first", ray_sort, xcd_claim=xcd,
-                                          super_tiles=order == "super_tiles")
+                                          super_tiles=order == "super_tiles", refracting_first=(shift + bpc) % 2 == 0)
                             for _ in range(2):   # the second launch has its predecessor's tile times
                                 pix, hits, counts = st.render_batch([ds, ds], views, hits=True)
                                 got = b"".join(p.tobytes() for p in pix) + b"".join(h.tobytes() for h in hits)
@@ -48,7 +49,8 @@ def test_all_launch_shapes_give_identical_frames():
 def test_xcd_claim_gives_identical_frames():
     """A batch over three scenes with enough tiles per wave (one block per
     CU) that the XCD-aware claim is used: the frames equal those of the
-    one-counter claim, in tile and super-tile order."""
+    one-counter claim, in tile and super-tile order, with the bounce stage
+    starting on either part of its queue."""
     built = [scenes.ex04_glass_mirror(), scenes.ex09_textures(), scenes.ex07_monkey()]
     ds = [st.DeviceScene(s.to_blob()) for s, _ in built]
     saved = st.get_tuning()
@@ -66,7 +68,9 @@ def test_xcd_claim_gives_identical_frames():
         ref = None
         for xcd in (False, True):
             for super_tiles in (False, True):
-                st.set_tuning(3, 1, True, False, False, 16, 4, xcd_claim=xcd, super_tiles=super_tiles)
+                # the glass-and-mirror scene fills both parts of the bounce queue
+                st.set_tuning(3, 1, True, False, False, 16, 4, xcd_claim=xcd, super_tiles=super_tiles,
+                              refracting_first=super_tiles)
                 pix, hits, counts = st.render_batch(rs, views, hits=True)
                 got = b"".join(p.tobytes() for p in pix) + b"".join(h.tobytes() for h in hits)
                 got += b"".join(np.array([c.shadow, c.bounce], np.uint64).tobytes() for c in counts)

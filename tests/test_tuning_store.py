@@ -12,7 +12,8 @@ from slimtracin_amd import tuning_store as ts
 
 
 def _shape(**kw):
-    d = dict(tile_shift=3, blocks_per_cu=4, persistent=2, cost_order=0, ray_sort=0, stack_entries=16, primary_waves=4)
+    d = dict(tile_shift=3, blocks_per_cu=4, persistent=2, cost_order=0, ray_sort=0, stack_entries=16, primary_waves=4,
+             bounce_order=1)
     d.update(kw)
     return st.Tuning(*[d[f] for f in ts.FIELDS])
 
@@ -21,7 +22,7 @@ def test_shape_record_round_trip():
     t = _shape(tile_shift=4, ray_sort=1)
     d = ts.tuning_to_dict(t)
     assert d == dict(tile_shift=4, blocks_per_cu=4, persistent=2, cost_order=0, ray_sort=1, stack_entries=16,
-                     primary_waves=4)
+                     primary_waves=4, bounce_order=1)
     assert ts.tuning_to_dict(ts.tuning_from_dict(d)) == d
 
 
