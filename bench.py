@@ -2,9 +2,11 @@
 """Benchmark of the MI355X tracing path (one JSON line on stdout).
 
 Headline workload (one step = one launch): the batched request mix - 256
-render requests over 8 resident scenes of 0.2-5M triangles (noise-displaced
-spheres with a Cook-Torrance material, a floor quad, a mirror or glass
-sphere and 3 point lights; mesh BVHs built on the device as Morton LBVHs),
+render requests over 8 resident scenes of 0.2-5M triangles (4 of the
+single-mesh PBR type: a noise-displaced icosphere with Cook-Torrance
+materials, a floor quad, a glass sphere, a mirror quad, 4 point lights and
+2 emissive quads; 4 of 64 instances of 3 textured meshes, sampled with ray
+cones; mesh BVHs built on the device as Morton LBVHs),
 resolutions log-uniform 128x128..1920x1080, trace depth uniform 1-6, mode
 uniform over Beauty/Depth/Normal/UV, cameras on a sphere around each scene
 (seed 7). A step fuses all 256 requests into one persistent launch: the
@@ -45,6 +47,10 @@ Beside it, in the same run:
     mean duration from HIP events recorded around each stage launch on the
     launch stream during the timed steps; HBM traffic per launch from the
     rocprofv3 FETCH_SIZE/WRITE_SIZE summary in profiles/.
+Every leg picks its own launch shape with st_autotune on its own requests
+(the headline unless --fixed-shape), as a caller tuning once per
+configuration would; the frame legs put the headline's shape back when
+they end, so the served leg runs at it.
 """
 from __future__ import annotations
 
@@ -410,7 +416,10 @@ def c1_leg(args):
     blob = st.gen_implicit(1)
     scene = st.DeviceScene(blob)
     view = make_view(width=640, height=480, mode=MODE_BEAUTY, depth=2, position=(0.0, 7.0, -11.0), pitch=-0.2)
+    saved = st.get_tuning()
+    tune, _ = st.autotune([scene], view, reps=7)    # this frame's own launch shape
     out = frame_leg(scene, view, max(args.steps, 20), 3, "c1")
+    out["launch_shape"] = tune.as_dict()
     out["workload"] = "16 seeded implicit primitives (4 quads, 4 boxes, 4 spheres, 4 tetrahedra), 3 point lights, " \
                       "Blinn-Phong, 640x480 (641x481 traced), depth 2, Beauty"
     if "cpu" in args.legs:
@@ -418,6 +427,7 @@ def c1_leg(args):
         assert ost.primary_rays + ost.shadow_rays + ost.bounce_rays == out["rays_per_frame"]
         out["cpu"] = cpu
         out["gpu_vs_cpu_all_cores"] = round(out["mrays"] / cpu["mrays"], 1)
+    st.restore_tuning(saved)
     scene.close()
     return out
 
@@ -431,7 +441,7 @@ def c2_leg(args):
         f"{info.device_bytes / 2**20:.1f} MiB in HBM, built in {time.time() - t0:.1f} s")
     view = make_view(width=1920, height=1080, mode=MODE_BEAUTY, depth=4, **PBR_CAM)
     saved = st.get_tuning()
-    tune, _ = st.autotune([scene], view)
+    tune, _ = st.autotune([scene], view, reps=7)
     out = frame_leg(scene, view, max(args.steps, 20), 3, "c2")
     stage_runs = [st.render_timed([scene], view, stages=True)[2] for _ in range(5)]
     names = st.stage_names(4)
@@ -490,6 +500,10 @@ def instanced_leg(args):
         return np.ascontiguousarray(p, dtype=PRIM_DTYPE)
 
     frames = [moved() for _ in range(args.instanced_frames + 2)]
+    saved = st.get_tuning()
+    ds.set_prims(frames[0])
+    ds.rebuild(st.ST_REBUILD_LBVH)
+    tune, _ = st.autotune([ds], view, reps=7)       # the frame's own launch shape, on its first layout
     for p in frames[:2]:                                            # warm-up
         ds.set_prims(p)
         ds.rebuild(st.ST_REBUILD_LBVH)
@@ -507,6 +521,7 @@ def instanced_leg(args):
     wall = time.perf_counter() - tstart
     info = ds.info()
     ds.close()
+    st.restore_tuning(saved)
     return {
         "instances": 64, "mesh_triangles": [50_000, 200_000, 500_000], "triangles": int(info.triangle_count),
         "texture": "2 x 2048^2 Perlin (albedo, normal), 10 mips", "cone_angle": 0.001, "depth": 3,
@@ -516,6 +531,7 @@ def instanced_leg(args):
         "frame_max_ms": round(float(np.max(frame_ms)), 3),
         "lbvh_rebuild_ms": round(float(np.median(rebuild_ms)), 4),
         "render_ms": round(float(np.median(kernel_ms)), 3),
+        "launch_shape": tune.as_dict(),
     }
 
 
