@@ -200,7 +200,7 @@ def c4_leg(args, blobs, views, sids):
     if args.fixed_shape:
         tune = st.get_tuning()
     else:
-        tune, _ = st.autotune(rs, views, reps=1)
+        tune, _ = st.autotune(rs, views, reps=3)
     log(f"c4: {len(dscenes)} scenes built in {build_s:.1f} s; {'fixed' if args.fixed_shape else 'autotune'} "
         f"{tune.as_dict()}")
     for _ in range(args.warmup):
