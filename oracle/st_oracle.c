@@ -2188,7 +2188,7 @@ uint32_t oracle_sah_build(const float *boxes, uint32_t n, uint32_t max_leaf, StB
 }
 
 /* ---- Morton LBVH (no live reference: the reference's Z-curve builder,
-   builder_bottom_up.h, never compiles) ------------------------
+   builder_bottom_up.h, is never compiled into the renderer) -------------
    Specification shared with the device builder:
      centroid c = (min + max) * 0.5f + 0.0f (the + 0 folds -0 to +0);
      bounds = componentwise min/max of the centroids;
