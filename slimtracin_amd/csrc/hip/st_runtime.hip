@@ -10,22 +10,28 @@
  *                   ST_DIR_MARK-th pixel (the primary stage finishes the sum
  *                   and normalises it). Rows of every request of a launch
  *                   are flattened into one grid.
- *   k_tile_order    the primary stage's tile claim order: tiles slower than
- *                   the mean in the previous launch first.
+ *   k_tile_order    the primary stage's tile claim order: tile order,
+ *                   super-tile order, or tiles slower than the mean in the
+ *                   previous launch first.
  *   k_primary       one wavefront per pixel tile: primary ray, and for
- *                   Beauty the first level of the depth loop (shading
- *                   entry, shadow rays, shading; st_wave.h). Tiles of all requests
+ *   k_primary4      Beauty the first level of the depth loop (shading
+ *                   entry, shadow rays, shading; st_wave.h); the same body
+ *                   held to 3 or 4 waves per SIMD. Tiles of all requests
  *                   of a launch form one list; in persistent mode each
  *                   wavefront keeps pulling the next tile from a global
- *                   counter until the list is exhausted (the loop's exit
+ *                   counter, or first from its XCD's eighth of the claim
+ *                   order, until the list is exhausted (the loop's exit
  *                   condition is the same for every wave), otherwise tile =
  *                   wave id with an XCD-aware block order. A single request
  *                   is just a list with one entry, so plain and batched
- *                   launches run the same code.
+ *                   launches run the same code. Persistent launches keep
+ *                   short traversal stacks in LDS and spill older entries
+ *                   to a per-launch buffer.
  *   k_bounce        the Beauty depth loop after its first level, one launch:
  *                   per queued pixel, in one lane, its bounce ray, then the
  *                   new surface's level as k_primary runs the first, and so
- *                   on while the pixel bounces.
+ *                   on while the pixel bounces; the part of the queue that
+ *                   holds the longest chains first.
  *   k_redo          the pixels whose traversal gave up in the two stages
  *                   above (a mesh traversal past ST_ABANDON_STEPS steps):
  *                   each pixel's whole path again by a group of blocks,
