@@ -10,7 +10,7 @@ import pytest
 
 import scenes
 import slimtracin_amd as st
-from slimtracin_amd.scene import MODE_BEAUTY, MODE_NORMALS, MODE_UVS, make_view
+from slimtracin_amd.scene import MODE_BEAUTY, MODE_DEPTH, MODE_NORMALS, MODE_UVS, make_view
 
 pytestmark = pytest.mark.gpu
 
@@ -53,4 +53,20 @@ def test_noalloc_rejects_a_request_beyond_the_scratch_array():
                                             **{k: v for k, v in cam.items() if k != "depth"}))
         _same(ds, make_view(width=64, height=48, mode=MODE_BEAUTY, **cam))
     finally:
+        ds.close()
+
+
+def test_noalloc_with_the_xcd_claim():
+    """A 1080p Depth frame at one block per CU has 32 tiles per wave, so
+    both launches claim tiles per XCD: the frames still agree."""
+    s, cam = scenes.ex07_monkey()
+    ds = st.DeviceScene(s.to_blob())
+    saved = st.get_tuning()
+    try:
+        st.set_tuning(3, 1, True, False, False, 16, 4, xcd_claim=True)
+        view = make_view(width=1920, height=1080, mode=MODE_DEPTH, **{k: v for k, v in cam.items() if k != "depth"})
+        assert ((1920 + 8) // 8) * ((1080 + 8) // 8) >= 32 * 4 * 256
+        _same(ds, view)
+    finally:
+        st.restore_tuning(saved)
         ds.close()
