@@ -6,7 +6,7 @@ render requests over 8 resident scenes of 0.2-5M triangles (4 of the
 single-mesh PBR type: a noise-displaced icosphere with Cook-Torrance
 materials, a floor quad, a glass sphere, a mirror quad, 4 point lights and
 2 emissive quads; 4 of 64 instances of 3 textured meshes, sampled with ray
-cones; mesh BVHs built on the device as Morton LBVHs),
+cones; mesh BVHs built on the host by the binned SAH builder),
 resolutions log-uniform 128x128..1920x1080, trace depth uniform 1-6, mode
 uniform over Beauty/Depth/Normal/UV, cameras on a sphere around each scene
 (seed 7). A step fuses all 256 requests into one persistent launch: the
@@ -81,6 +81,12 @@ BATCH_PBR_K = [100, 200, 316, 500]                       # 200k, 800k, ~2.0M, 5.
 BATCH_INSTANCED = [[50_000, 100_000, 150_000], [100_000, 200_000, 500_000],
                    [200_000, 500_000, 1_000_000], [500_000, 1_000_000, 2_000_000]]
 BATCH_TEXTURE = 1024
+# The headline scenes are built once and stay resident: their meshes get the
+# host binned SAH builder (SAH-quality trees, 14-16% fewer node steps per ray
+# than the device Morton LBVH, about a second per million triangles). The
+# served leg admits its scenes at run time and builds them on the device.
+BATCH_MESH_BUILD = {"mesh_binned": True}
+BATCH_MESH_LABEL = "host binned SAH (32 centroid bins per axis, leaves <= 4)"
 BATCH_CONE = 0.001                                       # ray-cone angle of requests on the textured scenes
 BATCH_SEED = 7
 BATCH_REQUESTS = 256
@@ -193,7 +199,7 @@ def plain_and_download_legs(rs, views, rays_per_step):
 
 def c4_leg(args, blobs, views, sids):
     t0 = time.perf_counter()
-    dscenes = [st.DeviceScene(b, mesh_lbvh=True) for b in blobs]
+    dscenes = [st.DeviceScene(b, **BATCH_MESH_BUILD) for b in blobs]
     build_s = time.perf_counter() - t0
     rs = [dscenes[i] for i in sids]
     st.render_timed(rs, views)                      # grows the launch arena to the whole batch
@@ -229,7 +235,7 @@ def c4_leg(args, blobs, views, sids):
     detail = {
         "requests": len(views), "scenes": len(dscenes),
         "scene_triangles": [d.info().triangle_count for d in dscenes],
-        "mesh_bvh": "device Morton LBVH (leaves <= 4)", "scene_build_s": round(build_s, 2),
+        "mesh_bvh": BATCH_MESH_LABEL, "scene_build_s": round(build_s, 2),
         "traced_pixels": int(sum(grid)), "rays": int(rays_per_step),
         "batched_mrays": round(batched_mrays, 2),
         "batched_ms_per_step": round(elapsed / args.steps * 1e3, 3),
@@ -280,7 +286,7 @@ def c4_parity_and_bytes(args, blobs, views, sids, rs):
         idx = [i for i in range(len(views)) if sids[i] == s]
         if not idx:
             continue
-        osc = ck.OracleScene(blobs[s], mesh_lbvh=True)
+        osc = ck.OracleScene(blobs[s], **BATCH_MESH_BUILD)
         _, ghits, gcounts = st.render_batch([rs[i] for i in idx], views[idx], hits=True, persistent=True)
         for k, i in enumerate(idx):
             _, oh, ost, _ = osc.render(views[i:i + 1], threads=threads, pixels=False)
@@ -323,7 +329,7 @@ def cpu_subset_baseline(blobs, views, sids):
             if s not in prebuilt:
                 info = blob_mesh_info(blobs[s])
                 prebuilt[s] = blob_with_prebuilt_meshes(
-                    blobs[s], [ck.oracle_mesh_lbvh(blobs[s], m, mi["triangle_count"]) for m, mi in enumerate(info)])
+                    blobs[s], [ck.oracle_mesh_binned(blobs[s], m, mi["triangle_count"]) for m, mi in enumerate(info)])
             osc = ck.OracleScene(prebuilt[s])
             _, _, ost, _ = osc.render(v, threads=cores, hits=False, pixels=False)
             osc.close()
@@ -335,7 +341,7 @@ def cpu_subset_baseline(blobs, views, sids):
             finally:
                 os.sched_setaffinity(0, saved)
         else:
-            osc = ck.OracleScene(blobs[s], mesh_lbvh=True)
+            osc = ck.OracleScene(blobs[s], **BATCH_MESH_BUILD)
             ts = []
             for _ in range(3):
                 _, _, ost, t = osc.render(v, threads=cores, hits=False, pixels=False)

@@ -78,12 +78,15 @@ int st_device_info(StDeviceInfo *info);
 /* Parse a scene blob, build its BVHs (the reference's SAH builder unless the
    blob carries prebuilt ones) and copy it to HBM. */
 int st_scene_create(const void *blob, size_t size, StSceneHandle **out);
-#define ST_SCENE_MESH_LBVH 1u  /* build meshes without a prebuilt BVH on the device (Morton LBVH, leaves <= 4) */
+#define ST_SCENE_MESH_LBVH 1u   /* build meshes without a prebuilt BVH on the device (Morton LBVH, leaves <= 4) */
+#define ST_SCENE_MESH_BINNED 2u /* build them on the host with the binned SAH builder (leaves <= 4) */
 
-/* st_scene_create with options: ST_SCENE_MESH_LBVH replaces the reference's
-   host SAH mesh build (updateMeshBVH, builder_top_down.h:272-364), which
-   takes minutes for meshes of millions of triangles, with a device build
-   of milliseconds. */
+/* st_scene_create with options. Both flags replace the reference's host SAH
+   mesh build (updateMeshBVH, builder_top_down.h:272-364), which takes
+   minutes for meshes of millions of triangles: ST_SCENE_MESH_LBVH with a
+   device build of milliseconds, ST_SCENE_MESH_BINNED with a host build of
+   seconds whose trees take fewer node steps per ray. At most one of them;
+   ST_ERR_ARG for both. */
 int st_scene_create_ex(const void *blob, size_t size, uint32_t flags, StSceneHandle **out);
 
 /* Mesh m's BVH node slots and Triangle records (leaf order) as resident on
@@ -266,6 +269,19 @@ int st_build_mesh(uint32_t tri_count, const float *pos, const uint32_t *pidx, co
                   const uint32_t *nidx, uint32_t normals_count, const float *uvs, const uint32_t *uidx,
                   uint32_t uvs_count, StBVHNodeRec *nodes, uint32_t *node_count, uint32_t *height,
                   StTriangleRec *tris, uint32_t *leaf_ids);
+/* Binned SAH build over n leaf boxes (layout as st_sah_build_boxes; 32
+   centroid bins per axis). No counterpart in the reference: a fast builder
+   of SAH-quality trees in the reference's BVH layout for meshes of millions
+   of triangles. Returns the node count, 0 on failure. */
+uint32_t st_binned_build_boxes(const float *boxes, uint32_t n, uint32_t max_leaf, StBVHNodeRec *nodes,
+                               uint32_t *leaf_ids, uint32_t *height);
+/* Mesh BVH (binned SAH, leaves <= 4) and triangle records in leaf order,
+   the outputs as st_build_mesh's (the triangle records are the same
+   function of the triangle, builder_top_down.h:272-364). */
+int st_build_mesh_binned(uint32_t tri_count, const float *pos, const uint32_t *pidx, const float *nrm,
+                         const uint32_t *nidx, uint32_t normals_count, const float *uvs, const uint32_t *uidx,
+                         uint32_t uvs_count, StBVHNodeRec *nodes, uint32_t *node_count, uint32_t *height,
+                         StTriangleRec *tris, uint32_t *leaf_ids);
 
 #ifdef __cplusplus
 }

@@ -1554,10 +1554,13 @@ static uint32_t build_bvh(OBuilder *b, uint32_t N, uint32_t max_leaf, ONode *nod
 
 /* Mesh builder used by scene_load: 0 = the reference's SAH (leaves <= 4),
    1 = the Morton LBVH with leaves of up to 4 triangles (the device mesh
-   builder's specification). */
-static int g_mesh_lbvh = 0;
+   builder's specification), 2 = the binned SAH builder (leaves <= 4; the
+   library's host builder for large meshes). */
+static int g_mesh_builder = 0;
 static uint32_t lbvh_nodes(const Box3 *bx, uint32_t n, uint32_t max_leaf, ONode *on, uint32_t *leaf_ids,
                            uint32_t *height);
+static uint32_t binned_nodes(const Box3 *bx, uint32_t n, uint32_t max_leaf, ONode *on, uint32_t *leaf_ids,
+                             uint32_t *height);
 
 /* builder_top_down.h:272-364 */
 static int build_mesh(OMesh *mesh, const float *pos, const uint32_t *pidx, const float *nrm, const uint32_t *nidx,
@@ -1591,7 +1594,9 @@ static int build_mesh(OMesh *mesh, const float *pos, const uint32_t *pidx, const
     mesh->nodes = (ONode *)calloc(2 * (size_t)T + 2, sizeof(ONode));
     mesh->leaf_ids = (uint32_t *)calloc(T + 1, 4);
     mesh->triangles = (OTri *)calloc(T + 1, sizeof(OTri));
-    if (g_mesh_lbvh) mesh->node_count = lbvh_nodes(b->leaf_boxes, T, 4, mesh->nodes, mesh->leaf_ids, &mesh->height);
+    if (g_mesh_builder == 1) mesh->node_count = lbvh_nodes(b->leaf_boxes, T, 4, mesh->nodes, mesh->leaf_ids, &mesh->height);
+    else if (g_mesh_builder == 2)
+        mesh->node_count = binned_nodes(b->leaf_boxes, T, 4, mesh->nodes, mesh->leaf_ids, &mesh->height);
     else mesh->node_count = build_bvh(b, T, 4, mesh->nodes, mesh->leaf_ids, &mesh->height);
     uint32_t *order = b->ids;
     memcpy(order, mesh->leaf_ids, T * sizeof(uint32_t));
@@ -1976,10 +1981,11 @@ int oracle_render_steps(const void *blob, size_t size, const StView *view, StFlo
    served workloads render hundreds of requests over a few large scenes). */
 void *oracle_scene_open(const void *blob, size_t size, uint32_t flags) {
     /* meshes without a prebuilt BVH: the reference SAH build, or with
-       flags & 1 the Morton LBVH the device builds for large meshes */
-    g_mesh_lbvh = (flags & 1u) != 0;
+       flags & 1 the Morton LBVH the device builds for large meshes, or with
+       flags & 2 the binned SAH tree of the library's host builder */
+    g_mesh_builder = (flags & 1u) ? 1 : (flags & 2u) ? 2 : 0;
     OScene *sc = scene_load(blob, size);
-    g_mesh_lbvh = 0;
+    g_mesh_builder = 0;
     return sc;
 }
 void oracle_scene_close(void *scene) {
@@ -2036,10 +2042,10 @@ int oracle_scene_bvh(const void *blob, size_t size, StBVHNodeRec *nodes, uint32_
 }
 
 static int mesh_bvh_export(const void *blob, size_t size, uint32_t m, StBVHNodeRec *nodes, StTriangleRec *tris,
-                           uint32_t *leaf_ids, uint32_t *node_count, uint32_t *height, int lbvh) {
-    g_mesh_lbvh = lbvh;
+                           uint32_t *leaf_ids, uint32_t *node_count, uint32_t *height, int builder) {
+    g_mesh_builder = builder;
     OScene *sc = scene_load(blob, size);
-    g_mesh_lbvh = 0;
+    g_mesh_builder = 0;
     if (!sc || m >= sc->mesh_count) { scene_free(sc); return -1; }
     const OMesh *mesh = sc->meshes + m;
     for (uint32_t n = 0; n < mesh->node_count; n++) node_to_rec(nodes + n, mesh->nodes + n);
@@ -2077,6 +2083,11 @@ int oracle_mesh_bvh(const void *blob, size_t size, uint32_t m, StBVHNodeRec *nod
 int oracle_mesh_lbvh(const void *blob, size_t size, uint32_t m, StBVHNodeRec *nodes, StTriangleRec *tris,
                      uint32_t *leaf_ids, uint32_t *node_count, uint32_t *height) {
     return mesh_bvh_export(blob, size, m, nodes, tris, leaf_ids, node_count, height, 1);
+}
+
+int oracle_mesh_binned(const void *blob, size_t size, uint32_t m, StBVHNodeRec *nodes, StTriangleRec *tris,
+                       uint32_t *leaf_ids, uint32_t *node_count, uint32_t *height) {
+    return mesh_bvh_export(blob, size, m, nodes, tris, leaf_ids, node_count, height, 2);
 }
 
 int oracle_ssb(const void *blob, size_t size, const StView *view, int32_t *rects, uint8_t *flags) {
@@ -2332,6 +2343,163 @@ static uint32_t lbvh_nodes(const Box3 *bx, uint32_t n, uint32_t max_leaf, ONode 
         count = 2 * n - 1;
     }
     free(keys); free(cent);
+    return count;
+}
+
+/* ---- binned SAH builder --------------------------------------------------
+ * No counterpart in the reference: the specification of the library's host
+ * builder for large meshes (a tree over leaf boxes in the reference's BVH
+ * layout), written here from that specification as a recursion:
+ *   centroid c = (min + max) * 0.5 per axis; a range of at most max_leaf
+ *   boxes is a leaf (first_child_id = range start, child_count = size);
+ *   per axis with centroid extent e > 0, 32 bins of width e / 32, bin =
+ *   (int)((c - cmin) * (32 / e)) clamped to 31; candidate plane k (1..31)
+ *   splits bins < k from bins >= k; cost = half_area(left) * n_left +
+ *   half_area(right) * n_right with both sides non-empty (the bins' boxes
+ *   merged upward for the left side, downward for the right); the first
+ *   strictly smallest cost over axes x, y, z and planes 1..31 wins; the
+ *   range is partitioned stably; with no positive extent it is halved in
+ *   range order; children take the next pair of slots, left subtree first;
+ *   node boxes merge the range in range order; depth 0 at the root, height
+ *   = deepest node + 1.
+ * Pinned by tree invariants, by the reference renderer's hits over it
+ * equalling its hits over the reference SAH tree, and by SAH cost against
+ * that tree (tests/test_binned_bvh.py). */
+#define O_BINS 32
+
+typedef struct {
+    const Box3 *bx;
+    V3 *cen;
+    uint32_t *ids, *tmp;
+    ONode *nodes;
+    uint32_t count, max_leaf, max_depth;
+} BinBuild;
+
+static float v3_axis(V3 v, int a) { return a == 0 ? v.x : a == 1 ? v.y : v.z; }
+
+static Box3 bin_range_box(const BinBuild *b, uint32_t s, uint32_t e) {
+    Box3 r = b->bx[b->ids[s]];
+    for (uint32_t i = s + 1; i < e; i++) r = merge_boxes(r, b->bx[b->ids[i]]);
+    return r;
+}
+
+static uint32_t bin_index(float c, float lo, float scale) {
+    int k = (int)((c - lo) * scale);
+    return k > O_BINS - 1 ? O_BINS - 1 : (uint32_t)k;
+}
+
+static void binned_node(BinBuild *b, uint32_t node, uint32_t s, uint32_t e, uint32_t depth) {
+    ONode *nd = b->nodes + node;
+    nd->depth = (uint16_t)depth;
+    if (depth > b->max_depth) b->max_depth = depth;
+    if (e - s <= b->max_leaf) {
+        nd->child_count = (uint16_t)(e - s);
+        nd->first_child_id = s;
+        return;
+    }
+    float lo[3], hi[3];
+    for (int a = 0; a < 3; a++) {
+        lo[a] = INFINITY;
+        hi[a] = -INFINITY;
+        for (uint32_t i = s; i < e; i++) {
+            float c = v3_axis(b->cen[b->ids[i]], a);
+            lo[a] = c < lo[a] ? c : lo[a];
+            hi[a] = c > hi[a] ? c : hi[a];
+        }
+    }
+    float best = INFINITY, best_scale = 0;
+    int axis = -1;
+    uint32_t plane = 0;
+    for (int a = 0; a < 3; a++) {
+        float ext = hi[a] - lo[a];
+        if (!(ext > 0)) continue;
+        float scale = 32.0f / ext;
+        Box3 bins[O_BINS];
+        uint32_t cnt[O_BINS] = {0};
+        memset(bins, 0, sizeof bins);
+        for (uint32_t i = s; i < e; i++) {
+            uint32_t id = b->ids[i], k = bin_index(v3_axis(b->cen[id], a), lo[a], scale);
+            bins[k] = cnt[k] ? merge_boxes(bins[k], b->bx[id]) : b->bx[id];
+            cnt[k]++;
+        }
+        /* below[k]: bins 0..k-1 merged upward, above[k]: bins 31..k merged
+           downward, with their box counts */
+        Box3 below[O_BINS + 1], above[O_BINS + 1];
+        uint32_t n_below[O_BINS + 1], n_above[O_BINS + 1];
+        n_below[0] = 0;
+        below[0] = bins[0];
+        for (uint32_t j = 0; j < O_BINS; j++) {
+            below[j + 1] = !cnt[j] ? below[j] : n_below[j] ? merge_boxes(below[j], bins[j]) : bins[j];
+            n_below[j + 1] = n_below[j] + cnt[j];
+        }
+        n_above[O_BINS] = 0;
+        above[O_BINS] = bins[0];
+        for (uint32_t j = O_BINS; j-- > 0;) {
+            above[j] = !cnt[j] ? above[j + 1] : n_above[j + 1] ? merge_boxes(above[j + 1], bins[j]) : bins[j];
+            n_above[j] = n_above[j + 1] + cnt[j];
+        }
+        for (uint32_t k = 1; k < O_BINS; k++) {
+            if (!n_below[k] || !n_above[k]) continue;
+            float cost = box_area(below[k]) * (float)n_below[k] + box_area(above[k]) * (float)n_above[k];
+            if (cost < best) { best = cost; axis = a; plane = k; best_scale = scale; }
+        }
+    }
+    uint32_t mid;
+    if (axis < 0) {
+        mid = s + (e - s) / 2;
+    } else {
+        uint32_t nl = 0, nr = 0;
+        for (uint32_t i = s; i < e; i++) {
+            uint32_t id = b->ids[i];
+            if (bin_index(v3_axis(b->cen[id], axis), lo[axis], best_scale) < plane) b->tmp[s + nl++] = id;
+        }
+        for (uint32_t i = s; i < e; i++) {
+            uint32_t id = b->ids[i];
+            if (bin_index(v3_axis(b->cen[id], axis), lo[axis], best_scale) >= plane) b->tmp[s + nl + nr++] = id;
+        }
+        memcpy(b->ids + s, b->tmp + s, (e - s) * sizeof(uint32_t));
+        mid = s + nl;
+    }
+    uint32_t lc = b->count;
+    b->count += 2;
+    nd->first_child_id = lc;
+    nd->child_count = 0;
+    b->nodes[lc].aabb = bin_range_box(b, s, mid);
+    b->nodes[lc + 1].aabb = bin_range_box(b, mid, e);
+    binned_node(b, lc, s, mid, depth + 1);
+    binned_node(b, lc + 1, mid, e, depth + 1);
+}
+
+static uint32_t binned_nodes(const Box3 *bx, uint32_t n, uint32_t max_leaf, ONode *on, uint32_t *leaf_ids,
+                             uint32_t *height) {
+    BinBuild b = {bx, (V3 *)calloc(n + 1, sizeof(V3)), leaf_ids, (uint32_t *)calloc(n + 1, 4), on, 1,
+                  max_leaf ? max_leaf : 1, 0};
+    for (uint32_t i = 0; i < n; i++) {
+        b.cen[i] = v3((bx[i].min.x + bx[i].max.x) * 0.5f, (bx[i].min.y + bx[i].max.y) * 0.5f,
+                      (bx[i].min.z + bx[i].max.z) * 0.5f);
+        leaf_ids[i] = i;
+    }
+    on[0].aabb = bin_range_box(&b, 0, n);
+    binned_node(&b, 0, 0, n, 0);
+    *height = b.max_depth + 1;
+    free(b.cen);
+    free(b.tmp);
+    return b.count;
+}
+
+uint32_t oracle_binned_build(const float *boxes, uint32_t n, uint32_t max_leaf, StBVHNodeRec *nodes,
+                             uint32_t *leaf_ids, uint32_t *height) {
+    if (n == 0) { *height = 0; return 0; }
+    Box3 *bx = (Box3 *)calloc(n, sizeof(Box3));
+    ONode *on = (ONode *)calloc(2 * (size_t)n + 1, sizeof(ONode));
+    for (uint32_t i = 0; i < n; i++) {
+        bx[i].min = v3(boxes[6 * i], boxes[6 * i + 1], boxes[6 * i + 2]);
+        bx[i].max = v3(boxes[6 * i + 3], boxes[6 * i + 4], boxes[6 * i + 5]);
+    }
+    uint32_t count = binned_nodes(bx, n, max_leaf, on, leaf_ids, height);
+    for (uint32_t i = 0; i < count; i++) node_to_rec(nodes + i, on + i);
+    free(bx);
+    free(on);
     return count;
 }
 

@@ -78,6 +78,12 @@ int oracle_mesh_bvh(const void *blob, size_t size, uint32_t m, StBVHNodeRec *nod
 int oracle_mesh_lbvh(const void *blob, size_t size, uint32_t m, StBVHNodeRec *nodes, StTriangleRec *tris,
                      uint32_t *leaf_ids, uint32_t *node_count, uint32_t *height);
 
+/* Mesh BVH of mesh m built by the binned SAH builder (leaves <= 4; the HIP
+   library's host builder for large meshes) and the triangle records in its
+   leaf order. nodes: 2*tris slots. */
+int oracle_mesh_binned(const void *blob, size_t size, uint32_t m, StBVHNodeRec *nodes, StTriangleRec *tris,
+                       uint32_t *leaf_ids, uint32_t *node_count, uint32_t *height);
+
 /* Screen-space bounds (4 ints per primitive) and flags for a view. */
 int oracle_ssb(const void *blob, size_t size, const StView *view, int32_t *rects, uint8_t *flags);
 
@@ -95,6 +101,11 @@ uint32_t oracle_sah_build(const float *boxes, uint32_t n, uint32_t max_leaf, StB
    nodes: 2*n entries. Returns node count. */
 uint32_t oracle_lbvh_build(const float *boxes, uint32_t n, StBVHNodeRec *nodes, uint32_t *leaf_ids,
                            uint32_t *height);
+
+/* Binned SAH build over n caller-given leaf boxes (6 floats each).
+   nodes: 2*n entries. Returns node count. */
+uint32_t oracle_binned_build(const float *boxes, uint32_t n, uint32_t max_leaf, StBVHNodeRec *nodes,
+                             uint32_t *leaf_ids, uint32_t *height);
 
 /* World AABBs of the blob's primitives (getPrimitiveAABB + transformAABB). */
 int oracle_prim_boxes(const void *blob, size_t size, float *boxes);

@@ -109,6 +109,7 @@ def gen_mesh_scene(seed: int, triangles: int) -> bytes:
 
 
 ST_SCENE_MESH_LBVH = 1
+ST_SCENE_MESH_BINNED = 2
 
 
 def gen_instanced(seed: int, instances: int, mesh_triangles, tex_size: int) -> bytes:
@@ -119,15 +120,23 @@ def gen_instanced(seed: int, instances: int, mesh_triangles, tex_size: int) -> b
                                              C.byref(out), C.byref(size)), out, size)
 
 
-class DeviceScene:
-    """A scene resident in HBM (st_scene_create_ex). mesh_lbvh: build meshes
-    without a prebuilt BVH on the device (Morton LBVH) instead of the host SAH."""
+def scene_flags(mesh_lbvh: bool = False, mesh_binned: bool = False) -> int:
+    """st_scene_create_ex flags of a mesh builder choice (at most one)."""
+    if mesh_lbvh and mesh_binned:
+        raise ValueError("mesh_lbvh and mesh_binned together")
+    return (ST_SCENE_MESH_LBVH if mesh_lbvh else 0) | (ST_SCENE_MESH_BINNED if mesh_binned else 0)
 
-    def __init__(self, blob: bytes, mesh_lbvh: bool = False):
+
+class DeviceScene:
+    """A scene resident in HBM (st_scene_create_ex). Meshes without a
+    prebuilt BVH get the reference's host SAH build, or with mesh_lbvh the
+    device Morton LBVH, or with mesh_binned the host binned SAH builder."""
+
+    def __init__(self, blob: bytes, mesh_lbvh: bool = False, mesh_binned: bool = False):
+        flags = scene_flags(mesh_lbvh, mesh_binned)
         self._blob = np.frombuffer(blob, dtype=np.uint8)
         h = C.c_void_p()
-        _check(lib().st_scene_create_ex(_ptr(self._blob), C.c_size_t(len(blob)),
-                                        C.c_uint32(ST_SCENE_MESH_LBVH if mesh_lbvh else 0), C.byref(h)))
+        _check(lib().st_scene_create_ex(_ptr(self._blob), C.c_size_t(len(blob)), C.c_uint32(flags), C.byref(h)))
         self.handle = h
 
     def close(self):

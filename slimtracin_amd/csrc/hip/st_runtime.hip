@@ -1275,8 +1275,13 @@ extern "C" int st_scene_create_ex(const void *blob, size_t size, uint32_t flags,
     RT_LOCK;
     int rc = ensure_init();
     if (rc) return rc;
+    if ((flags & ST_SCENE_MESH_LBVH) && (flags & ST_SCENE_MESH_BINNED))
+        return set_err(ST_ERR_ARG, "ST_SCENE_MESH_LBVH and ST_SCENE_MESH_BINNED together");
     const char *why = NULL;
-    StScene *sc = st_scene_from_blob_ex(blob, size, (flags & ST_SCENE_MESH_LBVH) ? 1u : 0u, &why);
+    uint32_t builder = (flags & ST_SCENE_MESH_LBVH)     ? ST_MESH_BUILD_DEVICE
+                       : (flags & ST_SCENE_MESH_BINNED) ? ST_MESH_BUILD_BINNED
+                                                        : ST_MESH_BUILD_SAH;
+    StScene *sc = st_scene_from_blob_ex(blob, size, builder, &why);
     if (!sc) return set_err(ST_ERR_SCENE, "invalid scene: %s", why ? why : "unknown");
     StSceneHandle *h = (StSceneHandle *)calloc(1, sizeof(StSceneHandle));
     if (!h) { st_scene_free(sc); return set_err(ST_ERR_MEMORY, "out of host memory"); }

@@ -299,7 +299,7 @@ extern "C" int st_server_add_scene_ex(StServer *srv, const void *blob, size_t si
     if (!srv || !blob || !scene_id) return fail(ST_ERR_ARG, "null argument");
     const char *why = NULL;
     /* size the scene without building any BVH (mesh builds are deferred) */
-    StScene *probe = st_scene_from_blob_ex(blob, size, 1u, &why);
+    StScene *probe = st_scene_from_blob_ex(blob, size, ST_MESH_BUILD_DEVICE, &why);
     if (!probe) return fail(ST_ERR_SCENE, why ? why : "invalid scene");
     uint64_t need = st_scene_device_bytes(probe) + 64 * 1024;
     st_scene_free(probe);

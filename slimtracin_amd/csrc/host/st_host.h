@@ -96,9 +96,15 @@ typedef struct StSceneParts {
     const StBVHNodeRec *scene_nodes;
     uint32_t scene_node_count, scene_height;
     const uint32_t *scene_leaf_ids;
-    /* leave meshes without a prebuilt BVH for the device builder */
-    uint32_t defer_mesh_build;
+    /* builder of meshes without a prebuilt BVH: ST_MESH_BUILD_SAH (the
+       reference's, host), ST_MESH_BUILD_DEVICE (left for the device LBVH),
+       ST_MESH_BUILD_BINNED (binned SAH, host) */
+    uint32_t mesh_builder;
 } StSceneParts;
+
+#define ST_MESH_BUILD_SAH 0u
+#define ST_MESH_BUILD_DEVICE 1u
+#define ST_MESH_BUILD_BINNED 2u
 
 /* Validate the parts and build the device-ready scene. Returns NULL on
    malformed input; *err (if given) receives a short reason. */
@@ -106,7 +112,7 @@ StScene *st_scene_from_parts(const StSceneParts *parts, const char **err);
 
 /* Parse a blob into parts and build its BVHs (st_scene_from_parts). */
 StScene *st_scene_from_blob(const void *blob, size_t size, const char **err);
-StScene *st_scene_from_blob_ex(const void *blob, size_t size, uint32_t defer_mesh_build, const char **err);
+StScene *st_scene_from_blob_ex(const void *blob, size_t size, uint32_t mesh_builder, const char **err);
 
 /* Device material record of a blob material record. */
 StDMat st_mat_from_rec(const StMaterialRec *q);
@@ -132,6 +138,12 @@ sbox st_prim_world_box(const StScene *sc, uint32_t i);
    node count; nodes needs 2n entries, leaf_ids n. */
 uint32_t st_sah_build(const sbox *boxes, uint32_t n, uint32_t max_leaf, StBVHNodeRec *nodes, uint32_t *leaf_ids,
                       uint32_t *height);
+
+/* Binned SAH BVH over leaf boxes (32 centroid bins per axis; no counterpart
+   in the reference): returns the node count (0 on failure); nodes needs
+   2n entries, leaf_ids n. */
+uint32_t st_binned_build(const sbox *boxes, uint32_t n, uint32_t max_leaf, StBVHNodeRec *nodes, uint32_t *leaf_ids,
+                         uint32_t *height);
 
 /* Mesh BVH + triangle records (updateMeshBVH, builder_top_down.h:272-364). */
 int st_build_mesh(uint32_t tri_count, const float *pos, const uint32_t *pidx, const float *nrm,

@@ -234,7 +234,7 @@ StScene *st_scene_from_parts(const StSceneParts *pp, const char **err) {
        loaded on the host; device-built meshes never have one */
     sc->mesh_host_data = pp->mesh_count ? 0u : 1u;
     for (uint32_t m = 0; m < pp->mesh_count; m++)
-        if (pp->meshes[m].rec.prebuilt || !pp->defer_mesh_build) sc->mesh_host_data = 1;
+        if (pp->meshes[m].rec.prebuilt || pp->mesh_builder != ST_MESH_BUILD_DEVICE) sc->mesh_host_data = 1;
     sc->mesh_nodes = (StBVHNodeRec *)calloc(sc->mesh_host_data ? node_total + 1 : 1, sizeof(StBVHNodeRec));
     sc->tri_hot = (StDTriHot *)calloc(sc->mesh_host_data ? tri_total + 1 : 1, sizeof(StDTriHot));
     sc->tri_cold = (StDTriCold *)calloc(sc->mesh_host_data ? tri_total + 1 : 1, sizeof(StDTriCold));
@@ -244,7 +244,7 @@ StScene *st_scene_from_parts(const StSceneParts *pp, const char **err) {
         const StMeshRec *q = &s->rec;
         StDMesh *dm = sc->meshes + m;
         uint32_t height = 0, count = 0;
-        if (!q->prebuilt && pp->defer_mesh_build) {
+        if (!q->prebuilt && pp->mesh_builder == ST_MESH_BUILD_DEVICE) {
             sc->mesh_on_device[m] = 1;
             sc->deferred[m] = *s;
             dm->height = 0;
@@ -259,9 +259,13 @@ StScene *st_scene_from_parts(const StSceneParts *pp, const char **err) {
         } else {
             StTriangleRec *tris = (StTriangleRec *)malloc(sizeof(StTriangleRec) * q->triangle_count);
             uint32_t *leaf = (uint32_t *)malloc(4u * (size_t)q->triangle_count);
-            if (!tris || !leaf || st_build_mesh(q->triangle_count, s->pos, s->pidx, s->nrm, s->nidx, q->normals_count,
-                                                s->uvs, s->uidx, q->uvs_count, sc->mesh_nodes + dm->node_offset, &count,
-                                                &height, tris, leaf)) {
+            int (*build)(uint32_t, const float *, const uint32_t *, const float *, const uint32_t *, uint32_t,
+                         const float *, const uint32_t *, uint32_t, StBVHNodeRec *, uint32_t *, uint32_t *,
+                         StTriangleRec *, uint32_t *) =
+                pp->mesh_builder == ST_MESH_BUILD_BINNED ? st_build_mesh_binned : st_build_mesh;
+            if (!tris || !leaf || build(q->triangle_count, s->pos, s->pidx, s->nrm, s->nidx, q->normals_count, s->uvs,
+                                        s->uidx, q->uvs_count, sc->mesh_nodes + dm->node_offset, &count, &height,
+                                        tris, leaf)) {
                 free(tris); free(leaf);
                 FAIL("mesh BVH build failed");
             }
@@ -350,7 +354,7 @@ StScene *st_scene_from_blob(const void *blob, size_t size, const char **err) {
     return st_scene_from_blob_ex(blob, size, 0, err);
 }
 
-StScene *st_scene_from_blob_ex(const void *blob, size_t size, uint32_t defer_mesh_build, const char **err) {
+StScene *st_scene_from_blob_ex(const void *blob, size_t size, uint32_t mesh_builder, const char **err) {
     const char *dummy;
     if (!err) err = &dummy;
     *err = NULL;
@@ -428,7 +432,7 @@ StScene *st_scene_from_blob_ex(const void *blob, size_t size, uint32_t defer_mes
         pp.scene_leaf_ids = (const uint32_t *)take(&r, 4u * (size_t)h->prim_count);
         if (r.bad) { *err = "truncated scene BVH"; goto out; }
     }
-    pp.defer_mesh_build = defer_mesh_build;
+    pp.mesh_builder = mesh_builder;
     sc = st_scene_from_parts(&pp, err);
 out:
     for (uint32_t t = 0; t < h->texture_count; t++) free(mip_arrays ? mip_arrays[t] : NULL);

@@ -5,7 +5,7 @@ import ctypes as C
 
 import numpy as np
 
-from . import PIXEL_DTYPE, VIEW_DTYPE, RayCounts, SlimTracinError, _check, _ptr, lib
+from . import PIXEL_DTYPE, VIEW_DTYPE, RayCounts, SlimTracinError, _check, _ptr, lib, scene_flags
 
 OUTPUT_BGRA8 = 0
 OUTPUT_FLOAT_PIXELS = 1
@@ -53,19 +53,19 @@ class Server:
         except Exception:
             pass
 
-    def add_scene(self, blob: bytes, mesh_lbvh: bool = False) -> int:
+    def add_scene(self, blob: bytes, mesh_lbvh: bool = False, mesh_binned: bool = False) -> int:
         b = np.frombuffer(blob, dtype=np.uint8)
         sid = C.c_uint32()
         _check(lib().st_server_add_scene_ex(self.handle, _ptr(b), C.c_size_t(len(blob)),
-                                            C.c_uint32(1 if mesh_lbvh else 0), C.byref(sid)))
+                                            C.c_uint32(scene_flags(mesh_lbvh, mesh_binned)), C.byref(sid)))
         return sid.value
 
-    def try_add_scene(self, blob: bytes, mesh_lbvh: bool = False):
+    def try_add_scene(self, blob: bytes, mesh_lbvh: bool = False, mesh_binned: bool = False):
         """Scene id, or None when the scene does not fit the HBM budget."""
         b = np.frombuffer(blob, dtype=np.uint8)
         sid = C.c_uint32()
         rc = lib().st_server_add_scene_ex(self.handle, _ptr(b), C.c_size_t(len(blob)),
-                                          C.c_uint32(1 if mesh_lbvh else 0), C.byref(sid))
+                                          C.c_uint32(scene_flags(mesh_lbvh, mesh_binned)), C.byref(sid))
         if rc == ST_ERR_BUSY:
             return None
         _check(rc)

@@ -109,13 +109,16 @@ def oracle_render_steps(blob, view, threads=1):
 class OracleScene:
     """A scene blob loaded into the oracle once, rendered from many views.
     mesh_lbvh: meshes without a prebuilt BVH get the device's Morton LBVH
-    (leaves <= 4), as DeviceScene(mesh_lbvh=True) builds them."""
+    (leaves <= 4), as DeviceScene(mesh_lbvh=True) builds them; mesh_binned:
+    the binned SAH tree, as DeviceScene(mesh_binned=True) builds them."""
 
-    def __init__(self, blob, mesh_lbvh=False):
+    def __init__(self, blob, mesh_lbvh=False, mesh_binned=False):
+        assert not (mesh_lbvh and mesh_binned)
         lib = _load(ORACLE_SO)
         lib.oracle_scene_open.restype = C.c_void_p
         self._keep = np.frombuffer(blob, dtype=np.uint8)
-        self.handle = lib.oracle_scene_open(_ptr(self._keep), C.c_size_t(len(blob)), C.c_uint32(1 if mesh_lbvh else 0))
+        flags = (1 if mesh_lbvh else 0) | (2 if mesh_binned else 0)
+        self.handle = lib.oracle_scene_open(_ptr(self._keep), C.c_size_t(len(blob)), C.c_uint32(flags))
         assert self.handle, "oracle_scene_open failed"
         self._keep = None
 
@@ -318,6 +321,35 @@ def oracle_lbvh_build(boxes):
     return nodes[:cnt], leaf[:n], h.value
 
 
+def oracle_binned_build(boxes, max_leaf):
+    boxes = np.ascontiguousarray(boxes, dtype=np.float32).reshape(-1, 6)
+    n = len(boxes)
+    nodes = np.zeros(2 * n + 2, dtype=NODE_DTYPE)
+    leaf = np.zeros(n + 1, dtype=np.uint32)
+    h = C.c_uint32()
+    lib = _load(ORACLE_SO)
+    lib.oracle_binned_build.restype = C.c_uint32
+    cnt = lib.oracle_binned_build(_ptr(boxes), C.c_uint32(n), C.c_uint32(max_leaf), _ptr(nodes), _ptr(leaf),
+                                  C.byref(h))
+    return nodes[:cnt], leaf[:n], h.value
+
+
+def sah_cost(nodes):
+    """SAH cost of a reference-layout BVH: sum over reachable nodes of the
+    node's half area over the root's, times 1 for an inner node and times its
+    box count for a leaf."""
+    root = nodes[0]
+    def area(nd):
+        e = (nd["max"] - nd["min"]).astype(np.float64)
+        return e[0] * e[1] + e[1] * e[2] + e[2] * e[0]
+    a0 = area(root)
+    cost = 0.0
+    for o in reachable_nodes(nodes):
+        nd = nodes[o]
+        cost += area(nd) / a0 * (int(nd["child_count"]) if nd["child_count"] else 1)
+    return cost
+
+
 def oracle_prim_boxes(blob, prim_count):
     out = np.zeros((prim_count, 6), dtype=np.float32)
     bp, bs, keep = _blob(blob)
@@ -361,6 +393,11 @@ def resolve_test_grid(rng, w1, h1):
 def oracle_mesh_lbvh(blob, m, tri_count):
     """Morton LBVH (leaves <= 4) of mesh m and its triangles in leaf order."""
     return _mesh_bvh(_load(ORACLE_SO), "oracle_mesh_lbvh", blob, m, tri_count)
+
+
+def oracle_mesh_binned(blob, m, tri_count):
+    """Binned SAH tree (leaves <= 4) of mesh m and its triangles in leaf order."""
+    return _mesh_bvh(_load(ORACLE_SO), "oracle_mesh_binned", blob, m, tri_count)
 
 
 def reachable_nodes(nodes):

@@ -10,9 +10,10 @@ host core:
       meshes moved and the scene BVH rebuilt on the device (Morton LBVH),
       2048^2 albedo and normal maps with mip chains, cone angle 0.001,
       1920x1080, depth 3, against the oracle over the device-built tree;
-  C4  the 256-request batch over 8 scenes of 0.2-5M triangles in one launch:
-      equal to the 256 plain launches byte for byte, and every request's hit
-      records, pixels and ray counts against the oracle over the same LBVHs;
+  C4  the 256-request batch over 8 scenes of 0.2-5M triangles in one launch
+      (meshes built as the benchmark builds them, binned SAH): equal to the
+      256 plain launches byte for byte, and every request's hit records,
+      pixels and ray counts against the oracle over the same trees;
   C5  served requests over three scenes of 1, 5 and 20M triangles: each
       frame the same alone and in the mix, and small requests against the
       oracle.
@@ -88,8 +89,9 @@ def _batch_workload():
 
 
 def test_c4_batch_of_256_full_size():
+    import bench
     blobs, views, sids = _batch_workload()
-    ds = [st.DeviceScene(b, mesh_lbvh=True) for b in blobs]
+    ds = [st.DeviceScene(b, **bench.BATCH_MESH_BUILD) for b in blobs]
     try:
         rs = [ds[i] for i in sids]
         bp, bh, bc = st.render_batch(rs, views, hits=True)
@@ -97,7 +99,7 @@ def test_c4_batch_of_256_full_size():
             idx = [i for i in range(len(views)) if sids[i] == s]
             if not idx:
                 continue
-            osc = ck.OracleScene(blobs[s], mesh_lbvh=True)
+            osc = ck.OracleScene(blobs[s], **bench.BATCH_MESH_BUILD)
             for i in idx:
                 v = views[i:i + 1]
                 pp, ph, pc = st.render(rs[i], v)
