@@ -79,7 +79,7 @@ int st_device_info(StDeviceInfo *info);
    blob carries prebuilt ones) and copy it to HBM. */
 int st_scene_create(const void *blob, size_t size, StSceneHandle **out);
 #define ST_SCENE_MESH_LBVH 1u   /* build meshes without a prebuilt BVH on the device (Morton LBVH, leaves <= 4) */
-#define ST_SCENE_MESH_BINNED 2u /* build them on the host with the binned SAH builder (leaves <= 4) */
+#define ST_SCENE_MESH_BINNED 2u /* build them on the host with the binned SAH builder (one triangle per leaf) */
 
 /* st_scene_create with options. Both flags replace the reference's host SAH
    mesh build (updateMeshBVH, builder_top_down.h:272-364), which takes
@@ -275,7 +275,7 @@ int st_build_mesh(uint32_t tri_count, const float *pos, const uint32_t *pidx, co
    of triangles. Returns the node count, 0 on failure. */
 uint32_t st_binned_build_boxes(const float *boxes, uint32_t n, uint32_t max_leaf, StBVHNodeRec *nodes,
                                uint32_t *leaf_ids, uint32_t *height);
-/* Mesh BVH (binned SAH, leaves <= 4) and triangle records in leaf order,
+/* Mesh BVH (binned SAH, one triangle per leaf) and triangle records in leaf order,
    the outputs as st_build_mesh's (the triangle records are the same
    function of the triangle, builder_top_down.h:272-364). */
 int st_build_mesh_binned(uint32_t tri_count, const float *pos, const uint32_t *pidx, const float *nrm,

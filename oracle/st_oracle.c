@@ -1554,8 +1554,8 @@ static uint32_t build_bvh(OBuilder *b, uint32_t N, uint32_t max_leaf, ONode *nod
 
 /* Mesh builder used by scene_load: 0 = the reference's SAH (leaves <= 4),
    1 = the Morton LBVH with leaves of up to 4 triangles (the device mesh
-   builder's specification), 2 = the binned SAH builder (leaves <= 4; the
-   library's host builder for large meshes). */
+   builder's specification), 2 = the binned SAH builder with one triangle
+   per leaf (the library's host builder for large meshes). */
 static int g_mesh_builder = 0;
 static uint32_t lbvh_nodes(const Box3 *bx, uint32_t n, uint32_t max_leaf, ONode *on, uint32_t *leaf_ids,
                            uint32_t *height);
@@ -1596,7 +1596,7 @@ static int build_mesh(OMesh *mesh, const float *pos, const uint32_t *pidx, const
     mesh->triangles = (OTri *)calloc(T + 1, sizeof(OTri));
     if (g_mesh_builder == 1) mesh->node_count = lbvh_nodes(b->leaf_boxes, T, 4, mesh->nodes, mesh->leaf_ids, &mesh->height);
     else if (g_mesh_builder == 2)
-        mesh->node_count = binned_nodes(b->leaf_boxes, T, 4, mesh->nodes, mesh->leaf_ids, &mesh->height);
+        mesh->node_count = binned_nodes(b->leaf_boxes, T, 1, mesh->nodes, mesh->leaf_ids, &mesh->height);
     else mesh->node_count = build_bvh(b, T, 4, mesh->nodes, mesh->leaf_ids, &mesh->height);
     uint32_t *order = b->ids;
     memcpy(order, mesh->leaf_ids, T * sizeof(uint32_t));

@@ -78,8 +78,8 @@ int oracle_mesh_bvh(const void *blob, size_t size, uint32_t m, StBVHNodeRec *nod
 int oracle_mesh_lbvh(const void *blob, size_t size, uint32_t m, StBVHNodeRec *nodes, StTriangleRec *tris,
                      uint32_t *leaf_ids, uint32_t *node_count, uint32_t *height);
 
-/* Mesh BVH of mesh m built by the binned SAH builder (leaves <= 4; the HIP
-   library's host builder for large meshes) and the triangle records in its
+/* Mesh BVH of mesh m built by the binned SAH builder (one triangle per leaf;
+   the HIP library's host builder for large meshes) and the triangle records in its
    leaf order. nodes: 2*tris slots. */
 int oracle_mesh_binned(const void *blob, size_t size, uint32_t m, StBVHNodeRec *nodes, StTriangleRec *tris,
                        uint32_t *leaf_ids, uint32_t *node_count, uint32_t *height);

@@ -2,8 +2,8 @@
 reference): the library's build equals the oracle's restatement field for
 field; the trees are valid reference-layout BVHs over every box once; the
 reference renderer finds the same hits over them as over its own SAH trees;
-and their SAH cost is that of the reference's exact SAH builder within a few
-percent, below the Morton LBVH's."""
+and their SAH cost is that of the reference's exact SAH builder within 2%
+at the same leaf cap, below the Morton LBVH's."""
 from __future__ import annotations
 
 import ctypes as C
@@ -96,21 +96,44 @@ def test_sah_split_beats_halving():
     assert l["max"][0] < 2 and r["min"][0] > 99
 
 
+def _sphere_boxes(n, seed):
+    """Boxes of small triangles over a unit sphere (a mesh-like distribution)."""
+    v = np.random.default_rng(seed).normal(size=(n, 3))
+    v /= np.linalg.norm(v, axis=1, keepdims=True)
+    return np.concatenate([v - 0.004, v + 0.004], 1).astype(np.float32)
+
+
+@pytest.mark.parametrize("kind", ["uniform", "sphere"])
+def test_sah_cost_against_reference_and_lbvh(kind):
+    """At the same leaf cap the binned tree costs what the reference's exact
+    SAH tree costs within 2%, and with one box per leaf clearly less than the
+    Morton LBVH's."""
+    b = _boxes("uniform", 20000, 1) if kind == "uniform" else _sphere_boxes(20000, 5)
+    for max_leaf in (1, 4):
+        cost = ck.sah_cost(ck.oracle_binned_build(b, max_leaf)[0])
+        ref = ck.sah_cost(ck.oracle_sah_build(b, max_leaf)[0])
+        assert cost <= 1.02 * ref, (kind, max_leaf, cost, ref)
+    lbvh = ck.sah_cost(ck.oracle_lbvh_build(b)[0])
+    assert ck.sah_cost(ck.oracle_binned_build(b, 1)[0]) < 0.95 * lbvh
+
+
 @pytest.mark.parametrize("tris", [1, 5, 2000, 30000])
-def test_mesh_tree_records_and_cost(tris):
+def test_mesh_tree_and_records(tris):
+    """Mesh trees hold one triangle per leaf; each triangle record equals the
+    reference SAH build's record of the same triangle."""
     blob = st.gen_mesh_scene(21, tris)
     T = blob_mesh_info(blob)[0]["triangle_count"]
     nodes, trirecs, leaf, h = ck.oracle_mesh_binned(blob, 0, T)
     snodes, stris, sleaf, _ = ck.oracle_mesh_bvh(blob, 0, T)        # reference SAH build (pinned elsewhere)
-    # leaf order permutes the triangles; each record equals the SAH build's record of the same triangle
+    assert len(nodes) == 2 * T - 1
+    leaves = nodes[nodes["child_count"] > 0]
+    assert len(leaves) == T and (leaves["child_count"] == 1).all()
+    assert sorted(leaves["first_child_id"].tolist()) == list(range(T))
+    # leaf order permutes the triangles
     tid = np.argsort(leaf)
     assert sorted(tid.tolist()) == list(range(T))
     assert trirecs.tobytes() == stris[sleaf[tid]].tobytes()
     assert h == int(nodes["depth"].max()) + 1
-    if T >= 2000:
-        lnodes = ck.oracle_mesh_lbvh(blob, 0, T)[0]
-        cost, ref_cost, lbvh_cost = ck.sah_cost(nodes), ck.sah_cost(snodes), ck.sah_cost(lnodes)
-        assert cost <= 1.03 * ref_cost and cost < 0.95 * lbvh_cost, (cost, ref_cost, lbvh_cost)
 
 
 @pytest.mark.skipif(not ck.ref_available(), reason="reference build missing")
