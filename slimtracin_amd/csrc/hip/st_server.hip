@@ -153,8 +153,13 @@ static void worker_loop(StServer *srv, uint32_t index) {
             srv->in_flight++;
             if (srv->in_flight > srv->max_in_flight_seen) srv->max_in_flight_seen = srv->in_flight;
         }
-        /* another worker may take what this one left queued */
-        srv->cv_work.notify_one();
+        /* another worker may take what this one left queued. Every waiter is
+           woken: with st_server_set_limits below the worker count only
+           workers 0..active_limit-1 may take work, and a single wake-up
+           could go to one that may not, which would leave the queue
+           waiting for the next submission (for the last request of a
+           stream of arrivals, forever). */
+        srv->cv_work.notify_all();
         const size_t n = ids.size();
         const double t_prep = now_ms();
         std::vector<StHostView> hv(n);
@@ -372,7 +377,9 @@ extern "C" int st_server_submit(StServer *srv, uint32_t scene_id, const StView *
         srv->alloc_ms += t_alloc;
         *ticket = id;
     }
-    srv->cv_work.notify_one();
+    /* all waiters, so that an eligible worker sees the request (see the
+       worker loop's wake-up) */
+    srv->cv_work.notify_all();
     return ST_OK;
 }
 

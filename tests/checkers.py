@@ -396,7 +396,7 @@ def oracle_mesh_lbvh(blob, m, tri_count):
 
 
 def oracle_mesh_binned(blob, m, tri_count):
-    """Binned SAH tree (leaves <= 4) of mesh m and its triangles in leaf order."""
+    """Binned SAH tree (one triangle per leaf) of mesh m and its triangles in leaf order."""
     return _mesh_bvh(_load(ORACLE_SO), "oracle_mesh_binned", blob, m, tri_count)
 
 

@@ -230,3 +230,29 @@ def test_plain_limits_one_request_per_launch():
         assert srv.stats()["completed"] == 24
     finally:
         srv.close()
+
+
+def test_limited_workers_take_requests_submitted_while_idle():
+    """With set_limits(1, 1) only worker 0 of the server's 3 may take work.
+    Each request here is submitted while every worker is idle and waited for
+    before the next: the submission must wake worker 0 (a single wake-up of
+    a worker that may not take work left such a request queued until the
+    next submission, and the last one for good). Waits are bounded, so a
+    request left queued fails the test instead of holding it."""
+    sc = _scene_set()[:1]
+    blob = sc[0][0].to_blob()
+    cam = {k: v for k, v in sc[0][1].items() if k != "depth"}
+    srv = Server(output=OUTPUT_BGRA8)
+    try:
+        sid = srv.add_scene(blob)
+        reqs = _views(np.random.default_rng(5), [cam], 24)
+        for limits in ((1, 1), (0, 2), (0, 0)):
+            srv.set_limits(*limits)
+            for _, v in reqs:
+                t = srv.submit(sid, v)
+                srv.wait(t, timeout_ms=20000, copy=False)
+                srv.release(t)
+        assert srv.stats()["completed"] == 3 * len(reqs)
+    finally:
+        srv.set_limits(0, 0)
+        srv.close()
