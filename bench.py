@@ -645,7 +645,7 @@ def served_leg(args):
         "scenes_requested": int(args.serve_scenes), "scenes_admitted": len(ids), "scenes_rejected": rejected,
         "scene_triangles_total": int(sum(admitted_tris)), "hbm_budget_gb": 200,
         "resident_gb": round(stats["resident_bytes"] / 1e9, 2), "scene_build_s": round(build_s, 1),
-        "mesh_bvh": "device Morton LBVH (leaves <= 4)", "output": "BGRA8 window image (device resolve)",
+        "mesh_bvh": "device Morton LBVH (one triangle per leaf)", "output": "BGRA8 window image (device resolve)",
         "burst_requests": 512, "burst_capacity_req_per_s": round(capacity, 1), "burst_mrays": round(burst_mrays, 2),
         "plain_burst": plain,
         "arrival_rate_per_s": rate, "arrival_rate": "80% of the plain loop's request rate", "requests": n,

@@ -78,7 +78,7 @@ int st_device_info(StDeviceInfo *info);
 /* Parse a scene blob, build its BVHs (the reference's SAH builder unless the
    blob carries prebuilt ones) and copy it to HBM. */
 int st_scene_create(const void *blob, size_t size, StSceneHandle **out);
-#define ST_SCENE_MESH_LBVH 1u   /* build meshes without a prebuilt BVH on the device (Morton LBVH, leaves <= 4) */
+#define ST_SCENE_MESH_LBVH 1u   /* build meshes without a prebuilt BVH on the device (Morton LBVH, one triangle per leaf) */
 #define ST_SCENE_MESH_BINNED 2u /* build them on the host with the binned SAH builder (one triangle per leaf) */
 
 /* st_scene_create with options. Both flags replace the reference's host SAH

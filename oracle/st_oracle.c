@@ -1553,7 +1553,7 @@ static uint32_t build_bvh(OBuilder *b, uint32_t N, uint32_t max_leaf, ONode *nod
 }
 
 /* Mesh builder used by scene_load: 0 = the reference's SAH (leaves <= 4),
-   1 = the Morton LBVH with leaves of up to 4 triangles (the device mesh
+   1 = the Morton LBVH with one triangle per leaf (the device mesh
    builder's specification), 2 = the binned SAH builder with one triangle
    per leaf (the library's host builder for large meshes). */
 static int g_mesh_builder = 0;
@@ -1594,7 +1594,7 @@ static int build_mesh(OMesh *mesh, const float *pos, const uint32_t *pidx, const
     mesh->nodes = (ONode *)calloc(2 * (size_t)T + 2, sizeof(ONode));
     mesh->leaf_ids = (uint32_t *)calloc(T + 1, 4);
     mesh->triangles = (OTri *)calloc(T + 1, sizeof(OTri));
-    if (g_mesh_builder == 1) mesh->node_count = lbvh_nodes(b->leaf_boxes, T, 4, mesh->nodes, mesh->leaf_ids, &mesh->height);
+    if (g_mesh_builder == 1) mesh->node_count = lbvh_nodes(b->leaf_boxes, T, 1, mesh->nodes, mesh->leaf_ids, &mesh->height);
     else if (g_mesh_builder == 2)
         mesh->node_count = binned_nodes(b->leaf_boxes, T, 1, mesh->nodes, mesh->leaf_ids, &mesh->height);
     else mesh->node_count = build_bvh(b, T, 4, mesh->nodes, mesh->leaf_ids, &mesh->height);

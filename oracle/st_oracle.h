@@ -71,10 +71,9 @@ int oracle_scene_bvh(const void *blob, size_t size, StBVHNodeRec *nodes, uint32_
 int oracle_mesh_bvh(const void *blob, size_t size, uint32_t m, StBVHNodeRec *nodes,
                     StTriangleRec *tris, uint32_t *leaf_ids, uint32_t *node_count, uint32_t *height);
 
-/* Mesh BVH of mesh m built as the Morton LBVH with leaves of up to 4
-   triangles (the device mesh builder's specification), and the triangle
-   records in its leaf order. nodes: 2*tris slots; slots under a range that
-   became one leaf are zero. */
+/* Mesh BVH of mesh m built as the Morton LBVH with one triangle per leaf
+   (the device mesh builder's specification), and the triangle records in
+   its leaf order. nodes: 2*tris slots (2*tris - 1 used). */
 int oracle_mesh_lbvh(const void *blob, size_t size, uint32_t m, StBVHNodeRec *nodes, StTriangleRec *tris,
                      uint32_t *leaf_ids, uint32_t *node_count, uint32_t *height);
 

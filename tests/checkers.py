@@ -109,7 +109,7 @@ def oracle_render_steps(blob, view, threads=1):
 class OracleScene:
     """A scene blob loaded into the oracle once, rendered from many views.
     mesh_lbvh: meshes without a prebuilt BVH get the device's Morton LBVH
-    (leaves <= 4), as DeviceScene(mesh_lbvh=True) builds them; mesh_binned:
+    (one triangle per leaf), as DeviceScene(mesh_lbvh=True) builds them; mesh_binned:
     the binned SAH tree, as DeviceScene(mesh_binned=True) builds them."""
 
     def __init__(self, blob, mesh_lbvh=False, mesh_binned=False):
@@ -391,7 +391,7 @@ def resolve_test_grid(rng, w1, h1):
 
 
 def oracle_mesh_lbvh(blob, m, tri_count):
-    """Morton LBVH (leaves <= 4) of mesh m and its triangles in leaf order."""
+    """Morton LBVH (one triangle per leaf) of mesh m and its triangles in leaf order."""
     return _mesh_bvh(_load(ORACLE_SO), "oracle_mesh_lbvh", blob, m, tri_count)
 
 

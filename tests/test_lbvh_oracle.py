@@ -181,7 +181,7 @@ def test_reference_traversal_over_lbvh(seed):
         assert ck.pixels_equal(ref, ora), mode
 
 
-# ---- mesh LBVH (leaves of up to 4 triangles) ------------------------------
+# ---- mesh LBVH (one triangle per leaf) -------------------------------------
 
 from slimtracin_amd.scene import blob_mesh_info, blob_with_prebuilt_meshes  # noqa: E402
 
