@@ -7,9 +7,10 @@ host core:
   C2  the 200,000-triangle PBR frame, 1920x1080, depth 4: every hit record
       bit for bit, every pixel, the ray counts;
   C3  the instanced textured frame: 64 instances of 50k/200k/500k-triangle
-      meshes moved and the scene BVH rebuilt on the device (Morton LBVH),
-      2048^2 albedo and normal maps with mip chains, cone angle 0.001,
-      1920x1080, depth 3, against the oracle over the device-built tree;
+      meshes (binned SAH trees, as the benchmark builds them) moved and the
+      scene BVH rebuilt on the device (Morton LBVH), 2048^2 albedo and
+      normal maps with mip chains, cone angle 0.001, 1920x1080, depth 3,
+      against the oracle over the same trees;
   C4  the 256-request batch over 8 scenes of 0.2-5M triangles in one launch
       (meshes built as the benchmark builds them, binned SAH): equal to the
       256 plain launches byte for byte, and every request's hit records,
@@ -69,7 +70,8 @@ def test_c3_instanced_full_frame():
     prims = np.ascontiguousarray(prims, dtype=PRIM_DTYPE)
     view = make_view(width=1920, height=1080, mode=MODE_BEAUTY, depth=3, cone_angle=0.001,
                      position=(0.0, 14.0, -35.0), pitch=-0.3)
-    ds = st.DeviceScene(blob)
+    import bench
+    ds = st.DeviceScene(blob, **bench.INSTANCED_MESH_BUILD)
     try:
         ds.set_prims(prims)
         ds.rebuild(st.ST_REBUILD_LBVH)
@@ -78,7 +80,11 @@ def test_c3_instanced_full_frame():
     finally:
         ds.close()
     ref_blob = blob_with_scene_bvh(blob_with_prims(blob, prims), nodes, leaf, h)
-    op, oh, oc = ck.oracle_render(ref_blob, view, threads=THREADS)
+    osc = ck.OracleScene(ref_blob, **bench.INSTANCED_MESH_BUILD)    # the same mesh trees
+    try:
+        op, oh, oc, _ = osc.render(view, threads=THREADS)
+    finally:
+        osc.close()
     assert oc.texel_fetches > 100_000
     _assert_frame(gp, gh, gc, op, oh, oc, MODE_BEAUTY, "c3")
 
