@@ -55,6 +55,29 @@ def test_library_build_equals_oracle(kind, n):
         assert (got[1] == want[1]).all() and got[2] == want[2]
 
 
+def _lopsided_boxes(n):
+    """Centroids spaced exponentially along x: every split peels a few boxes
+    off the top bin, a long chain of uneven splits."""
+    x = np.power(np.float64(1.00025), np.arange(n))
+    lo = np.stack([x, np.zeros(n), np.zeros(n)], 1)
+    return np.concatenate([lo, lo + 0.5], 1).astype(np.float32)
+
+
+@pytest.mark.parametrize("kind", ["uniform", "clusters", "sphere", "lopsided"])
+def test_large_build_equals_oracle(kind):
+    """300,000 boxes: the library builds the top of the tree, then subtrees
+    on worker threads, and places them where a single-threaded build puts
+    them (the lopsided input's top is a long chain of uneven splits); the
+    tree equals the oracle's slot for slot."""
+    n = 300_000
+    b = {"uniform": lambda: _boxes("uniform", n, 11), "clusters": lambda: _boxes("clusters", n, 12),
+         "sphere": lambda: _sphere_boxes(n, 13), "lopsided": lambda: _lopsided_boxes(n)}[kind]()
+    for max_leaf in ((1, 4) if kind == "uniform" else (1,)):
+        got, want = lib_binned_build(b, max_leaf), ck.oracle_binned_build(b, max_leaf)
+        assert len(got[0]) == len(want[0]) and got[0].tobytes() == want[0].tobytes(), (kind, max_leaf)
+        assert (got[1] == want[1]).all() and got[2] == want[2]
+
+
 @pytest.mark.parametrize("kind", ["uniform", "clusters", "equal", "sheet"])
 def test_tree_invariants(kind):
     n = 5000
