@@ -83,7 +83,7 @@ BATCH_INSTANCED = [[50_000, 100_000, 150_000], [100_000, 200_000, 500_000],
 BATCH_TEXTURE = 1024
 # The headline scenes are built once and stay resident: their meshes get the
 # host binned SAH builder (SAH-quality trees with one triangle per leaf,
-# about a second per million triangles; the headline step takes 14% less
+# about half a second per million triangles; the headline step takes 14% less
 # time than over leaves of 4 and 25% less than over the device Morton LBVH).
 # The served leg admits its scenes at run time and builds them on the device.
 BATCH_MESH_BUILD = {"mesh_binned": True}
