@@ -210,7 +210,9 @@ def c4_leg(args, blobs, views, sids):
     if args.fixed_shape:
         tune = st.get_tuning()
     else:
-        tune, _ = st.autotune(rs, views, reps=3)
+        # median of 5 timings per candidate: shapes 2% apart (4 or 8 blocks
+        # per CU) are told apart more reliably than with 3
+        tune, _ = st.autotune(rs, views, reps=5)
     log(f"c4: {len(dscenes)} scenes built in {build_s:.1f} s; {'fixed' if args.fixed_shape else 'autotune'} "
         f"{tune.as_dict()}")
     for _ in range(args.warmup):
