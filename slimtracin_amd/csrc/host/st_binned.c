@@ -395,6 +395,64 @@ uint32_t st_binned_build(const sbox *boxes, uint32_t n, uint32_t max_leaf, StBVH
     return count;
 }
 
+/* Per-index work over [0, n) in chunks on up to PAR_MAX_THREADS threads
+   (the calling thread too); single-threaded below PAR_MIN_BOXES. Each index
+   writes only its own outputs, so the result does not depend on the split. */
+typedef struct {
+    void (*fn)(void *, uint32_t, uint32_t);
+    void *arg;
+    uint32_t n, chunk, next;
+} PFor;
+
+static void *pfor_run(void *a) {
+    PFor *p = (PFor *)a;
+    for (;;) {
+        uint32_t b = __atomic_fetch_add(&p->next, p->chunk, __ATOMIC_RELAXED);
+        if (b >= p->n) break;
+        p->fn(p->arg, b, p->n - b < p->chunk ? p->n : b + p->chunk);
+    }
+    return NULL;
+}
+
+static void parallel_for(uint32_t n, void (*fn)(void *, uint32_t, uint32_t), void *arg) {
+    if (n < PAR_MIN_BOXES) { fn(arg, 0, n); return; }
+    long online = sysconf(_SC_NPROCESSORS_ONLN);
+    uint32_t nthreads = online > 1 ? (uint32_t)online : 1u;
+    if (nthreads > PAR_MAX_THREADS) nthreads = PAR_MAX_THREADS;
+    PFor p = {fn, arg, n, 1u << 15, 0};
+    pthread_t th[PAR_MAX_THREADS];
+    uint32_t started = 0;
+    for (uint32_t i = 1; i < nthreads; i++)
+        if (pthread_create(&th[started], NULL, pfor_run, &p) == 0) started++;
+    pfor_run(&p);
+    for (uint32_t i = 0; i < started; i++) pthread_join(th[i], NULL);
+}
+
+typedef struct {
+    const float *pos, *nrm, *uvs;
+    const uint32_t *pidx, *nidx, *uidx, *order;
+    sbox *boxes;
+    StTriangleRec *tris;
+    uint32_t *leaf_ids;
+} MeshJob;
+
+static void mesh_boxes(void *a, uint32_t b, uint32_t e) {
+    const MeshJob *j = (const MeshJob *)a;
+    for (uint32_t i = b; i < e; i++)
+        j->boxes[i] = st_tri_box(j->pos + 3 * (size_t)j->pidx[3 * (size_t)i],
+                                 j->pos + 3 * (size_t)j->pidx[3 * (size_t)i + 1],
+                                 j->pos + 3 * (size_t)j->pidx[3 * (size_t)i + 2]);
+}
+
+static void mesh_records(void *a, uint32_t b, uint32_t e) {
+    const MeshJob *j = (const MeshJob *)a;
+    for (uint32_t i = b; i < e; i++) {
+        const uint32_t tid = j->order[i];
+        j->leaf_ids[tid] = i;
+        st_tri_record(tid, j->pos, j->pidx, j->nrm, j->nidx, j->uvs, j->uidx, j->tris + i);
+    }
+}
+
 int st_build_mesh_binned(uint32_t T, const float *pos, const uint32_t *pidx, const float *nrm, const uint32_t *nidx,
                          uint32_t normals_count, const float *uvs, const uint32_t *uidx, uint32_t uvs_count,
                          StBVHNodeRec *nodes, uint32_t *node_count, uint32_t *height, StTriangleRec *tris,
@@ -402,16 +460,12 @@ int st_build_mesh_binned(uint32_t T, const float *pos, const uint32_t *pidx, con
     sbox *boxes = (sbox *)calloc((size_t)T + 1, sizeof(sbox));
     uint32_t *order = (uint32_t *)malloc(((size_t)T + 1) * sizeof(uint32_t));
     if (!boxes || !order) { free(boxes); free(order); return -1; }
-    for (uint32_t i = 0; i < T; i++)
-        boxes[i] = st_tri_box(pos + 3 * (size_t)pidx[3 * (size_t)i], pos + 3 * (size_t)pidx[3 * (size_t)i + 1],
-                              pos + 3 * (size_t)pidx[3 * (size_t)i + 2]);
+    MeshJob j = {pos, normals_count ? nrm : NULL, uvs_count ? uvs : NULL, pidx, nidx, uidx, order, boxes, tris,
+                 leaf_ids};
+    parallel_for(T, mesh_boxes, &j);
     *node_count = st_binned_build(boxes, T, MESH_LEAF, nodes, order, height);
     int ok = *node_count != 0;
-    for (uint32_t i = 0; ok && i < T; i++) {
-        uint32_t tid = order[i];
-        leaf_ids[tid] = i;
-        st_tri_record(tid, pos, pidx, normals_count ? nrm : NULL, nidx, uvs_count ? uvs : NULL, uidx, tris + i);
-    }
+    if (ok) parallel_for(T, mesh_records, &j);
     free(boxes);
     free(order);
     return ok ? 0 : -1;

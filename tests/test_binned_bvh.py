@@ -159,6 +159,48 @@ def test_mesh_tree_and_records(tris):
     assert h == int(nodes["depth"].max()) + 1
 
 
+def _mesh_arrays(blob, m=0):
+    """Mesh m's geometry arrays as the blob stores them after its record."""
+    mi = blob_mesh_info(blob)[m]
+    V, T, N, U = mi["vertex_count"], mi["triangle_count"], mi["normals_count"], mi["uvs_count"]
+    o = mi["offset"] + 56
+    def take(dtype, count):
+        nonlocal o
+        a = np.frombuffer(blob, dtype=dtype, count=count, offset=o).copy()
+        o += a.nbytes
+        return a
+    pos, pidx = take(np.float32, 3 * V), take(np.uint32, 3 * T)
+    nrm = nidx = uvs = uidx = None
+    if N:
+        nrm, nidx = take(np.float32, 3 * N), take(np.uint32, 3 * T)
+    if U:
+        uvs, uidx = take(np.float32, 2 * U), take(np.uint32, 3 * T)
+    return T, N, U, pos, pidx, nrm, nidx, uvs, uidx
+
+
+@pytest.mark.parametrize("tris", [5, 30000, 200000])
+def test_library_mesh_build_equals_oracle(tris):
+    """st_build_mesh_binned (tree, triangle records in leaf order, leaf ids)
+    equals the oracle's binned mesh build; 200k triangles take the threaded
+    path."""
+    from slimtracin_amd.scene import TRIANGLE_DTYPE
+    blob = st.gen_mesh_scene(23, tris)
+    T, N, U, pos, pidx, nrm, nidx, uvs, uidx = _mesh_arrays(blob)
+    nodes = np.zeros(2 * T + 2, dtype=NODE_DTYPE)
+    trirecs = np.zeros(T, dtype=TRIANGLE_DTYPE)
+    leaf = np.zeros(T, dtype=np.uint32)
+    cnt, h = C.c_uint32(), C.c_uint32()
+    P = lambda a: None if a is None else a.ctypes.data_as(C.c_void_p)
+    rc = st.lib().st_build_mesh_binned(C.c_uint32(T), P(pos), P(pidx), P(nrm), P(nidx), C.c_uint32(N), P(uvs),
+                                       P(uidx), C.c_uint32(U), P(nodes), C.byref(cnt), C.byref(h), P(trirecs),
+                                       P(leaf))
+    assert rc == 0
+    onodes, otris, oleaf, oh = ck.oracle_mesh_binned(blob, 0, T)
+    assert cnt.value == len(onodes) and h.value == oh
+    assert nodes[:cnt.value].tobytes() == onodes.tobytes()
+    assert trirecs.tobytes() == otris.tobytes() and (leaf == oleaf).all()
+
+
 @pytest.mark.skipif(not ck.ref_available(), reason="reference build missing")
 def test_reference_traversal_over_mesh_binned():
     """The reference renderer over binned mesh BVHs finds the same primary
